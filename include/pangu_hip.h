@@ -1,0 +1,167 @@
+/*
+ * pangu_hip: MI355X (gfx950) FP64 GRMHD per-RK-stage update path.
+ *
+ * C ABI replacing the per-stage task functions that PANGU's task list
+ * schedules (pangu/src/task_list/ideal_grmhd.cc:62-94,104-110).  A caller
+ * (the Parthenon task list, or this package's own host driver) owns every
+ * field array; the library owns geometry tables, scratch and launch state.
+ *
+ * Conventions
+ *  - All field pointers are device pointers to FP64 arrays of one meshblock,
+ *    laid out (k, j, i) with i fastest over the ghosted extent, exactly what
+ *    Parthenon's VariablePack(b, l).data() exposes.
+ *  - Components are packed in PANGU's registration order
+ *    (package_registration.cc:84-96): density, energy, weighted_velocity[3],
+ *    [magnetic_field[3]], entropy, [electron_entropy].
+ *  - Every function returns 0 on success or a negative PANGU_E* code;
+ *    pangu_last_error() gives the text.  Nothing throws across the ABI.
+ *  - `stream` is a hipStream_t (NULL = default stream).  Work is
+ *    stream-ordered and asynchronous unless stated otherwise.
+ */
+#ifndef PANGU_HIP_H
+#define PANGU_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  PANGU_OK = 0,
+  PANGU_EINVAL = -1,   /* invalid argument or configuration */
+  PANGU_EDEVICE = -2,  /* HIP runtime error */
+  PANGU_ENOMEM = -3,   /* device allocation failed */
+  PANGU_ESTATE = -4    /* call out of order (e.g. no container registered) */
+};
+
+enum { PANGU_METRIC_MINKOWSKI = 0, PANGU_METRIC_BL = 1, PANGU_METRIC_CKS = 2, PANGU_METRIC_MKS = 3 };
+enum { PANGU_RIEMANN_LAXF = 0, PANGU_RIEMANN_HLL = 1 };
+enum { PANGU_LIMITER_MC = 0, PANGU_LIMITER_PPM4 = 1 };
+/* chainer = 2D -> 1Dvsq -> 1D Newton with flag + FixRecovery; robuster =
+ * Kastaun false position; onedvsq = the 1Dvsq solver alone, with flag. */
+enum { PANGU_RECOVERY_CHAINER = 0, PANGU_RECOVERY_ROBUSTER = 1, PANGU_RECOVERY_ONEDVSQ = 2 };
+enum { PANGU_HEAT_CONSTANT = 0, PANGU_HEAT_HOWES, PANGU_HEAT_KAWAZURA, PANGU_HEAT_WERNER,
+       PANGU_HEAT_ROWAN, PANGU_HEAT_SHARMA };
+enum { PANGU_BC_PERIODIC = 0, PANGU_BC_OUTFLOW = 1, PANGU_BC_REFLECT = 2 };
+
+/* Physics parameters: the <core>, <metric> and <electron> blocks of PANGU's
+ * input file (package_registration.cc:21-57,158-167). */
+typedef struct pangu_config {
+  int metric;
+  double a, h;
+  double gamma;  /* adiabatic_index */
+  double cfl;
+  int riemann, limiter, recovery;
+  int enable_B, enable_heating;
+  double density_floor, energy_floor, density_floor_pow, energy_floor_pow;
+  double sigma_max, lorentz_max;
+  int heat_model;
+  double fel_constant, gamma_e, gamma_p, ratio_min, ratio_max;
+  int limit_kel, suppress_highb_heat, enforce_positive_dissipation;
+  int excise;
+  double r_excise, dexcise, pexcise;
+  /* mesh topology for ghost exchange and physical boundaries */
+  int nb[3];     /* meshblocks per dim */
+  int bc[3][2];  /* PANGU_BC_* for inner/outer faces of each dim */
+} pangu_config;
+
+/* One meshblock: Parthenon UniformCartesian coordinates,
+ * Xc(idx) = xmin - ng*dx + (idx + 0.5)*dx, Xf(idx) = xmin - ng*dx + idx*dx. */
+typedef struct pangu_block {
+  int nx[3];      /* interior cells (1 on inactive dims) */
+  int ng[3];      /* ghost layers (0 on inactive dims) */
+  int lx[3];      /* logical location in the block grid */
+  double xmin[3]; /* interior lower corner */
+  double dx[3];   /* cell widths */
+} pangu_block;
+
+/* A stage container (Parthenon MeshData): per block, per component device
+ * pointers.  prim and cons are [nblocks * ncomp] tables; flux[d] is
+ * [nblocks * ncons] or NULL on inactive directions. */
+typedef struct pangu_container {
+  double *const *prim;
+  double *const *cons;
+  double *const *flux[3];
+} pangu_container;
+
+/* Fields shared by all containers (Metadata::OneCopy) and optional
+ * per-cell diagnostics.  Any table may be NULL when the configuration does
+ * not use it (emf without B, flag for robuster). */
+typedef struct pangu_shared {
+  double *const *alfven;   /* [nblocks * 3] face wave speeds */
+  double *const *emf;      /* [nblocks * 3] corner EMFs */
+  double *const *flag;     /* [nblocks] 1.0 ok / 0.0 recovery failed */
+  uint8_t *const *mask;    /* [nblocks] floor bits of the last fix_primitive */
+  int32_t *const *c2p_status; /* [nblocks] recovery status code (interior) */
+  int32_t *const *c2p_iters;  /* [nblocks] recovery iterations (interior) */
+  double *const *flag_recovery; /* [nblocks] flag as left by recovery, before fix_recovery */
+} pangu_shared;
+
+typedef struct pangu_ctx pangu_ctx;
+
+/* Creates a context for one block partition: validates the configuration
+ * (replaces the PARTHENON_REQUIRE_THROWS / std::invalid_argument checks of
+ * package_registration.cc:38-40, riemann_solver.cc:25, heating_model.h:118),
+ * builds the FP64 geometry tables on the host and uploads them. */
+int pangu_create(const pangu_config *cfg, const pangu_block *blocks, int nblocks, int device,
+                 pangu_ctx **out);
+int pangu_destroy(pangu_ctx *ctx);
+const char *pangu_last_error(const pangu_ctx *ctx);
+
+/* Registers a container's pointer tables (copied to the device once);
+ * returns a handle used by the task functions.  Pointers must stay valid. */
+int pangu_register_container(pangu_ctx *ctx, const pangu_container *c, int *handle);
+int pangu_register_shared(pangu_ctx *ctx, const pangu_shared *s);
+
+/* ---- task functions (one per reference task) ---- */
+/* FixPrimitive (fixer/primitive_fixer.cc:15) */
+int pangu_fix_primitive(pangu_ctx *ctx, int md, void *stream);
+/* CalculateConservative (riemann_solver/conservative.cc:15) */
+int pangu_calculate_conservative(pangu_ctx *ctx, int md, void *stream);
+/* CalculateFluxes -> CalculateLAXF / CalculateHLL (riemann_solver/riemann_solver.cc:13) */
+int pangu_calculate_fluxes(pangu_ctx *ctx, int md, void *stream);
+/* ConstraintedTransport (constrained_transport/constrained_transport.cc:13) */
+int pangu_constrained_transport(pangu_ctx *ctx, int md, void *stream);
+/* Parthenon FluxDivergence(mc0, mdudt) (task_list/ideal_grmhd.cc:73) */
+int pangu_flux_divergence(pangu_ctx *ctx, int md_in, int md_dudt, void *stream);
+/* Parthenon UpdateIndependentData(mc_init, mdudt, beta*dt, mc1) (ideal_grmhd.cc:75) */
+int pangu_update_independent(pangu_ctx *ctx, int md_init, int md_dudt, double beta_dt, int md_out,
+                             void *stream);
+/* AddGeometricSource(md, dt) (riemann_solver/source_term.cc:15) */
+int pangu_add_geometric_source(pangu_ctx *ctx, int md, double beta_dt, void *stream);
+/* Recovery (recovery/invertor.cc:18) */
+int pangu_recovery(pangu_ctx *ctx, int md, void *stream);
+/* ApplyElectronHeating (riemann_solver/electron_heating.cc:14) */
+int pangu_apply_electron_heating(pangu_ctx *ctx, int md, void *stream);
+/* FixRecovery (fixer/recovery_fixer.cc:12), serial lexicographic semantics */
+int pangu_fix_recovery(pangu_ctx *ctx, int md, void *stream);
+/* EstimateTimestepMesh (initialization/timestep_estimation.cc:13); synchronous
+ * like the reference's par_reduce: writes the estimate to *dt_out. */
+int pangu_estimate_timestep(pangu_ctx *ctx, int md, double *dt_out, void *stream);
+
+/* ---- Parthenon-side pieces the path needs on one GPU ---- */
+/* Same-level ghost exchange (face/edge/corner neighbours, periodic wrap) of
+ * the fields read in ghost zones: primitives and flag
+ * (AddBoundaryExchangeTasks, ideal_grmhd.cc:78). */
+int pangu_exchange_ghosts(pangu_ctx *ctx, int md, void *stream);
+/* Physical outflow / reflecting boundaries (ApplyBoundaryConditionsMD, ideal_grmhd.cc:104). */
+int pangu_apply_boundary_conditions(pangu_ctx *ctx, int md, void *stream);
+
+/* ---- fused stage (performance path; no reference counterpart) ----
+ * Runs region 1 + region 2 of one RK stage (ideal_grmhd.cc:52-116) with
+ * fused kernels: fix+cons, three flux sweeps, CT, flux-div+update+source,
+ * exchange, recovery+heating, fix_recovery and boundaries.  dt_state is a
+ * device array {dt, time}: the stage reads beta*dt from it, and when
+ * last_stage != 0 it also estimates the timestep and applies the dt control
+ * time += dt, dt = min(dt_factor*dt, estimate) in place.  Nothing is read
+ * back to the host, so a whole cycle can be captured in a hipGraph and
+ * replayed.  exchange_after: 0 = reference order (after the update),
+ * 1 = after recovery. */
+int pangu_stage(pangu_ctx *ctx, int md_init, int md0, int md1, double beta, double *dt_state,
+                double dt_factor, int last_stage, int exchange_after, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PANGU_HIP_H */

@@ -1,0 +1,84 @@
+/*
+ * pangu_sim: host driver above the pangu_hip C ABI.
+ *
+ * It restates what Parthenon does around PANGU's task list on one GPU: the
+ * uniform block mesh, the "base"/"1"/"dUdt" stage containers, the RK2 stage
+ * sequence of Simulator::MakeTaskCollection (pangu/src/task_list/
+ * ideal_grmhd.cc:23-120), the initial data of the FM torus / Orszag-Tang
+ * problems and dt control.  Field arrays live in device memory, one arena
+ * per field laid out [block][component][k][j][i] over ghosted blocks.
+ */
+#ifndef PANGU_SIM_H
+#define PANGU_SIM_H
+#include <stdint.h>
+
+#include "pangu_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Run configuration: mesh, time, physics, problem.  Field meanings follow
+ * PANGU's input file. */
+typedef struct pangu_sim_cfg {
+  int nx[3];
+  int nb[3];
+  int ng;
+  double xmin[3], xmax[3];
+  int bc[3][2];
+  double dt_init, dt_factor;
+  int metric;
+  double a, h;
+  double gamma, cfl;
+  int riemann, limiter, recovery;
+  int enable_B, enable_heating;
+  double density_floor, energy_floor, density_floor_pow, energy_floor_pow;
+  double sigma_max, lorentz_max;
+  int heat_model;
+  double fel_constant, gamma_e, gamma_p, ratio_min, ratio_max, fel_0;
+  int limit_kel, suppress_highb_heat, enforce_positive_dissipation;
+  int excise;
+  double r_excise, dexcise, pexcise;
+  int stage_init;      /* 0: stage container starts zeroed, 1: copied from base */
+  int exchange_after;  /* 0: exchange after the update (reference order), 1: after recovery */
+  int cache_geometry;  /* unused by the device path (tables are always used) */
+  int problem;         /* 1 Orszag-Tang (unit box), 2 FM torus */
+  double rin, rmax, kappa, perturbation, beta, aphi_cut;
+  uint64_t seed;
+  int nthreads;
+} pangu_sim_cfg;
+
+enum { PANGU_SIM_TASKS = 0, PANGU_SIM_FUSED = 1, PANGU_SIM_GRAPH = 2 };
+
+typedef struct pangu_sim pangu_sim;
+
+int pangu_sim_create(const pangu_sim_cfg *cfg, int device, pangu_sim **out);
+int pangu_sim_destroy(pangu_sim *s);
+int pangu_sim_init(pangu_sim *s);
+/* Runs ncycles RK2 cycles in the given mode (tasks: one C-ABI call per
+ * reference task with a host-side dt; fused: pangu_stage with device dt;
+ * graph: one captured cycle replayed).  Returns when the work is enqueued;
+ * pangu_sim_sync waits. */
+int pangu_sim_step(pangu_sim *s, int ncycles, int mode);
+int pangu_sim_sync(pangu_sim *s);
+/* Runs ncycles and returns the elapsed milliseconds between HIP events
+ * recorded on the simulation stream around them. */
+int pangu_sim_timed_cycles(pangu_sim *s, int ncycles, int mode, double *ms);
+/* Times each kernel phase of one fused stage on the simulation stream:
+ * ms_out[n] = mean milliseconds over reps of phase n (see pangu_sim_phase_name). */
+int pangu_sim_time_phases(pangu_sim *s, int reps, double *ms_out, int max_phases);
+const char *pangu_sim_phase_name(int n);
+/* One task on container c0 (c1 output of update/source), host beta*dt. */
+int pangu_sim_task(pangu_sim *s, int task, int c0, int c1, double beta_dt, double *out);
+int64_t pangu_sim_field_size(pangu_sim *s, int field);
+int pangu_sim_get(pangu_sim *s, int field, int container, void *out);
+int pangu_sim_set(pangu_sim *s, int field, int container, const void *in);
+double pangu_sim_time(pangu_sim *s);
+double pangu_sim_dt(pangu_sim *s);
+int pangu_sim_block_geom(pangu_sim *s, int b, double *xg_dx_out, int *n_out);
+const char *pangu_sim_last_error(pangu_sim *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

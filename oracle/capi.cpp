@@ -1,0 +1,195 @@
+// extern "C" surface of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+#include <cstring>
+
+#include "oracle.h"
+#include "oracle_api.h"
+#include "oracle_sim.h"
+
+using namespace oracle;
+
+extern "C" {
+
+void *oracle_sim_create(const oracle_cfg *cfg) {
+  if (!cfg) return nullptr;
+  for (int d = 0; d < 3; ++d)
+    if (cfg->nb[d] <= 0 || cfg->nx[d] % cfg->nb[d] != 0) return nullptr;
+  Sim *s = new Sim();
+  s->setup(*cfg);
+  return s;
+}
+
+void oracle_sim_destroy(void *sim) { delete (Sim *)sim; }
+
+int oracle_sim_init(void *sim) { return ((Sim *)sim)->init_problem(); }
+
+int oracle_sim_step(void *sim, int n) {
+  Sim *s = (Sim *)sim;
+  for (int i = 0; i < n; ++i) s->cycle();
+  return 0;
+}
+
+int oracle_sim_task(void *sim, int task, int c0, int c1, double beta_dt, double *out) {
+  Sim *s = (Sim *)sim;
+  s->ensure_stage_container();
+  switch (task) {
+    case ORT_FIX_PRIMITIVE: s->fix_primitive(c0); break;
+    case ORT_CONSERVATIVE: s->calc_conservative(c0); break;
+    case ORT_FLUXES: s->fluxes(c0); break;
+    case ORT_CT: s->ct(); break;
+    case ORT_FLUX_DIV: s->flux_div(); break;
+    case ORT_UPDATE: s->update(beta_dt, c1); break;
+    case ORT_SOURCE: s->source(c1, beta_dt); break;
+    case ORT_RECOVERY: s->recovery(c1); break;
+    case ORT_HEATING: s->heating(c1); break;
+    case ORT_FIX_RECOVERY: s->fix_recovery(c1); break;
+    case ORT_EXCHANGE: s->exchange(c1); break;
+    case ORT_BC: s->apply_bc(c1); break;
+    case ORT_TIMESTEP:
+      if (out) *out = s->estimate_dt();
+      break;
+    default: return -1;
+  }
+  return 0;
+}
+
+static void *field_ptr(Sim *s, int field, int c, int64_t *bytes) {
+  const int64_t n = s->total_cells;
+  switch (field) {
+    case ORF_PRIM: *bytes = n * s->np * 8; return s->P[c].data();
+    case ORF_CONS: *bytes = n * s->nc * 8; return s->U[c].data();
+    case ORF_FLUX1: *bytes = n * s->nc * 8; return s->F[0].data();
+    case ORF_FLUX2: *bytes = n * s->nc * 8; return s->F[1].data();
+    case ORF_FLUX3: *bytes = n * s->nc * 8; return s->F[2].data();
+    case ORF_ALFVEN: *bytes = n * 3 * 8; return s->alfven.data();
+    case ORF_EMF: *bytes = n * 3 * 8; return s->emf.data();
+    case ORF_FLAG: *bytes = n * 8; return s->flag.data();
+    case ORF_MASK: *bytes = n; return s->mask.data();
+    case ORF_C2P_STATUS: *bytes = n * 4; return s->c2p_status.data();
+    case ORF_C2P_ITERS: *bytes = n * 4; return s->c2p_iters.data();
+    case ORF_FLAG_RECOVERY: *bytes = n * 8; return s->flag_rec.data();
+    case ORF_DUDT: *bytes = n * s->nc * 8; return s->dudt.data();
+  }
+  *bytes = 0;
+  return nullptr;
+}
+
+int64_t oracle_sim_field_size(void *sim, int field) {
+  int64_t b = 0;
+  field_ptr((Sim *)sim, field, 0, &b);
+  return b;
+}
+
+int oracle_sim_get(void *sim, int field, int c, void *out) {
+  int64_t b = 0;
+  void *p = field_ptr((Sim *)sim, field, c, &b);
+  if (!p) return -1;
+  std::memcpy(out, p, (size_t)b);
+  return 0;
+}
+
+int oracle_sim_set(void *sim, int field, int c, const void *in) {
+  int64_t b = 0;
+  void *p = field_ptr((Sim *)sim, field, c, &b);
+  if (!p) return -1;
+  std::memcpy(p, in, (size_t)b);
+  ((Sim *)sim)->stage1_ready = true;
+  return 0;
+}
+
+double oracle_sim_time(void *sim) { return ((Sim *)sim)->time; }
+double oracle_sim_dt(void *sim) { return ((Sim *)sim)->dt; }
+int oracle_sim_ncycle(void *sim) { return ((Sim *)sim)->ncycle; }
+
+int oracle_sim_block_geom(void *sim, int b, double *xd, int *n) {
+  Sim *s = (Sim *)sim;
+  if (b < 0 || b >= s->nblk) return -1;
+  const Block &B = s->blk[b];
+  for (int d = 0; d < 3; ++d) {
+    xd[d] = B.xg[d];
+    xd[3 + d] = B.dx[d];
+    n[d] = B.n[d];
+    n[3 + d] = B.s[d];
+    n[6 + d] = B.e[d];
+  }
+  return 0;
+}
+
+int oracle_metric_at(int kind, const double x[4], double a, double h, double gcov[16],
+                     double gcon[16], double *gdet) {
+  real gc[4][4], gn[4][4];
+  for (int i = 0; i < 16; ++i) gn[i / 4][i % 4] = 0.0;
+  const int ok = metric_at(kind, x, a, h, gc, gn, gdet);
+  std::memcpy(gcov, gc, sizeof(gc));
+  std::memcpy(gcon, gn, sizeof(gn));
+  return ok;
+}
+
+void oracle_christoffel(int kind, const double x[4], double a, double h, double conn[64]) {
+  real c[4][4][4];
+  christoffel(kind, x, a, h, c);
+  std::memcpy(conn, c, sizeof(c));
+}
+
+void oracle_state(const double prim[10], const double gcov[16], const double gcon[16],
+                  double out[17]) {
+  State s;
+  calc_state(prim, (const real(*)[4])gcov, (const real(*)[4])gcon, s);
+  for (int i = 0; i < 4; ++i) {
+    out[i] = s.ucon[i];
+    out[4 + i] = s.ucov[i];
+    out[8 + i] = s.bcon[i];
+    out[12 + i] = s.bcov[i];
+  }
+  out[16] = s.bsq;
+}
+
+void oracle_flux(double gamma, const double prim[10], const double gcov[16],
+                 const double gcon[16], double gdet, int dir, double flux[10]) {
+  contravariant_flux(gamma, prim, (const real(*)[4])gcov, (const real(*)[4])gcon, gdet, dir, flux);
+}
+
+void oracle_fms(double gamma, const double prim[10], const double gcov[16],
+                const double gcon[16], int dir, double out[2]) {
+  fast_ms_speed(gamma, prim, (const real(*)[4])gcov, (const real(*)[4])gcon, dir, &out[0],
+                &out[1]);
+}
+
+void oracle_recon(int limiter, const double q[5], double out[2]) {
+  if (limiter == LIM_MC)
+    recon_mc(q[0], q[1], q[2], q[3], q[4], &out[0], &out[1]);
+  else
+    recon_ppm4(q[0], q[1], q[2], q[3], q[4], &out[0], &out[1]);
+}
+
+int oracle_c2p(int scheme, const double U[9], double prim[9], double gamma, const double gcov[16],
+               const double gcon[16], int *iters) {
+  real Uc[9];
+  std::memcpy(Uc, U, sizeof(Uc));
+  const real(*gc)[4] = (const real(*)[4])gcov;
+  const real(*gn)[4] = (const real(*)[4])gcon;
+  switch (scheme) {
+    case 0: return c2p_2d(Uc, prim, gamma, gc, gn, iters);
+    case 1: return c2p_1dvsq(Uc, prim, gamma, gc, gn, iters);
+    case 2: return c2p_1d(Uc, prim, gamma, gc, gn, iters);
+    case 3: return c2p_kastaun(Uc, prim, gamma, gc, gn, iters);
+  }
+  return -1;
+}
+
+double oracle_heating(const oracle_cfg *c, const double cell[6]) {
+  Params p{};
+  p.gamma = c->gamma;
+  p.heat_model = c->heat_model;
+  p.fel_constant = c->fel_constant;
+  p.gamma_e = c->gamma_e;
+  p.gamma_p = c->gamma_p;
+  p.ratio_min = c->ratio_min;
+  p.ratio_max = c->ratio_max;
+  p.limit_kel = c->limit_kel;
+  p.suppress_highb_heat = c->suppress_highb_heat;
+  p.enforce_positive_dissipation = c->enforce_positive_dissipation;
+  HeatCell hc{cell[0], cell[1], cell[2], cell[3], cell[4], cell[5]};
+  return heating_apply(p, hc);
+}
+
+}  // extern "C"

@@ -1,0 +1,301 @@
+// Oracle: initial data.  TEST INFRASTRUCTURE ONLY.
+//   problem 1: relativistic Orszag-Tang vortex on [0,1]^2 (Minkowski):
+//     rho = 25/(36 pi), p = 5/(12 pi), v = 0.5(-sin 2 pi y, sin 2 pi x),
+//     B = (-sin 2 pi y, sin 4 pi x)/sqrt(4 pi).  (Not the reference's OT
+//     setup, which uses [-pi,pi]^2; this is the unit-box variant.)
+//   problem 2: Fishbone-Moncrief torus in MKS with a SANE vector potential
+//     (pangu/problem/fm_torus_mks/problem_generator.cpp:22-558); the random
+//     pressure perturbation uses a counter-based generator keyed by
+//     (seed, global cell index) instead of a thread-order-dependent pool.
+#include <cfloat>
+#include <cstring>
+
+#include "oracle.h"
+#include "oracle_sim.h"
+
+namespace oracle {
+
+// splitmix64 finaliser of (seed, cell id) -> uniform double in [-1, 1).
+static double keyed_uniform(uint64_t seed, uint64_t id) {
+  uint64_t z = id + seed * 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  z = z ^ (z >> 31);
+  const double u = (double)(z >> 11) * (1.0 / 9007199254740992.0);
+  return -1.0 + 2.0 * u;
+}
+
+// fm_torus_mks/problem_generator.cpp:22-35.
+static real lfish_calc(real r, real a) {
+  return (((std::pow(a, 2.) - 2. * a * std::sqrt(r) + std::pow(r, 2.)) *
+           ((-2. * a * r * (std::pow(a, 2.) - 2. * a * std::sqrt(r) + std::pow(r, 2.))) /
+                std::sqrt(2. * a * std::sqrt(r) + (-3. + r) * r) +
+            ((a + (-2. + r) * std::sqrt(r)) * (std::pow(r, 3.) + std::pow(a, 2.) * (2. + r))) /
+                std::sqrt(1 + (2. * a) / std::pow(r, 1.5) - 3. / r))) /
+          (std::pow(r, 3.) * std::sqrt(2. * a * std::sqrt(r) + (-3. + r) * r) *
+           (std::pow(a, 2.) + (-2. + r) * r)));
+}
+
+// :37-48.
+static real solve_ut(const real g[4][4], real u1, real u2, real u3) {
+  const real AA = g[0][0];
+  const real BB = 2.0 * (g[0][1] * u1 + g[0][2] * u2 + g[0][3] * u3);
+  const real CC = 1.0 + g[1][1] * u1 * u1 + g[2][2] * u2 * u2 + g[3][3] * u3 * u3 +
+                  2.0 * (g[1][2] * u1 * u2 + g[1][3] * u1 * u3 + g[2][3] * u2 * u3);
+  const real discr = BB * BB - 4.0 * AA * CC;
+  return (-BB - std::sqrt(discr)) / (2.0 * AA);
+}
+
+// :50-101.
+static void bl_to_code_ucon(const real x[4], real h, real a, const real ubl[4], real uc[4]) {
+  real y[4];
+  physical_coords(MET_BL, x, y, h);
+  const real r = y[1];
+  real tr[4][4], tmp[4], dxdxp[4][4], dxpdx[4][4];
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) tr[i][j] = (real)(i == j);
+  tr[0][1] = 2.0 * r / (r * r - 2.0 * r + a * a);
+  tr[3][1] = a / (r * r - 2.0 * r + a * a);
+  for (int i = 0; i < 4; i++) tmp[i] = 0.0;
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) tmp[i] += tr[i][j] * ubl[j];
+  for (int c = 0; c < 4; c++) {
+    real xh[4], xl[4], yh[4], yl[4];
+    for (int n = 0; n < 4; ++n) { xh[n] = x[n]; xl[n] = x[n]; }
+    xh[c] += 1e-5;
+    xl[c] -= 1e-5;
+    physical_coords(MET_BL, xh, yh, h);
+    physical_coords(MET_BL, xl, yl, h);
+    for (int r2 = 0; r2 < 4; r2++) dxdxp[r2][c] = (yh[r2] - yl[r2]) / (xh[c] - xl[c]);
+  }
+  invert4(dxdxp, dxpdx);
+  for (int i = 0; i < 4; i++) {
+    uc[i] = 0.0;
+    for (int j = 0; j < 4; j++) uc[i] += dxpdx[i][j] * tmp[j];
+  }
+}
+
+static void init_ot(Sim &S) {
+  const double gam = S.p.gamma;
+  for (int b = 0; b < S.nblk; ++b) {
+    const Block &B = S.blk[b];
+    for (int k = 0; k < B.n[2]; ++k)
+      for (int j = 0; j < B.n[1]; ++j)
+        for (int i = 0; i < B.n[0]; ++i) {
+          const int64_t id = B.idx(k, j, i);
+          const double x = Xc(B, 0, i), y = Xc(B, 1, j);
+          const double pi = M_PI;
+          const double rho = 25.0 / (36.0 * pi);
+          const double pr = 5.0 / (12.0 * pi);
+          const double u = pr / (gam - 1.0);
+          const double vx = -0.5 * std::sin(2.0 * pi * y);
+          const double vy = 0.5 * std::sin(2.0 * pi * x);
+          const double v2 = vx * vx + vy * vy;
+          const double W = 1.0 / std::sqrt(1.0 - v2);
+          const double b0 = 1.0 / std::sqrt(4.0 * pi);
+          S.Pc(0, b, 0)[id] = rho;
+          S.Pc(0, b, 1)[id] = u;
+          S.Pc(0, b, 2)[id] = W * vx;
+          S.Pc(0, b, 3)[id] = W * vy;
+          S.Pc(0, b, 4)[id] = 0.0;
+          if (S.p.enable_B) {
+            S.Pc(0, b, S.iBX)[id] = -b0 * std::sin(2.0 * pi * y);
+            S.Pc(0, b, S.iBX + 1)[id] = b0 * std::sin(4.0 * pi * x);
+            S.Pc(0, b, S.iBX + 2)[id] = 0.0;
+          }
+          const double ent = (gam - 1.0) * u * std::pow(rho, -gam);
+          S.Pc(0, b, S.iENT)[id] = ent;
+          if (S.iKEL >= 0) S.Pc(0, b, S.iKEL)[id] = S.cfg.fel_0 * ent;
+        }
+  }
+}
+
+static int init_torus(Sim &S) {
+  const oracle_cfg &c = S.cfg;
+  const double gam = S.p.gamma, a = S.p.a, h = S.p.h;
+  const double rin = c.rin, kappa = c.kappa, pert = c.perturbation;
+  const double a2 = a * a;
+  const double l = lfish_calc(c.rmax, a);
+  const double thin = M_PI_2;
+  const double sthin = std::sin(thin), cthin = std::cos(thin);
+  const double DDin = rin * rin - 2. * rin + a * a;
+  const double AAin = (rin * rin + a * a) * (rin * rin + a * a) - DDin * a * a * sthin * sthin;
+  const double SSin = rin * rin + a * a * cthin * cthin;
+  // global extended index for the keyed generator
+  int G[3], NX[3];
+  for (int d = 0; d < 3; ++d) {
+    G[d] = S.blk[0].ng[d];
+    NX[d] = c.nx[d] + 2 * G[d];
+  }
+  // problem_generator.cpp:161-270 (entire block, ghosts included)
+  for (int b = 0; b < S.nblk; ++b) {
+    const Block &B = S.blk[b];
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int k = 0; k < B.n[2]; ++k)
+      for (int j = 0; j < B.n[1]; ++j)
+        for (int i = 0; i < B.n[0]; ++i) {
+          const int64_t id = B.idx(k, j, i);
+          const int64_t gi = B.gofs[0] + i, gj = B.gofs[1] + j, gk = B.gofs[2] + k;
+          const uint64_t gid = (uint64_t)((gk * NX[1] + gj) * NX[0] + gi);
+          const double rnd = keyed_uniform(c.seed, gid);
+          const double x1 = Xc(B, 0, i), x2 = Xc(B, 1, j), x3 = Xc(B, 2, k);
+          const double xc[4] = {0.0, x1, x2, x3};
+          double y[4];
+          physical_coords(MET_MKS, xc, y, h);
+          const double r = y[1], th = y[2];
+          const double sth = std::sin(th), cth = std::cos(th);
+          double gcov[4][4], gbl[4][4], gcon[4][4];
+          code_metric(MET_MKS, xc, gcov, h, a);
+          code_metric(MET_BL, xc, gbl, h, a);
+          invert4(gcov, gcon);
+          const double alpha = 1.0 / std::sqrt(-gcon[0][0]);
+          const double beta1 = gcon[0][1] * alpha * alpha;
+          const double beta2 = gcon[0][2] * alpha * alpha;
+          const double beta3 = gcon[0][3] * alpha * alpha;
+          double rho = 0.0, eint = 0.0, ent = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0;
+          const double DD = r * r - 2.0 * r + a2;
+          const double AA = (r * r + a2) * (r * r + a2) - DD * a2 * sth * sth;
+          const double SS = r * r + a2 * cth * cth;
+          double lnh = 1.0;
+          if (r >= rin) {
+            lnh = 0.5 * std::log((1. + std::sqrt(1. + 4. * (l * l * SS * SS) * DD /
+                                                         (AA * sth * AA * sth))) /
+                                 (SS * DD / AA)) -
+                  0.5 * std::sqrt(1. + 4. * (l * l * SS * SS) * DD / (AA * AA * sth * sth)) -
+                  2. * a * r * l / AA -
+                  (0.5 * std::log((1. + std::sqrt(1. + 4. * (l * l * SSin * SSin) * DDin /
+                                                           (AAin * AAin * sthin * sthin))) /
+                                  (SSin * DDin / AAin)) -
+                   0.5 * std::sqrt(1. + 4. * (l * l * SSin * SSin) * DDin /
+                                            (AAin * AAin * sthin * sthin)) -
+                   2. * a * rin * l / AAin);
+          }
+          if (lnh >= 0.0 && r >= rin) {
+            const double hm1 = std::exp(lnh) - 1.0;
+            if (hm1 > 0.0) {
+              rho = std::pow(hm1 * (gam - 1.0) / (kappa * gam), 1.0 / (gam - 1.0));
+              eint = kappa * std::pow(rho, gam) / (gam - 1.0);
+              eint *= (1.0 + pert * rnd);
+              ent = (gam - 1) * eint * std::pow(rho, -gam);
+              const double e2c = SS * SS * DD / (AA * AA * sth * sth);
+              const double up1 = std::sqrt((-1.0 + std::sqrt(1.0 + 4.0 * l * l * e2c)) / 2.0);
+              const double upbl = 2.0 * a * r * std::sqrt(1.0 + up1 * up1) / std::sqrt(AA * SS * DD) +
+                                  std::sqrt(SS / AA) * up1 / sth;
+              const double u0bl = solve_ut(gbl, 0.0, 0.0, upbl);
+              const double ubl[4] = {u0bl, 0.0, 0.0, upbl};
+              double uc[4];
+              bl_to_code_ucon(xc, h, a, ubl, uc);
+              w1 = uc[1] + beta1 * uc[0];
+              w2 = uc[2] + beta2 * uc[0];
+              w3 = uc[3] + beta3 * uc[0];
+            }
+          }
+          S.Pc(0, b, 0)[id] = rho;
+          S.Pc(0, b, 1)[id] = eint;
+          S.Pc(0, b, 2)[id] = w1;
+          S.Pc(0, b, 3)[id] = w2;
+          S.Pc(0, b, 4)[id] = w3;
+          if (S.p.enable_B) {
+            S.Pc(0, b, S.iBX)[id] = 0.0;
+            S.Pc(0, b, S.iBX + 1)[id] = 0.0;
+            S.Pc(0, b, S.iBX + 2)[id] = 0.0;
+          }
+          S.Pc(0, b, S.iENT)[id] = ent;
+          if (S.iKEL >= 0) S.Pc(0, b, S.iKEL)[id] = c.fel_0 * ent;
+        }
+  }
+  // MeshPostInitialization (:291-558): maxima over entire blocks
+  double rhomax = 0.0, umax = 0.0;
+  for (int b = 0; b < S.nblk; ++b) {
+    const Block &B = S.blk[b];
+    double bm = 0.0, um = 0.0;
+    for (int64_t id = 0; id < B.cells; ++id) {
+      bm = kmax(bm, S.Pc(0, b, 0)[id]);
+      um = kmax(um, S.Pc(0, b, 1)[id]);
+    }
+    rhomax = kmax(rhomax, bm);
+    umax = kmax(umax, um);
+  }
+  for (int b = 0; b < S.nblk; ++b) {
+    const Block &B = S.blk[b];
+    for (int64_t id = 0; id < B.cells; ++id) {
+      S.Pc(0, b, 0)[id] /= rhomax;
+      S.Pc(0, b, 1)[id] /= rhomax;
+    }
+    if (!S.p.enable_B) continue;
+    // corner vector potential A_phi = max(<rho> - cut, 0) (:399-429, SANE)
+    std::vector<double> A((size_t)B.cells, 0.0);
+    for (int k = 0; k < B.n[2]; ++k)
+      for (int j = 1; j < B.n[1]; ++j)
+        for (int i = 1; i < B.n[0]; ++i) {
+          const double *rho = S.Pc(0, b, 0);
+          const double avg = 0.25 * (rho[B.idx(k, j, i)] + rho[B.idx(k, j, i - 1)] +
+                                     rho[B.idx(k, j - 1, i)] + rho[B.idx(k, j - 1, i - 1)]);
+          const double ex = avg - c.aphi_cut;
+          A[B.idx(k, j, i)] = (ex > 0.0) ? ex : 0.0;
+        }
+    // B from curl A (:431-458)
+    for (int k = B.s[2]; k <= B.e[2]; ++k)
+      for (int j = B.s[1]; j <= B.e[1]; ++j)
+        for (int i = B.s[0]; i <= B.e[0]; ++i) {
+          const double xc[4] = {0.0, Xc(B, 0, i), Xc(B, 1, j), Xc(B, 2, k)};
+          double gcov[4][4], gcon[4][4];
+          code_metric(MET_MKS, xc, gcov, h, a);
+          invert4(gcov, gcon);
+          const double gdet = determinant4(gcov);
+          const double sg = std::sqrt(std::fabs(gdet));
+          const double dx1 = B.dx[0], dx2 = B.dx[1];
+          const double a00 = A[B.idx(k, j - 1, i - 1)], a01 = A[B.idx(k, j, i - 1)];
+          const double a10 = A[B.idx(k, j - 1, i)], a11 = A[B.idx(k, j, i)];
+          S.Pc(0, b, S.iBX)[B.idx(k, j, i)] = -(a00 - a01 + a10 - a11) / (2.0 * dx2 * sg);
+          S.Pc(0, b, S.iBX + 1)[B.idx(k, j, i)] = (a00 + a01 - a10 - a11) / (2.0 * dx1 * sg);
+        }
+  }
+  if (!S.p.enable_B) return 0;
+  // beta normalisation (:464-551)
+  double bsqmax = 0.0;
+  for (int b = 0; b < S.nblk; ++b) {
+    const Block &B = S.blk[b];
+    double bm = 0.0;
+    for (int k = B.s[2]; k <= B.e[2]; ++k)
+      for (int j = B.s[1]; j <= B.e[1]; ++j)
+        for (int i = B.s[0]; i <= B.e[0]; ++i) {
+          const double xc[4] = {0.0, Xc(B, 0, i), Xc(B, 1, j), Xc(B, 2, k)};
+          double gcov[4][4], gcon[4][4];
+          code_metric(MET_MKS, xc, gcov, h, a);
+          invert4(gcov, gcon);
+          real pc[NPRIM];
+          S.load_prim(0, b, B.idx(k, j, i), pc);
+          State st;
+          calc_state(pc, gcov, gcon, st);
+          bm = kmax(bm, st.bsq);
+        }
+    bsqmax = kmax(bsqmax, bm);
+  }
+  const double beta_min = 2.0 * (gam - 1.0) * (umax / rhomax) / (bsqmax + 1.0e-30);
+  const double norm = std::sqrt(beta_min / c.beta);
+  for (int b = 0; b < S.nblk; ++b) {
+    const Block &B = S.blk[b];
+    for (int k = B.s[2]; k <= B.e[2]; ++k)
+      for (int j = B.s[1]; j <= B.e[1]; ++j)
+        for (int i = B.s[0]; i <= B.e[0]; ++i)
+          for (int q = 0; q < 3; ++q) S.Pc(0, b, S.iBX + q)[B.idx(k, j, i)] *= norm;
+  }
+  return 0;
+}
+
+// ProblemGenerator + MeshPostInitialization, then the initial ghost exchange
+// and physical boundary conditions of the "base" container.
+int Sim::init_problem() {
+  if (cfg.problem == 1)
+    init_ot(*this);
+  else if (cfg.problem == 2)
+    init_torus(*this);
+  else
+    return -1;
+  exchange(0);
+  apply_bc(0);
+  return 0;
+}
+
+}  // namespace oracle

@@ -1,0 +1,67 @@
+"""Standard run configurations.
+
+Each is a plain dict whose keys follow PANGU's input-file parameters
+(<parthenon/mesh>, <parthenon/time>, <core>, <metric>, <electron>,
+<fm_torus>).  The same dict drives the HIP simulation and, in tests, the
+CPU oracle.
+"""
+import math
+
+_BASE = dict(
+    ng=4, dt_init=1e-5, dt_factor=1.3,
+    metric=0, a=0.0, h=0.0, gamma=5.0 / 3.0, cfl=0.4,
+    riemann=0, limiter=1, recovery=0, enable_B=1, enable_heating=0,
+    density_floor=1.0e-6, energy_floor=1.0e-8, density_floor_pow=-1.5, energy_floor_pow=-2.5,
+    sigma_max=50.0, lorentz_max=50.0,
+    heat_model=0, fel_constant=0.1, gamma_e=4.0 / 3.0, gamma_p=5.0 / 3.0,
+    ratio_min=0.001, ratio_max=1000.0, fel_0=0.1,
+    limit_kel=1, suppress_highb_heat=0, enforce_positive_dissipation=0,
+    excise=0, r_excise=1.0, dexcise=1.0e-8, pexcise=0.333e-12,
+    stage_init=0, exchange_after=0, cache_geometry=1,
+    problem=1, rin=6.0, rmax=12.0, kappa=1.0e-3, perturbation=4.0e-2, beta=100.0,
+    aphi_cut=0.2, seed=0,
+)
+
+PERIODIC, OUTFLOW, REFLECT = 0, 1, 2
+METRIC = dict(minkowski=0, bl=1, cks=2, mks=3)
+RIEMANN = dict(laxf=0, hll=1)
+LIMITER = dict(mc=0, ppm4=1)
+RECOVERY = dict(chainer=0, robuster=1, kastaun=1, onedvsq=2)
+
+
+def orszag_tang(n=256, nb=(1, 1, 1)):
+    """Config 1: 2D relativistic Orszag-Tang on [0,1]^2, Minkowski, MC,
+    LAXF, CT, 1Dvsq recovery, RK2, CFL 0.4."""
+    d = dict(_BASE)
+    d.update(nx=[n, n, 1], nb=list(nb), xmin=[0.0, 0.0, -0.5], xmax=[1.0, 1.0, 0.5],
+             bc=[[PERIODIC] * 2] * 3, metric=0, riemann=0, limiter=0, recovery=2,
+             enable_heating=0, problem=1, stage_init=1)
+    return d
+
+
+def _torus_common(d):
+    d.update(metric=3, a=0.9375, h=0.7, gamma=5.0 / 3.0, cfl=0.4, enable_B=1,
+             enable_heating=1, density_floor=1.0e-5, energy_floor=4.641588833612779e-9,
+             problem=2, xmin=[0.2, -1.0, -3.1415926535897], xmax=[6.0, 1.0, 3.1415926535897],
+             bc=[[OUTFLOW, OUTFLOW], [REFLECT, REFLECT], [PERIODIC, PERIODIC]])
+    return d
+
+
+def torus2d(nx1=512, nx2=256, nb=(1, 1, 1)):
+    """Config 2: 2D FM torus, MKS a=0.9375 h=0.7, PPM4, LAXF, chainer,
+    heating, CT, RK2."""
+    d = _torus_common(dict(_BASE))
+    d.update(nx=[nx1, nx2, 1], nb=list(nb), riemann=0, limiter=1, recovery=0)
+    return d
+
+
+def torus3d(nx=(256, 128, 128), nb=(4, 2, 4)):
+    """Config 3 (headline): 3D FM torus 256x128x128 in 32 meshblocks of
+    64x64x32, PPM4, HLL, Kastaun, CT, heating, RK2."""
+    d = _torus_common(dict(_BASE))
+    d.update(nx=list(nx), nb=list(nb), riemann=1, limiter=1, recovery=1)
+    return d
+
+
+def zones(d):
+    return d["nx"][0] * d["nx"][1] * d["nx"][2]

@@ -1,0 +1,64 @@
+// Shared host/device definitions of pangu_hip.
+#pragma once
+#include <stdint.h>
+
+#include "../../include/pangu_hip.h"
+
+namespace pangu {
+
+// Slot order of a full state vector (variable_mnemonics.h:7-10).
+enum Slot { RHO = 0, ENY, UX1, UX2, UX3, BX1, BX2, BX3, ENT, KEL, NPRIM };
+
+// Geometry table components.  Every derived quantity is computed on the host
+// in FP64 with IEEE operations only, so device kernels consume the exact
+// values a per-cell recomputation would produce.
+//
+// Cell-centre table (one entry per ghosted (j,i), or (k,j,i) for x3-dependent
+// metrics):
+enum CentreComp {
+  CG_GCOV = 0,          // 16: gcov[r][c] at CG_GCOV + 4r + c
+  CG_GCON = 16,         // 16: gcon[r][c]
+  CG_SG = 32,           // sqrt|det g|
+  CG_ALPHA = 33,        // 1/sqrt(-gcon00)   (state lapse)
+  CG_LAPSE = 34,        // sqrt(-1/gcon00)   (recovery normal observer)
+  CG_RHOFL = 35,        // density_floor * max(r,1)^density_floor_pow
+  CG_ENGFL = 36,        // energy_floor  * max(r,1)^energy_floor_pow
+  CG_RAD = 37,          // r used by the floors / excision
+  CG_CONN = 38,         // 64: Gamma^a_{bc} at CG_CONN + 16a + 4b + c
+  CG_NCOMP = 102
+};
+// Face table (lower faces normal to direction d):
+enum FaceComp {
+  FG_GCOV = 0,    // 16
+  FG_GCON0 = 16,  // 4: gcon[0][0..3]
+  FG_GCONDD = 20, // gcon[d][d]
+  FG_SG = 21,     // sqrt|det g|
+  FG_ALPHA = 22,  // 1/sqrt(-gcon00)
+  FG_VALID = 23,  // 1.0, or 0.0 where the metric is singular (polar axis)
+  FG_NCOMP = 24
+};
+
+// Per-block launch descriptor (device copy).
+struct DevBlock {
+  int n[3];     // ghosted extent
+  int s[3];     // interior start
+  int e[3];     // interior end (inclusive)
+  int lx[3];
+  double dx[3];
+  int64_t ncell;        // n0*n1*n2
+  int geo;              // geometry set index
+  int pad;
+};
+
+// Geometry set (device copy): SoA tables, comp-major, (k?) j, i fastest.
+struct DevGeo {
+  const double *cen;    // CG_NCOMP * ncen
+  const double *face[3];// FG_NCOMP * nface[d]; face[2] may alias the centre set
+  int ni, nj, nk;       // table extent (nk = 1 for x3-independent metrics)
+  int has3d;
+  int64_t ncen;         // ni*nj*nk
+  int64_t nface[3];
+  int fi[3], fj[3];     // face table row lengths (ni+1 / nj+1 on the normal dim)
+};
+
+}  // namespace pangu

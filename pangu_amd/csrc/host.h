@@ -1,0 +1,52 @@
+// Host-only helpers of pangu_hip (no HIP headers: these files are built
+// with g++ -ffp-contract=off so host arithmetic is reproducible).
+#pragma once
+#include <stdint.h>
+
+#include <vector>
+
+#include "../../include/pangu_hip.h"
+
+namespace pangu {
+// Host block geometry (Parthenon UniformCartesian).
+struct HostBlock {
+  int nx[3], ng[3], n[3], s[3], e[3], lx[3];
+  double xg[3], dx[3];
+  int64_t cells;
+  double xc(int d, int idx) const { return xg[d] + (idx + 0.5) * dx[d]; }
+  double xf(int d, int idx) const { return xg[d] + idx * dx[d]; }
+  int64_t id(int k, int j, int i) const { return ((int64_t)k * n[1] + j) * n[0] + i; }
+};
+HostBlock make_host_block(const pangu_block &b);
+
+namespace geom {
+typedef double M4[4][4];
+void code_metric(int kind, const double x[4], M4 out, double h, double a);
+double det4(const M4 m);
+bool inv4(const M4 m, M4 out);
+void connection(int kind, const double x[4], const M4 gcon, double h, double a, double conn[64]);
+// Fill one centre / face table entry; `out` points at component 0 of the
+// entry and consecutive components are `stride` doubles apart.
+void fill_centre(const pangu_config &cfg, const double x[4], double *out, int64_t stride);
+void fill_face(const pangu_config &cfg, int d, const double x[4], double *out, int64_t stride);
+void fill_centre_table(const pangu_config &cfg, const HostBlock &H, int ni, int nj, int nk,
+                       bool has3d, double *out);
+void fill_face_table(const pangu_config &cfg, const HostBlock &H, int d, int fi, int fj, int fk,
+                     bool has3d, double *out);
+}  // namespace geom
+
+
+
+// Initial data written into host arrays laid out [block][comp][cell]
+// (entire ghosted blocks), for the "base" container.
+struct ProblemParams {
+  int problem;  // 1 Orszag-Tang (unit box), 2 FM torus
+  double rin, rmax, kappa, perturbation, beta, aphi_cut, fel_0;
+  uint64_t seed;
+  int mesh_nx[3];
+};
+int generate_problem(const pangu_config &cfg, const ProblemParams &pp,
+                     const std::vector<HostBlock> &blocks, int np, int iBX, int iENT, int iKEL,
+                     std::vector<double> &prim);
+
+}  // namespace pangu

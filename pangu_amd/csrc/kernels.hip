@@ -1,0 +1,995 @@
+// gfx950 kernels of the GRMHD per-RK-stage update path (FP64 vector ALU).
+//
+// Thread mapping: every kernel maps threadIdx.x to the contiguous index i
+// (64 lanes = one wavefront row, coalesced 512 B per load), threadIdx.y to
+// j and blockIdx.z to (block, k).  All three flux sweeps therefore load
+// coalesced rows; the reference's j- and k-innermost sweeps (laxf.cc:189,295)
+// would stride through memory on a GPU.
+#include <hip/hip_runtime.h>
+#include <float.h>
+
+#include "common.h"
+#include "kernels.h"
+#include "physics_dev.h"
+#include "recovery_dev.h"
+
+namespace pangu {
+
+// Packed component q -> slot, for the enabled field set
+// (package_registration.cc:84-96).
+template <bool HB, bool HK>
+__device__ __forceinline__ constexpr int slot_of(int q) {
+  return q < 5 ? q : (HB ? (q < 8 ? q : (q == 8 ? ENT : KEL)) : (q == 5 ? ENT : KEL));
+}
+template <bool HB, bool HK>
+__device__ __forceinline__ constexpr int ncomp() {
+  return 6 + (HB ? 3 : 0) + (HK ? 1 : 0);
+}
+template <bool HB>
+__device__ __forceinline__ constexpr int ient() {
+  return HB ? 8 : 5;
+}
+template <bool HB>
+__device__ __forceinline__ constexpr int ikel() {
+  return HB ? 9 : 6;
+}
+
+__device__ __forceinline__ int64_t cen_index(const DevGeo &G, int k, int j, int i) {
+  return ((int64_t)(G.has3d ? k : 0) * G.nj + j) * G.ni + i;
+}
+
+// Loads of the centre table.
+struct CenLoad {
+  double g[16], gc0[4], alpha, sg;
+};
+__device__ __forceinline__ void load_cen_state(const DevGeo &G, int64_t t, CenLoad &m) {
+  const double *c = G.cen;
+  const int64_t s = G.ncen;
+#pragma unroll
+  for (int n = 0; n < 16; ++n) m.g[n] = c[(CG_GCOV + n) * s + t];
+#pragma unroll
+  for (int n = 0; n < 4; ++n) m.gc0[n] = c[(CG_GCON + n) * s + t];
+  m.alpha = c[CG_ALPHA * s + t];
+  m.sg = c[CG_SG * s + t];
+}
+
+// ------------------------------------------------ fix_primitive + conservative
+// FixPrimitive over the entire block (primitive_fixer.cc:64-166) and, for
+// interior cells, CalculateConservative (conservative.cc:57-107).
+template <bool FIX, bool CONS, bool HB, bool HK>
+__global__ __launch_bounds__(256) void k_fix_cons(KCtx C, KFields F, uint8_t *const *mask) {
+  const int b = blockIdx.z / C.nk;
+  const int k = blockIdx.z % C.nk;
+  const DevBlock &B = C.blk[b];
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * 4 + threadIdx.y;
+  if (i >= B.n[0] || j >= B.n[1] || k >= B.n[2]) return;
+  const bool interior = i >= B.s[0] && i <= B.e[0] && j >= B.s[1] && j <= B.e[1] &&
+                        k >= B.s[2] && k <= B.e[2];
+  if (!FIX && !interior) return;
+  const DevGeo &G = C.geo[B.geo];
+  const int64_t t = cen_index(G, k, j, i);
+  const int64_t id = ((int64_t)k * B.n[1] + j) * B.n[0] + i;
+  constexpr int NC = ncomp<HB, HK>();
+  double *pp[NC];
+#pragma unroll
+  for (int q = 0; q < NC; ++q) pp[q] = F.prim[b * NC + q];
+  double p[NPRIM];
+#pragma unroll
+  for (int s = 0; s < NPRIM; ++s) p[s] = 0.0;
+#pragma unroll
+  for (int q = 0; q < NC; ++q) p[slot_of<HB, HK>(q)] = pp[q][id];
+  CenLoad m;
+  load_cen_state(G, t, m);
+  const Phys &P = C.ph;
+  if (FIX) {
+    const double small = 1.0e-20;
+    const double sms = kmax(P.sigma_max, small);
+    const double rho_before = kmax(p[RHO], small);
+    double rho_floor = G.cen[CG_RHOFL * G.ncen + t];
+    const double eng_floor = G.cen[CG_ENGFL * G.ncen + t];
+    St st;
+    calc_state(p, m.g, m.gc0, m.alpha, st);
+    const double rho = p[RHO], ug = p[ENY];
+    uint8_t mk = 0;
+    if (rho_floor < ug / sms) rho_floor = ug / sms;
+    const double sigma = st.bsq / rho;
+    if (sigma > P.sigma_max) {
+      rho_floor = st.bsq / sms;
+      mk |= 8;
+    }
+    if (p[RHO] < rho_floor) {
+      p[RHO] = rho_floor;
+      mk |= 1;
+    }
+    if (p[ENY] < eng_floor) {
+      p[ENY] = eng_floor;
+      mk |= 2;
+    }
+    const double u1 = p[UX1], u2 = p[UX2], u3 = p[UX3];
+    const double *g = m.g;
+    const double uvsq = g[5] * u1 * u1 + g[10] * u2 * u2 + g[15] * u3 * u3 +
+                        2.0 * (g[6] * u1 * u2 + g[7] * u1 * u3 + g[11] * u2 * u3);
+    const double umax2 = P.lorentz_max * P.lorentz_max - 1.0;
+    if (umax2 > 0.0 && uvsq > umax2) {
+      const double f = sqrt(umax2 / kmax(uvsq, small));
+      p[UX1] *= f;
+      p[UX2] *= f;
+      p[UX3] *= f;
+      mk |= 4;
+    }
+    const double rho_after = kmax(p[RHO], small);
+    if (HK && rho_after != rho_before) p[KEL] *= pow(rho_after / rho_before, -P.gamma);
+    p[ENT] = (P.gamma - 1.0) * p[ENY] * pow(p[RHO], -P.gamma);
+    if (HK) p[KEL] = clamp_by_ratio(P.ratio_min, P.ratio_max, p[ENT], p[KEL]);
+    if (P.metric == PANGU_METRIC_CKS && G.cen[CG_RAD * G.ncen + t] < P.r_excise) {
+      const double e_exc = P.pexcise / (P.gamma - 1.0);
+      p[RHO] = P.dexcise;
+      p[ENY] = e_exc;
+      p[UX1] = 0.0;
+      p[UX2] = 0.0;
+      p[UX3] = 0.0;
+      p[ENT] = (P.gamma - 1.0) * e_exc * pow(P.dexcise, -P.gamma);
+      if (HK) p[KEL] = e_exc;
+      mk |= 16;
+    }
+    pp[0][id] = p[RHO];
+    pp[1][id] = p[ENY];
+    pp[2][id] = p[UX1];
+    pp[3][id] = p[UX2];
+    pp[4][id] = p[UX3];
+    pp[ient<HB>()][id] = p[ENT];
+    if (HK) pp[ikel<HB>()][id] = p[KEL];
+    if (mask) mask[b][id] = mk;
+  }
+  if (CONS && interior) {
+    St st;
+    calc_state(p, m.g, m.gc0, m.alpha, st);
+    double f[NPRIM];
+    contravariant_flux<0>(P.gamma, p, st, m.sg, f);
+#pragma unroll
+    for (int q = 0; q < NC; ++q) F.cons[b * NC + q][id] = f[slot_of<HB, HK>(q)];
+  }
+}
+
+// ------------------------------------------------------------- flux sweep
+// One direction of CalculateLAXF / CalculateHLL (laxf.cc:78-181 and the
+// X2/X3 twins): reconstruction of every primitive, face metric, fast speeds,
+// four contravariant fluxes and the Riemann combine.  Face (k,j,i) is the
+// lower face of cell (k,j,i) along DIR.
+template <int DIR, int LIM, int RS, bool HB, bool HK>
+__global__ __launch_bounds__(256) void k_flux(KCtx C, KFields F, double *const *alfven, Range3 R) {
+  const int i = R.lo[0] + blockIdx.x * 64 + threadIdx.x;
+  const int j = R.lo[1] + blockIdx.y * 4 + threadIdx.y;
+  const int b = blockIdx.z / R.cnt[2];
+  const int k = R.lo[2] + blockIdx.z % R.cnt[2];
+  if (i > R.hi[0] || j > R.hi[1]) return;
+  const DevBlock &B = C.blk[b];
+  const int64_t sj = B.n[0], sk = (int64_t)B.n[0] * B.n[1];
+  const int64_t st = (DIR == 1) ? 1 : (DIR == 2 ? sj : sk);
+  const int64_t id = k * sk + j * sj + i;
+  constexpr int NC = ncomp<HB, HK>();
+  double pl[NPRIM], pr[NPRIM];
+#pragma unroll
+  for (int s = 0; s < NPRIM; ++s) {
+    pl[s] = 0.0;
+    pr[s] = 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
+    const double *a = F.prim[b * NC + q];
+    const int s = slot_of<HB, HK>(q);
+    if (LIM == PANGU_LIMITER_PPM4) {
+      const double m3 = a[id - 3 * st], m2 = a[id - 2 * st], m1 = a[id - st], c0 = a[id],
+                   p1 = a[id + st], p2 = a[id + 2 * st];
+      double t0, t1;
+      recon_ppm4(m3, m2, m1, c0, p1, pl[s], t0);
+      recon_ppm4(m2, m1, c0, p1, p2, t1, pr[s]);
+    } else {
+      const double m2 = a[id - 2 * st], m1 = a[id - st], c0 = a[id], p1 = a[id + st];
+      double t0, t1;
+      recon_mc(m2, m1, c0, pl[s], t0);
+      recon_mc(m1, c0, p1, t1, pr[s]);
+    }
+  }
+  const DevGeo &G = C.geo[B.geo];
+  double g[16], gc0[4], gdd, sg, alpha, valid;
+  if (DIR == 3 && !G.face[2]) {
+    const int64_t t = cen_index(G, k, j, i);
+    const double *c = G.cen;
+    const int64_t s = G.ncen;
+#pragma unroll
+    for (int n = 0; n < 16; ++n) g[n] = c[(CG_GCOV + n) * s + t];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) gc0[n] = c[(CG_GCON + n) * s + t];
+    gdd = c[(CG_GCON + 15) * s + t];
+    sg = c[CG_SG * s + t];
+    alpha = c[CG_ALPHA * s + t];
+    valid = 1.0;
+  } else {
+    const int d = DIR - 1;
+    const int64_t t = ((int64_t)(G.has3d ? k : 0) * G.fj[d] + j) * G.fi[d] + i;
+    const double *c = G.face[d];
+    const int64_t s = G.nface[d];
+#pragma unroll
+    for (int n = 0; n < 16; ++n) g[n] = c[(FG_GCOV + n) * s + t];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) gc0[n] = c[(FG_GCON0 + n) * s + t];
+    gdd = c[FG_GCONDD * s + t];
+    sg = c[FG_SG * s + t];
+    alpha = c[FG_ALPHA * s + t];
+    valid = c[FG_VALID * s + t];
+  }
+  double *al = alfven[b * 3 + (DIR - 1)];
+  if (valid == 0.0) {
+    // singular face metric (polar axis): zero flux and wave speed
+    al[id] = 0.0;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) F.flux[DIR - 1][b * NC + q][id] = 0.0;
+    return;
+  }
+  const double gam = C.ph.gamma;
+  St sl, sr;
+  calc_state(pl, g, gc0, alpha, sl);
+  calc_state(pr, g, gc0, alpha, sr);
+  double vmaxL, vminL, vmaxR, vminR;
+  fast_ms<DIR>(gam, pl, sl, gc0[0], gc0[DIR], gdd, vmaxL, vminL);
+  fast_ms<DIR>(gam, pr, sr, gc0[0], gc0[DIR], gdd, vmaxR, vminR);
+  const double cmax = fabs(kmax(kmax(0., vmaxL), vmaxR));
+  const double cmin = fabs(kmax(kmax(0., -vminL), -vminR));
+  const double cc = kmax(cmax, cmin);
+  al[id] = cc;
+  double UL[NPRIM], UR[NPRIM], FL[NPRIM], FR[NPRIM];
+  contravariant_flux<0>(gam, pl, sl, sg, UL);
+  contravariant_flux<0>(gam, pr, sr, sg, UR);
+  contravariant_flux<DIR>(gam, pl, sl, sg, FL);
+  contravariant_flux<DIR>(gam, pr, sr, sg, FR);
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
+    const int s = slot_of<HB, HK>(q);
+    double f;
+    if (RS == PANGU_RIEMANN_LAXF)
+      f = 0.5 * (FL[s] + FR[s] - cc * (UR[s] - UL[s]));
+    else
+      f = (cmax * FL[s] + cmin * FR[s] - cmax * cmin * (UR[s] - UL[s])) / (cmax + cmin + 1e-20);
+    F.flux[DIR - 1][b * NC + q][id] = f;
+  }
+}
+
+// ------------------------------------------------------ constrained transport
+// Corner EMFs (constrained_transport.cc:49-76).
+__global__ __launch_bounds__(256) void k_emf(KCtx C, KFields F, double *const *emf, Range3 R,
+                                             int e1on, int e2on, int e3on) {
+  const int i = R.lo[0] + blockIdx.x * 64 + threadIdx.x;
+  const int j = R.lo[1] + blockIdx.y * 4 + threadIdx.y;
+  const int b = blockIdx.z / R.cnt[2];
+  const int k = R.lo[2] + blockIdx.z % R.cnt[2];
+  if (i > R.hi[0] || j > R.hi[1]) return;
+  const DevBlock &B = C.blk[b];
+  const int64_t sj = B.n[0], sk = (int64_t)B.n[0] * B.n[1];
+  const int64_t id = k * sk + j * sj + i;
+  const int nc = C.nc;
+  const int b1 = C.iBX, b2 = C.iBX + 1, b3 = C.iBX + 2;
+  double e1 = 0.0, e2 = 0.0, e3 = 0.0;
+  if (e1on) {
+    const double *f2 = F.flux[1][b * nc + b3], *f3 = F.flux[2][b * nc + b2];
+    e1 = 0.25 * ((f2[id] + f2[id - sk]) - (f3[id] + f3[id - sj]));
+  }
+  if (e2on) {
+    const double *f3 = F.flux[2][b * nc + b1], *f1 = F.flux[0][b * nc + b3];
+    e2 = 0.25 * ((f3[id] + f3[id - 1]) - (f1[id] + f1[id - sk]));
+  }
+  if (e3on) {
+    const double *f1 = F.flux[0][b * nc + b2], *f2 = F.flux[1][b * nc + b1];
+    e3 = 0.25 * ((f1[id] + f1[id - sj]) - (f2[id] + f2[id - 1]));
+  }
+  emf[b * 3 + 0][id] = e1;
+  emf[b * 3 + 1][id] = e2;
+  emf[b * 3 + 2][id] = e3;
+}
+
+// Face B-flux overwrite from the EMFs (constrained_transport.cc:78-127).
+// One launch covers the union of the three face ranges.
+__global__ __launch_bounds__(256) void k_ct_faces(KCtx C, KFields F, double *const *emf, Range3 R,
+                                                  int o1, int o2, int o3) {
+  const int i = R.lo[0] + blockIdx.x * 64 + threadIdx.x;
+  const int j = R.lo[1] + blockIdx.y * 4 + threadIdx.y;
+  const int b = blockIdx.z / R.cnt[2];
+  const int k = R.lo[2] + blockIdx.z % R.cnt[2];
+  if (i > R.hi[0] || j > R.hi[1]) return;
+  const DevBlock &B = C.blk[b];
+  const int64_t sj = B.n[0], sk = (int64_t)B.n[0] * B.n[1];
+  const int64_t id = k * sk + j * sj + i;
+  const int nc = C.nc;
+  const int b1 = C.iBX, b2 = C.iBX + 1, b3 = C.iBX + 2;
+  const int E1 = o2 && o3, E2 = o3 && o1, E3 = o1 && o2;
+  const double *e1 = emf[b * 3 + 0], *e2 = emf[b * 3 + 1], *e3 = emf[b * 3 + 2];
+  const bool in_k = k <= B.e[2], in_j = j <= B.e[1], in_i = i <= B.e[0];
+  if (in_k && in_j) {  // X1 faces: i in [is, ie+o1]
+    if (o1) F.flux[0][b * nc + b1][id] = 0;
+    if (E2) F.flux[0][b * nc + b3][id] = -0.5 * (e2[id] + e2[id + sk]);
+    if (E3) F.flux[0][b * nc + b2][id] = 0.5 * (e3[id] + e3[id + sj]);
+  }
+  if (in_k && in_i) {  // X2 faces: j in [js, je+o2]
+    if (E1) F.flux[1][b * nc + b3][id] = 0.5 * (e1[id] + e1[id + sk]);
+    if (o2) F.flux[1][b * nc + b2][id] = 0.;
+    if (E3) F.flux[1][b * nc + b1][id] = -0.5 * (e3[id] + e3[id + 1]);
+  }
+  if (in_j && in_i && o3) {  // X3 faces: k in [ks, ke+o3]
+    if (E1) F.flux[2][b * nc + b2][id] = -0.5 * (e1[id] + e1[id + sj]);
+    if (E2) F.flux[2][b * nc + b1][id] = 0.5 * (e2[id] + e2[id + 1]);
+    F.flux[2][b * nc + b3][id] = 0;
+  }
+}
+
+// ------------------------------------ flux divergence, update, source term
+// dU/dt = -[sum_d (A_d F_d(+) - A_d F_d(-))]/V (Parthenon FluxDivergence),
+// U(out) = U(init) + beta dt dU/dt (UpdateIndependentData) and the
+// geometric source U_nu += beta dt sqrt|g| T^r_c Gamma^c_{nu r}
+// (source_term.cc:171-228).  MODE: 1 = divergence only (writes dU/dt),
+// 2 = update only (reads dU/dt), 3 = source only, 4 = all three fused.
+template <int MODE, bool SRC, bool HB, bool HK>
+__global__ __launch_bounds__(256) void k_div_update(KCtx C, KFields F0, KFields Finit, KFields F1,
+                                                    double *const *dudt, const double *dt_dev,
+                                                    double beta, double bdt_host) {
+  const int b = blockIdx.z / C.nk;
+  const DevBlock &B = C.blk[b];
+  const int k = B.s[2] + blockIdx.z % C.nk;
+  const int i = B.s[0] + blockIdx.x * 64 + threadIdx.x;
+  const int j = B.s[1] + blockIdx.y * 4 + threadIdx.y;
+  if (i > B.e[0] || j > B.e[1] || k > B.e[2]) return;
+  const int64_t sj = B.n[0], sk = (int64_t)B.n[0] * B.n[1];
+  const int64_t id = k * sk + j * sj + i;
+  constexpr int NC = ncomp<HB, HK>();
+  const double bdt = dt_dev ? beta * dt_dev[0] : bdt_host;
+  double unew[NC];
+  if (MODE == 1 || MODE == 4) {
+    const double a1 = B.dx[1] * B.dx[2], a2 = B.dx[0] * B.dx[2], a3 = B.dx[0] * B.dx[1];
+    const double vol = B.dx[0] * B.dx[1] * B.dx[2];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+      const double *f1 = F0.flux[0][b * NC + q];
+      double s = (a1 * f1[id + 1] - a1 * f1[id]);
+      if (C.ndim >= 2) {
+        const double *f2 = F0.flux[1][b * NC + q];
+        s += (a2 * f2[id + sj] - a2 * f2[id]);
+      }
+      if (C.ndim == 3) {
+        const double *f3 = F0.flux[2][b * NC + q];
+        s += (a3 * f3[id + sk] - a3 * f3[id]);
+      }
+      const double du = -s / vol;
+      if (MODE == 1)
+        dudt[b * NC + q][id] = du;
+      else
+        unew[q] = Finit.cons[b * NC + q][id] + bdt * du;
+    }
+  } else if (MODE == 2) {
+#pragma unroll
+    for (int q = 0; q < NC; ++q)
+      unew[q] = Finit.cons[b * NC + q][id] + bdt * dudt[b * NC + q][id];
+  } else {
+#pragma unroll
+    for (int q = 0; q < NC; ++q) unew[q] = F1.cons[b * NC + q][id];
+  }
+  if (MODE == 1) return;
+  if (SRC && (MODE == 3 || MODE == 4)) {
+    const DevGeo &G = C.geo[B.geo];
+    const int64_t t = cen_index(G, k, j, i);
+    CenLoad m;
+    load_cen_state(G, t, m);
+    double p[NPRIM];
+#pragma unroll
+    for (int s = 0; s < NPRIM; ++s) p[s] = 0.0;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) p[slot_of<HB, HK>(q)] = F1.prim[b * NC + q][id];
+    St s;
+    calc_state(p, m.g, m.gc0, m.alpha, s);
+    const double gam = C.ph.gamma;
+    const double pg = (gam - 1.) * p[ENY];
+    const double h = p[RHO] + gam * p[ENY];
+    const double etot = h + s.bsq, ptot = pg + 0.5 * s.bsq;
+    double T[4][4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      T[0][c] = tmunu<0>(s, etot, ptot, c);
+      T[1][c] = tmunu<1>(s, etot, ptot, c);
+      T[2][c] = tmunu<2>(s, etot, ptot, c);
+      T[3][c] = tmunu<3>(s, etot, ptot, c);
+    }
+    const double *conn = G.cen + (int64_t)CG_CONN * G.ncen + t;
+#pragma unroll
+    for (int dd = 0; dd < 4; ++dd) {
+      double con = 0.0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) con += T[r][c] * conn[(int64_t)(16 * c + 4 * dd + r) * G.ncen];
+      const int q = (dd == 0) ? 1 : 1 + dd;
+      unew[q] += con * m.sg * bdt;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NC; ++q) F1.cons[b * NC + q][id] = unew[q];
+}
+
+// -------------------------------------------------- recovery (+ heating)
+// Recovery (invertor.cc:184-319) and, when HEAT, ApplyElectronHeating
+// (electron_heating.cc:66-109) on the recovered cell.  REC: 0 chainer,
+// 1 Kastaun, 2 1Dvsq alone, -1 none (heating only).
+template <int REC, bool HB, bool HK, bool HEAT>
+__global__ __launch_bounds__(256) void k_recover(KCtx C, KFields F, SharedTabs S) {
+  const int b = blockIdx.z / C.nk;
+  const DevBlock &B = C.blk[b];
+  const int k = B.s[2] + blockIdx.z % C.nk;
+  const int i = B.s[0] + blockIdx.x * 64 + threadIdx.x;
+  const int j = B.s[1] + blockIdx.y * 4 + threadIdx.y;
+  if (i > B.e[0] || j > B.e[1] || k > B.e[2]) return;
+  const int64_t id = ((int64_t)k * B.n[1] + j) * B.n[0] + i;
+  constexpr int NC = ncomp<HB, HK>();
+  const DevGeo &G = C.geo[B.geo];
+  const int64_t t = cen_index(G, k, j, i);
+  const int64_t ns = G.ncen;
+  const Phys &P = C.ph;
+  double *pp[NC];
+#pragma unroll
+  for (int q = 0; q < NC; ++q) pp[q] = F.prim[b * NC + q];
+  if (REC >= 0) {
+    const bool chainer = (REC != 1);
+    bool excised = false;
+    if (P.metric == PANGU_METRIC_CKS && G.cen[CG_RAD * ns + t] < P.r_excise) excised = true;
+    if (excised) {
+      if (chainer && S.flag) S.flag[b][id] = 0.0;
+      if (S.status) S.status[b][id] = -1;
+      if (S.iters) S.iters[b][id] = 0;
+    } else {
+      CenMetric m;
+#pragma unroll
+      for (int n = 0; n < 16; ++n) {
+        m.gcov[n] = G.cen[(CG_GCOV + n) * ns + t];
+        m.gcon[n] = G.cen[(CG_GCON + n) * ns + t];
+      }
+      m.lapse = G.cen[CG_LAPSE * ns + t];
+      const double sg = G.cen[CG_SG * ns + t];
+      const double alpha = G.cen[CG_ALPHA * ns + t];
+      double U[9], Ph[9];
+      const double *const *cons = F.cons + b * NC;
+      const double c_rho = cons[0][id], c_eny = cons[1][id];
+      const double isg = 1.0 / sg;
+      const double aos = alpha * isg;
+      U[RHO] = aos * c_rho;
+      U[ENY] = aos * (c_eny - c_rho);
+      U[UX1] = aos * cons[2][id];
+      U[UX2] = aos * cons[3][id];
+      U[UX3] = aos * cons[4][id];
+      if (HB) {
+        const double cb1 = cons[5][id], cb2 = cons[6][id], cb3 = cons[7][id];
+        U[BX1] = aos * cb1;
+        U[BX2] = aos * cb2;
+        U[BX3] = aos * cb3;
+        pp[5][id] = cb1 * isg;
+        pp[6][id] = cb2 * isg;
+        pp[7][id] = cb3 * isg;
+      } else {
+        U[BX1] = 0.0;
+        U[BX2] = 0.0;
+        U[BX3] = 0.0;
+      }
+      U[ENT] = aos * cons[ient<HB>()][id];
+      Ph[RHO] = pp[0][id];
+      Ph[ENY] = pp[1][id];
+      Ph[UX1] = pp[2][id];
+      Ph[UX2] = pp[3][id];
+      Ph[UX3] = pp[4][id];
+      Ph[BX1] = 0.0;
+      Ph[BX2] = 0.0;
+      Ph[BX3] = 0.0;
+      Ph[ENT] = pp[ient<HB>()][id];
+      int st = 0, it = 0, itn = 0;
+      bool write = true;
+      if (REC == 1) {
+        st = c2p_kastaun(U, Ph, P.gamma, m, it);
+        itn = it;
+        write = (st == 0);
+      } else if (REC == 2) {
+        st = c2p_1dvsq(U, Ph, P.gamma, m, it);
+        itn = it;
+      } else {
+        st = c2p_2d(U, Ph, P.gamma, m, it);
+        itn = it;
+        if (st != 0) {
+          st = c2p_1dvsq(U, Ph, P.gamma, m, it);
+          itn += it;
+          if (st != 0) {
+            st = c2p_1d(U, Ph, P.gamma, m, it);
+            itn += it;
+          }
+        }
+      }
+      if (chainer && S.flag) {
+        const double fv = (st == 0) ? 1.0 : 0.0;
+        S.flag[b][id] = fv;
+        if (S.flag_rec) S.flag_rec[b][id] = fv;
+      }
+      if (S.status) S.status[b][id] = st;
+      if (S.iters) S.iters[b][id] = itn;
+      if (write) {
+        pp[0][id] = Ph[RHO];
+        pp[1][id] = Ph[ENY];
+        pp[2][id] = Ph[UX1];
+        pp[3][id] = Ph[UX2];
+        pp[4][id] = Ph[UX3];
+      }
+    }
+  }
+  if (HEAT && HK) {
+    double p[NPRIM];
+#pragma unroll
+    for (int s = 0; s < NPRIM; ++s) p[s] = 0.0;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) p[slot_of<HB, HK>(q)] = pp[q][id];
+    CenLoad m;
+    load_cen_state(G, t, m);
+    St s;
+    calc_state(p, m.g, m.gc0, m.alpha, s);
+    const double *const *cons = F.cons + b * NC;
+    const double crho = cons[0][id];
+    HeatIn c;
+    c.rho = p[RHO];
+    c.energy = p[ENY];
+    c.bsq = s.bsq;
+    c.adv_total = cons[ient<HB>()][id] / crho;
+    c.rec_total = p[ENT];
+    c.adv_e = cons[ikel<HB>()][id] / crho;
+    pp[ikel<HB>()][id] = heat_apply(P, c);
+  }
+}
+
+// ---------------------------------------------------------- FixRecovery
+// recovery_fixer.cc:216-285 with the serial (lexicographic, in-place)
+// semantics of a single-threaded host build.  Phase A fixes failed cells
+// with no failed 8-neighbour in parallel (nobody else reads or writes their
+// stencils); cells touching another failed cell are left for phase B,
+// which walks each (block, k) plane in lexicographic order.
+__device__ __forceinline__ void fix_cell(double *const *pp, int np, int64_t id, int64_t jp,
+                                         const double *fl) {
+  const int pf1 = (int)fl[id + jp - 1], pf2 = (int)fl[id + jp], pf3 = (int)fl[id + jp + 1];
+  const int pf4 = (int)fl[id + 1];
+  const int pf5 = (int)fl[id - jp + 1], pf6 = (int)fl[id - jp], pf7 = (int)fl[id - jp - 1];
+  const int pf8 = (int)fl[id - 1];
+  for (int q = 0; q < np; ++q) {
+    double *a = pp[q];
+    if (pf2 && pf4 && pf6 && pf8)
+      a[id] = 0.25 * (a[id + jp] + a[id - jp] + a[id - 1] + a[id + 1]);
+    else if (pf1 && pf3 && pf5 && pf7)
+      a[id] = 0.25 * (a[id + jp + 1] + a[id + jp - 1] + a[id - jp + 1] + a[id - jp - 1]);
+    else
+      a[id] = 0.125 * (a[id + jp - 1] + a[id + jp] + a[id + jp + 1] + a[id + 1] + a[id - jp + 1] +
+                       a[id - jp] + a[id - jp - 1] + a[id - 1]);
+  }
+  pp[2][id] = 0.;
+  pp[3][id] = 0.;
+  pp[4][id] = 0.;
+}
+
+__global__ __launch_bounds__(256) void k_fixrec_a(KCtx C, KFields F, double *const *flag,
+                                                  int *cluster) {
+  const int b = blockIdx.z / C.nk;
+  const DevBlock &B = C.blk[b];
+  const int kk = blockIdx.z % C.nk;
+  const int k = B.s[2] + kk;
+  const int i = B.s[0] + blockIdx.x * 64 + threadIdx.x;
+  const int j = B.s[1] + blockIdx.y * 4 + threadIdx.y;
+  if (i > B.e[0] || j > B.e[1] || k > B.e[2]) return;
+  const int64_t id = ((int64_t)k * B.n[1] + j) * B.n[0] + i;
+  double *fl = flag[b];
+  if (fl[id] != 0) return;
+  if (C.ph.metric == PANGU_METRIC_CKS) {
+    const DevGeo &G = C.geo[B.geo];
+    if (G.cen[CG_RAD * G.ncen + cen_index(G, k, j, i)] < C.ph.r_excise) return;
+  }
+  const int64_t jp = (B.s[1] != B.e[1]) ? (int64_t)B.n[0] : 0;
+  const bool isolated = fl[id + jp - 1] != 0 && fl[id + jp] != 0 && fl[id + jp + 1] != 0 &&
+                        fl[id + 1] != 0 && fl[id - jp + 1] != 0 && fl[id - jp] != 0 &&
+                        fl[id - jp - 1] != 0 && fl[id - 1] != 0;
+  if (!isolated) {
+    atomicOr(&cluster[b * C.nk + kk], 1);
+    return;
+  }
+  fix_cell(F.prim + b * C.np, C.np, id, jp, fl);
+  fl[id] = 1;
+}
+
+// Phase B: one wavefront per (block, k) plane that holds clustered cells.
+__global__ __launch_bounds__(64) void k_fixrec_b(KCtx C, KFields F, double *const *flag,
+                                                 int *cluster) {
+  const int b = blockIdx.x / C.nk;
+  const int kk = blockIdx.x % C.nk;
+  if (cluster[b * C.nk + kk] == 0) return;
+  const DevBlock &B = C.blk[b];
+  const int k = B.s[2] + kk;
+  const int64_t jp = (B.s[1] != B.e[1]) ? (int64_t)B.n[0] : 0;
+  double *fl = flag[b];
+  const int lane = threadIdx.x;
+  for (int j = B.s[1]; j <= B.e[1]; ++j) {
+    for (int i0 = B.s[0]; i0 <= B.e[0]; i0 += 64) {
+      const int i = i0 + lane;
+      const int64_t id = ((int64_t)k * B.n[1] + j) * B.n[0] + i;
+      bool bad = (i <= B.e[0]) && fl[id] == 0;
+      if (bad && C.ph.metric == PANGU_METRIC_CKS) {
+        const DevGeo &G = C.geo[B.geo];
+        if (G.cen[CG_RAD * G.ncen + cen_index(G, k, j, i)] < C.ph.r_excise) bad = false;
+      }
+      uint64_t m = __ballot(bad);
+      while (m) {
+        const int l = __ffsll((unsigned long long)m) - 1;
+        m &= m - 1;
+        if (lane == 0) {
+          const int64_t idl = id - lane + l;
+          fix_cell(F.prim + b * C.np, C.np, idl, jp, fl);
+          fl[idl] = 1;
+        }
+        __syncthreads();
+      }
+    }
+  }
+  if (lane == 0) cluster[b * C.nk + kk] = 0;
+}
+
+// ------------------------------------------------- ghost exchange and BCs
+// Same-level exchange: a ghost cell of block b takes the interior value of
+// the neighbour in its offset direction (periodic wrap encoded in nbr).
+__global__ __launch_bounds__(256) void k_exchange(KCtx C, KFields F, double *const *flag,
+                                                  const int *nbr) {
+  const int b = blockIdx.z / C.nkg;
+  const int k = blockIdx.z % C.nkg;
+  const DevBlock &B = C.blk[b];
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * 4 + threadIdx.y;
+  if (i >= B.n[0] || j >= B.n[1] || k >= B.n[2]) return;
+  const int o0 = (i < B.s[0]) ? -1 : (i > B.e[0] ? 1 : 0);
+  const int o1 = (j < B.s[1]) ? -1 : (j > B.e[1] ? 1 : 0);
+  const int o2 = (k < B.s[2]) ? -1 : (k > B.e[2] ? 1 : 0);
+  if ((o0 | o1 | o2) == 0) return;
+  const int nb = nbr[b * 27 + (o2 + 1) * 9 + (o1 + 1) * 3 + (o0 + 1)];
+  if (nb < 0) return;
+  const DevBlock &N = C.blk[nb];
+  const int si = i - o0 * (B.e[0] - B.s[0] + 1);
+  const int sj = j - o1 * (B.e[1] - B.s[1] + 1);
+  const int sk = k - o2 * (B.e[2] - B.s[2] + 1);
+  const int64_t dst = ((int64_t)k * B.n[1] + j) * B.n[0] + i;
+  const int64_t src = ((int64_t)sk * N.n[1] + sj) * N.n[0] + si;
+  for (int q = 0; q < C.np; ++q) F.prim[b * C.np + q][dst] = F.prim[nb * C.np + q][src];
+  if (flag) flag[b][dst] = flag[nb][src];
+}
+
+// Physical boundaries normal to dimension d: outflow copies the edge cell;
+// reflecting mirrors and flips the d-component of u~ and B.  Covers the full
+// extent of the other dims, so X1, X2, X3 launches in turn fill corners.
+__global__ __launch_bounds__(256) void k_bc(KCtx C, KFields F, double *const *flag, int d, int bc_in,
+                                            int bc_out, int nbd) {
+  const int b = blockIdx.z / C.nkg;
+  const int k = blockIdx.z % C.nkg;
+  const DevBlock &B = C.blk[b];
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * 4 + threadIdx.y;
+  if (i >= B.n[0] || j >= B.n[1] || k >= B.n[2]) return;
+  int ix[3] = {i, j, k};
+  int side, kind;
+  if (ix[d] < B.s[d] && B.lx[d] == 0 && bc_in != PANGU_BC_PERIODIC) {
+    side = 0;
+    kind = bc_in;
+  } else if (ix[d] > B.e[d] && B.lx[d] == nbd - 1 && bc_out != PANGU_BC_PERIODIC) {
+    side = 1;
+    kind = bc_out;
+  } else {
+    return;
+  }
+  int sx[3] = {i, j, k};
+  if (kind == PANGU_BC_OUTFLOW)
+    sx[d] = side == 0 ? B.s[d] : B.e[d];
+  else
+    sx[d] = side == 0 ? (2 * B.s[d] - 1 - ix[d]) : (2 * B.e[d] + 1 - ix[d]);
+  const int64_t dst = ((int64_t)k * B.n[1] + j) * B.n[0] + i;
+  const int64_t src = ((int64_t)sx[2] * B.n[1] + sx[1]) * B.n[0] + sx[0];
+  const int iBX = C.iBX;
+  for (int q = 0; q < C.np; ++q) {
+    double v = F.prim[b * C.np + q][src];
+    if (kind == PANGU_BC_REFLECT && (q == 2 + d || (iBX >= 0 && q == iBX + d))) v = -v;
+    F.prim[b * C.np + q][dst] = v;
+  }
+  if (flag) flag[b][dst] = flag[b][src];
+}
+
+// --------------------------------------------------------------- timestep
+// Per block and active direction the maximum face speed over interior
+// cells; min_cells(cfl dx / c) equals cfl dx / max_cells(c) because IEEE
+// division is monotone.  NaN speeds are skipped, as the reference's
+// `if (dt_min > dt)` reduction skips NaN.  Non-negative doubles order like
+// their bit patterns, so a 64-bit atomicMax suffices.
+__global__ void k_dt_reset(unsigned long long *red, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) red[t] = 0ull;
+}
+
+__global__ __launch_bounds__(256) void k_dt_reduce(KCtx C, double *const *alfven,
+                                                   unsigned long long *red) {
+  const int b = blockIdx.z / C.nk;
+  const DevBlock &B = C.blk[b];
+  const int k = B.s[2] + blockIdx.z % C.nk;
+  const int i = B.s[0] + blockIdx.x * 64 + threadIdx.x;
+  const int j = B.s[1] + blockIdx.y * 4 + threadIdx.y;
+  const bool in = (i <= B.e[0] && j <= B.e[1] && k <= B.e[2]);
+  const int64_t id = ((int64_t)k * B.n[1] + j) * B.n[0] + i;
+  for (int d = 0; d < C.ndim; ++d) {
+    double c = 0.0;
+    if (in) {
+      const double v = alfven[b * 3 + d][id];
+      if (v >= 0.0) c = v;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_xor(c, off, 64);
+      c = (o > c) ? o : c;
+    }
+    if ((threadIdx.x & 63) == 0 && c > 0.0)
+      atomicMax(&red[b * 3 + d], (unsigned long long)__double_as_longlong(c));
+  }
+}
+
+// Harmonic combination per block (timestep_estimation.cc:94-98), minimum
+// over blocks, then dt control dt_next = min(dt_factor * dt, estimate) and
+// time += dt.  state = [dt, time]; est_out (optional) gets the estimate.
+__global__ __launch_bounds__(256) void k_dt_final(KCtx C, const unsigned long long *red,
+                                                  double *state, double dt_factor, double *est_out) {
+  __shared__ double sm[256];
+  double mn = DBL_MAX;
+  for (int b = threadIdx.x; b < C.nblocks; b += blockDim.x) {
+    const DevBlock &B = C.blk[b];
+    double m[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
+    for (int d = 0; d < C.ndim; ++d) {
+      const double cmax = __longlong_as_double((long long)red[b * 3 + d]);
+      const double q = C.ph.cfl * B.dx[d] / cmax;
+      m[d] = (q < DBL_MAX) ? q : DBL_MAX;
+    }
+    const double t = 1.0 / (1.0 / m[0] + 1.0 / m[1] + 1.0 / m[2]);
+    if (t < mn) mn = t;
+  }
+  sm[threadIdx.x] = mn;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      const double o = sm[threadIdx.x + s];
+      if (o < sm[threadIdx.x]) sm[threadIdx.x] = o;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double est = sm[0];
+    if (est_out) *est_out = est;
+    if (state) {
+      const double dt = state[0];
+      state[1] = state[1] + dt;
+      const double grown = dt_factor * dt;
+      state[0] = (est < grown) ? est : grown;
+    }
+  }
+}
+
+// ===================================================================== host
+static inline dim3 grid_for(int ni, int nj, int nz) {
+  return dim3((ni + 63) / 64, (nj + 3) / 4, nz);
+}
+static const dim3 kBlock(64, 4, 1);
+
+#define PG_CHECK_LAUNCH() (hipGetLastError() == hipSuccess ? 0 : -2)
+
+int launch_fix_cons(const LaunchCtx &L, const KFields &F, uint8_t *const *mask, bool fix, bool cons,
+                    hipStream_t s) {
+  const DevBlock &B = L.hblk;
+  dim3 g = grid_for(B.n[0], B.n[1], L.k.nblocks * B.n[2]);
+  KCtx kc = L.k;
+  kc.nk = B.n[2];
+  const bool hb = L.k.iBX >= 0, hk = L.k.iKEL >= 0;
+#define FC(FX, CS, HB_, HK_) hipLaunchKernelGGL((k_fix_cons<FX, CS, HB_, HK_>), g, kBlock, 0, s, kc, F, mask)
+#define FC2(FX, CS)                      \
+  if (hb && hk) FC(FX, CS, true, true);  \
+  else if (hb) FC(FX, CS, true, false);  \
+  else if (hk) FC(FX, CS, false, true);  \
+  else FC(FX, CS, false, false);
+  if (fix && cons) {
+    FC2(true, true)
+  } else if (fix) {
+    FC2(true, false)
+  } else {
+    FC2(false, true)
+  }
+#undef FC2
+#undef FC
+  return PG_CHECK_LAUNCH();
+}
+
+template <int DIR>
+static void flux_dir(const LaunchCtx &L, const KFields &F, double *const *alfven, Range3 R,
+                     hipStream_t s) {
+  dim3 g = grid_for(R.cnt[0], R.cnt[1], L.k.nblocks * R.cnt[2]);
+  const bool hb = L.k.iBX >= 0, hk = L.k.iKEL >= 0;
+  const int lim = L.limiter, rs = L.riemann;
+#define FL(LIM, RS, HB_, HK_) \
+  hipLaunchKernelGGL((k_flux<DIR, LIM, RS, HB_, HK_>), g, kBlock, 0, s, L.k, F, alfven, R)
+#define FL2(LIM, RS)                    \
+  if (hb && hk) FL(LIM, RS, true, true); \
+  else if (hb) FL(LIM, RS, true, false); \
+  else if (hk) FL(LIM, RS, false, true); \
+  else FL(LIM, RS, false, false);
+  if (lim == PANGU_LIMITER_PPM4 && rs == PANGU_RIEMANN_LAXF) {
+    FL2(PANGU_LIMITER_PPM4, PANGU_RIEMANN_LAXF)
+  } else if (lim == PANGU_LIMITER_PPM4) {
+    FL2(PANGU_LIMITER_PPM4, PANGU_RIEMANN_HLL)
+  } else if (rs == PANGU_RIEMANN_LAXF) {
+    FL2(PANGU_LIMITER_MC, PANGU_RIEMANN_LAXF)
+  } else {
+    FL2(PANGU_LIMITER_MC, PANGU_RIEMANN_HLL)
+  }
+#undef FL2
+#undef FL
+}
+
+int launch_fluxes(const LaunchCtx &L, const KFields &F, double *const *alfven, hipStream_t s) {
+  const DevBlock &B = L.hblk;
+  const int o[3] = {B.n[0] > 1, B.n[1] > 1, B.n[2] > 1};
+  for (int dir = 1; dir <= L.k.ndim; ++dir) {
+    Range3 R;
+    for (int d = 0; d < 3; ++d) {
+      R.lo[d] = B.s[d] - o[d];
+      R.hi[d] = B.e[d] + o[d];
+    }
+    R.lo[dir - 1] = B.s[dir - 1];
+    R.hi[dir - 1] = B.e[dir - 1] + 1;
+    if (dir == 3 && L.riemann == PANGU_RIEMANN_HLL) {  // hll.cc:295-296
+      R.lo[0] = B.s[0];
+      R.hi[0] = B.e[0];
+      R.lo[1] = B.s[1];
+      R.hi[1] = B.e[1];
+    }
+    for (int d = 0; d < 3; ++d) R.cnt[d] = R.hi[d] - R.lo[d] + 1;
+    if (dir == 1) flux_dir<1>(L, F, alfven, R, s);
+    if (dir == 2) flux_dir<2>(L, F, alfven, R, s);
+    if (dir == 3) flux_dir<3>(L, F, alfven, R, s);
+  }
+  return PG_CHECK_LAUNCH();
+}
+
+int launch_ct(const LaunchCtx &L, const KFields &F, double *const *emf, hipStream_t s) {
+  const DevBlock &B = L.hblk;
+  const int o1 = B.n[0] > 1, o2 = B.n[1] > 1, o3 = B.n[2] > 1;
+  Range3 R;
+  R.lo[0] = B.s[0];
+  R.hi[0] = B.e[0] + o1;
+  R.lo[1] = B.s[1];
+  R.hi[1] = B.e[1] + o2;
+  R.lo[2] = B.s[2];
+  R.hi[2] = B.e[2] + o3;
+  for (int d = 0; d < 3; ++d) R.cnt[d] = R.hi[d] - R.lo[d] + 1;
+  dim3 g = grid_for(R.cnt[0], R.cnt[1], L.k.nblocks * R.cnt[2]);
+  hipLaunchKernelGGL(k_emf, g, kBlock, 0, s, L.k, F, emf, R, o2 && o3, o3 && o1, o1 && o2);
+  hipLaunchKernelGGL(k_ct_faces, g, kBlock, 0, s, L.k, F, emf, R, o1, o2, o3);
+  return PG_CHECK_LAUNCH();
+}
+
+int launch_div_update(const LaunchCtx &L, int mode, const KFields &F0, const KFields &Finit,
+                      const KFields &F1, double *const *dudt, const double *dt_dev, double beta,
+                      double bdt_host, hipStream_t s) {
+  const DevBlock &B = L.hblk;
+  const int nx0 = B.e[0] - B.s[0] + 1, nx1 = B.e[1] - B.s[1] + 1, nx2 = B.e[2] - B.s[2] + 1;
+  dim3 g = grid_for(nx0, nx1, L.k.nblocks * nx2);
+  KCtx kc = L.k;
+  kc.nk = nx2;
+  const bool hb = L.k.iBX >= 0, hk = L.k.iKEL >= 0;
+  const bool src = L.metric != PANGU_METRIC_MINKOWSKI;
+#define DU(M, SR, HB_, HK_)                                                                     \
+  hipLaunchKernelGGL((k_div_update<M, SR, HB_, HK_>), g, kBlock, 0, s, kc, F0, Finit, F1, dudt, \
+                     dt_dev, beta, bdt_host)
+#define DU2(M, SR)                     \
+  if (hb && hk) DU(M, SR, true, true); \
+  else if (hb) DU(M, SR, true, false); \
+  else if (hk) DU(M, SR, false, true); \
+  else DU(M, SR, false, false);
+  switch (mode) {
+    case 1: DU2(1, false) break;
+    case 2: DU2(2, false) break;
+    case 3:
+      if (src) {
+        DU2(3, true)
+      }
+      break;
+    default:
+      if (src) {
+        DU2(4, true)
+      } else {
+        DU2(4, false)
+      }
+  }
+#undef DU2
+#undef DU
+  return PG_CHECK_LAUNCH();
+}
+
+int launch_recover(const LaunchCtx &L, const KFields &F, const SharedTabs &S, int rec, bool heat,
+                   hipStream_t s) {
+  const DevBlock &B = L.hblk;
+  const int nx0 = B.e[0] - B.s[0] + 1, nx1 = B.e[1] - B.s[1] + 1, nx2 = B.e[2] - B.s[2] + 1;
+  dim3 g = grid_for(nx0, nx1, L.k.nblocks * nx2);
+  KCtx kc = L.k;
+  kc.nk = nx2;
+  const bool hb = L.k.iBX >= 0, hk = L.k.iKEL >= 0;
+#define RC(R_, HB_, HK_, HE_) \
+  hipLaunchKernelGGL((k_recover<R_, HB_, HK_, HE_>), g, kBlock, 0, s, kc, F, S)
+#define RC2(R_, HE_)                     \
+  if (hb && hk) RC(R_, true, true, HE_);  \
+  else if (hb) RC(R_, true, false, HE_);  \
+  else if (hk) RC(R_, false, true, HE_);  \
+  else RC(R_, false, false, HE_);
+  const bool h = heat && hk;
+  if (rec == 0) {
+    if (h) { RC2(0, true) } else { RC2(0, false) }
+  } else if (rec == 1) {
+    if (h) { RC2(1, true) } else { RC2(1, false) }
+  } else if (rec == 2) {
+    if (h) { RC2(2, true) } else { RC2(2, false) }
+  } else if (h) {
+    RC2(-1, true)
+  }
+#undef RC2
+#undef RC
+  return PG_CHECK_LAUNCH();
+}
+
+int launch_fix_recovery(const LaunchCtx &L, const KFields &F, double *const *flag, int *cluster,
+                        hipStream_t s) {
+  const DevBlock &B = L.hblk;
+  const int nx0 = B.e[0] - B.s[0] + 1, nx1 = B.e[1] - B.s[1] + 1, nx2 = B.e[2] - B.s[2] + 1;
+  KCtx kc = L.k;
+  kc.nk = nx2;
+  dim3 g = grid_for(nx0, nx1, L.k.nblocks * nx2);
+  hipLaunchKernelGGL(k_fixrec_a, g, kBlock, 0, s, kc, F, flag, cluster);
+  hipLaunchKernelGGL(k_fixrec_b, dim3(L.k.nblocks * nx2), dim3(64), 0, s, kc, F, flag, cluster);
+  return PG_CHECK_LAUNCH();
+}
+
+int launch_exchange(const LaunchCtx &L, const KFields &F, double *const *flag, const int *nbr,
+                    hipStream_t s) {
+  const DevBlock &B = L.hblk;
+  KCtx kc = L.k;
+  kc.nkg = B.n[2];
+  dim3 g = grid_for(B.n[0], B.n[1], L.k.nblocks * B.n[2]);
+  hipLaunchKernelGGL(k_exchange, g, kBlock, 0, s, kc, F, flag, nbr);
+  return PG_CHECK_LAUNCH();
+}
+
+int launch_bc(const LaunchCtx &L, const KFields &F, double *const *flag, hipStream_t s) {
+  const DevBlock &B = L.hblk;
+  KCtx kc = L.k;
+  kc.nkg = B.n[2];
+  dim3 g = grid_for(B.n[0], B.n[1], L.k.nblocks * B.n[2]);
+  for (int d = 0; d < L.k.ndim; ++d) {
+    if (L.bc[d][0] == PANGU_BC_PERIODIC && L.bc[d][1] == PANGU_BC_PERIODIC) continue;
+    hipLaunchKernelGGL(k_bc, g, kBlock, 0, s, kc, F, flag, d, L.bc[d][0], L.bc[d][1], L.nb[d]);
+  }
+  return PG_CHECK_LAUNCH();
+}
+
+int launch_dt(const LaunchCtx &L, double *const *alfven, unsigned long long *red, double *state,
+              double dt_factor, double *est_out, hipStream_t s) {
+  const DevBlock &B = L.hblk;
+  const int n = L.k.nblocks * 3;
+  hipLaunchKernelGGL(k_dt_reset, dim3((n + 255) / 256), dim3(256), 0, s, red, n);
+  const int nx0 = B.e[0] - B.s[0] + 1, nx1 = B.e[1] - B.s[1] + 1, nx2 = B.e[2] - B.s[2] + 1;
+  KCtx kc = L.k;
+  kc.nk = nx2;
+  dim3 g = grid_for(nx0, nx1, L.k.nblocks * nx2);
+  hipLaunchKernelGGL(k_dt_reduce, g, kBlock, 0, s, kc, alfven, red);
+  hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, s, kc, red, state, dt_factor, est_out);
+  return PG_CHECK_LAUNCH();
+}
+
+}  // namespace pangu

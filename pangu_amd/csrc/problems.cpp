@@ -1,0 +1,304 @@
+// Initial data for the "base" container (host, FP64, g++ -ffp-contract=off).
+//
+//  problem 1 - relativistic Orszag-Tang vortex on the unit box:
+//    rho = 25/(36 pi), p = 5/(12 pi), v = 0.5(-sin 2 pi y, sin 2 pi x, 0),
+//    B = (-sin 2 pi y, sin 4 pi x, 0)/sqrt(4 pi), Minkowski.
+//  problem 2 - Fishbone-Moncrief torus in MKS with a SANE A_phi field and
+//    beta normalisation (pangu/problem/fm_torus_mks/problem_generator.cpp).
+//    The 4% pressure perturbation uses a counter-based generator keyed by
+//    (seed, global cell index), so the data do not depend on thread order.
+#include <cmath>
+#include <cstring>
+
+#include "host.h"
+
+namespace pangu {
+
+HostBlock make_host_block(const pangu_block &b) {
+  HostBlock h;
+  for (int d = 0; d < 3; ++d) {
+    h.nx[d] = b.nx[d];
+    h.ng[d] = b.ng[d];
+    h.n[d] = b.nx[d] + 2 * b.ng[d];
+    h.s[d] = b.ng[d];
+    h.e[d] = b.ng[d] + b.nx[d] - 1;
+    h.lx[d] = b.lx[d];
+    h.dx[d] = b.dx[d];
+    h.xg[d] = b.xmin[d] - b.ng[d] * b.dx[d];
+  }
+  h.cells = (int64_t)h.n[0] * h.n[1] * h.n[2];
+  return h;
+}
+
+static double keyed_uniform(uint64_t seed, uint64_t id) {
+  uint64_t z = id + seed * 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  z ^= z >> 31;
+  return -1.0 + 2.0 * ((double)(z >> 11) * (1.0 / 9007199254740992.0));
+}
+
+static inline double kmax(double a, double b) { return (a < b) ? b : a; }
+
+// b^2 of a primitive state (state_calculation.h:19-69), host version used by
+// the beta normalisation.
+static double bsq_of(const double p[10], const geom::M4 g, const geom::M4 gi) {
+  const double alpha = 1.0 / std::sqrt(-gi[0][0]);
+  const double v[3] = {p[2], p[3], p[4]};
+  double sq = 0.0;
+  bool first = true;
+  for (int a = 0; a < 3; ++a)
+    for (int c = 0; c < 3; ++c) {
+      const double t = v[a] * g[a + 1][c + 1] * v[c];
+      sq = first ? t : sq + t;
+      first = false;
+    }
+  const double lor = std::sqrt(1.0 + sq);
+  double uc[4], ul[4], bc[4], bl[4];
+  uc[0] = lor / alpha;
+  for (int a = 1; a < 4; ++a) uc[a] = v[a - 1] - uc[0] * gi[0][a] * alpha * alpha;
+  for (int r = 0; r < 4; ++r) ul[r] = g[r][0] * uc[0] + g[r][1] * uc[1] + g[r][2] * uc[2] + g[r][3] * uc[3];
+  bc[0] = p[5] * ul[1] + p[6] * ul[2] + p[7] * ul[3];
+  for (int a = 1; a < 4; ++a) bc[a] = (p[4 + a] + bc[0] * uc[a]) / uc[0];
+  for (int r = 0; r < 4; ++r) bl[r] = g[r][0] * bc[0] + g[r][1] * bc[1] + g[r][2] * bc[2] + g[r][3] * bc[3];
+  return bc[0] * bl[0] + bc[1] * bl[1] + bc[2] * bl[2] + bc[3] * bl[3];
+}
+
+// fm_torus_mks/problem_generator.cpp:22-35 (specific angular momentum).
+static double fish_l(double r, double a) {
+  const double sr = std::sqrt(r);
+  const double q = std::pow(a, 2.) - 2. * a * sr + std::pow(r, 2.);
+  const double s1 = std::sqrt(2. * a * sr + (-3. + r) * r);
+  const double t1 = (-2. * a * r * q) / s1;
+  const double t2 = ((a + (-2. + r) * sr) * (std::pow(r, 3.) + std::pow(a, 2.) * (2. + r))) /
+                    std::sqrt(1 + (2. * a) / std::pow(r, 1.5) - 3. / r);
+  return (q * (t1 + t2)) / (std::pow(r, 3.) * s1 * (std::pow(a, 2.) + (-2. + r) * r));
+}
+
+// ln(h) of the FM torus up to the inner-edge constant (:208-222).
+static double fish_lnh_part(double l, double r, double a, double SS, double DD, double AA, double sth) {
+  return 0.5 * std::log((1. + std::sqrt(1. + 4. * (l * l * SS * SS) * DD / (AA * sth * AA * sth))) /
+                        (SS * DD / AA)) -
+         0.5 * std::sqrt(1. + 4. * (l * l * SS * SS) * DD / (AA * AA * sth * sth)) -
+         2. * a * r * l / AA;
+}
+
+static void ot_cell(const pangu_config &cfg, double x, double y, double p[10]) {
+  const double gam = cfg.gamma;
+  const double pi = M_PI;
+  const double rho = 25.0 / (36.0 * pi);
+  const double u = (5.0 / (12.0 * pi)) / (gam - 1.0);
+  const double vx = -0.5 * std::sin(2.0 * pi * y);
+  const double vy = 0.5 * std::sin(2.0 * pi * x);
+  const double W = 1.0 / std::sqrt(1.0 - (vx * vx + vy * vy));
+  const double b0 = 1.0 / std::sqrt(4.0 * pi);
+  p[0] = rho;
+  p[1] = u;
+  p[2] = W * vx;
+  p[3] = W * vy;
+  p[4] = 0.0;
+  p[5] = -b0 * std::sin(2.0 * pi * y);
+  p[6] = b0 * std::sin(4.0 * pi * x);
+  p[7] = 0.0;
+  p[8] = (gam - 1.0) * u * std::pow(rho, -gam);
+}
+
+int generate_problem(const pangu_config &cfg, const ProblemParams &pp,
+                     const std::vector<HostBlock> &blocks, int np, int iBX, int iENT, int iKEL,
+                     std::vector<double> &prim) {
+  int64_t total = 0;
+  std::vector<int64_t> off(blocks.size());
+  for (size_t b = 0; b < blocks.size(); ++b) {
+    off[b] = total;
+    total += blocks[b].cells;
+  }
+  prim.assign((size_t)(total * np), 0.0);
+  auto P = [&](size_t b, int q) { return prim.data() + off[b] * np + (int64_t)q * blocks[b].cells; };
+
+  if (pp.problem == 1) {
+    for (size_t b = 0; b < blocks.size(); ++b) {
+      const HostBlock &B = blocks[b];
+      for (int k = 0; k < B.n[2]; ++k)
+        for (int j = 0; j < B.n[1]; ++j)
+          for (int i = 0; i < B.n[0]; ++i) {
+            double p[10];
+            ot_cell(cfg, B.xc(0, i), B.xc(1, j), p);
+            const int64_t id = B.id(k, j, i);
+            for (int q = 0; q < 5; ++q) P(b, q)[id] = p[q];
+            if (iBX >= 0)
+              for (int q = 0; q < 3; ++q) P(b, iBX + q)[id] = p[5 + q];
+            P(b, iENT)[id] = p[8];
+            if (iKEL >= 0) P(b, iKEL)[id] = pp.fel_0 * p[8];
+          }
+    }
+    return 0;
+  }
+  if (pp.problem != 2 || cfg.metric != PANGU_METRIC_MKS) return -1;
+
+  const double gam = cfg.gamma, a = cfg.a, h = cfg.h, a2 = a * a;
+  const double rin = pp.rin, kap = pp.kappa;
+  const double l = fish_l(pp.rmax, a);
+  const double sthin = std::sin(M_PI_2), cthin = std::cos(M_PI_2);
+  const double DDin = rin * rin - 2. * rin + a * a;
+  const double AAin = (rin * rin + a * a) * (rin * rin + a * a) - DDin * a * a * sthin * sthin;
+  const double SSin = rin * rin + a * a * cthin * cthin;
+  const double lnh_in = fish_lnh_part(l, rin, a, SSin, DDin, AAin, sthin);
+  int G[3], NX[3];
+  for (int d = 0; d < 3; ++d) {
+    G[d] = blocks[0].ng[d];
+    NX[d] = pp.mesh_nx[d] + 2 * G[d];
+  }
+  for (size_t b = 0; b < blocks.size(); ++b) {
+    const HostBlock &B = blocks[b];
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int k = 0; k < B.n[2]; ++k)
+      for (int j = 0; j < B.n[1]; ++j)
+        for (int i = 0; i < B.n[0]; ++i) {
+          const int64_t gi = (int64_t)B.lx[0] * B.nx[0] + i, gj = (int64_t)B.lx[1] * B.nx[1] + j,
+                        gk = (int64_t)B.lx[2] * B.nx[2] + k;
+          const double rnd = keyed_uniform(pp.seed, (uint64_t)((gk * NX[1] + gj) * NX[0] + gi));
+          const double xc[4] = {0.0, B.xc(0, i), B.xc(1, j), B.xc(2, k)};
+          const double r = std::exp(xc[1]);
+          const double th = M_PI_2 * (xc[2] + 1.0) + 0.5 * h * std::sin(M_PI * (xc[2] + 1.0));
+          const double sth = std::sin(th), cth = std::cos(th);
+          geom::M4 g, gbl, gi_;
+          geom::code_metric(PANGU_METRIC_MKS, xc, g, h, a);
+          geom::code_metric(PANGU_METRIC_BL, xc, gbl, h, a);
+          geom::inv4(g, gi_);
+          const double alpha = 1.0 / std::sqrt(-gi_[0][0]);
+          double rho = 0.0, eint = 0.0, ent = 0.0, w[3] = {0.0, 0.0, 0.0};
+          const double DD = r * r - 2.0 * r + a2;
+          const double AA = (r * r + a2) * (r * r + a2) - DD * a2 * sth * sth;
+          const double SS = r * r + a2 * cth * cth;
+          double lnh = 1.0;
+          if (r >= rin) lnh = fish_lnh_part(l, r, a, SS, DD, AA, sth) - lnh_in;
+          if (lnh >= 0.0 && r >= rin) {
+            const double hm1 = std::exp(lnh) - 1.0;
+            if (hm1 > 0.0) {
+              rho = std::pow(hm1 * (gam - 1.0) / (kap * gam), 1.0 / (gam - 1.0));
+              eint = kap * std::pow(rho, gam) / (gam - 1.0);
+              eint *= (1.0 + pp.perturbation * rnd);
+              ent = (gam - 1) * eint * std::pow(rho, -gam);
+              const double e2c = SS * SS * DD / (AA * AA * sth * sth);
+              const double up1 = std::sqrt((-1.0 + std::sqrt(1.0 + 4.0 * l * l * e2c)) / 2.0);
+              const double upbl = 2.0 * a * r * std::sqrt(1.0 + up1 * up1) / std::sqrt(AA * SS * DD) +
+                                  std::sqrt(SS / AA) * up1 / sth;
+              // u^t from g_tt u^t^2 + 2 g_t3 u^t u^3 + (1 + g_33 u^3^2) = 0 (:37-48)
+              const double AAq = gbl[0][0];
+              const double BBq = 2.0 * (gbl[0][1] * 0.0 + gbl[0][2] * 0.0 + gbl[0][3] * upbl);
+              const double CCq = 1.0 + gbl[1][1] * 0.0 * 0.0 + gbl[2][2] * 0.0 * 0.0 +
+                                 gbl[3][3] * upbl * upbl +
+                                 2.0 * (gbl[1][2] * 0.0 * 0.0 + gbl[1][3] * 0.0 * upbl +
+                                        gbl[2][3] * 0.0 * upbl);
+              const double u0bl = (-BBq - std::sqrt(BBq * BBq - 4.0 * AAq * CCq)) / (2.0 * AAq);
+              // BL -> KS (:63-76): with u^r = u^theta = 0 the transformation
+              // leaves (u^t, 0, 0, u^phi) unchanged; then into the code basis
+              // with the inverse FD Jacobian of the coordinate map (:78-100).
+              const double ks[4] = {u0bl, 0.0, 0.0, upbl};
+              geom::M4 J, Ji;
+              for (int c = 0; c < 4; ++c) {
+                double xp[4], xm[4];
+                std::memcpy(xp, xc, sizeof(xp));
+                std::memcpy(xm, xc, sizeof(xm));
+                xp[c] += 1e-5;
+                xm[c] -= 1e-5;
+                const double yp[4] = {xp[0], std::exp(xp[1]),
+                                      M_PI_2 * (xp[2] + 1.0) + 0.5 * h * std::sin(M_PI * (xp[2] + 1.0)),
+                                      xp[3]};
+                const double ym[4] = {xm[0], std::exp(xm[1]),
+                                      M_PI_2 * (xm[2] + 1.0) + 0.5 * h * std::sin(M_PI * (xm[2] + 1.0)),
+                                      xm[3]};
+                for (int rr = 0; rr < 4; ++rr) J[rr][c] = (yp[rr] - ym[rr]) / (xp[c] - xm[c]);
+              }
+              geom::inv4(J, Ji);
+              double uc[4];
+              for (int rr = 0; rr < 4; ++rr)
+                uc[rr] = Ji[rr][0] * ks[0] + Ji[rr][1] * ks[1] + Ji[rr][2] * ks[2] + Ji[rr][3] * ks[3];
+              for (int q = 0; q < 3; ++q) w[q] = uc[q + 1] + (gi_[0][q + 1] * alpha * alpha) * uc[0];
+            }
+          }
+          const int64_t id = B.id(k, j, i);
+          P(b, 0)[id] = rho;
+          P(b, 1)[id] = eint;
+          P(b, 2)[id] = w[0];
+          P(b, 3)[id] = w[1];
+          P(b, 4)[id] = w[2];
+          P(b, iENT)[id] = ent;
+          if (iKEL >= 0) P(b, iKEL)[id] = pp.fel_0 * ent;
+        }
+  }
+  // maxima over entire blocks, then rho, u normalisation (:315-390)
+  double rhomax = 0.0, umax = 0.0;
+  for (size_t b = 0; b < blocks.size(); ++b) {
+    double bm = 0.0, um = 0.0;
+    for (int64_t id = 0; id < blocks[b].cells; ++id) {
+      bm = kmax(bm, P(b, 0)[id]);
+      um = kmax(um, P(b, 1)[id]);
+    }
+    rhomax = kmax(rhomax, bm);
+    umax = kmax(umax, um);
+  }
+  for (size_t b = 0; b < blocks.size(); ++b)
+    for (int64_t id = 0; id < blocks[b].cells; ++id) {
+      P(b, 0)[id] /= rhomax;
+      P(b, 1)[id] /= rhomax;
+    }
+  if (iBX < 0) return 0;
+  // SANE A_phi at corners and B = curl A on the interior (:394-458)
+  for (size_t b = 0; b < blocks.size(); ++b) {
+    const HostBlock &B = blocks[b];
+    std::vector<double> A((size_t)B.cells, 0.0);
+    const double *rho = P(b, 0);
+    for (int k = 0; k < B.n[2]; ++k)
+      for (int j = 1; j < B.n[1]; ++j)
+        for (int i = 1; i < B.n[0]; ++i) {
+          const double avg =
+              0.25 * (rho[B.id(k, j, i)] + rho[B.id(k, j, i - 1)] + rho[B.id(k, j - 1, i)] + rho[B.id(k, j - 1, i - 1)]);
+          const double ex = avg - pp.aphi_cut;
+          A[B.id(k, j, i)] = (ex > 0.0) ? ex : 0.0;
+        }
+    for (int k = B.s[2]; k <= B.e[2]; ++k)
+      for (int j = B.s[1]; j <= B.e[1]; ++j)
+        for (int i = B.s[0]; i <= B.e[0]; ++i) {
+          const double xc[4] = {0.0, B.xc(0, i), B.xc(1, j), B.xc(2, k)};
+          geom::M4 g;
+          geom::code_metric(PANGU_METRIC_MKS, xc, g, h, a);
+          const double sg = std::sqrt(std::fabs(geom::det4(g)));
+          const double a00 = A[B.id(k, j - 1, i - 1)], a01 = A[B.id(k, j, i - 1)];
+          const double a10 = A[B.id(k, j - 1, i)], a11 = A[B.id(k, j, i)];
+          P(b, iBX)[B.id(k, j, i)] = -(a00 - a01 + a10 - a11) / (2.0 * B.dx[1] * sg);
+          P(b, iBX + 1)[B.id(k, j, i)] = (a00 + a01 - a10 - a11) / (2.0 * B.dx[0] * sg);
+        }
+  }
+  // beta normalisation (:464-551)
+  double bsqmax = 0.0;
+  for (size_t b = 0; b < blocks.size(); ++b) {
+    const HostBlock &B = blocks[b];
+    double bm = 0.0;
+    for (int k = B.s[2]; k <= B.e[2]; ++k)
+      for (int j = B.s[1]; j <= B.e[1]; ++j)
+        for (int i = B.s[0]; i <= B.e[0]; ++i) {
+          const double xc[4] = {0.0, B.xc(0, i), B.xc(1, j), B.xc(2, k)};
+          geom::M4 g, gi_;
+          geom::code_metric(PANGU_METRIC_MKS, xc, g, h, a);
+          geom::inv4(g, gi_);
+          const int64_t id = B.id(k, j, i);
+          double p[10] = {P(b, 0)[id], P(b, 1)[id], P(b, 2)[id], P(b, 3)[id], P(b, 4)[id],
+                          P(b, iBX)[id], P(b, iBX + 1)[id], P(b, iBX + 2)[id], 0.0, 0.0};
+          bm = kmax(bm, bsq_of(p, g, gi_));
+        }
+    bsqmax = kmax(bsqmax, bm);
+  }
+  const double beta_min = 2.0 * (gam - 1.0) * (umax / rhomax) / (bsqmax + 1.0e-30);
+  const double norm = std::sqrt(beta_min / pp.beta);
+  for (size_t b = 0; b < blocks.size(); ++b) {
+    const HostBlock &B = blocks[b];
+    for (int k = B.s[2]; k <= B.e[2]; ++k)
+      for (int j = B.s[1]; j <= B.e[1]; ++j)
+        for (int i = B.s[0]; i <= B.e[0]; ++i)
+          for (int q = 0; q < 3; ++q) P(b, iBX + q)[B.id(k, j, i)] *= norm;
+  }
+  return 0;
+}
+
+}  // namespace pangu

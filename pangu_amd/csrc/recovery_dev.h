@@ -1,0 +1,473 @@
+// Device conserved-to-primitive recovery (gfx950, FP64).
+//
+// Inputs are the "HARM-normalised" 9-vectors of invertor.cc:149-174:
+// U = (alpha/sqrt|g|) * conserved with the energy slot holding (tau - D);
+// prim carries the previous primitives (initial guess; the 1D schemes take
+// the entropy K from it).  Operand order follows the reference schemes.
+#pragma once
+#include "physics_dev.h"
+
+namespace pangu {
+
+constexpr int MAX_NEWT_ITER = 30;          // recovery/constants.h:15-23
+constexpr double NEWT_TOL = 1e-10;
+constexpr double MIN_NEWT_TOL = 1e-10;
+constexpr int EXTRA_NEWT_ITER = 2;
+constexpr double W_TOO_BIG = 1.e20;
+constexpr double UTSQ_TOO_BIG = 1.e20;
+constexpr double FAIL_VAL = 1.e30;
+
+// Metric view for recovery: full gcov and gcon plus sqrt(-1/gcon00).
+struct CenMetric {
+  double gcov[16], gcon[16], lapse;
+};
+
+struct Noble {
+  double Bcon[4], Bcov[4], Qcov[4], Qcon[4], ncon[4];
+  double ncov0, Bsq, QdotB, QdotBsq, Qdotn, Qtsq, D;
+};
+
+__device__ __forceinline__ void noble_setup(const double U[9], const CenMetric &m, Noble &v) {
+  v.Bcon[0] = 0.;
+  v.Bcon[1] = U[BX1];
+  v.Bcon[2] = U[BX2];
+  v.Bcon[3] = U[BX3];
+  v.Qcov[0] = U[ENY];
+  v.Qcov[1] = U[UX1];
+  v.Qcov[2] = U[UX2];
+  v.Qcov[3] = U[UX3];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    v.Bcov[r] = m.gcov[4 * r] * v.Bcon[0] + m.gcov[4 * r + 1] * v.Bcon[1] +
+                m.gcov[4 * r + 2] * v.Bcon[2] + m.gcov[4 * r + 3] * v.Bcon[3];
+    v.Qcon[r] = m.gcon[4 * r] * v.Qcov[0] + m.gcon[4 * r + 1] * v.Qcov[1] +
+                m.gcon[4 * r + 2] * v.Qcov[2] + m.gcon[4 * r + 3] * v.Qcov[3];
+  }
+  v.Bsq = v.Bcon[1] * v.Bcov[1] + v.Bcon[2] * v.Bcov[2] + v.Bcon[3] * v.Bcov[3];
+  const double qb = v.Qcov[0] * v.Bcon[0] + v.Qcov[1] * v.Bcon[1] + v.Qcov[2] * v.Bcon[2] +
+                    v.Qcov[3] * v.Bcon[3];
+  v.QdotB = qb;
+  v.QdotBsq = qb * qb;
+  // n_mu = (-lapse, 0, 0, 0): n^mu = gcon[mu][0] * (-lapse)
+  v.ncov0 = -m.lapse;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) v.ncon[r] = m.gcon[4 * r] * v.ncov0;
+  v.Qdotn = v.Qcon[0] * v.ncov0;
+  const double qsq = v.Qcov[0] * v.Qcon[0] + v.Qcov[1] * v.Qcon[1] + v.Qcov[2] * v.Qcon[2] +
+                     v.Qcov[3] * v.Qcon[3];
+  v.Qtsq = qsq + v.Qdotn * v.Qdotn;
+  v.D = U[RHO];
+}
+
+__device__ __forceinline__ bool noble_utsq(const double p[9], const CenMetric &m, double &out) {
+  double u = 0.;
+#pragma unroll
+  for (int i = 1; i < 4; i++)
+#pragma unroll
+    for (int j = 1; j < 4; j++) u += m.gcov[4 * i + j] * p[UX1 + i - 1] * p[UX1 + j - 1];
+  if ((u < 0.) && (fabs(u) < 1.0e-13)) u = fabs(u);
+  if (u < 0. || u > UTSQ_TOO_BIG) return false;
+  out = u;
+  return true;
+}
+
+__device__ __forceinline__ void noble_velocity(const Noble &v, double W, double Gamma, double p[9]) {
+#pragma unroll
+  for (int i = 1; i < 4; i++) {
+    const double qt = v.Qcon[i] + v.ncon[i] * v.Qdotn;
+    p[UX1 + i - 1] = Gamma / (W + v.Bsq) * (qt + v.QdotB * v.Bcon[i] / W);
+  }
+}
+
+__device__ __forceinline__ double bump_w(double W, double Bsq, double QdotBsq, double Qtsq) {
+  int inc = 0;
+  while (((W * W * W * (W + 2. * Bsq) - QdotBsq * (2. * W + Bsq)) <= W * W * (Qtsq - Bsq * Bsq)) &&
+         (inc < 10)) {
+    W *= 10.;
+    inc++;
+  }
+  return W;
+}
+
+__device__ __forceinline__ double vsq_of_w(double Bsq, double QdotBsq, double Qtsq, double W) {
+  const double Wsq = W * W;
+  const double Xsq = (Bsq + W) * (Bsq + W);
+  return ((Wsq * Qtsq + QdotBsq * (Bsq + 2. * W)) / (Wsq * Xsq));
+}
+
+// ------------------------------------------------- 2D (W, v^2) Newton scheme
+// scheme_2d.h:99-404.
+__device__ int c2p_2d(const double U[9], double p[9], double gamma, const CenMetric &m, int &iters) {
+  iters = 0;
+  Noble v;
+  noble_setup(U, m, v);
+  double utsq;
+  if (!noble_utsq(p, m, utsq)) return 2;
+  const double gammasq = 1. + utsq;
+  double Gamma = sqrt(gammasq);
+  double rho0 = v.D / Gamma;
+  double uu = p[ENY];
+  double pr = ((gamma - 1.) * uu);
+  double w = rho0 + uu + pr;
+  double W_last = bump_w(w * gammasq, v.Bsq, v.QdotBsq, v.Qtsq);
+  double x0 = fabs(W_last);
+  double x1;
+  {
+    const double vs = fabs(vsq_of_w(v.Bsq, v.QdotBsq, v.Qtsq, W_last));
+    x1 = (vs > 1.) ? (1.0 - 1.e-15) : vs;
+  }
+  const double Bsq = v.Bsq, QdotBsq = v.QdotBsq, Qtsq = v.Qtsq, Qdotn = v.Qdotn, D = v.D;
+  double f = 1., df = 1., errx = 1.;
+  int i_extra = 0, doing_extra = 0, n_iter = 0;
+  bool keep = true;
+  while (keep) {
+    const double W = x0, vsq = x1, Wsq = W * W;
+    const double gtmp = 1. - vsq;
+    const double p_tmp = ((gamma - 1.) * (W * gtmp - D * sqrt(gtmp)) / gamma);
+    const double dPdW = ((gamma - 1.) * (1. - vsq) / gamma);
+    const double dPdvsq = ((gamma - 1.) * (0.5 * D / sqrt(1. - vsq) - W) / gamma);
+    const double t2 = -0.5 * Bsq + dPdvsq;
+    const double t3 = Bsq + W;
+    const double t4 = t3 * t3;
+    const double t9 = 1.0 / Wsq;
+    const double t11 = Qtsq - vsq * t4 + QdotBsq * (Bsq + 2.0 * W) * t9;
+    const double t16 = QdotBsq * t9;
+    const double t18 = -Qdotn - 0.5 * Bsq * (1.0 + vsq) + 0.5 * t16 - W + p_tmp;
+    const double t21 = 1.0 / t3;
+    const double t23 = 1.0 / W;
+    const double t24 = t16 * t23;
+    const double t25 = -1.0 + dPdW - t24;
+    const double t35 = t25 * t3 + (Bsq - 2.0 * dPdvsq) * (QdotBsq + vsq * Wsq * W) * t9 * t23;
+    const double t36 = 1.0 / t35;
+    const double dx0 = -(t2 * t11 + t4 * t18) * t21 * t36;
+    const double t40 = (vsq + t24) * t3;
+    const double dx1 = -(-t25 * t11 - 2.0 * t40 * t18) * t21 * t36;
+    df = -t11 * t11 - t18 * t18;
+    f = -0.5 * df;
+    const double o0 = x0;
+    x0 += dx0;
+    x1 += dx1;
+    errx = (x0 == 0.) ? fabs(dx0) : fabs(dx0 / x0);
+    errx += (x1 == 0.) ? fabs(dx1) : fabs(dx1 / x1);
+    x0 = fabs(x0);
+    x0 = (x0 > W_TOO_BIG) ? o0 : x0;
+    x1 = (x1 < 0.) ? 0. : x1;
+    x1 = (x1 > 1.) ? (1. - 1.e-15) : x1;
+    if ((fabs(errx) <= NEWT_TOL) && (doing_extra == 0) && (EXTRA_NEWT_ITER > 0)) doing_extra = 1;
+    if (doing_extra == 1) i_extra++;
+    if (((fabs(errx) <= NEWT_TOL) && (doing_extra == 0)) || (i_extra > EXTRA_NEWT_ITER) ||
+        (n_iter >= (MAX_NEWT_ITER - 1)))
+      keep = false;
+    n_iter++;
+  }
+  iters = n_iter;
+  int retval = 0;
+  if (!isfinite(f) || !isfinite(df))
+    retval = 2;
+  else if (fabs(errx) > MIN_NEWT_TOL)
+    retval = 1;
+  const double W = x0, vsq = x1;
+  if ((retval != 0) || (W == FAIL_VAL)) return retval * 100 + 1;
+  if (W <= 0. || W > W_TOO_BIG) return 3;
+  if (vsq >= 1.) return 4;
+  const double gt = sqrt(1. - vsq);
+  Gamma = 1. / gt;
+  rho0 = D * gt;
+  w = W * (1. - vsq);
+  pr = ((gamma - 1.) * (w - rho0) / gamma);
+  uu = w - (rho0 + pr);
+  if ((rho0 <= 0.) || (uu <= 0.)) return 5;
+  p[RHO] = rho0;
+  p[ENY] = uu;
+  noble_velocity(v, W, Gamma, p);
+  return 0;
+}
+
+// -------------------------------------------------- 1D v^2 isentropic scheme
+__device__ __forceinline__ double w_of_vsq(double D, double K, double vsq, double gamma,
+                                           double &pr, double &rho, double &uu) {
+  const double gtmp = (1. - vsq);
+  rho = D * sqrt(gtmp);
+  pr = (K * pow(rho, gamma));
+  uu = (pr / (gamma - 1.));
+  return ((rho + uu + pr) / gtmp);
+}
+
+// scheme_1d_vsq.h:52-351 (K = previous primitive entropy).
+__device__ int c2p_1dvsq(const double U[9], double p[9], double gamma, const CenMetric &m,
+                         int &iters) {
+  iters = 0;
+  const double K = p[ENT];
+  Noble v;
+  noble_setup(U, m, v);
+  double utsq;
+  if (!noble_utsq(p, m, utsq)) return 2;
+  const double gammasq = 1. + utsq;
+  double x = 1. - 1. / gammasq;
+  const double Bsq = v.Bsq, QdotBsq = v.QdotBsq, Qtsq = v.Qtsq, D = v.D;
+  double f = 1., df = 1., errx = 1., W = 0., pr, rho, uu;
+  int i_extra = 0, doing_extra = 0, n_iter = 0;
+  bool keep = true;
+  while (keep) {
+    double Wf = w_of_vsq(D, K, x, gamma, pr, rho, uu);
+    {
+      const double vsq = x;
+      const double Wsq = Wf * Wf;
+      const double W3 = Wf * Wsq;
+      const double dWdvsq =
+          ((gamma * (2. - gamma) * pr + (gamma - 1.) * rho) / (2. * (gamma - 1.) * (1. - vsq) * (1. - vsq)));
+      const double fact = (Bsq + Wf);
+      const double resid = Qtsq - vsq * fact * fact + QdotBsq * (Bsq + 2. * Wf) / Wsq;
+      const double jac = -fact * (fact + 2. * dWdvsq * (vsq + QdotBsq / W3));
+      const double dx = -resid / jac;
+      f = 0.5 * resid * resid;
+      df = -2. * f;
+      const double xo = x;
+      x += dx;
+      x = (x >= 1.0) ? (0.5 * (xo + 1.)) : x;
+      x = (x < -1.e-10) ? (0.5 * xo) : x;
+      x = fabs(x);
+      const double Wo = W;
+      W = w_of_vsq(D, K, x, gamma, pr, rho, uu);
+      errx = (W == 0.) ? fabs(W - Wo) : fabs((W - Wo) / W);
+      errx += (x == 0.) ? fabs(x - xo) : fabs((x - xo) / x);
+    }
+    if ((fabs(errx) <= NEWT_TOL) && (doing_extra == 0) && (EXTRA_NEWT_ITER > 0)) doing_extra = 1;
+    if (doing_extra == 1) i_extra++;
+    if (((fabs(errx) <= NEWT_TOL) && (doing_extra == 0)) || (i_extra > EXTRA_NEWT_ITER) ||
+        (n_iter >= (MAX_NEWT_ITER - 1)))
+      keep = false;
+    n_iter++;
+  }
+  iters = n_iter;
+  int retval = 0;
+  if (!isfinite(f) || !isfinite(df))
+    retval = 2;
+  else if (fabs(errx) > MIN_NEWT_TOL)
+    retval = 1;
+  if (retval != 0) return retval * 100 + 1;
+  const double vsq = x;
+  if ((vsq >= 1.) || (vsq < 0.)) return 4;
+  double rho0;
+  W = w_of_vsq(D, K, vsq, gamma, pr, rho0, uu);
+  const double Gamma = 1. / sqrt(1. - vsq);
+  if ((rho0 <= 0.) || (uu <= 0.)) return 5;
+  p[RHO] = rho0;
+  p[ENY] = uu;
+  noble_velocity(v, W, Gamma, p);
+  return 0;
+}
+
+// --------------------------------------------------- 1D W isentropic scheme
+// scheme_1d.h:196-568 with func_1d_orig2.
+__device__ int c2p_1d(const double U[9], double p[9], double gamma, const CenMetric &m, int &iters) {
+  iters = 0;
+  const double K = p[ENT];
+  Noble v;
+  noble_setup(U, m, v);
+  double utsq;
+  if (!noble_utsq(p, m, utsq)) return 2;
+  const double uvsq = 1. + utsq;
+  double Gamma = sqrt(uvsq);
+  double rho0 = v.D / Gamma;
+  double pr = (K * pow(rho0, gamma));
+  double uu = (pr / (gamma - 1.));
+  const double w = rho0 + uu + pr;
+  double x = bump_w(w * uvsq, v.Bsq, v.QdotBsq, v.Qtsq);
+  const double Bsq = v.Bsq, QdotBsq = v.QdotBsq, Qtsq = v.Qtsq, D = v.D;
+  double f = 1., df = 1., errx = 1.;
+  int i_extra = 0, doing_extra = 0, n_iter = 0;
+  bool keep = true;
+  while (keep) {
+    const double W = x;
+    const double t1 = D * D;
+    const double t2 = QdotBsq * t1;
+    const double t3 = t2 * Bsq;
+    const double t5 = Bsq * Bsq;
+    const double t8 = t1 * Bsq;
+    const double t10 = t1 * W;
+    const double t21 = W * W;
+    const double rho = t1 * (1. + gamma * K / (gamma - 1.)) / W;
+    const double t23 = rho * rho;
+    const double t26 = 1 / t1;
+    const double resid = (t3 + (2.0 * t2 + ((Qtsq - t5) * t1 + (-2.0 * t8 - t10) * W) * W) * W +
+                          (t5 + (2.0 * Bsq + W) * W) * t21 * t23) *
+                         t26 / t21;
+    const double jac =
+        -2.0 * (t3 + (t2 + (t8 + t10) * t21) * W + (t5 + Bsq * W) * t21 * t23) * t26 / t21 / W;
+    const double dx = -resid / jac;
+    f = 0.5 * resid * resid;
+    df = -2. * f;
+    x += dx;
+    int inc = 0;
+    while (((x * x * x * (x + 2. * Bsq) - QdotBsq * (2. * x + Bsq)) <= x * x * (Qtsq - Bsq * Bsq)) &&
+           (inc < 10)) {
+      x -= (1. * inc) * dx / 10.;
+      inc++;
+    }
+    errx = (x == 0.) ? fabs(dx) : fabs(dx / x);
+    x = fabs(x);
+    if ((fabs(errx) <= NEWT_TOL) && (doing_extra == 0) && (EXTRA_NEWT_ITER > 0)) doing_extra = 1;
+    if (doing_extra == 1) i_extra++;
+    if (((fabs(errx) <= NEWT_TOL) && (doing_extra == 0)) || (i_extra > EXTRA_NEWT_ITER) ||
+        (n_iter >= (MAX_NEWT_ITER - 1)))
+      keep = false;
+    n_iter++;
+  }
+  iters = n_iter;
+  int retval = 0;
+  if (!isfinite(f) || !isfinite(df) || !isfinite(x))
+    retval = 2;
+  else if (fabs(errx) > MIN_NEWT_TOL)
+    retval = 1;
+  const double W = x;
+  if ((retval != 0) || (W == FAIL_VAL)) return retval * 100 + 1;
+  if (W <= 0. || W > W_TOO_BIG) return 3;
+  const double vsq = vsq_of_w(Bsq, QdotBsq, Qtsq, W);
+  if (vsq >= 1.) return 4;
+  const double gt = sqrt(1. - vsq);
+  Gamma = 1. / gt;
+  rho0 = D * gt;
+  pr = (K * pow(rho0, gamma));
+  uu = (pr / (gamma - 1.));
+  if ((rho0 <= 0.) || (uu <= 0.)) return 5;
+  p[RHO] = rho0;
+  p[ENY] = uu;
+  noble_velocity(v, W, Gamma, p);
+  return 0;
+}
+
+// ----------------------------------------------------- Kastaun et al. 2021
+// scheme_kastaun.h:84-108.
+__device__ __forceinline__ double kastaun_root(double mu, double D, double q, double bsq, double rsq,
+                                               double rbsq, double gamma) {
+  const double x = 1.0 / (1.0 + mu * bsq);
+  const double xsq = x * x;
+  const double rbarsq = x * (rsq * x + mu * (x + 1.0) * rbsq);
+  const double musq = mu * mu;
+  double vsq = musq * rbarsq;
+  vsq = kmin(kmax(vsq, 0.0), 1.0 - 1.0e-14);
+  const double il = sqrt(1.0 - vsq);
+  const double qbar = q - 0.5 * bsq - 0.5 * musq * xsq * (bsq * rsq - rbsq);
+  const double eod = qbar - mu * rbarsq + 1.0;
+  const double rho = D * il;
+  double eps = D * eod - rho;
+  eps = kmax(eps, 1.0e-300);
+  const double pr = (gamma - 1.0) * eps;
+  const double hh = 1.0 + (eps + pr) / rho;
+  const double nu = kmax(hh * il, eod + pr / D);
+  return mu - 1.0 / (nu + mu * rbarsq);
+}
+
+// scheme_kastaun.h:110-259: Illinois false position on [0,1].
+__device__ int c2p_kastaun(const double U[9], double p[9], double gamma, const CenMetric &m,
+                           int &iters) {
+  iters = 0;
+  const double g0 = m.gcov[5], g1 = m.gcov[6], g2 = m.gcov[7], g3 = m.gcov[10], g4 = m.gcov[11],
+               g5 = m.gcov[15];
+  const double det = g0 * g3 * g5 + 2.0 * g1 * g2 * g4 - g0 * g4 * g4 - g1 * g1 * g5 - g2 * g2 * g3;
+  if (!(det > 0.0) || !isfinite(det)) return 1;
+  const double idet = 1.0 / det;
+  const double i0 = (g3 * g5 - g4 * g4) * idet, i1 = (g2 * g4 - g1 * g5) * idet,
+               i2 = (g1 * g4 - g2 * g3) * idet, i3 = (g0 * g5 - g2 * g2) * idet,
+               i4 = (g1 * g2 - g0 * g4) * idet, i5 = (g0 * g3 - g1 * g1) * idet;
+  const double D = U[RHO];
+  if (!(D > 0.0) || !isfinite(D)) return 2;
+  const double ncov0 = -m.lapse;
+  const double Qcon0 =
+      m.gcon[0] * U[ENY] + m.gcon[1] * U[UX1] + m.gcon[2] * U[UX2] + m.gcon[3] * U[UX3];
+  const double Qdotn = Qcon0 * ncov0;
+  const double tau = -Qdotn - D;
+  if (!isfinite(tau)) return 3;
+  const double S0 = U[UX1], S1 = U[UX2], S2 = U[UX3];
+  const double Su0 = i0 * S0 + i1 * S1 + i2 * S2;
+  const double Su1 = i1 * S0 + i3 * S1 + i4 * S2;
+  const double Su2 = i2 * S0 + i4 * S1 + i5 * S2;
+  const double B0 = U[BX1], B1 = U[BX2], B2 = U[BX3];
+  const double Bd0 = g0 * B0 + g1 * B1 + g2 * B2;
+  const double Bd1 = g1 * B0 + g3 * B1 + g4 * B2;
+  const double Bd2 = g2 * B0 + g4 * B1 + g5 * B2;
+  const double Bsq = B0 * Bd0 + B1 * Bd1 + B2 * Bd2;
+  const double Ssq = Su0 * S0 + Su1 * S1 + Su2 * S2;
+  const double SdotB = B0 * S0 + B1 * S1 + B2 * S2;
+  if (!isfinite(Bsq) || !isfinite(Ssq) || !isfinite(SdotB)) return 4;
+  const double inv_D = 1.0 / D;
+  const double sqrt_D = sqrt(D);
+  const double bsq = Bsq * inv_D;
+  const double rsq = Ssq * inv_D * inv_D;
+  const double rb = SdotB / (D * sqrt_D);
+  const double rbsq = rb * rb;
+  const double q = tau * inv_D;
+  // false position (scheme_kastaun.h:110-165)
+  double lo = 0.0, hi = 1.0;
+  double flo = kastaun_root(lo, D, q, bsq, rsq, rbsq, gamma);
+  double fhi = kastaun_root(hi, D, q, bsq, rsq, rbsq, gamma);
+  double mu = 0.0;
+  int rs = -1;
+  if (!isfinite(flo) || !isfinite(fhi)) {
+    rs = 1;
+  } else if (fabs(flo) <= NEWT_TOL) {
+    mu = lo;
+    rs = 0;
+  } else if (fabs(fhi) <= NEWT_TOL) {
+    mu = hi;
+    rs = 0;
+  } else if (flo * fhi > 0.0) {
+    rs = 2;
+  } else {
+    int side = 0;
+    mu = lo;
+    rs = 4;
+    for (int it = 0; it < MAX_NEWT_ITER; ++it) {
+      const double old = mu;
+      mu = (fhi * lo - flo * hi) / (fhi - flo);
+      const double fm = kastaun_root(mu, D, q, bsq, rsq, rbsq, gamma);
+      iters = it + 1;
+      if (!isfinite(mu) || !isfinite(fm)) {
+        rs = 3;
+        break;
+      }
+      if (fabs((mu - old) / (mu + 1.0e-300)) <= NEWT_TOL) {
+        rs = 0;
+        break;
+      }
+      if (fm * flo >= 0.0) {
+        if (side == 1) {
+          const double wgt = 1.0 - fm / flo;
+          fhi = (wgt > 0.0) ? fhi * wgt : 0.5 * fhi;
+        }
+        lo = mu;
+        flo = fm;
+        side = 1;
+      } else {
+        if (side == -1) {
+          const double wgt = 1.0 - fm / fhi;
+          flo = (wgt > 0.0) ? flo * wgt : 0.5 * flo;
+        }
+        hi = mu;
+        fhi = fm;
+        side = -1;
+      }
+    }
+  }
+  if (rs != 0) return 100 + rs;
+  const double x = 1.0 / (1.0 + mu * bsq);
+  const double rbarsq = x * (rsq * x + mu * (x + 1.0) * rbsq);
+  const double vsq = mu * mu * rbarsq;
+  if (!(vsq >= 0.0) || vsq >= 1.0 || !isfinite(vsq)) return 5;
+  const double lorentz = 1.0 / sqrt(1.0 - vsq);
+  const double rho = D / lorentz;
+  const double qbar = q - 0.5 * bsq - 0.5 * mu * mu * x * x * (bsq * rsq - rbsq);
+  const double eod = qbar - mu * rbarsq + 1.0;
+  const double eps = D * eod - rho;
+  if (!(rho > 0.0) || !(eps > 0.0) || !isfinite(eps)) return 6;
+  const double Wmux = lorentz * mu * x;
+  p[RHO] = rho;
+  p[ENY] = eps;
+  p[UX1] = Wmux * (Su0 * inv_D + rb * mu * (B0 / sqrt_D));
+  p[UX2] = Wmux * (Su1 * inv_D + rb * mu * (B1 / sqrt_D));
+  p[UX3] = Wmux * (Su2 * inv_D + rb * mu * (B2 / sqrt_D));
+  return 0;
+}
+
+}  // namespace pangu

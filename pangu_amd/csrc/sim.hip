@@ -1,0 +1,548 @@
+// Host driver above the pangu_hip C ABI (see include/pangu_sim.h).
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pangu_hip.h"
+#include "../../include/pangu_sim.h"
+#include "host.h"
+
+using namespace pangu;
+
+// field ids (shared with the test harness)
+enum {
+  F_PRIM = 0, F_CONS = 1, F_FLUX1 = 2, F_FLUX2 = 3, F_FLUX3 = 4, F_ALFVEN = 5, F_EMF = 6,
+  F_FLAG = 7, F_MASK = 8, F_STATUS = 9, F_ITERS = 10, F_FLAG_REC = 11, F_DUDT = 12
+};
+// task ids
+enum {
+  T_FIX = 0, T_CONS, T_FLUXES, T_CT, T_DIV, T_UPDATE, T_SOURCE, T_RECOVERY, T_HEATING,
+  T_FIXREC, T_EXCHANGE, T_BC, T_TIMESTEP
+};
+
+struct pangu_sim {
+  pangu_sim_cfg cfg;
+  int device = 0;
+  pangu_ctx *ctx = nullptr;
+  std::vector<pangu_block> blocks;
+  std::vector<HostBlock> hb;
+  int nblk = 0, np = 0, nc = 0, iBX = -1, iENT = 0, iKEL = -1;
+  int64_t ncell = 0, total = 0;
+  double *P[2] = {nullptr, nullptr}, *U[2] = {nullptr, nullptr}, *dudt = nullptr;
+  double *F[3] = {nullptr, nullptr, nullptr};
+  double *alfven = nullptr, *emf = nullptr, *flag = nullptr, *flag_rec = nullptr;
+  uint8_t *mask = nullptr;
+  int32_t *status = nullptr, *iters = nullptr;
+  int h_base = -1, h_s1 = -1, h_dudt = -1;
+  double *dstate = nullptr;  // {dt, time}
+  double dt = 0, time = 0, dt_est = DBL_MAX;
+  int ncycle = 0;
+  bool stage1_ready = false;
+  bool dt_on_device = false;  // the fused path owns dt/time on the device
+  hipStream_t stream = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  std::string err;
+};
+
+static int sfail(pangu_sim *s, int code, const std::string &m) {
+  s->err = m;
+  return code;
+}
+#define SCHK(s, call)                                                                       \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess) return sfail(s, PANGU_EDEVICE, std::string(#call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+#define PCHK(s, call)                                                               \
+  do {                                                                              \
+    int r_ = (call);                                                                \
+    if (r_ != 0) return sfail(s, r_, std::string(#call ": ") + pangu_last_error(s->ctx)); \
+  } while (0)
+
+// Parthenon/Athena++ uniform mesh generator for interior block edges; mesh
+// edges are exact.
+static double mesh_gen(int idx, int n, double xmin, double xmax) {
+  const long noff = idx - n / 2;
+  const long noffc = idx - (n + 1) / 2;
+  const double x = (double)(noff + noffc) / (2.0 * n);
+  return 0.5 * (xmin + xmax) + (x * xmax - x * xmin);
+}
+
+template <class T>
+static int dalloc(pangu_sim *s, T **p, int64_t n) {
+  SCHK(s, hipMalloc((void **)p, (size_t)(n > 0 ? n : 1) * sizeof(T)));
+  SCHK(s, hipMemset(*p, 0, (size_t)(n > 0 ? n : 1) * sizeof(T)));
+  return 0;
+}
+
+static void sync_dt_from_device(pangu_sim *s) {
+  if (!s->dt_on_device) return;
+  double st[2];
+  hipMemcpy(st, s->dstate, sizeof(st), hipMemcpyDeviceToHost);
+  s->dt = st[0];
+  s->time = st[1];
+}
+
+static void push_dt_to_device(pangu_sim *s) {
+  const double st[2] = {s->dt, s->time};
+  hipMemcpy(s->dstate, st, sizeof(st), hipMemcpyHostToDevice);
+}
+
+extern "C" {
+
+int pangu_sim_create(const pangu_sim_cfg *c, int device, pangu_sim **out) {
+  if (!c || !out) return PANGU_EINVAL;
+  *out = nullptr;
+  for (int d = 0; d < 3; ++d)
+    if (c->nb[d] <= 0 || c->nx[d] % c->nb[d] != 0) return PANGU_EINVAL;
+  pangu_sim *s = new pangu_sim();
+  s->cfg = *c;
+  s->device = device;
+  if (hipSetDevice(device) != hipSuccess) {
+    delete s;
+    return PANGU_EDEVICE;
+  }
+  s->nblk = c->nb[0] * c->nb[1] * c->nb[2];
+  for (int b = 0; b < s->nblk; ++b) {
+    pangu_block B;
+    const int lx[3] = {b % c->nb[0], (b / c->nb[0]) % c->nb[1], b / (c->nb[0] * c->nb[1])};
+    for (int d = 0; d < 3; ++d) {
+      B.lx[d] = lx[d];
+      B.nx[d] = c->nx[d] / c->nb[d];
+      B.ng[d] = (c->nx[d] > 1) ? c->ng : 0;
+      const double lo = (lx[d] == 0) ? c->xmin[d] : mesh_gen(lx[d], c->nb[d], c->xmin[d], c->xmax[d]);
+      const double hi =
+          (lx[d] == c->nb[d] - 1) ? c->xmax[d] : mesh_gen(lx[d] + 1, c->nb[d], c->xmin[d], c->xmax[d]);
+      B.xmin[d] = lo;
+      B.dx[d] = (hi - lo) / B.nx[d];
+    }
+    s->blocks.push_back(B);
+    s->hb.push_back(make_host_block(B));
+  }
+  pangu_config pc;
+  std::memset(&pc, 0, sizeof(pc));
+  pc.metric = c->metric;
+  pc.a = c->a;
+  pc.h = c->h;
+  pc.gamma = c->gamma;
+  pc.cfl = c->cfl;
+  pc.riemann = c->riemann;
+  pc.limiter = c->limiter;
+  pc.recovery = c->recovery;
+  pc.enable_B = c->enable_B;
+  pc.enable_heating = c->enable_heating;
+  pc.density_floor = c->density_floor;
+  pc.energy_floor = c->energy_floor;
+  pc.density_floor_pow = c->density_floor_pow;
+  pc.energy_floor_pow = c->energy_floor_pow;
+  pc.sigma_max = c->sigma_max;
+  pc.lorentz_max = c->lorentz_max;
+  pc.heat_model = c->heat_model;
+  pc.fel_constant = c->fel_constant;
+  pc.gamma_e = c->gamma_e;
+  pc.gamma_p = c->gamma_p;
+  pc.ratio_min = c->ratio_min;
+  pc.ratio_max = c->ratio_max;
+  pc.limit_kel = c->limit_kel;
+  pc.suppress_highb_heat = c->suppress_highb_heat;
+  pc.enforce_positive_dissipation = c->enforce_positive_dissipation;
+  pc.excise = c->excise;
+  pc.r_excise = c->r_excise;
+  pc.dexcise = c->dexcise;
+  pc.pexcise = c->pexcise;
+  for (int d = 0; d < 3; ++d) {
+    pc.nb[d] = c->nb[d];
+    pc.bc[d][0] = c->bc[d][0];
+    pc.bc[d][1] = c->bc[d][1];
+  }
+  int r = pangu_create(&pc, s->blocks.data(), s->nblk, device, &s->ctx);
+  if (r) {
+    delete s;
+    return r;
+  }
+  s->np = 5;
+  if (c->enable_B) {
+    s->iBX = s->np;
+    s->np += 3;
+  }
+  s->iENT = s->np++;
+  if (c->enable_heating) s->iKEL = s->np++;
+  s->nc = s->np;
+  s->ncell = s->hb[0].cells;
+  s->total = s->ncell * s->nblk;
+  const int64_t T = s->total;
+  if (dalloc(s, &s->P[0], T * s->np) || dalloc(s, &s->P[1], T * s->np) ||
+      dalloc(s, &s->U[0], T * s->nc) || dalloc(s, &s->U[1], T * s->nc) ||
+      dalloc(s, &s->dudt, T * s->nc) || dalloc(s, &s->F[0], T * s->nc) ||
+      dalloc(s, &s->F[1], T * s->nc) || dalloc(s, &s->F[2], T * s->nc) ||
+      dalloc(s, &s->alfven, T * 3) || dalloc(s, &s->emf, T * 3) || dalloc(s, &s->flag, T) ||
+      dalloc(s, &s->flag_rec, T) || dalloc(s, &s->mask, T) || dalloc(s, &s->status, T) ||
+      dalloc(s, &s->iters, T) || dalloc(s, &s->dstate, 2)) {
+    fprintf(stderr, "pangu_sim_create: %s\n", s->err.c_str());
+    pangu_sim_destroy(s);
+    return PANGU_ENOMEM;
+  }
+  auto table = [&](double *base, int ncomp) {
+    std::vector<double *> t((size_t)s->nblk * ncomp);
+    for (int b = 0; b < s->nblk; ++b)
+      for (int q = 0; q < ncomp; ++q) t[(size_t)b * ncomp + q] = base + (b * (int64_t)ncomp + q) * s->ncell;
+    return t;
+  };
+  auto register_cont = [&](double *Pa, double *Ua, bool with_flux, int *h) {
+    std::vector<double *> tp = table(Pa, s->np), tu = table(Ua, s->nc);
+    std::vector<double *> tf[3];
+    pangu_container cc;
+    cc.prim = tp.data();
+    cc.cons = tu.data();
+    for (int d = 0; d < 3; ++d) {
+      tf[d] = table(s->F[d], s->nc);
+      cc.flux[d] = with_flux ? tf[d].data() : nullptr;
+    }
+    return pangu_register_container(s->ctx, &cc, h);
+  };
+  // Fluxes are consumed within the stage that computes them, so the two
+  // stage containers share one set of flux arrays.
+  r = register_cont(s->P[0], s->U[0], true, &s->h_base);
+  if (!r) r = register_cont(s->P[1], s->U[1], true, &s->h_s1);
+  if (!r) r = register_cont(s->P[1], s->dudt, false, &s->h_dudt);
+  std::vector<double *> ta = table(s->alfven, 3), te = table(s->emf, 3), tfl = table(s->flag, 1),
+                        tfr = table(s->flag_rec, 1);
+  std::vector<uint8_t *> tm(s->nblk);
+  std::vector<int32_t *> tst(s->nblk), tit(s->nblk);
+  for (int b = 0; b < s->nblk; ++b) {
+    tm[b] = s->mask + b * s->ncell;
+    tst[b] = s->status + b * s->ncell;
+    tit[b] = s->iters + b * s->ncell;
+  }
+  pangu_shared sh;
+  sh.alfven = ta.data();
+  sh.emf = te.data();
+  sh.flag = tfl.data();
+  sh.mask = tm.data();
+  sh.c2p_status = tst.data();
+  sh.c2p_iters = tit.data();
+  sh.flag_recovery = tfr.data();
+  if (!r) r = pangu_register_shared(s->ctx, &sh);
+  if (r) {
+    fprintf(stderr, "pangu_sim_create: %s\n", pangu_last_error(s->ctx));
+    pangu_sim_destroy(s);
+    return r;
+  }
+  if (hipStreamCreate(&s->stream) != hipSuccess) {
+    pangu_sim_destroy(s);
+    return PANGU_EDEVICE;
+  }
+  s->dt = c->dt_init;
+  s->time = 0.0;
+  push_dt_to_device(s);
+  *out = s;
+  return PANGU_OK;
+}
+
+int pangu_sim_destroy(pangu_sim *s) {
+  if (!s) return 0;
+  hipSetDevice(s->device);
+  if (s->stream) hipStreamSynchronize(s->stream);
+  if (s->gexec) hipGraphExecDestroy(s->gexec);
+  if (s->graph) hipGraphDestroy(s->graph);
+  void *ptrs[] = {s->P[0], s->P[1], s->U[0], s->U[1], s->dudt, s->F[0], s->F[1], s->F[2],
+                  s->alfven, s->emf, s->flag, s->flag_rec, s->mask, s->status, s->iters, s->dstate};
+  for (void *p : ptrs)
+    if (p) hipFree(p);
+  if (s->ctx) pangu_destroy(s->ctx);
+  if (s->stream) hipStreamDestroy(s->stream);
+  delete s;
+  return 0;
+}
+
+int pangu_sim_init(pangu_sim *s) {
+  ProblemParams pp;
+  pp.problem = s->cfg.problem;
+  pp.rin = s->cfg.rin;
+  pp.rmax = s->cfg.rmax;
+  pp.kappa = s->cfg.kappa;
+  pp.perturbation = s->cfg.perturbation;
+  pp.beta = s->cfg.beta;
+  pp.aphi_cut = s->cfg.aphi_cut;
+  pp.fel_0 = s->cfg.fel_0;
+  pp.seed = s->cfg.seed;
+  for (int d = 0; d < 3; ++d) pp.mesh_nx[d] = s->cfg.nx[d];
+  pangu_config pc;
+  std::memset(&pc, 0, sizeof(pc));
+  pc.metric = s->cfg.metric;
+  pc.a = s->cfg.a;
+  pc.h = s->cfg.h;
+  pc.gamma = s->cfg.gamma;
+  std::vector<double> prim;
+  if (generate_problem(pc, pp, s->hb, s->np, s->iBX, s->iENT, s->iKEL, prim))
+    return sfail(s, PANGU_EINVAL, "unknown problem");
+  SCHK(s, hipMemcpy(s->P[0], prim.data(), prim.size() * sizeof(double), hipMemcpyHostToDevice));
+  PCHK(s, pangu_exchange_ghosts(s->ctx, s->h_base, s->stream));
+  PCHK(s, pangu_apply_boundary_conditions(s->ctx, s->h_base, s->stream));
+  SCHK(s, hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+static int ensure_stage1(pangu_sim *s) {
+  if (s->stage1_ready) return 0;
+  if (s->cfg.stage_init == 1)
+    SCHK(s, hipMemcpyAsync(s->P[1], s->P[0], (size_t)(s->total * s->np) * sizeof(double),
+                           hipMemcpyDeviceToDevice, s->stream));
+  s->stage1_ready = true;
+  return 0;
+}
+
+// One RK stage at the reference's task granularity (ideal_grmhd.cc:52-116).
+static int stage_tasks(pangu_sim *s, int stage) {
+  const double beta = (stage == 1) ? 0.5 : 1.0;
+  const int c0 = (stage == 1) ? s->h_base : s->h_s1;
+  const int c1 = (stage == 1) ? s->h_s1 : s->h_base;
+  const double bdt = beta * s->dt;
+  pangu_ctx *x = s->ctx;
+  void *st = s->stream;
+  PCHK(s, pangu_fix_primitive(x, c0, st));
+  PCHK(s, pangu_calculate_conservative(x, c0, st));
+  PCHK(s, pangu_calculate_fluxes(x, c0, st));
+  if (s->cfg.enable_B) PCHK(s, pangu_constrained_transport(x, c0, st));
+  PCHK(s, pangu_flux_divergence(x, c0, s->h_dudt, st));
+  PCHK(s, pangu_update_independent(x, s->h_base, s->h_dudt, bdt, c1, st));
+  if (s->cfg.exchange_after == 0) PCHK(s, pangu_exchange_ghosts(x, c1, st));
+  PCHK(s, pangu_add_geometric_source(x, c1, bdt, st));
+  PCHK(s, pangu_recovery(x, c1, st));
+  if (s->cfg.enable_heating) PCHK(s, pangu_apply_electron_heating(x, c1, st));
+  if (s->cfg.recovery != PANGU_RECOVERY_ROBUSTER) PCHK(s, pangu_fix_recovery(x, c1, st));
+  if (s->cfg.exchange_after == 1) PCHK(s, pangu_exchange_ghosts(x, c1, st));
+  PCHK(s, pangu_apply_boundary_conditions(x, c1, st));
+  if (stage == 2) PCHK(s, pangu_estimate_timestep(x, c1, &s->dt_est, st));
+  return 0;
+}
+
+static int cycle_fused(pangu_sim *s) {
+  PCHK(s, pangu_stage(s->ctx, s->h_base, s->h_base, s->h_s1, 0.5, s->dstate, s->cfg.dt_factor, 0,
+                      s->cfg.exchange_after, s->stream));
+  PCHK(s, pangu_stage(s->ctx, s->h_base, s->h_s1, s->h_base, 1.0, s->dstate, s->cfg.dt_factor, 1,
+                      s->cfg.exchange_after, s->stream));
+  return 0;
+}
+
+int pangu_sim_step(pangu_sim *s, int n, int mode) {
+  hipSetDevice(s->device);
+  if (int r = ensure_stage1(s)) return r;
+  if (mode == PANGU_SIM_TASKS) {
+    if (s->dt_on_device) {
+      sync_dt_from_device(s);
+      s->dt_on_device = false;
+    }
+    for (int c = 0; c < n; ++c) {
+      if (int r = stage_tasks(s, 1)) return r;
+      if (int r = stage_tasks(s, 2)) return r;
+      s->time += s->dt;
+      const double grown = s->cfg.dt_factor * s->dt;
+      s->dt = (s->dt_est < grown) ? s->dt_est : grown;
+      s->ncycle++;
+    }
+    return 0;
+  }
+  if (!s->dt_on_device) {
+    push_dt_to_device(s);
+    s->dt_on_device = true;
+  }
+  if (mode == PANGU_SIM_GRAPH) {
+    if (!s->gexec) {
+      SCHK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+      const int r = cycle_fused(s);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(s->stream, &g);
+      if (r) return r;
+      if (e != hipSuccess) return sfail(s, PANGU_EDEVICE, "hipStreamEndCapture failed");
+      s->graph = g;
+      SCHK(s, hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0));
+    }
+    for (int c = 0; c < n; ++c) SCHK(s, hipGraphLaunch(s->gexec, s->stream));
+  } else {
+    for (int c = 0; c < n; ++c)
+      if (int r = cycle_fused(s)) return r;
+  }
+  s->ncycle += n;
+  return 0;
+}
+
+int pangu_sim_sync(pangu_sim *s) {
+  SCHK(s, hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int pangu_sim_timed_cycles(pangu_sim *s, int n, int mode, double *ms) {
+  hipEvent_t a, b;
+  SCHK(s, hipEventCreate(&a));
+  SCHK(s, hipEventCreate(&b));
+  SCHK(s, hipEventRecord(a, s->stream));
+  if (int r = pangu_sim_step(s, n, mode)) return r;
+  SCHK(s, hipEventRecord(b, s->stream));
+  SCHK(s, hipEventSynchronize(b));
+  float f = 0.f;
+  SCHK(s, hipEventElapsedTime(&f, a, b));
+  *ms = f;
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  return 0;
+}
+
+static const char *kPhase[] = {"fix_cons",  "fluxes",  "ct",  "div_update_source",
+                               "exchange", "recovery_heating", "fix_recovery", "bc", "timestep"};
+
+const char *pangu_sim_phase_name(int n) {
+  return (n >= 0 && n < (int)(sizeof(kPhase) / sizeof(kPhase[0]))) ? kPhase[n] : "";
+}
+
+int pangu_sim_time_phases(pangu_sim *s, int reps, double *ms_out, int maxp) {
+  hipSetDevice(s->device);
+  if (int r = ensure_stage1(s)) return r;
+  if (!s->dt_on_device) {
+    push_dt_to_device(s);
+    s->dt_on_device = true;
+  }
+  const int nph = (int)(sizeof(kPhase) / sizeof(kPhase[0]));
+  pangu_ctx *x = s->ctx;
+  void *st = s->stream;
+  const int c0 = s->h_s1, c1 = s->h_base, ci = s->h_base;
+  for (int ph = 0; ph < nph && ph < maxp; ++ph) {
+    hipEvent_t a, b;
+    SCHK(s, hipEventCreate(&a));
+    SCHK(s, hipEventCreate(&b));
+    SCHK(s, hipEventRecord(a, s->stream));
+    for (int r = 0; r < reps; ++r) {
+      switch (ph) {
+        case 0: PCHK(s, pangu_fix_primitive(x, c0, st)); PCHK(s, pangu_calculate_conservative(x, c0, st)); break;
+        case 1: PCHK(s, pangu_calculate_fluxes(x, c0, st)); break;
+        case 2: if (s->cfg.enable_B) PCHK(s, pangu_constrained_transport(x, c0, st)); break;
+        case 3: PCHK(s, pangu_flux_divergence(x, c0, s->h_dudt, st));
+                PCHK(s, pangu_update_independent(x, ci, s->h_dudt, 0.0, c1, st));
+                PCHK(s, pangu_add_geometric_source(x, c1, 0.0, st)); break;
+        case 4: PCHK(s, pangu_exchange_ghosts(x, c1, st)); break;
+        case 5: PCHK(s, pangu_recovery(x, c1, st)); if (s->cfg.enable_heating) PCHK(s, pangu_apply_electron_heating(x, c1, st)); break;
+        case 6: if (s->cfg.recovery != PANGU_RECOVERY_ROBUSTER) PCHK(s, pangu_fix_recovery(x, c1, st)); break;
+        case 7: PCHK(s, pangu_apply_boundary_conditions(x, c1, st)); break;
+        case 8: { double e; PCHK(s, pangu_estimate_timestep(x, c1, &e, st)); } break;
+      }
+    }
+    SCHK(s, hipEventRecord(b, s->stream));
+    SCHK(s, hipEventSynchronize(b));
+    float f = 0.f;
+    SCHK(s, hipEventElapsedTime(&f, a, b));
+    ms_out[ph] = (double)f / reps;
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+  }
+  return nph;
+}
+
+int pangu_sim_task(pangu_sim *s, int task, int c0i, int c1i, double bdt, double *out) {
+  hipSetDevice(s->device);
+  if (int r = ensure_stage1(s)) return r;
+  const int c0 = c0i == 0 ? s->h_base : s->h_s1;
+  const int c1 = c1i == 0 ? s->h_base : s->h_s1;
+  pangu_ctx *x = s->ctx;
+  void *st = s->stream;
+  switch (task) {
+    case T_FIX: PCHK(s, pangu_fix_primitive(x, c0, st)); break;
+    case T_CONS: PCHK(s, pangu_calculate_conservative(x, c0, st)); break;
+    case T_FLUXES: PCHK(s, pangu_calculate_fluxes(x, c0, st)); break;
+    case T_CT: PCHK(s, pangu_constrained_transport(x, c0, st)); break;
+    case T_DIV: PCHK(s, pangu_flux_divergence(x, c0, s->h_dudt, st)); break;
+    case T_UPDATE: PCHK(s, pangu_update_independent(x, s->h_base, s->h_dudt, bdt, c1, st)); break;
+    case T_SOURCE: PCHK(s, pangu_add_geometric_source(x, c1, bdt, st)); break;
+    case T_RECOVERY: PCHK(s, pangu_recovery(x, c1, st)); break;
+    case T_HEATING: PCHK(s, pangu_apply_electron_heating(x, c1, st)); break;
+    case T_FIXREC: PCHK(s, pangu_fix_recovery(x, c1, st)); break;
+    case T_EXCHANGE: PCHK(s, pangu_exchange_ghosts(x, c1, st)); break;
+    case T_BC: PCHK(s, pangu_apply_boundary_conditions(x, c1, st)); break;
+    case T_TIMESTEP: {
+      double e = 0;
+      PCHK(s, pangu_estimate_timestep(x, c1, &e, st));
+      if (out) *out = e;
+    } break;
+    default: return sfail(s, PANGU_EINVAL, "unknown task");
+  }
+  SCHK(s, hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+static void *field_ptr(pangu_sim *s, int f, int c, int64_t *bytes) {
+  const int64_t T = s->total;
+  switch (f) {
+    case F_PRIM: *bytes = T * s->np * 8; return s->P[c];
+    case F_CONS: *bytes = T * s->nc * 8; return s->U[c];
+    case F_FLUX1: *bytes = T * s->nc * 8; return s->F[0];
+    case F_FLUX2: *bytes = T * s->nc * 8; return s->F[1];
+    case F_FLUX3: *bytes = T * s->nc * 8; return s->F[2];
+    case F_ALFVEN: *bytes = T * 3 * 8; return s->alfven;
+    case F_EMF: *bytes = T * 3 * 8; return s->emf;
+    case F_FLAG: *bytes = T * 8; return s->flag;
+    case F_MASK: *bytes = T; return s->mask;
+    case F_STATUS: *bytes = T * 4; return s->status;
+    case F_ITERS: *bytes = T * 4; return s->iters;
+    case F_FLAG_REC: *bytes = T * 8; return s->flag_rec;
+    case F_DUDT: *bytes = T * s->nc * 8; return s->dudt;
+  }
+  *bytes = 0;
+  return nullptr;
+}
+
+int64_t pangu_sim_field_size(pangu_sim *s, int f) {
+  int64_t b = 0;
+  field_ptr(s, f, 0, &b);
+  return b;
+}
+
+int pangu_sim_get(pangu_sim *s, int f, int c, void *out) {
+  int64_t b = 0;
+  void *p = field_ptr(s, f, c, &b);
+  if (!p) return PANGU_EINVAL;
+  SCHK(s, hipStreamSynchronize(s->stream));
+  SCHK(s, hipMemcpy(out, p, (size_t)b, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pangu_sim_set(pangu_sim *s, int f, int c, const void *in) {
+  int64_t b = 0;
+  void *p = field_ptr(s, f, c, &b);
+  if (!p) return PANGU_EINVAL;
+  SCHK(s, hipStreamSynchronize(s->stream));
+  SCHK(s, hipMemcpy(p, in, (size_t)b, hipMemcpyHostToDevice));
+  s->stage1_ready = true;
+  return 0;
+}
+
+double pangu_sim_time(pangu_sim *s) {
+  hipStreamSynchronize(s->stream);
+  sync_dt_from_device(s);
+  return s->time;
+}
+double pangu_sim_dt(pangu_sim *s) {
+  hipStreamSynchronize(s->stream);
+  sync_dt_from_device(s);
+  return s->dt;
+}
+
+int pangu_sim_block_geom(pangu_sim *s, int b, double *xd, int *n) {
+  if (b < 0 || b >= s->nblk) return PANGU_EINVAL;
+  const HostBlock &H = s->hb[b];
+  for (int d = 0; d < 3; ++d) {
+    xd[d] = H.xg[d];
+    xd[3 + d] = H.dx[d];
+    n[d] = H.n[d];
+    n[3 + d] = H.s[d];
+    n[6 + d] = H.e[d];
+  }
+  return 0;
+}
+
+const char *pangu_sim_last_error(pangu_sim *s) { return s ? s->err.c_str() : "null"; }
+
+}  // extern "C"

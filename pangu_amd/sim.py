@@ -1,0 +1,200 @@
+"""ctypes binding of the pangu_sim host driver (include/pangu_sim.h).
+
+Device memory, kernels and streams all live in libpangu_hip.so; this module
+moves configuration in and fields out.  It raises if the library is missing
+or cannot be loaded: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import lib_path
+
+
+class SimCfg(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int * 3), ("nb", C.c_int * 3), ("ng", C.c_int),
+        ("xmin", C.c_double * 3), ("xmax", C.c_double * 3), ("bc", (C.c_int * 2) * 3),
+        ("dt_init", C.c_double), ("dt_factor", C.c_double),
+        ("metric", C.c_int), ("a", C.c_double), ("h", C.c_double),
+        ("gamma", C.c_double), ("cfl", C.c_double),
+        ("riemann", C.c_int), ("limiter", C.c_int), ("recovery", C.c_int),
+        ("enable_B", C.c_int), ("enable_heating", C.c_int),
+        ("density_floor", C.c_double), ("energy_floor", C.c_double),
+        ("density_floor_pow", C.c_double), ("energy_floor_pow", C.c_double),
+        ("sigma_max", C.c_double), ("lorentz_max", C.c_double),
+        ("heat_model", C.c_int),
+        ("fel_constant", C.c_double), ("gamma_e", C.c_double), ("gamma_p", C.c_double),
+        ("ratio_min", C.c_double), ("ratio_max", C.c_double), ("fel_0", C.c_double),
+        ("limit_kel", C.c_int), ("suppress_highb_heat", C.c_int),
+        ("enforce_positive_dissipation", C.c_int),
+        ("excise", C.c_int), ("r_excise", C.c_double), ("dexcise", C.c_double),
+        ("pexcise", C.c_double),
+        ("stage_init", C.c_int), ("exchange_after", C.c_int), ("cache_geometry", C.c_int),
+        ("problem", C.c_int),
+        ("rin", C.c_double), ("rmax", C.c_double), ("kappa", C.c_double),
+        ("perturbation", C.c_double), ("beta", C.c_double), ("aphi_cut", C.c_double),
+        ("seed", C.c_uint64), ("nthreads", C.c_int),
+    ]
+
+
+FIELDS = dict(prim=0, cons=1, flux1=2, flux2=3, flux3=4, alfven=5, emf=6, flag=7, mask=8,
+              c2p_status=9, c2p_iters=10, flag_recovery=11, dudt=12)
+TASKS = dict(fix_primitive=0, conservative=1, fluxes=2, ct=3, flux_div=4, update=5, source=6,
+             recovery=7, heating=8, fix_recovery=9, exchange=10, bc=11, timestep=12)
+MODES = dict(tasks=0, fused=1, graph=2)
+
+_lib = None
+
+
+def lib():
+    """Load libpangu_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        path = lib_path()
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: run `python -m pangu_amd.build` first")
+        L = C.CDLL(path)
+        vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
+        L.pangu_sim_create.argtypes = [C.POINTER(SimCfg), C.c_int, C.POINTER(vp)]
+        L.pangu_sim_destroy.argtypes = [vp]
+        L.pangu_sim_init.argtypes = [vp]
+        L.pangu_sim_step.argtypes = [vp, C.c_int, C.c_int]
+        L.pangu_sim_sync.argtypes = [vp]
+        L.pangu_sim_timed_cycles.argtypes = [vp, C.c_int, C.c_int, dp]
+        L.pangu_sim_time_phases.argtypes = [vp, C.c_int, dp, C.c_int]
+        L.pangu_sim_phase_name.restype = C.c_char_p
+        L.pangu_sim_phase_name.argtypes = [C.c_int]
+        L.pangu_sim_task.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_double, dp]
+        L.pangu_sim_field_size.restype = C.c_int64
+        L.pangu_sim_field_size.argtypes = [vp, C.c_int]
+        L.pangu_sim_get.argtypes = [vp, C.c_int, C.c_int, vp]
+        L.pangu_sim_set.argtypes = [vp, C.c_int, C.c_int, vp]
+        L.pangu_sim_time.restype = C.c_double
+        L.pangu_sim_time.argtypes = [vp]
+        L.pangu_sim_dt.restype = C.c_double
+        L.pangu_sim_dt.argtypes = [vp]
+        L.pangu_sim_block_geom.argtypes = [vp, C.c_int, dp, ip]
+        L.pangu_sim_last_error.restype = C.c_char_p
+        L.pangu_sim_last_error.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+def make_cfg(d):
+    c = SimCfg()
+    for name, _ in SimCfg._fields_:
+        if name not in d:
+            continue
+        v = d[name]
+        if name == "bc":
+            for i in range(3):
+                for s in range(2):
+                    c.bc[i][s] = int(v[i][s])
+        elif name in ("nx", "nb", "xmin", "xmax"):
+            arr = getattr(c, name)
+            for i in range(3):
+                arr[i] = v[i]
+        else:
+            setattr(c, name, v)
+    return c
+
+
+class HipSim:
+    """One run of the HIP path on a GPU."""
+
+    def __init__(self, cfg_dict, device=0):
+        self.cfg = cfg_dict
+        self._cfg = make_cfg(cfg_dict)
+        p = C.c_void_p()
+        r = lib().pangu_sim_create(C.byref(self._cfg), device, C.byref(p))
+        if r != 0 or not p:
+            raise RuntimeError(f"pangu_sim_create failed ({r})")
+        self._p = p
+        self.nb = list(cfg_dict["nb"])
+        self.nblocks = self.nb[0] * self.nb[1] * self.nb[2]
+        xd = np.zeros(6)
+        n = np.zeros(9, dtype=np.int32)
+        lib().pangu_sim_block_geom(self._p, 0, xd.ctypes.data_as(C.POINTER(C.c_double)),
+                                   n.ctypes.data_as(C.POINTER(C.c_int)))
+        self.n = [int(n[0]), int(n[1]), int(n[2])]
+        self.s = [int(n[3]), int(n[4]), int(n[5])]
+        self.e = [int(n[6]), int(n[7]), int(n[8])]
+
+    def close(self):
+        if getattr(self, "_p", None):
+            lib().pangu_sim_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, r, what):
+        if r != 0:
+            msg = lib().pangu_sim_last_error(self._p)
+            raise RuntimeError(f"{what} failed ({r}): {msg.decode() if msg else ''}")
+
+    def block_geom(self, b):
+        xd = np.zeros(6)
+        n = np.zeros(9, dtype=np.int32)
+        lib().pangu_sim_block_geom(self._p, b, xd.ctypes.data_as(C.POINTER(C.c_double)),
+                                   n.ctypes.data_as(C.POINTER(C.c_int)))
+        return xd, n
+
+    def init(self):
+        self._check(lib().pangu_sim_init(self._p), "init")
+
+    def step(self, n=1, mode="tasks"):
+        self._check(lib().pangu_sim_step(self._p, n, MODES[mode]), "step")
+        self._check(lib().pangu_sim_sync(self._p), "sync")
+
+    def timed_cycles(self, n, mode="graph"):
+        ms = C.c_double(0.0)
+        self._check(lib().pangu_sim_timed_cycles(self._p, n, MODES[mode], C.byref(ms)), "timed")
+        return ms.value
+
+    def time_phases(self, reps=5):
+        out = np.zeros(16)
+        n = lib().pangu_sim_time_phases(self._p, reps, out.ctypes.data_as(C.POINTER(C.c_double)), 16)
+        if n < 0:
+            self._check(n, "time_phases")
+        return {lib().pangu_sim_phase_name(i).decode(): float(out[i]) for i in range(n)}
+
+    def task(self, name, c0=0, c1=0, beta_dt=0.0):
+        out = C.c_double(0.0)
+        self._check(lib().pangu_sim_task(self._p, TASKS[name], c0, c1, beta_dt, C.byref(out)), name)
+        return out.value
+
+    @property
+    def time(self):
+        return lib().pangu_sim_time(self._p)
+
+    @property
+    def dt(self):
+        return lib().pangu_sim_dt(self._p)
+
+    def _dtype(self, field):
+        if field == "mask":
+            return np.uint8
+        if field in ("c2p_status", "c2p_iters"):
+            return np.int32
+        return np.float64
+
+    def get(self, field, container=0):
+        fid = FIELDS[field]
+        nbytes = lib().pangu_sim_field_size(self._p, fid)
+        dt = np.dtype(self._dtype(field))
+        a = np.empty(nbytes // dt.itemsize, dtype=dt)
+        self._check(lib().pangu_sim_get(self._p, fid, container, a.ctypes.data_as(C.c_void_p)), "get")
+        return a.reshape(self.nblocks, -1, self.n[2], self.n[1], self.n[0])
+
+    def set(self, field, arr, container=0):
+        fid = FIELDS[field]
+        a = np.ascontiguousarray(arr, dtype=self._dtype(field))
+        assert a.nbytes == lib().pangu_sim_field_size(self._p, fid)
+        self._check(lib().pangu_sim_set(self._p, fid, container, a.ctypes.data_as(C.c_void_p)), "set")
+
+    def interior(self, arr):
+        s, e = self.s, self.e
+        return arr[..., s[2]:e[2] + 1, s[1]:e[1] + 1, s[0]:e[0] + 1]
