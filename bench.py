@@ -1,0 +1,162 @@
+"""Benchmark: FP64 zone-cycles/s of the 3D MKS Fishbone-Moncrief torus on one MI355X.
+
+Workload: 256x128x128 zones in 32 meshblocks of 64x64x32 (nghost 4), MKS
+a=0.9375 h=0.7, PPM4 + HLL + Kastaun recovery + constrained transport +
+electron heating, RK2.  A "step" is one RK2 cycle (two stages and the dt
+update) over the whole mesh; synthetic initial data (FM torus with a seeded
+4% pressure perturbation); inputs are resident in HBM before timing starts.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode graph|fused|tasks]
+
+Prints one JSON line.  Besides the metric it reports the HBM roofline of the
+dominant kernel phase (algorithmic bytes / HIP-event time) and the CPU
+baseline: the oracle restatement of the reference algorithm timed on this
+host's cores on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FP64 zone-cycles/s, 3D MKS torus, 1 MI355X; % HBM roofline; batched/learned vs plain"
+HBM_PEAK_GBS = 8000.0
+FP64_VALU_PEAK_TFLOPS = 78.6
+
+# Algorithmic HBM bytes per launch of each phase, per interior zone (FP64
+# words x 8).  n_p = n_c = 10 for this workload.  Counted once per array
+# element a phase must read or write (stencil re-reads and geometry tables
+# amortised over k excluded):
+#   fluxes       read P (10) + write 3 x (flux 10 + alfven 1)      = 43 words
+#   recovery     read U (10) + read P (10) + write P (8: rho,u,u~,B, K_e) = 28
+#   div_update   read 3 flux (30) + U_init (10) + P (10) + write U (10) = 60
+#   fix_cons     read P (10) + write P (7) + write U (10)             = 27
+WORDS_PER_ZONE = {"fluxes": 43, "recovery_heating": 28, "div_update_source": 60, "fix_cons": 27}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--mode", default="graph", choices=["graph", "fused", "tasks"])
+    ap.add_argument("--small", action="store_true", help="reduced mesh for quick checks")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-cycles", type=int, default=1)
+    return ap.parse_args()
+
+
+def cpu_baseline(cfg_small, cycles):
+    """Oracle (C++ restatement of the reference algorithm, metric recomputed
+    per evaluation as the reference does) on a bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as O
+    O.build()
+    d = dict(cfg_small)
+    d["cache_geometry"] = 0
+    nthreads = os.cpu_count() or 1
+    try:
+        nthreads = len(os.sched_getaffinity(0))
+    except Exception:
+        pass
+    nthreads = min(nthreads, 16)
+    s = O.OracleSim(d, nthreads=nthreads)
+    s.init()
+    t0 = time.perf_counter()
+    s.step(cycles)
+    dt = time.perf_counter() - t0
+    zones = d["nx"][0] * d["nx"][1] * d["nx"][2]
+    return {
+        "value": zones * cycles / dt,
+        "unit": "zone-cycles/s",
+        "cores": nthreads,
+        "kind": "port",
+        "sample": (f"{cycles} RK2 cycle(s) of the same 3D torus physics at "
+                   f"{d['nx'][0]}x{d['nx'][1]}x{d['nx'][2]} zones in "
+                   f"{d['nb'][0] * d['nb'][1] * d['nb'][2]} blocks of 64x64x32, "
+                   f"OpenMP {nthreads} threads, metric recomputed per evaluation; {dt:.2f} s"),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is described in DESIGN.md, not run)"}))
+        sys.exit(2)
+    from pangu_amd import configs
+    from pangu_amd.sim import HipSim
+
+    if args.small:
+        cfg = configs.torus3d((128, 64, 64), (2, 1, 2))
+    else:
+        cfg = configs.torus3d()
+    zones = configs.zones(cfg)
+    t_init = time.perf_counter()
+    sim = HipSim(cfg)
+    sim.init()
+    t_init = time.perf_counter() - t_init
+    sim.step(args.warmup, args.mode)
+    ms = sim.timed_cycles(args.steps, args.mode)
+    wall0 = time.perf_counter()
+    sim.step(1, args.mode)
+    wall_one = time.perf_counter() - wall0
+    value = zones * args.steps / (ms * 1e-3)
+
+    # dominant phase: HIP-event timing of each kernel phase on the sim stream
+    phases = sim.time_phases(reps=5)
+    tot = sum(phases.values())
+    dom = max((k for k in phases if k in WORDS_PER_ZONE), key=lambda k: phases[k])
+    bytes_dom = WORDS_PER_ZONE[dom] * 8 * zones
+    achieved = bytes_dom / (phases[dom] * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "zone-cycles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (FM torus initial data, seeded perturbation)",
+        "config": {
+            "workload": "3D Fishbone-Moncrief torus, MKS a=0.9375 h=0.7",
+            "zones": [cfg["nx"][0], cfg["nx"][1], cfg["nx"][2]],
+            "meshblocks": [cfg["nb"][0], cfg["nb"][1], cfg["nb"][2]],
+            "block": [cfg["nx"][d] // cfg["nb"][d] for d in range(3)],
+            "nghost": cfg["ng"],
+            "limiter": "ppm4", "riemann": "hll", "recovery": "kastaun",
+            "ct": True, "electron_heating": True, "integrator": "rk2",
+            "mode": args.mode,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+        },
+        "phase_ms": phases,
+        "phase_ms_sum": tot,
+        "init_s": t_init,
+        "wall_one_cycle_s": wall_one,
+        "sim_time": sim.time,
+        "dt": sim.dt,
+    }
+    if not args.no_cpu_baseline:
+        small = configs.torus3d((128, 64, 64), (2, 1, 2))
+        out["cpu_baseline"] = cpu_baseline(small, args.cpu_sample_cycles)
+    sim.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

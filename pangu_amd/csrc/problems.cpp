@@ -257,6 +257,7 @@ int generate_problem(const pangu_config &cfg, const ProblemParams &pp,
           const double ex = avg - pp.aphi_cut;
           A[B.id(k, j, i)] = (ex > 0.0) ? ex : 0.0;
         }
+#pragma omp parallel for schedule(dynamic, 1)
     for (int k = B.s[2]; k <= B.e[2]; ++k)
       for (int j = B.s[1]; j <= B.e[1]; ++j)
         for (int i = B.s[0]; i <= B.e[0]; ++i) {
@@ -275,6 +276,9 @@ int generate_problem(const pangu_config &cfg, const ProblemParams &pp,
   for (size_t b = 0; b < blocks.size(); ++b) {
     const HostBlock &B = blocks[b];
     double bm = 0.0;
+    // max of non-negative values: order-independent, so a parallel reduction
+    // gives the serial result
+#pragma omp parallel for schedule(dynamic, 1) reduction(max : bm)
     for (int k = B.s[2]; k <= B.e[2]; ++k)
       for (int j = B.s[1]; j <= B.e[1]; ++j)
         for (int i = B.s[0]; i <= B.e[0]; ++i) {
