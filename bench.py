@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--small", action="store_true", help="reduced mesh for quick checks")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-cycles", type=int, default=1)
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="also run a plain copy kernel of this many doubles (counter calibration)")
     return ap.parse_args()
 
 
@@ -151,6 +153,10 @@ def main():
         "sim_time": sim.time,
         "dt": sim.dt,
     }
+    if args.copy_calibration:
+        n = args.copy_calibration
+        cms = sim.copy_calibration(n, reps=5)
+        out["copy_calibration"] = {"doubles": n, "ms": cms, "GBps": 16.0 * n / (cms * 1e-3) / 1e9}
     if not args.no_cpu_baseline:
         small = configs.torus3d((128, 64, 64), (2, 1, 2))
         out["cpu_baseline"] = cpu_baseline(small, args.cpu_sample_cycles)

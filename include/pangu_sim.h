@@ -77,6 +77,9 @@ double pangu_sim_time(pangu_sim *s);
 double pangu_sim_dt(pangu_sim *s);
 int pangu_sim_block_geom(pangu_sim *s, int b, double *xg_dx_out, int *n_out);
 const char *pangu_sim_last_error(pangu_sim *s);
+/* Times a plain copy kernel of n doubles (mean ms over reps); used to
+ * calibrate hardware byte counters against a kernel of known traffic. */
+int pangu_sim_copy_calibration(pangu_sim *s, int64_t n, int reps, double *ms);
 
 #ifdef __cplusplus
 }

@@ -43,7 +43,7 @@ struct pangu_ctx {
   int32_t **dstatus = nullptr, **diters = nullptr;
   bool shared_ok = false;
   // scratch
-  unsigned long long *dred = nullptr;
+  double *dred = nullptr;
   int *dcluster = nullptr;
   double *dest = nullptr;
   LaunchCtx L;
@@ -241,9 +241,9 @@ int pangu_create(const pangu_config *cfg, const pangu_block *blocks, int nblocks
   for (int b = 0; b < nblocks; ++b) c->hb.push_back(make_host_block(blocks[b]));
   int r = build_geometry(c, blocks);
   if (!r) r = build_neighbours(c);
-  if (!r) r = upload(c, &c->dred, std::vector<unsigned long long>((size_t)nblocks * 3, 0ull).data(),
-                     (size_t)nblocks * 3);
   const int nplanes = c->hb[0].nx[2];
+  if (!r) r = upload(c, &c->dred, std::vector<double>((size_t)nblocks * nplanes * 3, 0.0).data(),
+                     (size_t)nblocks * nplanes * 3);
   if (!r) r = upload(c, &c->dcluster, std::vector<int>((size_t)nblocks * nplanes, 0).data(),
                      (size_t)nblocks * nplanes);
   if (!r) r = upload(c, &c->dest, std::vector<double>(1, 0.0).data(), 1);

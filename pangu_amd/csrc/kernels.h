@@ -63,7 +63,8 @@ int launch_fix_recovery(const LaunchCtx &L, const KFields &F, double *const *fla
 int launch_exchange(const LaunchCtx &L, const KFields &F, double *const *flag, const int *nbr,
                     hipStream_t s);
 int launch_bc(const LaunchCtx &L, const KFields &F, double *const *flag, hipStream_t s);
-int launch_dt(const LaunchCtx &L, double *const *alfven, unsigned long long *red, double *state,
+// red: nblocks * interior planes * 3 doubles of scratch
+int launch_dt(const LaunchCtx &L, double *const *alfven, double *red, double *state,
               double dt_factor, double *est_out, hipStream_t s);
 
 }  // namespace pangu

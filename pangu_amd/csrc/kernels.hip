@@ -1,10 +1,14 @@
 // gfx950 kernels of the GRMHD per-RK-stage update path (FP64 vector ALU).
 //
-// Thread mapping: every kernel maps threadIdx.x to the contiguous index i
-// (64 lanes = one wavefront row, coalesced 512 B per load), threadIdx.y to
-// j and blockIdx.z to (block, k).  All three flux sweeps therefore load
-// coalesced rows; the reference's j- and k-innermost sweeps (laxf.cc:189,295)
-// would stride through memory on a GPU.
+// Thread mapping: a launch is a 1D grid of 256-thread tiles.  The (i, j)
+// range of each (block, k) plane is flattened row-major, so consecutive lanes
+// take consecutive i (coalesced loads) and rows of any width (65, 66, 72 ...)
+// fill whole wavefronts.  All three flux sweeps therefore load coalesced
+// rows; the reference's j- and k-innermost sweeps (laxf.cc:189,295) would
+// stride through memory on a GPU.  Tiles are remapped so that consecutive
+// logical tiles run on the same XCD: the hardware deals workgroups to the 8
+// XCDs round-robin, and neighbouring tiles share stencil rows and planes
+// through that XCD's L2.
 #include <hip/hip_runtime.h>
 #include <float.h>
 
@@ -38,6 +42,29 @@ __device__ __forceinline__ int64_t cen_index(const DevGeo &G, int k, int j, int 
   return ((int64_t)(G.has3d ? k : 0) * G.nj + j) * G.ni + i;
 }
 
+constexpr int TILE = 256;
+
+// Decodes (block, k, j, i) of this thread; false for padding threads.
+__device__ __forceinline__ bool decode_tile(const Range3 &R, int nblocks, int &b, int &k, int &j,
+                                            int &i) {
+  const int plane = R.cnt[0] * R.cnt[1];
+  const int tpp = (plane + TILE - 1) / TILE;
+  const int N = nblocks * R.cnt[2] * tpp;
+  const int n8 = (N + 7) >> 3;
+  const int L = (int)blockIdx.x;
+  const int T = (L & 7) * n8 + (L >> 3);
+  if (T >= N) return false;
+  const int p = T / tpp;
+  const int t = (T - p * tpp) * TILE + (int)threadIdx.x;
+  if (t >= plane) return false;
+  b = p / R.cnt[2];
+  k = R.lo[2] + (p - b * R.cnt[2]);
+  const int jj = t / R.cnt[0];
+  j = R.lo[1] + jj;
+  i = R.lo[0] + (t - jj * R.cnt[0]);
+  return true;
+}
+
 // Loads of the centre table.
 struct CenLoad {
   double g[16], gc0[4], alpha, sg;
@@ -57,13 +84,10 @@ __device__ __forceinline__ void load_cen_state(const DevGeo &G, int64_t t, CenLo
 // FixPrimitive over the entire block (primitive_fixer.cc:64-166) and, for
 // interior cells, CalculateConservative (conservative.cc:57-107).
 template <bool FIX, bool CONS, bool HB, bool HK>
-__global__ __launch_bounds__(256) void k_fix_cons(KCtx C, KFields F, uint8_t *const *mask) {
-  const int b = blockIdx.z / C.nk;
-  const int k = blockIdx.z % C.nk;
+__global__ __launch_bounds__(256) void k_fix_cons(KCtx C, KFields F, uint8_t *const *mask, Range3 R) {
+  int b, k, j, i;
+  if (!decode_tile(R, C.nblocks, b, k, j, i)) return;
   const DevBlock &B = C.blk[b];
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  const int j = blockIdx.y * 4 + threadIdx.y;
-  if (i >= B.n[0] || j >= B.n[1] || k >= B.n[2]) return;
   const bool interior = i >= B.s[0] && i <= B.e[0] && j >= B.s[1] && j <= B.e[1] &&
                         k >= B.s[2] && k <= B.e[2];
   if (!FIX && !interior) return;
@@ -159,11 +183,8 @@ __global__ __launch_bounds__(256) void k_fix_cons(KCtx C, KFields F, uint8_t *co
 // lower face of cell (k,j,i) along DIR.
 template <int DIR, int LIM, int RS, bool HB, bool HK>
 __global__ __launch_bounds__(256) void k_flux(KCtx C, KFields F, double *const *alfven, Range3 R) {
-  const int i = R.lo[0] + blockIdx.x * 64 + threadIdx.x;
-  const int j = R.lo[1] + blockIdx.y * 4 + threadIdx.y;
-  const int b = blockIdx.z / R.cnt[2];
-  const int k = R.lo[2] + blockIdx.z % R.cnt[2];
-  if (i > R.hi[0] || j > R.hi[1]) return;
+  int b, k, j, i;
+  if (!decode_tile(R, C.nblocks, b, k, j, i)) return;
   const DevBlock &B = C.blk[b];
   const int64_t sj = B.n[0], sk = (int64_t)B.n[0] * B.n[1];
   const int64_t st = (DIR == 1) ? 1 : (DIR == 2 ? sj : sk);
@@ -260,11 +281,8 @@ __global__ __launch_bounds__(256) void k_flux(KCtx C, KFields F, double *const *
 // Corner EMFs (constrained_transport.cc:49-76).
 __global__ __launch_bounds__(256) void k_emf(KCtx C, KFields F, double *const *emf, Range3 R,
                                              int e1on, int e2on, int e3on) {
-  const int i = R.lo[0] + blockIdx.x * 64 + threadIdx.x;
-  const int j = R.lo[1] + blockIdx.y * 4 + threadIdx.y;
-  const int b = blockIdx.z / R.cnt[2];
-  const int k = R.lo[2] + blockIdx.z % R.cnt[2];
-  if (i > R.hi[0] || j > R.hi[1]) return;
+  int b, k, j, i;
+  if (!decode_tile(R, C.nblocks, b, k, j, i)) return;
   const DevBlock &B = C.blk[b];
   const int64_t sj = B.n[0], sk = (int64_t)B.n[0] * B.n[1];
   const int64_t id = k * sk + j * sj + i;
@@ -292,11 +310,8 @@ __global__ __launch_bounds__(256) void k_emf(KCtx C, KFields F, double *const *e
 // One launch covers the union of the three face ranges.
 __global__ __launch_bounds__(256) void k_ct_faces(KCtx C, KFields F, double *const *emf, Range3 R,
                                                   int o1, int o2, int o3) {
-  const int i = R.lo[0] + blockIdx.x * 64 + threadIdx.x;
-  const int j = R.lo[1] + blockIdx.y * 4 + threadIdx.y;
-  const int b = blockIdx.z / R.cnt[2];
-  const int k = R.lo[2] + blockIdx.z % R.cnt[2];
-  if (i > R.hi[0] || j > R.hi[1]) return;
+  int b, k, j, i;
+  if (!decode_tile(R, C.nblocks, b, k, j, i)) return;
   const DevBlock &B = C.blk[b];
   const int64_t sj = B.n[0], sk = (int64_t)B.n[0] * B.n[1];
   const int64_t id = k * sk + j * sj + i;
@@ -331,13 +346,10 @@ __global__ __launch_bounds__(256) void k_ct_faces(KCtx C, KFields F, double *con
 template <int MODE, bool SRC, bool HB, bool HK>
 __global__ __launch_bounds__(256) void k_div_update(KCtx C, KFields F0, KFields Finit, KFields F1,
                                                     double *const *dudt, const double *dt_dev,
-                                                    double beta, double bdt_host) {
-  const int b = blockIdx.z / C.nk;
+                                                    double beta, double bdt_host, Range3 R) {
+  int b, k, j, i;
+  if (!decode_tile(R, C.nblocks, b, k, j, i)) return;
   const DevBlock &B = C.blk[b];
-  const int k = B.s[2] + blockIdx.z % C.nk;
-  const int i = B.s[0] + blockIdx.x * 64 + threadIdx.x;
-  const int j = B.s[1] + blockIdx.y * 4 + threadIdx.y;
-  if (i > B.e[0] || j > B.e[1] || k > B.e[2]) return;
   const int64_t sj = B.n[0], sk = (int64_t)B.n[0] * B.n[1];
   const int64_t id = k * sk + j * sj + i;
   constexpr int NC = ncomp<HB, HK>();
@@ -418,13 +430,10 @@ __global__ __launch_bounds__(256) void k_div_update(KCtx C, KFields F0, KFields 
 // (electron_heating.cc:66-109) on the recovered cell.  REC: 0 chainer,
 // 1 Kastaun, 2 1Dvsq alone, -1 none (heating only).
 template <int REC, bool HB, bool HK, bool HEAT>
-__global__ __launch_bounds__(256) void k_recover(KCtx C, KFields F, SharedTabs S) {
-  const int b = blockIdx.z / C.nk;
+__global__ __launch_bounds__(256) void k_recover(KCtx C, KFields F, SharedTabs S, Range3 R) {
+  int b, k, j, i;
+  if (!decode_tile(R, C.nblocks, b, k, j, i)) return;
   const DevBlock &B = C.blk[b];
-  const int k = B.s[2] + blockIdx.z % C.nk;
-  const int i = B.s[0] + blockIdx.x * 64 + threadIdx.x;
-  const int j = B.s[1] + blockIdx.y * 4 + threadIdx.y;
-  if (i > B.e[0] || j > B.e[1] || k > B.e[2]) return;
   const int64_t id = ((int64_t)k * B.n[1] + j) * B.n[0] + i;
   constexpr int NC = ncomp<HB, HK>();
   const DevGeo &G = C.geo[B.geo];
@@ -573,14 +582,11 @@ __device__ __forceinline__ void fix_cell(double *const *pp, int np, int64_t id, 
 }
 
 __global__ __launch_bounds__(256) void k_fixrec_a(KCtx C, KFields F, double *const *flag,
-                                                  int *cluster) {
-  const int b = blockIdx.z / C.nk;
+                                                  int *cluster, Range3 R) {
+  int b, k, j, i;
+  if (!decode_tile(R, C.nblocks, b, k, j, i)) return;
   const DevBlock &B = C.blk[b];
-  const int kk = blockIdx.z % C.nk;
-  const int k = B.s[2] + kk;
-  const int i = B.s[0] + blockIdx.x * 64 + threadIdx.x;
-  const int j = B.s[1] + blockIdx.y * 4 + threadIdx.y;
-  if (i > B.e[0] || j > B.e[1] || k > B.e[2]) return;
+  const int kk = k - B.s[2];
   const int64_t id = ((int64_t)k * B.n[1] + j) * B.n[0] + i;
   double *fl = flag[b];
   if (fl[id] != 0) return;
@@ -640,13 +646,10 @@ __global__ __launch_bounds__(64) void k_fixrec_b(KCtx C, KFields F, double *cons
 // Same-level exchange: a ghost cell of block b takes the interior value of
 // the neighbour in its offset direction (periodic wrap encoded in nbr).
 __global__ __launch_bounds__(256) void k_exchange(KCtx C, KFields F, double *const *flag,
-                                                  const int *nbr) {
-  const int b = blockIdx.z / C.nkg;
-  const int k = blockIdx.z % C.nkg;
+                                                  const int *nbr, Range3 R) {
+  int b, k, j, i;
+  if (!decode_tile(R, C.nblocks, b, k, j, i)) return;
   const DevBlock &B = C.blk[b];
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  const int j = blockIdx.y * 4 + threadIdx.y;
-  if (i >= B.n[0] || j >= B.n[1] || k >= B.n[2]) return;
   const int o0 = (i < B.s[0]) ? -1 : (i > B.e[0] ? 1 : 0);
   const int o1 = (j < B.s[1]) ? -1 : (j > B.e[1] ? 1 : 0);
   const int o2 = (k < B.s[2]) ? -1 : (k > B.e[2] ? 1 : 0);
@@ -667,13 +670,10 @@ __global__ __launch_bounds__(256) void k_exchange(KCtx C, KFields F, double *con
 // reflecting mirrors and flips the d-component of u~ and B.  Covers the full
 // extent of the other dims, so X1, X2, X3 launches in turn fill corners.
 __global__ __launch_bounds__(256) void k_bc(KCtx C, KFields F, double *const *flag, int d, int bc_in,
-                                            int bc_out, int nbd) {
-  const int b = blockIdx.z / C.nkg;
-  const int k = blockIdx.z % C.nkg;
+                                            int bc_out, int nbd, Range3 R) {
+  int b, k, j, i;
+  if (!decode_tile(R, C.nblocks, b, k, j, i)) return;
   const DevBlock &B = C.blk[b];
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  const int j = blockIdx.y * 4 + threadIdx.y;
-  if (i >= B.n[0] || j >= B.n[1] || k >= B.n[2]) return;
   int ix[3] = {i, j, k};
   int side, kind;
   if (ix[d] < B.s[d] && B.lx[d] == 0 && bc_in != PANGU_BC_PERIODIC) {
@@ -705,49 +705,59 @@ __global__ __launch_bounds__(256) void k_bc(KCtx C, KFields F, double *const *fl
 // Per block and active direction the maximum face speed over interior
 // cells; min_cells(cfl dx / c) equals cfl dx / max_cells(c) because IEEE
 // division is monotone.  NaN speeds are skipped, as the reference's
-// `if (dt_min > dt)` reduction skips NaN.  Non-negative doubles order like
-// their bit patterns, so a 64-bit atomicMax suffices.
-__global__ void k_dt_reset(unsigned long long *red, int n) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) red[t] = 0ull;
-}
-
-__global__ __launch_bounds__(256) void k_dt_reduce(KCtx C, double *const *alfven,
-                                                   unsigned long long *red) {
-  const int b = blockIdx.z / C.nk;
+// `if (dt_min > dt)` reduction skips NaN.  One workgroup per (block, k)
+// plane writes its maxima to red[(b*nk + k)*3 + d]; no atomics, so the
+// result is deterministic.
+__global__ __launch_bounds__(256) void k_dt_reduce(KCtx C, double *const *alfven, double *red) {
+  __shared__ double sm[4][3];
+  const int b = blockIdx.x / C.nk;
+  const int kk = blockIdx.x % C.nk;
   const DevBlock &B = C.blk[b];
-  const int k = B.s[2] + blockIdx.z % C.nk;
-  const int i = B.s[0] + blockIdx.x * 64 + threadIdx.x;
-  const int j = B.s[1] + blockIdx.y * 4 + threadIdx.y;
-  const bool in = (i <= B.e[0] && j <= B.e[1] && k <= B.e[2]);
-  const int64_t id = ((int64_t)k * B.n[1] + j) * B.n[0] + i;
-  for (int d = 0; d < C.ndim; ++d) {
-    double c = 0.0;
-    if (in) {
+  const int k = B.s[2] + kk;
+  const int ni = B.e[0] - B.s[0] + 1, nj = B.e[1] - B.s[1] + 1;
+  double c[3] = {0.0, 0.0, 0.0};
+  for (int t = threadIdx.x; t < ni * nj; t += blockDim.x) {
+    const int j = B.s[1] + t / ni, i = B.s[0] + t % ni;
+    const int64_t id = ((int64_t)k * B.n[1] + j) * B.n[0] + i;
+    for (int d = 0; d < C.ndim; ++d) {
       const double v = alfven[b * 3 + d][id];
-      if (v >= 0.0) c = v;
+      if (v > c[d]) c[d] = v;
     }
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int d = 0; d < 3; ++d) {
+    double v = c[d];
     for (int off = 32; off > 0; off >>= 1) {
-      const double o = __shfl_xor(c, off, 64);
-      c = (o > c) ? o : c;
+      const double o = __shfl_xor(v, off, 64);
+      v = (o > v) ? o : v;
     }
-    if ((threadIdx.x & 63) == 0 && c > 0.0)
-      atomicMax(&red[b * 3 + d], (unsigned long long)__double_as_longlong(c));
+    if (lane == 0) sm[w][d] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int d = threadIdx.x;
+    double v = sm[0][d];
+    for (int q = 1; q < 4; ++q) v = (sm[q][d] > v) ? sm[q][d] : v;
+    red[((int64_t)b * C.nk + kk) * 3 + d] = v;
   }
 }
 
 // Harmonic combination per block (timestep_estimation.cc:94-98), minimum
 // over blocks, then dt control dt_next = min(dt_factor * dt, estimate) and
 // time += dt.  state = [dt, time]; est_out (optional) gets the estimate.
-__global__ __launch_bounds__(256) void k_dt_final(KCtx C, const unsigned long long *red,
-                                                  double *state, double dt_factor, double *est_out) {
+__global__ __launch_bounds__(256) void k_dt_final(KCtx C, const double *red, double *state,
+                                                  double dt_factor, double *est_out) {
   __shared__ double sm[256];
   double mn = DBL_MAX;
   for (int b = threadIdx.x; b < C.nblocks; b += blockDim.x) {
     const DevBlock &B = C.blk[b];
     double m[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
     for (int d = 0; d < C.ndim; ++d) {
-      const double cmax = __longlong_as_double((long long)red[b * 3 + d]);
+      double cmax = 0.0;
+      for (int kk = 0; kk < C.nk; ++kk) {
+        const double v = red[((int64_t)b * C.nk + kk) * 3 + d];
+        cmax = (v > cmax) ? v : cmax;
+      }
       const double q = C.ph.cfl * B.dx[d] / cmax;
       m[d] = (q < DBL_MAX) ? q : DBL_MAX;
     }
@@ -776,21 +786,36 @@ __global__ __launch_bounds__(256) void k_dt_final(KCtx C, const unsigned long lo
 }
 
 // ===================================================================== host
-static inline dim3 grid_for(int ni, int nj, int nz) {
-  return dim3((ni + 63) / 64, (nj + 3) / 4, nz);
+static Range3 make_range(int l0, int h0, int l1, int h1, int l2, int h2) {
+  Range3 R;
+  R.lo[0] = l0; R.hi[0] = h0;
+  R.lo[1] = l1; R.hi[1] = h1;
+  R.lo[2] = l2; R.hi[2] = h2;
+  for (int d = 0; d < 3; ++d) R.cnt[d] = R.hi[d] - R.lo[d] + 1;
+  return R;
 }
-static const dim3 kBlock(64, 4, 1);
+static Range3 interior_range(const DevBlock &B) {
+  return make_range(B.s[0], B.e[0], B.s[1], B.e[1], B.s[2], B.e[2]);
+}
+static Range3 entire_range(const DevBlock &B) {
+  return make_range(0, B.n[0] - 1, 0, B.n[1] - 1, 0, B.n[2] - 1);
+}
+// 1D grid of 256-thread tiles, padded to a multiple of the 8 XCDs.
+static dim3 tile_grid(const Range3 &R, int nblocks) {
+  const long plane = (long)R.cnt[0] * R.cnt[1];
+  const long N = (long)nblocks * R.cnt[2] * ((plane + TILE - 1) / TILE);
+  return dim3((unsigned)(((N + 7) / 8) * 8));
+}
+static const dim3 kTile(TILE, 1, 1);
 
 #define PG_CHECK_LAUNCH() (hipGetLastError() == hipSuccess ? 0 : -2)
 
 int launch_fix_cons(const LaunchCtx &L, const KFields &F, uint8_t *const *mask, bool fix, bool cons,
                     hipStream_t s) {
-  const DevBlock &B = L.hblk;
-  dim3 g = grid_for(B.n[0], B.n[1], L.k.nblocks * B.n[2]);
-  KCtx kc = L.k;
-  kc.nk = B.n[2];
+  const Range3 R = fix ? entire_range(L.hblk) : interior_range(L.hblk);
+  const dim3 g = tile_grid(R, L.k.nblocks);
   const bool hb = L.k.iBX >= 0, hk = L.k.iKEL >= 0;
-#define FC(FX, CS, HB_, HK_) hipLaunchKernelGGL((k_fix_cons<FX, CS, HB_, HK_>), g, kBlock, 0, s, kc, F, mask)
+#define FC(FX, CS, HB_, HK_) hipLaunchKernelGGL((k_fix_cons<FX, CS, HB_, HK_>), g, kTile, 0, s, L.k, F, mask, R)
 #define FC2(FX, CS)                      \
   if (hb && hk) FC(FX, CS, true, true);  \
   else if (hb) FC(FX, CS, true, false);  \
@@ -811,11 +836,11 @@ int launch_fix_cons(const LaunchCtx &L, const KFields &F, uint8_t *const *mask, 
 template <int DIR>
 static void flux_dir(const LaunchCtx &L, const KFields &F, double *const *alfven, Range3 R,
                      hipStream_t s) {
-  dim3 g = grid_for(R.cnt[0], R.cnt[1], L.k.nblocks * R.cnt[2]);
+  const dim3 g = tile_grid(R, L.k.nblocks);
   const bool hb = L.k.iBX >= 0, hk = L.k.iKEL >= 0;
   const int lim = L.limiter, rs = L.riemann;
 #define FL(LIM, RS, HB_, HK_) \
-  hipLaunchKernelGGL((k_flux<DIR, LIM, RS, HB_, HK_>), g, kBlock, 0, s, L.k, F, alfven, R)
+  hipLaunchKernelGGL((k_flux<DIR, LIM, RS, HB_, HK_>), g, kTile, 0, s, L.k, F, alfven, R)
 #define FL2(LIM, RS)                    \
   if (hb && hk) FL(LIM, RS, true, true); \
   else if (hb) FL(LIM, RS, true, false); \
@@ -834,61 +859,56 @@ static void flux_dir(const LaunchCtx &L, const KFields &F, double *const *alfven
 #undef FL
 }
 
-int launch_fluxes(const LaunchCtx &L, const KFields &F, double *const *alfven, hipStream_t s) {
+// Face ranges of the three sweeps (laxf.cc:78-83,184-189,290-295): the
+// normal index runs over s..e+1 and transverse indices include one ghost
+// layer on active dims, except HLL's X3 sweep, which stays on the interior
+// transversely (hll.cc:295-296).
+static Range3 face_range(const LaunchCtx &L, int dir) {
   const DevBlock &B = L.hblk;
   const int o[3] = {B.n[0] > 1, B.n[1] > 1, B.n[2] > 1};
-  for (int dir = 1; dir <= L.k.ndim; ++dir) {
-    Range3 R;
-    for (int d = 0; d < 3; ++d) {
-      R.lo[d] = B.s[d] - o[d];
-      R.hi[d] = B.e[d] + o[d];
-    }
-    R.lo[dir - 1] = B.s[dir - 1];
-    R.hi[dir - 1] = B.e[dir - 1] + 1;
-    if (dir == 3 && L.riemann == PANGU_RIEMANN_HLL) {  // hll.cc:295-296
-      R.lo[0] = B.s[0];
-      R.hi[0] = B.e[0];
-      R.lo[1] = B.s[1];
-      R.hi[1] = B.e[1];
-    }
-    for (int d = 0; d < 3; ++d) R.cnt[d] = R.hi[d] - R.lo[d] + 1;
-    if (dir == 1) flux_dir<1>(L, F, alfven, R, s);
-    if (dir == 2) flux_dir<2>(L, F, alfven, R, s);
-    if (dir == 3) flux_dir<3>(L, F, alfven, R, s);
+  int lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = B.s[d] - o[d];
+    hi[d] = B.e[d] + o[d];
   }
+  lo[dir - 1] = B.s[dir - 1];
+  hi[dir - 1] = B.e[dir - 1] + 1;
+  if (dir == 3 && L.riemann == PANGU_RIEMANN_HLL) {
+    lo[0] = B.s[0];
+    hi[0] = B.e[0];
+    lo[1] = B.s[1];
+    hi[1] = B.e[1];
+  }
+  return make_range(lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]);
+}
+
+int launch_fluxes(const LaunchCtx &L, const KFields &F, double *const *alfven, hipStream_t s) {
+  if (L.k.ndim >= 1) flux_dir<1>(L, F, alfven, face_range(L, 1), s);
+  if (L.k.ndim >= 2) flux_dir<2>(L, F, alfven, face_range(L, 2), s);
+  if (L.k.ndim >= 3) flux_dir<3>(L, F, alfven, face_range(L, 3), s);
   return PG_CHECK_LAUNCH();
 }
 
 int launch_ct(const LaunchCtx &L, const KFields &F, double *const *emf, hipStream_t s) {
   const DevBlock &B = L.hblk;
   const int o1 = B.n[0] > 1, o2 = B.n[1] > 1, o3 = B.n[2] > 1;
-  Range3 R;
-  R.lo[0] = B.s[0];
-  R.hi[0] = B.e[0] + o1;
-  R.lo[1] = B.s[1];
-  R.hi[1] = B.e[1] + o2;
-  R.lo[2] = B.s[2];
-  R.hi[2] = B.e[2] + o3;
-  for (int d = 0; d < 3; ++d) R.cnt[d] = R.hi[d] - R.lo[d] + 1;
-  dim3 g = grid_for(R.cnt[0], R.cnt[1], L.k.nblocks * R.cnt[2]);
-  hipLaunchKernelGGL(k_emf, g, kBlock, 0, s, L.k, F, emf, R, o2 && o3, o3 && o1, o1 && o2);
-  hipLaunchKernelGGL(k_ct_faces, g, kBlock, 0, s, L.k, F, emf, R, o1, o2, o3);
+  const Range3 R = make_range(B.s[0], B.e[0] + o1, B.s[1], B.e[1] + o2, B.s[2], B.e[2] + o3);
+  const dim3 g = tile_grid(R, L.k.nblocks);
+  hipLaunchKernelGGL(k_emf, g, kTile, 0, s, L.k, F, emf, R, o2 && o3, o3 && o1, o1 && o2);
+  hipLaunchKernelGGL(k_ct_faces, g, kTile, 0, s, L.k, F, emf, R, o1, o2, o3);
   return PG_CHECK_LAUNCH();
 }
 
 int launch_div_update(const LaunchCtx &L, int mode, const KFields &F0, const KFields &Finit,
                       const KFields &F1, double *const *dudt, const double *dt_dev, double beta,
                       double bdt_host, hipStream_t s) {
-  const DevBlock &B = L.hblk;
-  const int nx0 = B.e[0] - B.s[0] + 1, nx1 = B.e[1] - B.s[1] + 1, nx2 = B.e[2] - B.s[2] + 1;
-  dim3 g = grid_for(nx0, nx1, L.k.nblocks * nx2);
-  KCtx kc = L.k;
-  kc.nk = nx2;
+  const Range3 R = interior_range(L.hblk);
+  const dim3 g = tile_grid(R, L.k.nblocks);
   const bool hb = L.k.iBX >= 0, hk = L.k.iKEL >= 0;
   const bool src = L.metric != PANGU_METRIC_MINKOWSKI;
 #define DU(M, SR, HB_, HK_)                                                                     \
-  hipLaunchKernelGGL((k_div_update<M, SR, HB_, HK_>), g, kBlock, 0, s, kc, F0, Finit, F1, dudt, \
-                     dt_dev, beta, bdt_host)
+  hipLaunchKernelGGL((k_div_update<M, SR, HB_, HK_>), g, kTile, 0, s, L.k, F0, Finit, F1, dudt, \
+                     dt_dev, beta, bdt_host, R)
 #define DU2(M, SR)                     \
   if (hb && hk) DU(M, SR, true, true); \
   else if (hb) DU(M, SR, true, false); \
@@ -916,14 +936,11 @@ int launch_div_update(const LaunchCtx &L, int mode, const KFields &F0, const KFi
 
 int launch_recover(const LaunchCtx &L, const KFields &F, const SharedTabs &S, int rec, bool heat,
                    hipStream_t s) {
-  const DevBlock &B = L.hblk;
-  const int nx0 = B.e[0] - B.s[0] + 1, nx1 = B.e[1] - B.s[1] + 1, nx2 = B.e[2] - B.s[2] + 1;
-  dim3 g = grid_for(nx0, nx1, L.k.nblocks * nx2);
-  KCtx kc = L.k;
-  kc.nk = nx2;
+  const Range3 R = interior_range(L.hblk);
+  const dim3 g = tile_grid(R, L.k.nblocks);
   const bool hb = L.k.iBX >= 0, hk = L.k.iKEL >= 0;
 #define RC(R_, HB_, HK_, HE_) \
-  hipLaunchKernelGGL((k_recover<R_, HB_, HK_, HE_>), g, kBlock, 0, s, kc, F, S)
+  hipLaunchKernelGGL((k_recover<R_, HB_, HK_, HE_>), g, kTile, 0, s, L.k, F, S, R)
 #define RC2(R_, HE_)                     \
   if (hb && hk) RC(R_, true, true, HE_);  \
   else if (hb) RC(R_, true, false, HE_);  \
@@ -947,47 +964,37 @@ int launch_recover(const LaunchCtx &L, const KFields &F, const SharedTabs &S, in
 int launch_fix_recovery(const LaunchCtx &L, const KFields &F, double *const *flag, int *cluster,
                         hipStream_t s) {
   const DevBlock &B = L.hblk;
-  const int nx0 = B.e[0] - B.s[0] + 1, nx1 = B.e[1] - B.s[1] + 1, nx2 = B.e[2] - B.s[2] + 1;
+  const Range3 R = interior_range(B);
   KCtx kc = L.k;
-  kc.nk = nx2;
-  dim3 g = grid_for(nx0, nx1, L.k.nblocks * nx2);
-  hipLaunchKernelGGL(k_fixrec_a, g, kBlock, 0, s, kc, F, flag, cluster);
-  hipLaunchKernelGGL(k_fixrec_b, dim3(L.k.nblocks * nx2), dim3(64), 0, s, kc, F, flag, cluster);
+  kc.nk = R.cnt[2];
+  hipLaunchKernelGGL(k_fixrec_a, tile_grid(R, L.k.nblocks), kTile, 0, s, kc, F, flag, cluster, R);
+  hipLaunchKernelGGL(k_fixrec_b, dim3(L.k.nblocks * R.cnt[2]), dim3(64), 0, s, kc, F, flag, cluster);
   return PG_CHECK_LAUNCH();
 }
 
 int launch_exchange(const LaunchCtx &L, const KFields &F, double *const *flag, const int *nbr,
                     hipStream_t s) {
-  const DevBlock &B = L.hblk;
-  KCtx kc = L.k;
-  kc.nkg = B.n[2];
-  dim3 g = grid_for(B.n[0], B.n[1], L.k.nblocks * B.n[2]);
-  hipLaunchKernelGGL(k_exchange, g, kBlock, 0, s, kc, F, flag, nbr);
+  const Range3 R = entire_range(L.hblk);
+  hipLaunchKernelGGL(k_exchange, tile_grid(R, L.k.nblocks), kTile, 0, s, L.k, F, flag, nbr, R);
   return PG_CHECK_LAUNCH();
 }
 
 int launch_bc(const LaunchCtx &L, const KFields &F, double *const *flag, hipStream_t s) {
-  const DevBlock &B = L.hblk;
-  KCtx kc = L.k;
-  kc.nkg = B.n[2];
-  dim3 g = grid_for(B.n[0], B.n[1], L.k.nblocks * B.n[2]);
+  const Range3 R = entire_range(L.hblk);
+  const dim3 g = tile_grid(R, L.k.nblocks);
   for (int d = 0; d < L.k.ndim; ++d) {
     if (L.bc[d][0] == PANGU_BC_PERIODIC && L.bc[d][1] == PANGU_BC_PERIODIC) continue;
-    hipLaunchKernelGGL(k_bc, g, kBlock, 0, s, kc, F, flag, d, L.bc[d][0], L.bc[d][1], L.nb[d]);
+    hipLaunchKernelGGL(k_bc, g, kTile, 0, s, L.k, F, flag, d, L.bc[d][0], L.bc[d][1], L.nb[d], R);
   }
   return PG_CHECK_LAUNCH();
 }
 
-int launch_dt(const LaunchCtx &L, double *const *alfven, unsigned long long *red, double *state,
+int launch_dt(const LaunchCtx &L, double *const *alfven, double *red, double *state,
               double dt_factor, double *est_out, hipStream_t s) {
   const DevBlock &B = L.hblk;
-  const int n = L.k.nblocks * 3;
-  hipLaunchKernelGGL(k_dt_reset, dim3((n + 255) / 256), dim3(256), 0, s, red, n);
-  const int nx0 = B.e[0] - B.s[0] + 1, nx1 = B.e[1] - B.s[1] + 1, nx2 = B.e[2] - B.s[2] + 1;
   KCtx kc = L.k;
-  kc.nk = nx2;
-  dim3 g = grid_for(nx0, nx1, L.k.nblocks * nx2);
-  hipLaunchKernelGGL(k_dt_reduce, g, kBlock, 0, s, kc, alfven, red);
+  kc.nk = B.e[2] - B.s[2] + 1;
+  hipLaunchKernelGGL(k_dt_reduce, dim3(L.k.nblocks * kc.nk), dim3(256), 0, s, kc, alfven, red);
   hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, s, kc, red, state, dt_factor, est_out);
   return PG_CHECK_LAUNCH();
 }

@@ -545,4 +545,36 @@ int pangu_sim_block_geom(pangu_sim *s, int b, double *xd, int *n) {
 
 const char *pangu_sim_last_error(pangu_sim *s) { return s ? s->err.c_str() : "null"; }
 
+// Plain device copy of n doubles, used to calibrate the HBM byte counters
+// (FETCH_SIZE / WRITE_SIZE) against a kernel of known traffic.
+__global__ void k_copy_calibration(const double *__restrict__ a, double *__restrict__ b, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
+int pangu_sim_copy_calibration(pangu_sim *s, int64_t n, int reps, double *ms) {
+  double *a = nullptr, *b = nullptr;
+  SCHK(s, hipMalloc((void **)&a, (size_t)n * sizeof(double)));
+  SCHK(s, hipMalloc((void **)&b, (size_t)n * sizeof(double)));
+  SCHK(s, hipMemsetAsync(a, 0, (size_t)n * sizeof(double), s->stream));
+  hipEvent_t e0, e1;
+  SCHK(s, hipEventCreate(&e0));
+  SCHK(s, hipEventCreate(&e1));
+  const int64_t nblk = (n + 255) / 256;
+  hipLaunchKernelGGL(k_copy_calibration, dim3((unsigned)nblk), dim3(256), 0, s->stream, a, b, n);
+  SCHK(s, hipEventRecord(e0, s->stream));
+  for (int r = 0; r < reps; ++r)
+    hipLaunchKernelGGL(k_copy_calibration, dim3((unsigned)nblk), dim3(256), 0, s->stream, a, b, n);
+  SCHK(s, hipEventRecord(e1, s->stream));
+  SCHK(s, hipEventSynchronize(e1));
+  float f = 0.f;
+  SCHK(s, hipEventElapsedTime(&f, e0, e1));
+  *ms = (double)f / reps;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(a);
+  hipFree(b);
+  return 0;
+}
+
 }  // extern "C"

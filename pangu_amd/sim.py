@@ -76,6 +76,7 @@ def lib():
         L.pangu_sim_dt.restype = C.c_double
         L.pangu_sim_dt.argtypes = [vp]
         L.pangu_sim_block_geom.argtypes = [vp, C.c_int, dp, ip]
+        L.pangu_sim_copy_calibration.argtypes = [vp, C.c_int64, C.c_int, dp]
         L.pangu_sim_last_error.restype = C.c_char_p
         L.pangu_sim_last_error.argtypes = [vp]
         _lib = L
@@ -160,6 +161,11 @@ class HipSim:
         if n < 0:
             self._check(n, "time_phases")
         return {lib().pangu_sim_phase_name(i).decode(): float(out[i]) for i in range(n)}
+
+    def copy_calibration(self, n, reps=10):
+        ms = C.c_double(0.0)
+        self._check(lib().pangu_sim_copy_calibration(self._p, n, reps, C.byref(ms)), "copy")
+        return ms.value
 
     def task(self, name, c0=0, c1=0, beta_dt=0.0):
         out = C.c_double(0.0)
