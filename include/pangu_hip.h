@@ -109,6 +109,15 @@ int pangu_create(const pangu_config *cfg, const pangu_block *blocks, int nblocks
 int pangu_destroy(pangu_ctx *ctx);
 const char *pangu_last_error(const pangu_ctx *ctx);
 
+/* Host-only: one entry of the geometry tables at code coordinates x
+ * (loc 0 = cell centre, 1..3 = lower face normal to x1..x3).  out receives
+ * 102 (centre) or 24 (face) doubles in the table component order: gcov[16],
+ * gcon[16], sqrt|g|, 1/sqrt(-g^00), sqrt(-1/g^00), the two floors, r and
+ * Gamma[64] for centres; gcov[16], g^0mu[4], g^dd, sqrt|g|, 1/sqrt(-g^00),
+ * valid for faces.  Replaces ComputeMetricAtLocation/ComputeChristoffelConnection
+ * (metric/christoffel.h:53-148) as evaluated at table build time. */
+int pangu_geometry_entry(const pangu_config *cfg, int loc, const double x[4], double *out);
+
 /* Registers a container's pointer tables (copied to the device once);
  * returns a handle used by the task functions.  Pointers must stay valid. */
 int pangu_register_container(pangu_ctx *ctx, const pangu_container *c, int *handle);

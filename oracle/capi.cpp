@@ -114,6 +114,8 @@ int oracle_sim_block_geom(void *sim, int b, double *xd, int *n) {
   return 0;
 }
 
+void oracle_plant(int which) { g_plant = which; }
+
 int oracle_metric_at(int kind, const double x[4], double a, double h, double gcov[16],
                      double gcon[16], double *gdet) {
   real gc[4][4], gn[4][4];

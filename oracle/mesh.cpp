@@ -428,6 +428,8 @@ void Sim::ct() {
       const int64_t id = B.idx(k, j, i);
       if (o1) Fc(0, b, iB1)[id] = 0;
       if (E2) Fc(0, b, iB3)[id] = -0.5 * (e2[id] + e2[id + sk]);
+      if (E3 && g_plant == PLANT_CT_SIGN) Fc(0, b, iB2)[id] = -0.5 * (e3[id] + e3[id + sj]);
+      else
       if (E3) Fc(0, b, iB2)[id] = 0.5 * (e3[id] + e3[id + sj]);
     }
   });

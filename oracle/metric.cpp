@@ -4,6 +4,8 @@
 
 namespace oracle {
 
+int g_plant = PLANT_NONE;
+
 // MKS.h:15-23, BL.h:15-23 (identical maps), CKS.h:13-20, minkowski.h:13-20.
 void physical_coords(int kind, const real x[4], real y[4], real h) {
   if (kind == MET_MKS || kind == MET_BL) {
@@ -216,7 +218,10 @@ void christoffel(int kind, const real x[4], real a, real h, real conn[4][4][4]) 
   real cc[4][4][4];
   for (int i = 0; i < 4; ++i)
     for (int j = 0; j < 4; ++j)
-      for (int k = 0; k < 4; ++k) cc[i][j][k] = 0.5 * (dg[j][i][k] + dg[k][i][j] - dg[i][j][k]);
+      for (int k = 0; k < 4; ++k)
+        cc[i][j][k] = (g_plant == PLANT_CONN_INDEX)
+                          ? 0.5 * (dg[i][j][k] + dg[k][i][j] - dg[j][i][k])
+                          : 0.5 * (dg[j][i][k] + dg[k][i][j] - dg[i][j][k]);
   for (int i = 0; i < 4; ++i)
     for (int j = 0; j < 4; ++j)
       for (int k = 0; k < 4; ++k) {

@@ -83,6 +83,19 @@ struct Params {
   double r_excise, dexcise, pexcise;
 };
 
+// Planted-mistake switch for the tests that prove the pins catch plausible
+// bugs (0 = faithful; see tests/test_oracle_pins.py).  Never set otherwise.
+extern int g_plant;
+enum Plant {
+  PLANT_NONE = 0,
+  PLANT_CONN_INDEX = 1,   // Christoffel: swap two derivative indices
+  PLANT_SHIFT_SIGN = 2,   // state: wrong sign of the shift term in u^i
+  PLANT_PPM_TERM = 3,     // PPM4: drop the outer stencil term
+  PLANT_KASTAUN_QBAR = 4, // Kastaun: wrong sign of the b^2 term in qbar
+  PLANT_CT_SIGN = 5,      // CT: wrong sign of the X1-face B3 flux
+  PLANT_EMT_PRESSURE = 6  // T^mu_nu: drop the magnetic pressure
+};
+
 // Kokkos::max / Kokkos::min semantics (ternary on operator<).
 static inline real kmax(real a, real b) { return (a < b) ? b : a; }
 static inline real kmin(real a, real b) { return (b < a) ? b : a; }

@@ -75,6 +75,9 @@ double oracle_sim_dt(void *sim);
 int oracle_sim_ncycle(void *sim);
 int oracle_sim_block_geom(void *sim, int b, double *xg_dx_out /* 6 */, int *n_out /* 9 */);
 
+/* planted-mistake switch (tests only): 0 = faithful */
+void oracle_plant(int which);
+
 /* pointwise functions for unit tests */
 int oracle_metric_at(int kind, const double x[4], double a, double h, double gcov[16],
                      double gcon[16], double *gdet);

@@ -78,6 +78,7 @@ def lib():
         L.oracle_fms.argtypes = [C.c_double, dp, dp, dp, C.c_int, dp]
         L.oracle_recon.argtypes = [C.c_int, dp, dp]
         L.oracle_c2p.argtypes = [C.c_int, dp, dp, C.c_double, dp, dp, ip]
+        L.oracle_plant.argtypes = [C.c_int]
         L.oracle_heating.restype = C.c_double
         L.oracle_heating.argtypes = [C.POINTER(OracleCfg), dp]
         _lib = L
@@ -193,6 +194,15 @@ class OracleSim:
     def interior(self, arr):
         s, e = self.s, self.e
         return arr[..., s[2]:e[2] + 1, s[1]:e[1] + 1, s[0]:e[0] + 1]
+
+
+PLANTS = dict(none=0, conn_index=1, shift_sign=2, ppm_term=3, kastaun_qbar=4, ct_sign=5,
+              emt_pressure=6)
+
+
+def plant(name):
+    """Activate a planted mistake in the oracle (tests of the pins only)."""
+    lib().oracle_plant(PLANTS[name])
 
 
 # ---- pointwise functions ------------------------------------------------

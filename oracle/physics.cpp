@@ -24,6 +24,8 @@ void calc_state(const real prim[NPRIM], const real gcov[4][4], const real gcon[4
   s.ucon[1] = prim[UX1] - s.ucon[0] * gcon[0][1] * alpha * alpha;
   s.ucon[2] = prim[UX2] - s.ucon[0] * gcon[0][2] * alpha * alpha;
   s.ucon[3] = prim[UX3] - s.ucon[0] * gcon[0][3] * alpha * alpha;
+  if (g_plant == PLANT_SHIFT_SIGN)
+    for (int i = 1; i < 4; ++i) s.ucon[i] = prim[UX1 + i - 1] + s.ucon[0] * gcon[0][i] * alpha * alpha;
   for (int r = 0; r < 4; ++r) {
     real acc = 0.0;
     for (int c = 0; c < 4; ++c) acc += gcov[r][c] * s.ucon[c];
@@ -51,7 +53,7 @@ void tmunu_row(real gamma, const real prim[NPRIM], const real gcov[4][4],
   const real pg = (gamma - 1.) * prim[ENY];
   const real h = prim[RHO] + gamma * prim[ENY];
   const real etot = h + s.bsq;
-  const real ptot = pg + 0.5 * s.bsq;
+  const real ptot = (g_plant == PLANT_EMT_PRESSURE) ? pg : pg + 0.5 * s.bsq;
   for (int m = 0; m < 4; ++m)
     t[m] = etot * s.ucon[dir] * s.ucov[m] + ptot * (real)(dir == m) - s.bcon[dir] * s.bcov[m];
 }
@@ -149,7 +151,8 @@ void recon_mc(real qm2, real qm1, real q0, real qp1, real qp2, real *ql_ip1, rea
 
 // PPM with Colella-Woodward limiters (interpolater_ppm4.h:29-67).
 void recon_ppm4(real qm2, real qm1, real q0, real qp1, real qp2, real *ql_ip1, real *qr_i) {
-  real ql = (7.0 * (q0 + qm1) - (qm2 + qp1)) / 12.0;
+  real ql = (g_plant == PLANT_PPM_TERM) ? (7.0 * (q0 + qm1) - (qm2)) / 12.0
+                                        : (7.0 * (q0 + qm1) - (qm2 + qp1)) / 12.0;
   real qr = (7.0 * (q0 + qp1) - (qm1 + qp2)) / 12.0;
   ql = std::fmax(ql, std::fmin(q0, qm1));
   ql = std::fmin(ql, std::fmax(q0, qm1));

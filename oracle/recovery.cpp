@@ -407,7 +407,9 @@ real kastaun_root(real mu, real D, real q, real bsq, real rsq, real rbsq, real g
   real vsq = musq * rbarsq;
   vsq = kmin(kmax(vsq, 0.0), 1.0 - 1.0e-14);
   const real il = std::sqrt(1.0 - vsq);
-  const real qbar = q - 0.5 * bsq - 0.5 * musq * xsq * (bsq * rsq - rbsq);
+  const real qbar = (g_plant == PLANT_KASTAUN_QBAR)
+                        ? q + 0.5 * bsq - 0.5 * musq * xsq * (bsq * rsq - rbsq)
+                        : q - 0.5 * bsq - 0.5 * musq * xsq * (bsq * rsq - rbsq);
   const real eod = qbar - mu * rbarsq + 1.0;
   const real rho = D * il;
   real eps = D * eod - rho;

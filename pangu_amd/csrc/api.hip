@@ -320,6 +320,15 @@ int pangu_destroy(pangu_ctx *c) {
 
 const char *pangu_last_error(const pangu_ctx *c) { return c ? c->err.c_str() : "null context"; }
 
+int pangu_geometry_entry(const pangu_config *cfg, int loc, const double x[4], double *out) {
+  if (!cfg || !x || !out || loc < 0 || loc > 3) return PANGU_EINVAL;
+  if (loc == 0)
+    geom::fill_centre(*cfg, x, out, 1);
+  else
+    geom::fill_face(*cfg, loc - 1, x, out, 1);
+  return PANGU_OK;
+}
+
 int pangu_register_container(pangu_ctx *c, const pangu_container *src, int *handle) {
   if (!c || !src || !handle || !src->prim || !src->cons) return fail(c, PANGU_EINVAL, "null container");
   hipSetDevice(c->device);

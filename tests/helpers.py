@@ -20,6 +20,49 @@ def per_comp_l1(A, B):
     return [rel_l1(A[:, q], B[:, q]) for q in range(A.shape[1])]
 
 
+def corner_divb(U, iB, s, e, dx):
+    """Max |corner divergence| of U_B = sqrt|g| B over interior corners of
+    (nblocks, ncomp, k, j, i) conserved arrays, and max |U_B| for scaling.
+
+    The CT scheme of constrained_transport.cc preserves
+      sum_d [sum over the 2^(ndim-1) cells on the + side - those on the - side] U_Bd
+      / (2^(ndim-1) dx_d)
+    at every corner whose surrounding cells are all interior."""
+    ndim = sum(1 for d in range(3) if e[d] > s[d])
+    sl = lambda d, lo, hi: slice(lo, hi)
+    out = 0.0
+    scale = 0.0
+    for b in range(U.shape[0]):
+        comps = [U[b, iB + d] for d in range(3)]
+        scale = max(scale, max(np.abs(c).max() for c in comps[:ndim]))
+        # corner (k-1/2, j-1/2, i-1/2) for k in s2+1..e2 etc.
+        if ndim == 2:
+            k = s[2]
+            j0, j1, i0, i1 = s[1] + 1, e[1] + 1, s[0] + 1, e[0] + 1
+            B1, B2 = comps[0][k], comps[1][k]
+            d1 = (B1[j0:j1, i0:i1] + B1[j0 - 1:j1 - 1, i0:i1] - B1[j0:j1, i0 - 1:i1 - 1]
+                  - B1[j0 - 1:j1 - 1, i0 - 1:i1 - 1]) / (2 * dx[0])
+            d2 = (B2[j0:j1, i0:i1] + B2[j0:j1, i0 - 1:i1 - 1] - B2[j0 - 1:j1 - 1, i0:i1]
+                  - B2[j0 - 1:j1 - 1, i0 - 1:i1 - 1]) / (2 * dx[1])
+            out = max(out, np.abs(d1 + d2).max())
+        else:
+            k0, k1 = s[2] + 1, e[2] + 1
+            j0, j1, i0, i1 = s[1] + 1, e[1] + 1, s[0] + 1, e[0] + 1
+            tot = 0.0
+            for d in range(3):
+                A = comps[d]
+                acc = 0.0
+                for dk in (0, 1):
+                    for dj in (0, 1):
+                        for di in (0, 1):
+                            off = (di, dj, dk)[d]
+                            sign = 1.0 if off == 0 else -1.0
+                            acc = acc + sign * A[k0 - dk:k1 - dk, j0 - dj:j1 - dj, i0 - di:i1 - di]
+                tot = tot + acc / (4 * dx[d])
+            out = max(out, np.abs(tot).max())
+    return out, scale
+
+
 # Reduced versions of the three configurations, sized so the oracle runs
 # them in seconds.
 def small_configs():
