@@ -9,9 +9,10 @@ update) over the whole mesh; synthetic initial data (FM torus with a seeded
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode graph|fused|tasks]
 
 Prints one JSON line.  Besides the metric it reports the HBM roofline of the
-dominant kernel phase (algorithmic bytes / HIP-event time) and the CPU
-baseline: the oracle restatement of the reference algorithm timed on this
-host's cores on a bounded sample.
+dominant kernel (algorithmic bytes per launch / launch duration from HIP
+events on the simulation stream) and the CPU baseline: the oracle
+restatement of the reference algorithm timed on this host's cores on two
+cycles of the same mesh.
 """
 import argparse
 import json
@@ -26,15 +27,28 @@ METRIC = "FP64 zone-cycles/s, 3D MKS torus, 1 MI355X; % HBM roofline; batched/le
 HBM_PEAK_GBS = 8000.0
 FP64_VALU_PEAK_TFLOPS = 78.6
 
-# Algorithmic HBM bytes per launch of each phase, per interior zone (FP64
-# words x 8).  n_p = n_c = 10 for this workload.  Counted once per array
-# element a phase must read or write (stencil re-reads and geometry tables
-# amortised over k excluded):
-#   fluxes       read P (10) + write 3 x (flux 10 + alfven 1)      = 43 words
-#   recovery     read U (10) + read P (10) + write P (8: rho,u,u~,B, K_e) = 28
-#   div_update   read 3 flux (30) + U_init (10) + P (10) + write U (10) = 60
-#   fix_cons     read P (10) + write P (7) + write U (10)             = 27
-WORDS_PER_ZONE = {"fluxes": 43, "recovery_heating": 28, "div_update_source": 60, "fix_cons": 27}
+# Algorithmic HBM bytes of one launch of each kernel, per interior zone of
+# the mesh (FP64 words x 8; n_p = n_c = 10 for this workload).  Each array
+# element a kernel must read or write counts once; stencil re-reads, the
+# one-cell halo of faces that CT needs (<= 6 %) and the geometry tables
+# (x3-independent, amortised over the 32 planes of a block) are excluded:
+#   flux_x1/2/3        read P (10) + write flux (10) + alfven (1)            = 21
+#   div_update_source  read 3 fluxes (30) + U_init (10) + P (10) + write U (10) = 60
+#   recovery_heating   read U (10) + P (10) + write P (8: rho, u, u~, B, K_e)  = 28
+#   fix_cons           read P (10) + write P (7) + write U (10)              = 27
+#   ct_emf             read 6 B-fluxes + write 3 EMFs                        = 9
+#   ct_faces           read 3 EMFs + write 9 B-fluxes                        = 12
+WORDS_PER_ZONE = {"flux_x1": 21, "flux_x2": 21, "flux_x3": 21, "div_update_source": 60,
+                  "recovery_heating": 28, "fix_cons": 27, "ct_emf": 9, "ct_faces": 12}
+
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE on this workload, each scaled by the same counter's reading for
+# a plain copy kernel of known size (profiles/traffic_torus3d_256x128x128.md).
+MEASURED_TRAFFIC = {
+    "div_update_source": 3673015732, "flux_x1": 881187029, "flux_x2": 924225198,
+    "flux_x3": 1129770898, "fix_cons": 1226540096, "recovery_heating": 1185042238,
+    "ct_emf": 353580935, "ct_faces": 458015791,
+}
 
 
 def parse():
@@ -45,7 +59,7 @@ def parse():
     ap.add_argument("--mode", default="graph", choices=["graph", "fused", "tasks"])
     ap.add_argument("--small", action="store_true", help="reduced mesh for quick checks")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-cycles", type=int, default=1)
+    ap.add_argument("--cpu-sample-cycles", type=int, default=2)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="also run a plain copy kernel of this many doubles (counter calibration)")
     return ap.parse_args()
@@ -78,7 +92,7 @@ def cpu_baseline(cfg_small, cycles):
         "kind": "port",
         "sample": (f"{cycles} RK2 cycle(s) of the same 3D torus physics at "
                    f"{d['nx'][0]}x{d['nx'][1]}x{d['nx'][2]} zones in "
-                   f"{d['nb'][0] * d['nb'][1] * d['nb'][2]} blocks of 64x64x32, "
+                   f"{d['nb'][0] * d['nb'][1] * d['nb'][2]} blocks, "
                    f"OpenMP {nthreads} threads, metric recomputed per evaluation; {dt:.2f} s"),
     }
 
@@ -108,12 +122,15 @@ def main():
     wall_one = time.perf_counter() - wall0
     value = zones * args.steps / (ms * 1e-3)
 
-    # dominant phase: HIP-event timing of each kernel phase on the sim stream
-    phases = sim.time_phases(reps=5)
-    tot = sum(phases.values())
-    dom = max((k for k in phases if k in WORDS_PER_ZONE), key=lambda k: phases[k])
-    bytes_dom = WORDS_PER_ZONE[dom] * 8 * zones
-    achieved = bytes_dom / (phases[dom] * 1e-3) / 1e9
+    # Dominant kernel: HIP events recorded on the simulation stream after
+    # every launch of a few fused cycles give each kernel's launch duration.
+    kern = sim.time_kernels(ncycles=3)
+    dom = max((k for k in kern if k in WORDS_PER_ZONE), key=lambda k: kern[k][1])
+    calls, ms_cycle = kern[dom]
+    launch_ms = ms_cycle / calls
+    bytes_launch = WORDS_PER_ZONE[dom] * 8 * zones
+    achieved = bytes_launch / (launch_ms * 1e-3) / 1e9
+    traffic = MEASURED_TRAFFIC.get(dom)
     out = {
         "metric": METRIC,
         "value": value,
@@ -144,10 +161,12 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "launch_ms": launch_ms,
+            "algorithmic_bytes": bytes_launch,
         },
-        "phase_ms": phases,
-        "phase_ms_sum": tot,
+        "kernel_ms_per_cycle": {k: v[1] for k, v in kern.items()},
+        "kernel_launches_per_cycle": {k: v[0] for k, v in kern.items()},
         "init_s": t_init,
         "wall_one_cycle_s": wall_one,
         "sim_time": sim.time,
@@ -157,10 +176,9 @@ def main():
         n = args.copy_calibration
         cms = sim.copy_calibration(n, reps=5)
         out["copy_calibration"] = {"doubles": n, "ms": cms, "GBps": 16.0 * n / (cms * 1e-3) / 1e9}
-    if not args.no_cpu_baseline:
-        small = configs.torus3d((128, 64, 64), (2, 1, 2))
-        out["cpu_baseline"] = cpu_baseline(small, args.cpu_sample_cycles)
     sim.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_sample_cycles)
     print(json.dumps(out))
 
 

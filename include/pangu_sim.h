@@ -68,6 +68,13 @@ int pangu_sim_timed_cycles(pangu_sim *s, int ncycles, int mode, double *ms);
  * ms_out[n] = mean milliseconds over reps of phase n (see pangu_sim_phase_name). */
 int pangu_sim_time_phases(pangu_sim *s, int reps, double *ms_out, int max_phases);
 const char *pangu_sim_phase_name(int n);
+/* Runs ncycles fused cycles with a HIP event recorded on the simulation
+ * stream after every kernel launch.  Returns the number of distinct kernels
+ * n; ms_out[i] is kernel i's total milliseconds per cycle and calls_out[i]
+ * its launches per cycle (names from pangu_sim_kernel_name). */
+int pangu_sim_time_kernels(pangu_sim *s, int ncycles, double *ms_out, int *calls_out,
+                           int max_kernels);
+const char *pangu_sim_kernel_name(pangu_sim *s, int n);
 /* One task on container c0 (c1 output of update/source), host beta*dt. */
 int pangu_sim_task(pangu_sim *s, int task, int c0, int c1, double beta_dt, double *out);
 int64_t pangu_sim_field_size(pangu_sim *s, int field);

@@ -209,6 +209,12 @@ static int build_neighbours(pangu_ctx *c) {
   return upload(c, &c->dnbr, nbr.data(), nbr.size());
 }
 
+namespace pangu {
+void set_kernel_clock(pangu_ctx *c, KernelClock *clock) {
+  if (c) c->L.clock = clock;
+}
+}  // namespace pangu
+
 extern "C" {
 
 int pangu_create(const pangu_config *cfg, const pangu_block *blocks, int nblocks, int device,

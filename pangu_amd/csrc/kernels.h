@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "common.h"
 #include "physics_dev.h"
 
@@ -38,6 +40,19 @@ struct Range3 {
   int lo[3], hi[3], cnt[3];
 };
 
+// Optional per-launch clock: when a LaunchCtx carries one, every kernel
+// launch is followed by a HIP event on the launch stream, so the duration of
+// launch i is the time between events i and i+1.  Timing runs only: not for
+// code under graph capture.
+struct KernelClock {
+  std::vector<const char *> names;  // names[i]: the launch ended by events[i + 1]
+  std::vector<hipEvent_t> events;
+  size_t n = 0;                     // events recorded since reset()
+  hipError_t mark(const char *name, hipStream_t s);
+  void reset() { n = 0; }
+  ~KernelClock();
+};
+
 // Host-side launch context: kernel constants plus the (uniform) block shape.
 struct LaunchCtx {
   KCtx k;
@@ -45,7 +60,11 @@ struct LaunchCtx {
   int limiter, riemann, metric, recovery;
   int bc[3][2];
   int nb[3];
+  KernelClock *clock = nullptr;
 };
+
+// Attaches (or, with nullptr, detaches) a clock to a context's launches.
+void set_kernel_clock(pangu_ctx *c, KernelClock *clock);
 
 int launch_fix_cons(const LaunchCtx &L, const KFields &F, uint8_t *const *mask, bool fix, bool cons,
                     hipStream_t s);

@@ -809,6 +809,26 @@ static dim3 tile_grid(const Range3 &R, int nblocks) {
 static const dim3 kTile(TILE, 1, 1);
 
 #define PG_CHECK_LAUNCH() (hipGetLastError() == hipSuccess ? 0 : -2)
+#define TICK(name) \
+  if (L.clock) L.clock->mark(name, s)
+
+hipError_t KernelClock::mark(const char *name, hipStream_t s) {
+  if (n == events.size()) {
+    hipEvent_t e;
+    const hipError_t r = hipEventCreate(&e);
+    if (r != hipSuccess) return r;
+    events.push_back(e);
+  }
+  if (n > 0) {
+    if (names.size() < n) names.resize(n);
+    names[n - 1] = name;
+  }
+  return hipEventRecord(events[n++], s);
+}
+
+KernelClock::~KernelClock() {
+  for (hipEvent_t e : events) (void)hipEventDestroy(e);
+}
 
 int launch_fix_cons(const LaunchCtx &L, const KFields &F, uint8_t *const *mask, bool fix, bool cons,
                     hipStream_t s) {
@@ -830,6 +850,7 @@ int launch_fix_cons(const LaunchCtx &L, const KFields &F, uint8_t *const *mask, 
   }
 #undef FC2
 #undef FC
+  TICK(fix && cons ? "fix_cons" : fix ? "fix_primitive" : "conservative");
   return PG_CHECK_LAUNCH();
 }
 
@@ -857,6 +878,7 @@ static void flux_dir(const LaunchCtx &L, const KFields &F, double *const *alfven
   }
 #undef FL2
 #undef FL
+  TICK(DIR == 1 ? "flux_x1" : DIR == 2 ? "flux_x2" : "flux_x3");
 }
 
 // Face ranges of the three sweeps (laxf.cc:78-83,184-189,290-295): the
@@ -895,7 +917,9 @@ int launch_ct(const LaunchCtx &L, const KFields &F, double *const *emf, hipStrea
   const Range3 R = make_range(B.s[0], B.e[0] + o1, B.s[1], B.e[1] + o2, B.s[2], B.e[2] + o3);
   const dim3 g = tile_grid(R, L.k.nblocks);
   hipLaunchKernelGGL(k_emf, g, kTile, 0, s, L.k, F, emf, R, o2 && o3, o3 && o1, o1 && o2);
+  TICK("ct_emf");
   hipLaunchKernelGGL(k_ct_faces, g, kTile, 0, s, L.k, F, emf, R, o1, o2, o3);
+  TICK("ct_faces");
   return PG_CHECK_LAUNCH();
 }
 
@@ -931,6 +955,8 @@ int launch_div_update(const LaunchCtx &L, int mode, const KFields &F0, const KFi
   }
 #undef DU2
 #undef DU
+  if (mode != 3 || src)
+    TICK(mode == 1 ? "flux_divergence" : mode == 2 ? "update" : mode == 3 ? "source" : "div_update_source");
   return PG_CHECK_LAUNCH();
 }
 
@@ -958,6 +984,7 @@ int launch_recover(const LaunchCtx &L, const KFields &F, const SharedTabs &S, in
   }
 #undef RC2
 #undef RC
+  if (rec >= 0 || h) TICK(rec < 0 ? "heating" : h ? "recovery_heating" : "recovery");
   return PG_CHECK_LAUNCH();
 }
 
@@ -968,7 +995,9 @@ int launch_fix_recovery(const LaunchCtx &L, const KFields &F, double *const *fla
   KCtx kc = L.k;
   kc.nk = R.cnt[2];
   hipLaunchKernelGGL(k_fixrec_a, tile_grid(R, L.k.nblocks), kTile, 0, s, kc, F, flag, cluster, R);
+  TICK("fix_recovery_a");
   hipLaunchKernelGGL(k_fixrec_b, dim3(L.k.nblocks * R.cnt[2]), dim3(64), 0, s, kc, F, flag, cluster);
+  TICK("fix_recovery_b");
   return PG_CHECK_LAUNCH();
 }
 
@@ -976,6 +1005,7 @@ int launch_exchange(const LaunchCtx &L, const KFields &F, double *const *flag, c
                     hipStream_t s) {
   const Range3 R = entire_range(L.hblk);
   hipLaunchKernelGGL(k_exchange, tile_grid(R, L.k.nblocks), kTile, 0, s, L.k, F, flag, nbr, R);
+  TICK("exchange");
   return PG_CHECK_LAUNCH();
 }
 
@@ -985,6 +1015,7 @@ int launch_bc(const LaunchCtx &L, const KFields &F, double *const *flag, hipStre
   for (int d = 0; d < L.k.ndim; ++d) {
     if (L.bc[d][0] == PANGU_BC_PERIODIC && L.bc[d][1] == PANGU_BC_PERIODIC) continue;
     hipLaunchKernelGGL(k_bc, g, kTile, 0, s, L.k, F, flag, d, L.bc[d][0], L.bc[d][1], L.nb[d], R);
+    TICK(d == 0 ? "bc_x1" : d == 1 ? "bc_x2" : "bc_x3");
   }
   return PG_CHECK_LAUNCH();
 }
@@ -995,7 +1026,9 @@ int launch_dt(const LaunchCtx &L, double *const *alfven, double *red, double *st
   KCtx kc = L.k;
   kc.nk = B.e[2] - B.s[2] + 1;
   hipLaunchKernelGGL(k_dt_reduce, dim3(L.k.nblocks * kc.nk), dim3(256), 0, s, kc, alfven, red);
+  TICK("dt_reduce");
   hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, s, kc, red, state, dt_factor, est_out);
+  TICK("dt_final");
   return PG_CHECK_LAUNCH();
 }
 

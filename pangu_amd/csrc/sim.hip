@@ -10,6 +10,7 @@
 #include "../../include/pangu_hip.h"
 #include "../../include/pangu_sim.h"
 #include "host.h"
+#include "kernels.h"
 
 using namespace pangu;
 
@@ -46,6 +47,7 @@ struct pangu_sim {
   hipStream_t stream = nullptr;
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
+  std::vector<std::string> knames;  // kernels of the last pangu_sim_time_kernels
   std::string err;
 };
 
@@ -391,6 +393,55 @@ int pangu_sim_timed_cycles(pangu_sim *s, int n, int mode, double *ms) {
   hipEventDestroy(a);
   hipEventDestroy(b);
   return 0;
+}
+
+int pangu_sim_time_kernels(pangu_sim *s, int ncycles, double *ms_out, int *calls_out, int maxk) {
+  hipSetDevice(s->device);
+  if (ncycles < 1 || !ms_out || !calls_out) return sfail(s, PANGU_EINVAL, "bad arguments");
+  if (int r = ensure_stage1(s)) return r;
+  if (!s->dt_on_device) {
+    push_dt_to_device(s);
+    s->dt_on_device = true;
+  }
+  KernelClock clk;
+  std::vector<std::string> names;
+  std::vector<double> tot;
+  std::vector<int> calls;
+  for (int c = 0; c < ncycles; ++c) {
+    clk.reset();
+    SCHK(s, clk.mark(nullptr, s->stream));
+    set_kernel_clock(s->ctx, &clk);
+    const int r = cycle_fused(s);
+    set_kernel_clock(s->ctx, nullptr);
+    if (r) return r;
+    SCHK(s, hipStreamSynchronize(s->stream));
+    for (size_t i = 1; i < clk.n; ++i) {
+      float f = 0.f;
+      SCHK(s, hipEventElapsedTime(&f, clk.events[i - 1], clk.events[i]));
+      const std::string nm = clk.names[i - 1] ? clk.names[i - 1] : "?";
+      size_t j = 0;
+      while (j < names.size() && names[j] != nm) ++j;
+      if (j == names.size()) {
+        names.push_back(nm);
+        tot.push_back(0.0);
+        calls.push_back(0);
+      }
+      tot[j] += f;
+      calls[j] += 1;
+    }
+  }
+  s->ncycle += ncycles;
+  s->knames = names;
+  const int nk = (int)names.size();
+  for (int j = 0; j < nk && j < maxk; ++j) {
+    ms_out[j] = tot[j] / ncycles;
+    calls_out[j] = calls[j] / ncycles;
+  }
+  return nk;
+}
+
+const char *pangu_sim_kernel_name(pangu_sim *s, int n) {
+  return (s && n >= 0 && n < (int)s->knames.size()) ? s->knames[n].c_str() : "";
 }
 
 static const char *kPhase[] = {"fix_cons",  "fluxes",  "ct",  "div_update_source",

@@ -76,6 +76,9 @@ def lib():
         L.pangu_sim_dt.restype = C.c_double
         L.pangu_sim_dt.argtypes = [vp]
         L.pangu_sim_block_geom.argtypes = [vp, C.c_int, dp, ip]
+        L.pangu_sim_time_kernels.argtypes = [vp, C.c_int, dp, ip, C.c_int]
+        L.pangu_sim_kernel_name.restype = C.c_char_p
+        L.pangu_sim_kernel_name.argtypes = [vp, C.c_int]
         L.pangu_sim_copy_calibration.argtypes = [vp, C.c_int64, C.c_int, dp]
         L.pangu_sim_last_error.restype = C.c_char_p
         L.pangu_sim_last_error.argtypes = [vp]
@@ -161,6 +164,18 @@ class HipSim:
         if n < 0:
             self._check(n, "time_phases")
         return {lib().pangu_sim_phase_name(i).decode(): float(out[i]) for i in range(n)}
+
+    def time_kernels(self, ncycles=3):
+        """{kernel: (launches per cycle, ms per cycle)} from HIP events
+        recorded after every launch of ncycles fused cycles."""
+        ms = np.zeros(64)
+        calls = np.zeros(64, dtype=np.int32)
+        n = lib().pangu_sim_time_kernels(self._p, ncycles, ms.ctypes.data_as(C.POINTER(C.c_double)),
+                                         calls.ctypes.data_as(C.POINTER(C.c_int)), 64)
+        if n < 0:
+            self._check(n, "time_kernels")
+        return {lib().pangu_sim_kernel_name(self._p, i).decode(): (int(calls[i]), float(ms[i]))
+                for i in range(min(n, 64))}
 
     def copy_calibration(self, n, reps=10):
         ms = C.c_double(0.0)
