@@ -170,6 +170,25 @@ int pangu_apply_boundary_conditions(pangu_ctx *ctx, int md, void *stream);
 int pangu_stage(pangu_ctx *ctx, int md_init, int md0, int md1, double beta, double *dt_state,
                 double dt_factor, int last_stage, int exchange_after, void *stream);
 
+/* ---- launch tuning (no reference counterpart) ----
+ * Tile order of each kernel family: plane-major (consecutive workgroups
+ * sweep one k plane) or column-major (consecutive workgroups walk k over one
+ * 256-cell run of the (j, i) plane, reusing x3-independent geometry from L2).
+ * Results do not depend on it.  pangu_create sets the order measured best
+ * on gfx950 for each family.  Takes effect on the next launch (re-capture
+ * graphs). */
+enum {
+  PANGU_KF_FIX_CONS = 0, PANGU_KF_FLUX_X1, PANGU_KF_FLUX_X2, PANGU_KF_FLUX_X3, PANGU_KF_CT,
+  PANGU_KF_DIV_UPDATE, PANGU_KF_RECOVERY, PANGU_KF_COUNT
+};
+enum { PANGU_ORDER_PLANE = 0, PANGU_ORDER_COLUMN = 1 };
+int pangu_set_tile_order(pangu_ctx *ctx, int kernel_family, int order);
+int pangu_get_tile_order(const pangu_ctx *ctx, int kernel_family);
+/* Fused stage only: split = 1 runs the geometric source in a kernel of its
+ * own after the flux divergence + update kernel (fewer registers per
+ * kernel, one more pass over U); 0 fuses all three.  Results are identical. */
+int pangu_set_split_source(pangu_ctx *ctx, int split);
+
 #ifdef __cplusplus
 }
 #endif

@@ -75,6 +75,11 @@ const char *pangu_sim_phase_name(int n);
 int pangu_sim_time_kernels(pangu_sim *s, int ncycles, double *ms_out, int *calls_out,
                            int max_kernels);
 const char *pangu_sim_kernel_name(pangu_sim *s, int n);
+/* Sets a kernel family's tile order (pangu_set_tile_order) and drops the
+ * captured graph so the next graph-mode step captures the new launches. */
+int pangu_sim_set_tile_order(pangu_sim *s, int kernel_family, int order);
+/* pangu_set_split_source on the simulation's context (drops the graph). */
+int pangu_sim_set_split_source(pangu_sim *s, int split);
 /* One task on container c0 (c1 output of update/source), host beta*dt. */
 int pangu_sim_task(pangu_sim *s, int task, int c0, int c1, double beta_dt, double *out);
 int64_t pangu_sim_field_size(pangu_sim *s, int field);

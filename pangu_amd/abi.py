@@ -85,6 +85,9 @@ PROTOTYPES = {
     "pangu_estimate_timestep": (C.c_int, [_vp, C.c_int, _dp, _vp]),
     "pangu_exchange_ghosts": (C.c_int, [_vp, C.c_int, _vp]),
     "pangu_apply_boundary_conditions": (C.c_int, [_vp, C.c_int, _vp]),
+    "pangu_set_tile_order": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "pangu_get_tile_order": (C.c_int, [_vp, C.c_int]),
+    "pangu_set_split_source": (C.c_int, [_vp, C.c_int]),
     "pangu_stage": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_double, _dp, C.c_double,
                               C.c_int, C.c_int, _vp]),
 }

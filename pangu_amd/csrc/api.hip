@@ -324,6 +324,25 @@ int pangu_destroy(pangu_ctx *c) {
   return PANGU_OK;
 }
 
+int pangu_set_tile_order(pangu_ctx *c, int fam, int order) {
+  if (!c) return PANGU_EINVAL;
+  if (fam < 0 || fam >= PANGU_KF_COUNT || (order != PANGU_ORDER_PLANE && order != PANGU_ORDER_COLUMN))
+    return fail(c, PANGU_EINVAL, "tile order: unknown kernel family or order");
+  c->L.order[fam] = order;
+  return PANGU_OK;
+}
+
+int pangu_set_split_source(pangu_ctx *c, int split) {
+  if (!c) return PANGU_EINVAL;
+  c->L.split_source = split != 0;
+  return PANGU_OK;
+}
+
+int pangu_get_tile_order(const pangu_ctx *c, int fam) {
+  if (!c || fam < 0 || fam >= PANGU_KF_COUNT) return PANGU_EINVAL;
+  return c->L.order[fam];
+}
+
 const char *pangu_last_error(const pangu_ctx *c) { return c ? c->err.c_str() : "null context"; }
 
 int pangu_geometry_entry(const pangu_config *cfg, int loc, const double x[4], double *out) {
@@ -504,7 +523,12 @@ int pangu_stage(pangu_ctx *c, int md_init, int md0, int md1, double beta, double
   LCHK(c, launch_fix_cons(c->L, f0, c->dmask, true, true, s));
   LCHK(c, launch_fluxes(c->L, f0, c->dalfven, s));
   if (c->cfg.enable_B) LCHK(c, launch_ct(c->L, f0, c->demf, s));
-  LCHK(c, launch_div_update(c->L, 4, f0, fi, f1, nullptr, dt_state, beta, 0.0, s));
+  if (c->L.split_source && c->cfg.metric != PANGU_METRIC_MINKOWSKI) {
+    LCHK(c, launch_div_update(c->L, 5, f0, fi, f1, nullptr, dt_state, beta, 0.0, s));
+    LCHK(c, launch_div_update(c->L, 3, f0, fi, f1, nullptr, dt_state, beta, 0.0, s));
+  } else {
+    LCHK(c, launch_div_update(c->L, 4, f0, fi, f1, nullptr, dt_state, beta, 0.0, s));
+  }
   if (exchange_after == 0) LCHK(c, launch_exchange(c->L, f1, c->dflag, c->dnbr, s));
   LCHK(c, launch_recover(c->L, f1, shared_tabs(c), c->cfg.recovery, c->cfg.enable_heating != 0, s));
   if (c->cfg.recovery != PANGU_RECOVERY_ROBUSTER)

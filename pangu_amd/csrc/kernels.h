@@ -36,8 +36,13 @@ struct SharedTabs {
   int32_t *const *iters;
 };
 
+// Launch range of a kernel over every block.  order: 0 = plane-major tiles
+// (consecutive tiles sweep a k plane), 1 = column-major tiles (consecutive
+// tiles walk k for one 256-cell run of the (j, i) plane, so the x3-independent
+// geometry of that run is reused from L2 by every plane).
 struct Range3 {
   int lo[3], hi[3], cnt[3];
+  int order;
 };
 
 // Optional per-launch clock: when a LaunchCtx carries one, every kernel
@@ -60,6 +65,8 @@ struct LaunchCtx {
   int limiter, riemann, metric, recovery;
   int bc[3][2];
   int nb[3];
+  int order[PANGU_KF_COUNT] = {};  // tile order per kernel family (PANGU_ORDER_*)
+  int split_source = 0;            // fused stage: source term in a kernel of its own
   KernelClock *clock = nullptr;
 };
 
@@ -71,6 +78,7 @@ int launch_fix_cons(const LaunchCtx &L, const KFields &F, uint8_t *const *mask, 
 int launch_fluxes(const LaunchCtx &L, const KFields &F, double *const *alfven, hipStream_t s);
 int launch_ct(const LaunchCtx &L, const KFields &F, double *const *emf, hipStream_t s);
 // mode: 1 divergence -> dudt, 2 update from dudt, 3 source only, 4 fused
+// divergence + update + source, 5 fused divergence + update without source
 int launch_div_update(const LaunchCtx &L, int mode, const KFields &F0, const KFields &Finit,
                       const KFields &F1, double *const *dudt, const double *dt_dev, double beta,
                       double bdt_host, hipStream_t s);

@@ -54,11 +54,21 @@ __device__ __forceinline__ bool decode_tile(const Range3 &R, int nblocks, int &b
   const int L = (int)blockIdx.x;
   const int T = (L & 7) * n8 + (L >> 3);
   if (T >= N) return false;
-  const int p = T / tpp;
-  const int t = (T - p * tpp) * TILE + (int)threadIdx.x;
+  int tile, kk;
+  if (R.order) {
+    const int col = T / R.cnt[2];
+    kk = T - col * R.cnt[2];
+    b = col / tpp;
+    tile = col - b * tpp;
+  } else {
+    const int p = T / tpp;
+    tile = T - p * tpp;
+    b = p / R.cnt[2];
+    kk = p - b * R.cnt[2];
+  }
+  const int t = tile * TILE + (int)threadIdx.x;
   if (t >= plane) return false;
-  b = p / R.cnt[2];
-  k = R.lo[2] + (p - b * R.cnt[2]);
+  k = R.lo[2] + kk;
   const int jj = t / R.cnt[0];
   j = R.lo[1] + jj;
   i = R.lo[0] + (t - jj * R.cnt[0]);
@@ -344,7 +354,9 @@ __global__ __launch_bounds__(256) void k_ct_faces(KCtx C, KFields F, double *con
 // (source_term.cc:171-228).  MODE: 1 = divergence only (writes dU/dt),
 // 2 = update only (reads dU/dt), 3 = source only, 4 = all three fused.
 template <int MODE, bool SRC, bool HB, bool HK>
-__global__ __launch_bounds__(256) void k_div_update(KCtx C, KFields F0, KFields Finit, KFields F1,
+// Five waves per SIMD is the most the source-term variants reach without
+// spilling; the kernel is memory bound, so occupancy pays.
+__global__ __launch_bounds__(256, 5) void k_div_update(KCtx C, KFields F0, KFields Finit, KFields F1,
                                                     double *const *dudt, const double *dt_dev,
                                                     double beta, double bdt_host, Range3 R) {
   int b, k, j, i;
@@ -643,13 +655,54 @@ __global__ __launch_bounds__(64) void k_fixrec_b(KCtx C, KFields F, double *cons
 }
 
 // ------------------------------------------------- ghost exchange and BCs
+// Ghost cell g of a block, numbered over three slabs: the k-ghost planes
+// (full j, i extent), then the j-ghost rows of interior planes (full i),
+// then the i-ghost cells of interior rows.  Every ghost cell appears once,
+// with i fastest inside each slab.
+__device__ __forceinline__ void ghost_cell(const DevBlock &B, int g, int &k, int &j, int &i) {
+  const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
+  const int a2 = B.s[2] + (n2 - 1 - B.e[2]);
+  const int A = a2 * n1 * n0;
+  if (g < A) {
+    const int kk = g / (n1 * n0), r = g - kk * n1 * n0;
+    k = kk < B.s[2] ? kk : B.e[2] + 1 + (kk - B.s[2]);
+    j = r / n0;
+    i = r - j * n0;
+    return;
+  }
+  g -= A;
+  const int b1 = B.s[1] + (n1 - 1 - B.e[1]);
+  const int nk = B.e[2] - B.s[2] + 1;
+  const int Bc = nk * b1 * n0;
+  if (g < Bc) {
+    const int kk = g / (b1 * n0), r = g - kk * b1 * n0;
+    const int jj = r / n0;
+    k = B.s[2] + kk;
+    j = jj < B.s[1] ? jj : B.e[1] + 1 + (jj - B.s[1]);
+    i = r - jj * n0;
+    return;
+  }
+  g -= Bc;
+  const int nj = B.e[1] - B.s[1] + 1;
+  const int c0 = B.s[0] + (n0 - 1 - B.e[0]);
+  const int kk = g / (nj * c0), r = g - kk * nj * c0;
+  const int jj = r / c0, ii = r - jj * c0;
+  k = B.s[2] + kk;
+  j = B.s[1] + jj;
+  i = ii < B.s[0] ? ii : B.e[0] + 1 + (ii - B.s[0]);
+}
+
 // Same-level exchange: a ghost cell of block b takes the interior value of
 // the neighbour in its offset direction (periodic wrap encoded in nbr).
+// One thread per ghost cell (nghost per block).
 __global__ __launch_bounds__(256) void k_exchange(KCtx C, KFields F, double *const *flag,
-                                                  const int *nbr, Range3 R) {
-  int b, k, j, i;
-  if (!decode_tile(R, C.nblocks, b, k, j, i)) return;
+                                                  const int *nbr, int nghost) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (int64_t)C.nblocks * nghost) return;
+  const int b = (int)(gid / nghost);
   const DevBlock &B = C.blk[b];
+  int k, j, i;
+  ghost_cell(B, (int)(gid - (int64_t)b * nghost), k, j, i);
   const int o0 = (i < B.s[0]) ? -1 : (i > B.e[0] ? 1 : 0);
   const int o1 = (j < B.s[1]) ? -1 : (j > B.e[1] ? 1 : 0);
   const int o2 = (k < B.s[2]) ? -1 : (k > B.e[2] ? 1 : 0);
@@ -666,25 +719,62 @@ __global__ __launch_bounds__(256) void k_exchange(KCtx C, KFields F, double *con
   if (flag) flag[b][dst] = flag[nb][src];
 }
 
+// Cell g of the two ghost slabs normal to d (side 0 below s[d], side 1 above
+// e[d]), each ng layers deep and spanning the full ghosted extent of the
+// other dims; i is fastest except for d = 0, where the layer index is.
+__device__ __forceinline__ void bc_cell(const DevBlock &B, int d, int g, int &side, int &k, int &j,
+                                        int &i) {
+  const int ng = B.s[d];
+  int ix[3];
+  if (d == 0) {
+    const int na = B.n[1], nb = B.n[2], per = ng * na * nb;
+    side = g / per;
+    const int r = g - side * per, kk = r / (na * ng), r2 = r - kk * na * ng, jj = r2 / ng;
+    ix[0] = r2 - jj * ng;
+    ix[1] = jj;
+    ix[2] = kk;
+  } else if (d == 1) {
+    const int na = B.n[0], nb = B.n[2], per = ng * na * nb;
+    side = g / per;
+    const int r = g - side * per, kk = r / (ng * na), r2 = r - kk * ng * na, l = r2 / na;
+    ix[0] = r2 - l * na;
+    ix[1] = l;
+    ix[2] = kk;
+  } else {
+    const int na = B.n[0], nb = B.n[1], per = ng * na * nb;
+    side = g / per;
+    const int r = g - side * per, l = r / (nb * na), r2 = r - l * nb * na, jj = r2 / na;
+    ix[0] = r2 - jj * na;
+    ix[1] = jj;
+    ix[2] = l;
+  }
+  if (side == 1) ix[d] = B.e[d] + 1 + ix[d];
+  i = ix[0];
+  j = ix[1];
+  k = ix[2];
+}
+
 // Physical boundaries normal to dimension d: outflow copies the edge cell;
 // reflecting mirrors and flips the d-component of u~ and B.  Covers the full
 // extent of the other dims, so X1, X2, X3 launches in turn fill corners.
+// One thread per cell of the two ghost slabs of every block (per_block).
 __global__ __launch_bounds__(256) void k_bc(KCtx C, KFields F, double *const *flag, int d, int bc_in,
-                                            int bc_out, int nbd, Range3 R) {
-  int b, k, j, i;
-  if (!decode_tile(R, C.nblocks, b, k, j, i)) return;
+                                            int bc_out, int nbd, int per_block) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (int64_t)C.nblocks * per_block) return;
+  const int b = (int)(gid / per_block);
   const DevBlock &B = C.blk[b];
-  int ix[3] = {i, j, k};
-  int side, kind;
-  if (ix[d] < B.s[d] && B.lx[d] == 0 && bc_in != PANGU_BC_PERIODIC) {
-    side = 0;
+  int side, k, j, i;
+  bc_cell(B, d, (int)(gid - (int64_t)b * per_block), side, k, j, i);
+  int kind;
+  if (side == 0) {
+    if (B.lx[d] != 0 || bc_in == PANGU_BC_PERIODIC) return;
     kind = bc_in;
-  } else if (ix[d] > B.e[d] && B.lx[d] == nbd - 1 && bc_out != PANGU_BC_PERIODIC) {
-    side = 1;
-    kind = bc_out;
   } else {
-    return;
+    if (B.lx[d] != nbd - 1 || bc_out == PANGU_BC_PERIODIC) return;
+    kind = bc_out;
   }
+  const int ix[3] = {i, j, k};
   int sx[3] = {i, j, k};
   if (kind == PANGU_BC_OUTFLOW)
     sx[d] = side == 0 ? B.s[d] : B.e[d];
@@ -792,6 +882,7 @@ static Range3 make_range(int l0, int h0, int l1, int h1, int l2, int h2) {
   R.lo[1] = l1; R.hi[1] = h1;
   R.lo[2] = l2; R.hi[2] = h2;
   for (int d = 0; d < 3; ++d) R.cnt[d] = R.hi[d] - R.lo[d] + 1;
+  R.order = 0;
   return R;
 }
 static Range3 interior_range(const DevBlock &B) {
@@ -832,7 +923,8 @@ KernelClock::~KernelClock() {
 
 int launch_fix_cons(const LaunchCtx &L, const KFields &F, uint8_t *const *mask, bool fix, bool cons,
                     hipStream_t s) {
-  const Range3 R = fix ? entire_range(L.hblk) : interior_range(L.hblk);
+  Range3 R = fix ? entire_range(L.hblk) : interior_range(L.hblk);
+  R.order = L.order[PANGU_KF_FIX_CONS];
   const dim3 g = tile_grid(R, L.k.nblocks);
   const bool hb = L.k.iBX >= 0, hk = L.k.iKEL >= 0;
 #define FC(FX, CS, HB_, HK_) hipLaunchKernelGGL((k_fix_cons<FX, CS, HB_, HK_>), g, kTile, 0, s, L.k, F, mask, R)
@@ -857,6 +949,7 @@ int launch_fix_cons(const LaunchCtx &L, const KFields &F, uint8_t *const *mask, 
 template <int DIR>
 static void flux_dir(const LaunchCtx &L, const KFields &F, double *const *alfven, Range3 R,
                      hipStream_t s) {
+  R.order = L.order[PANGU_KF_FLUX_X1 + DIR - 1];
   const dim3 g = tile_grid(R, L.k.nblocks);
   const bool hb = L.k.iBX >= 0, hk = L.k.iKEL >= 0;
   const int lim = L.limiter, rs = L.riemann;
@@ -914,7 +1007,8 @@ int launch_fluxes(const LaunchCtx &L, const KFields &F, double *const *alfven, h
 int launch_ct(const LaunchCtx &L, const KFields &F, double *const *emf, hipStream_t s) {
   const DevBlock &B = L.hblk;
   const int o1 = B.n[0] > 1, o2 = B.n[1] > 1, o3 = B.n[2] > 1;
-  const Range3 R = make_range(B.s[0], B.e[0] + o1, B.s[1], B.e[1] + o2, B.s[2], B.e[2] + o3);
+  Range3 R = make_range(B.s[0], B.e[0] + o1, B.s[1], B.e[1] + o2, B.s[2], B.e[2] + o3);
+  R.order = L.order[PANGU_KF_CT];
   const dim3 g = tile_grid(R, L.k.nblocks);
   hipLaunchKernelGGL(k_emf, g, kTile, 0, s, L.k, F, emf, R, o2 && o3, o3 && o1, o1 && o2);
   TICK("ct_emf");
@@ -926,7 +1020,8 @@ int launch_ct(const LaunchCtx &L, const KFields &F, double *const *emf, hipStrea
 int launch_div_update(const LaunchCtx &L, int mode, const KFields &F0, const KFields &Finit,
                       const KFields &F1, double *const *dudt, const double *dt_dev, double beta,
                       double bdt_host, hipStream_t s) {
-  const Range3 R = interior_range(L.hblk);
+  Range3 R = interior_range(L.hblk);
+  R.order = L.order[PANGU_KF_DIV_UPDATE];
   const dim3 g = tile_grid(R, L.k.nblocks);
   const bool hb = L.k.iBX >= 0, hk = L.k.iKEL >= 0;
   const bool src = L.metric != PANGU_METRIC_MINKOWSKI;
@@ -946,6 +1041,7 @@ int launch_div_update(const LaunchCtx &L, int mode, const KFields &F0, const KFi
         DU2(3, true)
       }
       break;
+    case 5: DU2(4, false) break;
     default:
       if (src) {
         DU2(4, true)
@@ -956,13 +1052,18 @@ int launch_div_update(const LaunchCtx &L, int mode, const KFields &F0, const KFi
 #undef DU2
 #undef DU
   if (mode != 3 || src)
-    TICK(mode == 1 ? "flux_divergence" : mode == 2 ? "update" : mode == 3 ? "source" : "div_update_source");
+    TICK(mode == 1   ? "flux_divergence"
+         : mode == 2 ? "update"
+         : mode == 3 ? "source"
+         : mode == 5 ? "div_update"
+                     : "div_update_source");
   return PG_CHECK_LAUNCH();
 }
 
 int launch_recover(const LaunchCtx &L, const KFields &F, const SharedTabs &S, int rec, bool heat,
                    hipStream_t s) {
-  const Range3 R = interior_range(L.hblk);
+  Range3 R = interior_range(L.hblk);
+  R.order = L.order[PANGU_KF_RECOVERY];
   const dim3 g = tile_grid(R, L.k.nblocks);
   const bool hb = L.k.iBX >= 0, hk = L.k.iKEL >= 0;
 #define RC(R_, HB_, HK_, HE_) \
@@ -1003,18 +1104,26 @@ int launch_fix_recovery(const LaunchCtx &L, const KFields &F, double *const *fla
 
 int launch_exchange(const LaunchCtx &L, const KFields &F, double *const *flag, const int *nbr,
                     hipStream_t s) {
-  const Range3 R = entire_range(L.hblk);
-  hipLaunchKernelGGL(k_exchange, tile_grid(R, L.k.nblocks), kTile, 0, s, L.k, F, flag, nbr, R);
+  const DevBlock &B = L.hblk;
+  const int interior = (B.e[0] - B.s[0] + 1) * (B.e[1] - B.s[1] + 1) * (B.e[2] - B.s[2] + 1);
+  const int nghost = (int)B.ncell - interior;
+  const int64_t n = (int64_t)L.k.nblocks * nghost;
+  if (n > 0)
+    hipLaunchKernelGGL(k_exchange, dim3((unsigned)((n + TILE - 1) / TILE)), kTile, 0, s, L.k, F, flag,
+                       nbr, nghost);
   TICK("exchange");
   return PG_CHECK_LAUNCH();
 }
 
 int launch_bc(const LaunchCtx &L, const KFields &F, double *const *flag, hipStream_t s) {
-  const Range3 R = entire_range(L.hblk);
-  const dim3 g = tile_grid(R, L.k.nblocks);
+  const DevBlock &B = L.hblk;
   for (int d = 0; d < L.k.ndim; ++d) {
     if (L.bc[d][0] == PANGU_BC_PERIODIC && L.bc[d][1] == PANGU_BC_PERIODIC) continue;
-    hipLaunchKernelGGL(k_bc, g, kTile, 0, s, L.k, F, flag, d, L.bc[d][0], L.bc[d][1], L.nb[d], R);
+    const int per_block = 2 * B.s[d] * (d == 0 ? B.n[1] * B.n[2] : d == 1 ? B.n[0] * B.n[2] : B.n[0] * B.n[1]);
+    const int64_t n = (int64_t)L.k.nblocks * per_block;
+    if (n == 0) continue;
+    hipLaunchKernelGGL(k_bc, dim3((unsigned)((n + TILE - 1) / TILE)), kTile, 0, s, L.k, F, flag, d,
+                       L.bc[d][0], L.bc[d][1], L.nb[d], per_block);
     TICK(d == 0 ? "bc_x1" : d == 1 ? "bc_x2" : "bc_x3");
   }
   return PG_CHECK_LAUNCH();

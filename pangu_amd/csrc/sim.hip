@@ -246,18 +246,25 @@ int pangu_sim_create(const pangu_sim_cfg *c, int device, pangu_sim **out) {
   return PANGU_OK;
 }
 
+// Drops the captured cycle graph (after a change of launch parameters).
+static void drop_graph(pangu_sim *s) {
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  s->gexec = nullptr;
+  s->graph = nullptr;
+}
+
 int pangu_sim_destroy(pangu_sim *s) {
   if (!s) return 0;
-  hipSetDevice(s->device);
-  if (s->stream) hipStreamSynchronize(s->stream);
-  if (s->gexec) hipGraphExecDestroy(s->gexec);
-  if (s->graph) hipGraphDestroy(s->graph);
+  (void)hipSetDevice(s->device);
+  drop_graph(s);
   void *ptrs[] = {s->P[0], s->P[1], s->U[0], s->U[1], s->dudt, s->F[0], s->F[1], s->F[2],
                   s->alfven, s->emf, s->flag, s->flag_rec, s->mask, s->status, s->iters, s->dstate};
   for (void *p : ptrs)
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   if (s->ctx) pangu_destroy(s->ctx);
-  if (s->stream) hipStreamDestroy(s->stream);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
   return 0;
 }
@@ -438,6 +445,18 @@ int pangu_sim_time_kernels(pangu_sim *s, int ncycles, double *ms_out, int *calls
     calls_out[j] = calls[j] / ncycles;
   }
   return nk;
+}
+
+int pangu_sim_set_tile_order(pangu_sim *s, int fam, int order) {
+  PCHK(s, pangu_set_tile_order(s->ctx, fam, order));
+  drop_graph(s);
+  return 0;
+}
+
+int pangu_sim_set_split_source(pangu_sim *s, int split) {
+  PCHK(s, pangu_set_split_source(s->ctx, split));
+  drop_graph(s);
+  return 0;
 }
 
 const char *pangu_sim_kernel_name(pangu_sim *s, int n) {

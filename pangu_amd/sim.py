@@ -44,6 +44,7 @@ FIELDS = dict(prim=0, cons=1, flux1=2, flux2=3, flux3=4, alfven=5, emf=6, flag=7
 TASKS = dict(fix_primitive=0, conservative=1, fluxes=2, ct=3, flux_div=4, update=5, source=6,
              recovery=7, heating=8, fix_recovery=9, exchange=10, bc=11, timestep=12)
 MODES = dict(tasks=0, fused=1, graph=2)
+KERNEL_FAMILIES = dict(fix_cons=0, flux_x1=1, flux_x2=2, flux_x3=3, ct=4, div_update=5, recovery=6)
 
 _lib = None
 
@@ -76,6 +77,8 @@ def lib():
         L.pangu_sim_dt.restype = C.c_double
         L.pangu_sim_dt.argtypes = [vp]
         L.pangu_sim_block_geom.argtypes = [vp, C.c_int, dp, ip]
+        L.pangu_sim_set_tile_order.argtypes = [vp, C.c_int, C.c_int]
+        L.pangu_sim_set_split_source.argtypes = [vp, C.c_int]
         L.pangu_sim_time_kernels.argtypes = [vp, C.c_int, dp, ip, C.c_int]
         L.pangu_sim_kernel_name.restype = C.c_char_p
         L.pangu_sim_kernel_name.argtypes = [vp, C.c_int]
@@ -176,6 +179,17 @@ class HipSim:
             self._check(n, "time_kernels")
         return {lib().pangu_sim_kernel_name(self._p, i).decode(): (int(calls[i]), float(ms[i]))
                 for i in range(min(n, 64))}
+
+    def set_tile_order(self, family, order):
+        """Tile order of one kernel family (KERNEL_FAMILIES; 0 plane-major,
+        1 column-major).  Results are unchanged; only speed differs."""
+        fam = KERNEL_FAMILIES[family] if isinstance(family, str) else int(family)
+        self._check(lib().pangu_sim_set_tile_order(self._p, fam, int(order)), "set_tile_order")
+
+    def set_split_source(self, split):
+        """Fused stage: geometric source in a kernel of its own (1) or fused
+        with the flux divergence and update (0).  Results are unchanged."""
+        self._check(lib().pangu_sim_set_split_source(self._p, int(split)), "set_split_source")
 
     def copy_calibration(self, n, reps=10):
         ms = C.c_double(0.0)
