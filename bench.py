@@ -60,6 +60,11 @@ def parse():
     ap.add_argument("--small", action="store_true", help="reduced mesh for quick checks")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-cycles", type=int, default=2)
+    ap.add_argument("--ensemble-scale", type=float, default=0.25,
+                    help="fraction of the 96-torus + 3x121-block ensemble for the batched "
+                         "vs plain comparison (0 skips it)")
+    ap.add_argument("--learned-samples", type=int, default=1 << 20,
+                    help="training states for the learned Kastaun bracket (0 skips it)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="also run a plain copy kernel of this many doubles (counter calibration)")
     return ap.parse_args()
@@ -95,6 +100,73 @@ def cpu_baseline(cfg_small, cycles):
                    f"{d['nb'][0] * d['nb'][1] * d['nb'][2]} blocks, "
                    f"OpenMP {nthreads} threads, metric recomputed per evaluation; {dt:.2f} s"),
     }
+
+
+def batched_vs_plain(scale, cycles=10, warmup=2):
+    """The ensemble as one batched context vs the same meshes as separate
+    simulations run in turn on one stream (each replaying its own graph)."""
+    from pangu_amd import configs
+    from pangu_amd.sim import HipSim, plain_loop
+    ens = configs.ensemble(seed=0, n_tori=max(1, round(96 * scale)),
+                           n_boxes_each=max(1, round(121 * scale)))
+    zones = sum(configs.zones(d) for d in ens)
+    t0 = time.perf_counter()
+    batch = HipSim(ens)
+    batch.init()
+    t_init = time.perf_counter() - t0
+    tune = batch.autotune(cycles=2)
+    batch.step(warmup, "graph")
+    ms_b = batch.timed_cycles(cycles, "graph")
+    batch.close()
+    singles = [HipSim(d) for d in ens]
+    for s in singles:
+        s.init()
+    plain_loop(singles, warmup, "graph")
+    ms_p = plain_loop(singles, cycles, "graph")
+    for s in singles:
+        s.close()
+    return {
+        "meshes": len(ens), "zones": zones, "cycles": cycles,
+        "batched_zone_cycles_per_s": zones * cycles / (ms_b * 1e-3),
+        "plain_zone_cycles_per_s": zones * cycles / (ms_p * 1e-3),
+        "speedup": ms_p / ms_b, "batched_ms_per_cycle": ms_b / cycles,
+        "plain_ms_per_cycle": ms_p / cycles, "init_s": t_init, "autotune": tune,
+        "workload": (f"{sum(1 for d in ens if d['nx'][2] == 1)} 2D MKS tori (128x64..384x192) + "
+                     f"{sum(1 for d in ens if d['nx'][2] > 1)} single-block 3D tori (16^3, 32^3, 64^3), "
+                     "random a, beta, rin; PPM4+HLL+Kastaun+CT+heating"),
+    }
+
+
+def learned_vs_plain(cfg, samples, cycles=20, delta=0.02):
+    """Learned Kastaun bracket: trained on synthetic states, evaluated on a
+    held-out set and in the 3D torus run (iterations and speed)."""
+    from pangu_amd import mlp
+    from pangu_amd.sim import HipSim
+    m = mlp.Mlp(seed=1)
+    t0 = time.perf_counter()
+    m.make_dataset(samples, seed=11)
+    loss = m.train(epochs=8, batch=2048, lr=2e-3)
+    t_train = time.perf_counter() - t0
+    m.make_dataset(max(samples // 4, 1 << 16), seed=12)   # held-out states
+    ev = m.evaluate(delta=delta)
+    out = {"train_states": samples, "train_s": t_train, "loss_first_last": [float(loss[0]), float(loss[-1])],
+           "heldout": ev, "delta": delta,
+           "iteration_reduction": 1.0 - ev["learned_iters"] / ev["plain_iters"]}
+    zones = cfg["nx"][0] * cfg["nx"][1] * cfg["nx"][2]
+    for learned in (False, True):
+        h = HipSim(cfg)
+        h.init()
+        if learned:
+            m.attach(h, delta=delta)
+        h.step(2, "graph")
+        ms = h.timed_cycles(cycles, "graph")
+        it = float(h.interior(h.get("c2p_iters")).mean())
+        key = "learned" if learned else "plain"
+        out[f"torus3d_{key}_zone_cycles_per_s"] = zones * cycles / (ms * 1e-3)
+        out[f"torus3d_{key}_mean_iters"] = it
+        h.close()
+    m.close()
+    return out
 
 
 def main():
@@ -177,6 +249,10 @@ def main():
         cms = sim.copy_calibration(n, reps=5)
         out["copy_calibration"] = {"doubles": n, "ms": cms, "GBps": 16.0 * n / (cms * 1e-3) / 1e9}
     sim.close()
+    if args.ensemble_scale > 0:
+        out["batched"] = batched_vs_plain(args.ensemble_scale)
+    if args.learned_samples > 0:
+        out["learned"] = learned_vs_plain(cfg, args.learned_samples)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_sample_cycles)
     print(json.dumps(out))

@@ -106,6 +106,17 @@ typedef struct pangu_ctx pangu_ctx;
  * builds the FP64 geometry tables on the host and uploads them. */
 int pangu_create(const pangu_config *cfg, const pangu_block *blocks, int nblocks, int device,
                  pangu_ctx **out);
+/* Batched form (no reference counterpart): one context over several
+ * independent meshes, e.g. an ensemble of tori plus 3-D boxes, whose blocks
+ * may differ in shape between meshes.  cfgs[m] configures mesh m; they must
+ * agree on every physics parameter except the metric's a and h, the block
+ * topology nb and the boundaries bc.  Blocks are grouped by mesh:
+ * block_mesh[b] is non-decreasing from 0 to nmesh - 1, and the blocks of one
+ * mesh share a shape.  Each kernel launch then covers every block of every
+ * mesh; ghost exchange stays inside a mesh and each mesh keeps its own dt.
+ * pangu_create(cfg, blocks, n, dev, out) is the one-mesh case. */
+int pangu_create_batch(const pangu_config *cfgs, int nmesh, const pangu_block *blocks,
+                       const int *block_mesh, int nblocks, int device, pangu_ctx **out);
 int pangu_destroy(pangu_ctx *ctx);
 const char *pangu_last_error(const pangu_ctx *ctx);
 
@@ -146,7 +157,8 @@ int pangu_apply_electron_heating(pangu_ctx *ctx, int md, void *stream);
 /* FixRecovery (fixer/recovery_fixer.cc:12), serial lexicographic semantics */
 int pangu_fix_recovery(pangu_ctx *ctx, int md, void *stream);
 /* EstimateTimestepMesh (initialization/timestep_estimation.cc:13); synchronous
- * like the reference's par_reduce: writes the estimate to *dt_out. */
+ * like the reference's par_reduce: writes the estimate of mesh m to
+ * dt_out[m] (one value for a one-mesh context). */
 int pangu_estimate_timestep(pangu_ctx *ctx, int md, double *dt_out, void *stream);
 
 /* ---- Parthenon-side pieces the path needs on one GPU ---- */
@@ -161,9 +173,10 @@ int pangu_apply_boundary_conditions(pangu_ctx *ctx, int md, void *stream);
  * Runs region 1 + region 2 of one RK stage (ideal_grmhd.cc:52-116) with
  * fused kernels: fix+cons, three flux sweeps, CT, flux-div+update+source,
  * exchange, recovery+heating, fix_recovery and boundaries.  dt_state is a
- * device array {dt, time}: the stage reads beta*dt from it, and when
- * last_stage != 0 it also estimates the timestep and applies the dt control
- * time += dt, dt = min(dt_factor*dt, estimate) in place.  Nothing is read
+ * device array {dt, time} per mesh: the stage reads each mesh's beta*dt
+ * from it, and when last_stage != 0 it also estimates the timestep and
+ * applies the dt control time += dt, dt = min(dt_factor*dt, estimate) in
+ * place.  Nothing is read
  * back to the host, so a whole cycle can be captured in a hipGraph and
  * replayed.  exchange_after: 0 = reference order (after the update),
  * 1 = after recovery. */

@@ -1,0 +1,95 @@
+/*
+ * pangu_mlp: learned starting bracket for the Kastaun primitive recovery
+ * (no reference counterpart).
+ *
+ * scheme_kastaun.h:110-165 searches the root mu of its master function by
+ * false position on the fixed bracket [0, 1], with no initial guess.  This
+ * module trains a small network, 5 -> 64 -> 64 -> 64 -> 1 with SiLU, to
+ * predict mu from five dimensionless invariants of the conserved state, and
+ * lets the recovery kernel start from [g (1 - delta), g (1 + delta)] around
+ * the prediction g, with a sign check and a fall-back to [0, 1].  Weights
+ * and activations enter the matrix products as BF16 on gfx950 MFMA units
+ * with FP32 accumulation; FP32 master weights are trained with Adam.
+ *
+ * Features of a state (from D, q = tau/D, r^2 = S^2/D^2, b^2 = B^2/D and
+ * (r.b)^2 of the Kastaun setup): log(1 + max(q, -0.5)), log(1 + r^2),
+ * log(1 + b^2), log(1 + (r.b)^2) and (r.b)^2 / (r^2 b^2 + 1e-300) clamped
+ * to [0, 1], standardised with the training set's mean and deviation.
+ * Target: logit(mu); the prediction is g = 1 / (1 + exp(-z)).
+ */
+#ifndef PANGU_MLP_H
+#define PANGU_MLP_H
+#include <stdint.h>
+
+#include "pangu_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { PANGU_MLP_IN = 5, PANGU_MLP_HIDDEN = 64 };
+
+typedef struct pangu_mlp pangu_mlp;
+
+/* Parameter layout of pangu_mlp_get/set_params (FP32, this order):
+ * W1[5][64], b1[64], W2[64][64], b2[64], W3[64][64], b3[64], W4[64], b4,
+ * feature mean[5], feature deviation[5]: PANGU_MLP_NPARAM floats. */
+enum { PANGU_MLP_NPARAM = 5 * 64 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1 + 5 + 5 };
+
+int pangu_mlp_create(int device, uint64_t seed, pangu_mlp **out);
+int pangu_mlp_destroy(pangu_mlp *m);
+const char *pangu_mlp_last_error(const pangu_mlp *m);
+int pangu_mlp_get_params(pangu_mlp *m, float *out);
+int pangu_mlp_set_params(pangu_mlp *m, const float *in);
+
+/* Synthetic recovery states on the GPU (the training / evaluation set):
+ * rho ~ logU(1e-6, 1), u/rho ~ logU(1e-4, 10), W ~ U(1, 20),
+ * sigma = b^2/rho ~ logU(1e-3, 1e2), random directions of u~ and B, Kerr
+ * metric in MKS coordinates (a, h) at r ~ U(1.3, 50), x2 ~ U(-0.98, 0.98),
+ * gamma as given.  Each state is converted to conserved variables and
+ * solved by the plain Kastaun recovery; states it solves (status 0) are
+ * kept with their root as target.  Returns the number kept in *kept.  A
+ * new call replaces the previous set. */
+int pangu_mlp_make_dataset(pangu_mlp *m, int64_t n, uint64_t seed, double a, double h,
+                           double gamma, int64_t *kept);
+
+/* Trains on the current data set: `epochs` passes in minibatches of `batch`
+ * (a multiple of 32) with Adam at learning rate lr; the feature
+ * standardisation is fixed from the data set on the first call.
+ * loss_out (optional, epochs floats) receives each epoch's mean loss. */
+int pangu_mlp_train(pangu_mlp *m, int epochs, int batch, float lr, float *loss_out);
+
+/* One forward and backward pass on host data (n a multiple of 32): feats
+ * [n][5] raw features, target [n]; returns the loss and the FP32 gradient
+ * of every parameter (PANGU_MLP_NPARAM floats; the statistics' entries are
+ * zero).  For gradient checks. */
+int pangu_mlp_gradient(pangu_mlp *m, const float *feats, const float *target, int n, float *loss,
+                       float *grad);
+
+/* Forward pass on host features [n][5]; out[n] receives g in (0, 1). */
+int pangu_mlp_predict(pangu_mlp *m, const float *feats, int n, float *out);
+
+/* Recovery of the current data set by plain Kastaun and by the learned
+ * bracket (delta as in pangu_set_c2p_guess): mean false-position
+ * iterations and root evaluations of each, the largest |f(mu)| of the
+ * learned roots, the largest relative difference of the recovered rho, u
+ * and u~ between the two, and the number of states the learned path failed
+ * on.  out: {plain_iters, learned_iters, plain_evals, learned_evals,
+ * max_residual, max_rel_diff, failures}. */
+int pangu_mlp_evaluate(pangu_mlp *m, double delta, double *out);
+
+/* C = A B for host FP32 matrices A [M][K] and B [K][N] (row-major), rounded
+ * to BF16 and multiplied on the matrix cores with FP32 accumulation (the
+ * product every layer uses).  M, N multiples of 32, K of 16. */
+int pangu_mlp_gemm(int device, int M, int N, int K, const float *A, const float *B, float *C);
+
+/* Attaches the model to a context: its Kastaun recovery then starts from
+ * the learned bracket of half-width delta (relative).  NULL detaches.  The
+ * model must outlive the attachment.  Runs with a model attached are off the
+ * bit-for-bit parity path (results agree to the solver tolerance). */
+int pangu_set_c2p_guess(pangu_ctx *ctx, pangu_mlp *m, double delta);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PANGU_MLP_H */

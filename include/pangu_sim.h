@@ -13,6 +13,7 @@
 #include <stdint.h>
 
 #include "pangu_hip.h"
+#include "pangu_mlp.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -53,6 +54,16 @@ enum { PANGU_SIM_TASKS = 0, PANGU_SIM_FUSED = 1, PANGU_SIM_GRAPH = 2 };
 typedef struct pangu_sim pangu_sim;
 
 int pangu_sim_create(const pangu_sim_cfg *cfg, int device, pangu_sim **out);
+/* An ensemble of independent meshes in one context (pangu_create_batch):
+ * every kernel launch covers all of them, each mesh keeps its own dt.  The
+ * meshes must share the physics options, dt_factor, stage_init and
+ * exchange_after; mesh m's blocks follow mesh m-1's in the arenas.  Task
+ * mode (host dt) needs a single mesh. */
+int pangu_sim_create_batch(const pangu_sim_cfg *cfgs, int nmesh, int device, pangu_sim **out);
+/* dt, time, first block and block count of mesh m. */
+int pangu_sim_mesh_state(pangu_sim *s, int m, double *dt, double *time, int *first_block,
+                         int *nblocks);
+int pangu_sim_nblocks(pangu_sim *s);
 int pangu_sim_destroy(pangu_sim *s);
 int pangu_sim_init(pangu_sim *s);
 /* Runs ncycles RK2 cycles in the given mode (tasks: one C-ABI call per
@@ -75,6 +86,20 @@ const char *pangu_sim_phase_name(int n);
 int pangu_sim_time_kernels(pangu_sim *s, int ncycles, double *ms_out, int *calls_out,
                            int max_kernels);
 const char *pangu_sim_kernel_name(pangu_sim *s, int n);
+/* The loop a caller would write around the one-mesh API: ncycles cycles,
+ * each running every simulation's cycle in turn (mode fused or graph), all
+ * enqueued on the first simulation's stream.  ms = elapsed time between
+ * HIP events around the loop. */
+int pangu_sim_plain_loop(pangu_sim **sims, int n, int ncycles, int mode, double *ms);
+/* Autotuner: for each kernel family the faster tile order, then whether to
+ * split the source kernel, each candidate timed with the kernel clock over
+ * `cycles` fused cycles.  Every field, dt and time is restored afterwards,
+ * so tuning does not advance the run.  With use_cache != 0 a result stored
+ * for the same block shapes and options in this process is reused without
+ * measuring.  out_orders (PANGU_KF_COUNT ints) and out_split may be NULL. */
+int pangu_sim_autotune(pangu_sim *s, int cycles, int use_cache, int *out_orders, int *out_split);
+/* pangu_set_c2p_guess on the simulation's context (drops the graph). */
+int pangu_sim_set_c2p_guess(pangu_sim *s, pangu_mlp *m, double delta);
 /* Sets a kernel family's tile order (pangu_set_tile_order) and drops the
  * captured graph so the next graph-mode step captures the new launches. */
 int pangu_sim_set_tile_order(pangu_sim *s, int kernel_family, int order);

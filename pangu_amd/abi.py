@@ -67,6 +67,8 @@ _vp, _dp, _ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
 PROTOTYPES = {
     "pangu_create": (C.c_int, [C.POINTER(PanguConfig), C.POINTER(PanguBlock), C.c_int, C.c_int,
                                C.POINTER(_vp)]),
+    "pangu_create_batch": (C.c_int, [C.POINTER(PanguConfig), C.c_int, C.POINTER(PanguBlock),
+                                     C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(_vp)]),
     "pangu_destroy": (C.c_int, [_vp]),
     "pangu_last_error": (C.c_char_p, [_vp]),
     "pangu_geometry_entry": (C.c_int, [C.POINTER(PanguConfig), C.c_int, _dp, _dp]),

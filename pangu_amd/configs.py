@@ -65,3 +65,39 @@ def torus3d(nx=(256, 128, 128), nb=(4, 2, 4)):
 
 def zones(d):
     return d["nx"][0] * d["nx"][1] * d["nx"][2]
+
+
+def ensemble(seed=0, n_tori=96, n_boxes_each=121, sizes3d=(16, 32, 64),
+             tori_res=((128, 64), (256, 128), (384, 192))):
+    """Batched workload: independent MKS tori in one context.
+
+    n_tori 2D Fishbone-Moncrief tori with parameters drawn from a fixed
+    seed: a ~ U(0, 0.99), beta ~ logU(30, 300), rin ~ U(5, 7), rmax = rin + 6,
+    resolution uniform over tori_res, inner radius 0.9 r_H(a) (inside the
+    horizon: x1min = ln(0.9 (1 + sqrt(1 - a^2)))), each with its own
+    perturbation seed.  Plus n_boxes_each single-block 3D tori at each of
+    the cube sizes in sizes3d (same draws).  Physics as the 3D torus
+    configuration: PPM4, HLL, Kastaun, CT, heating.  The defaults give
+    ~3.7e6 zones of tori and ~3.6e7 zones of 3D blocks."""
+    import random
+    rng = random.Random(seed)
+
+    def one(nx, nb):
+        a = rng.uniform(0.0, 0.99)
+        beta = math.exp(rng.uniform(math.log(30.0), math.log(300.0)))
+        rin = rng.uniform(5.0, 7.0)
+        rh = 1.0 + math.sqrt(1.0 - a * a)
+        d = _torus_common(dict(_BASE))
+        d.update(nx=list(nx), nb=list(nb), riemann=1, limiter=1, recovery=1, a=a, beta=beta,
+                 rin=rin, rmax=rin + 6.0, seed=rng.randrange(1 << 62))
+        d["xmin"] = [math.log(0.9 * rh), -1.0, -3.1415926535897]
+        return d
+
+    out = []
+    for _ in range(n_tori):
+        res = tori_res[rng.randrange(len(tori_res))]
+        out.append(one((res[0], res[1], 1), (1, 1, 1)))
+    for n in sizes3d:
+        for _ in range(n_boxes_each):
+            out.append(one((n, n, n), (1, 1, 1)))
+    return out

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/pangu_hip.h"
+#include "../../include/pangu_mlp.h"
 #include "common.h"
 #include "host.h"
 #include "kernels.h"
@@ -28,10 +29,15 @@ struct DevContainer {
 };
 
 struct pangu_ctx {
-  pangu_config cfg;
+  pangu_config cfg;                 // physics shared by every mesh (mesh 0's)
+  std::vector<pangu_config> mcfg;   // per mesh: metric parameters, topology, boundaries
+  std::vector<int> bmesh;           // mesh of each block
+  std::vector<int> mfirst;          // first block of each mesh (nmesh + 1)
   int device = 0;
+  int nmesh = 1, nplanes = 0;
   int nblocks = 0, ndim = 1, np = 0, nc = 0, iBX = -1, iENT = 0, iKEL = -1;
   std::vector<HostBlock> hb;
+  std::vector<int *> tab_bufs;      // launch tables
   DevBlock *dblk = nullptr;
   DevGeo *dgeo = nullptr;
   std::vector<double *> geo_bufs;
@@ -42,6 +48,11 @@ struct pangu_ctx {
   uint8_t **dmask = nullptr;
   int32_t **dstatus = nullptr, **diters = nullptr;
   bool shared_ok = false;
+  // learned Kastaun bracket (pangu_set_c2p_guess)
+  pangu_mlp *mlp = nullptr;
+  double guess_delta = 0.0;
+  float *dguess = nullptr;             // one float per cell of every block
+  float **dguess_tab = nullptr;        // [nblocks] pointers into dguess
   // scratch
   double *dred = nullptr;
   int *dcluster = nullptr;
@@ -95,7 +106,12 @@ static int validate(const pangu_config *cfg, const pangu_block *blocks, int nb, 
   for (int b = 0; b < nb; ++b)
     for (int d = 0; d < 3; ++d) {
       if (blocks[b].nx[d] != blocks[0].nx[d] || blocks[b].ng[d] != blocks[0].ng[d]) {
-        msg = "all meshblocks of a context must share one shape";
+        msg = "all meshblocks of a mesh must share one shape";
+        return 0;
+      }
+      if (blocks[b].nx[d] < 1 || (blocks[b].nx[d] > 1 && blocks[b].lx[d] >= cfg->nb[d]) ||
+          blocks[b].lx[d] < 0) {
+        msg = "block extent or logical location out of range";
         return 0;
       }
       if (blocks[b].nx[d] > 1 && blocks[b].ng[d] < (cfg->limiter == PANGU_LIMITER_PPM4 ? 3 : 2)) {
@@ -106,33 +122,84 @@ static int validate(const pangu_config *cfg, const pangu_block *blocks, int nb, 
   return 1;
 }
 
+// Every mesh of a batched context runs the same kernels: only the metric
+// parameters (a, h), the block topology (nb) and the boundaries may differ.
+static int same_physics(const pangu_config &x, const pangu_config &y) {
+  return x.metric == y.metric && x.gamma == y.gamma && x.cfl == y.cfl && x.riemann == y.riemann &&
+         x.limiter == y.limiter && x.recovery == y.recovery && x.enable_B == y.enable_B &&
+         x.enable_heating == y.enable_heating && x.density_floor == y.density_floor &&
+         x.energy_floor == y.energy_floor && x.density_floor_pow == y.density_floor_pow &&
+         x.energy_floor_pow == y.energy_floor_pow && x.sigma_max == y.sigma_max &&
+         x.lorentz_max == y.lorentz_max && x.heat_model == y.heat_model &&
+         x.fel_constant == y.fel_constant && x.gamma_e == y.gamma_e && x.gamma_p == y.gamma_p &&
+         x.ratio_min == y.ratio_min && x.ratio_max == y.ratio_max && x.limit_kel == y.limit_kel &&
+         x.suppress_highb_heat == y.suppress_highb_heat &&
+         x.enforce_positive_dissipation == y.enforce_positive_dissipation && x.excise == y.excise &&
+         x.r_excise == y.r_excise && x.dexcise == y.dexcise && x.pexcise == y.pexcise;
+}
+
+static int validate_batch(const pangu_config *cfgs, int nmesh, const pangu_block *blocks,
+                          const int *bmesh, int nblocks, std::string &msg) {
+  if (!cfgs || nmesh <= 0 || !blocks || nblocks < nmesh) {
+    msg = "need at least one block per mesh";
+    return 0;
+  }
+  int first = 0;
+  for (int m = 0; m < nmesh; ++m) {
+    int last = first;
+    while (last < nblocks && (bmesh ? bmesh[last] : 0) == m) ++last;
+    if (last == first) {
+      msg = "blocks must be grouped by mesh in increasing mesh order, each mesh non-empty";
+      return 0;
+    }
+    if (!validate(&cfgs[m], blocks + first, last - first, msg)) return 0;
+    if (m > 0 && !same_physics(cfgs[0], cfgs[m])) {
+      msg = "meshes of one context must share every physics parameter except a, h, nb and bc";
+      return 0;
+    }
+    first = last;
+  }
+  if (first != nblocks) {
+    msg = "block mesh indices out of range";
+    return 0;
+  }
+  return 1;
+}
+
 // Geometry sets: blocks with identical x1/x2 footprints (and, for x3-dependent
-// metrics, identical x3 footprints) share one table set.
+// metrics, identical x3 footprints) and identical metric parameters share one
+// table set.
 static int build_geometry(pangu_ctx *c, const pangu_block *blocks) {
-  const pangu_config &cfg = c->cfg;
-  const bool has3d = (cfg.metric == PANGU_METRIC_CKS) && c->ndim == 3;
   std::map<std::vector<double>, int> key2geo;
   std::vector<DevGeo> geos;
   std::vector<DevBlock> db(c->nblocks);
   for (int b = 0; b < c->nblocks; ++b) {
+    const pangu_config &cfg = c->mcfg[c->bmesh[b]];
     const HostBlock &H = c->hb[b];
+    const bool has3d = (cfg.metric == PANGU_METRIC_CKS) && H.n[2] > 1;
     DevBlock &D = db[b];
+    std::memset(&D, 0, sizeof(D));
     for (int d = 0; d < 3; ++d) {
       D.n[d] = H.n[d];
       D.s[d] = H.s[d];
       D.e[d] = H.e[d];
       D.lx[d] = H.lx[d];
       D.dx[d] = H.dx[d];
+      const bool lo_edge = H.lx[d] == 0, hi_edge = H.lx[d] == cfg.nb[d] - 1;
+      D.bcf[d][0] = (H.n[d] > 1 && lo_edge && cfg.bc[d][0] != PANGU_BC_PERIODIC) ? cfg.bc[d][0] : -1;
+      D.bcf[d][1] = (H.n[d] > 1 && hi_edge && cfg.bc[d][1] != PANGU_BC_PERIODIC) ? cfg.bc[d][1] : -1;
     }
     D.ncell = H.cells;
-    std::vector<double> key = {H.xg[0], H.dx[0], (double)H.n[0], H.xg[1], H.dx[1], (double)H.n[1]};
+    D.mesh = c->bmesh[b];
+    std::vector<double> key = {(double)cfg.metric, cfg.a, cfg.h, H.xg[0], H.dx[0], (double)H.n[0],
+                               H.xg[1], H.dx[1], (double)H.n[1], (double)H.n[2]};
     if (has3d) {
       key.push_back(H.xg[2]);
       key.push_back(H.dx[2]);
-      key.push_back((double)H.n[2]);
     } else {
       key.push_back(H.xc(2, H.s[2]));
     }
+    // floors depend on the mesh's floor parameters, which all meshes share
     auto it = key2geo.find(key);
     if (it != key2geo.end()) {
       D.geo = it->second;
@@ -152,10 +219,11 @@ static int build_geometry(pangu_ctx *c, const pangu_block *blocks) {
     c->geo_bufs.push_back(dcen);
     G.cen = dcen;
     const int nfd = has3d ? 3 : 2;
+    const int bdim = H.n[2] > 1 ? 3 : (H.n[1] > 1 ? 2 : 1);
     for (int d = 0; d < 3; ++d) {
       G.fi[d] = G.ni + (d == 0 ? 1 : 0);
       G.fj[d] = G.nj + (d == 1 ? 1 : 0);
-      if (d >= c->ndim || d >= nfd) {
+      if (d >= bdim || d >= nfd) {
         G.face[d] = nullptr;
         G.nface[d] = 0;
         continue;
@@ -175,38 +243,165 @@ static int build_geometry(pangu_ctx *c, const pangu_block *blocks) {
   }
   if (upload(c, &c->dgeo, geos.data(), geos.size())) return PANGU_EDEVICE;
   if (upload(c, &c->dblk, db.data(), db.size())) return PANGU_EDEVICE;
-  c->L.hblk = db[0];
   return 0;
 }
 
 // Neighbour table for same-level exchange: 27 entries per block
 // (offset (o0,o1,o2) in {-1,0,1}^3), -1 where a physical boundary is.
+// Blocks only neighbour blocks of their own mesh.
 static int build_neighbours(pangu_ctx *c) {
-  const int *nb = c->cfg.nb;
   std::vector<int> nbr((size_t)c->nblocks * 27, -1);
   std::map<std::vector<int>, int> at;
-  for (int b = 0; b < c->nblocks; ++b) at[{c->hb[b].lx[0], c->hb[b].lx[1], c->hb[b].lx[2]}] = b;
   for (int b = 0; b < c->nblocks; ++b)
+    at[{c->bmesh[b], c->hb[b].lx[0], c->hb[b].lx[1], c->hb[b].lx[2]}] = b;
+  for (int b = 0; b < c->nblocks; ++b) {
+    const pangu_config &cfg = c->mcfg[c->bmesh[b]];
     for (int o2 = -1; o2 <= 1; ++o2)
       for (int o1 = -1; o1 <= 1; ++o1)
         for (int o0 = -1; o0 <= 1; ++o0) {
           const int o[3] = {o0, o1, o2};
-          std::vector<int> lx(3);
+          std::vector<int> key(4);
+          key[0] = c->bmesh[b];
           bool ok = true;
           for (int d = 0; d < 3; ++d) {
-            lx[d] = c->hb[b].lx[d] + o[d];
-            if (lx[d] < 0 || lx[d] >= nb[d]) {
-              if (c->cfg.bc[d][0] == PANGU_BC_PERIODIC)
-                lx[d] = (lx[d] + nb[d]) % nb[d];
+            int l = c->hb[b].lx[d] + o[d];
+            if (l < 0 || l >= cfg.nb[d]) {
+              if (cfg.bc[d][0] == PANGU_BC_PERIODIC)
+                l = (l + cfg.nb[d]) % cfg.nb[d];
               else
                 ok = false;
             }
+            key[d + 1] = l;
           }
           if (!ok) continue;
-          auto it = at.find(lx);
+          auto it = at.find(key);
           if (it != at.end()) nbr[(size_t)b * 27 + (o2 + 1) * 9 + (o1 + 1) * 3 + (o0 + 1)] = it->second;
         }
+  }
   return upload(c, &c->dnbr, nbr.data(), nbr.size());
+}
+
+// Uploads a tile -> block table and the per-block first tile (nblocks + 1).
+static int upload_tiles(pangu_ctx *c, const std::vector<int64_t> &tiles_per_block,
+                        const int **tblock, const int **tfirst, int *ntiles) {
+  std::vector<int> first(tiles_per_block.size() + 1, 0), owner;
+  int64_t n = 0;
+  for (size_t b = 0; b < tiles_per_block.size(); ++b) {
+    first[b] = (int)n;
+    for (int64_t t = 0; t < tiles_per_block[b]; ++t) owner.push_back((int)b);
+    n += tiles_per_block[b];
+  }
+  first[tiles_per_block.size()] = (int)n;
+  if (n > (int64_t)0x7fffff00) return fail(c, PANGU_EINVAL, "launch exceeds 2^31 tiles");
+  int *dt = nullptr, *df = nullptr;
+  if (upload(c, &dt, owner.data(), owner.size())) return PANGU_EDEVICE;
+  c->tab_bufs.push_back(dt);
+  if (upload(c, &df, first.data(), first.size())) return PANGU_EDEVICE;
+  c->tab_bufs.push_back(df);
+  *tblock = dt;
+  *tfirst = df;
+  *ntiles = (int)n;
+  return 0;
+}
+
+// Host replica of the device block_range: tiles of one block for a range.
+static int64_t range_tiles(const HostBlock &H, const int rlo[3], const int rhi[3], int entire) {
+  int cnt[3];
+  for (int d = 0; d < 3; ++d) {
+    int l, h;
+    if (entire) {
+      l = 0;
+      h = H.n[d] - 1;
+    } else if (H.n[d] > 1) {
+      l = H.s[d] + rlo[d];
+      h = H.e[d] + rhi[d];
+    } else {
+      l = H.s[d];
+      h = H.e[d];
+    }
+    cnt[d] = h - l + 1;
+    if (cnt[d] <= 0) return 0;
+  }
+  const int64_t plane = (int64_t)cnt[0] * cnt[1];
+  return (int64_t)cnt[2] * ((plane + 255) / 256);
+}
+
+static int make_range(pangu_ctx *c, Range3 &R, int l0, int l1, int l2, int h0, int h1, int h2,
+                      int entire, int need_dim) {
+  R.rlo[0] = l0;
+  R.rlo[1] = l1;
+  R.rlo[2] = l2;
+  R.rhi[0] = h0;
+  R.rhi[1] = h1;
+  R.rhi[2] = h2;
+  R.entire = entire;
+  R.order = 0;
+  std::vector<int64_t> tiles(c->nblocks);
+  for (int b = 0; b < c->nblocks; ++b)
+    tiles[b] = (need_dim >= 0 && c->hb[b].n[need_dim] == 1) ? 0 : range_tiles(c->hb[b], R.rlo, R.rhi, entire);
+  return upload_tiles(c, tiles, &R.tblock, &R.tfirst, &R.ntiles);
+}
+
+static int make_items(pangu_ctx *c, Items &I, const std::vector<int64_t> &items) {
+  std::vector<int64_t> tiles(items.size());
+  for (size_t b = 0; b < items.size(); ++b) tiles[b] = (items[b] + 255) / 256;
+  return upload_tiles(c, tiles, &I.tblock, &I.tfirst, &I.ntiles);
+}
+
+// Launch tables of every range the kernels use.  Face ranges follow the
+// three sweeps (laxf.cc:78-83,184-189,290-295): the normal index runs over
+// s..e+1 and transverse indices include one ghost layer on active dims,
+// except HLL's X3 sweep, which stays on the interior transversely
+// (hll.cc:295-296).  CT covers s..e+1 on active dims.
+static int build_tables(pangu_ctx *c) {
+  LaunchCtx &L = c->L;
+  int r = make_range(c, L.interior, 0, 0, 0, 0, 0, 0, 0, -1);
+  if (!r) r = make_range(c, L.entire, 0, 0, 0, 0, 0, 0, 1, -1);
+  if (!r) r = make_range(c, L.face[0], 0, -1, -1, 1, 1, 1, 0, 0);
+  if (!r) r = make_range(c, L.face[1], -1, 0, -1, 1, 1, 1, 0, 1);
+  if (!r) {
+    if (c->cfg.riemann == PANGU_RIEMANN_HLL)
+      r = make_range(c, L.face[2], 0, 0, 0, 0, 0, 1, 0, 2);
+    else
+      r = make_range(c, L.face[2], -1, -1, 0, 1, 1, 1, 0, 2);
+  }
+  if (!r) r = make_range(c, L.ct, 0, 0, 0, 1, 1, 1, 0, -1);
+  std::vector<int64_t> ghosts(c->nblocks), planes(c->nblocks);
+  for (int b = 0; b < c->nblocks; ++b) {
+    const HostBlock &H = c->hb[b];
+    const int64_t interior = (int64_t)(H.e[0] - H.s[0] + 1) * (H.e[1] - H.s[1] + 1) * (H.e[2] - H.s[2] + 1);
+    ghosts[b] = H.cells - interior;
+    planes[b] = H.e[2] - H.s[2] + 1;
+  }
+  if (!r) r = make_items(c, L.ghosts, ghosts);
+  for (int d = 0; d < 3 && !r; ++d) {
+    std::vector<int64_t> slab(c->nblocks, 0);
+    for (int b = 0; b < c->nblocks; ++b) {
+      const HostBlock &H = c->hb[b];
+      const pangu_config &cfg = c->mcfg[c->bmesh[b]];
+      if (H.n[d] == 1) continue;
+      const bool lo = H.lx[d] == 0 && cfg.bc[d][0] != PANGU_BC_PERIODIC;
+      const bool hi = H.lx[d] == cfg.nb[d] - 1 && cfg.bc[d][1] != PANGU_BC_PERIODIC;
+      if (!lo && !hi) continue;
+      const int other = d == 0 ? H.n[1] * H.n[2] : (d == 1 ? H.n[0] * H.n[2] : H.n[0] * H.n[1]);
+      slab[b] = 2 * (int64_t)H.s[d] * other;
+    }
+    r = make_items(c, L.bcs[d], slab);
+  }
+  if (r) return r;
+  // interior planes: one entry per (block, k) plane (dt reduction, FixRecovery)
+  const int *pb = nullptr, *pf = nullptr;
+  int np = 0;
+  if (int e = upload_tiles(c, planes, &pb, &pf, &np)) return e;
+  c->nplanes = np;
+  L.k.pblock = pb;
+  L.k.pfirst = pf;
+  L.k.nplanes = np;
+  int *dm = nullptr;
+  if (upload(c, &dm, c->mfirst.data(), c->mfirst.size())) return PANGU_EDEVICE;
+  c->tab_bufs.push_back(dm);
+  L.k.mfirst = dm;
+  return 0;
 }
 
 namespace pangu {
@@ -217,24 +412,37 @@ void set_kernel_clock(pangu_ctx *c, KernelClock *clock) {
 
 extern "C" {
 
-int pangu_create(const pangu_config *cfg, const pangu_block *blocks, int nblocks, int device,
-                 pangu_ctx **out) {
+int pangu_create_batch(const pangu_config *cfgs, int nmesh, const pangu_block *blocks,
+                       const int *block_mesh, int nblocks, int device, pangu_ctx **out) {
   if (!out) return PANGU_EINVAL;
   *out = nullptr;
   std::string msg;
-  if (!validate(cfg, blocks, nblocks, msg)) {
+  if (!validate_batch(cfgs, nmesh, blocks, block_mesh, nblocks, msg)) {
     fprintf(stderr, "pangu_create: %s\n", msg.c_str());
     return PANGU_EINVAL;
   }
   pangu_ctx *c = new pangu_ctx();
+  const pangu_config *cfg = &cfgs[0];
   c->cfg = *cfg;
+  c->mcfg.assign(cfgs, cfgs + nmesh);
+  c->nmesh = nmesh;
   c->device = device;
   c->nblocks = nblocks;
+  c->bmesh.resize(nblocks);
+  c->mfirst.assign(nmesh + 1, nblocks);
+  for (int b = nblocks - 1; b >= 0; --b) {
+    c->bmesh[b] = block_mesh ? block_mesh[b] : 0;
+    c->mfirst[c->bmesh[b]] = b;
+  }
   if (hipSetDevice(device) != hipSuccess) {
     delete c;
     return PANGU_EDEVICE;
   }
-  c->ndim = (blocks[0].nx[2] > 1) ? 3 : (blocks[0].nx[1] > 1 ? 2 : 1);
+  c->ndim = 1;
+  for (int b = 0; b < nblocks; ++b) {
+    const int nd = (blocks[b].nx[2] > 1) ? 3 : (blocks[b].nx[1] > 1 ? 2 : 1);
+    if (nd > c->ndim) c->ndim = nd;
+  }
   // packed component layout (package_registration.cc:84-96)
   c->np = 5;
   if (cfg->enable_B) {
@@ -247,12 +455,12 @@ int pangu_create(const pangu_config *cfg, const pangu_block *blocks, int nblocks
   for (int b = 0; b < nblocks; ++b) c->hb.push_back(make_host_block(blocks[b]));
   int r = build_geometry(c, blocks);
   if (!r) r = build_neighbours(c);
-  const int nplanes = c->hb[0].nx[2];
-  if (!r) r = upload(c, &c->dred, std::vector<double>((size_t)nblocks * nplanes * 3, 0.0).data(),
-                     (size_t)nblocks * nplanes * 3);
-  if (!r) r = upload(c, &c->dcluster, std::vector<int>((size_t)nblocks * nplanes, 0).data(),
-                     (size_t)nblocks * nplanes);
-  if (!r) r = upload(c, &c->dest, std::vector<double>(1, 0.0).data(), 1);
+  if (!r) r = build_tables(c);
+  if (!r) r = upload(c, &c->dred, std::vector<double>((size_t)c->nplanes * 3, 0.0).data(),
+                     (size_t)c->nplanes * 3);
+  if (!r) r = upload(c, &c->dcluster, std::vector<int>((size_t)c->nplanes, 0).data(),
+                     (size_t)c->nplanes);
+  if (!r) r = upload(c, &c->dest, std::vector<double>(nmesh, 0.0).data(), nmesh);
   if (r) {
     fprintf(stderr, "pangu_create: %s\n", c->err.c_str());
     pangu_destroy(c);
@@ -262,14 +470,13 @@ int pangu_create(const pangu_config *cfg, const pangu_block *blocks, int nblocks
   k.blk = c->dblk;
   k.geo = c->dgeo;
   k.nblocks = nblocks;
+  k.nmesh = nmesh;
   k.np = c->np;
   k.nc = c->nc;
   k.iBX = c->iBX;
   k.iENT = c->iENT;
   k.iKEL = c->iKEL;
   k.ndim = c->ndim;
-  k.nk = 1;
-  k.nkg = 1;
   Phys &P = k.ph;
   P.gamma = cfg->gamma;
   P.cfl = cfg->cfl;
@@ -298,26 +505,28 @@ int pangu_create(const pangu_config *cfg, const pangu_block *blocks, int nblocks
   c->L.riemann = cfg->riemann;
   c->L.metric = cfg->metric;
   c->L.recovery = cfg->recovery;
-  for (int d = 0; d < 3; ++d) {
-    c->L.nb[d] = cfg->nb[d];
-    c->L.bc[d][0] = cfg->bc[d][0];
-    c->L.bc[d][1] = cfg->bc[d][1];
-  }
   *out = c;
   return PANGU_OK;
+}
+
+int pangu_create(const pangu_config *cfg, const pangu_block *blocks, int nblocks, int device,
+                 pangu_ctx **out) {
+  return pangu_create_batch(cfg, 1, blocks, nullptr, nblocks, device, out);
 }
 
 int pangu_destroy(pangu_ctx *c) {
   if (!c) return PANGU_OK;
   hipSetDevice(c->device);
-  for (double *p : c->geo_bufs) hipFree(p);
+  for (double *p : c->geo_bufs) (void)hipFree(p);
+  for (int *p : c->tab_bufs) (void)hipFree(p);
   for (auto &ct : c->conts) {
     hipFree(ct.dprim);
     hipFree(ct.dcons);
     for (int d = 0; d < 3; ++d) hipFree(ct.dflux[d]);
   }
-  void *ptrs[] = {c->dblk, c->dgeo, c->dnbr, c->dalfven, c->demf, c->dflag, c->dflag_rec,
-                  c->dmask, c->dstatus, c->diters, c->dred, c->dcluster, c->dest};
+  void *ptrs[] = {c->dblk,    c->dgeo,  c->dnbr,     c->dalfven, c->demf,  c->dflag,
+                  c->dflag_rec, c->dmask, c->dstatus, c->diters,  c->dred,  c->dcluster,
+                  c->dest,    c->dguess, c->dguess_tab};
   for (void *p : ptrs)
     if (p) hipFree(p);
   delete c;
@@ -335,6 +544,29 @@ int pangu_set_tile_order(pangu_ctx *c, int fam, int order) {
 int pangu_set_split_source(pangu_ctx *c, int split) {
   if (!c) return PANGU_EINVAL;
   c->L.split_source = split != 0;
+  return PANGU_OK;
+}
+
+int pangu_set_c2p_guess(pangu_ctx *c, pangu_mlp *m, double delta) {
+  if (!c) return PANGU_EINVAL;
+  if (m && !(delta > 0.0 && delta < 1.0)) return fail(c, PANGU_EINVAL, "delta must be in (0, 1)");
+  if (m && c->cfg.recovery != PANGU_RECOVERY_ROBUSTER)
+    return fail(c, PANGU_EINVAL, "the learned bracket applies to the robuster (Kastaun) recovery");
+  (void)hipSetDevice(c->device);
+  if (m && !c->dguess) {
+    int64_t total = 0;
+    std::vector<int64_t> off(c->nblocks);
+    for (int b = 0; b < c->nblocks; ++b) {
+      off[b] = total;
+      total += c->hb[b].cells;
+    }
+    HIPCHK(c, hipMalloc((void **)&c->dguess, (size_t)total * sizeof(float)));
+    std::vector<float *> tab(c->nblocks);
+    for (int b = 0; b < c->nblocks; ++b) tab[b] = c->dguess + off[b];
+    if (upload(c, &c->dguess_tab, tab.data(), tab.size())) return PANGU_EDEVICE;
+  }
+  c->mlp = m;
+  c->guess_delta = m ? delta : 0.0;
   return PANGU_OK;
 }
 
@@ -467,11 +699,24 @@ static SharedTabs shared_tabs(pangu_ctx *c) {
   S.flag_rec = c->dflag_rec;
   S.status = c->dstatus;
   S.iters = c->diters;
+  S.guess = c->mlp ? c->dguess_tab : nullptr;
+  S.delta = c->guess_delta;
   return S;
+}
+
+// Learned Kastaun bracket: guesses for container md's cells before recovery.
+static int launch_guess(pangu_ctx *c, int md, hipStream_t s) {
+  if (!c->mlp || c->cfg.recovery != PANGU_RECOVERY_ROBUSTER) return 0;
+  MlpDev W;
+  if (!mlp_view(c->mlp, W)) return fail(c, PANGU_ESTATE, "learned recovery model has no weights");
+  if (launch_c2p_guess(c->L, c->conts[md].kf(), W, c->dguess_tab, s) != 0)
+    return fail(c, PANGU_EDEVICE, "c2p guess launch failed");
+  return 0;
 }
 
 int pangu_recovery(pangu_ctx *c, int md, void *stream) {
   if (int r = check_md(c, md)) return r;
+  if (int r = launch_guess(c, md, STREAM(stream))) return r;
   LCHK(c, launch_recover(c->L, c->conts[md].kf(), shared_tabs(c), c->cfg.recovery, false,
                          STREAM(stream)));
   return PANGU_OK;
@@ -495,7 +740,8 @@ int pangu_estimate_timestep(pangu_ctx *c, int md, double *dt_out, void *stream) 
   if (int r = check_md(c, md)) return r;
   if (!dt_out) return fail(c, PANGU_EINVAL, "dt_out is null");
   LCHK(c, launch_dt(c->L, c->dalfven, c->dred, nullptr, 1.0, c->dest, STREAM(stream)));
-  HIPCHK(c, hipMemcpyAsync(dt_out, c->dest, sizeof(double), hipMemcpyDeviceToHost, STREAM(stream)));
+  HIPCHK(c, hipMemcpyAsync(dt_out, c->dest, sizeof(double) * c->nmesh, hipMemcpyDeviceToHost,
+                           STREAM(stream)));
   HIPCHK(c, hipStreamSynchronize(STREAM(stream)));
   return PANGU_OK;
 }
@@ -530,6 +776,7 @@ int pangu_stage(pangu_ctx *c, int md_init, int md0, int md1, double beta, double
     LCHK(c, launch_div_update(c->L, 4, f0, fi, f1, nullptr, dt_state, beta, 0.0, s));
   }
   if (exchange_after == 0) LCHK(c, launch_exchange(c->L, f1, c->dflag, c->dnbr, s));
+  if (int r = launch_guess(c, md1, s)) return r;
   LCHK(c, launch_recover(c->L, f1, shared_tabs(c), c->cfg.recovery, c->cfg.enable_heating != 0, s));
   if (c->cfg.recovery != PANGU_RECOVERY_ROBUSTER)
     LCHK(c, launch_fix_recovery(c->L, f1, c->dflag, c->dcluster, s));

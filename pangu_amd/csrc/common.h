@@ -38,16 +38,19 @@ enum FaceComp {
   FG_NCOMP = 24
 };
 
-// Per-block launch descriptor (device copy).
+// Per-block launch descriptor (device copy).  Blocks of one context may
+// differ in shape and belong to different meshes (independent simulations).
 struct DevBlock {
-  int n[3];     // ghosted extent
+  int n[3];     // ghosted extent (1 on inactive dims)
   int s[3];     // interior start
   int e[3];     // interior end (inclusive)
-  int lx[3];
+  int lx[3];    // logical location in its mesh
   double dx[3];
   int64_t ncell;        // n0*n1*n2
   int geo;              // geometry set index
-  int pad;
+  int mesh;             // mesh the block belongs to
+  int bcf[3][2];        // physical boundary kind (PANGU_BC_*) of each face, -1 if exchanged
+  int pad[2];
 };
 
 // Geometry set (device copy): SoA tables, comp-major, (k?) j, i fastest.

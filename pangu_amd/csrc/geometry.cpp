@@ -255,6 +255,29 @@ void fill_centre(const pangu_config &cfg, const double x[4], double *out, int64_
   for (int n = 0; n < 64; ++n) out[(CG_CONN + n) * stride] = conn[n];
 }
 
+void fill_metric(const pangu_config &cfg, const double x[4], double out[35]) {
+  M4 gcov, gcon;
+  code_metric(cfg.metric, x, gcov, cfg.h, cfg.a);
+  zero(gcon);
+  inv4(gcov, gcon);
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c) {
+      out[4 * r + c] = gcov[r][c];
+      out[16 + 4 * r + c] = gcon[r][c];
+    }
+  out[32] = std::sqrt(std::fabs(det4(gcov)));
+  out[33] = 1.0 / std::sqrt(-gcon[0][0]);
+  out[34] = std::sqrt(-1.0 / gcon[0][0]);
+}
+
+void fill_metric_points(const pangu_config &cfg, const double *x12, int n, double *out) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) {
+    const double x[4] = {0.0, x12[2 * i], x12[2 * i + 1], 0.0};
+    fill_metric(cfg, x, out + (int64_t)i * 35);
+  }
+}
+
 void fill_face(const pangu_config &cfg, int d, const double x[4], double *out, int64_t stride) {
   M4 gcov, gcon;
   code_metric(cfg.metric, x, gcov, cfg.h, cfg.a);

@@ -29,6 +29,10 @@ void connection(int kind, const double x[4], const M4 gcon, double h, double a, 
 // entry and consecutive components are `stride` doubles apart.
 void fill_centre(const pangu_config &cfg, const double x[4], double *out, int64_t stride);
 void fill_face(const pangu_config &cfg, int d, const double x[4], double *out, int64_t stride);
+// gcov[16], gcon[16], sqrt|g|, 1/sqrt(-g^00), sqrt(-1/g^00) at x (no connection).
+void fill_metric(const pangu_config &cfg, const double x[4], double out[35]);
+// fill_metric at n points (x1, x2) = x12[2i], x12[2i+1] (x0 = x3 = 0), OpenMP.
+void fill_metric_points(const pangu_config &cfg, const double *x12, int n, double *out);
 void fill_centre_table(const pangu_config &cfg, const HostBlock &H, int ni, int nj, int nk,
                        bool has3d, double *out);
 void fill_face_table(const pangu_config &cfg, const HostBlock &H, int d, int fi, int fj, int fk,

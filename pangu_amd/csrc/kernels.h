@@ -4,6 +4,7 @@
 
 #include <vector>
 
+#include "../../include/pangu_mlp.h"
 #include "common.h"
 #include "physics_dev.h"
 
@@ -13,11 +14,13 @@ namespace pangu {
 struct KCtx {
   const DevBlock *blk;
   const DevGeo *geo;
-  int nblocks;
+  int nblocks, nmesh;
   int np, nc, iBX, iENT, iKEL;
-  int ndim;
-  int nk;   // interior (or launch) planes per block for blockIdx.z decoding
-  int nkg;  // ghosted planes per block
+  int ndim;            // largest dimensionality over the blocks
+  int nplanes;         // interior (block, k) planes over all blocks
+  const int *pblock;   // plane -> block
+  const int *pfirst;   // first plane of each block (nblocks + 1 entries)
+  const int *mfirst;   // first block of each mesh (nmesh + 1 entries)
   Phys ph;
 };
 
@@ -34,15 +37,42 @@ struct SharedTabs {
   double *const *flag_rec;
   int32_t *const *status;
   int32_t *const *iters;
+  const float *const *guess;  // learned Kastaun root guesses per cell, or null
+  double delta;               // relative half-width of the learned bracket
 };
 
-// Launch range of a kernel over every block.  order: 0 = plane-major tiles
-// (consecutive tiles sweep a k plane), 1 = column-major tiles (consecutive
-// tiles walk k for one 256-cell run of the (j, i) plane, so the x3-independent
-// geometry of that run is reused from L2 by every plane).
+// Device view of a learned recovery-bracket model (pangu_mlp): BF16 weights
+// W1 [32][64] (rows >= 5 zero), W2 and W3 [64][64], W4 [64][32] (columns
+// >= 1 zero), FP32 biases and the feature standardisation.
+struct MlpDev {
+  const uint16_t *w1, *w2, *w3, *w4;
+  const float *b1, *b2, *b3, *b4;
+  const float *mean, *dev;
+};
+
+// Launch range of a kernel over every block of a context, as 256-thread
+// tiles.  Blocks may differ in shape: tblock maps a tile to its block and
+// tfirst gives each block's first tile.  On an active dim d of a block the
+// range is s[d] + rlo[d] .. e[d] + rhi[d] (entire: 0 .. n[d] - 1); an
+// inactive dim has its single index.  A block's tiles cover its k planes,
+// each plane flattened row-major.  order: 0 = plane-major (consecutive tiles
+// sweep a plane), 1 = column-major (consecutive tiles walk k for one
+// 256-cell run of the plane, reusing x3-independent geometry from L2).
 struct Range3 {
-  int lo[3], hi[3], cnt[3];
+  const int *tblock;
+  const int *tfirst;
+  int ntiles;
+  int rlo[3], rhi[3];
+  int entire;
   int order;
+};
+
+// Work items counted per block (ghost cells, boundary slab cells), in tiles
+// of 256 items: tile -> block and each block's first tile.
+struct Items {
+  const int *tblock;
+  const int *tfirst;
+  int ntiles;
 };
 
 // Optional per-launch clock: when a LaunchCtx carries one, every kernel
@@ -58,17 +88,19 @@ struct KernelClock {
   ~KernelClock();
 };
 
-// Host-side launch context: kernel constants plus the (uniform) block shape.
+// Host-side launch context: kernel constants plus the launch ranges.
 struct LaunchCtx {
   KCtx k;
-  DevBlock hblk;
   int limiter, riemann, metric, recovery;
-  int bc[3][2];
-  int nb[3];
+  Range3 interior, entire, face[3], ct;
+  Items ghosts, bcs[3];
   int order[PANGU_KF_COUNT] = {};  // tile order per kernel family (PANGU_ORDER_*)
   int split_source = 0;            // fused stage: source term in a kernel of its own
   KernelClock *clock = nullptr;
 };
+
+// Device view of a model's current weights; false before any weights exist.
+bool mlp_view(const pangu_mlp *m, MlpDev &out);
 
 // Attaches (or, with nullptr, detaches) a clock to a context's launches.
 void set_kernel_clock(pangu_ctx *c, KernelClock *clock);
@@ -85,12 +117,17 @@ int launch_div_update(const LaunchCtx &L, int mode, const KFields &F0, const KFi
 // rec: 0 chainer, 1 kastaun, 2 1dvsq, -1 none (heating only)
 int launch_recover(const LaunchCtx &L, const KFields &F, const SharedTabs &S, int rec, bool heat,
                    hipStream_t s);
+// Learned Kastaun root guess of every interior cell of container F (the
+// inputs of k_recover), into guess[b][cell]; -1 where no guess applies.
+int launch_c2p_guess(const LaunchCtx &L, const KFields &F, const MlpDev &W, float *const *guess,
+                     hipStream_t s);
 int launch_fix_recovery(const LaunchCtx &L, const KFields &F, double *const *flag, int *cluster,
                         hipStream_t s);
 int launch_exchange(const LaunchCtx &L, const KFields &F, double *const *flag, const int *nbr,
                     hipStream_t s);
 int launch_bc(const LaunchCtx &L, const KFields &F, double *const *flag, hipStream_t s);
-// red: nblocks * interior planes * 3 doubles of scratch
+// red: 3 doubles of scratch per interior plane; state: {dt, time} per mesh;
+// est_out (optional): one estimate per mesh
 int launch_dt(const LaunchCtx &L, double *const *alfven, double *red, double *state,
               double dt_factor, double *est_out, hipStream_t s);
 

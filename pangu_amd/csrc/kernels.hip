@@ -16,6 +16,7 @@
 #include "kernels.h"
 #include "physics_dev.h"
 #include "recovery_dev.h"
+#include "tiles_dev.h"
 
 namespace pangu {
 
@@ -42,39 +43,6 @@ __device__ __forceinline__ int64_t cen_index(const DevGeo &G, int k, int j, int 
   return ((int64_t)(G.has3d ? k : 0) * G.nj + j) * G.ni + i;
 }
 
-constexpr int TILE = 256;
-
-// Decodes (block, k, j, i) of this thread; false for padding threads.
-__device__ __forceinline__ bool decode_tile(const Range3 &R, int nblocks, int &b, int &k, int &j,
-                                            int &i) {
-  const int plane = R.cnt[0] * R.cnt[1];
-  const int tpp = (plane + TILE - 1) / TILE;
-  const int N = nblocks * R.cnt[2] * tpp;
-  const int n8 = (N + 7) >> 3;
-  const int L = (int)blockIdx.x;
-  const int T = (L & 7) * n8 + (L >> 3);
-  if (T >= N) return false;
-  int tile, kk;
-  if (R.order) {
-    const int col = T / R.cnt[2];
-    kk = T - col * R.cnt[2];
-    b = col / tpp;
-    tile = col - b * tpp;
-  } else {
-    const int p = T / tpp;
-    tile = T - p * tpp;
-    b = p / R.cnt[2];
-    kk = p - b * R.cnt[2];
-  }
-  const int t = tile * TILE + (int)threadIdx.x;
-  if (t >= plane) return false;
-  k = R.lo[2] + kk;
-  const int jj = t / R.cnt[0];
-  j = R.lo[1] + jj;
-  i = R.lo[0] + (t - jj * R.cnt[0]);
-  return true;
-}
-
 // Loads of the centre table.
 struct CenLoad {
   double g[16], gc0[4], alpha, sg;
@@ -96,7 +64,7 @@ __device__ __forceinline__ void load_cen_state(const DevGeo &G, int64_t t, CenLo
 template <bool FIX, bool CONS, bool HB, bool HK>
 __global__ __launch_bounds__(256) void k_fix_cons(KCtx C, KFields F, uint8_t *const *mask, Range3 R) {
   int b, k, j, i;
-  if (!decode_tile(R, C.nblocks, b, k, j, i)) return;
+  if (!decode_tile(C, R, b, k, j, i)) return;
   const DevBlock &B = C.blk[b];
   const bool interior = i >= B.s[0] && i <= B.e[0] && j >= B.s[1] && j <= B.e[1] &&
                         k >= B.s[2] && k <= B.e[2];
@@ -194,7 +162,7 @@ __global__ __launch_bounds__(256) void k_fix_cons(KCtx C, KFields F, uint8_t *co
 template <int DIR, int LIM, int RS, bool HB, bool HK>
 __global__ __launch_bounds__(256) void k_flux(KCtx C, KFields F, double *const *alfven, Range3 R) {
   int b, k, j, i;
-  if (!decode_tile(R, C.nblocks, b, k, j, i)) return;
+  if (!decode_tile(C, R, b, k, j, i)) return;
   const DevBlock &B = C.blk[b];
   const int64_t sj = B.n[0], sk = (int64_t)B.n[0] * B.n[1];
   const int64_t st = (DIR == 1) ? 1 : (DIR == 2 ? sj : sk);
@@ -289,11 +257,12 @@ __global__ __launch_bounds__(256) void k_flux(KCtx C, KFields F, double *const *
 
 // ------------------------------------------------------ constrained transport
 // Corner EMFs (constrained_transport.cc:49-76).
-__global__ __launch_bounds__(256) void k_emf(KCtx C, KFields F, double *const *emf, Range3 R,
-                                             int e1on, int e2on, int e3on) {
+__global__ __launch_bounds__(256) void k_emf(KCtx C, KFields F, double *const *emf, Range3 R) {
   int b, k, j, i;
-  if (!decode_tile(R, C.nblocks, b, k, j, i)) return;
+  if (!decode_tile(C, R, b, k, j, i)) return;
   const DevBlock &B = C.blk[b];
+  const bool o1 = B.n[0] > 1, o2 = B.n[1] > 1, o3 = B.n[2] > 1;
+  const bool e1on = o2 && o3, e2on = o3 && o1, e3on = o1 && o2;
   const int64_t sj = B.n[0], sk = (int64_t)B.n[0] * B.n[1];
   const int64_t id = k * sk + j * sj + i;
   const int nc = C.nc;
@@ -318,11 +287,11 @@ __global__ __launch_bounds__(256) void k_emf(KCtx C, KFields F, double *const *e
 
 // Face B-flux overwrite from the EMFs (constrained_transport.cc:78-127).
 // One launch covers the union of the three face ranges.
-__global__ __launch_bounds__(256) void k_ct_faces(KCtx C, KFields F, double *const *emf, Range3 R,
-                                                  int o1, int o2, int o3) {
+__global__ __launch_bounds__(256) void k_ct_faces(KCtx C, KFields F, double *const *emf, Range3 R) {
   int b, k, j, i;
-  if (!decode_tile(R, C.nblocks, b, k, j, i)) return;
+  if (!decode_tile(C, R, b, k, j, i)) return;
   const DevBlock &B = C.blk[b];
+  const int o1 = B.n[0] > 1, o2 = B.n[1] > 1, o3 = B.n[2] > 1;
   const int64_t sj = B.n[0], sk = (int64_t)B.n[0] * B.n[1];
   const int64_t id = k * sk + j * sj + i;
   const int nc = C.nc;
@@ -353,19 +322,19 @@ __global__ __launch_bounds__(256) void k_ct_faces(KCtx C, KFields F, double *con
 // geometric source U_nu += beta dt sqrt|g| T^r_c Gamma^c_{nu r}
 // (source_term.cc:171-228).  MODE: 1 = divergence only (writes dU/dt),
 // 2 = update only (reads dU/dt), 3 = source only, 4 = all three fused.
-template <int MODE, bool SRC, bool HB, bool HK>
 // Five waves per SIMD is the most the source-term variants reach without
 // spilling; the kernel is memory bound, so occupancy pays.
+template <int MODE, bool SRC, bool HB, bool HK>
 __global__ __launch_bounds__(256, 5) void k_div_update(KCtx C, KFields F0, KFields Finit, KFields F1,
                                                     double *const *dudt, const double *dt_dev,
                                                     double beta, double bdt_host, Range3 R) {
   int b, k, j, i;
-  if (!decode_tile(R, C.nblocks, b, k, j, i)) return;
+  if (!decode_tile(C, R, b, k, j, i)) return;
   const DevBlock &B = C.blk[b];
   const int64_t sj = B.n[0], sk = (int64_t)B.n[0] * B.n[1];
   const int64_t id = k * sk + j * sj + i;
   constexpr int NC = ncomp<HB, HK>();
-  const double bdt = dt_dev ? beta * dt_dev[0] : bdt_host;
+  const double bdt = dt_dev ? beta * dt_dev[2 * B.mesh] : bdt_host;
   double unew[NC];
   if (MODE == 1 || MODE == 4) {
     const double a1 = B.dx[1] * B.dx[2], a2 = B.dx[0] * B.dx[2], a3 = B.dx[0] * B.dx[1];
@@ -374,11 +343,11 @@ __global__ __launch_bounds__(256, 5) void k_div_update(KCtx C, KFields F0, KFiel
     for (int q = 0; q < NC; ++q) {
       const double *f1 = F0.flux[0][b * NC + q];
       double s = (a1 * f1[id + 1] - a1 * f1[id]);
-      if (C.ndim >= 2) {
+      if (B.n[1] > 1) {
         const double *f2 = F0.flux[1][b * NC + q];
         s += (a2 * f2[id + sj] - a2 * f2[id]);
       }
-      if (C.ndim == 3) {
+      if (B.n[2] > 1) {
         const double *f3 = F0.flux[2][b * NC + q];
         s += (a3 * f3[id + sk] - a3 * f3[id]);
       }
@@ -444,7 +413,7 @@ __global__ __launch_bounds__(256, 5) void k_div_update(KCtx C, KFields F0, KFiel
 template <int REC, bool HB, bool HK, bool HEAT>
 __global__ __launch_bounds__(256) void k_recover(KCtx C, KFields F, SharedTabs S, Range3 R) {
   int b, k, j, i;
-  if (!decode_tile(R, C.nblocks, b, k, j, i)) return;
+  if (!decode_tile(C, R, b, k, j, i)) return;
   const DevBlock &B = C.blk[b];
   const int64_t id = ((int64_t)k * B.n[1] + j) * B.n[0] + i;
   constexpr int NC = ncomp<HB, HK>();
@@ -509,7 +478,12 @@ __global__ __launch_bounds__(256) void k_recover(KCtx C, KFields F, SharedTabs S
       int st = 0, it = 0, itn = 0;
       bool write = true;
       if (REC == 1) {
-        st = c2p_kastaun(U, Ph, P.gamma, m, it);
+        if (S.guess) {
+          int ev = 0;
+          st = c2p_kastaun_guess(U, Ph, P.gamma, m, (double)S.guess[b][id], S.delta, it, ev);
+        } else {
+          st = c2p_kastaun(U, Ph, P.gamma, m, it);
+        }
         itn = it;
         write = (st == 0);
       } else if (REC == 2) {
@@ -596,7 +570,7 @@ __device__ __forceinline__ void fix_cell(double *const *pp, int np, int64_t id, 
 __global__ __launch_bounds__(256) void k_fixrec_a(KCtx C, KFields F, double *const *flag,
                                                   int *cluster, Range3 R) {
   int b, k, j, i;
-  if (!decode_tile(R, C.nblocks, b, k, j, i)) return;
+  if (!decode_tile(C, R, b, k, j, i)) return;
   const DevBlock &B = C.blk[b];
   const int kk = k - B.s[2];
   const int64_t id = ((int64_t)k * B.n[1] + j) * B.n[0] + i;
@@ -611,19 +585,20 @@ __global__ __launch_bounds__(256) void k_fixrec_a(KCtx C, KFields F, double *con
                         fl[id + 1] != 0 && fl[id - jp + 1] != 0 && fl[id - jp] != 0 &&
                         fl[id - jp - 1] != 0 && fl[id - 1] != 0;
   if (!isolated) {
-    atomicOr(&cluster[b * C.nk + kk], 1);
+    atomicOr(&cluster[C.pfirst[b] + kk], 1);
     return;
   }
   fix_cell(F.prim + b * C.np, C.np, id, jp, fl);
   fl[id] = 1;
 }
 
-// Phase B: one wavefront per (block, k) plane that holds clustered cells.
+// Phase B: one wavefront per interior (block, k) plane that holds clustered
+// cells (workgroup index = plane index).
 __global__ __launch_bounds__(64) void k_fixrec_b(KCtx C, KFields F, double *const *flag,
                                                  int *cluster) {
-  const int b = blockIdx.x / C.nk;
-  const int kk = blockIdx.x % C.nk;
-  if (cluster[b * C.nk + kk] == 0) return;
+  const int b = C.pblock[blockIdx.x];
+  const int kk = (int)blockIdx.x - C.pfirst[b];
+  if (cluster[blockIdx.x] == 0) return;
   const DevBlock &B = C.blk[b];
   const int k = B.s[2] + kk;
   const int64_t jp = (B.s[1] != B.e[1]) ? (int64_t)B.n[0] : 0;
@@ -651,7 +626,7 @@ __global__ __launch_bounds__(64) void k_fixrec_b(KCtx C, KFields F, double *cons
       }
     }
   }
-  if (lane == 0) cluster[b * C.nk + kk] = 0;
+  if (lane == 0) cluster[blockIdx.x] = 0;
 }
 
 // ------------------------------------------------- ghost exchange and BCs
@@ -694,15 +669,16 @@ __device__ __forceinline__ void ghost_cell(const DevBlock &B, int g, int &k, int
 
 // Same-level exchange: a ghost cell of block b takes the interior value of
 // the neighbour in its offset direction (periodic wrap encoded in nbr).
-// One thread per ghost cell (nghost per block).
+// One thread per ghost cell.
 __global__ __launch_bounds__(256) void k_exchange(KCtx C, KFields F, double *const *flag,
-                                                  const int *nbr, int nghost) {
-  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= (int64_t)C.nblocks * nghost) return;
-  const int b = (int)(gid / nghost);
+                                                  const int *nbr, Items I) {
+  int b, g;
+  decode_item(I, b, g);
   const DevBlock &B = C.blk[b];
+  const int interior = (B.e[0] - B.s[0] + 1) * (B.e[1] - B.s[1] + 1) * (B.e[2] - B.s[2] + 1);
+  if (g >= (int)B.ncell - interior) return;
   int k, j, i;
-  ghost_cell(B, (int)(gid - (int64_t)b * nghost), k, j, i);
+  ghost_cell(B, g, k, j, i);
   const int o0 = (i < B.s[0]) ? -1 : (i > B.e[0] ? 1 : 0);
   const int o1 = (j < B.s[1]) ? -1 : (j > B.e[1] ? 1 : 0);
   const int o2 = (k < B.s[2]) ? -1 : (k > B.e[2] ? 1 : 0);
@@ -757,23 +733,19 @@ __device__ __forceinline__ void bc_cell(const DevBlock &B, int d, int g, int &si
 // Physical boundaries normal to dimension d: outflow copies the edge cell;
 // reflecting mirrors and flips the d-component of u~ and B.  Covers the full
 // extent of the other dims, so X1, X2, X3 launches in turn fill corners.
-// One thread per cell of the two ghost slabs of every block (per_block).
-__global__ __launch_bounds__(256) void k_bc(KCtx C, KFields F, double *const *flag, int d, int bc_in,
-                                            int bc_out, int nbd, int per_block) {
-  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= (int64_t)C.nblocks * per_block) return;
-  const int b = (int)(gid / per_block);
+// One thread per cell of the two ghost slabs of each block with a physical
+// face normal to d; B.bcf says which faces are physical and their kind.
+__global__ __launch_bounds__(256) void k_bc(KCtx C, KFields F, double *const *flag, int d, Items I) {
+  int b, g;
+  decode_item(I, b, g);
   const DevBlock &B = C.blk[b];
+  const int per_block = 2 * B.s[d] *
+                        (d == 0 ? B.n[1] * B.n[2] : (d == 1 ? B.n[0] * B.n[2] : B.n[0] * B.n[1]));
+  if (g >= per_block) return;
   int side, k, j, i;
-  bc_cell(B, d, (int)(gid - (int64_t)b * per_block), side, k, j, i);
-  int kind;
-  if (side == 0) {
-    if (B.lx[d] != 0 || bc_in == PANGU_BC_PERIODIC) return;
-    kind = bc_in;
-  } else {
-    if (B.lx[d] != nbd - 1 || bc_out == PANGU_BC_PERIODIC) return;
-    kind = bc_out;
-  }
+  bc_cell(B, d, g, side, k, j, i);
+  const int kind = B.bcf[d][side];
+  if (kind < 0) return;
   const int ix[3] = {i, j, k};
   int sx[3] = {i, j, k};
   if (kind == PANGU_BC_OUTFLOW)
@@ -800,8 +772,8 @@ __global__ __launch_bounds__(256) void k_bc(KCtx C, KFields F, double *const *fl
 // result is deterministic.
 __global__ __launch_bounds__(256) void k_dt_reduce(KCtx C, double *const *alfven, double *red) {
   __shared__ double sm[4][3];
-  const int b = blockIdx.x / C.nk;
-  const int kk = blockIdx.x % C.nk;
+  const int b = C.pblock[blockIdx.x];
+  const int kk = (int)blockIdx.x - C.pfirst[b];
   const DevBlock &B = C.blk[b];
   const int k = B.s[2] + kk;
   const int ni = B.e[0] - B.s[0] + 1, nj = B.e[1] - B.s[1] + 1;
@@ -809,7 +781,8 @@ __global__ __launch_bounds__(256) void k_dt_reduce(KCtx C, double *const *alfven
   for (int t = threadIdx.x; t < ni * nj; t += blockDim.x) {
     const int j = B.s[1] + t / ni, i = B.s[0] + t % ni;
     const int64_t id = ((int64_t)k * B.n[1] + j) * B.n[0] + i;
-    for (int d = 0; d < C.ndim; ++d) {
+    for (int d = 0; d < 3; ++d) {
+      if (B.n[d] == 1) continue;
       const double v = alfven[b * 3 + d][id];
       if (v > c[d]) c[d] = v;
     }
@@ -828,30 +801,33 @@ __global__ __launch_bounds__(256) void k_dt_reduce(KCtx C, double *const *alfven
     const int d = threadIdx.x;
     double v = sm[0][d];
     for (int q = 1; q < 4; ++q) v = (sm[q][d] > v) ? sm[q][d] : v;
-    red[((int64_t)b * C.nk + kk) * 3 + d] = v;
+    red[(int64_t)blockIdx.x * 3 + d] = v;
   }
 }
 
 // Harmonic combination per block (timestep_estimation.cc:94-98), minimum
-// over blocks, then dt control dt_next = min(dt_factor * dt, estimate) and
-// time += dt.  state = [dt, time]; est_out (optional) gets the estimate.
+// over the blocks of one mesh, then dt control dt_next = min(dt_factor * dt,
+// estimate) and time += dt.  One workgroup per mesh m: state + 2m = {dt,
+// time}; est_out (optional) + m gets the estimate.
 __global__ __launch_bounds__(256) void k_dt_final(KCtx C, const double *red, double *state,
                                                   double dt_factor, double *est_out) {
   __shared__ double sm[256];
+  const int m = blockIdx.x;
   double mn = DBL_MAX;
-  for (int b = threadIdx.x; b < C.nblocks; b += blockDim.x) {
+  for (int b = C.mfirst[m] + (int)threadIdx.x; b < C.mfirst[m + 1]; b += blockDim.x) {
     const DevBlock &B = C.blk[b];
-    double m[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
-    for (int d = 0; d < C.ndim; ++d) {
+    double q3[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
+    for (int d = 0; d < 3; ++d) {
+      if (B.n[d] == 1) continue;
       double cmax = 0.0;
-      for (int kk = 0; kk < C.nk; ++kk) {
-        const double v = red[((int64_t)b * C.nk + kk) * 3 + d];
+      for (int p = C.pfirst[b]; p < C.pfirst[b + 1]; ++p) {
+        const double v = red[(int64_t)p * 3 + d];
         cmax = (v > cmax) ? v : cmax;
       }
       const double q = C.ph.cfl * B.dx[d] / cmax;
-      m[d] = (q < DBL_MAX) ? q : DBL_MAX;
+      q3[d] = (q < DBL_MAX) ? q : DBL_MAX;
     }
-    const double t = 1.0 / (1.0 / m[0] + 1.0 / m[1] + 1.0 / m[2]);
+    const double t = 1.0 / (1.0 / q3[0] + 1.0 / q3[1] + 1.0 / q3[2]);
     if (t < mn) mn = t;
   }
   sm[threadIdx.x] = mn;
@@ -865,38 +841,20 @@ __global__ __launch_bounds__(256) void k_dt_final(KCtx C, const double *red, dou
   }
   if (threadIdx.x == 0) {
     const double est = sm[0];
-    if (est_out) *est_out = est;
+    if (est_out) est_out[m] = est;
     if (state) {
-      const double dt = state[0];
-      state[1] = state[1] + dt;
+      double *st = state + 2 * m;
+      const double dt = st[0];
+      st[1] = st[1] + dt;
       const double grown = dt_factor * dt;
-      state[0] = (est < grown) ? est : grown;
+      st[0] = (est < grown) ? est : grown;
     }
   }
 }
 
 // ===================================================================== host
-static Range3 make_range(int l0, int h0, int l1, int h1, int l2, int h2) {
-  Range3 R;
-  R.lo[0] = l0; R.hi[0] = h0;
-  R.lo[1] = l1; R.hi[1] = h1;
-  R.lo[2] = l2; R.hi[2] = h2;
-  for (int d = 0; d < 3; ++d) R.cnt[d] = R.hi[d] - R.lo[d] + 1;
-  R.order = 0;
-  return R;
-}
-static Range3 interior_range(const DevBlock &B) {
-  return make_range(B.s[0], B.e[0], B.s[1], B.e[1], B.s[2], B.e[2]);
-}
-static Range3 entire_range(const DevBlock &B) {
-  return make_range(0, B.n[0] - 1, 0, B.n[1] - 1, 0, B.n[2] - 1);
-}
-// 1D grid of 256-thread tiles, padded to a multiple of the 8 XCDs.
-static dim3 tile_grid(const Range3 &R, int nblocks) {
-  const long plane = (long)R.cnt[0] * R.cnt[1];
-  const long N = (long)nblocks * R.cnt[2] * ((plane + TILE - 1) / TILE);
-  return dim3((unsigned)(((N + 7) / 8) * 8));
-}
+// Grid of a tile launch, padded to a multiple of the 8 XCDs.
+static dim3 tile_grid(const Range3 &R) { return dim3((unsigned)(((R.ntiles + 7) / 8) * 8)); }
 static const dim3 kTile(TILE, 1, 1);
 
 #define PG_CHECK_LAUNCH() (hipGetLastError() == hipSuccess ? 0 : -2)
@@ -923,9 +881,10 @@ KernelClock::~KernelClock() {
 
 int launch_fix_cons(const LaunchCtx &L, const KFields &F, uint8_t *const *mask, bool fix, bool cons,
                     hipStream_t s) {
-  Range3 R = fix ? entire_range(L.hblk) : interior_range(L.hblk);
+  Range3 R = fix ? L.entire : L.interior;
   R.order = L.order[PANGU_KF_FIX_CONS];
-  const dim3 g = tile_grid(R, L.k.nblocks);
+  if (R.ntiles == 0) return 0;
+  const dim3 g = tile_grid(R);
   const bool hb = L.k.iBX >= 0, hk = L.k.iKEL >= 0;
 #define FC(FX, CS, HB_, HK_) hipLaunchKernelGGL((k_fix_cons<FX, CS, HB_, HK_>), g, kTile, 0, s, L.k, F, mask, R)
 #define FC2(FX, CS)                      \
@@ -947,10 +906,11 @@ int launch_fix_cons(const LaunchCtx &L, const KFields &F, uint8_t *const *mask, 
 }
 
 template <int DIR>
-static void flux_dir(const LaunchCtx &L, const KFields &F, double *const *alfven, Range3 R,
-                     hipStream_t s) {
+static void flux_dir(const LaunchCtx &L, const KFields &F, double *const *alfven, hipStream_t s) {
+  Range3 R = L.face[DIR - 1];
   R.order = L.order[PANGU_KF_FLUX_X1 + DIR - 1];
-  const dim3 g = tile_grid(R, L.k.nblocks);
+  if (R.ntiles == 0) return;
+  const dim3 g = tile_grid(R);
   const bool hb = L.k.iBX >= 0, hk = L.k.iKEL >= 0;
   const int lim = L.limiter, rs = L.riemann;
 #define FL(LIM, RS, HB_, HK_) \
@@ -974,45 +934,21 @@ static void flux_dir(const LaunchCtx &L, const KFields &F, double *const *alfven
   TICK(DIR == 1 ? "flux_x1" : DIR == 2 ? "flux_x2" : "flux_x3");
 }
 
-// Face ranges of the three sweeps (laxf.cc:78-83,184-189,290-295): the
-// normal index runs over s..e+1 and transverse indices include one ghost
-// layer on active dims, except HLL's X3 sweep, which stays on the interior
-// transversely (hll.cc:295-296).
-static Range3 face_range(const LaunchCtx &L, int dir) {
-  const DevBlock &B = L.hblk;
-  const int o[3] = {B.n[0] > 1, B.n[1] > 1, B.n[2] > 1};
-  int lo[3], hi[3];
-  for (int d = 0; d < 3; ++d) {
-    lo[d] = B.s[d] - o[d];
-    hi[d] = B.e[d] + o[d];
-  }
-  lo[dir - 1] = B.s[dir - 1];
-  hi[dir - 1] = B.e[dir - 1] + 1;
-  if (dir == 3 && L.riemann == PANGU_RIEMANN_HLL) {
-    lo[0] = B.s[0];
-    hi[0] = B.e[0];
-    lo[1] = B.s[1];
-    hi[1] = B.e[1];
-  }
-  return make_range(lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]);
-}
-
 int launch_fluxes(const LaunchCtx &L, const KFields &F, double *const *alfven, hipStream_t s) {
-  if (L.k.ndim >= 1) flux_dir<1>(L, F, alfven, face_range(L, 1), s);
-  if (L.k.ndim >= 2) flux_dir<2>(L, F, alfven, face_range(L, 2), s);
-  if (L.k.ndim >= 3) flux_dir<3>(L, F, alfven, face_range(L, 3), s);
+  flux_dir<1>(L, F, alfven, s);
+  flux_dir<2>(L, F, alfven, s);
+  flux_dir<3>(L, F, alfven, s);
   return PG_CHECK_LAUNCH();
 }
 
 int launch_ct(const LaunchCtx &L, const KFields &F, double *const *emf, hipStream_t s) {
-  const DevBlock &B = L.hblk;
-  const int o1 = B.n[0] > 1, o2 = B.n[1] > 1, o3 = B.n[2] > 1;
-  Range3 R = make_range(B.s[0], B.e[0] + o1, B.s[1], B.e[1] + o2, B.s[2], B.e[2] + o3);
+  Range3 R = L.ct;
   R.order = L.order[PANGU_KF_CT];
-  const dim3 g = tile_grid(R, L.k.nblocks);
-  hipLaunchKernelGGL(k_emf, g, kTile, 0, s, L.k, F, emf, R, o2 && o3, o3 && o1, o1 && o2);
+  if (R.ntiles == 0) return 0;
+  const dim3 g = tile_grid(R);
+  hipLaunchKernelGGL(k_emf, g, kTile, 0, s, L.k, F, emf, R);
   TICK("ct_emf");
-  hipLaunchKernelGGL(k_ct_faces, g, kTile, 0, s, L.k, F, emf, R, o1, o2, o3);
+  hipLaunchKernelGGL(k_ct_faces, g, kTile, 0, s, L.k, F, emf, R);
   TICK("ct_faces");
   return PG_CHECK_LAUNCH();
 }
@@ -1020,9 +956,10 @@ int launch_ct(const LaunchCtx &L, const KFields &F, double *const *emf, hipStrea
 int launch_div_update(const LaunchCtx &L, int mode, const KFields &F0, const KFields &Finit,
                       const KFields &F1, double *const *dudt, const double *dt_dev, double beta,
                       double bdt_host, hipStream_t s) {
-  Range3 R = interior_range(L.hblk);
+  Range3 R = L.interior;
   R.order = L.order[PANGU_KF_DIV_UPDATE];
-  const dim3 g = tile_grid(R, L.k.nblocks);
+  if (R.ntiles == 0) return 0;
+  const dim3 g = tile_grid(R);
   const bool hb = L.k.iBX >= 0, hk = L.k.iKEL >= 0;
   const bool src = L.metric != PANGU_METRIC_MINKOWSKI;
 #define DU(M, SR, HB_, HK_)                                                                     \
@@ -1062,9 +999,10 @@ int launch_div_update(const LaunchCtx &L, int mode, const KFields &F0, const KFi
 
 int launch_recover(const LaunchCtx &L, const KFields &F, const SharedTabs &S, int rec, bool heat,
                    hipStream_t s) {
-  Range3 R = interior_range(L.hblk);
+  Range3 R = L.interior;
   R.order = L.order[PANGU_KF_RECOVERY];
-  const dim3 g = tile_grid(R, L.k.nblocks);
+  if (R.ntiles == 0) return 0;
+  const dim3 g = tile_grid(R);
   const bool hb = L.k.iBX >= 0, hk = L.k.iKEL >= 0;
 #define RC(R_, HB_, HK_, HE_) \
   hipLaunchKernelGGL((k_recover<R_, HB_, HK_, HE_>), g, kTile, 0, s, L.k, F, S, R)
@@ -1091,39 +1029,27 @@ int launch_recover(const LaunchCtx &L, const KFields &F, const SharedTabs &S, in
 
 int launch_fix_recovery(const LaunchCtx &L, const KFields &F, double *const *flag, int *cluster,
                         hipStream_t s) {
-  const DevBlock &B = L.hblk;
-  const Range3 R = interior_range(B);
-  KCtx kc = L.k;
-  kc.nk = R.cnt[2];
-  hipLaunchKernelGGL(k_fixrec_a, tile_grid(R, L.k.nblocks), kTile, 0, s, kc, F, flag, cluster, R);
+  const Range3 &R = L.interior;
+  if (R.ntiles == 0) return 0;
+  hipLaunchKernelGGL(k_fixrec_a, tile_grid(R), kTile, 0, s, L.k, F, flag, cluster, R);
   TICK("fix_recovery_a");
-  hipLaunchKernelGGL(k_fixrec_b, dim3(L.k.nblocks * R.cnt[2]), dim3(64), 0, s, kc, F, flag, cluster);
+  hipLaunchKernelGGL(k_fixrec_b, dim3(L.k.nplanes), dim3(64), 0, s, L.k, F, flag, cluster);
   TICK("fix_recovery_b");
   return PG_CHECK_LAUNCH();
 }
 
 int launch_exchange(const LaunchCtx &L, const KFields &F, double *const *flag, const int *nbr,
                     hipStream_t s) {
-  const DevBlock &B = L.hblk;
-  const int interior = (B.e[0] - B.s[0] + 1) * (B.e[1] - B.s[1] + 1) * (B.e[2] - B.s[2] + 1);
-  const int nghost = (int)B.ncell - interior;
-  const int64_t n = (int64_t)L.k.nblocks * nghost;
-  if (n > 0)
-    hipLaunchKernelGGL(k_exchange, dim3((unsigned)((n + TILE - 1) / TILE)), kTile, 0, s, L.k, F, flag,
-                       nbr, nghost);
+  if (L.ghosts.ntiles > 0)
+    hipLaunchKernelGGL(k_exchange, dim3(L.ghosts.ntiles), kTile, 0, s, L.k, F, flag, nbr, L.ghosts);
   TICK("exchange");
   return PG_CHECK_LAUNCH();
 }
 
 int launch_bc(const LaunchCtx &L, const KFields &F, double *const *flag, hipStream_t s) {
-  const DevBlock &B = L.hblk;
-  for (int d = 0; d < L.k.ndim; ++d) {
-    if (L.bc[d][0] == PANGU_BC_PERIODIC && L.bc[d][1] == PANGU_BC_PERIODIC) continue;
-    const int per_block = 2 * B.s[d] * (d == 0 ? B.n[1] * B.n[2] : d == 1 ? B.n[0] * B.n[2] : B.n[0] * B.n[1]);
-    const int64_t n = (int64_t)L.k.nblocks * per_block;
-    if (n == 0) continue;
-    hipLaunchKernelGGL(k_bc, dim3((unsigned)((n + TILE - 1) / TILE)), kTile, 0, s, L.k, F, flag, d,
-                       L.bc[d][0], L.bc[d][1], L.nb[d], per_block);
+  for (int d = 0; d < 3; ++d) {
+    if (L.bcs[d].ntiles == 0) continue;
+    hipLaunchKernelGGL(k_bc, dim3(L.bcs[d].ntiles), kTile, 0, s, L.k, F, flag, d, L.bcs[d]);
     TICK(d == 0 ? "bc_x1" : d == 1 ? "bc_x2" : "bc_x3");
   }
   return PG_CHECK_LAUNCH();
@@ -1131,12 +1057,10 @@ int launch_bc(const LaunchCtx &L, const KFields &F, double *const *flag, hipStre
 
 int launch_dt(const LaunchCtx &L, double *const *alfven, double *red, double *state,
               double dt_factor, double *est_out, hipStream_t s) {
-  const DevBlock &B = L.hblk;
-  KCtx kc = L.k;
-  kc.nk = B.e[2] - B.s[2] + 1;
-  hipLaunchKernelGGL(k_dt_reduce, dim3(L.k.nblocks * kc.nk), dim3(256), 0, s, kc, alfven, red);
+  hipLaunchKernelGGL(k_dt_reduce, dim3(L.k.nplanes), dim3(256), 0, s, L.k, alfven, red);
   TICK("dt_reduce");
-  hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, s, kc, red, state, dt_factor, est_out);
+  hipLaunchKernelGGL(k_dt_final, dim3(L.k.nmesh), dim3(256), 0, s, L.k, red, state, dt_factor,
+                     est_out);
   TICK("dt_final");
   return PG_CHECK_LAUNCH();
 }

@@ -1,0 +1,958 @@
+// Learned starting bracket for the Kastaun recovery (include/pangu_mlp.h).
+//
+// Network 5 -> 64 -> 64 -> 64 -> 1, SiLU.  Matrix products run on the gfx950
+// matrix cores as v_mfma_f32_32x32x16_bf16: one 64-lane wavefront computes a
+// 32x32 FP32 tile of C = A B, stepping K by 16.  Lane l holds row (l mod 32)
+// of A and column (l mod 32) of B for the eight k of its half
+// (8 * (l / 32) + 0..7), and receives C rows 8 (v / 4) + 4 (l / 32) + (v mod 4)
+// of column (l mod 32) in accumulator v.  Inputs of every product are BF16
+// (round to nearest even), accumulation FP32; master weights FP32 (Adam).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/pangu_mlp.h"
+#include "common.h"
+#include "host.h"
+#include "kernels.h"
+#include "physics_dev.h"
+#include "recovery_dev.h"
+#include "tiles_dev.h"
+
+namespace pangu {
+
+constexpr int NH = 64;    // hidden width
+constexpr int KIN = 32;   // padded input width (5 features)
+constexpr int NOUT = 32;  // padded output width (1 output)
+constexpr int NMET = 35;  // metric sample: gcov, gcon, sqrt|g|, alpha, lapse
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(8 * sizeof(unsigned short)))) unsigned short u16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+
+__host__ __device__ inline uint16_t f2bf(float f) {
+  uint32_t u = __builtin_bit_cast(uint32_t, f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float bf2f(uint16_t h) {
+  return __builtin_bit_cast(float, (uint32_t)h << 16);
+}
+
+__device__ inline f32x16 mfma32(u16x8 a, u16x8 b, f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
+                                                 __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+
+__device__ inline float silu(float z) { return z / (1.0f + expf(-z)); }
+__device__ inline float silu_grad(float z) {
+  const float s = 1.0f / (1.0f + expf(-z));
+  return s * (1.0f + z * (1.0f - s));
+}
+
+// ------------------------------------------------------------------- GEMM
+// C (+)= A B, A(m, k) = A[m sam + k sak], B(k, n) = B[k sbk + n sbn] (BF16),
+// C FP32 row-major with row stride ldc.  Grid (M/32, N/32), 64 threads.
+__global__ __launch_bounds__(64) void k_gemm_bf16(int K, const uint16_t *A, int64_t sam, int64_t sak,
+                                                  const uint16_t *B, int64_t sbk, int64_t sbn, float *C,
+                                                  int64_t ldc, int accumulate) {
+  const int lane = threadIdx.x, r = lane & 31, half = lane >> 5;
+  const int64_t row = (int64_t)blockIdx.x * 32 + r, col = (int64_t)blockIdx.y * 32 + r;
+  f32x16 acc;
+#pragma unroll
+  for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    u16x8 a, b;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int64_t kk = k0 + half * 8 + q;
+      a[q] = A[row * sam + kk * sak];
+      b[q] = B[kk * sbk + col * sbn];
+    }
+    acc = mfma32(a, b, acc);
+  }
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    const int64_t i = (int64_t)blockIdx.x * 32 + (v >> 2) * 8 + half * 4 + (v & 3);
+    float *c = C + i * ldc + col;
+    *c = accumulate ? *c + acc[v] : acc[v];
+  }
+}
+
+static int gemm(int M, int N, int K, const uint16_t *A, int64_t sam, int64_t sak, const uint16_t *B,
+                int64_t sbk, int64_t sbn, float *C, int64_t ldc, int accumulate, hipStream_t s) {
+  if (M % 32 || N % 32 || K % 16 || M <= 0 || N <= 0 || K <= 0) return -1;
+  hipLaunchKernelGGL(k_gemm_bf16, dim3(M / 32, N / 32), dim3(64), 0, s, K, A, sam, sak, B, sbk, sbn,
+                     C, ldc, accumulate);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+// -------------------------------------------------------- element kernels
+__device__ inline void c2p_features(const KastaunIn &k, float f[5]) {
+  const double q = k.q < -0.5 ? -0.5 : k.q;
+  f[0] = (float)log1p(q);
+  f[1] = (float)log1p(k.rsq);
+  f[2] = (float)log1p(k.bsq);
+  f[3] = (float)log1p(k.rbsq);
+  double c = k.rbsq / (k.rsq * k.bsq + 1e-300);
+  c = c < 0.0 ? 0.0 : (c > 1.0 ? 1.0 : c);
+  f[4] = (float)c;
+}
+
+// Minibatch input: standardised features of rows idx[i] as BF16 [n][KIN].
+__global__ void k_gather(int n, const int *idx, const float *feats, const float *target,
+                         const float *mean, const float *dev, uint16_t *X, float *T) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t s = idx ? idx[i] : i;
+  for (int q = 0; q < KIN; ++q) {
+    float x = 0.0f;
+    if (q < PANGU_MLP_IN) x = (feats[s * PANGU_MLP_IN + q] - mean[q]) / dev[q];
+    X[(int64_t)i * KIN + q] = f2bf(x);
+  }
+  if (T) T[i] = target ? target[s] : 0.0f;
+}
+
+// Z += bias; A = BF16(silu(Z)) when A is given.
+__global__ void k_bias_act(int n, int cols, float *Z, const float *bias, uint16_t *A) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)n * cols) return;
+  const float z = Z[e] + bias[e % cols];
+  Z[e] = z;
+  if (A) A[e] = f2bf(silu(z));
+}
+
+// Squared-error loss on output column 0: dZ4 = 2 (z - t) / n as BF16 (other
+// columns zero); part[block] = partial sum of (z - t)^2.
+__global__ __launch_bounds__(256) void k_loss(int n, const float *Z4, const float *T, uint16_t *dZ4,
+                                              double *part) {
+  __shared__ double sm[256];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double l = 0.0;
+  if (i < n) {
+    const float d = Z4[(int64_t)i * NOUT] - T[i];
+    l = (double)d * d;
+    for (int c = 0; c < NOUT; ++c) dZ4[(int64_t)i * NOUT + c] = c == 0 ? f2bf(2.0f * d / n) : 0;
+  }
+  sm[threadIdx.x] = l;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+// dZ = BF16(dA silu'(Z)).
+__global__ void k_act_back(int64_t ne, const float *dA, const float *Z, uint16_t *dZ) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  dZ[e] = f2bf(dA[e] * silu_grad(Z[e]));
+}
+
+// out[c] = sum_i dZ[i][c] (one workgroup per column).
+__global__ __launch_bounds__(256) void k_colsum(int n, int cols, const uint16_t *dZ, float *out) {
+  __shared__ float sm[256];
+  const int c = blockIdx.x;
+  float s = 0.0f;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += bf2f(dZ[(int64_t)i * cols + c]);
+  sm[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) sm[threadIdx.x] += sm[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = sm[0];
+}
+
+__global__ void k_adam(int64_t n, float *P, const float *G, float *M, float *V, float lr, float b1,
+                       float b2, float eps, float bc1, float bc2) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const float g = G[e];
+  const float m = b1 * M[e] + (1.0f - b1) * g;
+  const float v = b2 * V[e] + (1.0f - b2) * g * g;
+  M[e] = m;
+  V[e] = v;
+  P[e] -= lr * (m / bc1) / (sqrtf(v / bc2) + eps);
+}
+
+__global__ void k_to_bf16(int64_t n, const float *src, uint16_t *dst) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) dst[e] = f2bf(src[e]);
+}
+
+// ------------------------------------------------------- synthetic states
+__device__ inline double unit_draw(uint64_t seed, uint64_t id) {
+  uint64_t z = id + seed * 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  z ^= z >> 31;
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// Conserved state in the form k_recover hands to the solver: alpha/sqrt|g|
+// times U, with the energy slot holding U_E - U_rho.
+__device__ inline void harm_input(double gamma, const double p[NPRIM], const double *mt, double U[9]) {
+  St st;
+  double g[16], gc0[4], f[NPRIM];
+  for (int n = 0; n < 16; ++n) g[n] = mt[n];
+  for (int n = 0; n < 4; ++n) gc0[n] = mt[16 + n];
+  const double sg = mt[32], alpha = mt[33];
+  calc_state(p, g, gc0, alpha, st);
+  contravariant_flux<0>(gamma, p, st, sg, f);
+  const double aos = alpha * (1.0 / sg);
+  U[RHO] = aos * f[RHO];
+  U[ENY] = aos * (f[ENY] - f[RHO]);
+  U[UX1] = aos * f[UX1];
+  U[UX2] = aos * f[UX2];
+  U[UX3] = aos * f[UX3];
+  U[BX1] = aos * f[BX1];
+  U[BX2] = aos * f[BX2];
+  U[BX3] = aos * f[BX3];
+  U[ENT] = aos * f[ENT];
+}
+
+__device__ inline void metric_of(const double *mt, CenMetric &m) {
+  for (int n = 0; n < 16; ++n) {
+    m.gcov[n] = mt[n];
+    m.gcon[n] = mt[16 + n];
+  }
+  m.lapse = mt[34];
+}
+
+__global__ void k_c2p_dataset(int64_t n, uint64_t seed, const double *mtab, int npts, double gamma,
+                              float *feats, float *target, double *Uout, int *midx, int *iters,
+                              int *status) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double u[11];
+  for (int k = 0; k < 11; ++k) u[k] = unit_draw(seed, (uint64_t)i * 16 + k);
+  const double rho = pow(10.0, -6.0 + 6.0 * u[0]);
+  const double ueps = rho * pow(10.0, -4.0 + 5.0 * u[1]);
+  const double W = 1.0 + 19.0 * u[2];
+  const double sigma = pow(10.0, -3.0 + 5.0 * u[3]);
+  int mi = (int)(u[4] * npts);
+  mi = mi < npts ? mi : npts - 1;
+  const double *mt = mtab + (int64_t)mi * NMET;
+  double p[NPRIM];
+  for (int s = 0; s < NPRIM; ++s) p[s] = 0.0;
+  p[RHO] = rho;
+  p[ENY] = ueps;
+  const double vd[3] = {2.0 * u[5] - 1.0, 2.0 * u[6] - 1.0, 2.0 * u[7] - 1.0};
+  double vq = 0.0;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) vq += mt[4 * (a + 1) + (b + 1)] * vd[a] * vd[b];
+  int ok = vq > 0.0;
+  const double sc = ok ? sqrt((W * W - 1.0) / vq) : 0.0;
+  for (int a = 0; a < 3; ++a) p[UX1 + a] = sc * vd[a];
+  p[ENT] = (gamma - 1.0) * ueps * pow(rho, -gamma);
+  for (int a = 0; a < 3; ++a) p[BX1 + a] = 2.0 * u[8 + a] - 1.0;
+  {
+    St st;
+    double g[16], gc0[4];
+    for (int q = 0; q < 16; ++q) g[q] = mt[q];
+    for (int q = 0; q < 4; ++q) gc0[q] = mt[16 + q];
+    calc_state(p, g, gc0, mt[33], st);
+    const double bscale = st.bsq > 0.0 ? sqrt(sigma * rho / st.bsq) : 0.0;
+    for (int a = 0; a < 3; ++a) p[BX1 + a] *= bscale;
+  }
+  double U[9], Ph[9];
+  harm_input(gamma, p, mt, U);
+  for (int s = 0; s < 9; ++s) Ph[s] = 0.0;
+  CenMetric m;
+  metric_of(mt, m);
+  int it = 0;
+  double mu = 0.0;
+  int st = ok ? c2p_kastaun(U, Ph, gamma, m, it, &mu) : -1;
+  KastaunIn kin;
+  if (st == 0 && kastaun_setup(U, m, kin) != 0) st = -2;
+  if (st == 0 && !(mu > 0.0 && mu < 1.0)) st = -3;
+  status[i] = st;
+  iters[i] = it;
+  midx[i] = mi;
+  for (int s = 0; s < 9; ++s) Uout[i * 9 + s] = U[s];
+  float f[5] = {0, 0, 0, 0, 0};
+  if (st == 0) c2p_features(kin, f);
+  for (int q = 0; q < 5; ++q) feats[i * 5 + q] = f[q];
+  target[i] = st == 0 ? (float)log(mu / (1.0 - mu)) : 0.0f;
+}
+
+// Plain and learned recovery of sample idx[i]; guess[i] from the model.
+__global__ void k_c2p_eval(int n, const int *idx, const double *Uin, const int *midx,
+                           const double *mtab, const float *guess, double gamma, double delta,
+                           int *it_plain, int *it_learn, int *ev_learn, double *resid, double *rdiff,
+                           int *fail) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t s = idx[i];
+  double U[9], Pa[9], Pb[9];
+  for (int q = 0; q < 9; ++q) {
+    U[q] = Uin[s * 9 + q];
+    Pa[q] = 0.0;
+    Pb[q] = 0.0;
+  }
+  CenMetric m;
+  metric_of(mtab + (int64_t)midx[s] * NMET, m);
+  int ia = 0, ib = 0, eb = 0;
+  double mub = 0.0;
+  const int sa = c2p_kastaun(U, Pa, gamma, m, ia);
+  const int sb = c2p_kastaun_guess(U, Pb, gamma, m, (double)guess[i], delta, ib, eb, &mub);
+  it_plain[i] = ia;
+  it_learn[i] = ib;
+  ev_learn[i] = eb;
+  fail[i] = (sa == 0 && sb != 0) ? 1 : 0;
+  double r = 0.0, d = 0.0;
+  if (sa == 0 && sb == 0) {
+    KastaunIn k;
+    kastaun_setup(U, m, k);
+    r = fabs(kastaun_root(mub, k.D, k.q, k.bsq, k.rsq, k.rbsq, gamma));
+    for (int q = 0; q < 5; ++q) {
+      const double den = fabs(Pa[q]) > 1e-300 ? fabs(Pa[q]) : 1e-300;
+      const double e = fabs(Pb[q] - Pa[q]) / den;
+      d = e > d ? e : d;
+    }
+  }
+  resid[i] = r;
+  rdiff[i] = d;
+}
+
+// --------------------------------------------------- fused inference kernel
+// One wavefront evaluates the network for its 64 cells: layer 1 (5 inputs)
+// and layer 4 (1 output) on the vector ALU, layers 2 and 3 as MFMA tiles
+// through LDS (rows padded to 72 BF16 to spread banks).
+constexpr int LROW = 72;
+
+__device__ inline void mfma_layer(const uint16_t *in, uint16_t *out, const uint16_t *W,
+                                  const float *bias, int lane) {
+  const int r = lane & 31, half = lane >> 5;
+#pragma unroll 1
+  for (int g = 0; g < 2; ++g)
+#pragma unroll 1
+    for (int nt = 0; nt < 2; ++nt) {
+      f32x16 acc;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
+#pragma unroll
+      for (int k0 = 0; k0 < NH; k0 += 16) {
+        u16x8 a, b;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int kk = k0 + half * 8 + q;
+          a[q] = in[(g * 32 + r) * LROW + kk];
+          b[q] = W[kk * NH + nt * 32 + r];
+        }
+        acc = mfma32(a, b, acc);
+      }
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int i = (v >> 2) * 8 + half * 4 + (v & 3);
+        const int j = nt * 32 + r;
+        out[(g * 32 + i) * LROW + j] = f2bf(silu(acc[v] + bias[j]));
+      }
+    }
+}
+
+__global__ __launch_bounds__(256, 2) void k_c2p_guess(KCtx C, KFields F, MlpDev Wt, float *const *guess,
+                                                   Range3 R) {
+  __shared__ uint16_t lds[4][2][64 * LROW];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint16_t *buf0 = lds[wave][0], *buf1 = lds[wave][1];
+  int b = 0, k = 0, j = 0, i = 0;
+  const bool cell = decode_tile(C, R, b, k, j, i);
+  float x[5] = {0, 0, 0, 0, 0};
+  bool valid = false;
+  int64_t id = 0;
+  if (cell) {
+    const DevBlock &B = C.blk[b];
+    id = ((int64_t)k * B.n[1] + j) * B.n[0] + i;
+    const DevGeo &G = C.geo[B.geo];
+    const int64_t t = ((int64_t)(G.has3d ? k : 0) * G.nj + j) * G.ni + i;
+    const int64_t ns = G.ncen;
+    CenMetric m;
+    for (int n = 0; n < 16; ++n) {
+      m.gcov[n] = G.cen[(CG_GCOV + n) * ns + t];
+      m.gcon[n] = G.cen[(CG_GCON + n) * ns + t];
+    }
+    m.lapse = G.cen[CG_LAPSE * ns + t];
+    const double sg = G.cen[CG_SG * ns + t];
+    const double alpha = G.cen[CG_ALPHA * ns + t];
+    const int NC = C.nc;
+    const double *const *cons = F.cons + (int64_t)b * NC;
+    const double aos = alpha * (1.0 / sg);
+    double U[9];
+    U[RHO] = aos * cons[0][id];
+    U[ENY] = aos * (cons[1][id] - cons[0][id]);
+    U[UX1] = aos * cons[2][id];
+    U[UX2] = aos * cons[3][id];
+    U[UX3] = aos * cons[4][id];
+    for (int a = 0; a < 3; ++a) U[BX1 + a] = C.iBX >= 0 ? aos * cons[C.iBX + a][id] : 0.0;
+    U[ENT] = aos * cons[C.iENT][id];
+    KastaunIn kin;
+    if (kastaun_setup(U, m, kin) == 0) {
+      float f[5];
+      c2p_features(kin, f);
+      for (int q = 0; q < 5; ++q) x[q] = bf2f(f2bf((f[q] - Wt.mean[q]) / Wt.dev[q]));
+      valid = true;
+    }
+  }
+  // layer 1 (vector ALU): BF16 inputs and weights, FP32 accumulation
+  for (int jn = 0; jn < NH; ++jn) {
+    float z = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) z += x[q] * bf2f(Wt.w1[q * NH + jn]);
+    buf0[lane * LROW + jn] = f2bf(silu(z + Wt.b1[jn]));
+  }
+  __syncthreads();
+  mfma_layer(buf0, buf1, Wt.w2, Wt.b2, lane);
+  __syncthreads();
+  mfma_layer(buf1, buf0, Wt.w3, Wt.b3, lane);
+  __syncthreads();
+  float z = 0.0f;
+  for (int jn = 0; jn < NH; ++jn) z += bf2f(buf0[lane * LROW + jn]) * bf2f(Wt.w4[jn * NOUT]);
+  z += Wt.b4[0];
+  if (cell) guess[b][id] = valid ? 1.0f / (1.0f + expf(-z)) : -1.0f;
+}
+
+int launch_c2p_guess(const LaunchCtx &L, const KFields &F, const MlpDev &W, float *const *guess,
+                     hipStream_t s) {
+  const Range3 &R = L.interior;
+  if (R.ntiles == 0) return 0;
+  hipLaunchKernelGGL(k_c2p_guess, dim3((unsigned)(((R.ntiles + 7) / 8) * 8)), dim3(256), 0, s, L.k, F, W,
+                     guess, R);
+  if (L.clock) (void)L.clock->mark("c2p_guess", s);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+}  // namespace pangu
+
+using namespace pangu;
+
+// --------------------------------------------------------------- host side
+struct pangu_mlp {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  uint64_t seed = 0;
+  // parameters (FP32 masters), gradients, Adam moments: one arena each,
+  // laid out W1 [KIN][NH], b1, W2, b2, W3, b3, W4 [NH][NOUT], b4 [NOUT]
+  float *P = nullptr, *G = nullptr, *M = nullptr, *V = nullptr;
+  uint16_t *W1h = nullptr, *W2h = nullptr, *W3h = nullptr, *W4h = nullptr;
+  float *mean = nullptr, *dev = nullptr;  // [5] each
+  bool stats_set = false, have_weights = false;
+  int adam_t = 0;
+  // data set
+  int64_t n = 0;
+  float *feats = nullptr, *target = nullptr;
+  double *U = nullptr, *mtab = nullptr;
+  int *midx = nullptr, *iters = nullptr, *status = nullptr;
+  std::vector<int> kept;
+  int *dkept = nullptr;
+  double gamma = 5.0 / 3.0;
+  // minibatch work buffers
+  int cap = 0;
+  uint16_t *X = nullptr, *A1 = nullptr, *A2 = nullptr, *A3 = nullptr, *dZ4 = nullptr, *dZ = nullptr;
+  float *T = nullptr, *Z1 = nullptr, *Z2 = nullptr, *Z3 = nullptr, *Z4 = nullptr, *dA = nullptr;
+  double *part = nullptr;
+  int *bidx = nullptr;
+};
+
+namespace {
+constexpr int64_t OFF_W1 = 0, OFF_B1 = OFF_W1 + KIN * NH, OFF_W2 = OFF_B1 + NH,
+                  OFF_B2 = OFF_W2 + NH * NH, OFF_W3 = OFF_B2 + NH, OFF_B3 = OFF_W3 + NH * NH,
+                  OFF_W4 = OFF_B3 + NH, OFF_B4 = OFF_W4 + NH * NOUT, NPAR = OFF_B4 + NOUT;
+
+int mfail(pangu_mlp *m, int code, const std::string &msg) {
+  if (m) m->err = msg;
+  return code;
+}
+#define MCHK(m, call)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess) return mfail(m, PANGU_EDEVICE, std::string(#call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+#define MRUN(m, call)                                                   \
+  do {                                                                  \
+    if ((call) != 0) return mfail(m, PANGU_EDEVICE, "launch failed: " #call); \
+  } while (0)
+
+template <class T>
+int dmalloc(pangu_mlp *m, T **p, int64_t n) {
+  MCHK(m, hipMalloc((void **)p, (size_t)(n > 0 ? n : 1) * sizeof(T)));
+  MCHK(m, hipMemset(*p, 0, (size_t)(n > 0 ? n : 1) * sizeof(T)));
+  return 0;
+}
+unsigned blocks_for(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+int refresh_bf16(pangu_mlp *m) {
+  hipStream_t s = m->stream;
+  hipLaunchKernelGGL(k_to_bf16, dim3(blocks_for(KIN * NH)), dim3(256), 0, s, (int64_t)KIN * NH, m->P + OFF_W1, m->W1h);
+  hipLaunchKernelGGL(k_to_bf16, dim3(blocks_for(NH * NH)), dim3(256), 0, s, (int64_t)NH * NH, m->P + OFF_W2, m->W2h);
+  hipLaunchKernelGGL(k_to_bf16, dim3(blocks_for(NH * NH)), dim3(256), 0, s, (int64_t)NH * NH, m->P + OFF_W3, m->W3h);
+  hipLaunchKernelGGL(k_to_bf16, dim3(blocks_for(NH * NOUT)), dim3(256), 0, s, (int64_t)NH * NOUT, m->P + OFF_W4, m->W4h);
+  MCHK(m, hipGetLastError());
+  return 0;
+}
+
+int ensure_batch(pangu_mlp *m, int B) {
+  if (B <= m->cap) return 0;
+  void *ptrs[] = {m->X, m->A1, m->A2, m->A3, m->dZ4, m->dZ, m->T, m->Z1, m->Z2, m->Z3, m->Z4, m->dA, m->part, m->bidx};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  m->cap = B;
+  if (dmalloc(m, &m->X, (int64_t)B * KIN) || dmalloc(m, &m->A1, (int64_t)B * NH) ||
+      dmalloc(m, &m->A2, (int64_t)B * NH) || dmalloc(m, &m->A3, (int64_t)B * NH) ||
+      dmalloc(m, &m->dZ4, (int64_t)B * NOUT) || dmalloc(m, &m->dZ, (int64_t)B * NH) ||
+      dmalloc(m, &m->T, B) || dmalloc(m, &m->Z1, (int64_t)B * NH) || dmalloc(m, &m->Z2, (int64_t)B * NH) ||
+      dmalloc(m, &m->Z3, (int64_t)B * NH) || dmalloc(m, &m->Z4, (int64_t)B * NOUT) ||
+      dmalloc(m, &m->dA, (int64_t)B * NH) || dmalloc(m, &m->part, blocks_for(B)) || dmalloc(m, &m->bidx, B))
+    return PANGU_ENOMEM;
+  return 0;
+}
+
+// Forward pass of the minibatch in X (B rows): Z1..Z4, A1..A3.
+int forward(pangu_mlp *m, int B) {
+  hipStream_t s = m->stream;
+  MRUN(m, gemm(B, NH, KIN, m->X, KIN, 1, m->W1h, NH, 1, m->Z1, NH, 0, s));
+  hipLaunchKernelGGL(k_bias_act, dim3(blocks_for((int64_t)B * NH)), dim3(256), 0, s, B, NH, m->Z1, m->P + OFF_B1, m->A1);
+  MRUN(m, gemm(B, NH, NH, m->A1, NH, 1, m->W2h, NH, 1, m->Z2, NH, 0, s));
+  hipLaunchKernelGGL(k_bias_act, dim3(blocks_for((int64_t)B * NH)), dim3(256), 0, s, B, NH, m->Z2, m->P + OFF_B2, m->A2);
+  MRUN(m, gemm(B, NH, NH, m->A2, NH, 1, m->W3h, NH, 1, m->Z3, NH, 0, s));
+  hipLaunchKernelGGL(k_bias_act, dim3(blocks_for((int64_t)B * NH)), dim3(256), 0, s, B, NH, m->Z3, m->P + OFF_B3, m->A3);
+  MRUN(m, gemm(B, NOUT, NH, m->A3, NH, 1, m->W4h, NOUT, 1, m->Z4, NOUT, 0, s));
+  hipLaunchKernelGGL(k_bias_act, dim3(blocks_for((int64_t)B * NOUT)), dim3(256), 0, s, B, NOUT, m->Z4, m->P + OFF_B4, (uint16_t *)nullptr);
+  MCHK(m, hipGetLastError());
+  return 0;
+}
+
+// Loss and backward pass (targets in T): gradients into G; returns the
+// summed squared error in *sse.
+int backward(pangu_mlp *m, int B, double *sse) {
+  hipStream_t s = m->stream;
+  const unsigned nb = blocks_for(B);
+  hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, s, B, m->Z4, m->T, m->dZ4, m->part);
+  // layer 4
+  MRUN(m, gemm(NH, NOUT, B, m->A3, 1, NH, m->dZ4, NOUT, 1, m->G + OFF_W4, NOUT, 0, s));
+  hipLaunchKernelGGL(k_colsum, dim3(NOUT), dim3(256), 0, s, B, NOUT, m->dZ4, m->G + OFF_B4);
+  MRUN(m, gemm(B, NH, NOUT, m->dZ4, NOUT, 1, m->W4h, 1, NOUT, m->dA, NH, 0, s));
+  // layer 3
+  hipLaunchKernelGGL(k_act_back, dim3(blocks_for((int64_t)B * NH)), dim3(256), 0, s, (int64_t)B * NH, m->dA, m->Z3, m->dZ);
+  MRUN(m, gemm(NH, NH, B, m->A2, 1, NH, m->dZ, NH, 1, m->G + OFF_W3, NH, 0, s));
+  hipLaunchKernelGGL(k_colsum, dim3(NH), dim3(256), 0, s, B, NH, m->dZ, m->G + OFF_B3);
+  MRUN(m, gemm(B, NH, NH, m->dZ, NH, 1, m->W3h, 1, NH, m->dA, NH, 0, s));
+  // layer 2
+  hipLaunchKernelGGL(k_act_back, dim3(blocks_for((int64_t)B * NH)), dim3(256), 0, s, (int64_t)B * NH, m->dA, m->Z2, m->dZ);
+  MRUN(m, gemm(NH, NH, B, m->A1, 1, NH, m->dZ, NH, 1, m->G + OFF_W2, NH, 0, s));
+  hipLaunchKernelGGL(k_colsum, dim3(NH), dim3(256), 0, s, B, NH, m->dZ, m->G + OFF_B2);
+  MRUN(m, gemm(B, NH, NH, m->dZ, NH, 1, m->W2h, 1, NH, m->dA, NH, 0, s));
+  // layer 1
+  hipLaunchKernelGGL(k_act_back, dim3(blocks_for((int64_t)B * NH)), dim3(256), 0, s, (int64_t)B * NH, m->dA, m->Z1, m->dZ);
+  MRUN(m, gemm(KIN, NH, B, m->X, 1, KIN, m->dZ, NH, 1, m->G + OFF_W1, NH, 0, s));
+  hipLaunchKernelGGL(k_colsum, dim3(NH), dim3(256), 0, s, B, NH, m->dZ, m->G + OFF_B1);
+  MCHK(m, hipGetLastError());
+  std::vector<double> part(nb);
+  MCHK(m, hipMemcpyAsync(part.data(), m->part, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+  MCHK(m, hipStreamSynchronize(s));
+  *sse = std::accumulate(part.begin(), part.end(), 0.0);
+  return 0;
+}
+
+int adam_step(pangu_mlp *m, float lr) {
+  m->adam_t += 1;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float bc1 = 1.0f - std::pow(b1, (float)m->adam_t), bc2 = 1.0f - std::pow(b2, (float)m->adam_t);
+  hipLaunchKernelGGL(k_adam, dim3(blocks_for(NPAR)), dim3(256), 0, m->stream, (int64_t)NPAR, m->P, m->G,
+                     m->M, m->V, lr, b1, b2, 1e-8f, bc1, bc2);
+  MCHK(m, hipGetLastError());
+  return refresh_bf16(m);
+}
+}  // namespace
+
+namespace pangu {
+bool mlp_view(const pangu_mlp *m, MlpDev &out) {
+  if (!m || !m->have_weights) return false;
+  out.w1 = m->W1h;
+  out.w2 = m->W2h;
+  out.w3 = m->W3h;
+  out.w4 = m->W4h;
+  out.b1 = m->P + OFF_B1;
+  out.b2 = m->P + OFF_B2;
+  out.b3 = m->P + OFF_B3;
+  out.b4 = m->P + OFF_B4;
+  out.mean = m->mean;
+  out.dev = m->dev;
+  return true;
+}
+}  // namespace pangu
+
+extern "C" {
+
+int pangu_mlp_create(int device, uint64_t seed, pangu_mlp **out) {
+  if (!out) return PANGU_EINVAL;
+  *out = nullptr;
+  if (hipSetDevice(device) != hipSuccess) return PANGU_EDEVICE;
+  pangu_mlp *m = new pangu_mlp();
+  m->device = device;
+  m->seed = seed;
+  if (hipStreamCreate(&m->stream) != hipSuccess || dmalloc(m, &m->P, NPAR) || dmalloc(m, &m->G, NPAR) ||
+      dmalloc(m, &m->M, NPAR) || dmalloc(m, &m->V, NPAR) || dmalloc(m, &m->W1h, KIN * NH) ||
+      dmalloc(m, &m->W2h, NH * NH) || dmalloc(m, &m->W3h, NH * NH) || dmalloc(m, &m->W4h, NH * NOUT) ||
+      dmalloc(m, &m->mean, 5) || dmalloc(m, &m->dev, 5)) {
+    pangu_mlp_destroy(m);
+    return PANGU_ENOMEM;
+  }
+  // Uniform(-1/sqrt(fan_in), 1/sqrt(fan_in)) weights, zero biases; padded
+  // rows and columns stay zero.
+  std::vector<float> P(NPAR, 0.0f);
+  std::mt19937_64 rng(seed);
+  std::uniform_real_distribution<float> U(-1.0f, 1.0f);
+  for (int k = 0; k < PANGU_MLP_IN; ++k)
+    for (int j = 0; j < NH; ++j) P[OFF_W1 + k * NH + j] = U(rng) / std::sqrt((float)PANGU_MLP_IN);
+  for (int k = 0; k < NH; ++k)
+    for (int j = 0; j < NH; ++j) {
+      P[OFF_W2 + k * NH + j] = U(rng) / 8.0f;
+      P[OFF_W3 + k * NH + j] = U(rng) / 8.0f;
+    }
+  for (int k = 0; k < NH; ++k) P[OFF_W4 + k * NOUT] = U(rng) / 8.0f;
+  const float one[5] = {1, 1, 1, 1, 1};
+  if (hipMemcpy(m->P, P.data(), NPAR * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(m->dev, one, sizeof(one), hipMemcpyHostToDevice) != hipSuccess || refresh_bf16(m)) {
+    pangu_mlp_destroy(m);
+    return PANGU_EDEVICE;
+  }
+  (void)hipStreamSynchronize(m->stream);
+  m->have_weights = true;
+  *out = m;
+  return PANGU_OK;
+}
+
+int pangu_mlp_destroy(pangu_mlp *m) {
+  if (!m) return PANGU_OK;
+  (void)hipSetDevice(m->device);
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  void *ptrs[] = {m->P,  m->G,    m->M,    m->V,  m->W1h,  m->W2h,   m->W3h, m->W4h, m->mean,
+                  m->dev, m->feats, m->target, m->U, m->mtab, m->midx, m->iters, m->status, m->dkept,
+                  m->X,  m->A1,   m->A2,   m->A3, m->dZ4,  m->dZ,    m->T,   m->Z1,  m->Z2,
+                  m->Z3, m->Z4,   m->dA,   m->part, m->bidx};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+  return PANGU_OK;
+}
+
+const char *pangu_mlp_last_error(const pangu_mlp *m) { return m ? m->err.c_str() : "null model"; }
+
+// API layout <-> padded device layout.
+static void api_to_dev(const float *in, std::vector<float> &P, float mean[5], float dev[5]) {
+  P.assign(NPAR, 0.0f);
+  const float *q = in;
+  for (int k = 0; k < PANGU_MLP_IN; ++k)
+    for (int j = 0; j < NH; ++j) P[OFF_W1 + k * NH + j] = *q++;
+  for (int j = 0; j < NH; ++j) P[OFF_B1 + j] = *q++;
+  for (int k = 0; k < NH * NH; ++k) P[OFF_W2 + k] = *q++;
+  for (int j = 0; j < NH; ++j) P[OFF_B2 + j] = *q++;
+  for (int k = 0; k < NH * NH; ++k) P[OFF_W3 + k] = *q++;
+  for (int j = 0; j < NH; ++j) P[OFF_B3 + j] = *q++;
+  for (int k = 0; k < NH; ++k) P[OFF_W4 + k * NOUT] = *q++;
+  P[OFF_B4] = *q++;
+  for (int i = 0; i < 5; ++i) mean[i] = *q++;
+  for (int i = 0; i < 5; ++i) dev[i] = *q++;
+}
+static void dev_to_api(const std::vector<float> &P, const float mean[5], const float dev[5], float *out) {
+  float *q = out;
+  for (int k = 0; k < PANGU_MLP_IN; ++k)
+    for (int j = 0; j < NH; ++j) *q++ = P[OFF_W1 + k * NH + j];
+  for (int j = 0; j < NH; ++j) *q++ = P[OFF_B1 + j];
+  for (int k = 0; k < NH * NH; ++k) *q++ = P[OFF_W2 + k];
+  for (int j = 0; j < NH; ++j) *q++ = P[OFF_B2 + j];
+  for (int k = 0; k < NH * NH; ++k) *q++ = P[OFF_W3 + k];
+  for (int j = 0; j < NH; ++j) *q++ = P[OFF_B3 + j];
+  for (int k = 0; k < NH; ++k) *q++ = P[OFF_W4 + k * NOUT];
+  *q++ = P[OFF_B4];
+  for (int i = 0; i < 5; ++i) *q++ = mean[i];
+  for (int i = 0; i < 5; ++i) *q++ = dev[i];
+}
+
+int pangu_mlp_get_params(pangu_mlp *m, float *out) {
+  if (!m || !out) return PANGU_EINVAL;
+  std::vector<float> P(NPAR);
+  float mean[5], dev[5];
+  MCHK(m, hipStreamSynchronize(m->stream));
+  MCHK(m, hipMemcpy(P.data(), m->P, NPAR * sizeof(float), hipMemcpyDeviceToHost));
+  MCHK(m, hipMemcpy(mean, m->mean, sizeof(mean), hipMemcpyDeviceToHost));
+  MCHK(m, hipMemcpy(dev, m->dev, sizeof(dev), hipMemcpyDeviceToHost));
+  dev_to_api(P, mean, dev, out);
+  return PANGU_OK;
+}
+
+int pangu_mlp_set_params(pangu_mlp *m, const float *in) {
+  if (!m || !in) return PANGU_EINVAL;
+  std::vector<float> P;
+  float mean[5], dev[5];
+  api_to_dev(in, P, mean, dev);
+  for (int i = 0; i < 5; ++i)
+    if (!(dev[i] > 0.0f)) return mfail(m, PANGU_EINVAL, "feature deviations must be positive");
+  MCHK(m, hipStreamSynchronize(m->stream));
+  MCHK(m, hipMemcpy(m->P, P.data(), NPAR * sizeof(float), hipMemcpyHostToDevice));
+  MCHK(m, hipMemcpy(m->mean, mean, sizeof(mean), hipMemcpyHostToDevice));
+  MCHK(m, hipMemcpy(m->dev, dev, sizeof(dev), hipMemcpyHostToDevice));
+  m->stats_set = true;
+  if (int r = refresh_bf16(m)) return r;
+  MCHK(m, hipStreamSynchronize(m->stream));
+  return PANGU_OK;
+}
+
+int pangu_mlp_make_dataset(pangu_mlp *m, int64_t n, uint64_t seed, double a, double h, double gamma,
+                           int64_t *kept) {
+  if (!m || n <= 0 || n > (int64_t)1 << 30) return PANGU_EINVAL;
+  (void)hipSetDevice(m->device);
+  void *old[] = {m->feats, m->target, m->U, m->mtab, m->midx, m->iters, m->status, m->dkept};
+  for (void *p : old)
+    if (p) (void)hipFree(p);
+  m->feats = m->target = nullptr;
+  m->U = m->mtab = nullptr;
+  m->midx = m->iters = m->status = m->dkept = nullptr;
+  m->gamma = gamma;
+  // Metric sample points: Kerr in MKS code coordinates at r ~ U(1.3, 50),
+  // x2 ~ U(-0.98, 0.98), evaluated on the host like the geometry tables.
+  const int npts = 8192;
+  std::vector<double> mt((size_t)npts * NMET);
+  pangu_config pc;
+  std::memset(&pc, 0, sizeof(pc));
+  pc.metric = PANGU_METRIC_MKS;
+  pc.a = a;
+  pc.h = h;
+  std::mt19937_64 rng(seed ^ 0x5bd1e995ULL);
+  std::uniform_real_distribution<double> ur(1.3, 50.0), ux(-0.98, 0.98);
+  std::vector<double> xs((size_t)npts * 2);
+  for (int i = 0; i < npts; ++i) {
+    xs[2 * i] = std::log(ur(rng));
+    xs[2 * i + 1] = ux(rng);
+  }
+  geom::fill_metric_points(pc, xs.data(), npts, mt.data());
+  if (dmalloc(m, &m->mtab, (int64_t)npts * NMET) || dmalloc(m, &m->feats, n * 5) ||
+      dmalloc(m, &m->target, n) || dmalloc(m, &m->U, n * 9) || dmalloc(m, &m->midx, n) ||
+      dmalloc(m, &m->iters, n) || dmalloc(m, &m->status, n))
+    return PANGU_ENOMEM;
+  MCHK(m, hipMemcpy(m->mtab, mt.data(), mt.size() * sizeof(double), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_c2p_dataset, dim3(blocks_for(n)), dim3(256), 0, m->stream, n, seed, m->mtab, npts,
+                     gamma, m->feats, m->target, m->U, m->midx, m->iters, m->status);
+  MCHK(m, hipGetLastError());
+  std::vector<int> st(n);
+  MCHK(m, hipMemcpyAsync(st.data(), m->status, n * sizeof(int), hipMemcpyDeviceToHost, m->stream));
+  MCHK(m, hipStreamSynchronize(m->stream));
+  m->kept.clear();
+  for (int64_t i = 0; i < n; ++i)
+    if (st[i] == 0) m->kept.push_back((int)i);
+  m->n = n;
+  if (dmalloc(m, &m->dkept, (int64_t)m->kept.size())) return PANGU_ENOMEM;
+  MCHK(m, hipMemcpy(m->dkept, m->kept.data(), m->kept.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (kept) *kept = (int64_t)m->kept.size();
+  return PANGU_OK;
+}
+
+// Feature standardisation and output bias from the data set (first call).
+static int set_stats(pangu_mlp *m) {
+  const int64_t nk = (int64_t)m->kept.size();
+  std::vector<float> f(m->n * 5), t(m->n);
+  MCHK(m, hipMemcpy(f.data(), m->feats, f.size() * sizeof(float), hipMemcpyDeviceToHost));
+  MCHK(m, hipMemcpy(t.data(), m->target, t.size() * sizeof(float), hipMemcpyDeviceToHost));
+  double s[5] = {0}, s2[5] = {0}, st = 0.0;
+  for (int64_t q = 0; q < nk; ++q) {
+    const int i = m->kept[q];
+    for (int c = 0; c < 5; ++c) {
+      s[c] += f[(size_t)i * 5 + c];
+      s2[c] += (double)f[(size_t)i * 5 + c] * f[(size_t)i * 5 + c];
+    }
+    st += t[i];
+  }
+  float mean[5], dev[5];
+  for (int c = 0; c < 5; ++c) {
+    mean[c] = (float)(s[c] / nk);
+    const double var = s2[c] / nk - (s[c] / nk) * (s[c] / nk);
+    dev[c] = (float)std::sqrt(var > 1e-12 ? var : 1e-12);
+  }
+  const float b4 = (float)(st / nk);
+  MCHK(m, hipMemcpy(m->mean, mean, sizeof(mean), hipMemcpyHostToDevice));
+  MCHK(m, hipMemcpy(m->dev, dev, sizeof(dev), hipMemcpyHostToDevice));
+  MCHK(m, hipMemcpy(m->P + OFF_B4, &b4, sizeof(float), hipMemcpyHostToDevice));
+  m->stats_set = true;
+  return 0;
+}
+
+int pangu_mlp_train(pangu_mlp *m, int epochs, int batch, float lr, float *loss_out) {
+  if (!m || epochs < 0 || batch <= 0 || batch % 32) return PANGU_EINVAL;
+  if (m->kept.empty()) return mfail(m, PANGU_ESTATE, "no data set");
+  (void)hipSetDevice(m->device);
+  if (!m->stats_set)
+    if (int r = set_stats(m)) return r;
+  if (int r = ensure_batch(m, batch)) return r;
+  std::vector<int> order(m->kept);
+  std::mt19937_64 rng(m->seed + 17);
+  const int64_t nb = (int64_t)order.size() / batch;
+  if (nb == 0) return mfail(m, PANGU_EINVAL, "data set smaller than one batch");
+  for (int e = 0; e < epochs; ++e) {
+    std::shuffle(order.begin(), order.end(), rng);
+    double sse = 0.0;
+    for (int64_t q = 0; q < nb; ++q) {
+      MCHK(m, hipMemcpyAsync(m->bidx, order.data() + q * batch, batch * sizeof(int), hipMemcpyHostToDevice, m->stream));
+      hipLaunchKernelGGL(k_gather, dim3(blocks_for(batch)), dim3(256), 0, m->stream, batch, m->bidx, m->feats,
+                         m->target, m->mean, m->dev, m->X, m->T);
+      if (int r = forward(m, batch)) return r;
+      double b = 0.0;
+      if (int r = backward(m, batch, &b)) return r;
+      sse += b;
+      if (int r = adam_step(m, lr)) return r;
+    }
+    if (loss_out) loss_out[e] = (float)(sse / ((double)nb * batch));
+  }
+  MCHK(m, hipStreamSynchronize(m->stream));
+  return PANGU_OK;
+}
+
+int pangu_mlp_gradient(pangu_mlp *m, const float *feats, const float *target, int n, float *loss, float *grad) {
+  if (!m || !feats || !target || n <= 0 || n % 32 || !grad) return PANGU_EINVAL;
+  (void)hipSetDevice(m->device);
+  if (int r = ensure_batch(m, n)) return r;
+  float *df = nullptr, *dt = nullptr;
+  if (dmalloc(m, &df, (int64_t)n * 5) || dmalloc(m, &dt, n)) return PANGU_ENOMEM;
+  int rc = 0;
+  if (hipMemcpy(df, feats, (size_t)n * 5 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dt, target, (size_t)n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+    rc = mfail(m, PANGU_EDEVICE, "copy failed");
+  double sse = 0.0;
+  if (!rc) {
+    hipLaunchKernelGGL(k_gather, dim3(blocks_for(n)), dim3(256), 0, m->stream, n, (const int *)nullptr, df, dt,
+                       m->mean, m->dev, m->X, m->T);
+    rc = forward(m, n);
+    if (!rc) rc = backward(m, n, &sse);
+  }
+  (void)hipFree(df);
+  (void)hipFree(dt);
+  if (rc) return rc;
+  std::vector<float> G(NPAR);
+  MCHK(m, hipMemcpy(G.data(), m->G, NPAR * sizeof(float), hipMemcpyDeviceToHost));
+  const float zero[5] = {0, 0, 0, 0, 0};
+  dev_to_api(G, zero, zero, grad);
+  if (loss) *loss = (float)(sse / n);
+  return PANGU_OK;
+}
+
+int pangu_mlp_predict(pangu_mlp *m, const float *feats, int n, float *out) {
+  if (!m || !feats || !out || n <= 0) return PANGU_EINVAL;
+  (void)hipSetDevice(m->device);
+  const int B = ((n + 31) / 32) * 32;
+  if (int r = ensure_batch(m, B)) return r;
+  std::vector<float> f((size_t)B * 5, 0.0f);
+  std::memcpy(f.data(), feats, (size_t)n * 5 * sizeof(float));
+  float *df = nullptr;
+  if (dmalloc(m, &df, (int64_t)B * 5)) return PANGU_ENOMEM;
+  int rc = 0;
+  if (hipMemcpy(df, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+    rc = mfail(m, PANGU_EDEVICE, "copy failed");
+  if (!rc) {
+    hipLaunchKernelGGL(k_gather, dim3(blocks_for(B)), dim3(256), 0, m->stream, B, (const int *)nullptr, df,
+                       (const float *)nullptr, m->mean, m->dev, m->X, (float *)nullptr);
+    rc = forward(m, B);
+  }
+  (void)hipFree(df);
+  if (rc) return rc;
+  std::vector<float> z((size_t)B * NOUT);
+  MCHK(m, hipMemcpy(z.data(), m->Z4, z.size() * sizeof(float), hipMemcpyDeviceToHost));
+  for (int i = 0; i < n; ++i) out[i] = 1.0f / (1.0f + std::exp(-z[(size_t)i * NOUT]));
+  return PANGU_OK;
+}
+
+int pangu_mlp_evaluate(pangu_mlp *m, double delta, double *out) {
+  if (!m || !out || !(delta > 0.0 && delta < 1.0)) return PANGU_EINVAL;
+  if (m->kept.empty()) return mfail(m, PANGU_ESTATE, "no data set");
+  (void)hipSetDevice(m->device);
+  const int nk = (int)m->kept.size();
+  // guesses for the kept states, in order, through the training forward pass
+  std::vector<float> f((size_t)m->n * 5), fk((size_t)nk * 5), g(nk);
+  MCHK(m, hipMemcpy(f.data(), m->feats, f.size() * sizeof(float), hipMemcpyDeviceToHost));
+  for (int q = 0; q < nk; ++q)
+    for (int c = 0; c < 5; ++c) fk[(size_t)q * 5 + c] = f[(size_t)m->kept[q] * 5 + c];
+  const int chunk = 1 << 16;
+  for (int q0 = 0; q0 < nk; q0 += chunk) {
+    const int cnt = std::min(chunk, nk - q0);
+    if (int r = pangu_mlp_predict(m, fk.data() + (size_t)q0 * 5, cnt, g.data() + q0)) return r;
+  }
+  float *dg = nullptr;
+  int *ia = nullptr, *ib = nullptr, *eb = nullptr, *fl = nullptr;
+  double *rs = nullptr, *rd = nullptr;
+  if (dmalloc(m, &dg, nk) || dmalloc(m, &ia, nk) || dmalloc(m, &ib, nk) || dmalloc(m, &eb, nk) ||
+      dmalloc(m, &fl, nk) || dmalloc(m, &rs, nk) || dmalloc(m, &rd, nk))
+    return PANGU_ENOMEM;
+  int rc = 0;
+  std::vector<int> va(nk), vb(nk), ve(nk), vf(nk);
+  std::vector<double> vr(nk), vd(nk);
+  if (hipMemcpy(dg, g.data(), (size_t)nk * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+    rc = mfail(m, PANGU_EDEVICE, "copy failed");
+  if (!rc) {
+    hipLaunchKernelGGL(k_c2p_eval, dim3(blocks_for(nk)), dim3(256), 0, m->stream, nk, m->dkept, m->U, m->midx,
+                       m->mtab, dg, m->gamma, delta, ia, ib, eb, rs, rd, fl);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(m->stream) != hipSuccess ||
+        hipMemcpy(va.data(), ia, nk * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(vb.data(), ib, nk * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(ve.data(), eb, nk * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(vf.data(), fl, nk * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(vr.data(), rs, nk * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(vd.data(), rd, nk * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+      rc = mfail(m, PANGU_EDEVICE, "evaluation failed");
+  }
+  void *ptrs[] = {dg, ia, ib, eb, fl, rs, rd};
+  for (void *p : ptrs) (void)hipFree(p);
+  if (rc) return rc;
+  double sa = 0, sb = 0, se = 0, mr = 0, md = 0, nf = 0;
+  for (int q = 0; q < nk; ++q) {
+    sa += va[q];
+    sb += vb[q];
+    se += ve[q];
+    mr = std::max(mr, vr[q]);
+    md = std::max(md, vd[q]);
+    nf += vf[q];
+  }
+  out[0] = sa / nk;
+  out[1] = sb / nk;
+  out[2] = 2.0 + sa / nk;
+  out[3] = se / nk;
+  out[4] = mr;
+  out[5] = md;
+  out[6] = nf;
+  return PANGU_OK;
+}
+
+int pangu_mlp_gemm(int device, int M, int N, int K, const float *A, const float *B, float *C) {
+  if (!A || !B || !C || M <= 0 || N <= 0 || K <= 0 || M % 32 || N % 32 || K % 16) return PANGU_EINVAL;
+  if (hipSetDevice(device) != hipSuccess) return PANGU_EDEVICE;
+  std::vector<uint16_t> a((size_t)M * K), b((size_t)K * N);
+  for (size_t i = 0; i < a.size(); ++i) a[i] = f2bf(A[i]);
+  for (size_t i = 0; i < b.size(); ++i) b[i] = f2bf(B[i]);
+  uint16_t *da = nullptr, *db = nullptr;
+  float *dc = nullptr;
+  int rc = PANGU_OK;
+  if (hipMalloc((void **)&da, a.size() * 2) != hipSuccess || hipMalloc((void **)&db, b.size() * 2) != hipSuccess ||
+      hipMalloc((void **)&dc, (size_t)M * N * 4) != hipSuccess)
+    rc = PANGU_ENOMEM;
+  if (!rc && (hipMemcpy(da, a.data(), a.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
+              hipMemcpy(db, b.data(), b.size() * 2, hipMemcpyHostToDevice) != hipSuccess))
+    rc = PANGU_EDEVICE;
+  if (!rc && gemm(M, N, K, da, K, 1, db, N, 1, dc, N, 0, nullptr) != 0) rc = PANGU_EDEVICE;
+  if (!rc && hipMemcpy(C, dc, (size_t)M * N * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = PANGU_EDEVICE;
+  if (da) (void)hipFree(da);
+  if (db) (void)hipFree(db);
+  if (dc) (void)hipFree(dc);
+  return rc;
+}
+
+}  // extern "C"

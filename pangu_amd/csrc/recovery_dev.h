@@ -359,10 +359,15 @@ __device__ __forceinline__ double kastaun_root(double mu, double D, double q, do
   return mu - 1.0 / (nu + mu * rbarsq);
 }
 
-// scheme_kastaun.h:110-259: Illinois false position on [0,1].
-__device__ int c2p_kastaun(const double U[9], double p[9], double gamma, const CenMetric &m,
-                           int &iters) {
-  iters = 0;
+// Kastaun inputs (scheme_kastaun.h:167-213): D, q = tau/D, r^2, b^2, (r.b)^2
+// and what the primitive reconstruction needs.
+struct KastaunIn {
+  double D, q, bsq, rsq, rbsq, rb, inv_D, sqrt_D;
+  double Su[3], B[3];
+};
+
+// Returns 0, or 1..4 for an invalid metric, D, tau or S/B invariants.
+__device__ __forceinline__ int kastaun_setup(const double U[9], const CenMetric &m, KastaunIn &k) {
   const double g0 = m.gcov[5], g1 = m.gcov[6], g2 = m.gcov[7], g3 = m.gcov[10], g4 = m.gcov[11],
                g5 = m.gcov[15];
   const double det = g0 * g3 * g5 + 2.0 * g1 * g2 * g4 - g0 * g4 * g4 - g1 * g1 * g5 - g2 * g2 * g3;
@@ -380,77 +385,86 @@ __device__ int c2p_kastaun(const double U[9], double p[9], double gamma, const C
   const double tau = -Qdotn - D;
   if (!isfinite(tau)) return 3;
   const double S0 = U[UX1], S1 = U[UX2], S2 = U[UX3];
-  const double Su0 = i0 * S0 + i1 * S1 + i2 * S2;
-  const double Su1 = i1 * S0 + i3 * S1 + i4 * S2;
-  const double Su2 = i2 * S0 + i4 * S1 + i5 * S2;
+  k.Su[0] = i0 * S0 + i1 * S1 + i2 * S2;
+  k.Su[1] = i1 * S0 + i3 * S1 + i4 * S2;
+  k.Su[2] = i2 * S0 + i4 * S1 + i5 * S2;
   const double B0 = U[BX1], B1 = U[BX2], B2 = U[BX3];
+  k.B[0] = B0;
+  k.B[1] = B1;
+  k.B[2] = B2;
   const double Bd0 = g0 * B0 + g1 * B1 + g2 * B2;
   const double Bd1 = g1 * B0 + g3 * B1 + g4 * B2;
   const double Bd2 = g2 * B0 + g4 * B1 + g5 * B2;
   const double Bsq = B0 * Bd0 + B1 * Bd1 + B2 * Bd2;
-  const double Ssq = Su0 * S0 + Su1 * S1 + Su2 * S2;
+  const double Ssq = k.Su[0] * S0 + k.Su[1] * S1 + k.Su[2] * S2;
   const double SdotB = B0 * S0 + B1 * S1 + B2 * S2;
   if (!isfinite(Bsq) || !isfinite(Ssq) || !isfinite(SdotB)) return 4;
-  const double inv_D = 1.0 / D;
-  const double sqrt_D = sqrt(D);
-  const double bsq = Bsq * inv_D;
-  const double rsq = Ssq * inv_D * inv_D;
-  const double rb = SdotB / (D * sqrt_D);
-  const double rbsq = rb * rb;
-  const double q = tau * inv_D;
-  // false position (scheme_kastaun.h:110-165)
-  double lo = 0.0, hi = 1.0;
+  k.D = D;
+  k.inv_D = 1.0 / D;
+  k.sqrt_D = sqrt(D);
+  k.bsq = Bsq * k.inv_D;
+  k.rsq = Ssq * k.inv_D * k.inv_D;
+  k.rb = SdotB / (D * k.sqrt_D);
+  k.rbsq = k.rb * k.rb;
+  k.q = tau * k.inv_D;
+  return 0;
+}
+
+// Illinois false position on [lo, hi] (scheme_kastaun.h:110-165).  Returns
+// 0 with the root in mu, 1 non-finite end values, 2 no sign change,
+// 3 non-finite iterate, 4 no convergence in MAX_NEWT_ITER.  iters counts
+// the false-position steps.
+__device__ __forceinline__ int kastaun_solve(const KastaunIn &k, double gamma, double lo, double hi,
+                                             double &mu, int &iters) {
+  const double D = k.D, q = k.q, bsq = k.bsq, rsq = k.rsq, rbsq = k.rbsq;
+  iters = 0;
   double flo = kastaun_root(lo, D, q, bsq, rsq, rbsq, gamma);
   double fhi = kastaun_root(hi, D, q, bsq, rsq, rbsq, gamma);
-  double mu = 0.0;
-  int rs = -1;
-  if (!isfinite(flo) || !isfinite(fhi)) {
-    rs = 1;
-  } else if (fabs(flo) <= NEWT_TOL) {
+  mu = 0.0;
+  if (!isfinite(flo) || !isfinite(fhi)) return 1;
+  if (fabs(flo) <= NEWT_TOL) {
     mu = lo;
-    rs = 0;
-  } else if (fabs(fhi) <= NEWT_TOL) {
+    return 0;
+  }
+  if (fabs(fhi) <= NEWT_TOL) {
     mu = hi;
-    rs = 0;
-  } else if (flo * fhi > 0.0) {
-    rs = 2;
-  } else {
-    int side = 0;
-    mu = lo;
-    rs = 4;
-    for (int it = 0; it < MAX_NEWT_ITER; ++it) {
-      const double old = mu;
-      mu = (fhi * lo - flo * hi) / (fhi - flo);
-      const double fm = kastaun_root(mu, D, q, bsq, rsq, rbsq, gamma);
-      iters = it + 1;
-      if (!isfinite(mu) || !isfinite(fm)) {
-        rs = 3;
-        break;
+    return 0;
+  }
+  if (flo * fhi > 0.0) return 2;
+  int side = 0;
+  mu = lo;
+  for (int it = 0; it < MAX_NEWT_ITER; ++it) {
+    const double old = mu;
+    mu = (fhi * lo - flo * hi) / (fhi - flo);
+    const double fm = kastaun_root(mu, D, q, bsq, rsq, rbsq, gamma);
+    iters = it + 1;
+    if (!isfinite(mu) || !isfinite(fm)) return 3;
+    if (fabs((mu - old) / (mu + 1.0e-300)) <= NEWT_TOL) return 0;
+    if (fm * flo >= 0.0) {
+      if (side == 1) {
+        const double wgt = 1.0 - fm / flo;
+        fhi = (wgt > 0.0) ? fhi * wgt : 0.5 * fhi;
       }
-      if (fabs((mu - old) / (mu + 1.0e-300)) <= NEWT_TOL) {
-        rs = 0;
-        break;
+      lo = mu;
+      flo = fm;
+      side = 1;
+    } else {
+      if (side == -1) {
+        const double wgt = 1.0 - fm / fhi;
+        flo = (wgt > 0.0) ? flo * wgt : 0.5 * flo;
       }
-      if (fm * flo >= 0.0) {
-        if (side == 1) {
-          const double wgt = 1.0 - fm / flo;
-          fhi = (wgt > 0.0) ? fhi * wgt : 0.5 * fhi;
-        }
-        lo = mu;
-        flo = fm;
-        side = 1;
-      } else {
-        if (side == -1) {
-          const double wgt = 1.0 - fm / fhi;
-          flo = (wgt > 0.0) ? flo * wgt : 0.5 * flo;
-        }
-        hi = mu;
-        fhi = fm;
-        side = -1;
-      }
+      hi = mu;
+      fhi = fm;
+      side = -1;
     }
   }
-  if (rs != 0) return 100 + rs;
+  return 4;
+}
+
+// Primitives from the root (scheme_kastaun.h:215-259); 0, 5 (v >= 1) or
+// 6 (rho or eps not positive).
+__device__ __forceinline__ int kastaun_finish(const KastaunIn &k, double mu, double p[9]) {
+  const double bsq = k.bsq, rsq = k.rsq, rbsq = k.rbsq, q = k.q, D = k.D;
   const double x = 1.0 / (1.0 + mu * bsq);
   const double rbarsq = x * (rsq * x + mu * (x + 1.0) * rbsq);
   const double vsq = mu * mu * rbarsq;
@@ -464,10 +478,57 @@ __device__ int c2p_kastaun(const double U[9], double p[9], double gamma, const C
   const double Wmux = lorentz * mu * x;
   p[RHO] = rho;
   p[ENY] = eps;
-  p[UX1] = Wmux * (Su0 * inv_D + rb * mu * (B0 / sqrt_D));
-  p[UX2] = Wmux * (Su1 * inv_D + rb * mu * (B1 / sqrt_D));
-  p[UX3] = Wmux * (Su2 * inv_D + rb * mu * (B2 / sqrt_D));
+  p[UX1] = Wmux * (k.Su[0] * k.inv_D + k.rb * mu * (k.B[0] / k.sqrt_D));
+  p[UX2] = Wmux * (k.Su[1] * k.inv_D + k.rb * mu * (k.B[1] / k.sqrt_D));
+  p[UX3] = Wmux * (k.Su[2] * k.inv_D + k.rb * mu * (k.B[2] / k.sqrt_D));
   return 0;
+}
+
+// scheme_kastaun.h:110-259: Illinois false position on [0,1].  Status 0,
+// 1..4 from the setup, 101..104 from the root search, 5..6 from the
+// reconstruction.  mu_out (optional) receives the root.
+__device__ int c2p_kastaun(const double U[9], double p[9], double gamma, const CenMetric &m,
+                           int &iters, double *mu_out = nullptr) {
+  iters = 0;
+  KastaunIn k;
+  if (int e = kastaun_setup(U, m, k)) return e;
+  double mu;
+  const int rs = kastaun_solve(k, gamma, 0.0, 1.0, mu, iters);
+  if (rs != 0) return 100 + rs;
+  if (mu_out) *mu_out = mu;
+  return kastaun_finish(k, mu, p);
+}
+
+// Kastaun with a learned starting bracket [g (1 - delta), min(1, g (1 + delta))]
+// around a guess g of the root.  When that bracket does not hold a sign
+// change it falls back to [0, 1], so the root found is a root of the same
+// function; only the false-position path differs (the result agrees with
+// c2p_kastaun to the 1e-10 step tolerance, not bit for bit).  evals counts
+// root-function evaluations, bracket ends included.
+__device__ int c2p_kastaun_guess(const double U[9], double p[9], double gamma, const CenMetric &m,
+                                 double guess, double delta, int &iters, int &evals,
+                                 double *mu_out = nullptr) {
+  iters = 0;
+  evals = 0;
+  KastaunIn k;
+  if (int e = kastaun_setup(U, m, k)) return e;
+  double mu = 0.0;
+  int rs = 2;
+  if (guess > 0.0 && guess <= 1.0) {
+    const double lo = guess * (1.0 - delta);
+    const double hi = kmin(1.0, guess * (1.0 + delta));
+    rs = kastaun_solve(k, gamma, lo, hi, mu, iters);
+    evals = 2 + iters;
+  }
+  if (rs == 1 || rs == 2) {
+    int it2 = 0;
+    rs = kastaun_solve(k, gamma, 0.0, 1.0, mu, it2);
+    iters += it2;
+    evals += 2 + it2;
+  }
+  if (rs != 0) return 100 + rs;
+  if (mu_out) *mu_out = mu;
+  return kastaun_finish(k, mu, p);
 }
 
 }  // namespace pangu

@@ -4,6 +4,7 @@
 #include <cfloat>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -39,8 +40,13 @@ struct pangu_sim {
   uint8_t *mask = nullptr;
   int32_t *status = nullptr, *iters = nullptr;
   int h_base = -1, h_s1 = -1, h_dudt = -1;
-  double *dstate = nullptr;  // {dt, time}
-  double dt = 0, time = 0, dt_est = DBL_MAX;
+  double *dstate = nullptr;  // {dt, time} per mesh
+  double dt = 0, time = 0, dt_est = DBL_MAX;  // mesh 0 (task mode: the only mesh)
+  int nmesh = 1;
+  std::vector<pangu_sim_cfg> mcfg;  // per mesh
+  std::vector<int> bmesh;           // mesh of each block
+  std::vector<int64_t> boff;        // first cell of each block in the arenas
+  std::vector<double> mdt, mtime;   // per mesh dt and time (host copy)
   int ncycle = 0;
   bool stage1_ready = false;
   bool dt_on_device = false;  // the fused path owns dt/time on the device
@@ -84,48 +90,28 @@ static int dalloc(pangu_sim *s, T **p, int64_t n) {
 
 static void sync_dt_from_device(pangu_sim *s) {
   if (!s->dt_on_device) return;
-  double st[2];
-  hipMemcpy(st, s->dstate, sizeof(st), hipMemcpyDeviceToHost);
-  s->dt = st[0];
-  s->time = st[1];
+  std::vector<double> st(2 * (size_t)s->nmesh);
+  (void)hipMemcpy(st.data(), s->dstate, st.size() * sizeof(double), hipMemcpyDeviceToHost);
+  for (int m = 0; m < s->nmesh; ++m) {
+    s->mdt[m] = st[2 * m];
+    s->mtime[m] = st[2 * m + 1];
+  }
+  s->dt = s->mdt[0];
+  s->time = s->mtime[0];
 }
 
 static void push_dt_to_device(pangu_sim *s) {
-  const double st[2] = {s->dt, s->time};
-  hipMemcpy(s->dstate, st, sizeof(st), hipMemcpyHostToDevice);
+  s->mdt[0] = s->dt;
+  s->mtime[0] = s->time;
+  std::vector<double> st(2 * (size_t)s->nmesh);
+  for (int m = 0; m < s->nmesh; ++m) {
+    st[2 * m] = s->mdt[m];
+    st[2 * m + 1] = s->mtime[m];
+  }
+  (void)hipMemcpy(s->dstate, st.data(), st.size() * sizeof(double), hipMemcpyHostToDevice);
 }
 
-extern "C" {
-
-int pangu_sim_create(const pangu_sim_cfg *c, int device, pangu_sim **out) {
-  if (!c || !out) return PANGU_EINVAL;
-  *out = nullptr;
-  for (int d = 0; d < 3; ++d)
-    if (c->nb[d] <= 0 || c->nx[d] % c->nb[d] != 0) return PANGU_EINVAL;
-  pangu_sim *s = new pangu_sim();
-  s->cfg = *c;
-  s->device = device;
-  if (hipSetDevice(device) != hipSuccess) {
-    delete s;
-    return PANGU_EDEVICE;
-  }
-  s->nblk = c->nb[0] * c->nb[1] * c->nb[2];
-  for (int b = 0; b < s->nblk; ++b) {
-    pangu_block B;
-    const int lx[3] = {b % c->nb[0], (b / c->nb[0]) % c->nb[1], b / (c->nb[0] * c->nb[1])};
-    for (int d = 0; d < 3; ++d) {
-      B.lx[d] = lx[d];
-      B.nx[d] = c->nx[d] / c->nb[d];
-      B.ng[d] = (c->nx[d] > 1) ? c->ng : 0;
-      const double lo = (lx[d] == 0) ? c->xmin[d] : mesh_gen(lx[d], c->nb[d], c->xmin[d], c->xmax[d]);
-      const double hi =
-          (lx[d] == c->nb[d] - 1) ? c->xmax[d] : mesh_gen(lx[d] + 1, c->nb[d], c->xmin[d], c->xmax[d]);
-      B.xmin[d] = lo;
-      B.dx[d] = (hi - lo) / B.nx[d];
-    }
-    s->blocks.push_back(B);
-    s->hb.push_back(make_host_block(B));
-  }
+static pangu_config physics_of(const pangu_sim_cfg *c) {
   pangu_config pc;
   std::memset(&pc, 0, sizeof(pc));
   pc.metric = c->metric;
@@ -162,7 +148,63 @@ int pangu_sim_create(const pangu_sim_cfg *c, int device, pangu_sim **out) {
     pc.bc[d][0] = c->bc[d][0];
     pc.bc[d][1] = c->bc[d][1];
   }
-  int r = pangu_create(&pc, s->blocks.data(), s->nblk, device, &s->ctx);
+  return pc;
+}
+
+extern "C" {
+
+int pangu_sim_create_batch(const pangu_sim_cfg *cfgs, int nmesh, int device, pangu_sim **out) {
+  if (!cfgs || nmesh <= 0 || !out) return PANGU_EINVAL;
+  *out = nullptr;
+  for (int m = 0; m < nmesh; ++m) {
+    const pangu_sim_cfg *c = &cfgs[m];
+    for (int d = 0; d < 3; ++d)
+      if (c->nb[d] <= 0 || c->nx[d] % c->nb[d] != 0) return PANGU_EINVAL;
+    if (c->dt_factor != cfgs[0].dt_factor || c->stage_init != cfgs[0].stage_init ||
+        c->exchange_after != cfgs[0].exchange_after) {
+      fprintf(stderr, "pangu_sim_create_batch: meshes must share dt_factor, stage_init, exchange_after\n");
+      return PANGU_EINVAL;
+    }
+  }
+  const pangu_sim_cfg *c = &cfgs[0];
+  pangu_sim *s = new pangu_sim();
+  s->cfg = *c;
+  s->mcfg.assign(cfgs, cfgs + nmesh);
+  s->nmesh = nmesh;
+  s->device = device;
+  if (hipSetDevice(device) != hipSuccess) {
+    delete s;
+    return PANGU_EDEVICE;
+  }
+  for (int m = 0; m < nmesh; ++m) {
+    const pangu_sim_cfg *cm = &cfgs[m];
+    const int nb = cm->nb[0] * cm->nb[1] * cm->nb[2];
+    for (int b = 0; b < nb; ++b) {
+      pangu_block B;
+      const int lx[3] = {b % cm->nb[0], (b / cm->nb[0]) % cm->nb[1], b / (cm->nb[0] * cm->nb[1])};
+      for (int d = 0; d < 3; ++d) {
+        B.lx[d] = lx[d];
+        B.nx[d] = cm->nx[d] / cm->nb[d];
+        B.ng[d] = (cm->nx[d] > 1) ? cm->ng : 0;
+        const double lo =
+            (lx[d] == 0) ? cm->xmin[d] : mesh_gen(lx[d], cm->nb[d], cm->xmin[d], cm->xmax[d]);
+        const double hi = (lx[d] == cm->nb[d] - 1) ? cm->xmax[d]
+                                                   : mesh_gen(lx[d] + 1, cm->nb[d], cm->xmin[d], cm->xmax[d]);
+        B.xmin[d] = lo;
+        B.dx[d] = (hi - lo) / B.nx[d];
+      }
+      s->blocks.push_back(B);
+      s->hb.push_back(make_host_block(B));
+      s->bmesh.push_back(m);
+    }
+    s->mdt.push_back(cm->dt_init);
+    s->mtime.push_back(0.0);
+  }
+  s->nblk = (int)s->blocks.size();
+  std::vector<pangu_config> pcs(nmesh);
+  for (int m = 0; m < nmesh; ++m) pcs[m] = physics_of(&cfgs[m]);
+  int r = pangu_create_batch(pcs.data(), nmesh, s->blocks.data(), s->bmesh.data(), s->nblk, device,
+                             &s->ctx);
   if (r) {
     delete s;
     return r;
@@ -176,7 +218,11 @@ int pangu_sim_create(const pangu_sim_cfg *c, int device, pangu_sim **out) {
   if (c->enable_heating) s->iKEL = s->np++;
   s->nc = s->np;
   s->ncell = s->hb[0].cells;
-  s->total = s->ncell * s->nblk;
+  s->total = 0;
+  for (int b = 0; b < s->nblk; ++b) {
+    s->boff.push_back(s->total);
+    s->total += s->hb[b].cells;
+  }
   const int64_t T = s->total;
   if (dalloc(s, &s->P[0], T * s->np) || dalloc(s, &s->P[1], T * s->np) ||
       dalloc(s, &s->U[0], T * s->nc) || dalloc(s, &s->U[1], T * s->nc) ||
@@ -184,7 +230,7 @@ int pangu_sim_create(const pangu_sim_cfg *c, int device, pangu_sim **out) {
       dalloc(s, &s->F[1], T * s->nc) || dalloc(s, &s->F[2], T * s->nc) ||
       dalloc(s, &s->alfven, T * 3) || dalloc(s, &s->emf, T * 3) || dalloc(s, &s->flag, T) ||
       dalloc(s, &s->flag_rec, T) || dalloc(s, &s->mask, T) || dalloc(s, &s->status, T) ||
-      dalloc(s, &s->iters, T) || dalloc(s, &s->dstate, 2)) {
+      dalloc(s, &s->iters, T) || dalloc(s, &s->dstate, 2 * (int64_t)nmesh)) {
     fprintf(stderr, "pangu_sim_create: %s\n", s->err.c_str());
     pangu_sim_destroy(s);
     return PANGU_ENOMEM;
@@ -192,7 +238,8 @@ int pangu_sim_create(const pangu_sim_cfg *c, int device, pangu_sim **out) {
   auto table = [&](double *base, int ncomp) {
     std::vector<double *> t((size_t)s->nblk * ncomp);
     for (int b = 0; b < s->nblk; ++b)
-      for (int q = 0; q < ncomp; ++q) t[(size_t)b * ncomp + q] = base + (b * (int64_t)ncomp + q) * s->ncell;
+      for (int q = 0; q < ncomp; ++q)
+        t[(size_t)b * ncomp + q] = base + s->boff[b] * ncomp + (int64_t)q * s->hb[b].cells;
     return t;
   };
   auto register_cont = [&](double *Pa, double *Ua, bool with_flux, int *h) {
@@ -217,9 +264,9 @@ int pangu_sim_create(const pangu_sim_cfg *c, int device, pangu_sim **out) {
   std::vector<uint8_t *> tm(s->nblk);
   std::vector<int32_t *> tst(s->nblk), tit(s->nblk);
   for (int b = 0; b < s->nblk; ++b) {
-    tm[b] = s->mask + b * s->ncell;
-    tst[b] = s->status + b * s->ncell;
-    tit[b] = s->iters + b * s->ncell;
+    tm[b] = s->mask + s->boff[b];
+    tst[b] = s->status + s->boff[b];
+    tit[b] = s->iters + s->boff[b];
   }
   pangu_shared sh;
   sh.alfven = ta.data();
@@ -246,6 +293,10 @@ int pangu_sim_create(const pangu_sim_cfg *c, int device, pangu_sim **out) {
   return PANGU_OK;
 }
 
+int pangu_sim_create(const pangu_sim_cfg *c, int device, pangu_sim **out) {
+  return pangu_sim_create_batch(c, 1, device, out);
+}
+
 // Drops the captured cycle graph (after a change of launch parameters).
 static void drop_graph(pangu_sim *s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
@@ -270,27 +321,31 @@ int pangu_sim_destroy(pangu_sim *s) {
 }
 
 int pangu_sim_init(pangu_sim *s) {
-  ProblemParams pp;
-  pp.problem = s->cfg.problem;
-  pp.rin = s->cfg.rin;
-  pp.rmax = s->cfg.rmax;
-  pp.kappa = s->cfg.kappa;
-  pp.perturbation = s->cfg.perturbation;
-  pp.beta = s->cfg.beta;
-  pp.aphi_cut = s->cfg.aphi_cut;
-  pp.fel_0 = s->cfg.fel_0;
-  pp.seed = s->cfg.seed;
-  for (int d = 0; d < 3; ++d) pp.mesh_nx[d] = s->cfg.nx[d];
-  pangu_config pc;
-  std::memset(&pc, 0, sizeof(pc));
-  pc.metric = s->cfg.metric;
-  pc.a = s->cfg.a;
-  pc.h = s->cfg.h;
-  pc.gamma = s->cfg.gamma;
-  std::vector<double> prim;
-  if (generate_problem(pc, pp, s->hb, s->np, s->iBX, s->iENT, s->iKEL, prim))
-    return sfail(s, PANGU_EINVAL, "unknown problem");
-  SCHK(s, hipMemcpy(s->P[0], prim.data(), prim.size() * sizeof(double), hipMemcpyHostToDevice));
+  int first = 0;
+  for (int m = 0; m < s->nmesh; ++m) {
+    const pangu_sim_cfg &cm = s->mcfg[m];
+    int last = first;
+    while (last < s->nblk && s->bmesh[last] == m) ++last;
+    ProblemParams pp;
+    pp.problem = cm.problem;
+    pp.rin = cm.rin;
+    pp.rmax = cm.rmax;
+    pp.kappa = cm.kappa;
+    pp.perturbation = cm.perturbation;
+    pp.beta = cm.beta;
+    pp.aphi_cut = cm.aphi_cut;
+    pp.fel_0 = cm.fel_0;
+    pp.seed = cm.seed;
+    for (int d = 0; d < 3; ++d) pp.mesh_nx[d] = cm.nx[d];
+    const pangu_config pc = physics_of(&cm);
+    const std::vector<HostBlock> hb(s->hb.begin() + first, s->hb.begin() + last);
+    std::vector<double> prim;
+    if (generate_problem(pc, pp, hb, s->np, s->iBX, s->iENT, s->iKEL, prim))
+      return sfail(s, PANGU_EINVAL, "unknown problem");
+    SCHK(s, hipMemcpy(s->P[0] + s->boff[first] * s->np, prim.data(), prim.size() * sizeof(double),
+                      hipMemcpyHostToDevice));
+    first = last;
+  }
   PCHK(s, pangu_exchange_ghosts(s->ctx, s->h_base, s->stream));
   PCHK(s, pangu_apply_boundary_conditions(s->ctx, s->h_base, s->stream));
   SCHK(s, hipStreamSynchronize(s->stream));
@@ -327,7 +382,11 @@ static int stage_tasks(pangu_sim *s, int stage) {
   if (s->cfg.recovery != PANGU_RECOVERY_ROBUSTER) PCHK(s, pangu_fix_recovery(x, c1, st));
   if (s->cfg.exchange_after == 1) PCHK(s, pangu_exchange_ghosts(x, c1, st));
   PCHK(s, pangu_apply_boundary_conditions(x, c1, st));
-  if (stage == 2) PCHK(s, pangu_estimate_timestep(x, c1, &s->dt_est, st));
+  if (stage == 2) {
+    std::vector<double> est(s->nmesh);
+    PCHK(s, pangu_estimate_timestep(x, c1, est.data(), st));
+    s->dt_est = est[0];
+  }
   return 0;
 }
 
@@ -339,10 +398,25 @@ static int cycle_fused(pangu_sim *s) {
   return 0;
 }
 
+// Captures one fused cycle as a graph (once per set of launch parameters).
+static int ensure_graph(pangu_sim *s) {
+  if (s->gexec) return 0;
+  SCHK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  const int r = cycle_fused(s);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(s->stream, &g);
+  if (r) return r;
+  if (e != hipSuccess) return sfail(s, PANGU_EDEVICE, "hipStreamEndCapture failed");
+  s->graph = g;
+  SCHK(s, hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0));
+  return 0;
+}
+
 int pangu_sim_step(pangu_sim *s, int n, int mode) {
   hipSetDevice(s->device);
   if (int r = ensure_stage1(s)) return r;
   if (mode == PANGU_SIM_TASKS) {
+    if (s->nmesh != 1) return sfail(s, PANGU_EINVAL, "task mode runs one mesh (host dt)");
     if (s->dt_on_device) {
       sync_dt_from_device(s);
       s->dt_on_device = false;
@@ -362,16 +436,7 @@ int pangu_sim_step(pangu_sim *s, int n, int mode) {
     s->dt_on_device = true;
   }
   if (mode == PANGU_SIM_GRAPH) {
-    if (!s->gexec) {
-      SCHK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-      const int r = cycle_fused(s);
-      hipGraph_t g = nullptr;
-      const hipError_t e = hipStreamEndCapture(s->stream, &g);
-      if (r) return r;
-      if (e != hipSuccess) return sfail(s, PANGU_EDEVICE, "hipStreamEndCapture failed");
-      s->graph = g;
-      SCHK(s, hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0));
-    }
+    if (int r = ensure_graph(s)) return r;
     for (int c = 0; c < n; ++c) SCHK(s, hipGraphLaunch(s->gexec, s->stream));
   } else {
     for (int c = 0; c < n; ++c)
@@ -402,18 +467,16 @@ int pangu_sim_timed_cycles(pangu_sim *s, int n, int mode, double *ms) {
   return 0;
 }
 
-int pangu_sim_time_kernels(pangu_sim *s, int ncycles, double *ms_out, int *calls_out, int maxk) {
-  hipSetDevice(s->device);
-  if (ncycles < 1 || !ms_out || !calls_out) return sfail(s, PANGU_EINVAL, "bad arguments");
+// Runs ncycles fused cycles with the kernel clock: per kernel name the total
+// milliseconds and launches over all cycles, in order of first launch.
+static int clock_cycles(pangu_sim *s, int ncycles, std::vector<std::string> &names,
+                        std::vector<double> &tot, std::vector<int> &calls) {
   if (int r = ensure_stage1(s)) return r;
   if (!s->dt_on_device) {
     push_dt_to_device(s);
     s->dt_on_device = true;
   }
   KernelClock clk;
-  std::vector<std::string> names;
-  std::vector<double> tot;
-  std::vector<int> calls;
   for (int c = 0; c < ncycles; ++c) {
     clk.reset();
     SCHK(s, clk.mark(nullptr, s->stream));
@@ -438,6 +501,16 @@ int pangu_sim_time_kernels(pangu_sim *s, int ncycles, double *ms_out, int *calls
     }
   }
   s->ncycle += ncycles;
+  return 0;
+}
+
+int pangu_sim_time_kernels(pangu_sim *s, int ncycles, double *ms_out, int *calls_out, int maxk) {
+  (void)hipSetDevice(s->device);
+  if (ncycles < 1 || !ms_out || !calls_out) return sfail(s, PANGU_EINVAL, "bad arguments");
+  std::vector<std::string> names;
+  std::vector<double> tot;
+  std::vector<int> calls;
+  if (int r = clock_cycles(s, ncycles, names, tot, calls)) return r;
   s->knames = names;
   const int nk = (int)names.size();
   for (int j = 0; j < nk && j < maxk; ++j) {
@@ -447,8 +520,197 @@ int pangu_sim_time_kernels(pangu_sim *s, int ncycles, double *ms_out, int *calls
   return nk;
 }
 
+int pangu_sim_plain_loop(pangu_sim **sims, int n, int ncycles, int mode, double *ms) {
+  if (!sims || n <= 0 || ncycles < 0 || !ms || (mode != PANGU_SIM_FUSED && mode != PANGU_SIM_GRAPH))
+    return PANGU_EINVAL;
+  pangu_sim *s0 = sims[0];
+  (void)hipSetDevice(s0->device);
+  for (int i = 0; i < n; ++i) {
+    pangu_sim *s = sims[i];
+    if (s->device != s0->device) return sfail(s0, PANGU_EINVAL, "simulations on different devices");
+    if (int r = ensure_stage1(s)) return r;
+    if (!s->dt_on_device) {
+      push_dt_to_device(s);
+      s->dt_on_device = true;
+    }
+    if (mode == PANGU_SIM_GRAPH)
+      if (int r = ensure_graph(s)) return r;
+    SCHK(s, hipStreamSynchronize(s->stream));
+  }
+  hipStream_t st = s0->stream;
+  hipEvent_t a, b;
+  SCHK(s0, hipEventCreate(&a));
+  SCHK(s0, hipEventCreate(&b));
+  SCHK(s0, hipEventRecord(a, st));
+  for (int c = 0; c < ncycles; ++c)
+    for (int i = 0; i < n; ++i) {
+      pangu_sim *s = sims[i];
+      if (mode == PANGU_SIM_GRAPH) {
+        SCHK(s0, hipGraphLaunch(s->gexec, st));
+      } else {
+        hipStream_t own = s->stream;
+        s->stream = st;
+        const int r = cycle_fused(s);
+        s->stream = own;
+        if (r) return r;
+      }
+      s->ncycle++;
+    }
+  SCHK(s0, hipEventRecord(b, st));
+  SCHK(s0, hipEventSynchronize(b));
+  float f = 0.f;
+  SCHK(s0, hipEventElapsedTime(&f, a, b));
+  *ms = f;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return 0;
+}
+
+// ------------------------------------------------------------- autotuner
+struct Tuning {
+  int order[PANGU_KF_COUNT];
+  int split;
+};
+static std::map<std::string, Tuning> g_tune_cache;
+
+// Identifies a launch workload: block shapes and kernel-selecting options.
+static std::string tune_signature(const pangu_sim *s) {
+  std::string k;
+  char buf[96];
+  const pangu_sim_cfg &c = s->cfg;
+  snprintf(buf, sizeof(buf), "m%d r%d l%d c%d B%d H%d g%d|", c.metric, c.riemann, c.limiter,
+           c.recovery, c.enable_B, c.enable_heating, c.ng);
+  k += buf;
+  int run = 0;
+  for (int b = 0; b <= s->nblk; ++b) {
+    const bool same = b < s->nblk && b > 0 && s->hb[b].n[0] == s->hb[b - 1].n[0] &&
+                      s->hb[b].n[1] == s->hb[b - 1].n[1] && s->hb[b].n[2] == s->hb[b - 1].n[2];
+    if (b > 0 && !same) {
+      const HostBlock &H = s->hb[b - 1];
+      snprintf(buf, sizeof(buf), "%dx%dx%d*%d,", H.n[0], H.n[1], H.n[2], run);
+      k += buf;
+      run = 0;
+    }
+    ++run;
+  }
+  return k;
+}
+
+static double family_ms(int fam, const std::vector<std::string> &names, const std::vector<double> &tot) {
+  static const char *const fam_kernels[PANGU_KF_COUNT][3] = {
+      {"fix_cons", nullptr, nullptr},     {"flux_x1", nullptr, nullptr},
+      {"flux_x2", nullptr, nullptr},      {"flux_x3", nullptr, nullptr},
+      {"ct_emf", "ct_faces", nullptr},    {"div_update_source", "div_update", "source"},
+      {"recovery_heating", "recovery", "heating"}};
+  double ms = 0.0;
+  for (size_t j = 0; j < names.size(); ++j)
+    for (int q = 0; q < 3; ++q)
+      if (fam_kernels[fam][q] && names[j] == fam_kernels[fam][q]) ms += tot[j];
+  return ms;
+}
+
+static int apply_tuning(pangu_sim *s, const Tuning &T) {
+  for (int f = 0; f < PANGU_KF_COUNT; ++f) PCHK(s, pangu_set_tile_order(s->ctx, f, T.order[f]));
+  PCHK(s, pangu_set_split_source(s->ctx, T.split));
+  drop_graph(s);
+  return 0;
+}
+
+int pangu_sim_autotune(pangu_sim *s, int cycles, int use_cache, int *out_orders, int *out_split) {
+  if (!s) return PANGU_EINVAL;
+  (void)hipSetDevice(s->device);
+  if (cycles < 1) cycles = 1;
+  const std::string sig = tune_signature(s);
+  Tuning best;
+  auto hit = g_tune_cache.find(sig);
+  if (use_cache && hit != g_tune_cache.end()) {
+    best = hit->second;
+  } else {
+    if (int r = ensure_stage1(s)) return r;
+    if (!s->dt_on_device) {
+      push_dt_to_device(s);
+      s->dt_on_device = true;
+    }
+    // Snapshot of everything a fused cycle reads or leaves behind.
+    struct Save { void *p; size_t bytes; void *copy; };
+    const int64_t T = s->total;
+    std::vector<Save> save = {
+        {s->P[0], (size_t)(T * s->np) * 8, nullptr}, {s->P[1], (size_t)(T * s->np) * 8, nullptr},
+        {s->U[0], (size_t)(T * s->nc) * 8, nullptr}, {s->U[1], (size_t)(T * s->nc) * 8, nullptr},
+        {s->flag, (size_t)T * 8, nullptr},           {s->flag_rec, (size_t)T * 8, nullptr},
+        {s->alfven, (size_t)(T * 3) * 8, nullptr},   {s->emf, (size_t)(T * 3) * 8, nullptr},
+        {s->mask, (size_t)T, nullptr},               {s->status, (size_t)T * 4, nullptr},
+        {s->iters, (size_t)T * 4, nullptr},          {s->dstate, (size_t)s->nmesh * 16, nullptr}};
+    auto release = [&]() {
+      for (auto &x : save)
+        if (x.copy) (void)hipFree(x.copy);
+    };
+    for (auto &x : save) {
+      if (hipMalloc(&x.copy, x.bytes) != hipSuccess) {
+        x.copy = nullptr;
+        release();
+        return sfail(s, PANGU_ENOMEM, "autotune: no memory for the state snapshot");
+      }
+      if (hipMemcpyAsync(x.copy, x.p, x.bytes, hipMemcpyDeviceToDevice, s->stream) != hipSuccess) {
+        release();
+        return sfail(s, PANGU_EDEVICE, "autotune: snapshot copy failed");
+      }
+    }
+    const int ncycle0 = s->ncycle;
+    for (int f = 0; f < PANGU_KF_COUNT; ++f) best.order[f] = pangu_get_tile_order(s->ctx, f);
+    best.split = 0;
+    int rc = apply_tuning(s, best);
+    // coordinate descent: one family at a time, candidates timed with the
+    // kernel clock over `cycles` fused cycles
+    auto measure = [&](const Tuning &cand, int fam, double *ms) -> int {
+      if (int r = apply_tuning(s, cand)) return r;
+      std::vector<std::string> names;
+      std::vector<double> tot;
+      std::vector<int> calls;
+      if (int r = clock_cycles(s, cycles, names, tot, calls)) return r;
+      *ms = family_ms(fam, names, tot);
+      return 0;
+    };
+    for (int f = 0; f < PANGU_KF_COUNT && !rc; ++f) {
+      double base_ms = 0, alt_ms = 0;
+      Tuning alt = best;
+      alt.order[f] = 1 - best.order[f];
+      rc = measure(best, f, &base_ms);
+      if (!rc) rc = measure(alt, f, &alt_ms);
+      if (!rc && alt_ms > 0 && alt_ms < base_ms) best = alt;
+    }
+    if (!rc) {
+      double base_ms = 0, alt_ms = 0;
+      Tuning alt = best;
+      alt.split = 1;
+      rc = measure(best, PANGU_KF_DIV_UPDATE, &base_ms);
+      if (!rc) rc = measure(alt, PANGU_KF_DIV_UPDATE, &alt_ms);
+      if (!rc && alt_ms > 0 && alt_ms < base_ms) best = alt;
+    }
+    for (auto &x : save)
+      if (hipMemcpyAsync(x.p, x.copy, x.bytes, hipMemcpyDeviceToDevice, s->stream) != hipSuccess)
+        rc = rc ? rc : sfail(s, PANGU_EDEVICE, "autotune: restore copy failed");
+    (void)hipStreamSynchronize(s->stream);
+    release();
+    s->ncycle = ncycle0;
+    if (rc) return rc;
+    g_tune_cache[sig] = best;
+  }
+  if (int r = apply_tuning(s, best)) return r;
+  if (out_orders)
+    for (int f = 0; f < PANGU_KF_COUNT; ++f) out_orders[f] = best.order[f];
+  if (out_split) *out_split = best.split;
+  return 0;
+}
+
 int pangu_sim_set_tile_order(pangu_sim *s, int fam, int order) {
   PCHK(s, pangu_set_tile_order(s->ctx, fam, order));
+  drop_graph(s);
+  return 0;
+}
+
+int pangu_sim_set_c2p_guess(pangu_sim *s, pangu_mlp *m, double delta) {
+  PCHK(s, pangu_set_c2p_guess(s->ctx, m, delta));
   drop_graph(s);
   return 0;
 }
@@ -498,7 +760,7 @@ int pangu_sim_time_phases(pangu_sim *s, int reps, double *ms_out, int maxp) {
         case 5: PCHK(s, pangu_recovery(x, c1, st)); if (s->cfg.enable_heating) PCHK(s, pangu_apply_electron_heating(x, c1, st)); break;
         case 6: if (s->cfg.recovery != PANGU_RECOVERY_ROBUSTER) PCHK(s, pangu_fix_recovery(x, c1, st)); break;
         case 7: PCHK(s, pangu_apply_boundary_conditions(x, c1, st)); break;
-        case 8: { double e; PCHK(s, pangu_estimate_timestep(x, c1, &e, st)); } break;
+        case 8: { std::vector<double> e(s->nmesh); PCHK(s, pangu_estimate_timestep(x, c1, e.data(), st)); } break;
       }
     }
     SCHK(s, hipEventRecord(b, s->stream));
@@ -533,9 +795,9 @@ int pangu_sim_task(pangu_sim *s, int task, int c0i, int c1i, double bdt, double 
     case T_EXCHANGE: PCHK(s, pangu_exchange_ghosts(x, c1, st)); break;
     case T_BC: PCHK(s, pangu_apply_boundary_conditions(x, c1, st)); break;
     case T_TIMESTEP: {
-      double e = 0;
-      PCHK(s, pangu_estimate_timestep(x, c1, &e, st));
-      if (out) *out = e;
+      std::vector<double> e(s->nmesh);
+      PCHK(s, pangu_estimate_timestep(x, c1, e.data(), st));
+      if (out) *out = e[0];
     } break;
     default: return sfail(s, PANGU_EINVAL, "unknown task");
   }
@@ -599,6 +861,24 @@ double pangu_sim_dt(pangu_sim *s) {
   sync_dt_from_device(s);
   return s->dt;
 }
+
+int pangu_sim_mesh_state(pangu_sim *s, int m, double *dt, double *time, int *first_block,
+                         int *nblocks) {
+  if (!s || m < 0 || m >= s->nmesh) return PANGU_EINVAL;
+  (void)hipStreamSynchronize(s->stream);
+  sync_dt_from_device(s);
+  int first = 0;
+  while (s->bmesh[first] != m) ++first;
+  int last = first;
+  while (last < s->nblk && s->bmesh[last] == m) ++last;
+  if (dt) *dt = s->mdt[m];
+  if (time) *time = s->mtime[m];
+  if (first_block) *first_block = first;
+  if (nblocks) *nblocks = last - first;
+  return 0;
+}
+
+int pangu_sim_nblocks(pangu_sim *s) { return s ? s->nblk : 0; }
 
 int pangu_sim_block_geom(pangu_sim *s, int b, double *xd, int *n) {
   if (b < 0 || b >= s->nblk) return PANGU_EINVAL;

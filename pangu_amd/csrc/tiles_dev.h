@@ -1,0 +1,72 @@
+// Thread -> (block, k, j, i) decoding of tile launches (see Range3, Items).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+#include "kernels.h"
+
+namespace pangu {
+
+constexpr int TILE = 256;
+
+// Range of a launch on one block (see Range3).
+__device__ __forceinline__ void block_range(const DevBlock &B, const Range3 &R, int lo[3],
+                                            int cnt[3]) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    int l, h;
+    if (R.entire) {
+      l = 0;
+      h = B.n[d] - 1;
+    } else if (B.n[d] > 1) {
+      l = B.s[d] + R.rlo[d];
+      h = B.e[d] + R.rhi[d];
+    } else {
+      l = B.s[d];
+      h = B.e[d];
+    }
+    lo[d] = l;
+    cnt[d] = h - l + 1;
+  }
+}
+
+// Decodes (block, k, j, i) of this thread; false for padding threads.  The
+// workgroup index L is remapped to tile T = (L mod 8) * ceil(N/8) + L / 8,
+// so each XCD runs a contiguous run of tiles.
+__device__ __forceinline__ bool decode_tile(const KCtx &C, const Range3 &R, int &b, int &k, int &j,
+                                            int &i) {
+  const int N = R.ntiles;
+  const int n8 = (N + 7) >> 3;
+  const int L = (int)blockIdx.x;
+  const int T = (L & 7) * n8 + (L >> 3);
+  if (T >= N) return false;
+  b = R.tblock[T];
+  const int local = T - R.tfirst[b];
+  int lo[3], cnt[3];
+  block_range(C.blk[b], R, lo, cnt);
+  const int plane = cnt[0] * cnt[1];
+  const int tpp = (plane + TILE - 1) / TILE;
+  int tile, kk;
+  if (R.order) {
+    tile = local / cnt[2];
+    kk = local - tile * cnt[2];
+  } else {
+    kk = local / tpp;
+    tile = local - kk * tpp;
+  }
+  const int t = tile * TILE + (int)threadIdx.x;
+  if (t >= plane) return false;
+  k = lo[2] + kk;
+  const int jj = t / cnt[0];
+  j = lo[1] + jj;
+  i = lo[0] + (t - jj * cnt[0]);
+  return true;
+}
+
+// Block and item index of this thread in an item launch.
+__device__ __forceinline__ void decode_item(const Items &I, int &b, int &item) {
+  b = I.tblock[blockIdx.x];
+  item = ((int)blockIdx.x - I.tfirst[b]) * TILE + (int)threadIdx.x;
+}
+
+}  // namespace pangu

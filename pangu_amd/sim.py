@@ -59,6 +59,11 @@ def lib():
         L = C.CDLL(path)
         vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
         L.pangu_sim_create.argtypes = [C.POINTER(SimCfg), C.c_int, C.POINTER(vp)]
+        L.pangu_sim_create_batch.argtypes = [C.POINTER(SimCfg), C.c_int, C.c_int, C.POINTER(vp)]
+        L.pangu_sim_mesh_state.argtypes = [vp, C.c_int, dp, dp, ip, ip]
+        L.pangu_sim_nblocks.argtypes = [vp]
+        L.pangu_sim_plain_loop.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_int, dp]
+        L.pangu_sim_autotune.argtypes = [vp, C.c_int, C.c_int, ip, ip]
         L.pangu_sim_destroy.argtypes = [vp]
         L.pangu_sim_init.argtypes = [vp]
         L.pangu_sim_step.argtypes = [vp, C.c_int, C.c_int]
@@ -109,25 +114,53 @@ def make_cfg(d):
 
 
 class HipSim:
-    """One run of the HIP path on a GPU."""
+    """One run of the HIP path on a GPU.  Given a list of configurations it
+    runs them as one batched ensemble (pangu_sim_create_batch)."""
 
     def __init__(self, cfg_dict, device=0):
-        self.cfg = cfg_dict
-        self._cfg = make_cfg(cfg_dict)
+        cfgs = list(cfg_dict) if isinstance(cfg_dict, (list, tuple)) else [cfg_dict]
+        self.cfg = cfgs[0]
+        self.cfgs = cfgs
+        self.nmesh = len(cfgs)
+        arr = (SimCfg * self.nmesh)()
+        for m, d in enumerate(cfgs):
+            arr[m] = make_cfg(d)
+        self._cfg = arr
         p = C.c_void_p()
-        r = lib().pangu_sim_create(C.byref(self._cfg), device, C.byref(p))
+        r = lib().pangu_sim_create_batch(arr, self.nmesh, device, C.byref(p))
         if r != 0 or not p:
             raise RuntimeError(f"pangu_sim_create failed ({r})")
         self._p = p
-        self.nb = list(cfg_dict["nb"])
-        self.nblocks = self.nb[0] * self.nb[1] * self.nb[2]
-        xd = np.zeros(6)
-        n = np.zeros(9, dtype=np.int32)
-        lib().pangu_sim_block_geom(self._p, 0, xd.ctypes.data_as(C.POINTER(C.c_double)),
-                                   n.ctypes.data_as(C.POINTER(C.c_int)))
-        self.n = [int(n[0]), int(n[1]), int(n[2])]
-        self.s = [int(n[3]), int(n[4]), int(n[5])]
-        self.e = [int(n[6]), int(n[7]), int(n[8])]
+        self.nb = list(cfgs[0]["nb"])
+        self.nblocks = lib().pangu_sim_nblocks(self._p)
+        self.shapes, self.starts, self.ends = [], [], []
+        for b in range(self.nblocks):
+            _, n = self.block_geom(b)
+            self.shapes.append((int(n[0]), int(n[1]), int(n[2])))
+            self.starts.append([int(n[3]), int(n[4]), int(n[5])])
+            self.ends.append([int(n[6]), int(n[7]), int(n[8])])
+        self.uniform = len(set(self.shapes)) == 1
+        self.n = list(self.shapes[0])
+        self.s = self.starts[0]
+        self.e = self.ends[0]
+
+    def mesh_state(self, m):
+        """(dt, time, first block, block count) of mesh m."""
+        dt, tm = C.c_double(), C.c_double()
+        fb, nb = C.c_int(), C.c_int()
+        self._check(lib().pangu_sim_mesh_state(self._p, m, C.byref(dt), C.byref(tm), C.byref(fb),
+                                               C.byref(nb)), "mesh_state")
+        return dt.value, tm.value, fb.value, nb.value
+
+    def autotune(self, cycles=3, use_cache=True):
+        """Runs the autotuner (state restored afterwards); returns the chosen
+        tile order per kernel family and the source split."""
+        orders = (C.c_int * len(KERNEL_FAMILIES))()
+        split = C.c_int()
+        self._check(lib().pangu_sim_autotune(self._p, cycles, int(use_cache), orders, C.byref(split)),
+                    "autotune")
+        names = sorted(KERNEL_FAMILIES, key=KERNEL_FAMILIES.get)
+        return {"order": {n: int(orders[KERNEL_FAMILIES[n]]) for n in names}, "split_source": split.value}
 
     def close(self):
         if getattr(self, "_p", None):
@@ -217,12 +250,31 @@ class HipSim:
         return np.float64
 
     def get(self, field, container=0):
+        """Whole arena of a field: (nblocks, ncomp, nk, nj, ni) when all
+        blocks share a shape, else the flat arena (see blocks())."""
         fid = FIELDS[field]
         nbytes = lib().pangu_sim_field_size(self._p, fid)
         dt = np.dtype(self._dtype(field))
         a = np.empty(nbytes // dt.itemsize, dtype=dt)
         self._check(lib().pangu_sim_get(self._p, fid, container, a.ctypes.data_as(C.c_void_p)), "get")
-        return a.reshape(self.nblocks, -1, self.n[2], self.n[1], self.n[0])
+        if self.uniform:
+            return a.reshape(self.nblocks, -1, self.n[2], self.n[1], self.n[0])
+        return a
+
+    def blocks(self, flat):
+        """Per-block (ncomp, nk, nj, ni) views of a flat arena."""
+        cells = [s[0] * s[1] * s[2] for s in self.shapes]
+        ncomp = flat.size // sum(cells)
+        out, off = [], 0
+        for s, c in zip(self.shapes, cells):
+            out.append(flat[off:off + ncomp * c].reshape(ncomp, s[2], s[1], s[0]))
+            off += ncomp * c
+        return out
+
+    def block_interior(self, arr, b):
+        """Interior of block b's (ncomp, nk, nj, ni) array."""
+        s, e = self.starts[b], self.ends[b]
+        return arr[..., s[2]:e[2] + 1, s[1]:e[1] + 1, s[0]:e[0] + 1]
 
     def set(self, field, arr, container=0):
         fid = FIELDS[field]
@@ -233,3 +285,14 @@ class HipSim:
     def interior(self, arr):
         s, e = self.s, self.e
         return arr[..., s[2]:e[2] + 1, s[1]:e[1] + 1, s[0]:e[0] + 1]
+
+
+def plain_loop(sims, ncycles, mode="graph"):
+    """Milliseconds for ncycles cycles of every simulation run in turn on one
+    stream (the loop around the one-mesh API that batching replaces)."""
+    arr = (C.c_void_p * len(sims))(*[s._p.value for s in sims])
+    ms = C.c_double()
+    r = lib().pangu_sim_plain_loop(arr, len(sims), ncycles, MODES[mode], C.byref(ms))
+    if r != 0:
+        raise RuntimeError(f"plain_loop failed ({r})")
+    return ms.value

@@ -13,7 +13,7 @@ import oracle_py as O
 from pangu_amd import abi, configs, sim
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("pangu_hip.h", "pangu_sim.h")]
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("pangu_hip.h", "pangu_sim.h", "pangu_mlp.h")]
 
 
 def _declared(path):
@@ -28,7 +28,7 @@ def L():
 
 
 def test_every_declared_function_is_exported(L):
-    names = _declared(HEADERS[0]) + _declared(HEADERS[1])
+    names = sum((_declared(h) for h in HEADERS), [])
     assert len(names) >= 35
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing
