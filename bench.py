@@ -137,7 +137,7 @@ def batched_vs_plain(scale, cycles=10, warmup=2):
     }
 
 
-def learned_vs_plain(cfg, samples, cycles=20, delta=0.02):
+def learned_vs_plain(cfg, samples, cycles=20, delta=0.01):
     """Learned Kastaun bracket: trained on synthetic states, evaluated on a
     held-out set and in the 3D torus run (iterations and speed)."""
     from pangu_amd import mlp
@@ -145,7 +145,7 @@ def learned_vs_plain(cfg, samples, cycles=20, delta=0.02):
     m = mlp.Mlp(seed=1)
     t0 = time.perf_counter()
     m.make_dataset(samples, seed=11)
-    loss = m.train(epochs=8, batch=2048, lr=2e-3)
+    loss = m.train(epochs=20, batch=1024, lr=2e-3)
     t_train = time.perf_counter() - t0
     m.make_dataset(max(samples // 4, 1 << 16), seed=12)   # held-out states
     ev = m.evaluate(delta=delta)

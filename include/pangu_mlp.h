@@ -6,8 +6,9 @@
  * false position on the fixed bracket [0, 1], with no initial guess.  This
  * module trains a small network, 5 -> 64 -> 64 -> 64 -> 1 with SiLU, to
  * predict mu from five dimensionless invariants of the conserved state, and
- * lets the recovery kernel start from [g (1 - delta), g (1 + delta)] around
- * the prediction g, with a sign check and a fall-back to [0, 1].  Weights
+ * lets the recovery kernel start from a one-sided bracket [g, g (1 +- delta)]
+ * on the side of the prediction g where the master function changes sign,
+ * widened when it holds no root and replaced by [0, 1] as a last resort.  Weights
  * and activations enter the matrix products as BF16 on gfx950 MFMA units
  * with FP32 accumulation; FP32 master weights are trained with Adam.
  *
@@ -54,7 +55,8 @@ int pangu_mlp_make_dataset(pangu_mlp *m, int64_t n, uint64_t seed, double a, dou
                            double gamma, int64_t *kept);
 
 /* Trains on the current data set: `epochs` passes in minibatches of `batch`
- * (a multiple of 32) with Adam at learning rate lr; the feature
+ * (a multiple of 32) with Adam, the learning rate decaying from lr to
+ * lr / 100 along a cosine over the epochs of the call; the feature
  * standardisation is fixed from the data set on the first call.
  * loss_out (optional, epochs floats) receives each epoch's mean loss. */
 int pangu_mlp_train(pangu_mlp *m, int epochs, int batch, float lr, float *loss_out);
@@ -84,7 +86,7 @@ int pangu_mlp_evaluate(pangu_mlp *m, double delta, double *out);
 int pangu_mlp_gemm(int device, int M, int N, int K, const float *A, const float *B, float *C);
 
 /* Attaches the model to a context: its Kastaun recovery then starts from
- * the learned bracket of half-width delta (relative).  NULL detaches.  The
+ * the learned one-sided bracket of initial relative width delta.  NULL detaches.  The
  * model must outlive the attachment.  Runs with a model attached are off the
  * bit-for-bit parity path (results agree to the solver tolerance). */
 int pangu_set_c2p_guess(pangu_ctx *ctx, pangu_mlp *m, double delta);

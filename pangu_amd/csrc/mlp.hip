@@ -801,6 +801,8 @@ int pangu_mlp_train(pangu_mlp *m, int epochs, int batch, float lr, float *loss_o
   const int64_t nb = (int64_t)order.size() / batch;
   if (nb == 0) return mfail(m, PANGU_EINVAL, "data set smaller than one batch");
   for (int e = 0; e < epochs; ++e) {
+    // cosine decay from lr to lr / 100 over the call's epochs
+    const float lr_e = lr * (0.01f + 0.99f * 0.5f * (1.0f + std::cos(3.14159265f * e / epochs)));
     std::shuffle(order.begin(), order.end(), rng);
     double sse = 0.0;
     for (int64_t q = 0; q < nb; ++q) {
@@ -811,7 +813,7 @@ int pangu_mlp_train(pangu_mlp *m, int epochs, int batch, float lr, float *loss_o
       double b = 0.0;
       if (int r = backward(m, batch, &b)) return r;
       sse += b;
-      if (int r = adam_step(m, lr)) return r;
+      if (int r = adam_step(m, lr_e)) return r;
     }
     if (loss_out) loss_out[e] = (float)(sse / ((double)nb * batch));
   }

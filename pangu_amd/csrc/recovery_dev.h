@@ -410,16 +410,14 @@ __device__ __forceinline__ int kastaun_setup(const double U[9], const CenMetric 
   return 0;
 }
 
-// Illinois false position on [lo, hi] (scheme_kastaun.h:110-165).  Returns
-// 0 with the root in mu, 1 non-finite end values, 2 no sign change,
-// 3 non-finite iterate, 4 no convergence in MAX_NEWT_ITER.  iters counts
-// the false-position steps.
-__device__ __forceinline__ int kastaun_solve(const KastaunIn &k, double gamma, double lo, double hi,
-                                             double &mu, int &iters) {
+// Illinois false position on [lo, hi] with end values flo, fhi
+// (scheme_kastaun.h:110-165).  Returns 0 with the root in mu, 1 non-finite
+// end values, 2 no sign change, 3 non-finite iterate, 4 no convergence in
+// MAX_NEWT_ITER.  iters counts the false-position steps.
+__device__ __forceinline__ int kastaun_iterate(const KastaunIn &k, double gamma, double lo, double flo,
+                                               double hi, double fhi, double &mu, int &iters) {
   const double D = k.D, q = k.q, bsq = k.bsq, rsq = k.rsq, rbsq = k.rbsq;
   iters = 0;
-  double flo = kastaun_root(lo, D, q, bsq, rsq, rbsq, gamma);
-  double fhi = kastaun_root(hi, D, q, bsq, rsq, rbsq, gamma);
   mu = 0.0;
   if (!isfinite(flo) || !isfinite(fhi)) return 1;
   if (fabs(flo) <= NEWT_TOL) {
@@ -461,6 +459,13 @@ __device__ __forceinline__ int kastaun_solve(const KastaunIn &k, double gamma, d
   return 4;
 }
 
+__device__ __forceinline__ int kastaun_solve(const KastaunIn &k, double gamma, double lo, double hi,
+                                             double &mu, int &iters) {
+  const double flo = kastaun_root(lo, k.D, k.q, k.bsq, k.rsq, k.rbsq, gamma);
+  const double fhi = kastaun_root(hi, k.D, k.q, k.bsq, k.rsq, k.rbsq, gamma);
+  return kastaun_iterate(k, gamma, lo, flo, hi, fhi, mu, iters);
+}
+
 // Primitives from the root (scheme_kastaun.h:215-259); 0, 5 (v >= 1) or
 // 6 (rho or eps not positive).
 __device__ __forceinline__ int kastaun_finish(const KastaunIn &k, double mu, double p[9]) {
@@ -499,12 +504,14 @@ __device__ int c2p_kastaun(const double U[9], double p[9], double gamma, const C
   return kastaun_finish(k, mu, p);
 }
 
-// Kastaun with a learned starting bracket [g (1 - delta), min(1, g (1 + delta))]
-// around a guess g of the root.  When that bracket does not hold a sign
-// change it falls back to [0, 1], so the root found is a root of the same
-// function; only the false-position path differs (the result agrees with
-// c2p_kastaun to the 1e-10 step tolerance, not bit for bit).  evals counts
-// root-function evaluations, bracket ends included.
+// Kastaun with a learned starting bracket.  The master function rises
+// through its root, so the sign of f(g) at the guess g says on which side the
+// root lies: the bracket is [g, g (1 + delta)] (or [g (1 - delta), g]), widened
+// by 4x up to three times while it holds no sign change, and [0, 1] when that
+// fails too.  The root found is a root of the same function; only the
+// false-position path differs from c2p_kastaun (agreement to the 1e-10 step
+// tolerance, not bit for bit).  iters counts false-position steps, evals
+// every evaluation of the master function.
 __device__ int c2p_kastaun_guess(const double U[9], double p[9], double gamma, const CenMetric &m,
                                  double guess, double delta, int &iters, int &evals,
                                  double *mu_out = nullptr) {
@@ -514,13 +521,39 @@ __device__ int c2p_kastaun_guess(const double U[9], double p[9], double gamma, c
   if (int e = kastaun_setup(U, m, k)) return e;
   double mu = 0.0;
   int rs = 2;
-  if (guess > 0.0 && guess <= 1.0) {
-    const double lo = guess * (1.0 - delta);
-    const double hi = kmin(1.0, guess * (1.0 + delta));
-    rs = kastaun_solve(k, gamma, lo, hi, mu, iters);
-    evals = 2 + iters;
+  if (guess > 0.0 && guess < 1.0) {
+    double x0 = guess;
+    double f0 = kastaun_root(x0, k.D, k.q, k.bsq, k.rsq, k.rbsq, gamma);
+    evals = 1;
+    if (isfinite(f0) && fabs(f0) <= NEWT_TOL) {
+      mu = x0;
+      rs = 0;
+    } else if (isfinite(f0)) {
+      const double dir = (f0 < 0.0) ? 1.0 : -1.0;
+      double dl = delta;
+      for (int e = 0; e < 4; ++e) {
+        const double x1 = kmin(1.0, kmax(0.0, x0 * (1.0 + dir * dl)));
+        const double f1 = kastaun_root(x1, k.D, k.q, k.bsq, k.rsq, k.rbsq, gamma);
+        evals += 1;
+        if (!isfinite(f1)) break;
+        if (f0 * f1 <= 0.0) {
+          int it = 0;
+          if (x0 < x1)
+            rs = kastaun_iterate(k, gamma, x0, f0, x1, f1, mu, it);
+          else
+            rs = kastaun_iterate(k, gamma, x1, f1, x0, f0, mu, it);
+          iters = it;
+          evals += it;
+          break;
+        }
+        if (x1 == 0.0 || x1 == 1.0) break;
+        x0 = x1;
+        f0 = f1;
+        dl *= 4.0;
+      }
+    }
   }
-  if (rs == 1 || rs == 2) {
+  if (rs != 0) {
     int it2 = 0;
     rs = kastaun_solve(k, gamma, 0.0, 1.0, mu, it2);
     iters += it2;

@@ -52,21 +52,22 @@ def test_gradient_matches_fp64_reference():
 def trained():
     from pangu_amd import mlp
     m = mlp.Mlp(seed=1)
-    kept = m.make_dataset(1 << 18, seed=2)
-    assert kept > 0.9 * (1 << 18)
-    loss = m.train(epochs=6, batch=1024, lr=2e-3)
+    kept = m.make_dataset(1 << 20, seed=2)
+    assert kept > 0.9 * (1 << 20)
+    loss = m.train(epochs=20, batch=1024, lr=2e-3)
     print("loss per epoch", loss, flush=True)
+    m.make_dataset(1 << 17, seed=3)   # held-out states for the evaluation
     return m, loss
 
 
 @pytest.mark.gpu
 def test_training_cuts_kastaun_iterations(trained):
     m, loss = trained
-    assert loss[-1] < 0.5 * loss[0]
-    ev = m.evaluate(delta=0.02)
+    assert loss[-1] < 0.2 * loss[0]
+    ev = m.evaluate(delta=0.01)
     print("evaluation", ev, flush=True)
     assert ev["failures"] == 0
-    assert ev["learned_iters"] <= 0.6 * ev["plain_iters"]
+    assert ev["learned_iters"] <= 0.6 * ev["plain_iters"]      # >= 40 % fewer steps
     assert ev["learned_evals"] <= 0.75 * ev["plain_evals"]
     assert ev["max_residual"] <= 1e-9
     assert ev["max_rel_diff"] <= 1e-8
@@ -96,7 +97,7 @@ def test_learned_bracket_in_torus_run(trained):
         h = HipSim(cfg)
         h.init()
         if learned:
-            m.attach(h, delta=0.02)
+            m.attach(h, delta=0.01)
         h.step(2, "graph")
         it = h.interior(h.get("c2p_iters"))
         runs.append((h.interior(h.get("prim", 0)), h.interior(h.get("cons", 0)), it.mean()))

@@ -39,6 +39,12 @@ MATRIX = {
     "ot_cks3d_laxf_mc_chainer_heat": _ot_curved("cks", (16, 16, 8), (2, 1, 2), (-4.0, 2.0, -0.5),
                                                 (-3.0, 3.0, 0.0), riemann=0, limiter=0,
                                                 recovery=0, enable_heating=1, heat_model=2),
+    # boxes straddling the excision radius r = 1 (inside the horizon)
+    "ot_cks3d_excision_kastaun": _ot_curved("cks", (16, 16, 8), (2, 2, 2), (0.3, 0.2, 0.25),
+                                            (1.3, 1.2, 0.75), riemann=1, limiter=1, recovery=1),
+    "ot_cks3d_excision_chainer": _ot_curved("cks", (16, 16, 8), (2, 2, 2), (0.6, 0.5, 0.3),
+                                            (2.6, 2.5, 1.3), riemann=0, limiter=1, recovery=0,
+                                            enable_heating=1),
     "ot_bl2d_hll_mc_chainer": _ot_curved("bl", (32, 32, 1), (2, 2, 1), (1.0, -0.5, 0.0),
                                          (2.0, 0.5, 1.0), riemann=1, limiter=0, recovery=0),
     "ot_mks2d_ppm4_onedvsq": _ot_curved("mks", (32, 32, 1), (1, 2, 1), (1.0, -0.5, 0.0),
