@@ -396,6 +396,8 @@ void Sim::fluxes(int c) {
 
 // ConstraintedTransport (constrained_transport.cc:13-130).
 void Sim::ct() {
+  // The task exists only with a magnetic field (ideal_grmhd.cc:67-68).
+  if (!p.enable_B) return;
   const Block &B0 = blk[0];
   const int o1 = (B0.n[0] > 1), o2 = (B0.n[1] > 1), o3 = (B0.n[2] > 1);
   const int E1 = o2 && o3, E2 = o3 && o1, E3 = o1 && o2;

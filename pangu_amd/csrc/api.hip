@@ -214,6 +214,28 @@ static int build_geometry(pangu_ctx *c, const pangu_block *blocks) {
     G.ncen = (int64_t)G.ni * G.nj * G.nk;
     std::vector<double> cen((size_t)(G.ncen * CG_NCOMP));
     geom::fill_centre_table(cfg, H, G.ni, G.nj, G.nk, has3d, cen.data());
+    // Compact Christoffel storage when the table is symmetric in its lower
+    // indices bit for bit (checked for every entry); the source term then
+    // loads 40 values per cell instead of 64 and sums the same terms.
+    G.conn_sym = 1;
+    for (int64_t e = 0; e < G.ncen && G.conn_sym; ++e)
+      for (int a = 0; a < 4 && G.conn_sym; ++a)
+        for (int b = 0; b < 4; ++b)
+          for (int cc = b + 1; cc < 4; ++cc) {
+            const double x = cen[(size_t)((CG_CONN + 16 * a + 4 * b + cc) * G.ncen + e)];
+            const double y = cen[(size_t)((CG_CONN + 16 * a + 4 * cc + b) * G.ncen + e)];
+            if (std::memcmp(&x, &y, sizeof(double)) != 0) G.conn_sym = 0;
+          }
+    if (G.conn_sym) {
+      std::vector<double> compact((size_t)(40 * G.ncen));
+      for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b)
+          for (int cc = b; cc < 4; ++cc)
+            std::memcpy(&compact[(size_t)((10 * a + conn_pair(b, cc)) * G.ncen)],
+                        &cen[(size_t)((CG_CONN + 16 * a + 4 * b + cc) * G.ncen)],
+                        sizeof(double) * (size_t)G.ncen);
+      std::memcpy(&cen[(size_t)(CG_CONN * G.ncen)], compact.data(), compact.size() * sizeof(double));
+    }
     double *dcen = nullptr;
     if (upload(c, &dcen, cen.data(), cen.size())) return PANGU_EDEVICE;
     c->geo_bufs.push_back(dcen);

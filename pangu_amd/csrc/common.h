@@ -2,6 +2,11 @@
 #pragma once
 #include <stdint.h>
 
+#ifndef __HIPCC__
+#define __host__
+#define __device__
+#endif
+
 #include "../../include/pangu_hip.h"
 
 namespace pangu {
@@ -24,7 +29,8 @@ enum CentreComp {
   CG_RHOFL = 35,        // density_floor * max(r,1)^density_floor_pow
   CG_ENGFL = 36,        // energy_floor  * max(r,1)^energy_floor_pow
   CG_RAD = 37,          // r used by the floors / excision
-  CG_CONN = 38,         // 64: Gamma^a_{bc} at CG_CONN + 16a + 4b + c
+  CG_CONN = 38,         // 64: Gamma^a_{bc} at CG_CONN + 16a + 4b + c, or, when the
+                        // set's conn_sym is set, 40 at CG_CONN + 10a + pair(b, c)
   CG_NCOMP = 102
 };
 // Face table (lower faces normal to direction d):
@@ -62,6 +68,14 @@ struct DevGeo {
   int64_t ncen;         // ni*nj*nk
   int64_t nface[3];
   int fi[3], fj[3];     // face table row lengths (ni+1 / nj+1 on the normal dim)
+  int conn_sym;         // every Gamma^a_{bc} equals Gamma^a_{cb} bit for bit: compact storage
+  int pad;
 };
+
+// Index of the unordered pair (b, c) among (0,0) (0,1) (0,2) (0,3) (1,1) (1,2)
+// (1,3) (2,2) (2,3) (3,3).
+__host__ __device__ constexpr int conn_pair(int b, int c) {
+  return b <= c ? (b * (7 - b)) / 2 + c : (c * (7 - c)) / 2 + b;
+}
 
 }  // namespace pangu

@@ -391,13 +391,17 @@ __global__ __launch_bounds__(256, 5) void k_div_update(KCtx C, KFields F0, KFiel
       T[3][c] = tmunu<3>(s, etot, ptot, c);
     }
     const double *conn = G.cen + (int64_t)CG_CONN * G.ncen + t;
+    const bool sym = G.conn_sym != 0;
 #pragma unroll
     for (int dd = 0; dd < 4; ++dd) {
       double con = 0.0;
 #pragma unroll
       for (int r = 0; r < 4; ++r)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) con += T[r][c] * conn[(int64_t)(16 * c + 4 * dd + r) * G.ncen];
+        for (int c = 0; c < 4; ++c) {
+          const int slot = sym ? 10 * c + conn_pair(dd, r) : 16 * c + 4 * dd + r;
+          con += T[r][c] * conn[(int64_t)slot * G.ncen];
+        }
       const int q = (dd == 0) ? 1 : 1 + dd;
       unew[q] += con * m.sg * bdt;
     }

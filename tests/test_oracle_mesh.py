@@ -127,7 +127,7 @@ def _torus_balance(n):
     d.update(perturbation=0.0, enable_B=0)
     s = O.OracleSim(d)
     s.init()
-    for t in ("fix_primitive", "conservative", "fluxes", "ct", "flux_div"):
+    for t in ("fix_primitive", "conservative", "fluxes", "flux_div"):
         s.task(t, 0)
     dudt = s.interior(s.get("dudt"))
     # source alone: P(base) in container 1 with U = 0, source with dt = 1
@@ -205,3 +205,19 @@ def test_geometry_memo_is_bitwise_identical():
         s.step(1)
         runs.append((s.get("prim", 0), s.get("cons", 0)))
     assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+
+
+def test_ct_task_without_field_is_a_no_op():
+    """CT is scheduled only with a magnetic field; called without one it
+    must leave every array untouched."""
+    d = configs.torus2d(32, 16)
+    d.update(enable_B=0)
+    s = O.OracleSim(d)
+    s.init()
+    for t in ("fix_primitive", "conservative", "fluxes"):
+        s.task(t, 0)
+    before = [s.get(f).copy() for f in ("flux1", "flux2", "emf")]
+    s.task("ct", 0)
+    after = [s.get(f) for f in ("flux1", "flux2", "emf")]
+    for a, b in zip(before, after):
+        assert np.array_equal(a, b)
