@@ -538,19 +538,19 @@ int pangu_create(const pangu_config *cfg, const pangu_block *blocks, int nblocks
 
 int pangu_destroy(pangu_ctx *c) {
   if (!c) return PANGU_OK;
-  hipSetDevice(c->device);
+  (void)hipSetDevice(c->device);
   for (double *p : c->geo_bufs) (void)hipFree(p);
   for (int *p : c->tab_bufs) (void)hipFree(p);
   for (auto &ct : c->conts) {
-    hipFree(ct.dprim);
-    hipFree(ct.dcons);
-    for (int d = 0; d < 3; ++d) hipFree(ct.dflux[d]);
+    (void)hipFree(ct.dprim);
+    (void)hipFree(ct.dcons);
+    for (int d = 0; d < 3; ++d) (void)hipFree(ct.dflux[d]);
   }
   void *ptrs[] = {c->dblk,    c->dgeo,  c->dnbr,     c->dalfven, c->demf,  c->dflag,
                   c->dflag_rec, c->dmask, c->dstatus, c->diters,  c->dred,  c->dcluster,
                   c->dest,    c->dguess, c->dguess_tab};
   for (void *p : ptrs)
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   delete c;
   return PANGU_OK;
 }
@@ -610,7 +610,7 @@ int pangu_geometry_entry(const pangu_config *cfg, int loc, const double x[4], do
 
 int pangu_register_container(pangu_ctx *c, const pangu_container *src, int *handle) {
   if (!c || !src || !handle || !src->prim || !src->cons) return fail(c, PANGU_EINVAL, "null container");
-  hipSetDevice(c->device);
+  HIPCHK(c, hipSetDevice(c->device));
   DevContainer d;
   d.hprim.assign(src->prim, src->prim + (size_t)c->nblocks * c->np);
   d.hcons.assign(src->cons, src->cons + (size_t)c->nblocks * c->nc);
@@ -631,7 +631,7 @@ int pangu_register_container(pangu_ctx *c, const pangu_container *src, int *hand
 
 int pangu_register_shared(pangu_ctx *c, const pangu_shared *s) {
   if (!c || !s || !s->alfven) return fail(c, PANGU_EINVAL, "shared fields need alfven");
-  hipSetDevice(c->device);
+  HIPCHK(c, hipSetDevice(c->device));
   const size_t nb = (size_t)c->nblocks;
   if (upload(c, &c->dalfven, s->alfven, nb * 3)) return PANGU_EDEVICE;
   if (s->emf && upload(c, &c->demf, s->emf, nb * 3)) return PANGU_EDEVICE;

@@ -863,7 +863,7 @@ static const dim3 kTile(TILE, 1, 1);
 
 #define PG_CHECK_LAUNCH() (hipGetLastError() == hipSuccess ? 0 : -2)
 #define TICK(name) \
-  if (L.clock) L.clock->mark(name, s)
+  if (L.clock) (void)L.clock->mark(name, s)
 
 hipError_t KernelClock::mark(const char *name, hipStream_t s) {
   if (n == events.size()) {

@@ -413,7 +413,7 @@ static int ensure_graph(pangu_sim *s) {
 }
 
 int pangu_sim_step(pangu_sim *s, int n, int mode) {
-  hipSetDevice(s->device);
+  SCHK(s, hipSetDevice(s->device));
   if (int r = ensure_stage1(s)) return r;
   if (mode == PANGU_SIM_TASKS) {
     if (s->nmesh != 1) return sfail(s, PANGU_EINVAL, "task mode runs one mesh (host dt)");
@@ -462,8 +462,8 @@ int pangu_sim_timed_cycles(pangu_sim *s, int n, int mode, double *ms) {
   float f = 0.f;
   SCHK(s, hipEventElapsedTime(&f, a, b));
   *ms = f;
-  hipEventDestroy(a);
-  hipEventDestroy(b);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
   return 0;
 }
 
@@ -733,7 +733,7 @@ const char *pangu_sim_phase_name(int n) {
 }
 
 int pangu_sim_time_phases(pangu_sim *s, int reps, double *ms_out, int maxp) {
-  hipSetDevice(s->device);
+  SCHK(s, hipSetDevice(s->device));
   if (int r = ensure_stage1(s)) return r;
   if (!s->dt_on_device) {
     push_dt_to_device(s);
@@ -768,14 +768,14 @@ int pangu_sim_time_phases(pangu_sim *s, int reps, double *ms_out, int maxp) {
     float f = 0.f;
     SCHK(s, hipEventElapsedTime(&f, a, b));
     ms_out[ph] = (double)f / reps;
-    hipEventDestroy(a);
-    hipEventDestroy(b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
   }
   return nph;
 }
 
 int pangu_sim_task(pangu_sim *s, int task, int c0i, int c1i, double bdt, double *out) {
-  hipSetDevice(s->device);
+  SCHK(s, hipSetDevice(s->device));
   if (int r = ensure_stage1(s)) return r;
   const int c0 = c0i == 0 ? s->h_base : s->h_s1;
   const int c1 = c1i == 0 ? s->h_base : s->h_s1;
@@ -852,12 +852,12 @@ int pangu_sim_set(pangu_sim *s, int f, int c, const void *in) {
 }
 
 double pangu_sim_time(pangu_sim *s) {
-  hipStreamSynchronize(s->stream);
+  (void)hipStreamSynchronize(s->stream);
   sync_dt_from_device(s);
   return s->time;
 }
 double pangu_sim_dt(pangu_sim *s) {
-  hipStreamSynchronize(s->stream);
+  (void)hipStreamSynchronize(s->stream);
   sync_dt_from_device(s);
   return s->dt;
 }
@@ -920,10 +920,10 @@ int pangu_sim_copy_calibration(pangu_sim *s, int64_t n, int reps, double *ms) {
   float f = 0.f;
   SCHK(s, hipEventElapsedTime(&f, e0, e1));
   *ms = (double)f / reps;
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  hipFree(a);
-  hipFree(b);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
   return 0;
 }
 
