@@ -201,6 +201,10 @@ int pangu_get_tile_order(const pangu_ctx *ctx, int kernel_family);
  * own after the flux divergence + update kernel (fewer registers per
  * kernel, one more pass over U); 0 fuses all three.  Results are identical. */
 int pangu_set_split_source(pangu_ctx *ctx, int split);
+/* share = 1: the X1 sweep reconstructs each cell once and hands the right
+ * interface value to the next face through a wavefront shuffle, instead of
+ * reconstructing both neighbours per face.  Results are identical. */
+int pangu_set_share_reconstruction(pangu_ctx *ctx, int share);
 
 #ifdef __cplusplus
 }

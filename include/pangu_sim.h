@@ -105,6 +105,8 @@ int pangu_sim_set_c2p_guess(pangu_sim *s, pangu_mlp *m, double delta);
 int pangu_sim_set_tile_order(pangu_sim *s, int kernel_family, int order);
 /* pangu_set_split_source on the simulation's context (drops the graph). */
 int pangu_sim_set_split_source(pangu_sim *s, int split);
+/* pangu_set_share_reconstruction on the simulation's context (drops the graph). */
+int pangu_sim_set_share_reconstruction(pangu_sim *s, int share);
 /* One task on container c0 (c1 output of update/source), host beta*dt. */
 int pangu_sim_task(pangu_sim *s, int task, int c0, int c1, double beta_dt, double *out);
 int64_t pangu_sim_field_size(pangu_sim *s, int field);

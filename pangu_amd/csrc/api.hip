@@ -364,6 +364,28 @@ static int make_range(pangu_ctx *c, Range3 &R, int l0, int l1, int l2, int h0, i
   return upload_tiles(c, tiles, &R.tblock, &R.tfirst, &R.ntiles);
 }
 
+// Cell range of the shared-reconstruction X1 sweep: per plane, waves of 64
+// lanes over the row-major cells (i = s-1 .. e+1) overlapping by one cell,
+// four waves per tile.
+static int make_range_x1s(pangu_ctx *c, Range3 &R) {
+  R.rlo[0] = R.rlo[1] = R.rlo[2] = -1;
+  R.rhi[0] = R.rhi[1] = R.rhi[2] = 1;
+  R.entire = 0;
+  R.order = 0;
+  std::vector<int64_t> tiles(c->nblocks, 0);
+  for (int b = 0; b < c->nblocks; ++b) {
+    const HostBlock &H = c->hb[b];
+    if (H.n[0] == 1) continue;
+    int cnt[3];
+    for (int d = 0; d < 3; ++d)
+      cnt[d] = H.n[d] > 1 ? (H.e[d] + 1) - (H.s[d] - 1) + 1 : 1;
+    const int64_t plane = (int64_t)cnt[0] * cnt[1];
+    const int64_t wpp = (plane - 1 + 62) / 63;
+    tiles[b] = (int64_t)cnt[2] * ((wpp + 3) / 4);
+  }
+  return upload_tiles(c, tiles, &R.tblock, &R.tfirst, &R.ntiles);
+}
+
 static int make_items(pangu_ctx *c, Items &I, const std::vector<int64_t> &items) {
   std::vector<int64_t> tiles(items.size());
   for (size_t b = 0; b < items.size(); ++b) tiles[b] = (items[b] + 255) / 256;
@@ -388,6 +410,7 @@ static int build_tables(pangu_ctx *c) {
       r = make_range(c, L.face[2], -1, -1, 0, 1, 1, 1, 0, 2);
   }
   if (!r) r = make_range(c, L.ct, 0, 0, 0, 1, 1, 1, 0, -1);
+  if (!r) r = make_range_x1s(c, L.x1cells);
   std::vector<int64_t> ghosts(c->nblocks), planes(c->nblocks);
   for (int b = 0; b < c->nblocks; ++b) {
     const HostBlock &H = c->hb[b];
@@ -589,6 +612,12 @@ int pangu_set_c2p_guess(pangu_ctx *c, pangu_mlp *m, double delta) {
   }
   c->mlp = m;
   c->guess_delta = m ? delta : 0.0;
+  return PANGU_OK;
+}
+
+int pangu_set_share_reconstruction(pangu_ctx *c, int share) {
+  if (!c) return PANGU_EINVAL;
+  c->L.share_x1 = share != 0;
   return PANGU_OK;
 }
 

@@ -93,9 +93,11 @@ struct LaunchCtx {
   KCtx k;
   int limiter, riemann, metric, recovery;
   Range3 interior, entire, face[3], ct;
+  Range3 x1cells;                  // cell range of the shared-reconstruction X1 sweep
   Items ghosts, bcs[3];
   int order[PANGU_KF_COUNT] = {};  // tile order per kernel family (PANGU_ORDER_*)
   int split_source = 0;            // fused stage: source term in a kernel of its own
+  int share_x1 = 0;                // X1 sweep reconstructs each cell once (k_flux_x1s)
   KernelClock *clock = nullptr;
 };
 

@@ -159,38 +159,17 @@ __global__ __launch_bounds__(256) void k_fix_cons(KCtx C, KFields F, uint8_t *co
 // X2/X3 twins): reconstruction of every primitive, face metric, fast speeds,
 // four contravariant fluxes and the Riemann combine.  Face (k,j,i) is the
 // lower face of cell (k,j,i) along DIR.
-template <int DIR, int LIM, int RS, bool HB, bool HK>
-__global__ __launch_bounds__(256) void k_flux(KCtx C, KFields F, double *const *alfven, Range3 R) {
-  int b, k, j, i;
-  if (!decode_tile(C, R, b, k, j, i)) return;
+// Face physics of one sweep direction for face (k, j, i) of block b, the
+// lower face of cell (k, j, i) along DIR: face metric, fluid states, fast
+// speeds, the four contravariant fluxes and the Riemann combine, from the
+// reconstructed left (pl) and right (pr) primitives.
+template <int DIR, int RS, bool HB, bool HK>
+__device__ __forceinline__ void face_flux(const KCtx &C, const KFields &F, double *const *alfven, int b,
+                                          int k, int j, int i, const double pl[NPRIM],
+                                          const double pr[NPRIM]) {
   const DevBlock &B = C.blk[b];
-  const int64_t sj = B.n[0], sk = (int64_t)B.n[0] * B.n[1];
-  const int64_t st = (DIR == 1) ? 1 : (DIR == 2 ? sj : sk);
-  const int64_t id = k * sk + j * sj + i;
+  const int64_t id = ((int64_t)k * B.n[1] + j) * B.n[0] + i;
   constexpr int NC = ncomp<HB, HK>();
-  double pl[NPRIM], pr[NPRIM];
-#pragma unroll
-  for (int s = 0; s < NPRIM; ++s) {
-    pl[s] = 0.0;
-    pr[s] = 0.0;
-  }
-#pragma unroll
-  for (int q = 0; q < NC; ++q) {
-    const double *a = F.prim[b * NC + q];
-    const int s = slot_of<HB, HK>(q);
-    if (LIM == PANGU_LIMITER_PPM4) {
-      const double m3 = a[id - 3 * st], m2 = a[id - 2 * st], m1 = a[id - st], c0 = a[id],
-                   p1 = a[id + st], p2 = a[id + 2 * st];
-      double t0, t1;
-      recon_ppm4(m3, m2, m1, c0, p1, pl[s], t0);
-      recon_ppm4(m2, m1, c0, p1, p2, t1, pr[s]);
-    } else {
-      const double m2 = a[id - 2 * st], m1 = a[id - st], c0 = a[id], p1 = a[id + st];
-      double t0, t1;
-      recon_mc(m2, m1, c0, pl[s], t0);
-      recon_mc(m1, c0, p1, t1, pr[s]);
-    }
-  }
   const DevGeo &G = C.geo[B.geo];
   double g[16], gc0[4], gdd, sg, alpha, valid;
   if (DIR == 3 && !G.face[2]) {
@@ -253,6 +232,97 @@ __global__ __launch_bounds__(256) void k_flux(KCtx C, KFields F, double *const *
       f = (cmax * FL[s] + cmin * FR[s] - cmax * cmin * (UR[s] - UL[s])) / (cmax + cmin + 1e-20);
     F.flux[DIR - 1][b * NC + q][id] = f;
   }
+}
+
+
+template <int DIR, int LIM, int RS, bool HB, bool HK>
+__global__ __launch_bounds__(256) void k_flux(KCtx C, KFields F, double *const *alfven, Range3 R) {
+  int b, k, j, i;
+  if (!decode_tile(C, R, b, k, j, i)) return;
+  const DevBlock &B = C.blk[b];
+  const int64_t sj = B.n[0], sk = (int64_t)B.n[0] * B.n[1];
+  const int64_t st = (DIR == 1) ? 1 : (DIR == 2 ? sj : sk);
+  const int64_t id = k * sk + j * sj + i;
+  constexpr int NC = ncomp<HB, HK>();
+  double pl[NPRIM], pr[NPRIM];
+#pragma unroll
+  for (int s = 0; s < NPRIM; ++s) {
+    pl[s] = 0.0;
+    pr[s] = 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
+    const double *a = F.prim[b * NC + q];
+    const int s = slot_of<HB, HK>(q);
+    if (LIM == PANGU_LIMITER_PPM4) {
+      const double m3 = a[id - 3 * st], m2 = a[id - 2 * st], m1 = a[id - st], c0 = a[id],
+                   p1 = a[id + st], p2 = a[id + 2 * st];
+      double t0, t1;
+      recon_ppm4(m3, m2, m1, c0, p1, pl[s], t0);
+      recon_ppm4(m2, m1, c0, p1, p2, t1, pr[s]);
+    } else {
+      const double m2 = a[id - 2 * st], m1 = a[id - st], c0 = a[id], p1 = a[id + st];
+      double t0, t1;
+      recon_mc(m2, m1, c0, pl[s], t0);
+      recon_mc(m1, c0, p1, t1, pr[s]);
+    }
+  }
+  face_flux<DIR, RS, HB, HK>(C, F, alfven, b, k, j, i, pl, pr);
+}
+
+// X1 sweep with one reconstruction per cell.  Lane l of a wavefront
+// reconstructs cell c (both interface values, the same PPM4 / MC call the
+// face kernel makes) and takes the right interface value of cell c - 1 from
+// lane l - 1, so each cell's reconstruction runs once instead of twice.
+// Waves cover 64 consecutive cells of a plane's row-major cell range
+// (i = s - 1 .. e + 1) and overlap by one cell, so both neighbours of every
+// face sit in one wave.  Same arithmetic as k_flux<1>: identical results.
+template <int LIM, int RS, bool HB, bool HK>
+__global__ __launch_bounds__(256) void k_flux_x1s(KCtx C, KFields F, double *const *alfven, Range3 R) {
+  const int N = R.ntiles;
+  const int n8 = (N + 7) >> 3;
+  const int L = (int)blockIdx.x;
+  const int T = (L & 7) * n8 + (L >> 3);
+  if (T >= N) return;  // uniform over the workgroup
+  const int b = R.tblock[T];
+  const int local = T - R.tfirst[b];
+  const DevBlock &B = C.blk[b];
+  int lo[3], cnt[3];
+  block_range(B, R, lo, cnt);
+  const int plane = cnt[0] * cnt[1];
+  const int wpp = (plane - 1 + 62) / 63;
+  const int tpp = (wpp + 3) / 4;
+  const int kk = local / tpp;
+  const int w = (local - kk * tpp) * 4 + ((int)threadIdx.x >> 6);
+  const int lane = (int)threadIdx.x & 63;
+  const int cidx = w * 63 + lane;
+  const bool have = (w < wpp) && (cidx < plane);
+  const int jj = have ? cidx / cnt[0] : 0;
+  const int ic = have ? cidx - jj * cnt[0] : 0;
+  const int k = lo[2] + kk, j = lo[1] + jj, i = lo[0] + ic;
+  const int64_t id = ((int64_t)k * B.n[1] + j) * B.n[0] + i;
+  constexpr int NC = ncomp<HB, HK>();
+  double rv[NPRIM], pl[NPRIM], pr[NPRIM];
+#pragma unroll
+  for (int s = 0; s < NPRIM; ++s) {
+    rv[s] = 0.0;
+    pl[s] = 0.0;
+    pr[s] = 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
+    const int s = slot_of<HB, HK>(q);
+    if (have) {
+      const double *a = F.prim[b * NC + q];
+      if (LIM == PANGU_LIMITER_PPM4)
+        recon_ppm4(a[id - 2], a[id - 1], a[id], a[id + 1], a[id + 2], rv[s], pr[s]);
+      else
+        recon_mc(a[id - 1], a[id], a[id + 1], rv[s], pr[s]);
+    }
+    pl[s] = __shfl_up(rv[s], 1, 64);
+  }
+  if (!have || lane == 0 || ic == 0) return;
+  face_flux<1, RS, HB, HK>(C, F, alfven, b, k, j, i, pl, pr);
 }
 
 // ------------------------------------------------------ constrained transport
@@ -911,14 +981,18 @@ int launch_fix_cons(const LaunchCtx &L, const KFields &F, uint8_t *const *mask, 
 
 template <int DIR>
 static void flux_dir(const LaunchCtx &L, const KFields &F, double *const *alfven, hipStream_t s) {
-  Range3 R = L.face[DIR - 1];
+  const bool shared = DIR == 1 && L.share_x1;
+  Range3 R = shared ? L.x1cells : L.face[DIR - 1];
   R.order = L.order[PANGU_KF_FLUX_X1 + DIR - 1];
   if (R.ntiles == 0) return;
   const dim3 g = tile_grid(R);
   const bool hb = L.k.iBX >= 0, hk = L.k.iKEL >= 0;
   const int lim = L.limiter, rs = L.riemann;
-#define FL(LIM, RS, HB_, HK_) \
-  hipLaunchKernelGGL((k_flux<DIR, LIM, RS, HB_, HK_>), g, kTile, 0, s, L.k, F, alfven, R)
+#define FL(LIM, RS, HB_, HK_)                                                                      \
+  if (shared)                                                                                     \
+    hipLaunchKernelGGL((k_flux_x1s<LIM, RS, HB_, HK_>), g, kTile, 0, s, L.k, F, alfven, R);       \
+  else                                                                                            \
+    hipLaunchKernelGGL((k_flux<DIR, LIM, RS, HB_, HK_>), g, kTile, 0, s, L.k, F, alfven, R)
 #define FL2(LIM, RS)                    \
   if (hb && hk) FL(LIM, RS, true, true); \
   else if (hb) FL(LIM, RS, true, false); \

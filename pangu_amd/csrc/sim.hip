@@ -715,6 +715,12 @@ int pangu_sim_set_c2p_guess(pangu_sim *s, pangu_mlp *m, double delta) {
   return 0;
 }
 
+int pangu_sim_set_share_reconstruction(pangu_sim *s, int share) {
+  PCHK(s, pangu_set_share_reconstruction(s->ctx, share));
+  drop_graph(s);
+  return 0;
+}
+
 int pangu_sim_set_split_source(pangu_sim *s, int split) {
   PCHK(s, pangu_set_split_source(s->ctx, split));
   drop_graph(s);

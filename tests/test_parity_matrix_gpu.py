@@ -154,11 +154,13 @@ def test_launch_tuning_leaves_results_unchanged():
     from pangu_amd.sim import HipSim, KERNEL_FAMILIES
     cfg = _torus3d()
     runs = []
-    for variant in ("default", "split", "column", "plane"):
+    for variant in ("default", "split", "column", "plane", "share"):
         h = HipSim(cfg)
         h.init()
         if variant == "split":
             h.set_split_source(1)
+        if variant == "share":
+            h.set_share_reconstruction(1)
         for fam in KERNEL_FAMILIES:
             if variant == "column":
                 h.set_tile_order(fam, 1)
@@ -171,3 +173,23 @@ def test_launch_tuning_leaves_results_unchanged():
         assert np.array_equal(P, runs[0][1]), v
         assert np.array_equal(U, runs[0][2]), v
         assert np.array_equal(P1, runs[0][3]), v
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["ot_minkowski_noB", "torus2d_mc_hll", "ot_cks3d_laxf_mc_chainer_heat"])
+def test_shared_reconstruction_is_bit_identical(name):
+    """The X1 sweep with one reconstruction per cell (wavefront shuffle of
+    the right interface value) gives the same bits as the per-face sweep,
+    for both limiters and both solvers."""
+    from pangu_amd.sim import HipSim
+    cfg = MATRIX[name]
+    out = []
+    for share in (0, 1):
+        h = HipSim(cfg)
+        h.init()
+        h.set_share_reconstruction(share)
+        h.step(2, "graph")
+        out.append((h.get("prim", 0), h.get("cons", 0), h.get("flux1")))
+        h.close()
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
