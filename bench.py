@@ -187,6 +187,8 @@ def main():
     sim = HipSim(cfg)
     sim.init()
     t_init = time.perf_counter() - t_init
+    # launch parameters from the autotuner (fields, dt and time are restored)
+    tuning = sim.autotune(cycles=2) if args.mode != "tasks" else None
     sim.step(args.warmup, args.mode)
     ms = sim.timed_cycles(args.steps, args.mode)
     wall0 = time.perf_counter()
@@ -237,6 +239,7 @@ def main():
             "launch_ms": launch_ms,
             "algorithmic_bytes": bytes_launch,
         },
+        "autotune": tuning,
         "kernel_ms_per_cycle": {k: v[1] for k, v in kern.items()},
         "kernel_launches_per_cycle": {k: v[0] for k, v in kern.items()},
         "init_s": t_init,

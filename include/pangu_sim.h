@@ -91,13 +91,16 @@ const char *pangu_sim_kernel_name(pangu_sim *s, int n);
  * enqueued on the first simulation's stream.  ms = elapsed time between
  * HIP events around the loop. */
 int pangu_sim_plain_loop(pangu_sim **sims, int n, int ncycles, int mode, double *ms);
-/* Autotuner: for each kernel family the faster tile order, then whether to
- * split the source kernel, each candidate timed with the kernel clock over
+/* Autotuner: whether the X1 sweep shares reconstructions, then for each
+ * kernel family the faster tile order, then whether to split the source
+ * kernel, each candidate timed with the kernel clock over
  * `cycles` fused cycles.  Every field, dt and time is restored afterwards,
  * so tuning does not advance the run.  With use_cache != 0 a result stored
  * for the same block shapes and options in this process is reused without
- * measuring.  out_orders (PANGU_KF_COUNT ints) and out_split may be NULL. */
-int pangu_sim_autotune(pangu_sim *s, int cycles, int use_cache, int *out_orders, int *out_split);
+ * measuring.  out_orders (PANGU_KF_COUNT ints), out_split and out_share may
+ * be NULL. */
+int pangu_sim_autotune(pangu_sim *s, int cycles, int use_cache, int *out_orders, int *out_split,
+                       int *out_share);
 /* pangu_set_c2p_guess on the simulation's context (drops the graph). */
 int pangu_sim_set_c2p_guess(pangu_sim *s, pangu_mlp *m, double delta);
 /* Sets a kernel family's tile order (pangu_set_tile_order) and drops the

@@ -361,6 +361,33 @@ static int make_range(pangu_ctx *c, Range3 &R, int l0, int l1, int l2, int h0, i
   std::vector<int64_t> tiles(c->nblocks);
   for (int b = 0; b < c->nblocks; ++b)
     tiles[b] = (need_dim >= 0 && c->hb[b].n[need_dim] == 1) ? 0 : range_tiles(c->hb[b], R.rlo, R.rhi, entire);
+  // one block shape: arithmetic decoding
+  R.uni = 0;
+  bool same = true;
+  for (int b = 1; b < c->nblocks && same; ++b)
+    for (int d = 0; d < 3; ++d)
+      if (c->hb[b].n[d] != c->hb[0].n[d] || c->hb[b].s[d] != c->hb[0].s[d] || c->hb[b].e[d] != c->hb[0].e[d])
+        same = false;
+  if (same && tiles[0] > 0) {
+    const HostBlock &H = c->hb[0];
+    for (int d = 0; d < 3; ++d) {
+      int l, h;
+      if (entire) {
+        l = 0;
+        h = H.n[d] - 1;
+      } else if (H.n[d] > 1) {
+        l = H.s[d] + R.rlo[d];
+        h = H.e[d] + R.rhi[d];
+      } else {
+        l = H.s[d];
+        h = H.e[d];
+      }
+      R.ulo[d] = l;
+      R.ucnt[d] = h - l + 1;
+    }
+    R.utpp = (int)(((int64_t)R.ucnt[0] * R.ucnt[1] + 255) / 256);
+    R.uni = 1;
+  }
   return upload_tiles(c, tiles, &R.tblock, &R.tfirst, &R.ntiles);
 }
 
@@ -372,6 +399,7 @@ static int make_range_x1s(pangu_ctx *c, Range3 &R) {
   R.rhi[0] = R.rhi[1] = R.rhi[2] = 1;
   R.entire = 0;
   R.order = 0;
+  R.uni = 0;
   std::vector<int64_t> tiles(c->nblocks, 0);
   for (int b = 0; b < c->nblocks; ++b) {
     const HostBlock &H = c->hb[b];

@@ -65,6 +65,10 @@ struct Range3 {
   int rlo[3], rhi[3];
   int entire;
   int order;
+  // When every block of the context has one shape the tiles are decoded
+  // arithmetically from these (no table loads): lo and cnt of the range on
+  // each block, tiles per block plane.
+  int uni, ulo[3], ucnt[3], utpp;
 };
 
 // Work items counted per block (ghost cells, boundary slab cells), in tiles

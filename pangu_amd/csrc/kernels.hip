@@ -484,7 +484,7 @@ __global__ __launch_bounds__(256, 5) void k_div_update(KCtx C, KFields F0, KFiel
 // Recovery (invertor.cc:184-319) and, when HEAT, ApplyElectronHeating
 // (electron_heating.cc:66-109) on the recovered cell.  REC: 0 chainer,
 // 1 Kastaun, 2 1Dvsq alone, -1 none (heating only).
-template <int REC, bool HB, bool HK, bool HEAT>
+template <int REC, bool HB, bool HK, bool HEAT, bool GUESS>
 __global__ __launch_bounds__(256) void k_recover(KCtx C, KFields F, SharedTabs S, Range3 R) {
   int b, k, j, i;
   if (!decode_tile(C, R, b, k, j, i)) return;
@@ -552,7 +552,7 @@ __global__ __launch_bounds__(256) void k_recover(KCtx C, KFields F, SharedTabs S
       int st = 0, it = 0, itn = 0;
       bool write = true;
       if (REC == 1) {
-        if (S.guess) {
+        if (GUESS) {
           int ev = 0;
           st = c2p_kastaun_guess(U, Ph, P.gamma, m, (double)S.guess[b][id], S.delta, it, ev);
         } else {
@@ -1082,8 +1082,11 @@ int launch_recover(const LaunchCtx &L, const KFields &F, const SharedTabs &S, in
   if (R.ntiles == 0) return 0;
   const dim3 g = tile_grid(R);
   const bool hb = L.k.iBX >= 0, hk = L.k.iKEL >= 0;
-#define RC(R_, HB_, HK_, HE_) \
-  hipLaunchKernelGGL((k_recover<R_, HB_, HK_, HE_>), g, kTile, 0, s, L.k, F, S, R)
+#define RC(R_, HB_, HK_, HE_)                                                                 \
+  if (R_ == 1 && S.guess)                                                                      \
+    hipLaunchKernelGGL((k_recover<R_, HB_, HK_, HE_, (R_ == 1)>), g, kTile, 0, s, L.k, F, S, R); \
+  else                                                                                         \
+    hipLaunchKernelGGL((k_recover<R_, HB_, HK_, HE_, false>), g, kTile, 0, s, L.k, F, S, R)
 #define RC2(R_, HE_)                     \
   if (hb && hk) RC(R_, true, true, HE_);  \
   else if (hb) RC(R_, true, false, HE_);  \

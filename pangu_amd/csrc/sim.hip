@@ -570,6 +570,7 @@ int pangu_sim_plain_loop(pangu_sim **sims, int n, int ncycles, int mode, double 
 struct Tuning {
   int order[PANGU_KF_COUNT];
   int split;
+  int share;
 };
 static std::map<std::string, Tuning> g_tune_cache;
 
@@ -612,11 +613,13 @@ static double family_ms(int fam, const std::vector<std::string> &names, const st
 static int apply_tuning(pangu_sim *s, const Tuning &T) {
   for (int f = 0; f < PANGU_KF_COUNT; ++f) PCHK(s, pangu_set_tile_order(s->ctx, f, T.order[f]));
   PCHK(s, pangu_set_split_source(s->ctx, T.split));
+  PCHK(s, pangu_set_share_reconstruction(s->ctx, T.share));
   drop_graph(s);
   return 0;
 }
 
-int pangu_sim_autotune(pangu_sim *s, int cycles, int use_cache, int *out_orders, int *out_split) {
+int pangu_sim_autotune(pangu_sim *s, int cycles, int use_cache, int *out_orders, int *out_split,
+                       int *out_share) {
   if (!s) return PANGU_EINVAL;
   (void)hipSetDevice(s->device);
   if (cycles < 1) cycles = 1;
@@ -659,6 +662,7 @@ int pangu_sim_autotune(pangu_sim *s, int cycles, int use_cache, int *out_orders,
     const int ncycle0 = s->ncycle;
     for (int f = 0; f < PANGU_KF_COUNT; ++f) best.order[f] = pangu_get_tile_order(s->ctx, f);
     best.split = 0;
+    best.share = 0;
     int rc = apply_tuning(s, best);
     // coordinate descent: one family at a time, candidates timed with the
     // kernel clock over `cycles` fused cycles
@@ -671,6 +675,14 @@ int pangu_sim_autotune(pangu_sim *s, int cycles, int use_cache, int *out_orders,
       *ms = family_ms(fam, names, tot);
       return 0;
     };
+    if (!rc) {
+      double base_ms = 0, alt_ms = 0;
+      Tuning alt = best;
+      alt.share = 1;
+      rc = measure(best, PANGU_KF_FLUX_X1, &base_ms);
+      if (!rc) rc = measure(alt, PANGU_KF_FLUX_X1, &alt_ms);
+      if (!rc && alt_ms > 0 && alt_ms < base_ms) best = alt;
+    }
     for (int f = 0; f < PANGU_KF_COUNT && !rc; ++f) {
       double base_ms = 0, alt_ms = 0;
       Tuning alt = best;
@@ -700,6 +712,7 @@ int pangu_sim_autotune(pangu_sim *s, int cycles, int use_cache, int *out_orders,
   if (out_orders)
     for (int f = 0; f < PANGU_KF_COUNT; ++f) out_orders[f] = best.order[f];
   if (out_split) *out_split = best.split;
+  if (out_share) *out_share = best.share;
   return 0;
 }
 

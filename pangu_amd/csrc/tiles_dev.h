@@ -40,12 +40,24 @@ __device__ __forceinline__ bool decode_tile(const KCtx &C, const Range3 &R, int 
   const int L = (int)blockIdx.x;
   const int T = (L & 7) * n8 + (L >> 3);
   if (T >= N) return false;
-  b = R.tblock[T];
-  const int local = T - R.tfirst[b];
-  int lo[3], cnt[3];
-  block_range(C.blk[b], R, lo, cnt);
+  int lo[3], cnt[3], local, tpp;
+  if (R.uni) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = R.ulo[d];
+      cnt[d] = R.ucnt[d];
+    }
+    tpp = R.utpp;
+    const int per_block = cnt[2] * tpp;
+    b = T / per_block;
+    local = T - b * per_block;
+  } else {
+    b = R.tblock[T];
+    local = T - R.tfirst[b];
+    block_range(C.blk[b], R, lo, cnt);
+    tpp = (cnt[0] * cnt[1] + TILE - 1) / TILE;
+  }
   const int plane = cnt[0] * cnt[1];
-  const int tpp = (plane + TILE - 1) / TILE;
   int tile, kk;
   if (R.order) {
     tile = local / cnt[2];

@@ -63,7 +63,7 @@ def lib():
         L.pangu_sim_mesh_state.argtypes = [vp, C.c_int, dp, dp, ip, ip]
         L.pangu_sim_nblocks.argtypes = [vp]
         L.pangu_sim_plain_loop.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_int, dp]
-        L.pangu_sim_autotune.argtypes = [vp, C.c_int, C.c_int, ip, ip]
+        L.pangu_sim_autotune.argtypes = [vp, C.c_int, C.c_int, ip, ip, ip]
         L.pangu_sim_destroy.argtypes = [vp]
         L.pangu_sim_init.argtypes = [vp]
         L.pangu_sim_step.argtypes = [vp, C.c_int, C.c_int]
@@ -157,11 +157,12 @@ class HipSim:
         """Runs the autotuner (state restored afterwards); returns the chosen
         tile order per kernel family and the source split."""
         orders = (C.c_int * len(KERNEL_FAMILIES))()
-        split = C.c_int()
-        self._check(lib().pangu_sim_autotune(self._p, cycles, int(use_cache), orders, C.byref(split)),
-                    "autotune")
+        split, share = C.c_int(), C.c_int()
+        self._check(lib().pangu_sim_autotune(self._p, cycles, int(use_cache), orders, C.byref(split),
+                                             C.byref(share)), "autotune")
         names = sorted(KERNEL_FAMILIES, key=KERNEL_FAMILIES.get)
-        return {"order": {n: int(orders[KERNEL_FAMILIES[n]]) for n in names}, "split_source": split.value}
+        return {"order": {n: int(orders[KERNEL_FAMILIES[n]]) for n in names},
+                "split_source": split.value, "share_reconstruction": share.value}
 
     def close(self):
         if getattr(self, "_p", None):
