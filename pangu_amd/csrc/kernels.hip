@@ -222,6 +222,7 @@ __device__ __forceinline__ void face_flux(const KCtx &C, const KFields &F, doubl
   contravariant_flux<0>(gam, pr, sr, sg, UR);
   contravariant_flux<DIR>(gam, pl, sl, sg, FL);
   contravariant_flux<DIR>(gam, pr, sr, sg, FR);
+  const double den = cmax + cmin + 1e-20, rden = 1.0 / den;
 #pragma unroll
   for (int q = 0; q < NC; ++q) {
     const int s = slot_of<HB, HK>(q);
@@ -229,7 +230,7 @@ __device__ __forceinline__ void face_flux(const KCtx &C, const KFields &F, doubl
     if (RS == PANGU_RIEMANN_LAXF)
       f = 0.5 * (FL[s] + FR[s] - cc * (UR[s] - UL[s]));
     else
-      f = (cmax * FL[s] + cmin * FR[s] - cmax * cmin * (UR[s] - UL[s])) / (cmax + cmin + 1e-20);
+      f = div_by(cmax * FL[s] + cmin * FR[s] - cmax * cmin * (UR[s] - UL[s]), den, rden);
     F.flux[DIR - 1][b * NC + q][id] = f;
   }
 }
@@ -392,10 +393,11 @@ __global__ __launch_bounds__(256) void k_ct_faces(KCtx C, KFields F, double *con
 // geometric source U_nu += beta dt sqrt|g| T^r_c Gamma^c_{nu r}
 // (source_term.cc:171-228).  MODE: 1 = divergence only (writes dU/dt),
 // 2 = update only (reads dU/dt), 3 = source only, 4 = all three fused.
-// Five waves per SIMD is the most the source-term variants reach without
-// spilling; the kernel is memory bound, so occupancy pays.
+// Five waves per SIMD is the most the magnetised source-term variants reach
+// without spilling (the field-free ones spill at five and run at four); the
+// kernel is memory bound, so occupancy pays.
 template <int MODE, bool SRC, bool HB, bool HK>
-__global__ __launch_bounds__(256, 5) void k_div_update(KCtx C, KFields F0, KFields Finit, KFields F1,
+__global__ __launch_bounds__(256, HB ? 5 : 4) void k_div_update(KCtx C, KFields F0, KFields Finit, KFields F1,
                                                     double *const *dudt, const double *dt_dev,
                                                     double beta, double bdt_host, Range3 R) {
   int b, k, j, i;

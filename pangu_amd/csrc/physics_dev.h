@@ -2,7 +2,8 @@
 //
 // Each function keeps the reference's operand order so results match an
 // FP64 host evaluation bit for bit (builds use -ffp-contract=off; division
-// and sqrt are IEEE-rounded on gfx950).  Differences from a host evaluation
+// and sqrt are IEEE-rounded on gfx950, and the divisions of exact_div.h
+// return the IEEE quotient).  Differences from a host evaluation
 // can only come from pow/exp/log, whose device implementations may differ by
 // an ulp.  The metric arrives from host-built tables instead of being
 // recomputed, and a face's fluid state is computed once per side instead of
@@ -11,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include "common.h"
+#include "exact_div.h"
 
 namespace pangu {
 
@@ -58,9 +60,10 @@ __device__ __forceinline__ void calc_state(const double p[NPRIM], const double g
     s.ucov[r] = g[4 * r] * s.ucon[0] + g[4 * r + 1] * s.ucon[1] + g[4 * r + 2] * s.ucon[2] +
                 g[4 * r + 3] * s.ucon[3];
   s.bcon[0] = p[BX1] * s.ucov[1] + p[BX2] * s.ucov[2] + p[BX3] * s.ucov[3];
-  s.bcon[1] = (p[BX1] + s.bcon[0] * s.ucon[1]) / s.ucon[0];
-  s.bcon[2] = (p[BX2] + s.bcon[0] * s.ucon[2]) / s.ucon[0];
-  s.bcon[3] = (p[BX3] + s.bcon[0] * s.ucon[3]) / s.ucon[0];
+  const double ru0 = 1.0 / s.ucon[0];
+  s.bcon[1] = div_by(p[BX1] + s.bcon[0] * s.ucon[1], s.ucon[0], ru0);
+  s.bcon[2] = div_by(p[BX2] + s.bcon[0] * s.ucon[2], s.ucon[0], ru0);
+  s.bcon[3] = div_by(p[BX3] + s.bcon[0] * s.ucon[3], s.ucon[0], ru0);
 #pragma unroll
   for (int r = 0; r < 4; ++r)
     s.bcov[r] = g[4 * r] * s.bcon[0] + g[4 * r + 1] * s.bcon[1] + g[4 * r + 2] * s.bcon[2] +
@@ -126,8 +129,9 @@ __device__ __forceinline__ void fast_ms(double gamma, const double p[NPRIM], con
     disc = 0.;
   }
   disc = sqrt(disc);
-  const double vp = -(-B + disc) / (2. * A);
-  const double vm = -(-B - disc) / (2. * A);
+  const double a2 = 2. * A, ra2 = 1.0 / a2;
+  const double vp = div_by(-(-B + disc), a2, ra2);
+  const double vm = div_by(-(-B - disc), a2, ra2);
   if (vp > vm) {
     vmax = vp;
     vmin = vm;
@@ -154,8 +158,8 @@ __device__ __forceinline__ void recon_mc(double qm1, double q0, double qp1, doub
 // PPM with Colella-Woodward limiting (interpolater_ppm4.h:29-67).
 __device__ __forceinline__ void recon_ppm4(double qm2, double qm1, double q0, double qp1,
                                            double qp2, double &ql_ip1, double &qr_i) {
-  double ql = (7.0 * (q0 + qm1) - (qm2 + qp1)) / 12.0;
-  double qr = (7.0 * (q0 + qp1) - (qm1 + qp2)) / 12.0;
+  double ql = div12(7.0 * (q0 + qm1) - (qm2 + qp1));
+  double qr = div12(7.0 * (q0 + qp1) - (qm1 + qp2));
   ql = fmax(ql, fmin(q0, qm1));
   ql = fmin(ql, fmax(q0, qm1));
   qr = fmax(qr, fmin(q0, qp1));
