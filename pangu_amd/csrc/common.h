@@ -11,6 +11,21 @@
 
 namespace pangu {
 
+#ifdef __HIPCC__
+// Every data pointer the kernels receive points into device global memory,
+// but a pointer loaded from a pointer table is generic to the compiler, and
+// accesses through it become flat_* instructions (which also count against
+// the LDS/scalar counter and take a 64-bit vector address).  Accesses
+// through glb(p) compile to global_* instructions (scalar base + 32-bit
+// vector offset where possible).
+template <class T>
+using gptr = __attribute__((address_space(1))) T *;
+template <class T>
+__device__ __forceinline__ gptr<T> glb(T *p) {
+  return (gptr<T>)p;
+}
+#endif
+
 // Slot order of a full state vector (variable_mnemonics.h:7-10).
 enum Slot { RHO = 0, ENY, UX1, UX2, UX3, BX1, BX2, BX3, ENT, KEL, NPRIM };
 

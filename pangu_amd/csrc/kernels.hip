@@ -48,7 +48,7 @@ struct CenLoad {
   double g[16], gc0[4], alpha, sg;
 };
 __device__ __forceinline__ void load_cen_state(const DevGeo &G, int64_t t, CenLoad &m) {
-  const double *c = G.cen;
+  const auto c = glb(G.cen);
   const int64_t s = G.ncen;
 #pragma unroll
   for (int n = 0; n < 16; ++n) m.g[n] = c[(CG_GCOV + n) * s + t];
@@ -73,9 +73,9 @@ __global__ __launch_bounds__(256) void k_fix_cons(KCtx C, KFields F, uint8_t *co
   const int64_t t = cen_index(G, k, j, i);
   const int64_t id = ((int64_t)k * B.n[1] + j) * B.n[0] + i;
   constexpr int NC = ncomp<HB, HK>();
-  double *pp[NC];
+  gptr<double> pp[NC];
 #pragma unroll
-  for (int q = 0; q < NC; ++q) pp[q] = F.prim[b * NC + q];
+  for (int q = 0; q < NC; ++q) pp[q] = glb(F.prim[b * NC + q]);
   double p[NPRIM];
 #pragma unroll
   for (int s = 0; s < NPRIM; ++s) p[s] = 0.0;
@@ -88,8 +88,8 @@ __global__ __launch_bounds__(256) void k_fix_cons(KCtx C, KFields F, uint8_t *co
     const double small = 1.0e-20;
     const double sms = kmax(P.sigma_max, small);
     const double rho_before = kmax(p[RHO], small);
-    double rho_floor = G.cen[CG_RHOFL * G.ncen + t];
-    const double eng_floor = G.cen[CG_ENGFL * G.ncen + t];
+    double rho_floor = glb(G.cen)[CG_RHOFL * G.ncen + t];
+    const double eng_floor = glb(G.cen)[CG_ENGFL * G.ncen + t];
     St st;
     calc_state(p, m.g, m.gc0, m.alpha, st);
     const double rho = p[RHO], ug = p[ENY];
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(256) void k_fix_cons(KCtx C, KFields F, uint8_t *co
     if (HK && rho_after != rho_before) p[KEL] *= pow(rho_after / rho_before, -P.gamma);
     p[ENT] = (P.gamma - 1.0) * p[ENY] * pow(p[RHO], -P.gamma);
     if (HK) p[KEL] = clamp_by_ratio(P.ratio_min, P.ratio_max, p[ENT], p[KEL]);
-    if (P.metric == PANGU_METRIC_CKS && G.cen[CG_RAD * G.ncen + t] < P.r_excise) {
+    if (P.metric == PANGU_METRIC_CKS && glb(G.cen)[CG_RAD * G.ncen + t] < P.r_excise) {
       const double e_exc = P.pexcise / (P.gamma - 1.0);
       p[RHO] = P.dexcise;
       p[ENY] = e_exc;
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(256) void k_fix_cons(KCtx C, KFields F, uint8_t *co
     pp[4][id] = p[UX3];
     pp[ient<HB>()][id] = p[ENT];
     if (HK) pp[ikel<HB>()][id] = p[KEL];
-    if (mask) mask[b][id] = mk;
+    if (mask) glb(mask[b])[id] = mk;
   }
   if (CONS && interior) {
     St st;
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(256) void k_fix_cons(KCtx C, KFields F, uint8_t *co
     double f[NPRIM];
     contravariant_flux<0>(P.gamma, p, st, m.sg, f);
 #pragma unroll
-    for (int q = 0; q < NC; ++q) F.cons[b * NC + q][id] = f[slot_of<HB, HK>(q)];
+    for (int q = 0; q < NC; ++q) glb(F.cons[b * NC + q])[id] = f[slot_of<HB, HK>(q)];
   }
 }
 
@@ -174,7 +174,7 @@ __device__ __forceinline__ void face_flux(const KCtx &C, const KFields &F, doubl
   double g[16], gc0[4], gdd, sg, alpha, valid;
   if (DIR == 3 && !G.face[2]) {
     const int64_t t = cen_index(G, k, j, i);
-    const double *c = G.cen;
+    const auto c = glb(G.cen);
     const int64_t s = G.ncen;
 #pragma unroll
     for (int n = 0; n < 16; ++n) g[n] = c[(CG_GCOV + n) * s + t];
@@ -187,7 +187,7 @@ __device__ __forceinline__ void face_flux(const KCtx &C, const KFields &F, doubl
   } else {
     const int d = DIR - 1;
     const int64_t t = ((int64_t)(G.has3d ? k : 0) * G.fj[d] + j) * G.fi[d] + i;
-    const double *c = G.face[d];
+    const auto c = glb(G.face[d]);
     const int64_t s = G.nface[d];
 #pragma unroll
     for (int n = 0; n < 16; ++n) g[n] = c[(FG_GCOV + n) * s + t];
@@ -198,12 +198,12 @@ __device__ __forceinline__ void face_flux(const KCtx &C, const KFields &F, doubl
     alpha = c[FG_ALPHA * s + t];
     valid = c[FG_VALID * s + t];
   }
-  double *al = alfven[b * 3 + (DIR - 1)];
+  const auto al = glb(alfven[b * 3 + (DIR - 1)]);
   if (valid == 0.0) {
     // singular face metric (polar axis): zero flux and wave speed
     al[id] = 0.0;
 #pragma unroll
-    for (int q = 0; q < NC; ++q) F.flux[DIR - 1][b * NC + q][id] = 0.0;
+    for (int q = 0; q < NC; ++q) glb(F.flux[DIR - 1][b * NC + q])[id] = 0.0;
     return;
   }
   const double gam = C.ph.gamma;
@@ -231,7 +231,7 @@ __device__ __forceinline__ void face_flux(const KCtx &C, const KFields &F, doubl
       f = 0.5 * (FL[s] + FR[s] - cc * (UR[s] - UL[s]));
     else
       f = div_by(cmax * FL[s] + cmin * FR[s] - cmax * cmin * (UR[s] - UL[s]), den, rden);
-    F.flux[DIR - 1][b * NC + q][id] = f;
+    glb(F.flux[DIR - 1][b * NC + q])[id] = f;
   }
 }
 
@@ -253,7 +253,7 @@ __global__ __launch_bounds__(256) void k_flux(KCtx C, KFields F, double *const *
   }
 #pragma unroll
   for (int q = 0; q < NC; ++q) {
-    const double *a = F.prim[b * NC + q];
+    const auto a = glb(F.prim[b * NC + q]);
     const int s = slot_of<HB, HK>(q);
     if (LIM == PANGU_LIMITER_PPM4) {
       const double m3 = a[id - 3 * st], m2 = a[id - 2 * st], m1 = a[id - st], c0 = a[id],
@@ -314,7 +314,7 @@ __global__ __launch_bounds__(256) void k_flux_x1s(KCtx C, KFields F, double *con
   for (int q = 0; q < NC; ++q) {
     const int s = slot_of<HB, HK>(q);
     if (have) {
-      const double *a = F.prim[b * NC + q];
+      const auto a = glb(F.prim[b * NC + q]);
       if (LIM == PANGU_LIMITER_PPM4)
         recon_ppm4(a[id - 2], a[id - 1], a[id], a[id + 1], a[id + 2], rv[s], pr[s]);
       else
@@ -340,20 +340,20 @@ __global__ __launch_bounds__(256) void k_emf(KCtx C, KFields F, double *const *e
   const int b1 = C.iBX, b2 = C.iBX + 1, b3 = C.iBX + 2;
   double e1 = 0.0, e2 = 0.0, e3 = 0.0;
   if (e1on) {
-    const double *f2 = F.flux[1][b * nc + b3], *f3 = F.flux[2][b * nc + b2];
+    const auto f2 = glb(F.flux[1][b * nc + b3]), f3 = glb(F.flux[2][b * nc + b2]);
     e1 = 0.25 * ((f2[id] + f2[id - sk]) - (f3[id] + f3[id - sj]));
   }
   if (e2on) {
-    const double *f3 = F.flux[2][b * nc + b1], *f1 = F.flux[0][b * nc + b3];
+    const auto f3 = glb(F.flux[2][b * nc + b1]), f1 = glb(F.flux[0][b * nc + b3]);
     e2 = 0.25 * ((f3[id] + f3[id - 1]) - (f1[id] + f1[id - sk]));
   }
   if (e3on) {
-    const double *f1 = F.flux[0][b * nc + b2], *f2 = F.flux[1][b * nc + b1];
+    const auto f1 = glb(F.flux[0][b * nc + b2]), f2 = glb(F.flux[1][b * nc + b1]);
     e3 = 0.25 * ((f1[id] + f1[id - sj]) - (f2[id] + f2[id - 1]));
   }
-  emf[b * 3 + 0][id] = e1;
-  emf[b * 3 + 1][id] = e2;
-  emf[b * 3 + 2][id] = e3;
+  glb(emf[b * 3 + 0])[id] = e1;
+  glb(emf[b * 3 + 1])[id] = e2;
+  glb(emf[b * 3 + 2])[id] = e3;
 }
 
 // Face B-flux overwrite from the EMFs (constrained_transport.cc:78-127).
@@ -368,22 +368,22 @@ __global__ __launch_bounds__(256) void k_ct_faces(KCtx C, KFields F, double *con
   const int nc = C.nc;
   const int b1 = C.iBX, b2 = C.iBX + 1, b3 = C.iBX + 2;
   const int E1 = o2 && o3, E2 = o3 && o1, E3 = o1 && o2;
-  const double *e1 = emf[b * 3 + 0], *e2 = emf[b * 3 + 1], *e3 = emf[b * 3 + 2];
+  const auto e1 = glb(emf[b * 3 + 0]), e2 = glb(emf[b * 3 + 1]), e3 = glb(emf[b * 3 + 2]);
   const bool in_k = k <= B.e[2], in_j = j <= B.e[1], in_i = i <= B.e[0];
   if (in_k && in_j) {  // X1 faces: i in [is, ie+o1]
-    if (o1) F.flux[0][b * nc + b1][id] = 0;
-    if (E2) F.flux[0][b * nc + b3][id] = -0.5 * (e2[id] + e2[id + sk]);
-    if (E3) F.flux[0][b * nc + b2][id] = 0.5 * (e3[id] + e3[id + sj]);
+    if (o1) glb(F.flux[0][b * nc + b1])[id] = 0;
+    if (E2) glb(F.flux[0][b * nc + b3])[id] = -0.5 * (e2[id] + e2[id + sk]);
+    if (E3) glb(F.flux[0][b * nc + b2])[id] = 0.5 * (e3[id] + e3[id + sj]);
   }
   if (in_k && in_i) {  // X2 faces: j in [js, je+o2]
-    if (E1) F.flux[1][b * nc + b3][id] = 0.5 * (e1[id] + e1[id + sk]);
-    if (o2) F.flux[1][b * nc + b2][id] = 0.;
-    if (E3) F.flux[1][b * nc + b1][id] = -0.5 * (e3[id] + e3[id + 1]);
+    if (E1) glb(F.flux[1][b * nc + b3])[id] = 0.5 * (e1[id] + e1[id + sk]);
+    if (o2) glb(F.flux[1][b * nc + b2])[id] = 0.;
+    if (E3) glb(F.flux[1][b * nc + b1])[id] = -0.5 * (e3[id] + e3[id + 1]);
   }
   if (in_j && in_i && o3) {  // X3 faces: k in [ks, ke+o3]
-    if (E1) F.flux[2][b * nc + b2][id] = -0.5 * (e1[id] + e1[id + sj]);
-    if (E2) F.flux[2][b * nc + b1][id] = 0.5 * (e2[id] + e2[id + 1]);
-    F.flux[2][b * nc + b3][id] = 0;
+    if (E1) glb(F.flux[2][b * nc + b2])[id] = -0.5 * (e1[id] + e1[id + sj]);
+    if (E2) glb(F.flux[2][b * nc + b1])[id] = 0.5 * (e2[id] + e2[id + 1]);
+    glb(F.flux[2][b * nc + b3])[id] = 0;
   }
 }
 
@@ -413,29 +413,29 @@ __global__ __launch_bounds__(256, HB ? 5 : 4) void k_div_update(KCtx C, KFields 
     const double vol = B.dx[0] * B.dx[1] * B.dx[2];
 #pragma unroll
     for (int q = 0; q < NC; ++q) {
-      const double *f1 = F0.flux[0][b * NC + q];
+      const auto f1 = glb(F0.flux[0][b * NC + q]);
       double s = (a1 * f1[id + 1] - a1 * f1[id]);
       if (B.n[1] > 1) {
-        const double *f2 = F0.flux[1][b * NC + q];
+        const auto f2 = glb(F0.flux[1][b * NC + q]);
         s += (a2 * f2[id + sj] - a2 * f2[id]);
       }
       if (B.n[2] > 1) {
-        const double *f3 = F0.flux[2][b * NC + q];
+        const auto f3 = glb(F0.flux[2][b * NC + q]);
         s += (a3 * f3[id + sk] - a3 * f3[id]);
       }
       const double du = -s / vol;
       if (MODE == 1)
-        dudt[b * NC + q][id] = du;
+        glb(dudt[b * NC + q])[id] = du;
       else
-        unew[q] = Finit.cons[b * NC + q][id] + bdt * du;
+        unew[q] = glb(Finit.cons[b * NC + q])[id] + bdt * du;
     }
   } else if (MODE == 2) {
 #pragma unroll
     for (int q = 0; q < NC; ++q)
-      unew[q] = Finit.cons[b * NC + q][id] + bdt * dudt[b * NC + q][id];
+      unew[q] = glb(Finit.cons[b * NC + q])[id] + bdt * glb(dudt[b * NC + q])[id];
   } else {
 #pragma unroll
-    for (int q = 0; q < NC; ++q) unew[q] = F1.cons[b * NC + q][id];
+    for (int q = 0; q < NC; ++q) unew[q] = glb(F1.cons[b * NC + q])[id];
   }
   if (MODE == 1) return;
   if (SRC && (MODE == 3 || MODE == 4)) {
@@ -447,7 +447,7 @@ __global__ __launch_bounds__(256, HB ? 5 : 4) void k_div_update(KCtx C, KFields 
 #pragma unroll
     for (int s = 0; s < NPRIM; ++s) p[s] = 0.0;
 #pragma unroll
-    for (int q = 0; q < NC; ++q) p[slot_of<HB, HK>(q)] = F1.prim[b * NC + q][id];
+    for (int q = 0; q < NC; ++q) p[slot_of<HB, HK>(q)] = glb(F1.prim[b * NC + q])[id];
     St s;
     calc_state(p, m.g, m.gc0, m.alpha, s);
     const double gam = C.ph.gamma;
@@ -462,7 +462,7 @@ __global__ __launch_bounds__(256, HB ? 5 : 4) void k_div_update(KCtx C, KFields 
       T[2][c] = tmunu<2>(s, etot, ptot, c);
       T[3][c] = tmunu<3>(s, etot, ptot, c);
     }
-    const double *conn = G.cen + (int64_t)CG_CONN * G.ncen + t;
+    const auto conn = glb(G.cen) + (int64_t)CG_CONN * G.ncen + t;
     const bool sym = G.conn_sym != 0;
 #pragma unroll
     for (int dd = 0; dd < 4; ++dd) {
@@ -479,7 +479,7 @@ __global__ __launch_bounds__(256, HB ? 5 : 4) void k_div_update(KCtx C, KFields 
     }
   }
 #pragma unroll
-  for (int q = 0; q < NC; ++q) F1.cons[b * NC + q][id] = unew[q];
+  for (int q = 0; q < NC; ++q) glb(F1.cons[b * NC + q])[id] = unew[q];
 }
 
 // -------------------------------------------------- recovery (+ heating)
@@ -497,29 +497,31 @@ __global__ __launch_bounds__(256) void k_recover(KCtx C, KFields F, SharedTabs S
   const int64_t t = cen_index(G, k, j, i);
   const int64_t ns = G.ncen;
   const Phys &P = C.ph;
-  double *pp[NC];
+  gptr<double> pp[NC];
 #pragma unroll
-  for (int q = 0; q < NC; ++q) pp[q] = F.prim[b * NC + q];
+  for (int q = 0; q < NC; ++q) pp[q] = glb(F.prim[b * NC + q]);
   if (REC >= 0) {
     const bool chainer = (REC != 1);
     bool excised = false;
-    if (P.metric == PANGU_METRIC_CKS && G.cen[CG_RAD * ns + t] < P.r_excise) excised = true;
+    if (P.metric == PANGU_METRIC_CKS && glb(G.cen)[CG_RAD * ns + t] < P.r_excise) excised = true;
     if (excised) {
-      if (chainer && S.flag) S.flag[b][id] = 0.0;
-      if (S.status) S.status[b][id] = -1;
-      if (S.iters) S.iters[b][id] = 0;
+      if (chainer && S.flag) glb(S.flag[b])[id] = 0.0;
+      if (S.status) glb(S.status[b])[id] = -1;
+      if (S.iters) glb(S.iters[b])[id] = 0;
     } else {
       CenMetric m;
 #pragma unroll
       for (int n = 0; n < 16; ++n) {
-        m.gcov[n] = G.cen[(CG_GCOV + n) * ns + t];
-        m.gcon[n] = G.cen[(CG_GCON + n) * ns + t];
+        m.gcov[n] = glb(G.cen)[(CG_GCOV + n) * ns + t];
+        m.gcon[n] = glb(G.cen)[(CG_GCON + n) * ns + t];
       }
-      m.lapse = G.cen[CG_LAPSE * ns + t];
-      const double sg = G.cen[CG_SG * ns + t];
-      const double alpha = G.cen[CG_ALPHA * ns + t];
+      m.lapse = glb(G.cen)[CG_LAPSE * ns + t];
+      const double sg = glb(G.cen)[CG_SG * ns + t];
+      const double alpha = glb(G.cen)[CG_ALPHA * ns + t];
       double U[9], Ph[9];
-      const double *const *cons = F.cons + b * NC;
+      gptr<double> cons[NC];
+#pragma unroll
+      for (int q = 0; q < NC; ++q) cons[q] = glb(F.cons[b * NC + q]);
       const double c_rho = cons[0][id], c_eny = cons[1][id];
       const double isg = 1.0 / sg;
       const double aos = alpha * isg;
@@ -556,7 +558,7 @@ __global__ __launch_bounds__(256) void k_recover(KCtx C, KFields F, SharedTabs S
       if (REC == 1) {
         if (GUESS) {
           int ev = 0;
-          st = c2p_kastaun_guess(U, Ph, P.gamma, m, (double)S.guess[b][id], S.delta, it, ev);
+          st = c2p_kastaun_guess(U, Ph, P.gamma, m, (double)glb(S.guess[b])[id], S.delta, it, ev);
         } else {
           st = c2p_kastaun(U, Ph, P.gamma, m, it);
         }
@@ -579,11 +581,11 @@ __global__ __launch_bounds__(256) void k_recover(KCtx C, KFields F, SharedTabs S
       }
       if (chainer && S.flag) {
         const double fv = (st == 0) ? 1.0 : 0.0;
-        S.flag[b][id] = fv;
-        if (S.flag_rec) S.flag_rec[b][id] = fv;
+        glb(S.flag[b])[id] = fv;
+        if (S.flag_rec) glb(S.flag_rec[b])[id] = fv;
       }
-      if (S.status) S.status[b][id] = st;
-      if (S.iters) S.iters[b][id] = itn;
+      if (S.status) glb(S.status[b])[id] = st;
+      if (S.iters) glb(S.iters[b])[id] = itn;
       if (write) {
         pp[0][id] = Ph[RHO];
         pp[1][id] = Ph[ENY];
@@ -603,15 +605,14 @@ __global__ __launch_bounds__(256) void k_recover(KCtx C, KFields F, SharedTabs S
     load_cen_state(G, t, m);
     St s;
     calc_state(p, m.g, m.gc0, m.alpha, s);
-    const double *const *cons = F.cons + b * NC;
-    const double crho = cons[0][id];
+    const double crho = glb(F.cons[b * NC])[id];
     HeatIn c;
     c.rho = p[RHO];
     c.energy = p[ENY];
     c.bsq = s.bsq;
-    c.adv_total = cons[ient<HB>()][id] / crho;
+    c.adv_total = glb(F.cons[b * NC + ient<HB>()])[id] / crho;
     c.rec_total = p[ENT];
-    c.adv_e = cons[ikel<HB>()][id] / crho;
+    c.adv_e = glb(F.cons[b * NC + ikel<HB>()])[id] / crho;
     pp[ikel<HB>()][id] = heat_apply(P, c);
   }
 }
@@ -767,8 +768,8 @@ __global__ __launch_bounds__(256) void k_exchange(KCtx C, KFields F, double *con
   const int sk = k - o2 * (B.e[2] - B.s[2] + 1);
   const int64_t dst = ((int64_t)k * B.n[1] + j) * B.n[0] + i;
   const int64_t src = ((int64_t)sk * N.n[1] + sj) * N.n[0] + si;
-  for (int q = 0; q < C.np; ++q) F.prim[b * C.np + q][dst] = F.prim[nb * C.np + q][src];
-  if (flag) flag[b][dst] = flag[nb][src];
+  for (int q = 0; q < C.np; ++q) glb(F.prim[b * C.np + q])[dst] = glb(F.prim[nb * C.np + q])[src];
+  if (flag) glb(flag[b])[dst] = glb(flag[nb])[src];
 }
 
 // Cell g of the two ghost slabs normal to d (side 0 below s[d], side 1 above
@@ -832,11 +833,11 @@ __global__ __launch_bounds__(256) void k_bc(KCtx C, KFields F, double *const *fl
   const int64_t src = ((int64_t)sx[2] * B.n[1] + sx[1]) * B.n[0] + sx[0];
   const int iBX = C.iBX;
   for (int q = 0; q < C.np; ++q) {
-    double v = F.prim[b * C.np + q][src];
+    double v = glb(F.prim[b * C.np + q])[src];
     if (kind == PANGU_BC_REFLECT && (q == 2 + d || (iBX >= 0 && q == iBX + d))) v = -v;
-    F.prim[b * C.np + q][dst] = v;
+    glb(F.prim[b * C.np + q])[dst] = v;
   }
-  if (flag) flag[b][dst] = flag[b][src];
+  if (flag) glb(flag[b])[dst] = glb(flag[b])[src];
 }
 
 // --------------------------------------------------------------- timestep
@@ -859,7 +860,7 @@ __global__ __launch_bounds__(256) void k_dt_reduce(KCtx C, double *const *alfven
     const int64_t id = ((int64_t)k * B.n[1] + j) * B.n[0] + i;
     for (int d = 0; d < 3; ++d) {
       if (B.n[d] == 1) continue;
-      const double v = alfven[b * 3 + d][id];
+      const double v = glb(alfven[b * 3 + d])[id];
       if (v > c[d]) c[d] = v;
     }
   }

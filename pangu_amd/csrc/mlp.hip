@@ -379,23 +379,23 @@ __global__ __launch_bounds__(256, 2) void k_c2p_guess(KCtx C, KFields F, MlpDev 
     const int64_t ns = G.ncen;
     CenMetric m;
     for (int n = 0; n < 16; ++n) {
-      m.gcov[n] = G.cen[(CG_GCOV + n) * ns + t];
-      m.gcon[n] = G.cen[(CG_GCON + n) * ns + t];
+      m.gcov[n] = glb(G.cen)[(CG_GCOV + n) * ns + t];
+      m.gcon[n] = glb(G.cen)[(CG_GCON + n) * ns + t];
     }
-    m.lapse = G.cen[CG_LAPSE * ns + t];
-    const double sg = G.cen[CG_SG * ns + t];
-    const double alpha = G.cen[CG_ALPHA * ns + t];
+    m.lapse = glb(G.cen)[CG_LAPSE * ns + t];
+    const double sg = glb(G.cen)[CG_SG * ns + t];
+    const double alpha = glb(G.cen)[CG_ALPHA * ns + t];
     const int NC = C.nc;
     const double *const *cons = F.cons + (int64_t)b * NC;
     const double aos = alpha * (1.0 / sg);
     double U[9];
-    U[RHO] = aos * cons[0][id];
-    U[ENY] = aos * (cons[1][id] - cons[0][id]);
-    U[UX1] = aos * cons[2][id];
-    U[UX2] = aos * cons[3][id];
-    U[UX3] = aos * cons[4][id];
-    for (int a = 0; a < 3; ++a) U[BX1 + a] = C.iBX >= 0 ? aos * cons[C.iBX + a][id] : 0.0;
-    U[ENT] = aos * cons[C.iENT][id];
+    U[RHO] = aos * glb(cons[0])[id];
+    U[ENY] = aos * (glb(cons[1])[id] - glb(cons[0])[id]);
+    U[UX1] = aos * glb(cons[2])[id];
+    U[UX2] = aos * glb(cons[3])[id];
+    U[UX3] = aos * glb(cons[4])[id];
+    for (int a = 0; a < 3; ++a) U[BX1 + a] = C.iBX >= 0 ? aos * glb(cons[C.iBX + a])[id] : 0.0;
+    U[ENT] = aos * glb(cons[C.iENT])[id];
     KastaunIn kin;
     if (kastaun_setup(U, m, kin) == 0) {
       float f[5];
@@ -419,7 +419,7 @@ __global__ __launch_bounds__(256, 2) void k_c2p_guess(KCtx C, KFields F, MlpDev 
   float z = 0.0f;
   for (int jn = 0; jn < NH; ++jn) z += bf2f(buf0[lane * LROW + jn]) * bf2f(Wt.w4[jn * NOUT]);
   z += Wt.b4[0];
-  if (cell) guess[b][id] = valid ? 1.0f / (1.0f + expf(-z)) : -1.0f;
+  if (cell) glb(guess[b])[id] = valid ? 1.0f / (1.0f + expf(-z)) : -1.0f;
 }
 
 int launch_c2p_guess(const LaunchCtx &L, const KFields &F, const MlpDev &W, float *const *guess,
