@@ -251,21 +251,29 @@ __global__ __launch_bounds__(256) void k_flux(KCtx C, KFields F, double *const *
     pl[s] = 0.0;
     pr[s] = 0.0;
   }
+  // Every stencil value of every variable is loaded before the first
+  // reconstruction, so a wavefront waits for memory once rather than once
+  // per variable (the compiler keeps the loads of an unrolled per-variable
+  // loop behind each variable's arithmetic).  The peak register use stays in
+  // the face physics.
+  constexpr int NS = (LIM == PANGU_LIMITER_PPM4) ? 6 : 4;  // offsets -NS/2 .. NS/2 - 1
+  double v[NC][NS];
 #pragma unroll
   for (int q = 0; q < NC; ++q) {
     const auto a = glb(F.prim[b * NC + q]);
+#pragma unroll
+    for (int n = 0; n < NS; ++n) v[q][n] = a[id + (n - NS / 2) * st];
+  }
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
     const int s = slot_of<HB, HK>(q);
+    double t0, t1;
     if (LIM == PANGU_LIMITER_PPM4) {
-      const double m3 = a[id - 3 * st], m2 = a[id - 2 * st], m1 = a[id - st], c0 = a[id],
-                   p1 = a[id + st], p2 = a[id + 2 * st];
-      double t0, t1;
-      recon_ppm4(m3, m2, m1, c0, p1, pl[s], t0);
-      recon_ppm4(m2, m1, c0, p1, p2, t1, pr[s]);
+      recon_ppm4(v[q][0], v[q][1], v[q][2], v[q][3], v[q][4], pl[s], t0);
+      recon_ppm4(v[q][1], v[q][2], v[q][3], v[q][4], v[q][5], t1, pr[s]);
     } else {
-      const double m2 = a[id - 2 * st], m1 = a[id - st], c0 = a[id], p1 = a[id + st];
-      double t0, t1;
-      recon_mc(m2, m1, c0, pl[s], t0);
-      recon_mc(m1, c0, p1, t1, pr[s]);
+      recon_mc(v[q][0], v[q][1], v[q][2], pl[s], t0);
+      recon_mc(v[q][1], v[q][2], v[q][3], t1, pr[s]);
     }
   }
   face_flux<DIR, RS, HB, HK>(C, F, alfven, b, k, j, i, pl, pr);
@@ -393,11 +401,51 @@ __global__ __launch_bounds__(256) void k_ct_faces(KCtx C, KFields F, double *con
 // geometric source U_nu += beta dt sqrt|g| T^r_c Gamma^c_{nu r}
 // (source_term.cc:171-228).  MODE: 1 = divergence only (writes dU/dt),
 // 2 = update only (reads dU/dt), 3 = source only, 4 = all three fused.
+// Flux divergence of one cell (and, for MODE 4, the update from the initial
+// container) with the X2 / X3 terms present when D2 / D3.
+template <bool D2, bool D3, int MODE, int NC>
+__device__ __forceinline__ void flux_div(const DevBlock &B, const KFields &F0, const KFields &Finit,
+                                         double *const *dudt, int b, int64_t id, int64_t sj,
+                                         int64_t sk, double bdt, double unew[NC]) {
+  const double a1 = B.dx[1] * B.dx[2], a2 = B.dx[0] * B.dx[2], a3 = B.dx[0] * B.dx[1];
+  const double vol = B.dx[0] * B.dx[1] * B.dx[2];
+  const double rvol = 1.0 / vol;
+  double f1p[NC], f1m[NC], f2p[NC], f2m[NC], f3p[NC], f3m[NC], u0[NC];
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
+    const auto f1 = glb(F0.flux[0][b * NC + q]);
+    f1p[q] = f1[id + 1];
+    f1m[q] = f1[id];
+    if (D2) {
+      const auto f2 = glb(F0.flux[1][b * NC + q]);
+      f2p[q] = f2[id + sj];
+      f2m[q] = f2[id];
+    }
+    if (D3) {
+      const auto f3 = glb(F0.flux[2][b * NC + q]);
+      f3p[q] = f3[id + sk];
+      f3m[q] = f3[id];
+    }
+    if (MODE == 4) u0[q] = glb(Finit.cons[b * NC + q])[id];
+  }
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
+    double s = (a1 * f1p[q] - a1 * f1m[q]);
+    if (D2) s += (a2 * f2p[q] - a2 * f2m[q]);
+    if (D3) s += (a3 * f3p[q] - a3 * f3m[q]);
+    const double du = div_by(-s, vol, rvol);
+    if (MODE == 1)
+      glb(dudt[b * NC + q])[id] = du;
+    else
+      unew[q] = u0[q] + bdt * du;
+  }
+}
+
 // Five waves per SIMD is the most the magnetised source-term variants reach
 // without spilling (the field-free ones spill at five and run at four); the
 // kernel is memory bound, so occupancy pays.
 template <int MODE, bool SRC, bool HB, bool HK>
-__global__ __launch_bounds__(256, HB ? 5 : 4) void k_div_update(KCtx C, KFields F0, KFields Finit, KFields F1,
+__global__ __launch_bounds__(256, (HB && MODE != 1) ? 5 : 4) void k_div_update(KCtx C, KFields F0, KFields Finit, KFields F1,
                                                     double *const *dudt, const double *dt_dev,
                                                     double beta, double bdt_host, Range3 R) {
   int b, k, j, i;
@@ -409,26 +457,17 @@ __global__ __launch_bounds__(256, HB ? 5 : 4) void k_div_update(KCtx C, KFields 
   const double bdt = dt_dev ? beta * dt_dev[2 * B.mesh] : bdt_host;
   double unew[NC];
   if (MODE == 1 || MODE == 4) {
-    const double a1 = B.dx[1] * B.dx[2], a2 = B.dx[0] * B.dx[2], a3 = B.dx[0] * B.dx[1];
-    const double vol = B.dx[0] * B.dx[1] * B.dx[2];
-#pragma unroll
-    for (int q = 0; q < NC; ++q) {
-      const auto f1 = glb(F0.flux[0][b * NC + q]);
-      double s = (a1 * f1[id + 1] - a1 * f1[id]);
-      if (B.n[1] > 1) {
-        const auto f2 = glb(F0.flux[1][b * NC + q]);
-        s += (a2 * f2[id + sj] - a2 * f2[id]);
-      }
-      if (B.n[2] > 1) {
-        const auto f3 = glb(F0.flux[2][b * NC + q]);
-        s += (a3 * f3[id + sk] - a3 * f3[id]);
-      }
-      const double du = -s / vol;
-      if (MODE == 1)
-        glb(dudt[b * NC + q])[id] = du;
-      else
-        unew[q] = glb(Finit.cons[b * NC + q])[id] + bdt * du;
-    }
+    // one uniform branch on the block's active dimensions, so the loads of
+    // all components sit in one basic block and overlap
+    const bool d2 = B.n[1] > 1, d3 = B.n[2] > 1;
+    if (d2 && d3)
+      flux_div<true, true, MODE, NC>(B, F0, Finit, dudt, b, id, sj, sk, bdt, unew);
+    else if (d2)
+      flux_div<true, false, MODE, NC>(B, F0, Finit, dudt, b, id, sj, sk, bdt, unew);
+    else if (d3)
+      flux_div<false, true, MODE, NC>(B, F0, Finit, dudt, b, id, sj, sk, bdt, unew);
+    else
+      flux_div<false, false, MODE, NC>(B, F0, Finit, dudt, b, id, sj, sk, bdt, unew);
   } else if (MODE == 2) {
 #pragma unroll
     for (int q = 0; q < NC; ++q)
