@@ -184,7 +184,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not run)"}))
+                                   "(sharding across GPUs is designed but not run here)"}))
         sys.exit(2)
     from pangu_amd import configs
     from pangu_amd.sim import HipSim
