@@ -20,6 +20,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -265,6 +267,20 @@ def main():
         "wall_one_cycle_s": wall_one,
         "sim_time": sim.time,
         "dt": sim.dt,
+    }
+    # run diagnostics on the final state: shell fluxes at r = 10 (x1 = ln r),
+    # volume integrals and the CT invariant, with the cost of one call
+    shell = int((np.log(10.0) - cfg["xmin"][0]) / (cfg["xmax"][0] - cfg["xmin"][0]) * cfg["nx"][0])
+    sim.diagnostics(shell)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        diag = sim.diagnostics(shell)
+    diag_ms = (time.perf_counter() - t0) / 5 * 1e3
+    dx_min = min((cfg["xmax"][d] - cfg["xmin"][d]) / cfg["nx"][d] for d in range(3))
+    out["diagnostics"] = {
+        "shell_x1_index": shell, "ms_per_call": diag_ms,
+        **{k: float(v) for k, v in diag.items()},
+        "divb_relative": float(diag["divb_max"] * dx_min / diag["ub_max"]) if diag["ub_max"] else 0.0,
     }
     if args.copy_calibration:
         n = args.copy_calibration

@@ -206,6 +206,26 @@ int pangu_set_split_source(pangu_ctx *ctx, int split);
  * reconstructing both neighbours per face.  Results are identical. */
 int pangu_set_share_reconstruction(pangu_ctx *ctx, int share);
 
+/* ---- run diagnostics (no reference counterpart) ----
+ * doc/grmhd_intro.md:556-574 defines the fluxes through a sphere that a
+ * torus run is monitored by; the reference code computes none of them.
+ * Evaluated on container md at the cell centres of one interior x1 index
+ * per mesh (shell[m], 0 <= shell[m] < mesh cells along x1), in code
+ * coordinates over dx2 dx3 (a surface flux does not depend on them):
+ *   MDOT = -sum rho u^1 sqrt|g|,  EDOT = sum T^1_0 sqrt|g|,
+ *   JDOT = -sum T^1_3 sqrt|g|,    PHI  = 1/2 sum |B^1| sqrt|g|;
+ * over the interior: MASS = sum U_rho dV, ENERGY = sum U_energy dV,
+ * DIVB = max |corner divergence of U_B| (the quantity constrained transport
+ * keeps at round-off), UBMAX = max |U_B| (its scale).  out is a device array
+ * of nmesh * PANGU_NDIAG doubles, written in stream order; the reductions run
+ * in a fixed order, so repeated calls give identical bits.  shell is a host
+ * array of one index per mesh. */
+enum {
+  PANGU_DIAG_MDOT = 0, PANGU_DIAG_EDOT, PANGU_DIAG_JDOT, PANGU_DIAG_PHI, PANGU_DIAG_MASS,
+  PANGU_DIAG_ENERGY, PANGU_DIAG_DIVB, PANGU_DIAG_UBMAX, PANGU_NDIAG
+};
+int pangu_diagnostics(pangu_ctx *ctx, int md, const int *shell, double *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

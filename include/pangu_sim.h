@@ -64,6 +64,9 @@ int pangu_sim_create_batch(const pangu_sim_cfg *cfgs, int nmesh, int device, pan
 int pangu_sim_mesh_state(pangu_sim *s, int m, double *dt, double *time, int *first_block,
                          int *nblocks);
 int pangu_sim_nblocks(pangu_sim *s);
+/* pangu_diagnostics on container 0 (base) or 1 (stage "1"), one shell index
+ * per mesh; out (host) receives nmesh * PANGU_NDIAG doubles. */
+int pangu_sim_diagnostics(pangu_sim *s, int container, const int *shell, double *out);
 int pangu_sim_destroy(pangu_sim *s);
 int pangu_sim_init(pangu_sim *s);
 /* Runs ncycles RK2 cycles in the given mode (tasks: one C-ABI call per

@@ -100,6 +100,15 @@ double oracle_sim_time(void *sim) { return ((Sim *)sim)->time; }
 double oracle_sim_dt(void *sim) { return ((Sim *)sim)->dt; }
 int oracle_sim_ncycle(void *sim) { return ((Sim *)sim)->ncycle; }
 
+int oracle_sim_diagnostics(void *sim, int c, int shell, double *out, double *mag) {
+  if (c < 0 || c > 1) return 1;
+  double m[ODIAG_N];
+  ((Sim *)sim)->diagnostics(c, shell, out, m);
+  if (mag)
+    for (int q = 0; q < ODIAG_N; ++q) mag[q] = m[q];
+  return 0;
+}
+
 int oracle_sim_block_geom(void *sim, int b, double *xd, int *n) {
   Sim *s = (Sim *)sim;
   if (b < 0 || b >= s->nblk) return -1;

@@ -75,6 +75,16 @@ double oracle_sim_dt(void *sim);
 int oracle_sim_ncycle(void *sim);
 int oracle_sim_block_geom(void *sim, int b, double *xg_dx_out /* 6 */, int *n_out /* 9 */);
 
+/* Diagnostics of container c (diagnostics.cpp): shell integrals at the
+ * interior x1 cell index `shell` of the mesh, volume integrals, div B.
+ * mag (optional) receives the sum of |contribution| of each summed entry. */
+enum {
+  ODIAG_MDOT = 0, ODIAG_EDOT = 1, ODIAG_JDOT = 2, ODIAG_PHI = 3, ODIAG_MASS = 4,
+  ODIAG_ENERGY = 5, ODIAG_DIVB = 6, ODIAG_UBMAX = 7, ODIAG_N = 8
+};
+int oracle_sim_diagnostics(void *sim, int container, int shell, double *out /* ODIAG_N */,
+                           double *mag /* ODIAG_N or NULL */);
+
 /* planted-mistake switch (tests only): 0 = faithful */
 void oracle_plant(int which);
 

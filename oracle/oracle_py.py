@@ -71,6 +71,7 @@ def lib():
         L.oracle_sim_dt.argtypes = [vp]
         L.oracle_sim_ncycle.argtypes = [vp]
         L.oracle_sim_block_geom.argtypes = [vp, C.c_int, dp, ip]
+        L.oracle_sim_diagnostics.argtypes = [vp, C.c_int, C.c_int, dp, dp]
         L.oracle_metric_at.argtypes = [C.c_int, dp, C.c_double, C.c_double, dp, dp, dp]
         L.oracle_christoffel.argtypes = [C.c_int, dp, C.c_double, C.c_double, dp]
         L.oracle_state.argtypes = [dp, dp, dp, dp]
@@ -115,6 +116,9 @@ TASKS = dict(fix_primitive=0, conservative=1, fluxes=2, ct=3, flux_div=4, update
              recovery=7, heating=8, fix_recovery=9, exchange=10, bc=11, timestep=12)
 
 
+DIAG_NAMES = ("mdot", "edot", "jdot", "phi", "mass", "energy", "divb_max", "ub_max")
+
+
 class OracleSim:
     """One oracle run: Parthenon-like blocks, stage containers, RK2 cycles."""
 
@@ -149,6 +153,16 @@ class OracleSim:
 
     def init(self):
         assert lib().oracle_sim_init(self._p) == 0
+
+    def diagnostics(self, shell, container=0, magnitudes=False):
+        """Shell integrals at interior x1 index `shell`, volume integrals and
+        div B of one container (oracle/diagnostics.cpp), as a dict; with
+        magnitudes, also the dict of sums of |contribution| per entry."""
+        out = np.zeros(len(DIAG_NAMES))
+        mag = np.zeros(len(DIAG_NAMES))
+        assert lib().oracle_sim_diagnostics(self._p, container, shell, _dp(out), _dp(mag)) == 0
+        res = dict(zip(DIAG_NAMES, out))
+        return (res, dict(zip(DIAG_NAMES, mag))) if magnitudes else res
 
     def step(self, n=1):
         lib().oracle_sim_step(self._p, n)

@@ -74,6 +74,7 @@ struct Sim {
   void exchange(int c);
   void apply_bc(int c);
   double estimate_dt() const;
+  void diagnostics(int c, int shell, double out[ODIAG_N], double mag[ODIAG_N]) const;
 
   void ensure_stage_container();
   void run_stage(int stage);

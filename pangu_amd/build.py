@@ -17,7 +17,7 @@ LIB = os.path.join(HERE, "libpangu_hip.so")
 ARCH = os.environ.get("PANGU_OFFLOAD_ARCH", "gfx950")
 
 HOST_SRC = ["geometry.cpp", "problems.cpp"]
-HIP_SRC = ["kernels.hip", "api.hip", "sim.hip", "mlp.hip"]
+HIP_SRC = ["kernels.hip", "api.hip", "sim.hip", "mlp.hip", "diag.hip"]
 
 GXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-fno-fast-math",
              "-Wall", "-Wno-unused-variable"]

@@ -57,6 +57,9 @@ struct pangu_ctx {
   double *dred = nullptr;
   int *dcluster = nullptr;
   double *dest = nullptr;
+  // diagnostics: per-tile partial sums, per-mesh tile ranges, shell indices
+  double *ddiag_part = nullptr;
+  int *ddiag_mtile = nullptr, *ddiag_shell = nullptr;
   LaunchCtx L;
   std::string err;
 };
@@ -599,7 +602,8 @@ int pangu_destroy(pangu_ctx *c) {
   }
   void *ptrs[] = {c->dblk,    c->dgeo,  c->dnbr,     c->dalfven, c->demf,  c->dflag,
                   c->dflag_rec, c->dmask, c->dstatus, c->diters,  c->dred,  c->dcluster,
-                  c->dest,    c->dguess, c->dguess_tab};
+                  c->dest,    c->dguess, c->dguess_tab, c->ddiag_part, c->ddiag_mtile,
+                  c->ddiag_shell};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   delete c;
@@ -866,3 +870,37 @@ int pangu_stage(pangu_ctx *c, int md_init, int md0, int md1, double beta, double
 }
 
 }  // extern "C"
+
+int pangu_diagnostics(pangu_ctx *c, int md, const int *shell, double *out, void *stream) {
+  if (int r = check_md(c, md)) return r;
+  if (!shell || !out) return fail(c, PANGU_EINVAL, "diagnostics: null shell or output array");
+  for (int m = 0; m < c->nmesh; ++m) {
+    const HostBlock &H = c->hb[c->mfirst[m]];
+    const int nx1 = c->mcfg[m].nb[0] * (H.e[0] - H.s[0] + 1);
+    if (shell[m] < 0 || shell[m] >= nx1)
+      return fail(c, PANGU_EINVAL, "diagnostics: shell index outside the mesh's x1 cells");
+  }
+  HIPCHK(c, hipSetDevice(c->device));
+  if (!c->ddiag_part) {
+    // tiles of the interior range, grouped by block and therefore by mesh
+    std::vector<int> mtile(c->nmesh + 1, 0);
+    const int zero[3] = {0, 0, 0};
+    int64_t tot = 0;
+    for (int m = 0; m < c->nmesh; ++m) {
+      mtile[m] = (int)tot;
+      for (int b = c->mfirst[m]; b < c->mfirst[m + 1]; ++b) tot += range_tiles(c->hb[b], zero, zero, 0);
+    }
+    mtile[c->nmesh] = (int)tot;
+    if (tot != c->L.interior.ntiles) return fail(c, PANGU_EINVAL, "diagnostics: tile count mismatch");
+    HIPCHK(c, hipMalloc((void **)&c->ddiag_part, (size_t)(tot > 0 ? tot : 1) * PANGU_NDIAG * sizeof(double)));
+    if (int r = upload(c, &c->ddiag_mtile, mtile.data(), mtile.size())) return r;
+    HIPCHK(c, hipMalloc((void **)&c->ddiag_shell, (size_t)c->nmesh * sizeof(int)));
+  }
+  hipStream_t s = STREAM(stream);
+  HIPCHK(c, hipMemcpyAsync(c->ddiag_shell, shell, (size_t)c->nmesh * sizeof(int), hipMemcpyHostToDevice, s));
+  LCHK(c, launch_diagnostics(c->L, c->conts[md].kf(), c->ddiag_shell, c->ddiag_part, c->ddiag_mtile,
+                             c->nmesh, out, s));
+  // the shell indices are read from pageable host memory: wait before returning
+  HIPCHK(c, hipStreamSynchronize(s));
+  return PANGU_OK;
+}

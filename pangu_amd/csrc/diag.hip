@@ -1,0 +1,159 @@
+// Run diagnostics on the GPU: shell integrals of the mass, energy, angular
+// momentum and magnetic fluxes through a surface x1 = const (the quantities
+// of doc/grmhd_intro.md:556-574 in code coordinates), volume integrals of
+// the conserved mass and energy, and the CT invariant (corner div B).  The
+// reference computes none of these itself.
+//
+// Reductions run in a fixed order (a tree per 256-thread tile, then one
+// workgroup per mesh over the tiles), so results repeat bit for bit from
+// run to run.
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+#include "kernels.h"
+#include "physics_dev.h"
+#include "tiles_dev.h"
+
+namespace pangu {
+
+template <bool HB>
+__global__ __launch_bounds__(256) void k_diag(KCtx C, KFields F, const int *shell, double *part,
+                                              Range3 R) {
+  __shared__ double red[PANGU_NDIAG][TILE];
+  const int N = R.ntiles;
+  const int n8 = (N + 7) >> 3;
+  const int L = (int)blockIdx.x;
+  const int T = (L & 7) * n8 + (L >> 3);
+  if (T >= N) return;  // uniform over the workgroup
+  double v[PANGU_NDIAG];
+#pragma unroll
+  for (int q = 0; q < PANGU_NDIAG; ++q) v[q] = 0.0;
+  int b, k, j, i;
+  if (decode_tile(C, R, b, k, j, i)) {
+    const DevBlock &B = C.blk[b];
+    const int nc = C.nc;
+    const int64_t sj = B.n[0], sk = (int64_t)B.n[0] * B.n[1];
+    const int64_t id = k * sk + j * sj + i;
+    const double dA = B.dx[1] * B.dx[2];
+    const double dV = B.dx[0] * B.dx[1] * B.dx[2];
+    v[PANGU_DIAG_MASS] = glb(F.cons[b * nc])[id] * dV;
+    v[PANGU_DIAG_ENERGY] = glb(F.cons[b * nc + 1])[id] * dV;
+    const int nx0 = B.e[0] - B.s[0] + 1;
+    if (B.lx[0] * nx0 + (i - B.s[0]) == shell[B.mesh]) {
+      const DevGeo &G = C.geo[B.geo];
+      const int64_t t = ((int64_t)(G.has3d ? k : 0) * G.nj + j) * G.ni + i;
+      const auto c = glb(G.cen);
+      const int64_t s = G.ncen;
+      double g[16], gc0[4];
+#pragma unroll
+      for (int n = 0; n < 16; ++n) g[n] = c[(CG_GCOV + n) * s + t];
+#pragma unroll
+      for (int n = 0; n < 4; ++n) gc0[n] = c[(CG_GCON + n) * s + t];
+      const double alpha = c[CG_ALPHA * s + t], sg = c[CG_SG * s + t];
+      double p[NPRIM];
+#pragma unroll
+      for (int n = 0; n < NPRIM; ++n) p[n] = 0.0;
+      for (int q = 0; q < 5; ++q) p[q] = glb(F.prim[b * nc + q])[id];
+      if (C.iBX >= 0)
+        for (int q = 0; q < 3; ++q) p[BX1 + q] = glb(F.prim[b * nc + C.iBX + q])[id];
+      St st;
+      calc_state(p, g, gc0, alpha, st);
+      const double gam = C.ph.gamma;
+      const double pg = (gam - 1.) * p[ENY];
+      const double h = p[RHO] + gam * p[ENY];
+      const double etot = h + st.bsq, ptot = pg + 0.5 * st.bsq;
+      const double t0 = tmunu<1>(st, etot, ptot, 0), t3 = tmunu<1>(st, etot, ptot, 3);
+      v[PANGU_DIAG_MDOT] = -(p[RHO] * st.ucon[1] * sg) * dA;
+      v[PANGU_DIAG_EDOT] = t0 * sg * dA;
+      v[PANGU_DIAG_JDOT] = -(t3 * sg) * dA;
+      v[PANGU_DIAG_PHI] = 0.5 * fabs(p[BX1]) * sg * dA;
+    }
+    if (HB) {
+      const int ndim = (B.n[0] > 1) + (B.n[1] > 1) + (B.n[2] > 1);
+      gptr<double> ub[3];
+      for (int d = 0; d < 3; ++d) ub[d] = glb(F.cons[b * nc + C.iBX + d]);
+      double m = 0.0;
+      for (int d = 0; d < ndim; ++d) m = fmax(m, fabs(ub[d][id]));
+      v[PANGU_DIAG_UBMAX] = m;
+      if (ndim == 2 && i > B.s[0] && j > B.s[1]) {
+        const auto b1 = ub[0], b2 = ub[1];
+        const double d1 = (b1[id] + b1[id - sj] - b1[id - 1] - b1[id - sj - 1]) / (2 * B.dx[0]);
+        const double d2 = (b2[id] + b2[id - 1] - b2[id - sj] - b2[id - sj - 1]) / (2 * B.dx[1]);
+        v[PANGU_DIAG_DIVB] = fabs(d1 + d2);
+      } else if (ndim == 3 && i > B.s[0] && j > B.s[1] && k > B.s[2]) {
+        double tot = 0.0;
+        for (int d = 0; d < 3; ++d) {
+          double acc = 0.0;
+          for (int dk = 0; dk < 2; ++dk)
+            for (int dj = 0; dj < 2; ++dj)
+              for (int di = 0; di < 2; ++di) {
+                const int off = d == 0 ? di : (d == 1 ? dj : dk);
+                const double sign = off == 0 ? 1.0 : -1.0;
+                acc = acc + sign * ub[d][id - dk * sk - dj * sj - di];
+              }
+          tot = tot + acc / (4 * B.dx[d]);
+        }
+        v[PANGU_DIAG_DIVB] = fabs(tot);
+      }
+    }
+  }
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < PANGU_NDIAG; ++q) red[q][tid] = v[q];
+  __syncthreads();
+  for (int w = TILE / 2; w > 0; w >>= 1) {
+    if (tid < w) {
+#pragma unroll
+      for (int q = 0; q < PANGU_NDIAG; ++q)
+        red[q][tid] = (q < PANGU_DIAG_DIVB) ? red[q][tid] + red[q][tid + w]
+                                            : fmax(red[q][tid], red[q][tid + w]);
+    }
+    __syncthreads();
+  }
+  if (tid < PANGU_NDIAG) glb(part)[(int64_t)T * PANGU_NDIAG + tid] = red[tid][0];
+}
+
+// One workgroup per mesh: the tiles of mesh m are [mtile[m], mtile[m + 1]).
+__global__ __launch_bounds__(256) void k_diag_final(const double *part, const int *mtile, double *out) {
+  __shared__ double red[PANGU_NDIAG][TILE];
+  const int m = blockIdx.x, tid = threadIdx.x;
+  const int t0 = mtile[m], t1 = mtile[m + 1];
+  double v[PANGU_NDIAG];
+#pragma unroll
+  for (int q = 0; q < PANGU_NDIAG; ++q) v[q] = 0.0;
+  for (int T = t0 + tid; T < t1; T += TILE)
+#pragma unroll
+    for (int q = 0; q < PANGU_NDIAG; ++q) {
+      const double x = glb(part)[(int64_t)T * PANGU_NDIAG + q];
+      v[q] = (q < PANGU_DIAG_DIVB) ? v[q] + x : fmax(v[q], x);
+    }
+#pragma unroll
+  for (int q = 0; q < PANGU_NDIAG; ++q) red[q][tid] = v[q];
+  __syncthreads();
+  for (int w = TILE / 2; w > 0; w >>= 1) {
+    if (tid < w) {
+#pragma unroll
+      for (int q = 0; q < PANGU_NDIAG; ++q)
+        red[q][tid] = (q < PANGU_DIAG_DIVB) ? red[q][tid] + red[q][tid + w]
+                                            : fmax(red[q][tid], red[q][tid + w]);
+    }
+    __syncthreads();
+  }
+  if (tid < PANGU_NDIAG) glb(out)[(int64_t)m * PANGU_NDIAG + tid] = red[tid][0];
+}
+
+int launch_diagnostics(const LaunchCtx &L, const KFields &F, const int *shell, double *part,
+                       const int *mtile, int nmesh, double *out, hipStream_t s) {
+  const Range3 &R = L.interior;
+  if (R.ntiles > 0) {
+    const dim3 g((unsigned)(((R.ntiles + 7) / 8) * 8));
+    if (L.k.iBX >= 0)
+      hipLaunchKernelGGL((k_diag<true>), g, dim3(TILE), 0, s, L.k, F, shell, part, R);
+    else
+      hipLaunchKernelGGL((k_diag<false>), g, dim3(TILE), 0, s, L.k, F, shell, part, R);
+  }
+  hipLaunchKernelGGL(k_diag_final, dim3((unsigned)nmesh), dim3(TILE), 0, s, part, mtile, out);
+  return (int)hipGetLastError();
+}
+
+}  // namespace pangu

@@ -134,6 +134,8 @@ int launch_exchange(const LaunchCtx &L, const KFields &F, double *const *flag, c
 int launch_bc(const LaunchCtx &L, const KFields &F, double *const *flag, hipStream_t s);
 // red: 3 doubles of scratch per interior plane; state: {dt, time} per mesh;
 // est_out (optional): one estimate per mesh
+int launch_diagnostics(const LaunchCtx &L, const KFields &F, const int *shell, double *part,
+                       const int *mtile, int nmesh, double *out, hipStream_t s);
 int launch_dt(const LaunchCtx &L, double *const *alfven, double *red, double *state,
               double dt_factor, double *est_out, hipStream_t s);
 

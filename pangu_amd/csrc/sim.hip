@@ -41,6 +41,7 @@ struct pangu_sim {
   int32_t *status = nullptr, *iters = nullptr;
   int h_base = -1, h_s1 = -1, h_dudt = -1;
   double *dstate = nullptr;  // {dt, time} per mesh
+  double *ddiag = nullptr;   // diagnostics output, PANGU_NDIAG per mesh
   double dt = 0, time = 0, dt_est = DBL_MAX;  // mesh 0 (task mode: the only mesh)
   int nmesh = 1;
   std::vector<pangu_sim_cfg> mcfg;  // per mesh
@@ -311,7 +312,8 @@ int pangu_sim_destroy(pangu_sim *s) {
   (void)hipSetDevice(s->device);
   drop_graph(s);
   void *ptrs[] = {s->P[0], s->P[1], s->U[0], s->U[1], s->dudt, s->F[0], s->F[1], s->F[2],
-                  s->alfven, s->emf, s->flag, s->flag_rec, s->mask, s->status, s->iters, s->dstate};
+                  s->alfven, s->emf, s->flag, s->flag_rec, s->mask, s->status, s->iters, s->dstate,
+                  s->ddiag};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (s->ctx) pangu_destroy(s->ctx);
@@ -898,6 +900,14 @@ int pangu_sim_mesh_state(pangu_sim *s, int m, double *dt, double *time, int *fir
 }
 
 int pangu_sim_nblocks(pangu_sim *s) { return s ? s->nblk : 0; }
+
+int pangu_sim_diagnostics(pangu_sim *s, int container, const int *shell, double *out) {
+  if (!s || !shell || !out || container < 0 || container > 1) return PANGU_EINVAL;
+  if (!s->ddiag) SCHK(s, hipMalloc((void **)&s->ddiag, (size_t)s->nmesh * PANGU_NDIAG * sizeof(double)));
+  PCHK(s, pangu_diagnostics(s->ctx, container == 0 ? s->h_base : s->h_s1, shell, s->ddiag, s->stream));
+  SCHK(s, hipMemcpy(out, s->ddiag, (size_t)s->nmesh * PANGU_NDIAG * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
 
 int pangu_sim_block_geom(pangu_sim *s, int b, double *xd, int *n) {
   if (b < 0 || b >= s->nblk) return PANGU_EINVAL;

@@ -44,6 +44,8 @@ FIELDS = dict(prim=0, cons=1, flux1=2, flux2=3, flux3=4, alfven=5, emf=6, flag=7
 TASKS = dict(fix_primitive=0, conservative=1, fluxes=2, ct=3, flux_div=4, update=5, source=6,
              recovery=7, heating=8, fix_recovery=9, exchange=10, bc=11, timestep=12)
 MODES = dict(tasks=0, fused=1, graph=2)
+# pangu_diagnostics outputs, in PANGU_DIAG_* order
+DIAG_NAMES = ("mdot", "edot", "jdot", "phi", "mass", "energy", "divb_max", "ub_max")
 KERNEL_FAMILIES = dict(fix_cons=0, flux_x1=1, flux_x2=2, flux_x3=3, ct=4, div_update=5, recovery=6)
 
 _lib = None
@@ -61,6 +63,7 @@ def lib():
         L.pangu_sim_create.argtypes = [C.POINTER(SimCfg), C.c_int, C.POINTER(vp)]
         L.pangu_sim_create_batch.argtypes = [C.POINTER(SimCfg), C.c_int, C.c_int, C.POINTER(vp)]
         L.pangu_sim_mesh_state.argtypes = [vp, C.c_int, dp, dp, ip, ip]
+        L.pangu_sim_diagnostics.argtypes = [vp, C.c_int, ip, dp]
         L.pangu_sim_nblocks.argtypes = [vp]
         L.pangu_sim_plain_loop.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_int, dp]
         L.pangu_sim_autotune.argtypes = [vp, C.c_int, C.c_int, ip, ip, ip]
@@ -152,6 +155,29 @@ class HipSim:
         self._check(lib().pangu_sim_mesh_state(self._p, m, C.byref(dt), C.byref(tm), C.byref(fb),
                                                C.byref(nb)), "mesh_state")
         return dt.value, tm.value, fb.value, nb.value
+
+    def diagnostics(self, shell, container=0):
+        """Shell fluxes at interior x1 index `shell` (one per mesh, or one for
+        all), volume integrals and div B of a container (pangu_diagnostics):
+        a dict of DIAG_NAMES for a one-mesh simulation, else a list of them."""
+        nm = self.nmesh
+        sh = np.asarray([shell] * nm if np.isscalar(shell) else shell, dtype=np.int32)
+        out = np.zeros(nm * len(DIAG_NAMES))
+        self._check(lib().pangu_sim_diagnostics(self._p, container, sh.ctypes.data_as(C.POINTER(C.c_int)),
+                                                out.ctypes.data_as(C.POINTER(C.c_double))), "diagnostics")
+        res = [dict(zip(DIAG_NAMES, out[m * len(DIAG_NAMES):(m + 1) * len(DIAG_NAMES)])) for m in range(nm)]
+        return res[0] if nm == 1 else res
+
+    def history(self, cycles, every, shell, mode="graph"):
+        """Steps `cycles` cycles and records the diagnostics every `every`
+        cycles: a list of (cycles done, time of mesh 0, diagnostics)."""
+        rows, done = [], 0
+        while done < cycles:
+            n = min(every, cycles - done)
+            self.step(n, mode)
+            done += n
+            rows.append((done, self.mesh_state(0)[1], self.diagnostics(shell)))
+        return rows
 
     def autotune(self, cycles=3, use_cache=True):
         """Runs the autotuner (state restored afterwards); returns the chosen
