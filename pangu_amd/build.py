@@ -14,6 +14,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpangu_hip.so")
+PHDF_LIB = os.path.join(HERE, "libpangu_phdf.so")
+# HDF5 (C library) for the Parthenon-style output writer
+HDF5_ROOT = os.environ.get("PANGU_HDF5_ROOT", "/opt/conda")
 ARCH = os.environ.get("PANGU_OFFLOAD_ARCH", "gfx950")
 
 HOST_SRC = ["geometry.cpp", "problems.cpp"]
@@ -67,7 +70,23 @@ def build(jobs=8, verbose=False):
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lgomp"])
     os.replace(tmp, LIB)
+    build_phdf()
     return LIB
+
+
+def build_phdf():
+    """libpangu_phdf.so (host only, links HDF5); skipped when the HDF5
+    headers are absent, in which case pangu_amd.phdf raises on use."""
+    inc = os.path.join(HDF5_ROOT, "include")
+    libdir = os.path.join(HDF5_ROOT, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        sys.stderr.write(f"pangu_amd.build: no hdf5.h under {inc}; libpangu_phdf.so not built\n")
+        return None
+    tmp = PHDF_LIB + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", f"-I{inc}", "phdf.cpp", "-o", tmp,
+          f"-L{libdir}", "-lhdf5", f"-Wl,-rpath,{libdir}"])
+    os.replace(tmp, PHDF_LIB)
+    return PHDF_LIB
 
 
 if __name__ == "__main__":
