@@ -72,8 +72,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--autotune", action="store_true",
                     help="re-tune launch parameters instead of applying the recorded tuning")
-    ap.add_argument("--cpu-sample-cycles", type=int, default=2)
-    ap.add_argument("--ensemble-scale", type=float, default=0.25,
+    ap.add_argument("--cpu-sample-cycles", type=int, default=3)
+    ap.add_argument("--ensemble-scale", type=float, default=1.0,
                     help="fraction of the 96-torus + 3x121-block ensemble for the batched "
                          "vs plain comparison (0 skips it)")
     ap.add_argument("--learned-samples", type=int, default=1 << 20,
