@@ -53,9 +53,9 @@ __global__ __launch_bounds__(256) void k_diag(KCtx C, KFields F, const int *shel
       double p[NPRIM];
 #pragma unroll
       for (int n = 0; n < NPRIM; ++n) p[n] = 0.0;
-      for (int q = 0; q < 5; ++q) p[q] = glb(F.prim[b * nc + q])[id];
+      for (int q = 0; q < 5; ++q) p[q] = glb(F.prim[b * C.np + q])[id];
       if (C.iBX >= 0)
-        for (int q = 0; q < 3; ++q) p[BX1 + q] = glb(F.prim[b * nc + C.iBX + q])[id];
+        for (int q = 0; q < 3; ++q) p[BX1 + q] = glb(F.prim[b * C.np + C.iBX + q])[id];
       St st;
       calc_state(p, g, gc0, alpha, st);
       const double gam = C.ph.gamma;
