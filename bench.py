@@ -112,7 +112,34 @@ def cpu_baseline(cfg_small, cycles):
                    f"{d['nx'][0]}x{d['nx'][1]}x{d['nx'][2]} zones in "
                    f"{d['nb'][0] * d['nb'][1] * d['nb'][2]} blocks, "
                    f"OpenMP {nthreads} threads, metric recomputed per evaluation; {dt:.2f} s"),
+        "algorithmic_ops_per_zone_cycle": algorithmic_ops(O, cfg_small),
     }
+
+
+def algorithmic_ops(O, cfg):
+    """Operations of the reference algorithm per interior zone and cycle,
+    counted by the oracle's counting build on one 64x64x32 block of the
+    torus mesh (the block at x1 in [1.65, 3.1], x2 in [-1, 0], x3 in
+    [-pi, -pi/2], which holds part of the torus): with the metric
+    recomputed per evaluation as the reference does, and with tabulated
+    geometry as the GPU path does.  flops = add + mul + div + sqrt."""
+    import math
+    d = dict(cfg)
+    d.update(nx=[64, 64, 32], nb=[1, 1, 1], xmin=[cfg["xmin"][0] + 1.45, -1.0, -math.pi],
+             xmax=[cfg["xmin"][0] + 2 * 1.45, 0.0, -math.pi / 2])
+    zones = 64 * 64 * 32
+    out = {}
+    for name, cache in (("reference", 0), ("tabulated_geometry", 1)):
+        d["cache_geometry"] = cache
+        c = O.op_counts(d, 1)
+        tot = c.sum(axis=0) / zones
+        row = {op: round(float(v), 1) for op, v in zip(O.OP_NAMES, tot)}
+        row["flops"] = round(float(sum(tot[O.OP_NAMES.index(k)] for k in ("add", "mul", "div", "sqrt"))), 1)
+        row["per_task_flops"] = {
+            task: round(float(sum(c[ti, O.OP_NAMES.index(k)] for k in ("add", "mul", "div", "sqrt")) / zones), 1)
+            for ti, task in enumerate(O.TASK_NAMES) if c[ti].sum() > 0}
+        out[name] = row
+    return out
 
 
 def batched_vs_plain(scale, cycles=10, warmup=2):

@@ -125,6 +125,24 @@ int oracle_sim_block_geom(void *sim, int b, double *xd, int *n) {
 
 void oracle_plant(int which) { g_plant = which; }
 
+#ifdef ORACLE_COUNTING
+static_assert((int)OP_N == (int)ORACLE_NOPS && (int)ORACLE_NTASKS == 13, "operation kinds and tasks");
+int oracle_op_counts(uint64_t *out) {
+  for (int t = 0; t < ORACLE_NTASKS; ++t)
+    for (int q = 0; q < ORACLE_NOPS; ++q) out[t * ORACLE_NOPS + q] = g_task_ops[t][q];
+  return 0;
+}
+void oracle_op_reset(void) {
+  for (int t = 0; t < ORACLE_NTASKS; ++t)
+    for (int q = 0; q < ORACLE_NOPS; ++q) g_task_ops[t][q] = 0;
+}
+#else
+int oracle_op_counts(uint64_t *) { return -1; }
+void oracle_op_reset(void) {}
+#endif
+
+#ifndef ORACLE_COUNTING
+// Per-cell helpers for the tests (double interface; not in the counting build).
 int oracle_metric_at(int kind, const double x[4], double a, double h, double gcov[16],
                      double gcon[16], double *gdet) {
   real gc[4][4], gn[4][4];
@@ -202,5 +220,7 @@ double oracle_heating(const oracle_cfg *c, const double cell[6]) {
   HeatCell hc{cell[0], cell[1], cell[2], cell[3], cell[4], cell[5]};
   return heating_apply(p, hc);
 }
+
+#endif  // ORACLE_COUNTING
 
 }  // extern "C"

@@ -5,7 +5,9 @@
 #include <algorithm>
 #include <cfloat>
 #include <cstring>
+#ifdef _OPENMP
 #include <omp.h>
+#endif
 
 #include "oracle.h"
 #include "oracle_api.h"
@@ -124,7 +126,9 @@ void Sim::setup(const oracle_cfg &c) {
   dt = c.dt_init;
   dt_est = DBL_MAX;
   ncycle = 0;
+#ifdef _OPENMP
   if (c.nthreads > 0) omp_set_num_threads(c.nthreads);
+#endif
   build_geometry_memo();
 }
 
@@ -412,14 +416,14 @@ void Sim::ct() {
       e2[id] = 0.0;
       e3[id] = 0.0;
       if (E1)
-        e1[id] = 0.25 * ((Fc(1, b, iB3)[id] + Fc(1, b, iB3)[id - sk]) -
-                         (Fc(2, b, iB2)[id] + Fc(2, b, iB2)[id - sj]));
+        e1[id] = 0.25 * (((real)Fc(1, b, iB3)[id] + Fc(1, b, iB3)[id - sk]) -
+                         ((real)Fc(2, b, iB2)[id] + Fc(2, b, iB2)[id - sj]));
       if (E2)
-        e2[id] = 0.25 * ((Fc(2, b, iB1)[id] + Fc(2, b, iB1)[id - 1]) -
-                         (Fc(0, b, iB3)[id] + Fc(0, b, iB3)[id - sk]));
+        e2[id] = 0.25 * (((real)Fc(2, b, iB1)[id] + Fc(2, b, iB1)[id - 1]) -
+                         ((real)Fc(0, b, iB3)[id] + Fc(0, b, iB3)[id - sk]));
       if (E3)
-        e3[id] = 0.25 * ((Fc(0, b, iB2)[id] + Fc(0, b, iB2)[id - sj]) -
-                         (Fc(1, b, iB1)[id] + Fc(1, b, iB1)[id - 1]));
+        e3[id] = 0.25 * (((real)Fc(0, b, iB2)[id] + Fc(0, b, iB2)[id - sj]) -
+                         ((real)Fc(1, b, iB1)[id] + Fc(1, b, iB1)[id - 1]));
     }
   });
   for_bkj(nblk, B0.s[2], B0.e[2], B0.s[1], B0.e[1], [&](int b, int k, int j) {
@@ -429,10 +433,10 @@ void Sim::ct() {
     for (int i = B.s[0]; i <= B.e[0] + o1; ++i) {
       const int64_t id = B.idx(k, j, i);
       if (o1) Fc(0, b, iB1)[id] = 0;
-      if (E2) Fc(0, b, iB3)[id] = -0.5 * (e2[id] + e2[id + sk]);
-      if (E3 && g_plant == PLANT_CT_SIGN) Fc(0, b, iB2)[id] = -0.5 * (e3[id] + e3[id + sj]);
+      if (E2) Fc(0, b, iB3)[id] = -0.5 * ((real)e2[id] + e2[id + sk]);
+      if (E3 && g_plant == PLANT_CT_SIGN) Fc(0, b, iB2)[id] = -0.5 * ((real)e3[id] + e3[id + sj]);
       else
-      if (E3) Fc(0, b, iB2)[id] = 0.5 * (e3[id] + e3[id + sj]);
+      if (E3) Fc(0, b, iB2)[id] = 0.5 * ((real)e3[id] + e3[id + sj]);
     }
   });
   for_bkj(nblk, B0.s[2], B0.e[2], B0.s[1], B0.e[1] + o2, [&](int b, int k, int j) {
@@ -441,9 +445,9 @@ void Sim::ct() {
     const double *e1 = Ec(b, 0), *e3 = Ec(b, 2);
     for (int i = B.s[0]; i <= B.e[0]; ++i) {
       const int64_t id = B.idx(k, j, i);
-      if (E1) Fc(1, b, iB3)[id] = 0.5 * (e1[id] + e1[id + sk]);
+      if (E1) Fc(1, b, iB3)[id] = 0.5 * ((real)e1[id] + e1[id + sk]);
       if (o2) Fc(1, b, iB2)[id] = 0.;
-      if (E3) Fc(1, b, iB1)[id] = -0.5 * (e3[id] + e3[id + 1]);
+      if (E3) Fc(1, b, iB1)[id] = -0.5 * ((real)e3[id] + e3[id + 1]);
     }
   });
   for_bkj(nblk, B0.s[2], B0.e[2] + o3, B0.s[1], B0.e[1], [&](int b, int k, int j) {
@@ -452,8 +456,8 @@ void Sim::ct() {
     const double *e1 = Ec(b, 0), *e2 = Ec(b, 1);
     for (int i = B.s[0]; i <= B.e[0]; ++i) {
       const int64_t id = B.idx(k, j, i);
-      if (E1) Fc(2, b, iB2)[id] = -0.5 * (e1[id] + e1[id + sj]);
-      if (E2) Fc(2, b, iB1)[id] = 0.5 * (e2[id] + e2[id + 1]);
+      if (E1) Fc(2, b, iB2)[id] = -0.5 * ((real)e1[id] + e1[id + sj]);
+      if (E2) Fc(2, b, iB1)[id] = 0.5 * ((real)e2[id] + e2[id + 1]);
       if (o3) Fc(2, b, iB3)[id] = 0;
     }
   });
@@ -467,8 +471,8 @@ void Sim::flux_div() {
   for_bkj(nblk, B0.s[2], B0.e[2], B0.s[1], B0.e[1], [&](int b, int k, int j) {
     const Block &B = blk[b];
     const int64_t sj = B.n[0], sk = (int64_t)B.n[0] * B.n[1];
-    const double a1 = B.dx[1] * B.dx[2], a2 = B.dx[0] * B.dx[2], a3 = B.dx[0] * B.dx[1];
-    const double vol = B.dx[0] * B.dx[1] * B.dx[2];
+    const real a1 = B.dx[1] * B.dx[2], a2 = B.dx[0] * B.dx[2], a3 = B.dx[0] * B.dx[1];
+    const real vol = B.dx[0] * B.dx[1] * B.dx[2];
     for (int q = 0; q < nc; ++q) {
       const double *f1 = Fc(0, b, q), *f2 = Fc(1, b, q), *f3 = Fc(2, b, q);
       double *du = dudt.data() + B.off * nc + (int64_t)q * B.cells;
@@ -494,7 +498,7 @@ void Sim::update(double bdt, int out) {
       double *o = Uc(out, b, q);
       for (int i = B.s[0]; i <= B.e[0]; ++i) {
         const int64_t id = B.idx(k, j, i);
-        o[id] = in[id] + bdt * du[id];
+        o[id] = in[id] + (real)bdt * du[id];
       }
     }
   });
@@ -818,7 +822,7 @@ double Sim::estimate_dt() const {
       for (int k = B.s[2]; k <= B.e[2]; ++k)
         for (int j = B.s[1]; j <= B.e[1]; ++j)
           for (int i = B.s[0]; i <= B.e[0]; ++i) {
-            const double dtc = p.cfl * B.dx[d] / al[B.idx(k, j, i)];
+            const real dtc = (real)p.cfl * B.dx[d] / al[B.idx(k, j, i)];
             if (mm > dtc) mm = dtc;
           }
       m[d] = mm;
@@ -833,26 +837,41 @@ double Sim::estimate_dt() const {
 // One RK stage of Simulator::MakeTaskCollection (ideal_grmhd.cc:23-120).
 // rk2: beta = {1, 1/2}, stage_name = {"base", "1", "base"}; the stage uses
 // beta[stage % 2] (:29).
+#ifdef ORACLE_COUNTING
+uint64_t g_ops[OP_N];
+uint64_t g_task_ops[ORACLE_NTASKS][OP_N];
+// Runs one task and adds the operations it performed to its bucket.
+#define COUNTED(task, call)                                                 \
+  do {                                                                      \
+    uint64_t s_[OP_N];                                                      \
+    for (int q_ = 0; q_ < OP_N; ++q_) s_[q_] = g_ops[q_];                   \
+    call;                                                                   \
+    for (int q_ = 0; q_ < OP_N; ++q_) g_task_ops[task][q_] += g_ops[q_] - s_[q_]; \
+  } while (0)
+#else
+#define COUNTED(task, call) call
+#endif
+
 void Sim::run_stage(int stage) {
   const double beta = (stage == 1) ? 0.5 : 1.0;
   const int c0 = (stage == 1) ? 0 : 1;
   const int c1 = (stage == 1) ? 1 : 0;
   ensure_stage_container();
   const double bdt = beta * dt;
-  fix_primitive(c0);
-  calc_conservative(c0);
-  fluxes(c0);
-  if (p.enable_B) ct();
-  flux_div();
-  update(bdt, c1);
-  if (cfg.exchange_after == 0) exchange(c1);
-  source(c1, bdt);
-  recovery(c1);
-  if (p.enable_heating) heating(c1);
-  if (p.recovery != REC_KASTAUN) fix_recovery(c1);
-  if (cfg.exchange_after == 1) exchange(c1);
-  apply_bc(c1);
-  if (stage == 2) dt_est = estimate_dt();
+  COUNTED(ORT_FIX_PRIMITIVE, fix_primitive(c0));
+  COUNTED(ORT_CONSERVATIVE, calc_conservative(c0));
+  COUNTED(ORT_FLUXES, fluxes(c0));
+  if (p.enable_B) COUNTED(ORT_CT, ct());
+  COUNTED(ORT_FLUX_DIV, flux_div());
+  COUNTED(ORT_UPDATE, update(bdt, c1));
+  if (cfg.exchange_after == 0) COUNTED(ORT_EXCHANGE, exchange(c1));
+  COUNTED(ORT_SOURCE, source(c1, bdt));
+  COUNTED(ORT_RECOVERY, recovery(c1));
+  if (p.enable_heating) COUNTED(ORT_HEATING, heating(c1));
+  if (p.recovery != REC_KASTAUN) COUNTED(ORT_FIX_RECOVERY, fix_recovery(c1));
+  if (cfg.exchange_after == 1) COUNTED(ORT_EXCHANGE, exchange(c1));
+  COUNTED(ORT_BC, apply_bc(c1));
+  if (stage == 2) COUNTED(ORT_TIMESTEP, dt_est = estimate_dt());
 }
 
 void Sim::ensure_stage_container() {

@@ -73,12 +73,12 @@ static void metric_cks(const real x[4], real g[4][4], real a) {
   const real tmp = r2c - a2;
   const real sqrtarg = tmp * tmp + 4.0 * a2 * zz * zz;
   const real r = std::sqrt(0.5 * (tmp + std::sqrt(sqrtarg)));
-  const real rs = (r > 1.0e-15) ? r : 1.0e-15;
+  const real rs = (r > 1.0e-15) ? r : real(1.0e-15);
   const real r2 = rs * rs;
   const real r3 = r2 * rs;
   const real r4 = r2 * r2;
   const real xi_den = r4 + a2 * zz * zz;
-  const real xi = (xi_den > 0.0) ? (2.0 * r3 / xi_den) : 0.0;
+  const real xi = (xi_den > 0.0) ? (2.0 * r3 / xi_den) : real(0.0);
   const real den_l = r2 + a2;
   const real l[4] = {1.0, (rs * xx + a * yy) / den_l, (rs * yy - a * xx) / den_l, zz / rs};
   for (int m = 0; m < 4; ++m)

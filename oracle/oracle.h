@@ -37,9 +37,17 @@
 #include <cstdint>
 #include <vector>
 
+#ifdef ORACLE_COUNTING
+#include "counting.h"
+#endif
+
 namespace oracle {
 
+#ifdef ORACLE_COUNTING
+typedef Real real;  // operation-counting build (counting.h)
+#else
 typedef double real;
+#endif
 
 // Slot order of a full primitive/conserved vector
 // (pangu/src/initialization/variable_mnemonics.h:7-10).

@@ -85,6 +85,15 @@ enum {
 int oracle_sim_diagnostics(void *sim, int container, int shell, double *out /* ODIAG_N */,
                            double *mag /* ODIAG_N or NULL */);
 
+/* Operation counts of the counting build (liboracle_count.so, counting.h):
+ * out[task * ORACLE_NOPS + op] accumulates, per ORT_* task of run_stage,
+ * the operations of kind op (add/sub, mul, div, sqrt, compare, fmin/fmax,
+ * fabs, exp, log, pow, sin/cos) since the last reset.  Returns -1 in the
+ * ordinary build. */
+enum { ORACLE_NOPS = 11, ORACLE_NTASKS = 13 };
+int oracle_op_counts(uint64_t *out /* ORACLE_NTASKS * ORACLE_NOPS */);
+void oracle_op_reset(void);
+
 /* planted-mistake switch (tests only): 0 = faithful */
 void oracle_plant(int which);
 

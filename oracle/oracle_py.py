@@ -118,6 +118,63 @@ TASKS = dict(fix_primitive=0, conservative=1, fluxes=2, ct=3, flux_div=4, update
 
 DIAG_NAMES = ("mdot", "edot", "jdot", "phi", "mass", "energy", "divb_max", "ub_max")
 
+# Operation counting (liboracle_count.so, counting.h): task rows follow the
+# ORT_* order, operation columns the OpKind order.
+TASK_NAMES = ("fix_primitive", "conservative", "fluxes", "ct", "flux_div", "update", "source",
+              "recovery", "heating", "fix_recovery", "exchange", "bc", "timestep")
+OP_NAMES = ("add", "mul", "div", "sqrt", "cmp", "minmax", "fabs", "exp", "log", "pow", "sincos")
+_clib = None
+
+
+def lib_count():
+    """The operation-counting build of the oracle (single-threaded)."""
+    global _clib
+    if _clib is None:
+        path = os.path.join(_HERE, "_build", "liboracle_count.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        vp = C.c_void_p
+        L.oracle_sim_create.restype = vp
+        L.oracle_sim_create.argtypes = [C.POINTER(OracleCfg)]
+        L.oracle_sim_destroy.argtypes = [vp]
+        L.oracle_sim_init.argtypes = [vp]
+        L.oracle_sim_step.argtypes = [vp, C.c_int]
+        L.oracle_op_counts.argtypes = [C.POINTER(C.c_uint64)]
+        L.oracle_op_reset.argtypes = []
+        L.oracle_sim_field_size.restype = C.c_int64
+        L.oracle_sim_field_size.argtypes = [vp, C.c_int]
+        L.oracle_sim_get.argtypes = [vp, C.c_int, C.c_int, vp]
+        _clib = L
+    return _clib
+
+
+def op_counts(cfg_dict, cycles=1, return_prim=False):
+    """Operations performed by each task over `cycles` RK2 cycles after the
+    initial data, as a (task, operation) array (TASK_NAMES x OP_NAMES); with
+    return_prim, also the flat primitive arena of the base container."""
+    L = lib_count()
+    d = dict(cfg_dict)
+    d["nthreads"] = 1
+    cfg = make_cfg(d)
+    p = L.oracle_sim_create(C.byref(cfg))
+    if not p:
+        raise ValueError("oracle_sim_create rejected the configuration")
+    try:
+        assert L.oracle_sim_init(p) == 0
+        L.oracle_op_reset()
+        assert L.oracle_sim_step(p, cycles) == 0
+        out = np.zeros(len(TASK_NAMES) * len(OP_NAMES), dtype=np.uint64)
+        assert L.oracle_op_counts(out.ctypes.data_as(C.POINTER(C.c_uint64))) == 0
+        prim = None
+        if return_prim:
+            prim = np.empty(L.oracle_sim_field_size(p, FIELDS["prim"]) // 8)
+            assert L.oracle_sim_get(p, FIELDS["prim"], 0, prim.ctypes.data_as(C.c_void_p)) == 0
+    finally:
+        L.oracle_sim_destroy(p)
+    counts = out.reshape(len(TASK_NAMES), len(OP_NAMES))
+    return (counts, prim) if return_prim else counts
+
 
 class OracleSim:
     """One oracle run: Parthenon-like blocks, stage containers, RK2 cycles."""

@@ -20,7 +20,7 @@ static inline double Xc(const Block &B, int d, int idx) { return B.xg[d] + (idx 
 static inline double Xf(const Block &B, int d, int idx) { return B.xg[d] + idx * B.dx[d]; }
 
 struct Geo {
-  double gcov[4][4], gcon[4][4], gdet;
+  real gcov[4][4], gcon[4][4], gdet;
   int ok;
 };
 struct BlockMemo {

@@ -189,7 +189,7 @@ real heating_fraction(const Params &p, const HeatCell &c) {
       const real ltr = std::log(trat) / std::log(10.0);
       const real mbeta = 2.0 - 0.2 * ltr;
       const real c2 = (trat <= 1.0) ? 1.6 / trat : 1.2 / trat;
-      const real c3 = (trat <= 1.0) ? 18.0 + 5.0 * ltr : 18.0;
+      const real c3 = (trat <= 1.0) ? 18.0 + 5.0 * ltr : real(18.0);
       const real bp = std::pow(beta, mbeta);
       const real qrat = 0.92 * (c2 * c2 + bp) / (c3 * c3 + bp) * std::exp(-1.0 / beta) *
                         std::sqrt(mp / me * trat);

@@ -15,6 +15,43 @@
 
 namespace oracle {
 
+#ifdef ORACLE_COUNTING
+// The initial data are computed in double; in the counting build `real` is a
+// counting type, so these adapters convert at the calls (init is not counted).
+static void physical_coords(int kind, const double x[4], double y[4], double h) {
+  real xr[4] = {x[0], x[1], x[2], x[3]}, yr[4];
+  oracle::physical_coords(kind, xr, yr, h);
+  for (int n = 0; n < 4; ++n) y[n] = yr[n];
+}
+static void code_metric(int kind, const double x[4], double g[4][4], double h, double a) {
+  real xr[4] = {x[0], x[1], x[2], x[3]}, gr[4][4];
+  oracle::code_metric(kind, xr, gr, h, a);
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c) g[r][c] = gr[r][c];
+}
+static int invert4(const double m[4][4], double inv[4][4]) {
+  real mr[4][4], ir[4][4];
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c) { mr[r][c] = m[r][c]; ir[r][c] = inv[r][c]; }
+  const int ok = oracle::invert4(mr, ir);
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c) inv[r][c] = ir[r][c];
+  return ok;
+}
+static double determinant4(const double m[4][4]) {
+  real mr[4][4];
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c) mr[r][c] = m[r][c];
+  return oracle::determinant4(mr);
+}
+static void calc_state(const real prim[NPRIM], const double gcov[4][4], const double gcon[4][4], State &s) {
+  real gc[4][4], gn[4][4];
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c) { gc[r][c] = gcov[r][c]; gn[r][c] = gcon[r][c]; }
+  oracle::calc_state(prim, gc, gn, s);
+}
+#endif
+
 // splitmix64 finaliser of (seed, cell id) -> uniform double in [-1, 1).
 static double keyed_uniform(uint64_t seed, uint64_t id) {
   uint64_t z = id + seed * 0x9E3779B97F4A7C15ULL;
@@ -74,6 +111,21 @@ static void bl_to_code_ucon(const real x[4], real h, real a, const real ubl[4], 
     for (int j = 0; j < 4; j++) uc[i] += dxpdx[i][j] * tmp[j];
   }
 }
+
+#ifdef ORACLE_COUNTING
+static double solve_ut(const double g[4][4], double u1, double u2, double u3) {
+  real gr[4][4];
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c) gr[r][c] = g[r][c];
+  return solve_ut(gr, real(u1), real(u2), real(u3));
+}
+static void bl_to_code_ucon(const double x[4], double h, double a, const double ubl[4], double uc[4]) {
+  real xr[4], ur[4], cr[4];
+  for (int n = 0; n < 4; ++n) { xr[n] = x[n]; ur[n] = ubl[n]; }
+  bl_to_code_ucon(xr, real(h), real(a), ur, cr);
+  for (int n = 0; n < 4; ++n) uc[n] = cr[n];
+}
+#endif
 
 static void init_ot(Sim &S) {
   const double gam = S.p.gamma;

@@ -138,8 +138,8 @@ static int gnr_2d(real Bsq, real QdotBsq, real Qtsq, real Qdotn, real D, real x[
     // validate_x (scheme_2d.h:86-97)
     x[0] = std::fabs(x[0]);
     x[0] = (x[0] > W_TOO_BIG) ? x_old[0] : x[0];
-    x[1] = (x[1] < 0.) ? 0. : x[1];
-    x[1] = (x[1] > 1.) ? (1. - 1.e-15) : x[1];
+    x[1] = (x[1] < 0.) ? real(0.) : x[1];
+    x[1] = (x[1] > 1.) ? real(1. - 1.e-15) : x[1];
     if ((std::fabs(errx) <= NEWT_TOL) && (doing_extra == 0) && (EXTRA_NEWT_ITER > 0)) doing_extra = 1;
     if (doing_extra == 1) i_extra++;
     if (((std::fabs(errx) <= NEWT_TOL) && (doing_extra == 0)) || (i_extra > EXTRA_NEWT_ITER) ||
@@ -176,7 +176,7 @@ int c2p_2d(real U[NPRIM_RECV], real prim[NPRIM_RECV], real gamma, const real gco
   x[0] = std::fabs(W_last);
   {
     const real vs = std::fabs(vsq_of_w(v.Bsq, v.QdotBsq, v.Qtsq, W_last));
-    x[1] = (vs > 1.) ? (1.0 - 1.e-15) : vs;
+    x[1] = (vs > 1.) ? real(1.0 - 1.e-15) : vs;
   }
   int retval = gnr_2d(v.Bsq, v.QdotBsq, v.Qtsq, v.Qdotn, v.D, x, gamma, iters);
   const real W = x[0];
