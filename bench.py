@@ -56,9 +56,9 @@ TUNED = {"order": {"fix_cons": 1, "flux_x1": 1, "flux_x2": 0, "flux_x3": 1, "ct"
 # and WRITE_SIZE on this workload, each scaled by the same counter's reading
 # for a plain copy kernel of known size (profiles/counters_torus3d_256x128x128.md).
 MEASURED_TRAFFIC = {
-    "div_update_source": 2991504673, "flux_x1": 886524023, "flux_x2": 932537489,
-    "flux_x3": 987521579, "fix_cons": 1226207998, "recovery_heating": 1148362233,
-    "ct_emf": 353691596, "ct_faces": 459127929,
+    "div_update_source": 2314613241, "flux_x1": 881765554, "flux_x2": 931152461,
+    "flux_x3": 857041148, "fix_cons": 1222316222, "recovery_heating": 1121018821,
+    "ct_emf": 353581429, "ct_faces": 458266608,
 }
 
 
