@@ -65,6 +65,10 @@ MATRIX = {
     "torus2d_heat_nolimit": _torus2d(limit_kel=0),
     "torus2d_noheat": _torus2d(enable_heating=0),
     "torus3d_chainer_laxf_mc": _torus3d(recovery=0, riemann=0, limiter=0),
+    # 1-D mesh (two blocks along x1) and blocks of odd sizes whose face rows
+    # do not fill whole wavefronts
+    "ot_minkowski_1d": dict(configs.orszag_tang(64, (2, 1, 1)), nx=[64, 1, 1]),
+    "torus3d_odd_blocks": configs.torus3d((40, 24, 12), (2, 2, 1)),
     "torus3d_exchange_after_recovery": _torus3d(exchange_after=1),
 }
 
