@@ -111,6 +111,8 @@ int pangu_sim_set_c2p_guess(pangu_sim *s, pangu_mlp *m, double delta);
 int pangu_sim_set_tile_order(pangu_sim *s, int kernel_family, int order);
 /* pangu_set_split_source on the simulation's context (drops the graph). */
 int pangu_sim_set_split_source(pangu_sim *s, int split);
+/* pangu_set_fuse_ct on the simulation's context (drops the graph). */
+int pangu_sim_set_fuse_ct(pangu_sim *s, int fuse);
 /* pangu_set_share_reconstruction on the simulation's context (drops the graph). */
 int pangu_sim_set_share_reconstruction(pangu_sim *s, int share);
 /* One task on container c0 (c1 output of update/source), host beta*dt. */

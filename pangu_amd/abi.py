@@ -91,6 +91,7 @@ PROTOTYPES = {
     "pangu_get_tile_order": (C.c_int, [_vp, C.c_int]),
     "pangu_set_split_source": (C.c_int, [_vp, C.c_int]),
     "pangu_set_share_reconstruction": (C.c_int, [_vp, C.c_int]),
+    "pangu_set_fuse_ct": (C.c_int, [_vp, C.c_int]),
     "pangu_diagnostics": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), _dp, _vp]),
     "pangu_stage": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_double, _dp, C.c_double,
                               C.c_int, C.c_int, _vp]),

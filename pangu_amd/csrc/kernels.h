@@ -101,6 +101,7 @@ struct LaunchCtx {
   Items ghosts, bcs[3];
   int order[PANGU_KF_COUNT] = {};  // tile order per kernel family (PANGU_ORDER_*)
   int split_source = 0;            // fused stage: source term in a kernel of its own
+  int fuse_ct = 1;                 // fused stage: CT face fluxes formed in the update kernel
   int share_x1 = 0;                // X1 sweep reconstructs each cell once (k_flux_x1s)
   KernelClock *clock = nullptr;
 };
@@ -114,12 +115,17 @@ void set_kernel_clock(pangu_ctx *c, KernelClock *clock);
 int launch_fix_cons(const LaunchCtx &L, const KFields &F, uint8_t *const *mask, bool fix, bool cons,
                     hipStream_t s);
 int launch_fluxes(const LaunchCtx &L, const KFields &F, double *const *alfven, hipStream_t s);
-int launch_ct(const LaunchCtx &L, const KFields &F, double *const *emf, hipStream_t s);
+// faces = false computes the edge EMFs only (the fused stage then forms the
+// CT face fluxes inside the update kernel).
+int launch_ct(const LaunchCtx &L, const KFields &F, double *const *emf, bool faces, hipStream_t s);
 // mode: 1 divergence -> dudt, 2 update from dudt, 3 source only, 4 fused
 // divergence + update + source, 5 fused divergence + update without source
+// emf (modes 4 and 5, with a field): the constrained-transport face B
+// fluxes are formed from these edge EMFs inside the kernel instead of being
+// read from the flux arrays (k_ct_faces is then not needed); null reads them.
 int launch_div_update(const LaunchCtx &L, int mode, const KFields &F0, const KFields &Finit,
-                      const KFields &F1, double *const *dudt, const double *dt_dev, double beta,
-                      double bdt_host, hipStream_t s);
+                      const KFields &F1, double *const *dudt, double *const *emf,
+                      const double *dt_dev, double beta, double bdt_host, hipStream_t s);
 // rec: 0 chainer, 1 kastaun, 2 1dvsq, -1 none (heating only)
 int launch_recover(const LaunchCtx &L, const KFields &F, const SharedTabs &S, int rec, bool heat,
                    hipStream_t s);

@@ -742,6 +742,12 @@ int pangu_sim_set_split_source(pangu_sim *s, int split) {
   return 0;
 }
 
+int pangu_sim_set_fuse_ct(pangu_sim *s, int fuse) {
+  PCHK(s, pangu_set_fuse_ct(s->ctx, fuse));
+  drop_graph(s);
+  return 0;
+}
+
 const char *pangu_sim_kernel_name(pangu_sim *s, int n) {
   return (s && n >= 0 && n < (int)s->knames.size()) ? s->knames[n].c_str() : "";
 }

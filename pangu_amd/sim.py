@@ -87,6 +87,7 @@ def lib():
         L.pangu_sim_block_geom.argtypes = [vp, C.c_int, dp, ip]
         L.pangu_sim_set_tile_order.argtypes = [vp, C.c_int, C.c_int]
         L.pangu_sim_set_split_source.argtypes = [vp, C.c_int]
+        L.pangu_sim_set_fuse_ct.argtypes = [vp, C.c_int]
         L.pangu_sim_set_share_reconstruction.argtypes = [vp, C.c_int]
         L.pangu_sim_time_kernels.argtypes = [vp, C.c_int, dp, ip, C.c_int]
         L.pangu_sim_kernel_name.restype = C.c_char_p
@@ -251,6 +252,12 @@ class HipSim:
         """Fused stage: geometric source in a kernel of its own (1) or fused
         with the flux divergence and update (0).  Results are unchanged."""
         self._check(lib().pangu_sim_set_split_source(self._p, int(split)), "set_split_source")
+
+    def set_fuse_ct(self, fuse):
+        """Fused stage: constrained-transport face B fluxes formed inside the
+        update kernel (1, default) or written by their own kernel first (0).
+        Results are unchanged."""
+        self._check(lib().pangu_sim_set_fuse_ct(self._p, int(fuse)), "set_fuse_ct")
 
     def set_share_reconstruction(self, share):
         """X1 sweep: one reconstruction per cell shared between its two faces

@@ -153,16 +153,19 @@ def test_fix_recovery_patterns():
 
 @pytest.mark.gpu
 def test_launch_tuning_leaves_results_unchanged():
-    """Tile order and the split of the source kernel change only the launch
-    schedule: one graph cycle gives bit-identical fields either way."""
+    """Tile order, the split of the source kernel and forming the CT face
+    fluxes inside the update kernel change only the launch schedule: one
+    graph cycle gives bit-identical fields either way."""
     from pangu_amd.sim import HipSim, KERNEL_FAMILIES
     cfg = _torus3d()
     runs = []
-    for variant in ("default", "split", "column", "plane", "share"):
+    for variant in ("default", "split", "column", "plane", "share", "ct_faces", "split_ct_faces"):
         h = HipSim(cfg)
         h.init()
-        if variant == "split":
+        if variant in ("split", "split_ct_faces"):
             h.set_split_source(1)
+        if variant in ("ct_faces", "split_ct_faces"):
+            h.set_fuse_ct(0)
         if variant == "share":
             h.set_share_reconstruction(1)
         for fam in KERNEL_FAMILIES:
@@ -177,6 +180,27 @@ def test_launch_tuning_leaves_results_unchanged():
         assert np.array_equal(P, runs[0][1]), v
         assert np.array_equal(U, runs[0][2]), v
         assert np.array_equal(P1, runs[0][3]), v
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["ot_minkowski_exchange_after_recovery", "torus2d_kastaun",
+                                  "ot_minkowski_1d", "ot_cks3d_hll_ppm4_kastaun"])
+def test_fused_ct_is_bit_identical(name):
+    """The update kernel's own CT face fluxes (from the edge EMFs) give the
+    same bits as writing them with the face kernel first, in 1-D, 2-D and
+    3-D."""
+    from pangu_amd.sim import HipSim
+    cfg = MATRIX[name]
+    out = []
+    for fuse in (1, 0):
+        h = HipSim(cfg)
+        h.init()
+        h.set_fuse_ct(fuse)
+        h.step(3, "graph")
+        out.append((h.get("prim", 0), h.get("cons", 0), h.get("prim", 1)))
+        h.close()
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
 
 
 @pytest.mark.gpu
