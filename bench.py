@@ -35,13 +35,14 @@ FP64_VALU_PEAK_TFLOPS = 78.6
 # one-cell halo of faces that CT needs (<= 6 %) and the geometry tables
 # (x3-independent, amortised over the 32 planes of a block) are excluded:
 #   flux_x1/2/3        read P (10) + write flux (10) + alfven (1)            = 21
-#   div_update_source  read 3 fluxes (30) + U_init (10) + P (10) + write U (10) = 60
+#   div_update_source  read the 7 non-B fluxes of 3 directions (21) + 3 EMFs
+#                      (the CT face B fluxes are formed from them in the
+#                      kernel) + U_init (10) + P (10) + write U (10)           = 54
 #   recovery_heating   read U (10) + P (10) + write P (8: rho, u, u~, B, K_e)  = 28
 #   fix_cons           read P (10) + write P (7) + write U (10)              = 27
 #   ct_emf             read 6 B-fluxes + write 3 EMFs                        = 9
-#   ct_faces           read 3 EMFs + write 9 B-fluxes                        = 12
-WORDS_PER_ZONE = {"flux_x1": 21, "flux_x2": 21, "flux_x3": 21, "div_update_source": 60,
-                  "recovery_heating": 28, "fix_cons": 27, "ct_emf": 9, "ct_faces": 12}
+WORDS_PER_ZONE = {"flux_x1": 21, "flux_x2": 21, "flux_x3": 21, "div_update_source": 54,
+                  "recovery_heating": 28, "fix_cons": 27, "ct_emf": 9}
 
 # Launch parameters of this workload as pangu_sim_autotune chose them on an
 # MI355X (tile order per kernel family: 0 plane-major, 1 column-major).  The
@@ -56,9 +57,9 @@ TUNED = {"order": {"fix_cons": 1, "flux_x1": 1, "flux_x2": 0, "flux_x3": 1, "ct"
 # and WRITE_SIZE on this workload, each scaled by the same counter's reading
 # for a plain copy kernel of known size (profiles/counters_torus3d_256x128x128.md).
 MEASURED_TRAFFIC = {
-    "div_update_source": 2314613241, "flux_x1": 881765554, "flux_x2": 931152461,
-    "flux_x3": 857041148, "fix_cons": 1222316222, "recovery_heating": 1121018821,
-    "ct_emf": 353581429, "ct_faces": 458266608,
+    "div_update_source": 2063902361, "flux_x1": 881801660, "flux_x2": 931134937,
+    "flux_x3": 857023605, "fix_cons": 1222290595, "recovery_heating": 1121017255,
+    "ct_emf": 353569598,
 }
 
 
