@@ -62,18 +62,24 @@ typedef struct pangu_config {
   int excise;
   double r_excise, dexcise, pexcise;
   /* mesh topology for ghost exchange and physical boundaries */
-  int nb[3];     /* meshblocks per dim */
+  int nb[3];     /* meshblocks per dim of the root grid (refinement level 0) */
   int bc[3][2];  /* PANGU_BC_* for inner/outer faces of each dim */
 } pangu_config;
 
 /* One meshblock: Parthenon UniformCartesian coordinates,
- * Xc(idx) = xmin - ng*dx + (idx + 0.5)*dx, Xf(idx) = xmin - ng*dx + idx*dx. */
+ * Xc(idx) = xmin - ng*dx + (idx + 0.5)*dx, Xf(idx) = xmin - ng*dx + idx*dx.
+ * With static mesh refinement (Parthenon LogicalLocation), `level` counts
+ * refinements of the root grid and lx is the location among the
+ * nb[d] << level blocks of that level on active dims.  The blocks of a mesh
+ * must tile its domain, share one shape, and neighbour (faces, edges and
+ * corners) only blocks at most one level apart. */
 typedef struct pangu_block {
   int nx[3];      /* interior cells (1 on inactive dims) */
   int ng[3];      /* ghost layers (0 on inactive dims) */
   int lx[3];      /* logical location in the block grid */
   double xmin[3]; /* interior lower corner */
   double dx[3];   /* cell widths */
+  int level;      /* refinement level (0 = root grid) */
 } pangu_block;
 
 /* A stage container (Parthenon MeshData): per block, per component device
@@ -162,17 +168,29 @@ int pangu_fix_recovery(pangu_ctx *ctx, int md, void *stream);
 int pangu_estimate_timestep(pangu_ctx *ctx, int md, double *dt_out, void *stream);
 
 /* ---- Parthenon-side pieces the path needs on one GPU ---- */
-/* Same-level ghost exchange (face/edge/corner neighbours, periodic wrap) of
- * the fields read in ghost zones: primitives and flag
- * (AddBoundaryExchangeTasks, ideal_grmhd.cc:78). */
+/* Ghost exchange (face/edge/corner neighbours, periodic wrap) of the fields
+ * read in ghost zones: primitives and flag (AddBoundaryExchangeTasks,
+ * ideal_grmhd.cc:78).  Across refinement levels a ghost cell covered by a
+ * finer block takes the mean of the 2^dim cells it covers (the flag: their
+ * minimum); one covered by a coarser block takes the coarse value plus
+ * minmod-limited slopes times +-1/4 of a coarse cell per active dim (the
+ * flag: the coarse value). */
 int pangu_exchange_ghosts(pangu_ctx *ctx, int md, void *stream);
+/* Flux correction at refinement boundaries (parthenon::AddFluxCorrectionTasks,
+ * ideal_grmhd.cc:71-72): every face flux of a coarse block on a face shared
+ * with finer blocks becomes the mean of the 2^(dim-1) fine face fluxes that
+ * cover it, for all conserved components.  Runs after CalculateFluxes and
+ * ConstraintedTransport, before FluxDivergence; a no-op on a single-level
+ * mesh. */
+int pangu_flux_correction(pangu_ctx *ctx, int md, void *stream);
 /* Physical outflow / reflecting boundaries (ApplyBoundaryConditionsMD, ideal_grmhd.cc:104). */
 int pangu_apply_boundary_conditions(pangu_ctx *ctx, int md, void *stream);
 
 /* ---- fused stage (performance path; no reference counterpart) ----
  * Runs region 1 + region 2 of one RK stage (ideal_grmhd.cc:52-116) with
- * fused kernels: fix+cons, three flux sweeps, CT, flux-div+update+source,
- * exchange, recovery+heating, fix_recovery and boundaries.  dt_state is a
+ * fused kernels: fix+cons, three flux sweeps, CT, flux correction (on a
+ * refined mesh), flux-div+update+source, exchange, recovery+heating,
+ * fix_recovery and boundaries.  dt_state is a
  * device array {dt, time} per mesh: the stage reads each mesh's beta*dt
  * from it, and when last_stage != 0 it also estimates the timestep and
  * applies the dt control time += dt, dt = min(dt_factor*dt, estimate) in
@@ -210,7 +228,8 @@ int pangu_set_share_reconstruction(pangu_ctx *ctx, int share);
  * from the edge EMFs itself (the same arithmetic as the face overwrite, so
  * results are identical); the flux arrays then keep the Riemann B fluxes
  * after pangu_stage.  0 writes the CT face fluxes first, as the task-level
- * entry points always do. */
+ * entry points always do.  On a refined mesh the CT face fluxes are always
+ * written, because flux correction overwrites some of them. */
 int pangu_set_fuse_ct(pangu_ctx *ctx, int fuse);
 
 /* ---- run diagnostics (no reference counterpart) ----

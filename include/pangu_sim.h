@@ -19,6 +19,8 @@
 extern "C" {
 #endif
 
+#define PANGU_SIM_MAX_REF 8
+
 /* Run configuration: mesh, time, physics, problem.  Field meanings follow
  * PANGU's input file. */
 typedef struct pangu_sim_cfg {
@@ -47,6 +49,13 @@ typedef struct pangu_sim_cfg {
   double rin, rmax, kappa, perturbation, beta, aphi_cut;
   uint64_t seed;
   int nthreads;
+  /* static mesh refinement (<parthenon/static_refinementN>): region r asks
+   * for blocks of level ref_level[r] (1 .. 20) wherever they overlap
+   * [ref_min[r], ref_max[r]]; neighbours are refined so that adjacent blocks
+   * differ by at most one level.  nref = 0: a uniform mesh. */
+  int nref;
+  int ref_level[PANGU_SIM_MAX_REF];
+  double ref_min[PANGU_SIM_MAX_REF][3], ref_max[PANGU_SIM_MAX_REF][3];
 } pangu_sim_cfg;
 
 enum { PANGU_SIM_TASKS = 0, PANGU_SIM_FUSED = 1, PANGU_SIM_GRAPH = 2 };
@@ -123,6 +132,8 @@ int pangu_sim_set(pangu_sim *s, int field, int container, const void *in);
 double pangu_sim_time(pangu_sim *s);
 double pangu_sim_dt(pangu_sim *s);
 int pangu_sim_block_geom(pangu_sim *s, int b, double *xg_dx_out, int *n_out);
+/* Refinement level and logical location of block b: out = {level, lx1, lx2, lx3}. */
+int pangu_sim_block_loc(pangu_sim *s, int b, int *out);
 const char *pangu_sim_last_error(pangu_sim *s);
 /* Times a plain copy kernel of n doubles (mean ms over reps); used to
  * calibrate hardware byte counters against a kernel of known traffic. */

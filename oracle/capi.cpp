@@ -44,6 +44,7 @@ int oracle_sim_task(void *sim, int task, int c0, int c1, double beta_dt, double 
     case ORT_FIX_RECOVERY: s->fix_recovery(c1); break;
     case ORT_EXCHANGE: s->exchange(c1); break;
     case ORT_BC: s->apply_bc(c1); break;
+    case ORT_FLUX_CORRECTION: s->flux_correction(); break;
     case ORT_TIMESTEP:
       if (out) *out = s->estimate_dt();
       break;
@@ -109,6 +110,16 @@ int oracle_sim_diagnostics(void *sim, int c, int shell, double *out, double *mag
   return 0;
 }
 
+int oracle_sim_nblocks(void *sim) { return ((Sim *)sim)->nblk; }
+
+int oracle_sim_block_loc(void *sim, int b, int *out) {
+  Sim *s = (Sim *)sim;
+  if (b < 0 || b >= s->nblk) return -1;
+  out[0] = s->blk[b].level;
+  for (int d = 0; d < 3; ++d) out[1 + d] = s->blk[b].lx[d];
+  return 0;
+}
+
 int oracle_sim_block_geom(void *sim, int b, double *xd, int *n) {
   Sim *s = (Sim *)sim;
   if (b < 0 || b >= s->nblk) return -1;
@@ -126,7 +137,7 @@ int oracle_sim_block_geom(void *sim, int b, double *xd, int *n) {
 void oracle_plant(int which) { g_plant = which; }
 
 #ifdef ORACLE_COUNTING
-static_assert((int)OP_N == (int)ORACLE_NOPS && (int)ORACLE_NTASKS == 13, "operation kinds and tasks");
+static_assert((int)OP_N == (int)ORACLE_NOPS && (int)ORACLE_NTASKS == 14, "operation kinds and tasks");
 int oracle_op_counts(uint64_t *out) {
   for (int t = 0; t < ORACLE_NTASKS; ++t)
     for (int q = 0; q < ORACLE_NOPS; ++q) out[t * ORACLE_NOPS + q] = g_task_ops[t][q];

@@ -28,7 +28,7 @@ enum OpKind {
   OP_N
 };
 extern uint64_t g_ops[OP_N];
-extern uint64_t g_task_ops[13][OP_N];  // per ORT_* task (oracle_api.h)
+extern uint64_t g_task_ops[14][OP_N];  // per ORT_* task (oracle_api.h)
 
 struct Real {
   double v;

@@ -18,11 +18,15 @@ namespace oracle {
 // ------------------------------------------------------------------ set-up
 // Uniform mesh generator (Parthenon/Athena++ ComputeMeshGeneratorX with a
 // symmetric interval and UniformMeshGeneratorX; mesh edges are exact).
-static double mesh_gen(int idx, int n, double xmin, double xmax) {
+static double mesh_gen(int64_t idx, int64_t n, double xmin, double xmax) {
   const long noff = idx - n / 2;
   const long noffc = idx - (n + 1) / 2;
   const double x = (double)(noff + noffc) / (2.0 * n);
   return 0.5 * (xmin + xmax) + (x * xmax - x * xmin);
+}
+
+double mesh_edge(int64_t idx, int64_t n, double xmin, double xmax) {
+  return idx == 0 ? xmin : (idx == n ? xmax : mesh_gen(idx, n, xmin, xmax));
 }
 
 void Sim::setup(const oracle_cfg &c) {
@@ -81,24 +85,30 @@ void Sim::setup(const oracle_cfg &c) {
   }
   nc = np;
 
-  nblk = c.nb[0] * c.nb[1] * c.nb[2];
+  // blocks: the root grid, refined by the static refinement regions
+  // (refine.cpp); without regions, root blocks in x1-fastest order
+  std::vector<std::array<int64_t, 4>> leaves;
+  make_leaves(leaves);
+  nblk = (int)leaves.size();
   blk.resize(nblk);
+  leaf_of.clear();
+  refined = false;
   int64_t off = 0;
   for (int b = 0; b < nblk; ++b) {
     Block &B = blk[b];
-    B.lx[0] = b % c.nb[0];
-    B.lx[1] = (b / c.nb[0]) % c.nb[1];
-    B.lx[2] = b / (c.nb[0] * c.nb[1]);
+    B.level = (int)leaves[b][0];
+    if (B.level > 0) refined = true;
+    leaf_of[leaves[b]] = b;
     for (int d = 0; d < 3; ++d) {
-      const int nbd = c.nb[d];
-      B.nx[d] = c.nx[d] / nbd;
+      B.lx[d] = (int)leaves[b][1 + d];
+      const int64_t nbd = (c.nx[d] > 1) ? ((int64_t)c.nb[d] << B.level) : 1;
+      B.nx[d] = c.nx[d] / c.nb[d];
       B.ng[d] = (c.nx[d] > 1) ? c.ng : 0;
       B.n[d] = B.nx[d] + 2 * B.ng[d];
       B.s[d] = B.ng[d];
       B.e[d] = B.ng[d] + B.nx[d] - 1;
-      B.xmin[d] = (B.lx[d] == 0) ? c.xmin[d] : mesh_gen(B.lx[d], nbd, c.xmin[d], c.xmax[d]);
-      B.xmax[d] =
-          (B.lx[d] == nbd - 1) ? c.xmax[d] : mesh_gen(B.lx[d] + 1, nbd, c.xmin[d], c.xmax[d]);
+      B.xmin[d] = mesh_edge(B.lx[d], nbd, c.xmin[d], c.xmax[d]);
+      B.xmax[d] = mesh_edge(B.lx[d] + 1, nbd, c.xmin[d], c.xmax[d]);
       B.dx[d] = (B.xmax[d] - B.xmin[d]) / B.nx[d];
       B.xg[d] = B.xmin[d] - B.ng[d] * B.dx[d];
       B.gofs[d] = B.lx[d] * B.nx[d];
@@ -730,6 +740,10 @@ void Sim::fix_recovery(int c) {
 // Same-level ghost exchange of the fields read in ghost zones (primitives
 // and flag) from face, edge and corner neighbours; periodic dims wrap.
 void Sim::exchange(int c) {
+  if (refined) {
+    exchange_refined(c);
+    return;
+  }
   const int nbd[3] = {cfg.nb[0], cfg.nb[1], cfg.nb[2]};
   for (int b = 0; b < nblk; ++b) {
     const Block &B = blk[b];
@@ -778,7 +792,7 @@ void Sim::apply_bc(int c) {
         const int kind = cfg.bc[d][side];
         if (kind == BC_PERIODIC) continue;
         if (side == 0 && B.lx[d] != 0) continue;
-        if (side == 1 && B.lx[d] != cfg.nb[d] - 1) continue;
+        if (side == 1 && (int64_t)B.lx[d] != ((int64_t)cfg.nb[d] << B.level) - 1) continue;
         const int g0 = (side == 0) ? 0 : B.e[d] + 1;
         const int g1 = (side == 0) ? B.s[d] - 1 : B.n[d] - 1;
         for (int k = 0; k < B.n[2]; ++k)
@@ -862,6 +876,7 @@ void Sim::run_stage(int stage) {
   COUNTED(ORT_CONSERVATIVE, calc_conservative(c0));
   COUNTED(ORT_FLUXES, fluxes(c0));
   if (p.enable_B) COUNTED(ORT_CT, ct());
+  if (refined) COUNTED(ORT_FLUX_CORRECTION, flux_correction());
   COUNTED(ORT_FLUX_DIV, flux_div());
   COUNTED(ORT_UPDATE, update(bdt, c1));
   if (cfg.exchange_after == 0) COUNTED(ORT_EXCHANGE, exchange(c1));

@@ -101,7 +101,9 @@ enum Plant {
   PLANT_PPM_TERM = 3,     // PPM4: drop the outer stencil term
   PLANT_KASTAUN_QBAR = 4, // Kastaun: wrong sign of the b^2 term in qbar
   PLANT_CT_SIGN = 5,      // CT: wrong sign of the X1-face B3 flux
-  PLANT_EMT_PRESSURE = 6  // T^mu_nu: drop the magnetic pressure
+  PLANT_EMT_PRESSURE = 6, // T^mu_nu: drop the magnetic pressure
+  PLANT_NO_FLUX_CORRECTION = 7,  // refined mesh: skip flux correction
+  PLANT_PROLONG_SLOPE = 8        // prolongation: +-1/2 instead of +-1/4 of the slope
 };
 
 // Kokkos::max / Kokkos::min semantics (ternary on operator<).

@@ -7,6 +7,8 @@
 extern "C" {
 #endif
 
+#define ORACLE_MAX_REF 8
+
 /* Whole-run configuration.  Field meanings follow PANGU's input file
  * (<core>, <metric>, <electron>, <parthenon/mesh>, <parthenon/time>). */
 typedef struct oracle_cfg {
@@ -44,6 +46,12 @@ typedef struct oracle_cfg {
   double rin, rmax, kappa, perturbation, beta, aphi_cut;
   uint64_t seed;
   int nthreads;        /* OpenMP threads (0 = default) */
+  /* static mesh refinement (<parthenon/static_refinementN>): region r asks
+   * for blocks of level ref_level[r] wherever they overlap
+   * [ref_min[r], ref_max[r]]; nref = 0: a uniform mesh */
+  int nref;
+  int ref_level[ORACLE_MAX_REF];
+  double ref_min[ORACLE_MAX_REF][3], ref_max[ORACLE_MAX_REF][3];
 } oracle_cfg;
 
 /* fields for oracle_sim_get / oracle_sim_set */
@@ -57,7 +65,7 @@ enum {
   ORT_FIX_PRIMITIVE = 0, ORT_CONSERVATIVE = 1, ORT_FLUXES = 2, ORT_CT = 3,
   ORT_FLUX_DIV = 4, ORT_UPDATE = 5, ORT_SOURCE = 6, ORT_RECOVERY = 7,
   ORT_HEATING = 8, ORT_FIX_RECOVERY = 9, ORT_EXCHANGE = 10, ORT_BC = 11,
-  ORT_TIMESTEP = 12
+  ORT_TIMESTEP = 12, ORT_FLUX_CORRECTION = 13
 };
 
 void *oracle_sim_create(const oracle_cfg *cfg);
@@ -74,6 +82,9 @@ double oracle_sim_time(void *sim);
 double oracle_sim_dt(void *sim);
 int oracle_sim_ncycle(void *sim);
 int oracle_sim_block_geom(void *sim, int b, double *xg_dx_out /* 6 */, int *n_out /* 9 */);
+int oracle_sim_nblocks(void *sim);
+/* {level, lx1, lx2, lx3} of block b */
+int oracle_sim_block_loc(void *sim, int b, int *out /* 4 */);
 
 /* Diagnostics of container c (diagnostics.cpp): shell integrals at the
  * interior x1 cell index `shell` of the mesh, volume integrals, div B.
@@ -90,7 +101,7 @@ int oracle_sim_diagnostics(void *sim, int container, int shell, double *out /* O
  * the operations of kind op (add/sub, mul, div, sqrt, compare, fmin/fmax,
  * fabs, exp, log, pow, sin/cos) since the last reset.  Returns -1 in the
  * ordinary build. */
-enum { ORACLE_NOPS = 11, ORACLE_NTASKS = 13 };
+enum { ORACLE_NOPS = 11, ORACLE_NTASKS = 14 };
 int oracle_op_counts(uint64_t *out /* ORACLE_NTASKS * ORACLE_NOPS */);
 void oracle_op_reset(void);
 

@@ -13,6 +13,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
 
 
+MAX_REF = 8  # ORACLE_MAX_REF
+
+
 class OracleCfg(C.Structure):
     _fields_ = [
         ("nx", C.c_int * 3), ("nb", C.c_int * 3), ("ng", C.c_int),
@@ -37,6 +40,8 @@ class OracleCfg(C.Structure):
         ("rin", C.c_double), ("rmax", C.c_double), ("kappa", C.c_double),
         ("perturbation", C.c_double), ("beta", C.c_double), ("aphi_cut", C.c_double),
         ("seed", C.c_uint64), ("nthreads", C.c_int),
+        ("nref", C.c_int), ("ref_level", C.c_int * MAX_REF),
+        ("ref_min", (C.c_double * 3) * MAX_REF), ("ref_max", (C.c_double * 3) * MAX_REF),
     ]
 
 
@@ -71,6 +76,8 @@ def lib():
         L.oracle_sim_dt.argtypes = [vp]
         L.oracle_sim_ncycle.argtypes = [vp]
         L.oracle_sim_block_geom.argtypes = [vp, C.c_int, dp, ip]
+        L.oracle_sim_nblocks.argtypes = [vp]
+        L.oracle_sim_block_loc.argtypes = [vp, C.c_int, ip]
         L.oracle_sim_diagnostics.argtypes = [vp, C.c_int, C.c_int, dp, dp]
         L.oracle_metric_at.argtypes = [C.c_int, dp, C.c_double, C.c_double, dp, dp, dp]
         L.oracle_christoffel.argtypes = [C.c_int, dp, C.c_double, C.c_double, dp]
@@ -101,8 +108,19 @@ def make_cfg(d):
             arr = getattr(c, name)
             for i in range(3):
                 arr[i] = v[i]
+        elif name in ("nref", "ref_level", "ref_min", "ref_max"):
+            raise ValueError("give refinement regions as cfg['refine']")
         else:
             setattr(c, name, v)
+    regions = d.get("refine", ())
+    if len(regions) > MAX_REF:
+        raise ValueError(f"at most {MAX_REF} refinement regions")
+    c.nref = len(regions)
+    for r, (level, lo, hi) in enumerate(regions):
+        c.ref_level[r] = int(level)
+        for i in range(3):
+            c.ref_min[r][i] = float(lo[i])
+            c.ref_max[r][i] = float(hi[i])
     return c
 
 
@@ -113,7 +131,8 @@ def _dp(a):
 FIELDS = dict(prim=0, cons=1, flux1=2, flux2=3, flux3=4, alfven=5, emf=6, flag=7, mask=8,
               c2p_status=9, c2p_iters=10, flag_recovery=11, dudt=12)
 TASKS = dict(fix_primitive=0, conservative=1, fluxes=2, ct=3, flux_div=4, update=5, source=6,
-             recovery=7, heating=8, fix_recovery=9, exchange=10, bc=11, timestep=12)
+             recovery=7, heating=8, fix_recovery=9, exchange=10, bc=11, timestep=12,
+             flux_correction=13)
 
 
 DIAG_NAMES = ("mdot", "edot", "jdot", "phi", "mass", "energy", "divb_max", "ub_max")
@@ -121,7 +140,7 @@ DIAG_NAMES = ("mdot", "edot", "jdot", "phi", "mass", "energy", "divb_max", "ub_m
 # Operation counting (liboracle_count.so, counting.h): task rows follow the
 # ORT_* order, operation columns the OpKind order.
 TASK_NAMES = ("fix_primitive", "conservative", "fluxes", "ct", "flux_div", "update", "source",
-              "recovery", "heating", "fix_recovery", "exchange", "bc", "timestep")
+              "recovery", "heating", "fix_recovery", "exchange", "bc", "timestep", "flux_correction")
 OP_NAMES = ("add", "mul", "div", "sqrt", "cmp", "minmax", "fabs", "exp", "log", "pow", "sincos")
 _clib = None
 
@@ -189,7 +208,7 @@ class OracleSim:
         self.cfg = cfg_dict
         self.nx = list(cfg_dict["nx"])
         self.nb = list(cfg_dict["nb"])
-        self.nblocks = self.nb[0] * self.nb[1] * self.nb[2]
+        self.nblocks = lib().oracle_sim_nblocks(self._p)
         xd = np.zeros(6)
         n = np.zeros(9, dtype=np.int32)
         lib().oracle_sim_block_geom(self._p, 0, _dp(xd), n.ctypes.data_as(C.POINTER(C.c_int)))
@@ -201,6 +220,12 @@ class OracleSim:
         if getattr(self, "_p", None):
             lib().oracle_sim_destroy(self._p)
             self._p = None
+
+    def block_loc(self, b):
+        """(level, lx1, lx2, lx3) of block b."""
+        out = np.zeros(4, dtype=np.int32)
+        assert lib().oracle_sim_block_loc(self._p, b, out.ctypes.data_as(C.POINTER(C.c_int))) == 0
+        return tuple(int(x) for x in out)
 
     def block_geom(self, b):
         xd = np.zeros(6)
@@ -268,7 +293,7 @@ class OracleSim:
 
 
 PLANTS = dict(none=0, conn_index=1, shift_sign=2, ppm_term=3, kastaun_qbar=4, ct_sign=5,
-              emt_pressure=6)
+              emt_pressure=6, no_flux_correction=7, prolong_slope=8)
 
 
 def plant(name):

@@ -1,5 +1,7 @@
 // Oracle mesh/driver state.  TEST INFRASTRUCTURE ONLY.
 #pragma once
+#include <array>
+#include <map>
 #include <vector>
 
 #include "oracle.h"
@@ -8,7 +10,7 @@
 namespace oracle {
 
 struct Block {
-  int nx[3], ng[3], n[3], s[3], e[3], lx[3], gofs[3];
+  int nx[3], ng[3], n[3], s[3], e[3], lx[3], gofs[3], level;
   double xmin[3], xmax[3], dx[3], xg[3];
   int64_t cells, off;
   int64_t idx(int k, int j, int i) const { return ((int64_t)k * n[1] + j) * n[0] + i; }
@@ -18,6 +20,10 @@ struct Block {
 // with xg the ghost-inclusive origin.
 static inline double Xc(const Block &B, int d, int idx) { return B.xg[d] + (idx + 0.5) * B.dx[d]; }
 static inline double Xf(const Block &B, int d, int idx) { return B.xg[d] + idx * B.dx[d]; }
+
+// Edge idx of n equal cells over [xmin, xmax] (Parthenon's uniform mesh
+// generator; the mesh edges are exact).
+double mesh_edge(int64_t idx, int64_t n, double xmin, double xmax);
 
 struct Geo {
   real gcov[4][4], gcon[4][4], gdet;
@@ -55,6 +61,14 @@ struct Sim {
   double *Ec(int b, int d) { return emf.data() + blk[b].off * 3 + (int64_t)d * blk[b].cells; }
 
   void setup(const oracle_cfg &c);
+  // static mesh refinement (refine.cpp)
+  bool refined = false;
+  std::map<std::array<int64_t, 4>, int> leaf_of;  // (level, lx1, lx2, lx3) -> block
+  void make_leaves(std::vector<std::array<int64_t, 4>> &out) const;
+  int leaf_at(int level, const int64_t g[3]) const;
+  bool wrap_cell(int d, int level, int64_t &g) const;
+  double level_value(int c, int q, int level, const int64_t g[3]) const;
+  void exchange_refined(int c);
   void build_geometry_memo();
   int metric(int b, int loc, int k, int j, int i, real gcov[4][4], real gcon[4][4], real *gdet) const;
   void connection(int b, int k, int j, int i, real conn[4][4][4]) const;
@@ -72,6 +86,7 @@ struct Sim {
   void heating(int c);
   void fix_recovery(int c);
   void exchange(int c);
+  void flux_correction();
   void apply_bc(int c);
   double estimate_dt() const;
   void diagnostics(int c, int shell, double out[ODIAG_N], double mag[ODIAG_N]) const;

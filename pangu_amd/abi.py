@@ -32,7 +32,7 @@ class PanguConfig(C.Structure):
 
 class PanguBlock(C.Structure):
     _fields_ = [("nx", C.c_int * 3), ("ng", C.c_int * 3), ("lx", C.c_int * 3),
-                ("xmin", C.c_double * 3), ("dx", C.c_double * 3)]
+                ("xmin", C.c_double * 3), ("dx", C.c_double * 3), ("level", C.c_int)]
 
 
 _PPD = C.POINTER(C.POINTER(C.c_double))
@@ -86,6 +86,7 @@ PROTOTYPES = {
     "pangu_fix_recovery": (C.c_int, [_vp, C.c_int, _vp]),
     "pangu_estimate_timestep": (C.c_int, [_vp, C.c_int, _dp, _vp]),
     "pangu_exchange_ghosts": (C.c_int, [_vp, C.c_int, _vp]),
+    "pangu_flux_correction": (C.c_int, [_vp, C.c_int, _vp]),
     "pangu_apply_boundary_conditions": (C.c_int, [_vp, C.c_int, _vp]),
     "pangu_set_tile_order": (C.c_int, [_vp, C.c_int, C.c_int]),
     "pangu_get_tile_order": (C.c_int, [_vp, C.c_int]),

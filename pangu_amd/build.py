@@ -19,8 +19,8 @@ PHDF_LIB = os.path.join(HERE, "libpangu_phdf.so")
 HDF5_ROOT = os.environ.get("PANGU_HDF5_ROOT", "/opt/conda")
 ARCH = os.environ.get("PANGU_OFFLOAD_ARCH", "gfx950")
 
-HOST_SRC = ["geometry.cpp", "problems.cpp"]
-HIP_SRC = ["kernels.hip", "api.hip", "sim.hip", "mlp.hip", "diag.hip"]
+HOST_SRC = ["geometry.cpp", "problems.cpp", "refine_host.cpp"]
+HIP_SRC = ["kernels.hip", "api.hip", "sim.hip", "mlp.hip", "diag.hip", "refine.hip"]
 
 GXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-fno-fast-math",
              "-Wall", "-Wno-unused-variable"]

@@ -42,6 +42,10 @@ struct pangu_ctx {
   DevGeo *dgeo = nullptr;
   std::vector<double *> geo_bufs;
   int *dnbr = nullptr;
+  // static mesh refinement: gather tables (L.ml points into these)
+  XFill *dxfill = nullptr;
+  XProl *dxprol = nullptr;
+  XFlux *dxflux = nullptr;
   std::vector<DevContainer> conts;
   // shared tables
   double **dalfven = nullptr, **demf = nullptr, **dflag = nullptr, **dflag_rec = nullptr;
@@ -112,7 +116,12 @@ static int validate(const pangu_config *cfg, const pangu_block *blocks, int nb, 
         msg = "all meshblocks of a mesh must share one shape";
         return 0;
       }
-      if (blocks[b].nx[d] < 1 || (blocks[b].nx[d] > 1 && blocks[b].lx[d] >= cfg->nb[d]) ||
+      if (blocks[b].level < 0 || blocks[b].level > 20) {
+        msg = "refinement level out of range (0..20)";
+        return 0;
+      }
+      if (blocks[b].nx[d] < 1 ||
+          (blocks[b].nx[d] > 1 && (int64_t)blocks[b].lx[d] >= ((int64_t)cfg->nb[d] << blocks[b].level)) ||
           blocks[b].lx[d] < 0) {
         msg = "block extent or logical location out of range";
         return 0;
@@ -188,12 +197,13 @@ static int build_geometry(pangu_ctx *c, const pangu_block *blocks) {
       D.e[d] = H.e[d];
       D.lx[d] = H.lx[d];
       D.dx[d] = H.dx[d];
-      const bool lo_edge = H.lx[d] == 0, hi_edge = H.lx[d] == cfg.nb[d] - 1;
+      const bool lo_edge = H.lx[d] == 0, hi_edge = (int64_t)H.lx[d] == ((int64_t)cfg.nb[d] << H.level) - 1;
       D.bcf[d][0] = (H.n[d] > 1 && lo_edge && cfg.bc[d][0] != PANGU_BC_PERIODIC) ? cfg.bc[d][0] : -1;
       D.bcf[d][1] = (H.n[d] > 1 && hi_edge && cfg.bc[d][1] != PANGU_BC_PERIODIC) ? cfg.bc[d][1] : -1;
     }
     D.ncell = H.cells;
     D.mesh = c->bmesh[b];
+    D.level = H.level;
     std::vector<double> key = {(double)cfg.metric, cfg.a, cfg.h, H.xg[0], H.dx[0], (double)H.n[0],
                                H.xg[1], H.dx[1], (double)H.n[1], (double)H.n[2]};
     if (has3d) {
@@ -304,6 +314,33 @@ static int build_neighbours(pangu_ctx *c) {
         }
   }
   return upload(c, &c->dnbr, nbr.data(), nbr.size());
+}
+
+// Static mesh refinement: checks that the blocks tile each mesh and uploads
+// the gather tables of the cross-level exchange and of flux correction.
+static int build_multilevel(pangu_ctx *c) {
+  if (!refined_mesh(c->hb)) return 0;
+  std::string why = check_tiling(c->hb, c->bmesh, c->mcfg);
+  if (!why.empty()) return fail(c, PANGU_EINVAL, why);
+  std::vector<XFill> fill;
+  std::vector<XProl> prol;
+  std::vector<XFlux> flux;
+  why = build_refinement_tables(c->hb, c->bmesh, c->mcfg, fill, prol, flux);
+  if (!why.empty()) return fail(c, PANGU_EINVAL, why);
+  if (fill.size() > (size_t)0x7fffff00 || prol.size() > (size_t)0x7fffff00)
+    return fail(c, PANGU_EINVAL, "refinement tables exceed 2^31 entries");
+  if (upload(c, &c->dxfill, fill.data(), fill.size()) || upload(c, &c->dxprol, prol.data(), prol.size()) ||
+      upload(c, &c->dxflux, flux.data(), flux.size()))
+    return PANGU_EDEVICE;
+  MultiLevel &M = c->L.ml;
+  M.on = 1;
+  M.fill = c->dxfill;
+  M.prol = c->dxprol;
+  M.flux = c->dxflux;
+  M.nfill = (int)fill.size();
+  M.nprol = (int)prol.size();
+  M.nflux = (int)flux.size();
+  return 0;
 }
 
 // Uploads a tile -> block table and the per-block first tile (nblocks + 1).
@@ -531,6 +568,7 @@ int pangu_create_batch(const pangu_config *cfgs, int nmesh, const pangu_block *b
   for (int b = 0; b < nblocks; ++b) c->hb.push_back(make_host_block(blocks[b]));
   int r = build_geometry(c, blocks);
   if (!r) r = build_neighbours(c);
+  if (!r) r = build_multilevel(c);
   if (!r) r = build_tables(c);
   if (!r) r = upload(c, &c->dred, std::vector<double>((size_t)c->nplanes * 3, 0.0).data(),
                      (size_t)c->nplanes * 3);
@@ -603,7 +641,7 @@ int pangu_destroy(pangu_ctx *c) {
   void *ptrs[] = {c->dblk,    c->dgeo,  c->dnbr,     c->dalfven, c->demf,  c->dflag,
                   c->dflag_rec, c->dmask, c->dstatus, c->diters,  c->dred,  c->dcluster,
                   c->dest,    c->dguess, c->dguess_tab, c->ddiag_part, c->ddiag_mtile,
-                  c->ddiag_shell};
+                  c->ddiag_shell, c->dxfill, c->dxprol, c->dxflux};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   delete c;
@@ -836,9 +874,19 @@ int pangu_estimate_timestep(pangu_ctx *c, int md, double *dt_out, void *stream) 
   return PANGU_OK;
 }
 
+static int exchange(pangu_ctx *c, const KFields &f, hipStream_t s) {
+  return c->L.ml.on ? launch_exchange_ml(c->L, f, c->dflag, s) : launch_exchange(c->L, f, c->dflag, c->dnbr, s);
+}
+
 int pangu_exchange_ghosts(pangu_ctx *c, int md, void *stream) {
   if (int r = check_md(c, md)) return r;
-  LCHK(c, launch_exchange(c->L, c->conts[md].kf(), c->dflag, c->dnbr, STREAM(stream)));
+  LCHK(c, exchange(c, c->conts[md].kf(), STREAM(stream)));
+  return PANGU_OK;
+}
+
+int pangu_flux_correction(pangu_ctx *c, int md, void *stream) {
+  if (int r = check_md(c, md)) return r;
+  LCHK(c, launch_flux_correction(c->L, c->conts[md].kf(), STREAM(stream)));
   return PANGU_OK;
 }
 
@@ -860,22 +908,24 @@ int pangu_stage(pangu_ctx *c, int md_init, int md0, int md1, double beta, double
   LCHK(c, launch_fluxes(c->L, f0, c->dalfven, s));
   // With fuse_ct the update kernel forms the CT face B fluxes from the edge
   // EMFs itself, so the flux arrays keep the Riemann B fluxes after a fused
-  // stage (nothing after the update reads them).
-  const bool fuse_ct = c->cfg.enable_B && c->L.fuse_ct;
+  // stage (nothing after the update reads them).  Flux correction on a
+  // refined mesh overwrites face fluxes, so there they are written first.
+  const bool fuse_ct = c->cfg.enable_B && c->L.fuse_ct && !c->L.ml.on;
   double *const *emf = fuse_ct ? c->demf : nullptr;
   if (c->cfg.enable_B) LCHK(c, launch_ct(c->L, f0, c->demf, !fuse_ct, s));
+  if (c->L.ml.on) LCHK(c, launch_flux_correction(c->L, f0, s));
   if (c->L.split_source && c->cfg.metric != PANGU_METRIC_MINKOWSKI) {
     LCHK(c, launch_div_update(c->L, 5, f0, fi, f1, nullptr, emf, dt_state, beta, 0.0, s));
     LCHK(c, launch_div_update(c->L, 3, f0, fi, f1, nullptr, nullptr, dt_state, beta, 0.0, s));
   } else {
     LCHK(c, launch_div_update(c->L, 4, f0, fi, f1, nullptr, emf, dt_state, beta, 0.0, s));
   }
-  if (exchange_after == 0) LCHK(c, launch_exchange(c->L, f1, c->dflag, c->dnbr, s));
+  if (exchange_after == 0) LCHK(c, exchange(c, f1, s));
   if (int r = launch_guess(c, md1, s)) return r;
   LCHK(c, launch_recover(c->L, f1, shared_tabs(c), c->cfg.recovery, c->cfg.enable_heating != 0, s));
   if (c->cfg.recovery != PANGU_RECOVERY_ROBUSTER)
     LCHK(c, launch_fix_recovery(c->L, f1, c->dflag, c->dcluster, s));
-  if (exchange_after != 0) LCHK(c, launch_exchange(c->L, f1, c->dflag, c->dnbr, s));
+  if (exchange_after != 0) LCHK(c, exchange(c, f1, s));
   LCHK(c, launch_bc(c->L, f1, c->dflag, s));
   if (last_stage) LCHK(c, launch_dt(c->L, c->dalfven, c->dred, dt_state, dt_factor, nullptr, s));
   return PANGU_OK;

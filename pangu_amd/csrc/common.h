@@ -26,6 +26,25 @@ __device__ __forceinline__ gptr<T> glb(T *p) {
 }
 #endif
 
+// Static mesh refinement: gather tables built on the host from the block
+// tree (refine_host.cpp).  Cell indices are within a block's ghosted array.
+struct XFill {   // a ghost cell filled from another block's interior
+  int db, dst;   // destination block and cell
+  int sb, src;   // source block; its cell (copy) or the lowest of the 2^dim fine cells (mean)
+  int mean;      // 0 copy, 1 mean of the 2^dim fine cells a coarse ghost cell covers
+};
+struct XProl {   // a coarse cell prolongated into the fine ghost cells it covers
+  int b, base;   // fine block and the lowest fine cell the coarse cell covers
+  int mask;      // bit c0 + 2 c1 + 4 c2 set: child (c0, c1, c2) is a ghost cell of b
+  int sb[7];     // coarse values centre, -x1, +x1, -x2, +x2, -x3, +x3: a coarse block,
+  int sc[7];     // its cell; or sb = -1: the mean of b's own 2^dim fine cells from sc
+};
+struct XFlux {   // a coarse face flux replaced by the mean of the fine ones covering it
+  int cb, cface; // coarse block, its face (flux index along d)
+  int fb, fface; // fine block, the lowest of the 2^(dim-1) fine faces
+  int d;
+};
+
 // Slot order of a full state vector (variable_mnemonics.h:7-10).
 enum Slot { RHO = 0, ENY, UX1, UX2, UX3, BX1, BX2, BX3, ENT, KEL, NPRIM };
 
@@ -65,13 +84,14 @@ struct DevBlock {
   int n[3];     // ghosted extent (1 on inactive dims)
   int s[3];     // interior start
   int e[3];     // interior end (inclusive)
-  int lx[3];    // logical location in its mesh
+  int lx[3];    // logical location in its mesh, at its refinement level
   double dx[3];
   int64_t ncell;        // n0*n1*n2
   int geo;              // geometry set index
   int mesh;             // mesh the block belongs to
   int bcf[3][2];        // physical boundary kind (PANGU_BC_*) of each face, -1 if exchanged
-  int pad[2];
+  int level;            // refinement level
+  int pad;
 };
 
 // Geometry set (device copy): SoA tables, comp-major, (k?) j, i fastest.

@@ -3,6 +3,7 @@
 #pragma once
 #include <stdint.h>
 
+#include <string>
 #include <vector>
 
 #include "../../include/pangu_hip.h"
@@ -10,7 +11,7 @@
 namespace pangu {
 // Host block geometry (Parthenon UniformCartesian).
 struct HostBlock {
-  int nx[3], ng[3], n[3], s[3], e[3], lx[3];
+  int nx[3], ng[3], n[3], s[3], e[3], lx[3], level;
   double xg[3], dx[3];
   int64_t cells;
   double xc(int d, int idx) const { return xg[d] + (idx + 0.5) * dx[d]; }
@@ -18,6 +19,23 @@ struct HostBlock {
   int64_t id(int k, int j, int i) const { return ((int64_t)k * n[1] + j) * n[0] + i; }
 };
 HostBlock make_host_block(const pangu_block &b);
+
+// Static mesh refinement (refine_host.cpp).  refined_mesh: any block of the
+// context above level 0.  check_tiling: the blocks of each mesh tile its
+// domain (no overlap, no gap); "" or the reason they do not.
+// build_refinement_tables: gather tables of the cross-level ghost exchange
+// and of flux correction; "" or why the mesh cannot be handled (neighbours
+// more than one level apart, a prolongation stencil across a physical
+// boundary or beyond the ghost zones).
+struct XFill;
+struct XProl;
+struct XFlux;
+bool refined_mesh(const std::vector<HostBlock> &hb);
+std::string check_tiling(const std::vector<HostBlock> &hb, const std::vector<int> &bmesh,
+                         const std::vector<pangu_config> &mcfg);
+std::string build_refinement_tables(const std::vector<HostBlock> &hb, const std::vector<int> &bmesh,
+                                    const std::vector<pangu_config> &mcfg, std::vector<XFill> &fill,
+                                    std::vector<XProl> &prol, std::vector<XFlux> &flux);
 
 namespace geom {
 typedef double M4[4][4];

@@ -92,6 +92,15 @@ struct KernelClock {
   ~KernelClock();
 };
 
+// Static mesh refinement tables (XFill, XProl, XFlux: common.h).
+struct MultiLevel {
+  int on = 0;
+  const XFill *fill = nullptr;
+  const XProl *prol = nullptr;
+  const XFlux *flux = nullptr;
+  int nfill = 0, nprol = 0, nflux = 0;
+};
+
 // Host-side launch context: kernel constants plus the launch ranges.
 struct LaunchCtx {
   KCtx k;
@@ -103,6 +112,7 @@ struct LaunchCtx {
   int split_source = 0;            // fused stage: source term in a kernel of its own
   int fuse_ct = 1;                 // fused stage: CT face fluxes formed in the update kernel
   int share_x1 = 0;                // X1 sweep reconstructs each cell once (k_flux_x1s)
+  MultiLevel ml;                   // refinement tables (ml.on = 0 on a single-level mesh)
   KernelClock *clock = nullptr;
 };
 
@@ -138,6 +148,10 @@ int launch_fix_recovery(const LaunchCtx &L, const KFields &F, double *const *fla
 int launch_exchange(const LaunchCtx &L, const KFields &F, double *const *flag, const int *nbr,
                     hipStream_t s);
 int launch_bc(const LaunchCtx &L, const KFields &F, double *const *flag, hipStream_t s);
+// Refined mesh: ghost cells across and within levels (fills, then
+// prolongation), and coarse face fluxes from the fine ones.
+int launch_exchange_ml(const LaunchCtx &L, const KFields &F, double *const *flag, hipStream_t s);
+int launch_flux_correction(const LaunchCtx &L, const KFields &F, hipStream_t s);
 // red: 3 doubles of scratch per interior plane; state: {dt, time} per mesh;
 // est_out (optional): one estimate per mesh
 int launch_diagnostics(const LaunchCtx &L, const KFields &F, const int *shell, double *part,

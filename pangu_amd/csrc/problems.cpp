@@ -26,6 +26,7 @@ HostBlock make_host_block(const pangu_block &b) {
     h.dx[d] = b.dx[d];
     h.xg[d] = b.xmin[d] - b.ng[d] * b.dx[d];
   }
+  h.level = b.level;
   h.cells = (int64_t)h.n[0] * h.n[1] * h.n[2];
   return h;
 }

@@ -23,7 +23,7 @@ enum {
 // task ids
 enum {
   T_FIX = 0, T_CONS, T_FLUXES, T_CT, T_DIV, T_UPDATE, T_SOURCE, T_RECOVERY, T_HEATING,
-  T_FIXREC, T_EXCHANGE, T_BC, T_TIMESTEP
+  T_FIXREC, T_EXCHANGE, T_BC, T_TIMESTEP, T_FLUXCORR
 };
 
 struct pangu_sim {
@@ -80,6 +80,164 @@ static double mesh_gen(int idx, int n, double xmin, double xmax) {
   const long noffc = idx - (n + 1) / 2;
   const double x = (double)(noff + noffc) / (2.0 * n);
   return 0.5 * (xmin + xmax) + (x * xmax - x * xmin);
+}
+
+// Edge idx of n equal cells over [xmin, xmax] (mesh edges exact).
+static double edge_at(int64_t idx, int64_t n, double xmin, double xmax) {
+  return idx == 0 ? xmin : (idx == n ? xmax : mesh_gen((int)idx, (int)n, xmin, xmax));
+}
+
+// Block tree of a statically refined mesh (Parthenon's Mesh constructor with
+// <parthenon/static_refinementN> blocks).  Nodes are (level, lx); a refined
+// node has 2^dim children ordered x1 fastest.  Refining a node first makes
+// every neighbour (faces, edges, corners; periodic wrap) at least as fine as
+// the node, so leaves next to each other differ by at most one level.
+struct LeafTree {
+  struct Node {
+    int level;
+    int64_t l[3];
+    int child0;  // first child, -1 for a leaf
+  };
+  std::vector<Node> nodes;  // root nodes first, x1 fastest
+  int dim = 1;
+  int64_t nb[3] = {1, 1, 1};
+  bool periodic[3] = {false, false, false};
+
+  LeafTree(const pangu_sim_cfg *c) {
+    dim = c->nx[2] > 1 ? 3 : (c->nx[1] > 1 ? 2 : 1);
+    for (int d = 0; d < 3; ++d) {
+      nb[d] = c->nb[d];
+      periodic[d] = c->bc[d][0] == PANGU_BC_PERIODIC;
+    }
+    for (int64_t k = 0; k < nb[2]; ++k)
+      for (int64_t j = 0; j < nb[1]; ++j)
+        for (int64_t i = 0; i < nb[0]; ++i) nodes.push_back({0, {i, j, k}, -1});
+  }
+  // Deepest node on the path to (level, l): the node itself or the leaf
+  // that covers it.
+  int descend(int level, const int64_t l[3]) const {
+    int64_t r[3];
+    for (int d = 0; d < 3; ++d) r[d] = d < dim ? l[d] >> level : 0;
+    int n = (int)((r[2] * nb[1] + r[1]) * nb[0] + r[0]);
+    while (nodes[n].level < level && nodes[n].child0 >= 0) {
+      const int sh = level - nodes[n].level - 1;
+      int c = 0;
+      for (int d = 0; d < dim; ++d) c |= (int)((l[d] >> sh) & 1) << d;
+      n = nodes[n].child0 + c;
+    }
+    return n;
+  }
+  void refine(int n) {
+    const int L = nodes[n].level;
+    // neighbours of n must reach level L first
+    for (int o2 = (dim > 2 ? -1 : 0); o2 <= (dim > 2 ? 1 : 0); ++o2)
+      for (int o1 = (dim > 1 ? -1 : 0); o1 <= (dim > 1 ? 1 : 0); ++o1)
+        for (int o0 = -1; o0 <= 1; ++o0) {
+          const int o[3] = {o0, o1, o2};
+          int64_t l[3];
+          bool inside = true;
+          for (int d = 0; d < 3; ++d) {
+            l[d] = nodes[n].l[d] + o[d];
+            const int64_t N = d < dim ? nb[d] << L : 1;
+            if (l[d] < 0 || l[d] >= N) {
+              if (periodic[d] && d < dim)
+                l[d] = (l[d] + N) % N;
+              else
+                inside = false;
+            }
+          }
+          if (inside && nodes[descend(L, l)].level < L) add(L, l);
+        }
+    if (nodes[n].child0 >= 0) return;  // refined meanwhile through a neighbour
+    const int first = (int)nodes.size();
+    for (int c = 0; c < (1 << dim); ++c) {
+      Node ch;
+      ch.level = L + 1;
+      ch.child0 = -1;
+      for (int d = 0; d < 3; ++d) ch.l[d] = d < dim ? 2 * nodes[n].l[d] + ((c >> d) & 1) : 0;
+      nodes.push_back(ch);
+    }
+    nodes[n].child0 = first;
+  }
+  // Makes (level, l) a node of the tree.
+  void add(int level, const int64_t l[3]) {
+    for (;;) {
+      const int n = descend(level, l);
+      if (nodes[n].level == level) return;
+      refine(n);
+    }
+  }
+  void leaves(int n, std::vector<int> &out) const {
+    if (nodes[n].child0 < 0) {
+      out.push_back(n);
+      return;
+    }
+    for (int c = 0; c < (1 << dim); ++c) leaves(nodes[n].child0 + c, out);
+  }
+};
+
+// Leaves of a mesh after its refinement regions (in the order given).  Per
+// active dim at level L (N = nb << L blocks), a region spans the blocks from
+// the first whose upper edge lies above its lower bound to the first whose
+// upper edge reaches its upper bound, widened to whole sibling pairs.
+static int mesh_leaves(const pangu_sim_cfg *c, std::vector<pangu_block> &out, std::string &err) {
+  if (c->nref < 0 || c->nref > PANGU_SIM_MAX_REF) {
+    err = "nref out of range";
+    return PANGU_EINVAL;
+  }
+  LeafTree T(c);
+  for (int r = 0; r < c->nref; ++r) {
+    const int L = c->ref_level[r];
+    if (L < 0 || L > 20) {
+      err = "refinement level out of range (0..20)";
+      return PANGU_EINVAL;
+    }
+    if (L == 0) continue;
+    int64_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    bool empty = false;
+    for (int d = 0; d < T.dim; ++d) {
+      const int64_t N = T.nb[d] << L;
+      int64_t a = 0;
+      while (a < N && !(edge_at(a + 1, N, c->xmin[d], c->xmax[d]) > c->ref_min[r][d])) ++a;
+      int64_t b = a;
+      while (b < N && !(edge_at(b + 1, N, c->xmin[d], c->xmax[d]) >= c->ref_max[r][d])) ++b;
+      if (a >= N) empty = true;
+      if (b >= N) b = N - 1;
+      a -= a & 1;
+      if (!(b & 1)) ++b;
+      lo[d] = a;
+      hi[d] = b;
+    }
+    if (empty) continue;
+    for (int64_t k = lo[2]; k <= hi[2]; k += (T.dim > 2 ? 2 : 1))
+      for (int64_t j = lo[1]; j <= hi[1]; j += (T.dim > 1 ? 2 : 1))
+        for (int64_t i = lo[0]; i <= hi[0]; i += 2) {
+          const int64_t l[3] = {i, j, k};
+          T.add(L, l);
+        }
+  }
+  std::vector<int> lv;
+  const int nroot = (int)(T.nb[0] * T.nb[1] * T.nb[2]);
+  for (int n = 0; n < nroot; ++n) T.leaves(n, lv);
+  out.clear();
+  for (int n : lv) {
+    const LeafTree::Node &N = T.nodes[n];
+    pangu_block B;
+    std::memset(&B, 0, sizeof(B));
+    B.level = N.level;
+    for (int d = 0; d < 3; ++d) {
+      B.lx[d] = (int)N.l[d];
+      B.nx[d] = c->nx[d] / c->nb[d];
+      B.ng[d] = (c->nx[d] > 1) ? c->ng : 0;
+      const int64_t cnt = d < T.dim ? T.nb[d] << N.level : 1;
+      const double lo = edge_at(N.l[d], cnt, c->xmin[d], c->xmax[d]);
+      const double hi = edge_at(N.l[d] + 1, cnt, c->xmin[d], c->xmax[d]);
+      B.xmin[d] = lo;
+      B.dx[d] = (hi - lo) / B.nx[d];
+    }
+    out.push_back(B);
+  }
+  return 0;
 }
 
 template <class T>
@@ -179,21 +337,13 @@ int pangu_sim_create_batch(const pangu_sim_cfg *cfgs, int nmesh, int device, pan
   }
   for (int m = 0; m < nmesh; ++m) {
     const pangu_sim_cfg *cm = &cfgs[m];
-    const int nb = cm->nb[0] * cm->nb[1] * cm->nb[2];
-    for (int b = 0; b < nb; ++b) {
-      pangu_block B;
-      const int lx[3] = {b % cm->nb[0], (b / cm->nb[0]) % cm->nb[1], b / (cm->nb[0] * cm->nb[1])};
-      for (int d = 0; d < 3; ++d) {
-        B.lx[d] = lx[d];
-        B.nx[d] = cm->nx[d] / cm->nb[d];
-        B.ng[d] = (cm->nx[d] > 1) ? cm->ng : 0;
-        const double lo =
-            (lx[d] == 0) ? cm->xmin[d] : mesh_gen(lx[d], cm->nb[d], cm->xmin[d], cm->xmax[d]);
-        const double hi = (lx[d] == cm->nb[d] - 1) ? cm->xmax[d]
-                                                   : mesh_gen(lx[d] + 1, cm->nb[d], cm->xmin[d], cm->xmax[d]);
-        B.xmin[d] = lo;
-        B.dx[d] = (hi - lo) / B.nx[d];
-      }
+    std::vector<pangu_block> leaves;
+    if (int r = mesh_leaves(cm, leaves, s->err)) {
+      fprintf(stderr, "pangu_sim_create: %s\n", s->err.c_str());
+      delete s;
+      return r;
+    }
+    for (const pangu_block &B : leaves) {
       s->blocks.push_back(B);
       s->hb.push_back(make_host_block(B));
       s->bmesh.push_back(m);
@@ -375,6 +525,7 @@ static int stage_tasks(pangu_sim *s, int stage) {
   PCHK(s, pangu_calculate_conservative(x, c0, st));
   PCHK(s, pangu_calculate_fluxes(x, c0, st));
   if (s->cfg.enable_B) PCHK(s, pangu_constrained_transport(x, c0, st));
+  PCHK(s, pangu_flux_correction(x, c0, st));
   PCHK(s, pangu_flux_divergence(x, c0, s->h_dudt, st));
   PCHK(s, pangu_update_independent(x, s->h_base, s->h_dudt, bdt, c1, st));
   if (s->cfg.exchange_after == 0) PCHK(s, pangu_exchange_ghosts(x, c1, st));
@@ -821,6 +972,7 @@ int pangu_sim_task(pangu_sim *s, int task, int c0i, int c1i, double bdt, double 
     case T_FIXREC: PCHK(s, pangu_fix_recovery(x, c1, st)); break;
     case T_EXCHANGE: PCHK(s, pangu_exchange_ghosts(x, c1, st)); break;
     case T_BC: PCHK(s, pangu_apply_boundary_conditions(x, c1, st)); break;
+    case T_FLUXCORR: PCHK(s, pangu_flux_correction(x, c0, st)); break;
     case T_TIMESTEP: {
       std::vector<double> e(s->nmesh);
       PCHK(s, pangu_estimate_timestep(x, c1, e.data(), st));
@@ -925,6 +1077,13 @@ int pangu_sim_block_geom(pangu_sim *s, int b, double *xd, int *n) {
     n[3 + d] = H.s[d];
     n[6 + d] = H.e[d];
   }
+  return 0;
+}
+
+int pangu_sim_block_loc(pangu_sim *s, int b, int *out) {
+  if (!s || b < 0 || b >= s->nblk) return PANGU_EINVAL;
+  out[0] = s->hb[b].level;
+  for (int d = 0; d < 3; ++d) out[1 + d] = s->hb[b].lx[d];
   return 0;
 }
 

@@ -12,6 +12,23 @@ import numpy as np
 from . import lib_path
 
 
+MAX_REF = 8  # PANGU_SIM_MAX_REF
+
+
+def refine_fields(c, regions):
+    """Static refinement regions [(level, (x1min, x2min, x3min), (x1max, x2max,
+    x3max)), ...] into the nref / ref_level / ref_min / ref_max fields of a
+    configuration struct."""
+    if len(regions) > MAX_REF:
+        raise ValueError(f"at most {MAX_REF} refinement regions")
+    c.nref = len(regions)
+    for r, (level, lo, hi) in enumerate(regions):
+        c.ref_level[r] = int(level)
+        for d in range(3):
+            c.ref_min[r][d] = float(lo[d])
+            c.ref_max[r][d] = float(hi[d])
+
+
 class SimCfg(C.Structure):
     _fields_ = [
         ("nx", C.c_int * 3), ("nb", C.c_int * 3), ("ng", C.c_int),
@@ -36,13 +53,16 @@ class SimCfg(C.Structure):
         ("rin", C.c_double), ("rmax", C.c_double), ("kappa", C.c_double),
         ("perturbation", C.c_double), ("beta", C.c_double), ("aphi_cut", C.c_double),
         ("seed", C.c_uint64), ("nthreads", C.c_int),
+        ("nref", C.c_int), ("ref_level", C.c_int * MAX_REF),
+        ("ref_min", (C.c_double * 3) * MAX_REF), ("ref_max", (C.c_double * 3) * MAX_REF),
     ]
 
 
 FIELDS = dict(prim=0, cons=1, flux1=2, flux2=3, flux3=4, alfven=5, emf=6, flag=7, mask=8,
               c2p_status=9, c2p_iters=10, flag_recovery=11, dudt=12)
 TASKS = dict(fix_primitive=0, conservative=1, fluxes=2, ct=3, flux_div=4, update=5, source=6,
-             recovery=7, heating=8, fix_recovery=9, exchange=10, bc=11, timestep=12)
+             recovery=7, heating=8, fix_recovery=9, exchange=10, bc=11, timestep=12,
+             flux_correction=13)
 MODES = dict(tasks=0, fused=1, graph=2)
 # pangu_diagnostics outputs, in PANGU_DIAG_* order
 DIAG_NAMES = ("mdot", "edot", "jdot", "phi", "mass", "energy", "divb_max", "ub_max")
@@ -85,6 +105,7 @@ def lib():
         L.pangu_sim_dt.restype = C.c_double
         L.pangu_sim_dt.argtypes = [vp]
         L.pangu_sim_block_geom.argtypes = [vp, C.c_int, dp, ip]
+        L.pangu_sim_block_loc.argtypes = [vp, C.c_int, ip]
         L.pangu_sim_set_tile_order.argtypes = [vp, C.c_int, C.c_int]
         L.pangu_sim_set_split_source.argtypes = [vp, C.c_int]
         L.pangu_sim_set_fuse_ct.argtypes = [vp, C.c_int]
@@ -113,8 +134,11 @@ def make_cfg(d):
             arr = getattr(c, name)
             for i in range(3):
                 arr[i] = v[i]
+        elif name in ("nref", "ref_level", "ref_min", "ref_max"):
+            raise ValueError("give refinement regions as cfg['refine']")
         else:
             setattr(c, name, v)
+    refine_fields(c, d.get("refine", ()))
     return c
 
 
@@ -210,6 +234,13 @@ class HipSim:
         lib().pangu_sim_block_geom(self._p, b, xd.ctypes.data_as(C.POINTER(C.c_double)),
                                    n.ctypes.data_as(C.POINTER(C.c_int)))
         return xd, n
+
+    def block_loc(self, b):
+        """(level, lx1, lx2, lx3) of block b."""
+        out = np.zeros(4, dtype=np.int32)
+        self._check(lib().pangu_sim_block_loc(self._p, b, out.ctypes.data_as(C.POINTER(C.c_int))),
+                    "block_loc")
+        return tuple(int(x) for x in out)
 
     def init(self):
         self._check(lib().pangu_sim_init(self._p), "init")

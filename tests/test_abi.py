@@ -43,7 +43,7 @@ def test_struct_layouts_match_c(tmp_path):
     equal the ctypes mirrors used by the Python host side."""
     checks = {
         "pangu_config": (abi.PanguConfig, ["metric", "gamma", "heat_model", "pexcise", "nb", "bc"]),
-        "pangu_block": (abi.PanguBlock, ["nx", "ng", "lx", "xmin", "dx"]),
+        "pangu_block": (abi.PanguBlock, ["nx", "ng", "lx", "xmin", "dx", "level"]),
         "pangu_container": (abi.PanguContainer, ["prim", "cons", "flux"]),
         "pangu_shared": (abi.PanguShared, ["alfven", "mask", "flag_recovery"]),
         "pangu_sim_cfg": (sim.SimCfg, ["nx", "bc", "fel_0", "cache_geometry", "seed", "nthreads"]),

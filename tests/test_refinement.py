@@ -1,0 +1,181 @@
+"""Static mesh refinement in the oracle (CPU): the block tree, ghost cells
+across levels and flux correction (oracle/refine.cpp).
+
+Parthenon, which the reference relies on for these, is an empty submodule in
+the reference checkout, so no reference output exists for them.  They are
+pinned instead by hand counts of the block tree, by exactness on linear
+data, by bitwise equality of an everywhere-refined mesh with the uniform mesh
+of the same resolution, and by mass conservation across refinement faces;
+planted mistakes show that the pins catch a wrong slope factor and a missing
+flux correction."""
+import collections
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+from pangu_amd import configs
+
+
+def cks_smr_mesh():
+    """The block tree of fm_torus_cks/inputfile: 64^3 cells over [-64, 64]^3
+    in 16^3 blocks, static refinement regions [-64/2^L, 64/2^L]^3 at levels
+    L = 1..4 (inputfile:87-122)."""
+    d = dict(configs._BASE)
+    d.update(nx=[64, 64, 64], nb=[4, 4, 4], xmin=[-64.0] * 3, xmax=[64.0] * 3,
+             bc=[[configs.OUTFLOW] * 2] * 3, metric=2, a=0.9375, h=0.7, problem=1,
+             enable_B=0, recovery=1, limiter=1, riemann=0)
+    d["refine"] = [(L, (-64.0 / 2 ** L,) * 3, (64.0 / 2 ** L,) * 3) for L in (1, 2, 3, 4)]
+    return d
+
+
+def _locs(o):
+    return [o.block_loc(b) for b in range(o.nblocks)]
+
+
+def test_fm_torus_cks_block_tree():
+    """Hand count: each level-L region covers the middle 2x2x2 blocks of
+    level L-1 (edges of level L-1 blocks fall exactly on +-64/2^L), so level L
+    replaces 8 blocks of level L-1 by 64: 64-8 = 56 leaves on levels 0-3 and
+    64 on level 4, 288 in all.  Neighbours already differ by one level, so
+    the 2:1 rule adds nothing."""
+    o = O.OracleSim(cks_smr_mesh(), nthreads=1)
+    c = collections.Counter(l[0] for l in _locs(o))
+    assert o.nblocks == 288
+    assert dict(c) == {0: 56, 1: 56, 2: 56, 3: 56, 4: 64}
+    # level-4 leaves are exactly the 4x4x4 blocks of level-4 locations 30..33
+    l4 = sorted(l[1:] for l in _locs(o) if l[0] == 4)
+    assert l4 == sorted((i, j, k) for k in range(30, 34) for j in range(30, 34) for i in range(30, 34))
+
+
+def test_two_to_one_rule_refines_neighbours():
+    """2-D, 4x4 root blocks on [0,1]^2; one level-2 region [0.26, 0.3]^2.
+    Level-2 blocks are 1/16 wide: the region spans block 4 (upper edge 0.3125
+    is the first above 0.26 and the first >= 0.3), widened to the pair 4-5.
+    Creating them refines level-1 block (2,2), whose level-1 neighbours
+    (1..3, 1..3) lie in root blocks (0,0), (1,0), (0,1) and (1,1): all four
+    roots are refined.  Leaves: 12 roots, 4x4 - 1 = 15 level-1, 4 level-2."""
+    d = configs.orszag_tang(64, (4, 4, 1))
+    d["refine"] = [(2, (0.26, 0.26, -0.5), (0.3, 0.3, 0.5))]
+    o = O.OracleSim(d, nthreads=1)
+    locs = _locs(o)
+    c = collections.Counter(l[0] for l in locs)
+    assert dict(c) == {0: 12, 1: 15, 2: 4}
+    assert sorted(l[1:3] for l in locs if l[0] == 2) == [(4, 4), (4, 5), (5, 4), (5, 5)]
+    roots = {l[1:3] for l in locs if l[0] == 0}
+    assert not roots & {(0, 0), (1, 0), (0, 1), (1, 1)}
+
+
+def _set_linear(o, coef, container=1):
+    """P_q = c0 + c1 x + c2 y (+ c3 z) at every interior cell centre, zeros
+    in ghost cells; returns the exact values for every cell."""
+    P = o.get("prim", container)
+    exact = np.zeros_like(P)
+    for b in range(o.nblocks):
+        xd, n = o.block_geom(b)
+        xs = [xd[d] + (np.arange(n[d]) + 0.5) * xd[3 + d] for d in range(3)]
+        Z, Y, X = np.meshgrid(xs[2], xs[1], xs[0], indexing="ij")
+        f = coef[0] + coef[1] * X + coef[2] * Y + (coef[3] * Z if n[2] > 1 else 0.0)
+        for q in range(P.shape[1]):
+            exact[b, q] = f * (q + 1)
+    P[...] = 0.0
+    s, e = o.s, o.e
+    P[:, :, s[2]:e[2] + 1, s[1]:e[1] + 1, s[0]:e[0] + 1] = exact[:, :, s[2]:e[2] + 1, s[1]:e[1] + 1,
+                                                                 s[0]:e[0] + 1]
+    o.set("prim", P, container)
+    return exact
+
+
+def _ghost_error(o, exact, container=1):
+    """Largest |ghost - exact| over ghost cells inside the periodic domain."""
+    P = o.get("prim", container)
+    return np.abs(P - exact).max()
+
+
+@pytest.mark.parametrize("plant", ["none", "prolong_slope"])
+def test_linear_data_is_exchanged_exactly(plant):
+    """Linear data: restriction of 2x2 fine cells is the value at the coarse
+    centre, and minmod slopes equal the exact differences, so +-1/4 of them
+    gives the fine centres.  Dyadic coefficients and cell centres keep every
+    operation exact; a domain of [0, 1]^2 with periodic wrap is not linear
+    across the wrap, so the data is placed on a box with outflow faces."""
+    d = configs.orszag_tang(64, (4, 4, 1))
+    d["refine"] = [(1, (0.3, 0.3, -0.5), (0.7, 0.7, 0.5))]
+    d["bc"] = [[configs.OUTFLOW] * 2, [configs.OUTFLOW] * 2, [configs.PERIODIC] * 2]
+    o = O.OracleSim(d, nthreads=1)
+    O.plant(plant)
+    try:
+        exact = _set_linear(o, (0.5, 0.25, -0.125, 0.0))
+        o.task("exchange", 0, 1)
+    finally:
+        O.plant("none")
+    P = o.get("prim", 1)
+    # compare ghost cells that lie inside the domain (physical ones are 0)
+    err = 0.0
+    for b in range(o.nblocks):
+        xd, n = o.block_geom(b)
+        xs = [xd[dd] + (np.arange(n[dd]) + 0.5) * xd[3 + dd] for dd in range(2)]
+        Y, X = np.meshgrid(xs[1], xs[0], indexing="ij")
+        inside = (X > 0) & (X < 1) & (Y > 0) & (Y < 1)
+        err = max(err, np.abs(P[b, :, 0][:, inside] - exact[b, :, 0][:, inside]).max())
+    if plant == "none":
+        assert err == 0.0
+    else:
+        assert err > 1e-4
+
+
+def test_everywhere_refined_equals_uniform_mesh():
+    """A region covering the whole domain at level 1 gives the blocks (and
+    block edges) of the uniform mesh with twice the blocks per dim; every
+    ghost is then a same-level copy, so one cycle matches bit for bit."""
+    base = configs.orszag_tang(32, (2, 2, 1))
+    base["refine"] = [(1, (0.0, 0.0, -0.5), (1.0, 1.0, 0.5))]
+    fine = configs.orszag_tang(32, (4, 4, 1))
+    fine["nx"] = [64, 64, 1]
+    a, b = O.OracleSim(base, nthreads=1), O.OracleSim(fine, nthreads=1)
+    assert a.nblocks == b.nblocks == 16
+    a.init()
+    b.init()
+    a.step(2)
+    b.step(2)
+    la, lb = _locs(a), _locs(b)
+    Pa, Pb = a.get("prim", 0), b.get("prim", 0)
+    for i, l in enumerate(la):
+        j = lb.index((0,) + l[1:])
+        np.testing.assert_array_equal(Pa[i], Pb[j])
+    assert a.dt == b.dt
+
+
+def _mass(o, c=0):
+    U = o.get("cons", c)
+    tot = 0.0
+    for b in range(o.nblocks):
+        xd, n = o.block_geom(b)
+        s, e = n[3:6], n[6:9]
+        tot += U[b, 0, s[2]:e[2] + 1, s[1]:e[1] + 1, s[0]:e[0] + 1].sum() * xd[3] * xd[4]
+    return tot
+
+
+@pytest.mark.parametrize("plant", ["none", "no_flux_correction"])
+def test_flux_correction_conserves_mass_across_levels(plant):
+    """Periodic OT with a refined centre, ghosts exchanged after recovery:
+    with flux correction each coarse face flux equals the mean of the fine
+    ones, so mass leaves one level exactly as it enters the other (drift at
+    round-off); without it the drift is many orders larger."""
+    d = configs.orszag_tang(64, (4, 4, 1))
+    d["refine"] = [(1, (0.3, 0.3, -0.5), (0.7, 0.7, 0.5))]
+    d["exchange_after"] = 1
+    o = O.OracleSim(d, nthreads=4)
+    o.init()
+    o.step(1)
+    m0 = _mass(o)
+    O.plant(plant)
+    try:
+        o.step(6)
+    finally:
+        O.plant("none")
+    drift = abs(_mass(o) - m0) / m0
+    if plant == "none":
+        assert drift < 1e-14
+    else:
+        assert drift > 1e-9
