@@ -71,6 +71,8 @@ def parse():
     ap.add_argument("--mode", default="graph", choices=["graph", "fused", "tasks"])
     ap.add_argument("--small", action="store_true", help="reduced mesh for quick checks")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-refined", action="store_true",
+                    help="skip the static-refinement run (fm_torus_cks)")
     ap.add_argument("--autotune", action="store_true",
                     help="re-tune launch parameters instead of applying the recorded tuning")
     ap.add_argument("--cpu-sample-cycles", type=int, default=3)
@@ -176,6 +178,41 @@ def batched_vs_plain(scale, cycles=10, warmup=2):
                      f"{sum(1 for d in ens if d['nx'][2] > 1)} single-block 3D tori (16^3, 32^3, 64^3), "
                      "random a, beta, rin; PPM4+HLL+Kastaun+CT+heating"),
     }
+
+
+def refined_run(cycles=20, warmup=2):
+    """The reference's static-refinement run (fm_torus_cks/inputfile): a
+    hydrodynamic FM torus in Cartesian Kerr-Schild coordinates with
+    excision, 64^3 root cells in 16^3 blocks refined to four nested levels
+    (288 blocks), LAXF + PPM4 + Kastaun, one global dt."""
+    import collections
+    from pangu_amd import configs
+    from pangu_amd.sim import HipSim
+    cfg = configs.fm_torus_cks()
+    t0 = time.perf_counter()
+    h = HipSim(cfg)
+    h.init()
+    t_init = time.perf_counter() - t0
+    nb = h.nblocks
+    per_block = (h.e[0] - h.s[0] + 1) * (h.e[1] - h.s[1] + 1) * (h.e[2] - h.s[2] + 1)
+    zones = nb * per_block
+    levels = collections.Counter(h.block_loc(b)[0] for b in range(nb))
+    h.step(warmup, "graph")
+    ms = h.timed_cycles(cycles, "graph")
+    kern = h.time_kernels(ncycles=2)
+    refine_keys = ("exchange_fill", "exchange_prolong", "flux_correction")
+    out = {
+        "workload": "fm_torus_cks: hydro FM torus, CKS a=0.9375, excision, 64^3 root cells, "
+                    "16^3 blocks, 4 nested refinement levels",
+        "blocks": nb, "blocks_per_level": {str(k): v for k, v in sorted(levels.items())},
+        "zones": zones, "cycles": cycles, "ms_per_cycle": ms / cycles,
+        "zone_cycles_per_s": zones * cycles / (ms * 1e-3), "init_s": t_init,
+        "kernel_ms_per_cycle": {k: v[1] for k, v in kern.items()},
+        "refinement_ms_per_cycle": sum(kern[k][1] for k in refine_keys if k in kern),
+        "dt": h.dt,
+    }
+    h.close()
+    return out
 
 
 def learned_vs_plain(cfg, samples, cycles=20, delta=0.01):
@@ -319,6 +356,8 @@ def main():
         out["batched"] = batched_vs_plain(args.ensemble_scale)
     if args.learned_samples > 0:
         out["learned"] = learned_vs_plain(cfg, args.learned_samples)
+    if not args.no_refined and not args.small:
+        out["refined"] = refined_run()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_sample_cycles)
     print(json.dumps(out))

@@ -336,6 +336,195 @@ static int init_torus(Sim &S) {
   return 0;
 }
 
+// ---- fm_torus_cks (hydrodynamic: the reference input has enable_B = false)
+// fm_torus_cks/problem_generator.cpp:37-61: Boyer-Lindquist metric at (r, th).
+static void bl_metric_rth(double r, double th, double a, double g[4][4]) {
+  for (int row = 0; row < 4; ++row)
+    for (int col = 0; col < 4; ++col) g[row][col] = 0.0;
+  const double cth = std::cos(th);
+  const double sth = std::sin(th);
+  const double sth2 = sth * sth;
+  const double r2 = r * r;
+  const double a2 = a * a;
+  const double mu = 1.0 + a2 * cth * cth / r2;
+  const double delta = 1.0 - 2.0 / r + a2 / r2;
+  const double inv_r_mu = 1.0 / (r * mu);
+  g[0][0] = -1.0 + 2.0 * inv_r_mu;
+  g[0][3] = -2.0 * a * sth2 * inv_r_mu;
+  g[1][1] = mu / delta;
+  g[2][2] = r2 * mu;
+  g[3][0] = g[0][3];
+  g[3][3] = r2 * sth2 * (1.0 + a2 / r2 + 2.0 * a2 * sth2 / (r * r2 * mu));
+}
+
+// Kerr-Schild radius clamped at 1 and polar angle of a CKS point (:82-90,
+// :186-190).
+static void ks_r_th(double x, double y, double z, double a2, double &r, double &th) {
+  const double rad2 = x * x + y * y + z * z;
+  r = std::fmax(std::sqrt((rad2 - a2 + std::sqrt((rad2 - a2) * (rad2 - a2) + 4.0 * a2 * z * z)) / 2.0), 1.0);
+  th = (std::fabs(z / r) < 1.0) ? std::acos(z / r) : ((z > 0) ? 0.0 : M_PI);
+}
+
+// :76-141: BL four-velocity -> KS (radial terms) -> CKS Cartesian Jacobian.
+static void bl_to_cks_ucon(double x, double y, double z, double a, const double ubl_in[4], double uc[4]) {
+  const double a2 = a * a;
+  double r, th;
+  ks_r_th(x, y, z, a2, r, th);
+  double gbl[4][4];
+  bl_metric_rth(r, th, a, gbl);
+  double ubl[4];
+  for (int row = 0; row < 4; ++row) ubl[row] = ubl_in[row];
+  ubl[0] = solve_ut(gbl, ubl[1], ubl[2], ubl[3]);
+  const double delta = r * r - 2.0 * r + a2;
+  double uks[4];
+  uks[0] = ubl[0] + (2.0 * r / delta) * ubl[1];
+  uks[1] = ubl[1];
+  uks[2] = ubl[2];
+  uks[3] = ubl[3] + (a / delta) * ubl[1];
+  const double inv_r2pa2 = 1.0 / (r * r + a2);
+  const double rho2 = x * x + y * y;
+  const double sq = (rho2 > 1e-30) ? std::sqrt((r * r + a2) / rho2) : 0.0;
+  double tr[4][4];
+  for (int row = 0; row < 4; ++row)
+    for (int col = 0; col < 4; ++col) tr[row][col] = 0.0;
+  tr[0][0] = 1.0;
+  tr[1][1] = (r * x + a * y) * inv_r2pa2;
+  tr[1][2] = x * z / r * sq;
+  tr[1][3] = -y;
+  tr[2][1] = (r * y - a * x) * inv_r2pa2;
+  tr[2][2] = y * z / r * sq;
+  tr[2][3] = x;
+  tr[3][1] = z / r;
+  tr[3][2] = -r * std::sqrt(rho2 * inv_r2pa2);
+  for (int row = 0; row < 4; ++row) {
+    uc[row] = 0.0;
+    for (int col = 0; col < 4; ++col) uc[row] += tr[row][col] * uks[col];
+  }
+}
+
+// ProblemGenerator (:143-334) over entire blocks, then
+// MeshPostInitialization (:336-474): rho and u divided by the maximum
+// density, then the radius-dependent floors (excision values inside
+// r_excise, from the unclamped Boyer-Lindquist radius).  Entropy keeps its
+// pre-normalisation value, as in the reference.
+static int init_torus_cks(Sim &S) {
+  const oracle_cfg &c = S.cfg;
+  if (c.metric != MET_CKS || S.p.enable_B) return -1;
+  const double gam = c.gamma, a = c.a, a2 = a * a;
+  const double rin = c.rin, kappa = c.kappa, pert = c.perturbation;
+  const double l = lfish_calc(c.rmax, a);
+  const double thin = M_PI_2;
+  const double sthin = std::sin(thin), cthin = std::cos(thin);
+  const double DDin = rin * rin - 2. * rin + a * a;
+  const double AAin = (rin * rin + a * a) * (rin * rin + a * a) - DDin * a * a * sthin * sthin;
+  const double SSin = rin * rin + a * a * cthin * cthin;
+  int G[3], NX[3];
+  for (int d = 0; d < 3; ++d) {
+    G[d] = S.blk[0].ng[d];
+    NX[d] = c.nx[d] + 2 * G[d];
+  }
+  for (int b = 0; b < S.nblk; ++b) {
+    const Block &B = S.blk[b];
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int k = 0; k < B.n[2]; ++k)
+      for (int j = 0; j < B.n[1]; ++j)
+        for (int i = 0; i < B.n[0]; ++i) {
+          const int64_t id = B.idx(k, j, i);
+          const int64_t gi = B.gofs[0] + i, gj = B.gofs[1] + j, gk = B.gofs[2] + k;
+          const double rnd = keyed_uniform(c.seed, (uint64_t)((gk * NX[1] + gj) * NX[0] + gi));
+          const double x = Xc(B, 0, i), y = Xc(B, 1, j), z = Xc(B, 2, k);
+          const double xc[4] = {0.0, x, y, z};
+          double r, th;
+          ks_r_th(x, y, z, a2, r, th);
+          const double sth = std::sin(th), cth = std::cos(th);
+          double gcov[4][4], gcon[4][4], gbl[4][4];
+          code_metric(MET_CKS, xc, gcov, c.h, a);
+          bl_metric_rth(r, th, a, gbl);
+          invert4(gcov, gcon);
+          const double alpha = 1.0 / std::sqrt(-gcon[0][0]);
+          const double beta1 = gcon[0][1] * alpha * alpha;
+          const double beta2 = gcon[0][2] * alpha * alpha;
+          const double beta3 = gcon[0][3] * alpha * alpha;
+          double rho = 0.0, eint = 0.0, ent = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0;
+          const double DD = r * r - 2.0 * r + a2;
+          const double AA = (r * r + a2) * (r * r + a2) - DD * a2 * sth * sth;
+          const double SS = r * r + a2 * cth * cth;
+          double lnh = 1.0;
+          if (r >= rin) {
+            lnh = 0.5 * std::log((1. + std::sqrt(1. + 4. * (l * l * SS * SS) * DD /
+                                                         (AA * sth * AA * sth))) /
+                                 (SS * DD / AA)) -
+                  0.5 * std::sqrt(1. + 4. * (l * l * SS * SS) * DD / (AA * AA * sth * sth)) -
+                  2. * a * r * l / AA -
+                  (0.5 * std::log((1. + std::sqrt(1. + 4. * (l * l * SSin * SSin) * DDin /
+                                                           (AAin * AAin * sthin * sthin))) /
+                                  (SSin * DDin / AAin)) -
+                   0.5 * std::sqrt(1. + 4. * (l * l * SSin * SSin) * DDin /
+                                            (AAin * AAin * sthin * sthin)) -
+                   2. * a * rin * l / AAin);
+          }
+          if (lnh >= 0.0 && r >= rin) {
+            const double hm1 = std::exp(lnh) - 1.0;
+            if (hm1 > 0.0) {
+              rho = std::pow(hm1 * (gam - 1.0) / (kappa * gam), 1.0 / (gam - 1.0));
+              eint = kappa * std::pow(rho, gam) / (gam - 1.0);
+              eint *= (1.0 + pert * rnd);
+              ent = (gam - 1) * eint * std::pow(rho, -gam);
+              const double e2c = SS * SS * DD / (AA * AA * sth * sth);
+              const double up1 = std::sqrt((-1.0 + std::sqrt(1.0 + 4.0 * l * l * e2c)) / 2.0);
+              const double upbl = 2.0 * a * r * std::sqrt(1.0 + up1 * up1) / std::sqrt(AA * SS * DD) +
+                                  std::sqrt(SS / AA) * up1 / sth;
+              const double u0bl = solve_ut(gbl, 0.0, 0.0, upbl);
+              const double ubl[4] = {u0bl, 0.0, 0.0, upbl};
+              double uc[4];
+              bl_to_cks_ucon(x, y, z, a, ubl, uc);
+              w1 = uc[1] + beta1 * uc[0];
+              w2 = uc[2] + beta2 * uc[0];
+              w3 = uc[3] + beta3 * uc[0];
+            }
+          }
+          S.Pc(0, b, 0)[id] = rho;
+          S.Pc(0, b, 1)[id] = eint;
+          S.Pc(0, b, 2)[id] = w1;
+          S.Pc(0, b, 3)[id] = w2;
+          S.Pc(0, b, 4)[id] = w3;
+          S.Pc(0, b, S.iENT)[id] = ent;
+          if (S.iKEL >= 0) S.Pc(0, b, S.iKEL)[id] = c.fel_0 * ent;
+        }
+  }
+  double rhomax = 0.0;
+  for (int b = 0; b < S.nblk; ++b)
+    for (int64_t id = 0; id < S.blk[b].cells; ++id) rhomax = kmax(rhomax, S.Pc(0, b, 0)[id]);
+  const double e_excise = c.pexcise / (gam - 1.0);
+  for (int b = 0; b < S.nblk; ++b) {
+    const Block &B = S.blk[b];
+    for (int64_t id = 0; id < B.cells; ++id) {
+      S.Pc(0, b, 0)[id] /= rhomax;
+      S.Pc(0, b, 1)[id] /= rhomax;
+    }
+    for (int k = 0; k < B.n[2]; ++k)
+      for (int j = 0; j < B.n[1]; ++j)
+        for (int i = 0; i < B.n[0]; ++i) {
+          const double xc = Xc(B, 0, i), yc = Xc(B, 1, j), zc = Xc(B, 2, k);
+          const double rad2 = xc * xc + yc * yc + zc * zc;
+          const double disc = (rad2 - a2) * (rad2 - a2) + 4.0 * a2 * zc * zc;
+          const double r_bl = std::sqrt(0.5 * (rad2 - a2 + std::sqrt(disc)));
+          double rho_floor, eng_floor;
+          if (r_bl < c.r_excise) {
+            rho_floor = c.dexcise;
+            eng_floor = e_excise;
+          } else {
+            rho_floor = c.density_floor * std::pow(std::fmax(r_bl, 1.0), c.density_floor_pow);
+            eng_floor = c.energy_floor * std::pow(std::fmax(r_bl, 1.0), c.energy_floor_pow);
+          }
+          const int64_t id = B.idx(k, j, i);
+          if (S.Pc(0, b, 0)[id] < rho_floor) S.Pc(0, b, 0)[id] = rho_floor;
+          if (S.Pc(0, b, 1)[id] < eng_floor) S.Pc(0, b, 1)[id] = eng_floor;
+        }
+  }
+  return 0;
+}
+
 // ProblemGenerator + MeshPostInitialization, then the initial ghost exchange
 // and physical boundary conditions of the "base" container.
 int Sim::init_problem() {
@@ -343,7 +532,9 @@ int Sim::init_problem() {
     init_ot(*this);
   else if (cfg.problem == 2)
     init_torus(*this);
-  else
+  else if (cfg.problem == 3) {
+    if (init_torus_cks(*this)) return -1;
+  } else
     return -1;
   exchange(0);
   apply_bc(0);

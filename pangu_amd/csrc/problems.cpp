@@ -9,6 +9,7 @@
 //    (seed, global cell index), so the data do not depend on thread order.
 #include <cmath>
 #include <cstring>
+#include <functional>
 
 #include "host.h"
 
@@ -104,6 +105,165 @@ static void ot_cell(const pangu_config &cfg, double x, double y, double p[10]) {
   p[8] = (gam - 1.0) * u * std::pow(rho, -gam);
 }
 
+// fm_torus_cks/problem_generator.cpp:37-61: Boyer-Lindquist metric at (r, th).
+static void bl_metric_rth(double r, double th, double a, geom::M4 g) {
+  for (int row = 0; row < 4; ++row)
+    for (int col = 0; col < 4; ++col) g[row][col] = 0.0;
+  const double cth = std::cos(th), sth = std::sin(th), sth2 = sth * sth;
+  const double r2 = r * r, a2 = a * a;
+  const double mu = 1.0 + a2 * cth * cth / r2;
+  const double delta = 1.0 - 2.0 / r + a2 / r2;
+  const double inv_r_mu = 1.0 / (r * mu);
+  g[0][0] = -1.0 + 2.0 * inv_r_mu;
+  g[0][3] = -2.0 * a * sth2 * inv_r_mu;
+  g[1][1] = mu / delta;
+  g[2][2] = r2 * mu;
+  g[3][0] = g[0][3];
+  g[3][3] = r2 * sth2 * (1.0 + a2 / r2 + 2.0 * a2 * sth2 / (r * r2 * mu));
+}
+
+// u^t of a BL four-velocity with u^r = u^theta = 0 (:63-74).
+static double bl_ut(const geom::M4 g, double u3) {
+  const double AA = g[0][0];
+  const double BB = 2.0 * (g[0][1] * 0.0 + g[0][2] * 0.0 + g[0][3] * u3);
+  const double CC = 1.0 + g[1][1] * 0.0 * 0.0 + g[2][2] * 0.0 * 0.0 + g[3][3] * u3 * u3 +
+                    2.0 * (g[1][2] * 0.0 * 0.0 + g[1][3] * 0.0 * u3 + g[2][3] * 0.0 * u3);
+  return (-BB - std::sqrt(BB * BB - 4.0 * AA * CC)) / (2.0 * AA);
+}
+
+// Kerr-Schild radius (clamped at 1) and polar angle of a CKS point (:82-90).
+static void ks_r_th(double x, double y, double z, double a2, double &r, double &th) {
+  const double rad2 = x * x + y * y + z * z;
+  r = std::fmax(std::sqrt((rad2 - a2 + std::sqrt((rad2 - a2) * (rad2 - a2) + 4.0 * a2 * z * z)) / 2.0), 1.0);
+  th = (std::fabs(z / r) < 1.0) ? std::acos(z / r) : ((z > 0) ? 0.0 : M_PI);
+}
+
+// FM torus on Cartesian Kerr-Schild coordinates without a magnetic field
+// (fm_torus_cks/problem_generator.cpp:143-474): the BL torus state, its
+// four-velocity taken to KS and then to the Cartesian basis (:76-141), rho
+// and u divided by the maximum density, then radius-dependent floors
+// (excision values inside r_excise).  Entropy keeps its pre-normalisation
+// value, as the reference leaves it.
+static int torus_cks(const pangu_config &cfg, const ProblemParams &pp,
+                     const std::vector<HostBlock> &blocks, int iENT, int iKEL,
+                     const std::function<double *(size_t, int)> &P) {
+  const double gam = cfg.gamma, a = cfg.a, a2 = a * a;
+  const double rin = pp.rin, kap = pp.kappa;
+  const double l = fish_l(pp.rmax, a);
+  const double sthin = std::sin(M_PI_2), cthin = std::cos(M_PI_2);
+  const double DDin = rin * rin - 2. * rin + a * a;
+  const double AAin = (rin * rin + a * a) * (rin * rin + a * a) - DDin * a * a * sthin * sthin;
+  const double SSin = rin * rin + a * a * cthin * cthin;
+  const double lnh_in = fish_lnh_part(l, rin, a, SSin, DDin, AAin, sthin);
+  int NX[3];
+  for (int d = 0; d < 3; ++d) NX[d] = pp.mesh_nx[d] + 2 * blocks[0].ng[d];
+  for (size_t b = 0; b < blocks.size(); ++b) {
+    const HostBlock &B = blocks[b];
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int k = 0; k < B.n[2]; ++k)
+      for (int j = 0; j < B.n[1]; ++j)
+        for (int i = 0; i < B.n[0]; ++i) {
+          const int64_t gi = (int64_t)B.lx[0] * B.nx[0] + i, gj = (int64_t)B.lx[1] * B.nx[1] + j,
+                        gk = (int64_t)B.lx[2] * B.nx[2] + k;
+          const double rnd = keyed_uniform(pp.seed, (uint64_t)((gk * NX[1] + gj) * NX[0] + gi));
+          const double x = B.xc(0, i), y = B.xc(1, j), z = B.xc(2, k);
+          const double xc[4] = {0.0, x, y, z};
+          double r, th;
+          ks_r_th(x, y, z, a2, r, th);
+          const double sth = std::sin(th), cth = std::cos(th);
+          geom::M4 g, gi_, gbl;
+          geom::code_metric(PANGU_METRIC_CKS, xc, g, cfg.h, a);
+          bl_metric_rth(r, th, a, gbl);
+          geom::inv4(g, gi_);
+          const double alpha = 1.0 / std::sqrt(-gi_[0][0]);
+          double rho = 0.0, eint = 0.0, ent = 0.0, w[3] = {0.0, 0.0, 0.0};
+          const double DD = r * r - 2.0 * r + a2;
+          const double AA = (r * r + a2) * (r * r + a2) - DD * a2 * sth * sth;
+          const double SS = r * r + a2 * cth * cth;
+          double lnh = 1.0;
+          if (r >= rin) lnh = fish_lnh_part(l, r, a, SS, DD, AA, sth) - lnh_in;
+          if (lnh >= 0.0 && r >= rin) {
+            const double hm1 = std::exp(lnh) - 1.0;
+            if (hm1 > 0.0) {
+              rho = std::pow(hm1 * (gam - 1.0) / (kap * gam), 1.0 / (gam - 1.0));
+              eint = kap * std::pow(rho, gam) / (gam - 1.0);
+              eint *= (1.0 + pp.perturbation * rnd);
+              ent = (gam - 1) * eint * std::pow(rho, -gam);
+              const double e2c = SS * SS * DD / (AA * AA * sth * sth);
+              const double up1 = std::sqrt((-1.0 + std::sqrt(1.0 + 4.0 * l * l * e2c)) / 2.0);
+              const double upbl = 2.0 * a * r * std::sqrt(1.0 + up1 * up1) / std::sqrt(AA * SS * DD) +
+                                  std::sqrt(SS / AA) * up1 / sth;
+              // BL -> KS: u^t += 2r/Delta u^r, u^phi += a/Delta u^r with u^r = 0
+              const double u0 = bl_ut(gbl, upbl);
+              const double delta = r * r - 2.0 * r + a2;
+              const double ks[4] = {u0 + (2.0 * r / delta) * 0.0, 0.0, 0.0, upbl + (a / delta) * 0.0};
+              // KS -> Cartesian KS Jacobian
+              const double inv_r2pa2 = 1.0 / (r * r + a2);
+              const double rho2 = x * x + y * y;
+              const double sq = (rho2 > 1e-30) ? std::sqrt((r * r + a2) / rho2) : 0.0;
+              geom::M4 tr;
+              for (int rr = 0; rr < 4; ++rr)
+                for (int cc = 0; cc < 4; ++cc) tr[rr][cc] = 0.0;
+              tr[0][0] = 1.0;
+              tr[1][1] = (r * x + a * y) * inv_r2pa2;
+              tr[1][2] = x * z / r * sq;
+              tr[1][3] = -y;
+              tr[2][1] = (r * y - a * x) * inv_r2pa2;
+              tr[2][2] = y * z / r * sq;
+              tr[2][3] = x;
+              tr[3][1] = z / r;
+              tr[3][2] = -r * std::sqrt(rho2 * inv_r2pa2);
+              double uc[4];
+              for (int rr = 0; rr < 4; ++rr) {
+                uc[rr] = 0.0;
+                for (int cc = 0; cc < 4; ++cc) uc[rr] += tr[rr][cc] * ks[cc];
+              }
+              for (int q = 0; q < 3; ++q) w[q] = uc[q + 1] + (gi_[0][q + 1] * alpha * alpha) * uc[0];
+            }
+          }
+          const int64_t id = B.id(k, j, i);
+          P(b, 0)[id] = rho;
+          P(b, 1)[id] = eint;
+          P(b, 2)[id] = w[0];
+          P(b, 3)[id] = w[1];
+          P(b, 4)[id] = w[2];
+          P(b, iENT)[id] = ent;
+          if (iKEL >= 0) P(b, iKEL)[id] = pp.fel_0 * ent;
+        }
+  }
+  double rhomax = 0.0;
+  for (size_t b = 0; b < blocks.size(); ++b)
+    for (int64_t id = 0; id < blocks[b].cells; ++id) rhomax = kmax(rhomax, P(b, 0)[id]);
+  const double e_excise = cfg.pexcise / (gam - 1.0);
+  for (size_t b = 0; b < blocks.size(); ++b) {
+    const HostBlock &B = blocks[b];
+    for (int64_t id = 0; id < B.cells; ++id) {
+      P(b, 0)[id] /= rhomax;
+      P(b, 1)[id] /= rhomax;
+    }
+    for (int k = 0; k < B.n[2]; ++k)
+      for (int j = 0; j < B.n[1]; ++j)
+        for (int i = 0; i < B.n[0]; ++i) {
+          const double xc = B.xc(0, i), yc = B.xc(1, j), zc = B.xc(2, k);
+          const double rad2 = xc * xc + yc * yc + zc * zc;
+          const double disc = (rad2 - a2) * (rad2 - a2) + 4.0 * a2 * zc * zc;
+          const double r_bl = std::sqrt(0.5 * (rad2 - a2 + std::sqrt(disc)));
+          double rf, ef;
+          if (r_bl < cfg.r_excise) {
+            rf = cfg.dexcise;
+            ef = e_excise;
+          } else {
+            rf = cfg.density_floor * std::pow(std::fmax(r_bl, 1.0), cfg.density_floor_pow);
+            ef = cfg.energy_floor * std::pow(std::fmax(r_bl, 1.0), cfg.energy_floor_pow);
+          }
+          const int64_t id = B.id(k, j, i);
+          if (P(b, 0)[id] < rf) P(b, 0)[id] = rf;
+          if (P(b, 1)[id] < ef) P(b, 1)[id] = ef;
+        }
+  }
+  return 0;
+}
+
 int generate_problem(const pangu_config &cfg, const ProblemParams &pp,
                      const std::vector<HostBlock> &blocks, int np, int iBX, int iENT, int iKEL,
                      std::vector<double> &prim) {
@@ -133,6 +293,10 @@ int generate_problem(const pangu_config &cfg, const ProblemParams &pp,
           }
     }
     return 0;
+  }
+  if (pp.problem == 3) {
+    if (cfg.metric != PANGU_METRIC_CKS || iBX >= 0) return -1;
+    return torus_cks(cfg, pp, blocks, iENT, iKEL, P);
   }
   if (pp.problem != 2 || cfg.metric != PANGU_METRIC_MKS) return -1;
 

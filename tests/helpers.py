@@ -5,9 +5,19 @@ from pangu_amd import configs
 
 
 def rel_l1(a, b):
-    """Relative L1 sum|a-b| / sum|b| (interior arrays); 0 when both vanish."""
+    """Relative L1 sum|a-b| / sum|b| (interior arrays); 0 when both vanish.
+    Non-finite entries must sit at the same places in both (else inf) and
+    must be the same kind (+inf, -inf or NaN); the sums run over the rest."""
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
+    fa, fb = np.isfinite(a), np.isfinite(b)
+    if not (fa.all() and fb.all()):
+        if not np.array_equal(fa, fb):
+            return float("inf")
+        na, nb = a[~fa], b[~fb]
+        if not (np.array_equal(np.isnan(na), np.isnan(nb)) and np.array_equal(na[~np.isnan(na)], nb[~np.isnan(nb)])):
+            return float("inf")
+        a, b = a[fa], b[fb]
     den = np.abs(b).sum()
     num = np.abs(a - b).sum()
     if den == 0.0:

@@ -50,6 +50,13 @@ CASES = {
     "torus3d_hll_ppm4_kastaun_heat": _torus3d_refined(),
     "torus3d_chainer_laxf_mc": _torus3d_refined(recovery=0, riemann=0, limiter=0),
     "cks3d_excision_hydro": _cks_refined(),
+    # the reference's own refined problem (fm_torus_cks) on a smaller mesh:
+    # 32^3 root cells over [-32, 32]^3 in 8^3 blocks, two levels.  Its
+    # symmetric box puts x3 faces on z = 0, where the Kerr-Schild radius
+    # vanishes inside x^2 + y^2 < a^2: those face fluxes, and the conserved
+    # state of the excised cells next to them, are non-finite in the oracle
+    # as in the reference; the comparison requires them at the same places.
+    "fm_torus_cks_2level": configs.fm_torus_cks(nx=32, nb=4, levels=2, extent=32.0),
 }
 
 
