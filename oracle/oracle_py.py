@@ -293,7 +293,8 @@ class OracleSim:
 
 
 PLANTS = dict(none=0, conn_index=1, shift_sign=2, ppm_term=3, kastaun_qbar=4, ct_sign=5,
-              emt_pressure=6, no_flux_correction=7, prolong_slope=8)
+              emt_pressure=6, no_flux_correction=7, prolong_slope=8,
+              cks_jacobian=9)
 
 
 def plant(name):

@@ -390,7 +390,7 @@ static void bl_to_cks_ucon(double x, double y, double z, double a, const double 
   tr[0][0] = 1.0;
   tr[1][1] = (r * x + a * y) * inv_r2pa2;
   tr[1][2] = x * z / r * sq;
-  tr[1][3] = -y;
+  tr[1][3] = (g_plant == PLANT_CKS_JACOBIAN) ? y : -y;
   tr[2][1] = (r * y - a * x) * inv_r2pa2;
   tr[2][2] = y * z / r * sq;
   tr[2][3] = x;

@@ -179,3 +179,60 @@ def test_flux_correction_conserves_mass_across_levels(plant):
         assert drift < 1e-14
     else:
         assert drift > 1e-9
+
+
+def _specific_angular_momentum(o, kind, a, h, cks, rho_min, nsample=400):
+    """Fishbone-Moncrief's l = u^t u_phi of torus cells (rho > rho_min) from
+    the initial primitives: u from the state at the cell centre, u_phi the
+    KS azimuthal component (x3 in MKS; -y u_x + x u_y in CKS, since
+    dx/dphi = -y and dy/dphi = x at fixed KS r, theta)."""
+    P = o.get("prim", 0)
+    out = []
+    rng = np.random.default_rng(0)
+    cand = []
+    for b in range(o.nblocks):
+        xd, n = o.block_geom(b)
+        s, e = n[3:6], n[6:9]
+        rho = P[b, 0]
+        for k, j, i in np.argwhere(rho[s[2]:e[2] + 1, s[1]:e[1] + 1, s[0]:e[0] + 1] > rho_min):
+            cand.append((b, k + s[2], j + s[1], i + s[0], xd))
+    for idx in rng.choice(len(cand), size=min(nsample, len(cand)), replace=False):
+        b, k, j, i, xd = cand[idx]
+        x = [0.0, xd[0] + (i + 0.5) * xd[3], xd[1] + (j + 0.5) * xd[4], xd[2] + (k + 0.5) * xd[5]]
+        gcov, gcon, _, ok = O.metric_at(kind, x, a, h)
+        p = np.zeros(10)
+        p[0:5] = P[b, 0:5, k, j, i]
+        st = O.state(p, gcov, gcon)
+        uc = st["ucov"]
+        uphi = (-x[2] * uc[1] + x[1] * uc[2]) if cks else uc[3]
+        out.append(st["ucon"][0] * uphi)
+    return np.array(out)
+
+
+@pytest.mark.parametrize("plant", ["none", "cks_jacobian"])
+def test_cks_torus_has_the_mks_torus_angular_momentum(plant):
+    """The FM torus has one angular momentum l = u^t u_phi in every cell
+    (Fishbone & Moncrief 1976; BL, KS and CKS share u^t and u_phi when
+    u^r = 0, the transformations being t_KS = t_BL + f(r),
+    phi_KS = phi_BL + g(r) and the CKS map at fixed r, theta).  The CKS initial data (fm_torus_cks/problem_generator.cpp)
+    must therefore give, cell by cell, the l of the MKS torus initial data
+    (fm_torus_mks), which the equilibrium pins of the MKS path cover; a
+    wrong Jacobian sign breaks it."""
+    mks = configs.torus2d(64, 32, (1, 1, 1))
+    om = O.OracleSim(mks, nthreads=4)
+    om.init()
+    l_mks = _specific_angular_momentum(om, 3, mks["a"], mks["h"], False, 0.05)
+    assert l_mks.std() < 1e-10 * abs(l_mks.mean())
+    cks = configs.fm_torus_cks(nx=32, nb=4, levels=2, extent=32.0)
+    O.plant(plant)
+    try:
+        oc = O.OracleSim(cks, nthreads=4)
+        oc.init()
+    finally:
+        O.plant("none")
+    l_cks = _specific_angular_momentum(oc, 2, cks["a"], cks["h"], True, 0.05)
+    err = np.abs(l_cks - l_mks.mean()).max() / abs(l_mks.mean())
+    if plant == "none":
+        assert err < 1e-10
+    else:
+        assert err > 1e-3
