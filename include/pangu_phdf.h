@@ -10,8 +10,9 @@
  *                            (nb, ncomp, nz, ny, nx) for several
  *   /VolumeLocations/{x,y,z} cell centres per block, (nb, n)
  *   /Locations/{x,y,z}       cell faces per block, (nb, n + 1)
- *   /LogicalLocations        int64 (nb, 3); /Blocks/loc.lx123 the same
- *   /Levels                  int32 (nb), all 0 (one level)
+ *   /LogicalLocations        int64 (nb, 3), each at its block's level;
+ *                            /Blocks/loc.lx123 the same
+ *   /Levels                  int32 (nb), refinement level over the root grid
  *   /Info                    attributes Time, NCycle, dt, NumDims,
  *                            NumMeshBlocks, MeshBlockSize, RootGridSize,
  *                            RootGridDomain, IncludesGhost (0), NGhost,
@@ -40,6 +41,7 @@ typedef struct pangu_phdf_mesh {
   const int64_t *lx;       /* [nblocks][3] logical block locations */
   const double *xmin;      /* [nblocks][3] lower interior corner of each block */
   const double *dx;        /* [nblocks][3] cell widths */
+  const int *level;        /* [nblocks] refinement levels, or NULL: all 0 */
 } pangu_phdf_mesh;
 
 typedef struct pangu_phdf_var {

@@ -140,12 +140,15 @@ extern "C" int pangu_write_phdf(const char *path, const pangu_phdf_mesh *m, int 
     H info(H5Gcreate2(file, "Info", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT), H5Gclose);
     const int one_i[1] = {0};
     const int ver = 3;
+    int max_level = 0;
+    for (int b = 0; m->level && b < nb; ++b)
+      if (m->level[b] > max_level) max_level = m->level[b];
     bool ok = info >= 0 && attr_i(info, "OutputFormatVersion", &ver, 1) &&
               attr_i(info, "NCycle", &m->ncycle, 1) && attr_d(info, "Time", &m->time, 1) &&
               attr_d(info, "dt", &m->dt, 1) && attr_i(info, "NumDims", &m->ndim, 1) &&
               attr_i(info, "NumMeshBlocks", &nb, 1) && attr_i(info, "MeshBlockSize", m->nx, 3) &&
               attr_i(info, "RootGridSize", m->root_nx, 3) && attr_i(info, "IncludesGhost", one_i, 1) &&
-              attr_i(info, "NGhost", &m->ng, 1) && attr_i(info, "MaxLevel", one_i, 1) &&
+              attr_i(info, "NGhost", &m->ng, 1) && attr_i(info, "MaxLevel", &max_level, 1) &&
               attr_s(info, "Coordinates", {"UniformCartesian"});
     double dom[9];
     for (int d = 0; d < 3; ++d) {
@@ -169,6 +172,7 @@ extern "C" int pangu_write_phdf(const char *path, const pangu_phdf_mesh *m, int 
     const hsize_t d2[2] = {hnb, 3};
     H blocks(H5Gcreate2(file, "Blocks", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT), H5Gclose);
     std::vector<int> levels(nb, 0);
+    if (m->level) levels.assign(m->level, m->level + nb);
     if (blocks < 0 || !dataset(blocks, "loc.lx123", H5T_NATIVE_INT64, H5T_STD_I64LE, m->lx, 2, d2) ||
         !dataset(file, "LogicalLocations", H5T_NATIVE_INT64, H5T_STD_I64LE, m->lx, 2, d2) ||
         !dataset(file, "Levels", H5T_NATIVE_INT, H5T_STD_I32LE, levels.data(), 1, &hnb))

@@ -18,7 +18,7 @@ class Mesh(C.Structure):
                 ("ncycle", C.c_int), ("time", C.c_double), ("dt", C.c_double),
                 ("root_nx", C.c_int * 3), ("root_min", C.c_double * 3), ("root_max", C.c_double * 3),
                 ("lx", C.POINTER(C.c_int64)), ("xmin", C.POINTER(C.c_double)),
-                ("dx", C.POINTER(C.c_double))]
+                ("dx", C.POINTER(C.c_double)), ("level", C.POINTER(C.c_int))]
 
 
 class Var(C.Structure):
@@ -68,6 +68,7 @@ def write(sim, path, variables=None, container=0, mesh=0, ncycle=0):
     flat = sim.get("prim", container)
     per_block = sim.blocks(flat.reshape(-1)) if sim.uniform else sim.blocks(flat)
     lx = np.zeros((nb, 3), dtype=np.int64)
+    level = np.zeros(nb, dtype=np.int32)
     xmin = np.zeros((nb, 3))
     dx = np.zeros((nb, 3))
     cells = []
@@ -75,11 +76,12 @@ def write(sim, path, variables=None, container=0, mesh=0, ncycle=0):
         b = first + bi
         xd, n = sim.block_geom(b)
         s = n[3:6]
+        loc = sim.block_loc(b)
+        level[bi] = loc[0]
         for d in range(3):
             dx[bi, d] = xd[3 + d]
             xmin[bi, d] = xd[d] + s[d] * xd[3 + d]
-            nxb = n[6 + d] - s[d] + 1
-            lx[bi, d] = int(round((xmin[bi, d] - cfg["xmin"][d]) / (nxb * dx[bi, d])))
+            lx[bi, d] = loc[1 + d]
         cells.append(sim.block_interior(per_block[b], b))
     nxb = [int(cells[0].shape[3]), int(cells[0].shape[2]), int(cells[0].shape[1])]
     arrays, vs = [], (Var * len(names))()
@@ -105,6 +107,7 @@ def write(sim, path, variables=None, container=0, mesh=0, ncycle=0):
     m.lx = lx.ctypes.data_as(C.POINTER(C.c_int64))
     m.xmin = xmin.ctypes.data_as(C.POINTER(C.c_double))
     m.dx = dx.ctypes.data_as(C.POINTER(C.c_double))
+    m.level = level.ctypes.data_as(C.POINTER(C.c_int))
     if lib().pangu_write_phdf(os.fsencode(path), C.byref(m), len(names), vs) != 0:
         raise RuntimeError(lib().pangu_phdf_last_error().decode())
     return path

@@ -124,6 +124,37 @@ def test_layout_and_values_round_trip(tmp_path):
     assert [int(x) for x in read_attr(path, "/Info", "MeshBlockSize")] == list(nxb)
 
 
+def test_levels_of_a_refined_mesh(tmp_path):
+    """Per-block refinement levels go to /Levels and their maximum to
+    /Info MaxLevel (NULL levels write zeros, as the round trip above)."""
+    phdf = _phdf()
+    nb, nxb = 3, (4, 4, 1)
+    lx = np.array([[0, 0, 0], [2, 0, 0], [3, 1, 0]], dtype=np.int64)
+    level = np.array([0, 1, 1], dtype=np.int32)
+    xmin = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [1.5, 0.5, 0.0]])
+    dx = np.array([[0.25, 0.25, 1.0], [0.125, 0.125, 1.0], [0.125, 0.125, 1.0]])
+    dens = np.arange(nb * 16, dtype=np.float64).reshape(nb, 1, 4, 4)
+    m = phdf.Mesh()
+    m.nblocks = nb
+    for d in range(3):
+        m.nx[d] = nxb[d]
+        m.root_nx[d] = (8, 4, 1)[d]
+        m.root_max[d] = (2.0, 1.0, 1.0)[d]
+    m.ng, m.ndim = 4, 2
+    m.lx = lx.ctypes.data_as(C.POINTER(C.c_int64))
+    m.xmin = xmin.ctypes.data_as(C.POINTER(C.c_double))
+    m.dx = dx.ctypes.data_as(C.POINTER(C.c_double))
+    m.level = level.ctypes.data_as(C.POINTER(C.c_int))
+    vs = (phdf.Var * 1)()
+    vs[0] = phdf.Var(b"density", 1, dens.ctypes.data_as(C.POINTER(C.c_double)))
+    path = str(tmp_path / "refined.phdf")
+    assert phdf.lib().pangu_write_phdf(path.encode(), C.byref(m), 1, vs) == 0
+    np.testing.assert_array_equal(read_dataset(path, "/Levels"), level)
+    np.testing.assert_array_equal(read_dataset(path, "/LogicalLocations"), lx)
+    assert int(read_attr(path, "/Info", "MaxLevel")[0]) == 1
+    np.testing.assert_array_equal(read_dataset(path, "/VolumeLocations/x")[2], 1.5 + (np.arange(4) + 0.5) * 0.125)
+
+
 def test_reference_plot_script_assembles_the_global_field(tmp_path):
     phdf = _phdf()
     path, dens, vel, lx, xmin, dxs, nxb = _synthetic(tmp_path, phdf)
