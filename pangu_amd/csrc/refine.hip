@@ -3,8 +3,8 @@
 // multilevel mesh: AddBoundaryExchangeTasks and AddFluxCorrectionTasks,
 // ideal_grmhd.cc:71-72,78).
 //
-// The host resolves the block tree into gather tables (api.hip); the kernels
-// run one table entry per thread:
+// The host resolves the block tree into gather tables (refine_host.cpp); the
+// kernels run one table entry per thread:
 // - k_xfill: a ghost cell covered by a block of its own level copies that
 //   block's interior cell, one covered by a finer block takes the mean of
 //   the 2^dim fine cells (interior cells of the finer block);
@@ -16,6 +16,11 @@
 // - k_flux_correct: a coarse face flux on a face shared with finer blocks
 //   becomes the mean of the 2^(dim-1) fine face fluxes.
 // Sums run in a fixed tree order, so results match the oracle bit for bit.
+// k_xfill runs one entry per thread over all components (its entries are
+// cheap copies: re-reading the entry per component costs more than it
+// hides); k_xprol and k_flux_correct run one entry per thread and component
+// (blockIdx.y; the recovery flag is the last y index), so their seven and
+// four gathers per value are independent across a wavefront.
 #include <hip/hip_runtime.h>
 
 #include "common.h"
@@ -77,27 +82,25 @@ __global__ __launch_bounds__(256) void k_xprol(KCtx C, KFields F, double *const 
   const XProl X = tab[e];
   const DevBlock &B = C.blk[X.b];
   const int dim = block_dim(B), sj = B.n[0], sk = B.n[0] * B.n[1];
-  const int np = C.np, ns = 1 + 2 * dim;
-  for (int q = 0; q < np; ++q) {
-    double v[7];
-    for (int s = 0; s < ns; ++s)
-      v[s] = X.sb[s] >= 0 ? glb(F.prim[X.sb[s] * np + q])[X.sc[s]]
-                          : fine_mean(glb(F.prim[X.b * np + q]), X.sc[s], dim, sj, sk);
-    double sl[3];
-    for (int d = 0; d < dim; ++d) sl[d] = 0.25 * minmod(v[0] - v[1 + 2 * d], v[2 + 2 * d] - v[0]);
-    const auto out = glb(F.prim[X.b * np + q]);
-    for (int c = 0; c < (1 << dim); ++c) {
-      if (!((X.mask >> c) & 1)) continue;
-      double w = v[0];
-      for (int d = 0; d < dim; ++d) w = ((c >> d) & 1) ? w + sl[d] : w - sl[d];
-      out[X.base + (c & 1) + ((c >> 1) & 1) * sj + ((c >> 2) & 1) * sk] = w;
-    }
-  }
-  if (flag) {
+  const int np = C.np, ns = 1 + 2 * dim, q = (int)blockIdx.y;
+  if (q == np) {  // the flag row (only when flag != null): copy of the coarse value
     const double f = glb(flag[X.sb[0]])[X.sc[0]];
     const auto out = glb(flag[X.b]);
     for (int c = 0; c < (1 << dim); ++c)
       if ((X.mask >> c) & 1) out[X.base + (c & 1) + ((c >> 1) & 1) * sj + ((c >> 2) & 1) * sk] = f;
+    return;
+  }
+  const auto own = glb(F.prim[X.b * np + q]);
+  double v[7];
+  for (int s = 0; s < ns; ++s)
+    v[s] = X.sb[s] >= 0 ? glb(F.prim[X.sb[s] * np + q])[X.sc[s]] : fine_mean(own, X.sc[s], dim, sj, sk);
+  double sl[3];
+  for (int d = 0; d < dim; ++d) sl[d] = 0.25 * minmod(v[0] - v[1 + 2 * d], v[2 + 2 * d] - v[0]);
+  for (int c = 0; c < (1 << dim); ++c) {
+    if (!((X.mask >> c) & 1)) continue;
+    double w = v[0];
+    for (int d = 0; d < dim; ++d) w = ((c >> d) & 1) ? w + sl[d] : w - sl[d];
+    own[X.base + (c & 1) + ((c >> 1) & 1) * sj + ((c >> 2) & 1) * sk] = w;
   }
 }
 
@@ -110,18 +113,16 @@ __global__ __launch_bounds__(256) void k_flux_correct(KCtx C, KFields F, const X
   const int stride[3] = {1, B.n[0], B.n[0] * B.n[1]};
   // tangential dims in increasing order
   const int a = stride[X.d == 0 ? 1 : 0], b = stride[X.d == 2 ? 1 : 2];
-  const int nc = C.nc, c = X.fface;
-  for (int q = 0; q < nc; ++q) {
-    const auto f = glb(F.flux[X.d][X.fb * nc + q]);
-    double m;
-    if (dim == 1)
-      m = f[c];
-    else if (dim == 2)
-      m = (f[c] + f[c + a]) * 0.5;
-    else
-      m = ((f[c] + f[c + a]) + (f[c + b] + f[c + a + b])) * 0.25;
-    glb(F.flux[X.d][X.cb * nc + q])[X.cface] = m;
-  }
+  const int nc = C.nc, c = X.fface, q = (int)blockIdx.y;
+  const auto f = glb(F.flux[X.d][X.fb * nc + q]);
+  double m;
+  if (dim == 1)
+    m = f[c];
+  else if (dim == 2)
+    m = (f[c] + f[c + a]) * 0.5;
+  else
+    m = ((f[c] + f[c + a]) + (f[c + b] + f[c + a + b])) * 0.25;
+  glb(F.flux[X.d][X.cb * nc + q])[X.cface] = m;
 }
 
 int launch_exchange_ml(const LaunchCtx &L, const KFields &F, double *const *flag, hipStream_t s) {
@@ -132,8 +133,8 @@ int launch_exchange_ml(const LaunchCtx &L, const KFields &F, double *const *flag
     if (L.clock) (void)L.clock->mark("exchange_fill", s);
   }
   if (M.nprol > 0) {
-    hipLaunchKernelGGL(k_xprol, dim3((M.nprol + 255) / 256), dim3(256), 0, s, L.k, F, flag, M.prol,
-                       M.nprol);
+    hipLaunchKernelGGL(k_xprol, dim3((M.nprol + 255) / 256, L.k.np + (flag ? 1 : 0)), dim3(256), 0, s, L.k,
+                       F, flag, M.prol, M.nprol);
     if (L.clock) (void)L.clock->mark("exchange_prolong", s);
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
@@ -142,7 +143,7 @@ int launch_exchange_ml(const LaunchCtx &L, const KFields &F, double *const *flag
 int launch_flux_correction(const LaunchCtx &L, const KFields &F, hipStream_t s) {
   const MultiLevel &M = L.ml;
   if (M.nflux > 0) {
-    hipLaunchKernelGGL(k_flux_correct, dim3((M.nflux + 255) / 256), dim3(256), 0, s, L.k, F, M.flux,
+    hipLaunchKernelGGL(k_flux_correct, dim3((M.nflux + 255) / 256, L.k.nc), dim3(256), 0, s, L.k, F, M.flux,
                        M.nflux);
     if (L.clock) (void)L.clock->mark("flux_correction", s);
   }
