@@ -136,3 +136,24 @@ def test_gpu_mass_conserved_across_levels():
     m0 = mass()
     h.step(10, "graph")
     assert abs(mass() - m0) / m0 < 1e-14
+
+
+@pytest.mark.gpu
+def test_refined_output_records_levels(tmp_path):
+    """The .phdf of a refined run lists each block's level and logical
+    location as the host driver holds them."""
+    import os
+    from test_phdf import H5DUMP, read_dataset
+    from pangu_amd import phdf
+    from pangu_amd.sim import HipSim
+    if not os.path.exists(H5DUMP):
+        pytest.skip("h5dump not present")
+    h = HipSim(CASES["ot2d_two_to_one"])
+    h.init()
+    h.step(1, "graph")
+    path = str(tmp_path / "refined.phdf")
+    phdf.write(h, path)
+    locs = _locs(h)
+    h.close()
+    np.testing.assert_array_equal(read_dataset(path, "/Levels"), [l[0] for l in locs])
+    np.testing.assert_array_equal(read_dataset(path, "/LogicalLocations"), [l[1:] for l in locs])
