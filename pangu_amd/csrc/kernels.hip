@@ -138,11 +138,16 @@ __global__ __launch_bounds__(256) void k_fix_cons(KCtx C, KFields F, uint8_t *co
       if (HK) p[KEL] = e_exc;
       mk |= 16;
     }
-    __builtin_nontemporal_store(p[RHO], &pp[0][id]);
-    __builtin_nontemporal_store(p[ENY], &pp[1][id]);
-    __builtin_nontemporal_store(p[UX1], &pp[2][id]);
-    __builtin_nontemporal_store(p[UX2], &pp[3][id]);
-    __builtin_nontemporal_store(p[UX3], &pp[4][id]);
+    // rho, u and u~ change only where a floor, the Lorentz cap or excision
+    // set a mask bit; elsewhere memory already holds them (the entropy is
+    // recomputed in every cell)
+    if (mk) {
+      __builtin_nontemporal_store(p[RHO], &pp[0][id]);
+      __builtin_nontemporal_store(p[ENY], &pp[1][id]);
+      __builtin_nontemporal_store(p[UX1], &pp[2][id]);
+      __builtin_nontemporal_store(p[UX2], &pp[3][id]);
+      __builtin_nontemporal_store(p[UX3], &pp[4][id]);
+    }
     __builtin_nontemporal_store(p[ENT], &pp[ient<HB>()][id]);
     if (HK) pp[ikel<HB>()][id] = p[KEL];
     if (mask) glb(mask[b])[id] = mk;
