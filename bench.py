@@ -58,7 +58,7 @@ TUNED = {"order": {"fix_cons": 1, "flux_x1": 1, "flux_x2": 0, "flux_x3": 1, "ct"
 # for a plain copy kernel of known size (profiles/counters_torus3d_256x128x128.md).
 MEASURED_TRAFFIC = {
     "div_update_source": 2063902361, "flux_x1": 881801660, "flux_x2": 931134937,
-    "flux_x3": 857023605, "fix_cons": 1222290595, "recovery_heating": 1121017255,
+    "flux_x3": 857023605, "fix_cons": 1120759286, "recovery_heating": 1121017255,
     "ct_emf": 353569598,
 }
 
