@@ -337,6 +337,8 @@ static int init_torus(Sim &S) {
 }
 
 // ---- fm_torus_cks (hydrodynamic: the reference input has enable_B = false)
+// Pinned by tests/test_refinement.py: Fishbone-Moncrief's u^t u_phi is the
+// same in every torus cell and equals the MKS torus's (init_torus above).
 // fm_torus_cks/problem_generator.cpp:37-61: Boyer-Lindquist metric at (r, th).
 static void bl_metric_rth(double r, double th, double a, double g[4][4]) {
   for (int row = 0; row < 4; ++row)

@@ -21,6 +21,10 @@
 //    mean of the 2^(dim-1) fine face fluxes (equal face areas).
 // The summation orders are fixed (x1 pairs first, then x2, then x3); the
 // recovery flag restricts by minimum and prolongates by copy.
+// Pinned by tests/test_refinement.py (hand-counted trees, exact exchange of
+// linear data, an everywhere-refined mesh equal to the uniform mesh, mass
+// conservation across levels).  Parity with Parthenon's own code is unpinned:
+// that code is absent from the reference checkout.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
