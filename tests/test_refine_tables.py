@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 import oracle_py as O
+from pangu_amd import configs
 from test_refinement import cks_smr_mesh
 from test_refinement_gpu import CASES
 
@@ -42,6 +43,9 @@ def _ip(a):
 
 CHECK_CASES = dict(CASES)
 CHECK_CASES["fm_torus_cks_tree"] = dict(cks_smr_mesh(), metric=0, excise=0)
+# a refined quadrant at the corner of a periodic box: stencils wrap
+CHECK_CASES["ot2d_periodic_corner"] = dict(configs.orszag_tang(32, (2, 2, 1)),
+                                           refine=[(1, (0.5, 0.5, -0.5), (1.0, 1.0, 0.5))])
 
 
 @pytest.mark.parametrize("name", sorted(CHECK_CASES))
