@@ -56,6 +56,7 @@ typedef struct pangu_sim_cfg {
   int nref;
   int ref_level[PANGU_SIM_MAX_REF];
   double ref_min[PANGU_SIM_MAX_REF][3], ref_max[PANGU_SIM_MAX_REF][3];
+  int magnetic_mad;    /* CKS torus field: 0 sane, 1 mad (<fm_torus>/magnetic_field) */
 } pangu_sim_cfg;
 
 enum { PANGU_SIM_TASKS = 0, PANGU_SIM_FUSED = 1, PANGU_SIM_GRAPH = 2 };

@@ -104,7 +104,8 @@ enum Plant {
   PLANT_EMT_PRESSURE = 6, // T^mu_nu: drop the magnetic pressure
   PLANT_NO_FLUX_CORRECTION = 7,  // refined mesh: skip flux correction
   PLANT_PROLONG_SLOPE = 8,       // prolongation: +-1/2 instead of +-1/4 of the slope
-  PLANT_CKS_JACOBIAN = 9         // CKS torus: wrong sign of dx/dphi in the KS -> CKS Jacobian
+  PLANT_CKS_JACOBIAN = 9,        // CKS torus: wrong sign of dx/dphi in the KS -> CKS Jacobian
+  PLANT_CKS_CURL = 10            // CKS torus field: wrong sign of B^1 = -dA2/dz
 };
 
 // Kokkos::max / Kokkos::min semantics (ternary on operator<).

@@ -52,6 +52,7 @@ typedef struct oracle_cfg {
   int nref;
   int ref_level[ORACLE_MAX_REF];
   double ref_min[ORACLE_MAX_REF][3], ref_max[ORACLE_MAX_REF][3];
+  int magnetic_mad;    /* CKS torus field: 0 sane, 1 mad */
 } oracle_cfg;
 
 /* fields for oracle_sim_get / oracle_sim_set */

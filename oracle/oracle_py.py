@@ -42,6 +42,7 @@ class OracleCfg(C.Structure):
         ("seed", C.c_uint64), ("nthreads", C.c_int),
         ("nref", C.c_int), ("ref_level", C.c_int * MAX_REF),
         ("ref_min", (C.c_double * 3) * MAX_REF), ("ref_max", (C.c_double * 3) * MAX_REF),
+        ("magnetic_mad", C.c_int),
     ]
 
 
@@ -294,7 +295,7 @@ class OracleSim:
 
 PLANTS = dict(none=0, conn_index=1, shift_sign=2, ppm_term=3, kastaun_qbar=4, ct_sign=5,
               emt_pressure=6, no_flux_correction=7, prolong_slope=8,
-              cks_jacobian=9)
+              cks_jacobian=9, cks_curl=10)
 
 
 def plant(name):

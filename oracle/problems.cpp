@@ -411,7 +411,7 @@ static void bl_to_cks_ucon(double x, double y, double z, double a, const double 
 // pre-normalisation value, as in the reference.
 static int init_torus_cks(Sim &S) {
   const oracle_cfg &c = S.cfg;
-  if (c.metric != MET_CKS || S.p.enable_B) return -1;
+  if (c.metric != MET_CKS) return -1;
   const double gam = c.gamma, a = c.a, a2 = a * a;
   const double rin = c.rin, kappa = c.kappa, pert = c.perturbation;
   const double l = lfish_calc(c.rmax, a);
@@ -523,6 +523,81 @@ static int init_torus_cks(Sim &S) {
           if (S.Pc(0, b, 0)[id] < rho_floor) S.Pc(0, b, 0)[id] = rho_floor;
           if (S.Pc(0, b, 1)[id] < eng_floor) S.Pc(0, b, 1)[id] = eng_floor;
         }
+  }
+  if (!S.p.enable_B) return 0;
+  // Magnetic field (:476-738).  Vector potential at cell centres over
+  // entire blocks from the normalised, floored density (:519-545).
+  for (int b = 0; b < S.nblk; ++b) {
+    const Block &B = S.blk[b];
+    std::vector<double> A1((size_t)B.cells), A2((size_t)B.cells);
+    for (int k = 0; k < B.n[2]; ++k)
+      for (int j = 0; j < B.n[1]; ++j)
+        for (int i = 0; i < B.n[0]; ++i) {
+          const double rho = S.Pc(0, b, 0)[B.idx(k, j, i)];
+          double ascale = (rho > c.aphi_cut) ? (rho - c.aphi_cut) : 0.0;
+          if (c.magnetic_mad && ascale > 0.0) {
+            const double x = Xc(B, 0, i), y = Xc(B, 1, j), zc = Xc(B, 2, k);
+            const double rad2 = x * x + y * y + zc * zc;
+            const double r = std::fmax(
+                std::sqrt((rad2 - a2 + std::sqrt((rad2 - a2) * (rad2 - a2) + 4.0 * a2 * zc * zc)) / 2.0), 1.0);
+            const double cos_theta = std::fmin(std::fmax(zc / r, -1.0), 1.0);
+            const double sin_theta = std::sqrt(1.0 - cos_theta * cos_theta);
+            const double mad_scale = std::pow((r / c.rin) * sin_theta, 3.0) * std::exp(-r / 400.0);
+            ascale *= mad_scale;
+          }
+          A1[(size_t)B.idx(k, j, i)] = -Xc(B, 1, j) * ascale;
+          A2[(size_t)B.idx(k, j, i)] = Xc(B, 0, i) * ascale;
+        }
+    // B = curl A / sqrt|g| on the interior (:548-578)
+    for (int k = B.s[2]; k <= B.e[2]; ++k)
+      for (int j = B.s[1]; j <= B.e[1]; ++j)
+        for (int i = B.s[0]; i <= B.e[0]; ++i) {
+          const double xc[4] = {0.0, Xc(B, 0, i), Xc(B, 1, j), Xc(B, 2, k)};
+          double gcov[4][4];
+          code_metric(MET_CKS, xc, gcov, 0.0, a);
+          const double sqrt_abs_gdet = std::sqrt(std::fabs(determinant4(gcov)));
+          const double dx1 = B.dx[0], dx2 = B.dx[1], dx3 = B.dx[2];
+          const double dA2_dx = (A2[(size_t)B.idx(k, j, i + 1)] - A2[(size_t)B.idx(k, j, i - 1)]) / (2.0 * dx1);
+          const double dA1_dy = (A1[(size_t)B.idx(k, j + 1, i)] - A1[(size_t)B.idx(k, j - 1, i)]) / (2.0 * dx2);
+          const double dA2_dz = (A2[(size_t)B.idx(k + 1, j, i)] - A2[(size_t)B.idx(k - 1, j, i)]) / (2.0 * dx3);
+          const double dA1_dz = (A1[(size_t)B.idx(k + 1, j, i)] - A1[(size_t)B.idx(k - 1, j, i)]) / (2.0 * dx3);
+          const int64_t id = B.idx(k, j, i);
+          S.Pc(0, b, S.iBX)[id] = (g_plant == PLANT_CKS_CURL ? dA2_dz : -dA2_dz) / sqrt_abs_gdet;
+          S.Pc(0, b, S.iBX + 1)[id] = dA1_dz / sqrt_abs_gdet;
+          S.Pc(0, b, S.iBX + 2)[id] = (dA2_dx - dA1_dy) / sqrt_abs_gdet;
+        }
+  }
+  // beta normalisation (:581-730): max b^2 over interiors, max u and rho
+  // over entire blocks
+  double bsq_max = 0.0, umax_n = 0.0, rhomax_n = 0.0;
+  for (int b = 0; b < S.nblk; ++b) {
+    const Block &B = S.blk[b];
+    for (int k = B.s[2]; k <= B.e[2]; ++k)
+      for (int j = B.s[1]; j <= B.e[1]; ++j)
+        for (int i = B.s[0]; i <= B.e[0]; ++i) {
+          const double xc[4] = {0.0, Xc(B, 0, i), Xc(B, 1, j), Xc(B, 2, k)};
+          double gcov[4][4], gcon[4][4];
+          code_metric(MET_CKS, xc, gcov, 0.0, a);
+          invert4(gcov, gcon);
+          real pc[NPRIM];
+          S.load_prim(0, b, B.idx(k, j, i), pc);
+          State st;
+          calc_state(pc, gcov, gcon, st);
+          bsq_max = kmax(bsq_max, st.bsq);
+        }
+    for (int64_t id = 0; id < B.cells; ++id) {
+      umax_n = kmax(umax_n, S.Pc(0, b, 1)[id]);
+      rhomax_n = kmax(rhomax_n, S.Pc(0, b, 0)[id]);
+    }
+  }
+  const double beta_min = 2.0 * (gam - 1.0) * (umax_n / rhomax_n) / (bsq_max + 1.0e-30);
+  const double magnetic_norm = std::sqrt(beta_min / c.beta);
+  for (int b = 0; b < S.nblk; ++b) {
+    const Block &B = S.blk[b];
+    for (int k = B.s[2]; k <= B.e[2]; ++k)
+      for (int j = B.s[1]; j <= B.e[1]; ++j)
+        for (int i = B.s[0]; i <= B.e[0]; ++i)
+          for (int q = 0; q < 3; ++q) S.Pc(0, b, S.iBX + q)[B.idx(k, j, i)] *= magnetic_norm;
   }
   return 0;
 }

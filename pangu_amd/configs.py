@@ -19,7 +19,7 @@ _BASE = dict(
     excise=0, r_excise=1.0, dexcise=1.0e-8, pexcise=0.333e-12,
     stage_init=0, exchange_after=0, cache_geometry=1,
     problem=1, rin=6.0, rmax=12.0, kappa=1.0e-3, perturbation=4.0e-2, beta=100.0,
-    aphi_cut=0.2, seed=0,
+    aphi_cut=0.2, seed=0, magnetic_mad=0,
 )
 
 PERIODIC, OUTFLOW, REFLECT = 0, 1, 2
@@ -103,19 +103,22 @@ def ensemble(seed=0, n_tori=96, n_boxes_each=121, sizes3d=(16, 32, 64),
     return out
 
 
-def fm_torus_cks(nx=64, nb=4, levels=4, extent=64.0):
+def fm_torus_cks(nx=64, nb=4, levels=4, extent=64.0, enable_B=0, mad=0):
     """The reference's static-refinement run (fm_torus_cks/inputfile): a
     hydrodynamic FM torus (rin 6, rmax 12) around a spinning black hole
     (a = 0.9375) in Cartesian Kerr-Schild coordinates with excision,
     LAXF + PPM4, Kastaun recovery, gamma 4/3, CFL 0.3; nx^3 root cells over
     [-extent, extent]^3 in nb^3 blocks, refined to `levels` nested cubes
-    [-extent/2^L, extent/2^L]^3."""
+    [-extent/2^L, extent/2^L]^3.  enable_B adds the reference's SANE (or,
+    with mad, MAD) vector-potential field at beta = 100; the reference input
+    runs without it."""
     d = dict(_BASE)
     d.update(nx=[nx] * 3, nb=[nb] * 3, xmin=[-extent] * 3, xmax=[extent] * 3,
              bc=[[OUTFLOW, OUTFLOW]] * 3, dt_init=1e-5, dt_factor=1.3,
              metric=METRIC["cks"], a=0.9375, h=0.7, gamma=4.0 / 3.0, cfl=0.3,
              riemann=RIEMANN["laxf"], limiter=LIMITER["ppm4"], recovery=RECOVERY["robuster"],
-             enable_B=0, enable_heating=0, density_floor=1e-5, energy_floor=1e-7,
+             enable_B=enable_B, magnetic_mad=mad, enable_heating=0, density_floor=1e-5,
+             energy_floor=1e-7,
              density_floor_pow=-1.5, energy_floor_pow=-2.5,
              excise=1, r_excise=1.0, dexcise=1.0e-8, pexcise=0.333e-10,
              problem=3, rin=6.0, rmax=12.0, kappa=1e-3, perturbation=2e-2)

@@ -62,10 +62,11 @@ void fill_face_table(const pangu_config &cfg, const HostBlock &H, int d, int fi,
 // Initial data written into host arrays laid out [block][comp][cell]
 // (entire ghosted blocks), for the "base" container.
 struct ProblemParams {
-  int problem;  // 1 Orszag-Tang (unit box), 2 FM torus
+  int problem;  // 1 Orszag-Tang (unit box), 2 FM torus (MKS), 3 FM torus (CKS)
   double rin, rmax, kappa, perturbation, beta, aphi_cut, fel_0;
   uint64_t seed;
   int mesh_nx[3];
+  int mad;      // CKS torus field: 0 sane, 1 mad
 };
 int generate_problem(const pangu_config &cfg, const ProblemParams &pp,
                      const std::vector<HostBlock> &blocks, int np, int iBX, int iENT, int iKEL,

@@ -145,7 +145,7 @@ static void ks_r_th(double x, double y, double z, double a2, double &r, double &
 // (excision values inside r_excise).  Entropy keeps its pre-normalisation
 // value, as the reference leaves it.
 static int torus_cks(const pangu_config &cfg, const ProblemParams &pp,
-                     const std::vector<HostBlock> &blocks, int iENT, int iKEL,
+                     const std::vector<HostBlock> &blocks, int iBX, int iENT, int iKEL,
                      const std::function<double *(size_t, int)> &P) {
   const double gam = cfg.gamma, a = cfg.a, a2 = a * a;
   const double rin = pp.rin, kap = pp.kappa;
@@ -261,6 +261,83 @@ static int torus_cks(const pangu_config &cfg, const ProblemParams &pp,
           if (P(b, 1)[id] < ef) P(b, 1)[id] = ef;
         }
   }
+  if (iBX < 0) return 0;
+  // Field (:476-738): A = (-y, x, 0) max(rho - cut, 0) at cell centres (MAD:
+  // times ((r / rin) sin(theta))^3 exp(-r / 400)), B = curl A / sqrt|g| by
+  // central differences over the interior, then scaled so that
+  // 2 (gamma - 1) (u_max / rho_max) / max b^2 = beta.
+  for (size_t b = 0; b < blocks.size(); ++b) {
+    const HostBlock &B = blocks[b];
+    std::vector<double> A1((size_t)B.cells), A2((size_t)B.cells);
+    for (int k = 0; k < B.n[2]; ++k)
+      for (int j = 0; j < B.n[1]; ++j)
+        for (int i = 0; i < B.n[0]; ++i) {
+          const int64_t id = B.id(k, j, i);
+          const double rho = P(b, 0)[id];
+          double ascale = (rho > pp.aphi_cut) ? (rho - pp.aphi_cut) : 0.0;
+          if (pp.mad && ascale > 0.0) {
+            const double x = B.xc(0, i), y = B.xc(1, j), zc = B.xc(2, k);
+            const double rad2 = x * x + y * y + zc * zc;
+            const double r = std::fmax(
+                std::sqrt((rad2 - a2 + std::sqrt((rad2 - a2) * (rad2 - a2) + 4.0 * a2 * zc * zc)) / 2.0), 1.0);
+            const double cth = std::fmin(std::fmax(zc / r, -1.0), 1.0);
+            const double sth = std::sqrt(1.0 - cth * cth);
+            ascale *= std::pow((r / rin) * sth, 3.0) * std::exp(-r / 400.0);
+          }
+          A1[(size_t)id] = -B.xc(1, j) * ascale;
+          A2[(size_t)id] = B.xc(0, i) * ascale;
+        }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int k = B.s[2]; k <= B.e[2]; ++k)
+      for (int j = B.s[1]; j <= B.e[1]; ++j)
+        for (int i = B.s[0]; i <= B.e[0]; ++i) {
+          const double xc[4] = {0.0, B.xc(0, i), B.xc(1, j), B.xc(2, k)};
+          geom::M4 g;
+          geom::code_metric(PANGU_METRIC_CKS, xc, g, 0.0, a);
+          const double sg = std::sqrt(std::fabs(geom::det4(g)));
+          const int64_t id = B.id(k, j, i), sj = B.n[0], sk = (int64_t)B.n[0] * B.n[1];
+          const double dA2_dx = (A2[(size_t)(id + 1)] - A2[(size_t)(id - 1)]) / (2.0 * B.dx[0]);
+          const double dA1_dy = (A1[(size_t)(id + sj)] - A1[(size_t)(id - sj)]) / (2.0 * B.dx[1]);
+          const double dA2_dz = (A2[(size_t)(id + sk)] - A2[(size_t)(id - sk)]) / (2.0 * B.dx[2]);
+          const double dA1_dz = (A1[(size_t)(id + sk)] - A1[(size_t)(id - sk)]) / (2.0 * B.dx[2]);
+          P(b, iBX)[id] = -dA2_dz / sg;
+          P(b, iBX + 1)[id] = dA1_dz / sg;
+          P(b, iBX + 2)[id] = (dA2_dx - dA1_dy) / sg;
+        }
+  }
+  double bsqmax = 0.0, umax = 0.0, rmax_n = 0.0;
+  for (size_t b = 0; b < blocks.size(); ++b) {
+    const HostBlock &B = blocks[b];
+    double bm = 0.0;
+    // max of non-negative values: order-independent
+#pragma omp parallel for schedule(dynamic, 1) reduction(max : bm)
+    for (int k = B.s[2]; k <= B.e[2]; ++k)
+      for (int j = B.s[1]; j <= B.e[1]; ++j)
+        for (int i = B.s[0]; i <= B.e[0]; ++i) {
+          const double xc[4] = {0.0, B.xc(0, i), B.xc(1, j), B.xc(2, k)};
+          geom::M4 g, gi_;
+          geom::code_metric(PANGU_METRIC_CKS, xc, g, 0.0, a);
+          geom::inv4(g, gi_);
+          const int64_t id = B.id(k, j, i);
+          double p[10] = {P(b, 0)[id], P(b, 1)[id], P(b, 2)[id], P(b, 3)[id], P(b, 4)[id],
+                          P(b, iBX)[id], P(b, iBX + 1)[id], P(b, iBX + 2)[id], 0.0, 0.0};
+          bm = kmax(bm, bsq_of(p, g, gi_));
+        }
+    bsqmax = kmax(bsqmax, bm);
+    for (int64_t id = 0; id < B.cells; ++id) {
+      umax = kmax(umax, P(b, 1)[id]);
+      rmax_n = kmax(rmax_n, P(b, 0)[id]);
+    }
+  }
+  const double beta_min = 2.0 * (gam - 1.0) * (umax / rmax_n) / (bsqmax + 1.0e-30);
+  const double norm = std::sqrt(beta_min / pp.beta);
+  for (size_t b = 0; b < blocks.size(); ++b) {
+    const HostBlock &B = blocks[b];
+    for (int k = B.s[2]; k <= B.e[2]; ++k)
+      for (int j = B.s[1]; j <= B.e[1]; ++j)
+        for (int i = B.s[0]; i <= B.e[0]; ++i)
+          for (int q = 0; q < 3; ++q) P(b, iBX + q)[B.id(k, j, i)] *= norm;
+  }
   return 0;
 }
 
@@ -295,8 +372,8 @@ int generate_problem(const pangu_config &cfg, const ProblemParams &pp,
     return 0;
   }
   if (pp.problem == 3) {
-    if (cfg.metric != PANGU_METRIC_CKS || iBX >= 0) return -1;
-    return torus_cks(cfg, pp, blocks, iENT, iKEL, P);
+    if (cfg.metric != PANGU_METRIC_CKS) return -1;
+    return torus_cks(cfg, pp, blocks, iBX, iENT, iKEL, P);
   }
   if (pp.problem != 2 || cfg.metric != PANGU_METRIC_MKS) return -1;
 

@@ -488,6 +488,7 @@ int pangu_sim_init(pangu_sim *s) {
     pp.aphi_cut = cm.aphi_cut;
     pp.fel_0 = cm.fel_0;
     pp.seed = cm.seed;
+    pp.mad = cm.magnetic_mad;
     for (int d = 0; d < 3; ++d) pp.mesh_nx[d] = cm.nx[d];
     const pangu_config pc = physics_of(&cm);
     const std::vector<HostBlock> hb(s->hb.begin() + first, s->hb.begin() + last);

@@ -55,6 +55,7 @@ class SimCfg(C.Structure):
         ("seed", C.c_uint64), ("nthreads", C.c_int),
         ("nref", C.c_int), ("ref_level", C.c_int * MAX_REF),
         ("ref_min", (C.c_double * 3) * MAX_REF), ("ref_max", (C.c_double * 3) * MAX_REF),
+        ("magnetic_mad", C.c_int),
     ]
 
 

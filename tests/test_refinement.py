@@ -236,3 +236,78 @@ def test_cks_torus_has_the_mks_torus_angular_momentum(plant):
         assert err < 1e-10
     else:
         assert err > 1e-3
+
+
+def _cks_mhd_torus(mad=0):
+    return configs.fm_torus_cks(nx=32, nb=2, levels=0, extent=32.0, enable_B=1, mad=mad)
+
+
+@pytest.mark.parametrize("plant", ["none", "cks_curl"])
+def test_cks_torus_field_is_divergence_free(plant):
+    """B = curl A / sqrt|g| by central differences (problem_generator.cpp:
+    548-578): the central-difference divergence of sqrt|g| B is then zero
+    identically (central differences commute), to round-off, over interior
+    cells whose stencil stays in their block.  A wrong curl sign breaks it."""
+    cfg = _cks_mhd_torus()
+    O.plant(plant)
+    try:
+        o = O.OracleSim(cfg, nthreads=4)
+        o.init()
+    finally:
+        O.plant("none")
+    P = o.get("prim", 0)
+    iB = 5
+    worst, scale = 0.0, 0.0
+    for b in range(o.nblocks):
+        xd, n = o.block_geom(b)
+        s, e = n[3:6], n[6:9]
+        dx = xd[3:6]
+        X = [xd[d] + (np.arange(n[d]) + 0.5) * xd[3 + d] for d in range(3)]
+        sg = np.zeros(tuple(int(v) for v in n[2::-1][:3]))
+        # sqrt|g| at the centres the divergence reads (interior of the block)
+        sg[...] = np.nan
+        for k in range(s[2], e[2] + 1):
+            for j in range(s[1], e[1] + 1):
+                for i in range(s[0], e[0] + 1):
+                    _, _, gdet, _ = O.metric_at(2, [0.0, X[0][i], X[1][j], X[2][k]], cfg["a"], 0.0)
+                    sg[k, j, i] = np.sqrt(abs(gdet))
+        F = [P[b, iB + d] * sg for d in range(3)]
+        sl = (slice(s[2] + 1, e[2]), slice(s[1] + 1, e[1]), slice(s[0] + 1, e[0]))
+        div = np.zeros_like(F[0][sl])
+        for d, ax in enumerate((2, 1, 0)):
+            hi = [slice(a.start, a.stop) for a in sl]
+            lo = [slice(a.start, a.stop) for a in sl]
+            hi[ax] = slice(sl[ax].start + 1, sl[ax].stop + 1)
+            lo[ax] = slice(sl[ax].start - 1, sl[ax].stop - 1)
+            div += (F[d][tuple(hi)] - F[d][tuple(lo)]) / (2.0 * dx[d])
+        worst = max(worst, np.abs(div).max())
+        scale = max(scale, max(np.abs(F[d][sl]).max() for d in range(3)) / min(dx))
+    assert scale > 0
+    if plant == "none":
+        assert worst <= 1e-12 * scale
+    else:
+        assert worst > 1e-3 * scale
+
+
+def test_cks_torus_field_meets_beta():
+    """After the normalisation (:712-730) the largest b^2 over the interior
+    is 2 (gamma - 1) (u_max / rho_max) / beta, with the maxima over entire
+    blocks (:653-704)."""
+    cfg = _cks_mhd_torus()
+    o = O.OracleSim(cfg, nthreads=4)
+    o.init()
+    P = o.get("prim", 0)
+    umax, rhomax = P[:, 1].max(), P[:, 0].max()
+    bsq = 0.0
+    for b in range(o.nblocks):
+        xd, n = o.block_geom(b)
+        s, e = n[3:6], n[6:9]
+        for k, j, i in np.argwhere(np.abs(P[b, 5:8, s[2]:e[2] + 1, s[1]:e[1] + 1, s[0]:e[0] + 1]).sum(0) > 0):
+            k, j, i = k + s[2], j + s[1], i + s[0]
+            x = [0.0] + [xd[d] + ((i, j, k)[d] + 0.5) * xd[3 + d] for d in range(3)]
+            gcov, gcon, _, _ = O.metric_at(2, x, cfg["a"], 0.0)
+            p = np.zeros(10)
+            p[0:8] = P[b, 0:8, k, j, i]
+            bsq = max(bsq, O.state(p, gcov, gcon)["bsq"])
+    want = 2.0 * (cfg["gamma"] - 1.0) * (umax / rhomax) / cfg["beta"]
+    assert bsq > 0 and abs(bsq / want - 1.0) < 1e-12

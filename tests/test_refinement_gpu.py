@@ -57,6 +57,9 @@ CASES = {
     # state of the excised cells next to them, are non-finite in the oracle
     # as in the reference; the comparison requires them at the same places.
     "fm_torus_cks_2level": configs.fm_torus_cks(nx=32, nb=4, levels=2, extent=32.0),
+    # the same with the problem generator's SANE field (CT and flux
+    # correction of the B fluxes across levels)
+    "fm_torus_cks_2level_mhd": configs.fm_torus_cks(nx=32, nb=4, levels=2, extent=32.0, enable_B=1),
 }
 
 
