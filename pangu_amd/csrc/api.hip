@@ -316,19 +316,33 @@ static int build_neighbours(pangu_ctx *c) {
   return upload(c, &c->dnbr, nbr.data(), nbr.size());
 }
 
-// Static mesh refinement: checks that the blocks tile each mesh and uploads
-// the gather tables of the cross-level exchange and of flux correction.
-static int build_multilevel(pangu_ctx *c) {
-  if (!refined_mesh(c->hb)) return 0;
-  std::string why = check_tiling(c->hb, c->bmesh, c->mcfg);
-  if (!why.empty()) return fail(c, PANGU_EINVAL, why);
+// Static mesh refinement, on the host before any device work: checks that
+// the blocks tile each mesh and resolves the gather tables of the
+// cross-level exchange and of flux correction ("" or why the mesh cannot
+// be handled).
+struct RefineTables {
   std::vector<XFill> fill;
   std::vector<XProl> prol;
   std::vector<XFlux> flux;
-  why = build_refinement_tables(c->hb, c->bmesh, c->mcfg, fill, prol, flux);
-  if (!why.empty()) return fail(c, PANGU_EINVAL, why);
-  if (fill.size() > (size_t)0x7fffff00 || prol.size() > (size_t)0x7fffff00)
-    return fail(c, PANGU_EINVAL, "refinement tables exceed 2^31 entries");
+};
+static std::string plan_multilevel(const std::vector<HostBlock> &hb, const std::vector<int> &bmesh,
+                                   const std::vector<pangu_config> &mcfg, RefineTables &T) {
+  if (!refined_mesh(hb)) return "";
+  std::string why = check_tiling(hb, bmesh, mcfg);
+  if (!why.empty()) return why;
+  why = build_refinement_tables(hb, bmesh, mcfg, T.fill, T.prol, T.flux);
+  if (!why.empty()) return why;
+  if (T.fill.size() > (size_t)0x7fffff00 || T.prol.size() > (size_t)0x7fffff00)
+    return "refinement tables exceed 2^31 entries";
+  return "";
+}
+
+// Uploads the refinement tables (nothing on a single-level mesh).
+static int build_multilevel(pangu_ctx *c, const RefineTables &T) {
+  if (!refined_mesh(c->hb)) return 0;
+  const std::vector<XFill> &fill = T.fill;
+  const std::vector<XProl> &prol = T.prol;
+  const std::vector<XFlux> &flux = T.flux;
   if (upload(c, &c->dxfill, fill.data(), fill.size()) || upload(c, &c->dxprol, prol.data(), prol.size()) ||
       upload(c, &c->dxflux, flux.data(), flux.size()))
     return PANGU_EDEVICE;
@@ -534,6 +548,20 @@ int pangu_create_batch(const pangu_config *cfgs, int nmesh, const pangu_block *b
     fprintf(stderr, "pangu_create: %s\n", msg.c_str());
     return PANGU_EINVAL;
   }
+  RefineTables rt;
+  {
+    std::vector<HostBlock> hb;
+    std::vector<int> bm(nblocks);
+    for (int b = 0; b < nblocks; ++b) {
+      hb.push_back(make_host_block(blocks[b]));
+      bm[b] = block_mesh ? block_mesh[b] : 0;
+    }
+    msg = plan_multilevel(hb, bm, std::vector<pangu_config>(cfgs, cfgs + nmesh), rt);
+    if (!msg.empty()) {
+      fprintf(stderr, "pangu_create: %s\n", msg.c_str());
+      return PANGU_EINVAL;
+    }
+  }
   pangu_ctx *c = new pangu_ctx();
   const pangu_config *cfg = &cfgs[0];
   c->cfg = *cfg;
@@ -568,7 +596,7 @@ int pangu_create_batch(const pangu_config *cfgs, int nmesh, const pangu_block *b
   for (int b = 0; b < nblocks; ++b) c->hb.push_back(make_host_block(blocks[b]));
   int r = build_geometry(c, blocks);
   if (!r) r = build_neighbours(c);
-  if (!r) r = build_multilevel(c);
+  if (!r) r = build_multilevel(c, rt);
   if (!r) r = build_tables(c);
   if (!r) r = upload(c, &c->dred, std::vector<double>((size_t)c->nplanes * 3, 0.0).data(),
                      (size_t)c->nplanes * 3);
@@ -778,16 +806,24 @@ int pangu_calculate_conservative(pangu_ctx *c, int md, void *stream) {
   return PANGU_OK;
 }
 
-int pangu_calculate_fluxes(pangu_ctx *c, int md, void *stream) {
-  if (int r = check_md(c, md)) return r;
+// Entry points that read or write face fluxes need the container's flux
+// tables on every active direction.
+static int need_fluxes(pangu_ctx *c, int md) {
   for (int d = 0; d < c->ndim; ++d)
     if (!c->conts[md].dflux[d]) return fail(c, PANGU_EINVAL, "container has no flux tables");
+  return 0;
+}
+
+int pangu_calculate_fluxes(pangu_ctx *c, int md, void *stream) {
+  if (int r = check_md(c, md)) return r;
+  if (int r = need_fluxes(c, md)) return r;
   LCHK(c, launch_fluxes(c->L, c->conts[md].kf(), c->dalfven, STREAM(stream)));
   return PANGU_OK;
 }
 
 int pangu_constrained_transport(pangu_ctx *c, int md, void *stream) {
   if (int r = check_md(c, md)) return r;
+  if (int r = need_fluxes(c, md)) return r;
   if (!c->cfg.enable_B) return PANGU_OK;
   LCHK(c, launch_ct(c->L, c->conts[md].kf(), c->demf, true, STREAM(stream)));
   return PANGU_OK;
@@ -795,6 +831,7 @@ int pangu_constrained_transport(pangu_ctx *c, int md, void *stream) {
 
 int pangu_flux_divergence(pangu_ctx *c, int md_in, int md_dudt, void *stream) {
   if (int r = check_md(c, md_in)) return r;
+  if (int r = need_fluxes(c, md_in)) return r;
   if (int r = check_md(c, md_dudt)) return r;
   const KFields f = c->conts[md_in].kf();
   LCHK(c, launch_div_update(c->L, 1, f, f, f, c->conts[md_dudt].dcons, nullptr, nullptr, 0.0,
@@ -886,6 +923,7 @@ int pangu_exchange_ghosts(pangu_ctx *c, int md, void *stream) {
 
 int pangu_flux_correction(pangu_ctx *c, int md, void *stream) {
   if (int r = check_md(c, md)) return r;
+  if (int r = need_fluxes(c, md)) return r;
   LCHK(c, launch_flux_correction(c->L, c->conts[md].kf(), STREAM(stream)));
   return PANGU_OK;
 }
@@ -901,6 +939,7 @@ int pangu_stage(pangu_ctx *c, int md_init, int md0, int md1, double beta, double
   if (int r = check_md(c, md_init)) return r;
   if (int r = check_md(c, md0)) return r;
   if (int r = check_md(c, md1)) return r;
+  if (int r = need_fluxes(c, md0)) return r;
   if (!dt_state) return fail(c, PANGU_EINVAL, "dt_state is null");
   hipStream_t s = STREAM(stream);
   const KFields f0 = c->conts[md0].kf(), fi = c->conts[md_init].kf(), f1 = c->conts[md1].kf();

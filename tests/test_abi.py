@@ -173,3 +173,55 @@ def test_sim_binding_fails_loudly_without_library(monkeypatch, tmp_path):
     monkeypatch.setattr(sim, "lib_path", lambda: str(tmp_path / "missing.so"))
     with pytest.raises(RuntimeError):
         sim.lib()
+
+
+def _refined_blocks(d, leaves):
+    """pangu_block list of (level, lx1, lx2) leaves of a 2-D configuration."""
+    out = (abi.PanguBlock * len(leaves))()
+    nbk = [d["nx"][i] // d["nb"][i] for i in range(3)]
+    for b, (level, i, j) in enumerate(leaves):
+        B = out[b]
+        B.level = level
+        for dd, l in enumerate((i, j, 0)):
+            cnt = d["nb"][dd] << level if d["nx"][dd] > 1 else 1
+            x0, x1 = d["xmin"][dd], d["xmax"][dd]
+            B.nx[dd] = nbk[dd]
+            B.ng[dd] = d["ng"] if d["nx"][dd] > 1 else 0
+            B.lx[dd] = l
+            B.xmin[dd] = x0 + (x1 - x0) * l / cnt
+            B.dx[dd] = (x1 - x0) / cnt / nbk[dd]
+    return out
+
+
+def _quadrant_refined(child_level=1):
+    roots = [(0, 0, 0), (0, 1, 0), (0, 0, 1)]
+    kids = [(child_level, i, j) for j in (2, 3) for i in (2, 3)]
+    return roots, kids
+
+
+def test_create_checks_refined_meshes(L):
+    """Refined meshes are checked on the host before any device work:
+    blocks must tile the domain, neighbours may differ by one level, and a
+    block next to a coarser one may not touch an outflow face (its
+    prolongation stencil would need cells beyond the boundary).  A valid
+    mesh passes the checks (it then fails only for want of a GPU here)."""
+    d = configs.orszag_tang(32, (2, 2, 1))
+    roots, kids = _quadrant_refined()
+
+    def create(leaves, dd=d):
+        cfg = abi.config_from(dd)
+        blk = _refined_blocks(dd, leaves)
+        ctx = C.c_void_p()
+        r = L.pangu_create(C.byref(cfg), blk, len(leaves), 0, C.byref(ctx))
+        if r == abi.OK:
+            L.pangu_destroy(ctx)
+        return r
+
+    assert create(roots + kids) != abi.EINVAL                      # periodic box: valid
+    assert create(roots + kids[:3]) == abi.EINVAL                  # a gap
+    assert create(roots + [(0, 1, 1)] + kids) == abi.EINVAL        # a block inside another
+    assert create(roots + [(1, 4, 4)] + kids) == abi.EINVAL        # outside the level-1 grid
+    deep = roots + kids[1:] + [(2, i, j) for j in (4, 5) for i in (4, 5)]
+    assert create(deep) == abi.EINVAL                              # level 2 next to level 0
+    out = dict(d, bc=[[configs.OUTFLOW] * 2, [configs.OUTFLOW] * 2, [configs.PERIODIC] * 2])
+    assert create(roots + kids, out) == abi.EINVAL                 # fine block on an outflow face
