@@ -37,12 +37,15 @@ FP64_VALU_PEAK_TFLOPS = 78.6
 #   flux_x1/2/3        read P (10) + write flux (10) + alfven (1)            = 21
 #   div_update_source  read the 7 non-B fluxes of 3 directions (21) + 3 EMFs
 #                      (the CT face B fluxes are formed from them in the
-#                      kernel) + U_init (10) + P (10) + write U (10)           = 54
-#   recovery_heating   read U (10) + P (10) + write P (8: rho, u, u~, B, K_e)  = 28
-#   fix_cons           read P (10) + write P (7) + write U (10)              = 27
+#                      kernel) + U_init (10) + the P the source term uses
+#                      (8: rho, u, u~, B) + write U (10)                     = 52
+#   recovery_heating   read U (10) + the stale entropy P_ENT the heating
+#                      reads (1) + write P (9: rho, u, u~, B, K_e)           = 20
+#   fix_cons           read P (10) + write P (2: the entropies; rho, u, u~
+#                      only where floored) + write U (10)                    = 22
 #   ct_emf             read 6 B-fluxes + write 3 EMFs                        = 9
-WORDS_PER_ZONE = {"flux_x1": 21, "flux_x2": 21, "flux_x3": 21, "div_update_source": 54,
-                  "recovery_heating": 28, "fix_cons": 27, "ct_emf": 9}
+WORDS_PER_ZONE = {"flux_x1": 21, "flux_x2": 21, "flux_x3": 21, "div_update_source": 52,
+                  "recovery_heating": 20, "fix_cons": 22, "ct_emf": 9}
 
 # Launch parameters of this workload as pangu_sim_autotune chose them on an
 # MI355X (tile order per kernel family: 0 plane-major, 1 column-major).  The
