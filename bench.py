@@ -35,16 +35,17 @@ FP64_VALU_PEAK_TFLOPS = 78.6
 # one-cell halo of faces that CT needs (<= 6 %) and the geometry tables
 # (x3-independent, amortised over the 32 planes of a block) are excluded:
 #   flux_x1/2/3        read P (10) + write flux (10) + alfven (1)            = 21
-#   div_update_source  read the 7 non-B fluxes of 3 directions (21) + 3 EMFs
-#                      (the CT face B fluxes are formed from them in the
-#                      kernel) + U_init (10) + the P the source term uses
-#                      (8: rho, u, u~, B) + write U (10)                     = 52
+#   div_update_source  read the 7 non-B fluxes of 3 directions (21) + the 6
+#                      transverse Riemann B fluxes (the kernel forms the edge
+#                      EMFs and CT face B fluxes from them) + U_init (10) +
+#                      the P the source term uses (8: rho, u, u~, B) + write
+#                      U (10)                                                = 55
 #   recovery_heating   read U (10) + the stale entropy P_ENT the heating
 #                      reads (1) + write P (9: rho, u, u~, B, K_e)           = 20
 #   fix_cons           read P (10) + write P (2: the entropies; rho, u, u~
 #                      only where floored) + write U (10)                    = 22
 #   ct_emf             read 6 B-fluxes + write 3 EMFs                        = 9
-WORDS_PER_ZONE = {"flux_x1": 21, "flux_x2": 21, "flux_x3": 21, "div_update_source": 52,
+WORDS_PER_ZONE = {"flux_x1": 21, "flux_x2": 21, "flux_x3": 21, "div_update_source": 55,
                   "recovery_heating": 20, "fix_cons": 22, "ct_emf": 9}
 
 # Launch parameters of this workload as pangu_sim_autotune chose them on an
@@ -60,7 +61,7 @@ TUNED = {"order": {"fix_cons": 1, "flux_x1": 1, "flux_x2": 0, "flux_x3": 1, "ct"
 # and WRITE_SIZE on this workload, each scaled by the same counter's reading
 # for a plain copy kernel of known size (profiles/counters_torus3d_256x128x128.md).
 MEASURED_TRAFFIC = {
-    "div_update_source": 2063902361, "flux_x1": 881801660, "flux_x2": 931134937,
+    "div_update_source": 2218881421, "flux_x1": 881801660, "flux_x2": 931134937,
     "flux_x3": 857023605, "fix_cons": 1120759286, "recovery_heating": 1121017255,
     "ct_emf": 353569598,
 }

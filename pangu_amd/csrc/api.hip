@@ -825,7 +825,7 @@ int pangu_constrained_transport(pangu_ctx *c, int md, void *stream) {
   if (int r = check_md(c, md)) return r;
   if (int r = need_fluxes(c, md)) return r;
   if (!c->cfg.enable_B) return PANGU_OK;
-  LCHK(c, launch_ct(c->L, c->conts[md].kf(), c->demf, true, STREAM(stream)));
+  LCHK(c, launch_ct(c->L, c->conts[md].kf(), c->demf, STREAM(stream)));
   return PANGU_OK;
 }
 
@@ -834,7 +834,7 @@ int pangu_flux_divergence(pangu_ctx *c, int md_in, int md_dudt, void *stream) {
   if (int r = need_fluxes(c, md_in)) return r;
   if (int r = check_md(c, md_dudt)) return r;
   const KFields f = c->conts[md_in].kf();
-  LCHK(c, launch_div_update(c->L, 1, f, f, f, c->conts[md_dudt].dcons, nullptr, nullptr, 0.0,
+  LCHK(c, launch_div_update(c->L, 1, f, f, f, c->conts[md_dudt].dcons, false, nullptr, 0.0,
                             0.0, STREAM(stream)));
   return PANGU_OK;
 }
@@ -845,7 +845,7 @@ int pangu_update_independent(pangu_ctx *c, int md_init, int md_dudt, double beta
   if (int r = check_md(c, md_dudt)) return r;
   if (int r = check_md(c, md_out)) return r;
   LCHK(c, launch_div_update(c->L, 2, c->conts[md_init].kf(), c->conts[md_init].kf(),
-                            c->conts[md_out].kf(), c->conts[md_dudt].dcons, nullptr, nullptr, 0.0,
+                            c->conts[md_out].kf(), c->conts[md_dudt].dcons, false, nullptr, 0.0,
                             beta_dt, STREAM(stream)));
   return PANGU_OK;
 }
@@ -853,7 +853,7 @@ int pangu_update_independent(pangu_ctx *c, int md_init, int md_dudt, double beta
 int pangu_add_geometric_source(pangu_ctx *c, int md, double beta_dt, void *stream) {
   if (int r = check_md(c, md)) return r;
   const KFields f = c->conts[md].kf();
-  LCHK(c, launch_div_update(c->L, 3, f, f, f, nullptr, nullptr, nullptr, 0.0, beta_dt,
+  LCHK(c, launch_div_update(c->L, 3, f, f, f, nullptr, false, nullptr, 0.0, beta_dt,
                             STREAM(stream)));
   return PANGU_OK;
 }
@@ -945,19 +945,19 @@ int pangu_stage(pangu_ctx *c, int md_init, int md0, int md1, double beta, double
   const KFields f0 = c->conts[md0].kf(), fi = c->conts[md_init].kf(), f1 = c->conts[md1].kf();
   LCHK(c, launch_fix_cons(c->L, f0, c->dmask, true, true, s));
   LCHK(c, launch_fluxes(c->L, f0, c->dalfven, s));
-  // With fuse_ct the update kernel forms the CT face B fluxes from the edge
-  // EMFs itself, so the flux arrays keep the Riemann B fluxes after a fused
-  // stage (nothing after the update reads them).  Flux correction on a
-  // refined mesh overwrites face fluxes, so there they are written first.
+  // With fuse_ct the update kernel forms the edge EMFs and the CT face B
+  // fluxes itself, so no CT kernel runs: the flux arrays keep the Riemann B
+  // fluxes and the EMF arrays their last values after a fused stage
+  // (nothing after the update reads them).  Flux correction on a refined
+  // mesh overwrites face fluxes, so there CT writes them first.
   const bool fuse_ct = c->cfg.enable_B && c->L.fuse_ct && !c->L.ml.on;
-  double *const *emf = fuse_ct ? c->demf : nullptr;
-  if (c->cfg.enable_B) LCHK(c, launch_ct(c->L, f0, c->demf, !fuse_ct, s));
+  if (c->cfg.enable_B && !fuse_ct) LCHK(c, launch_ct(c->L, f0, c->demf, s));
   if (c->L.ml.on) LCHK(c, launch_flux_correction(c->L, f0, s));
   if (c->L.split_source && c->cfg.metric != PANGU_METRIC_MINKOWSKI) {
-    LCHK(c, launch_div_update(c->L, 5, f0, fi, f1, nullptr, emf, dt_state, beta, 0.0, s));
-    LCHK(c, launch_div_update(c->L, 3, f0, fi, f1, nullptr, nullptr, dt_state, beta, 0.0, s));
+    LCHK(c, launch_div_update(c->L, 5, f0, fi, f1, nullptr, fuse_ct, dt_state, beta, 0.0, s));
+    LCHK(c, launch_div_update(c->L, 3, f0, fi, f1, nullptr, false, dt_state, beta, 0.0, s));
   } else {
-    LCHK(c, launch_div_update(c->L, 4, f0, fi, f1, nullptr, emf, dt_state, beta, 0.0, s));
+    LCHK(c, launch_div_update(c->L, 4, f0, fi, f1, nullptr, fuse_ct, dt_state, beta, 0.0, s));
   }
   if (exchange_after == 0) LCHK(c, exchange(c, f1, s));
   if (int r = launch_guess(c, md1, s)) return r;

@@ -125,16 +125,16 @@ void set_kernel_clock(pangu_ctx *c, KernelClock *clock);
 int launch_fix_cons(const LaunchCtx &L, const KFields &F, uint8_t *const *mask, bool fix, bool cons,
                     hipStream_t s);
 int launch_fluxes(const LaunchCtx &L, const KFields &F, double *const *alfven, hipStream_t s);
-// faces = false computes the edge EMFs only (the fused stage then forms the
-// CT face fluxes inside the update kernel).
-int launch_ct(const LaunchCtx &L, const KFields &F, double *const *emf, bool faces, hipStream_t s);
+// Edge EMFs, then the CT face B fluxes (the task-level ConstraintedTransport).
+int launch_ct(const LaunchCtx &L, const KFields &F, double *const *emf, hipStream_t s);
 // mode: 1 divergence -> dudt, 2 update from dudt, 3 source only, 4 fused
 // divergence + update + source, 5 fused divergence + update without source
-// emf (modes 4 and 5, with a field): the constrained-transport face B
-// fluxes are formed from these edge EMFs inside the kernel instead of being
-// read from the flux arrays (k_ct_faces is then not needed); null reads them.
+// ct_fused (modes 4 and 5, with a field): the kernel forms the edge EMFs
+// from the Riemann B fluxes and from them the constrained-transport face B
+// fluxes, with the expressions of k_emf and k_ct_faces (neither is then
+// launched); false reads the face B fluxes from the flux arrays.
 int launch_div_update(const LaunchCtx &L, int mode, const KFields &F0, const KFields &Finit,
-                      const KFields &F1, double *const *dudt, double *const *emf,
+                      const KFields &F1, double *const *dudt, bool ct_fused,
                       const double *dt_dev, double beta, double bdt_host, hipStream_t s);
 // rec: 0 chainer, 1 kastaun, 2 1dvsq, -1 none (heating only)
 int launch_recover(const LaunchCtx &L, const KFields &F, const SharedTabs &S, int rec, bool heat,

@@ -412,24 +412,39 @@ __global__ __launch_bounds__(256) void k_ct_faces(KCtx C, KFields F, double *con
 // Flux divergence of one cell (and, for MODE 4, the update from the initial
 // container) with the X2 / X3 terms present when D2 / D3.
 // CTF: the face B fluxes are the constrained-transport values, formed here
-// from the edge EMFs exactly as k_ct_faces would write them (same operands,
-// same operations), instead of being read from the flux arrays after a
-// k_ct_faces launch.  B components sit at packed indices 5..7.
+// from edge EMFs that are themselves formed here from the Riemann B fluxes,
+// exactly as k_emf and k_ct_faces would compute them (same operands, same
+// operations), so neither CT kernel runs.  Each cell forms the 12 EMFs on
+// its edges (each edge is shared by four cells; the extra arithmetic is
+// cheaper than a kernel that writes and rereads the EMF arrays).  B
+// components sit at packed indices 5..7.
 template <bool D2, bool D3, int MODE, int NC, bool CTF>
 __device__ __forceinline__ void flux_div(const DevBlock &B, const KFields &F0, const KFields &Finit,
-                                         double *const *dudt, double *const *emf, int b,
+                                         double *const *dudt, int b,
                                          int64_t id, int64_t sj, int64_t sk, double bdt,
                                          double unew[NC]) {
   const double a1 = B.dx[1] * B.dx[2], a2 = B.dx[0] * B.dx[2], a3 = B.dx[0] * B.dx[1];
   const double vol = B.dx[0] * B.dx[1] * B.dx[2];
   const double rvol = 1.0 / vol;
   double f1p[NC], f1m[NC], f2p[NC], f2m[NC], f3p[NC], f3m[NC], u0[NC];
-  gptr<double> e1 = nullptr, e2 = nullptr, e3 = nullptr;
+  // edge EMFs at offset o from the cell, with the k_emf expressions
+  gptr<double> f1b2 = nullptr, f1b3 = nullptr, f2b1 = nullptr, f2b3 = nullptr, f3b1 = nullptr,
+               f3b2 = nullptr;
   if (CTF) {
-    e1 = glb(emf[b * 3]);
-    e2 = glb(emf[b * 3 + 1]);
-    e3 = glb(emf[b * 3 + 2]);
+    f1b2 = glb(F0.flux[0][b * NC + 6]);
+    f1b3 = glb(F0.flux[0][b * NC + 7]);
+    if (D2) {
+      f2b1 = glb(F0.flux[1][b * NC + 5]);
+      f2b3 = glb(F0.flux[1][b * NC + 7]);
+    }
+    if (D3) {
+      f3b1 = glb(F0.flux[2][b * NC + 5]);
+      f3b2 = glb(F0.flux[2][b * NC + 6]);
+    }
   }
+  auto E1 = [&](int64_t o) { return 0.25 * ((f2b3[o] + f2b3[o - sk]) - (f3b2[o] + f3b2[o - sj])); };
+  auto E2 = [&](int64_t o) { return 0.25 * ((f3b1[o] + f3b1[o - 1]) - (f1b3[o] + f1b3[o - sk])); };
+  auto E3 = [&](int64_t o) { return 0.25 * ((f1b2[o] + f1b2[o - sj]) - (f2b1[o] + f2b1[o - 1])); };
 #pragma unroll
   for (int q = 0; q < NC; ++q) {
     const int c = q - 5;  // B component when CTF and 0 <= c < 3
@@ -439,11 +454,11 @@ __device__ __forceinline__ void flux_div(const DevBlock &B, const KFields &F0, c
       f1p[q] = 0;
       f1m[q] = 0;
     } else if (ctb && c == 1 && D2) {
-      f1m[q] = 0.5 * (e3[id] + e3[id + sj]);
-      f1p[q] = 0.5 * (e3[id + 1] + e3[id + 1 + sj]);
+      f1m[q] = 0.5 * (E3(id) + E3(id + sj));
+      f1p[q] = 0.5 * (E3(id + 1) + E3(id + 1 + sj));
     } else if (ctb && c == 2 && D3) {
-      f1m[q] = -0.5 * (e2[id] + e2[id + sk]);
-      f1p[q] = -0.5 * (e2[id + 1] + e2[id + 1 + sk]);
+      f1m[q] = -0.5 * (E2(id) + E2(id + sk));
+      f1p[q] = -0.5 * (E2(id + 1) + E2(id + 1 + sk));
     } else {
       f1p[q] = f1[id + 1];
       f1m[q] = f1[id];
@@ -451,14 +466,14 @@ __device__ __forceinline__ void flux_div(const DevBlock &B, const KFields &F0, c
     if (D2) {
       const auto f2 = glb(F0.flux[1][b * NC + q]);
       if (ctb && c == 0) {
-        f2m[q] = -0.5 * (e3[id] + e3[id + 1]);
-        f2p[q] = -0.5 * (e3[id + sj] + e3[id + sj + 1]);
+        f2m[q] = -0.5 * (E3(id) + E3(id + 1));
+        f2p[q] = -0.5 * (E3(id + sj) + E3(id + sj + 1));
       } else if (ctb && c == 1) {
         f2p[q] = 0.;
         f2m[q] = 0.;
       } else if (ctb && c == 2 && D3) {
-        f2m[q] = 0.5 * (e1[id] + e1[id + sk]);
-        f2p[q] = 0.5 * (e1[id + sj] + e1[id + sj + sk]);
+        f2m[q] = 0.5 * (E1(id) + E1(id + sk));
+        f2p[q] = 0.5 * (E1(id + sj) + E1(id + sj + sk));
       } else {
         f2p[q] = f2[id + sj];
         f2m[q] = f2[id];
@@ -467,11 +482,11 @@ __device__ __forceinline__ void flux_div(const DevBlock &B, const KFields &F0, c
     if (D3) {
       const auto f3 = glb(F0.flux[2][b * NC + q]);
       if (ctb && c == 0) {
-        f3m[q] = 0.5 * (e2[id] + e2[id + 1]);
-        f3p[q] = 0.5 * (e2[id + sk] + e2[id + sk + 1]);
+        f3m[q] = 0.5 * (E2(id) + E2(id + 1));
+        f3p[q] = 0.5 * (E2(id + sk) + E2(id + sk + 1));
       } else if (ctb && c == 1 && D2) {
-        f3m[q] = -0.5 * (e1[id] + e1[id + sj]);
-        f3p[q] = -0.5 * (e1[id + sk] + e1[id + sk + sj]);
+        f3m[q] = -0.5 * (E1(id) + E1(id + sj));
+        f3p[q] = -0.5 * (E1(id + sk) + E1(id + sk + sj));
       } else if (ctb && c == 2) {
         f3p[q] = 0;
         f3m[q] = 0;
@@ -500,7 +515,7 @@ __device__ __forceinline__ void flux_div(const DevBlock &B, const KFields &F0, c
 // kernel is memory bound, so occupancy pays.
 template <int MODE, bool SRC, bool HB, bool HK, bool CTF>
 __global__ __launch_bounds__(256, (HB && MODE != 1) ? 5 : 4) void k_div_update(KCtx C, KFields F0, KFields Finit, KFields F1,
-                                                    double *const *dudt, double *const *emf,
+                                                    double *const *dudt,
                                                     const double *dt_dev, double beta,
                                                     double bdt_host, Range3 R) {
   int b, k, j, i;
@@ -516,13 +531,13 @@ __global__ __launch_bounds__(256, (HB && MODE != 1) ? 5 : 4) void k_div_update(K
     // all components sit in one basic block and overlap
     const bool d2 = B.n[1] > 1, d3 = B.n[2] > 1;
     if (d2 && d3)
-      flux_div<true, true, MODE, NC, CTF>(B, F0, Finit, dudt, emf, b, id, sj, sk, bdt, unew);
+      flux_div<true, true, MODE, NC, CTF>(B, F0, Finit, dudt, b, id, sj, sk, bdt, unew);
     else if (d2)
-      flux_div<true, false, MODE, NC, CTF>(B, F0, Finit, dudt, emf, b, id, sj, sk, bdt, unew);
+      flux_div<true, false, MODE, NC, CTF>(B, F0, Finit, dudt, b, id, sj, sk, bdt, unew);
     else if (d3)
-      flux_div<false, true, MODE, NC, CTF>(B, F0, Finit, dudt, emf, b, id, sj, sk, bdt, unew);
+      flux_div<false, true, MODE, NC, CTF>(B, F0, Finit, dudt, b, id, sj, sk, bdt, unew);
     else
-      flux_div<false, false, MODE, NC, CTF>(B, F0, Finit, dudt, emf, b, id, sj, sk, bdt, unew);
+      flux_div<false, false, MODE, NC, CTF>(B, F0, Finit, dudt, b, id, sj, sk, bdt, unew);
   } else if (MODE == 2) {
 #pragma unroll
     for (int q = 0; q < NC; ++q)
@@ -1116,22 +1131,20 @@ int launch_fluxes(const LaunchCtx &L, const KFields &F, double *const *alfven, h
   return PG_CHECK_LAUNCH();
 }
 
-int launch_ct(const LaunchCtx &L, const KFields &F, double *const *emf, bool faces, hipStream_t s) {
+int launch_ct(const LaunchCtx &L, const KFields &F, double *const *emf, hipStream_t s) {
   Range3 R = L.ct;
   R.order = L.order[PANGU_KF_CT];
   if (R.ntiles == 0) return 0;
   const dim3 g = tile_grid(R);
   hipLaunchKernelGGL(k_emf, g, kTile, 0, s, L.k, F, emf, R);
   TICK("ct_emf");
-  if (faces) {
-    hipLaunchKernelGGL(k_ct_faces, g, kTile, 0, s, L.k, F, emf, R);
-    TICK("ct_faces");
-  }
+  hipLaunchKernelGGL(k_ct_faces, g, kTile, 0, s, L.k, F, emf, R);
+  TICK("ct_faces");
   return PG_CHECK_LAUNCH();
 }
 
 int launch_div_update(const LaunchCtx &L, int mode, const KFields &F0, const KFields &Finit,
-                      const KFields &F1, double *const *dudt, double *const *emf,
+                      const KFields &F1, double *const *dudt, bool ct_fused,
                       const double *dt_dev, double beta, double bdt_host, hipStream_t s) {
   Range3 R = L.interior;
   R.order = L.order[PANGU_KF_DIV_UPDATE];
@@ -1139,11 +1152,11 @@ int launch_div_update(const LaunchCtx &L, int mode, const KFields &F0, const KFi
   const dim3 g = tile_grid(R);
   const bool hb = L.k.iBX >= 0, hk = L.k.iKEL >= 0;
   const bool src = L.metric != PANGU_METRIC_MINKOWSKI;
-  // CT fused into the divergence (modes 4 and 5 with a field and EMFs given)
-  const bool ctf = emf != nullptr && hb && (mode == 4 || mode == 5);
+  // CT fused into the divergence (modes 4 and 5 with a field)
+  const bool ctf = ct_fused && hb && (mode == 4 || mode == 5);
 #define DU(M, SR, HB_, HK_, CT_)                                                               \
   hipLaunchKernelGGL((k_div_update<M, SR, HB_, HK_, CT_>), g, kTile, 0, s, L.k, F0, Finit, F1, \
-                     dudt, emf, dt_dev, beta, bdt_host, R)
+                     dudt, dt_dev, beta, bdt_host, R)
 #define DU2(M, SR)                                      \
   if (hb && hk) {                                       \
     if ((M) == 4 && ctf) DU(M, SR, true, true, (M) == 4); \
