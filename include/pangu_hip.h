@@ -223,12 +223,13 @@ int pangu_set_split_source(pangu_ctx *ctx, int split);
  * interface value to the next face through a wavefront shuffle, instead of
  * reconstructing both neighbours per face.  Results are identical. */
 int pangu_set_share_reconstruction(pangu_ctx *ctx, int share);
-/* fuse = 1 (default): the fused stage runs only the EMF half of constrained
- * transport, and the flux-divergence + update kernel forms the face B fluxes
- * from the edge EMFs itself (the same arithmetic as the face overwrite, so
- * results are identical); the flux arrays then keep the Riemann B fluxes
- * after pangu_stage.  0 writes the CT face fluxes first, as the task-level
- * entry points always do.  On a refined mesh the CT face fluxes are always
+/* fuse = 1 (default): the fused stage runs no constrained-transport kernel;
+ * the flux-divergence + update kernel forms the edge EMFs from the Riemann B
+ * fluxes and the face B fluxes from them (the same arithmetic as
+ * ConstraintedTransport, so results are identical).  The flux arrays then
+ * keep the Riemann B fluxes and the EMF arrays are not written by
+ * pangu_stage.  0 runs ConstraintedTransport first (EMFs, then the face
+ * fluxes), as the task-level entry points always do.  On a refined mesh the CT face fluxes are always
  * written, because flux correction overwrites some of them. */
 int pangu_set_fuse_ct(pangu_ctx *ctx, int fuse);
 
