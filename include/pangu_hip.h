@@ -229,8 +229,8 @@ int pangu_set_share_reconstruction(pangu_ctx *ctx, int share);
  * ConstraintedTransport, so results are identical).  The flux arrays then
  * keep the Riemann B fluxes and the EMF arrays are not written by
  * pangu_stage.  0 runs ConstraintedTransport first (EMFs, then the face
- * fluxes), as the task-level entry points always do.  On a refined mesh the CT face fluxes are always
- * written, because flux correction overwrites some of them. */
+ * fluxes), as the task-level entry points always do.  On a refined mesh it
+ * always runs first, because flux correction overwrites some face fluxes. */
 int pangu_set_fuse_ct(pangu_ctx *ctx, int fuse);
 
 /* ---- run diagnostics (no reference counterpart) ----
