@@ -188,3 +188,20 @@ def test_batch_of_refined_meshes_equals_each_alone():
                 assert np.array_equal(arr[first + i], ref[i]), (m, f, c, i)
         one.close()
     batch.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["ot2d_mc_laxf_onedvsq", "ot2d_two_to_one"])
+def test_refined_hundred_cycles(name):
+    """A hundred cycles on a refined mesh in graph mode: rel. L1 within 1e-9
+    (the multi-step bar), with time and dt tracking the oracle."""
+    from test_parity_long_gpu import _check
+    from pangu_amd.sim import HipSim
+    cfg = CASES[name]
+    o = O.OracleSim(cfg)
+    o.init()
+    h = HipSim(cfg)
+    h.init()
+    o.step(100)
+    h.step(100, "graph")
+    _check(o, h, name, 1e-9)
