@@ -30,7 +30,7 @@ RECOVERY = dict(chainer=0, robuster=1, kastaun=1, onedvsq=2)
 
 
 def orszag_tang(n=256, nb=(1, 1, 1)):
-    """Config 1: 2D relativistic Orszag-Tang on [0,1]^2, Minkowski, MC,
+    """2D relativistic Orszag-Tang on [0,1]^2 (n^2 zones), Minkowski, MC,
     LAXF, CT, 1Dvsq recovery, RK2, CFL 0.4."""
     d = dict(_BASE)
     d.update(nx=[n, n, 1], nb=list(nb), xmin=[0.0, 0.0, -0.5], xmax=[1.0, 1.0, 0.5],
@@ -48,16 +48,17 @@ def _torus_common(d):
 
 
 def torus2d(nx1=512, nx2=256, nb=(1, 1, 1)):
-    """Config 2: 2D FM torus, MKS a=0.9375 h=0.7, PPM4, LAXF, chainer,
-    heating, CT, RK2."""
+    """2D FM torus (nx1 x nx2 zones), MKS a=0.9375 h=0.7, PPM4, LAXF,
+    chainer, heating, CT, RK2."""
     d = _torus_common(dict(_BASE))
     d.update(nx=[nx1, nx2, 1], nb=list(nb), riemann=0, limiter=1, recovery=0)
     return d
 
 
 def torus3d(nx=(256, 128, 128), nb=(4, 2, 4)):
-    """Config 3 (headline): 3D FM torus 256x128x128 in 32 meshblocks of
-    64x64x32, PPM4, HLL, Kastaun, CT, heating, RK2."""
+    """3D FM torus, by default 256x128x128 zones in 32 meshblocks of
+    64x64x32 (the benchmark workload), MKS a=0.9375 h=0.7, PPM4, HLL,
+    Kastaun, CT, heating, RK2."""
     d = _torus_common(dict(_BASE))
     d.update(nx=list(nx), nb=list(nb), riemann=1, limiter=1, recovery=1)
     return d
