@@ -123,7 +123,11 @@ int pangu_create(const pangu_config *cfg, const pangu_block *blocks, int nblocks
  * pangu_create(cfg, blocks, n, dev, out) is the one-mesh case. */
 int pangu_create_batch(const pangu_config *cfgs, int nmesh, const pangu_block *blocks,
                        const int *block_mesh, int nblocks, int device, pangu_ctx **out);
+/* Frees what pangu_create allocated (geometry tables, pointer tables,
+ * refinement tables); never the caller's arrays.  No reference counterpart:
+ * Parthenon owns the device fields there. */
 int pangu_destroy(pangu_ctx *ctx);
+/* The message of the last failed call on ctx (no reference counterpart). */
 const char *pangu_last_error(const pangu_ctx *ctx);
 
 /* Host-only: one entry of the geometry tables at code coordinates x
@@ -136,8 +140,14 @@ const char *pangu_last_error(const pangu_ctx *ctx);
 int pangu_geometry_entry(const pangu_config *cfg, int loc, const double x[4], double *out);
 
 /* Registers a container's pointer tables (copied to the device once);
- * returns a handle used by the task functions.  Pointers must stay valid. */
+ * returns a handle used by the task functions.  Pointers must stay valid.
+ * Stands in for the variable packs every reference task builds on entry
+ * (md->PackVariables / PackVariablesAndFluxes, riemann_solver/hll.cc:37,49):
+ * the packs are resolved once here instead of per call. */
 int pangu_register_container(pangu_ctx *ctx, const pangu_container *c, int *handle);
+/* Registers the tables shared by all containers (wave speeds, EMFs, flags,
+ * masks, recovery codes): the "alfven" pack of riemann_solver/hll.cc:51-53
+ * and the other package-level fields the reference tasks pack by name. */
 int pangu_register_shared(pangu_ctx *ctx, const pangu_shared *s);
 
 /* ---- task functions (one per reference task) ---- */
@@ -202,6 +212,7 @@ int pangu_stage(pangu_ctx *ctx, int md_init, int md0, int md1, double beta, doub
                 double dt_factor, int last_stage, int exchange_after, void *stream);
 
 /* ---- launch tuning (no reference counterpart) ----
+ * These options change how kernels are launched, never what they compute.
  * Tile order of each kernel family: plane-major (consecutive workgroups
  * sweep one k plane) or column-major (consecutive workgroups walk k over one
  * 256-cell run of the (j, i) plane, reusing x3-independent geometry from L2).

@@ -7,6 +7,8 @@
  * ideal_grmhd.cc:23-120), the initial data of the FM torus / Orszag-Tang
  * problems and dt control.  Field arrays live in device memory, one arena
  * per field laid out [block][component][k][j][i] over ghosted blocks.
+ * Entry points with no cited counterpart are the driver's own (field
+ * access, timing, tuning); a Parthenon build needs none of them.
  */
 #ifndef PANGU_SIM_H
 #define PANGU_SIM_H
@@ -78,6 +80,11 @@ int pangu_sim_nblocks(pangu_sim *s);
  * per mesh; out (host) receives nmesh * PANGU_NDIAG doubles. */
 int pangu_sim_diagnostics(pangu_sim *s, int container, const int *shell, double *out);
 int pangu_sim_destroy(pangu_sim *s);
+/* Writes the initial data of cfg.problem into the base container:
+ * ProblemGenerator of pangu/problem/orszag_tang_vortex/problem_generator.cpp:16-69
+ * (problem 1), fm_torus_mks/problem_generator.cpp:103-289 (2) and
+ * fm_torus_cks/problem_generator.cpp:143-334 (3), then fills the base
+ * container's ghost cells (exchange, then physical boundaries). */
 int pangu_sim_init(pangu_sim *s);
 /* Runs ncycles RK2 cycles in the given mode (tasks: one C-ABI call per
  * reference task with a host-side dt; fused: pangu_stage with device dt;
