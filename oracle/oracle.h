@@ -31,6 +31,9 @@
 // Parity unpinned (no reference build or fixture exists; semantics inferred
 // from Parthenon and restated): ghost exchange order, stage-container
 // initialisation, dt control, flux divergence form, block coordinates.
+// Also unpinned: the MAD factor ((r/rin) sin theta)^3 exp(-r/400) of the CKS
+// torus potential (problems.cpp, init_torus_cks).  The divergence and beta
+// checks hold for any potential, so they cannot see a wrong factor.
 #pragma once
 
 #include <cmath>

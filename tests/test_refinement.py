@@ -242,13 +242,15 @@ def _cks_mhd_torus(mad=0):
     return configs.fm_torus_cks(nx=32, nb=2, levels=0, extent=32.0, enable_B=1, mad=mad)
 
 
+@pytest.mark.parametrize("mad", [0, 1])
 @pytest.mark.parametrize("plant", ["none", "cks_curl"])
-def test_cks_torus_field_is_divergence_free(plant):
+def test_cks_torus_field_is_divergence_free(plant, mad):
     """B = curl A / sqrt|g| by central differences (problem_generator.cpp:
     548-578): the central-difference divergence of sqrt|g| B is then zero
     identically (central differences commute), to round-off, over interior
-    cells whose stencil stays in their block.  A wrong curl sign breaks it."""
-    cfg = _cks_mhd_torus()
+    cells whose stencil stays in their block, whatever the potential (SANE,
+    or MAD with its extra radial factor).  A wrong curl sign breaks it."""
+    cfg = _cks_mhd_torus(mad)
     O.plant(plant)
     try:
         o = O.OracleSim(cfg, nthreads=4)
@@ -289,11 +291,12 @@ def test_cks_torus_field_is_divergence_free(plant):
         assert worst > 1e-3 * scale
 
 
-def test_cks_torus_field_meets_beta():
+@pytest.mark.parametrize("mad", [0, 1])
+def test_cks_torus_field_meets_beta(mad):
     """After the normalisation (:712-730) the largest b^2 over the interior
     is 2 (gamma - 1) (u_max / rho_max) / beta, with the maxima over entire
-    blocks (:653-704)."""
-    cfg = _cks_mhd_torus()
+    blocks (:653-704), for the SANE and the MAD potential alike."""
+    cfg = _cks_mhd_torus(mad)
     o = O.OracleSim(cfg, nthreads=4)
     o.init()
     P = o.get("prim", 0)

@@ -60,6 +60,9 @@ CASES = {
     # the same with the problem generator's SANE field (CT and flux
     # correction of the B fluxes across levels)
     "fm_torus_cks_2level_mhd": configs.fm_torus_cks(nx=32, nb=4, levels=2, extent=32.0, enable_B=1),
+    # and with its MAD field (the potential times ((r / rin) sin theta)^3 e^(-r/400))
+    "fm_torus_cks_2level_mad": configs.fm_torus_cks(nx=32, nb=4, levels=2, extent=32.0, enable_B=1,
+                                                    mad=1),
 }
 
 
