@@ -248,13 +248,18 @@ int pangu_set_fuse_ct(pangu_ctx *ctx, int fuse);
  * doc/grmhd_intro.md:556-574 defines the fluxes through a sphere that a
  * torus run is monitored by; the reference code computes none of them.
  * Evaluated on container md at the cell centres of one interior x1 index
- * per mesh (shell[m], 0 <= shell[m] < mesh cells along x1), in code
+ * per mesh (shell[m], 0 <= shell[m] < mesh cells along x1; -1 skips the
+ * shell integrals, and a refined mesh accepts only -1, since its blocks
+ * count x1 cells at their own levels), in code
  * coordinates over dx2 dx3 (a surface flux does not depend on them):
  *   MDOT = -sum rho u^1 sqrt|g|,  EDOT = sum T^1_0 sqrt|g|,
  *   JDOT = -sum T^1_3 sqrt|g|,    PHI  = 1/2 sum |B^1| sqrt|g|;
  * over the interior: MASS = sum U_rho dV, ENERGY = sum U_energy dV,
  * DIVB = max |corner divergence of U_B| (the quantity constrained transport
- * keeps at round-off), UBMAX = max |U_B| (its scale).  out is a device array
+ * keeps at round-off), UBMAX = max |U_B| (its scale).  On a Cartesian
+ * Kerr-Schild mesh the excised cells (r < r_excise, which Recovery skips and
+ * whose U is not evolved) are left out of MASS, ENERGY and UBMAX, and
+ * corners touching one out of DIVB.  out is a device array
  * of nmesh * PANGU_NDIAG doubles, written in stream order; the reductions run
  * in a fixed order, so repeated calls give identical bits.  shell is a host
  * array of one index per mesh. */

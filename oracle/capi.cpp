@@ -164,6 +164,12 @@ int oracle_metric_at(int kind, const double x[4], double a, double h, double gco
   return ok;
 }
 
+void oracle_physical_metric(int kind, const double x[4], double a, double h, double gcov[16]) {
+  real g[4][4];
+  physical_metric(kind, x, g, h, a);
+  std::memcpy(gcov, g, sizeof(g));
+}
+
 void oracle_christoffel(int kind, const double x[4], double a, double h, double conn[64]) {
   real c[4][4][4];
   christoffel(kind, x, a, h, c);
@@ -216,7 +222,7 @@ int oracle_c2p(int scheme, const double U[9], double prim[9], double gamma, cons
   return -1;
 }
 
-double oracle_heating(const oracle_cfg *c, const double cell[6]) {
+static Params heat_params(const oracle_cfg *c) {
   Params p{};
   p.gamma = c->gamma;
   p.heat_model = c->heat_model;
@@ -228,8 +234,17 @@ double oracle_heating(const oracle_cfg *c, const double cell[6]) {
   p.limit_kel = c->limit_kel;
   p.suppress_highb_heat = c->suppress_highb_heat;
   p.enforce_positive_dissipation = c->enforce_positive_dissipation;
+  return p;
+}
+
+double oracle_heating(const oracle_cfg *c, const double cell[6]) {
   HeatCell hc{cell[0], cell[1], cell[2], cell[3], cell[4], cell[5]};
-  return heating_apply(p, hc);
+  return heating_apply(heat_params(c), hc);
+}
+
+double oracle_heating_fraction(const oracle_cfg *c, const double cell[6]) {
+  HeatCell hc{cell[0], cell[1], cell[2], cell[3], cell[4], cell[5]};
+  return heating_fraction(heat_params(c), hc);
 }
 
 #endif  // ORACLE_COUNTING

@@ -13,6 +13,10 @@
 //   mass   = sum U_RHO dV,  energy = sum U_ENY dV   (interior cells)
 //   divB   = max corner |div U_B| (the CT invariant, as in tests/helpers.py)
 //   UBmax  = max |U_B| over the interior and the active dimensions.
+// On a Cartesian Kerr-Schild mesh the cells Recovery excises
+// (invertor.cc:83-91, r_BL < r_excise) hold no evolved state and their U can
+// be non-finite: they are left out of mass, energy and UBmax, and corners
+// touching one are left out of divB.
 #include <cmath>
 
 #include "oracle_sim.h"
@@ -26,6 +30,14 @@ void Sim::diagnostics(int c, int shell, double out[ODIAG_N], double mag[ODIAG_N]
     out[q] += v;
     mag[q] += std::fabs(v);
   };
+  const double a2 = p.a * p.a;
+  auto excised = [&](const Block &B, int k, int j, int i) {
+    if (p.metric != MET_CKS) return false;
+    const real x = Xc(B, 0, i), y = Xc(B, 1, j), z = Xc(B, 2, k);
+    const real rad2 = x * x + y * y + z * z;
+    const real sa = (rad2 - a2) * (rad2 - a2) + 4.0 * a2 * z * z;
+    return std::sqrt(0.5 * (rad2 - a2 + std::sqrt(sa))) < p.r_excise;
+  };
   for (int b = 0; b < nblk; ++b) {
     const Block &B = blk[b];
     const double dA = B.dx[1] * B.dx[2];
@@ -36,6 +48,7 @@ void Sim::diagnostics(int c, int shell, double out[ODIAG_N], double mag[ODIAG_N]
       for (int j = B.s[1]; j <= B.e[1]; ++j)
         for (int i = B.s[0]; i <= B.e[0]; ++i) {
           const int64_t id = B.idx(k, j, i);
+          if (excised(B, k, j, i)) continue;
           add(ODIAG_MASS, u_rho[id] * dV);
           add(ODIAG_ENERGY, u_eny[id] * dV);
         }
@@ -69,6 +82,7 @@ void Sim::diagnostics(int c, int shell, double out[ODIAG_N], double mag[ODIAG_N]
       for (int j = B.s[1]; j <= B.e[1]; ++j)
         for (int i = B.s[0]; i <= B.e[0]; ++i) {
           const int64_t id = B.idx(k, j, i);
+          if (excised(B, k, j, i)) continue;
           for (int d = 0; d < ndim; ++d)
             out[ODIAG_UBMAX] = std::fmax(out[ODIAG_UBMAX], std::fabs(ub[d][id]));
         }
@@ -78,6 +92,9 @@ void Sim::diagnostics(int c, int shell, double out[ODIAG_N], double mag[ODIAG_N]
       for (int j = B.s[1] + 1; j <= B.e[1]; ++j)
         for (int i = B.s[0] + 1; i <= B.e[0]; ++i) {
           const int64_t id = B.idx(k, j, i);
+          if (excised(B, k, j, i) || excised(B, k, j, i - 1) || excised(B, k, j - 1, i) ||
+              excised(B, k, j - 1, i - 1))
+            continue;
           const double *b1 = ub[0], *b2 = ub[1];
           const double d1 = (b1[id] + b1[id - sj] - b1[id - 1] - b1[id - sj - 1]) / (2 * B.dx[0]);
           const double d2 = (b2[id] + b2[id - 1] - b2[id - sj] - b2[id - sj - 1]) / (2 * B.dx[1]);
@@ -89,6 +106,10 @@ void Sim::diagnostics(int c, int shell, double out[ODIAG_N], double mag[ODIAG_N]
         for (int j = B.s[1] + 1; j <= B.e[1]; ++j)
           for (int i = B.s[0] + 1; i <= B.e[0]; ++i) {
             const int64_t id = B.idx(k, j, i);
+            bool touch = false;
+            for (int c8 = 0; c8 < 8; ++c8)
+              touch = touch || excised(B, k - (c8 >> 2), j - ((c8 >> 1) & 1), i - (c8 & 1));
+            if (touch) continue;
             double tot = 0.0;
             for (int d = 0; d < 3; ++d) {
               double acc = 0.0;

@@ -60,7 +60,10 @@ static void metric_bl(const real x[4], real g[4][4], real h, real a) {
   g[0][0] = -1.0 + 2.0 * inv_r_mu;
   g[0][3] = -2.0 * a * sth2 * inv_r_mu;
   g[1][1] = mu / delta;
-  g[2][2] = r2 / mu;
+  // BL.h:54 writes r^2/mu here; Kerr's Sigma is r^2 mu (as in
+  // fm_torus_cks/problem_generator.cpp:56).  The restatement keeps the
+  // reference's form; the planted switch gives Kerr's.
+  g[2][2] = (g_plant == PLANT_BL_GTHTH) ? r2 * mu : r2 / mu;
   g[3][0] = g[0][3];
   g[3][3] = r2 * sth2 * (1.0 + a2 / r2 + 2.0 * a2 * sth2 / (r * r2 * mu));
 }

@@ -108,7 +108,13 @@ enum Plant {
   PLANT_NO_FLUX_CORRECTION = 7,  // refined mesh: skip flux correction
   PLANT_PROLONG_SLOPE = 8,       // prolongation: +-1/2 instead of +-1/4 of the slope
   PLANT_CKS_JACOBIAN = 9,        // CKS torus: wrong sign of dx/dphi in the KS -> CKS Jacobian
-  PLANT_CKS_CURL = 10            // CKS torus field: wrong sign of B^1 = -dA2/dz
+  PLANT_CKS_CURL = 10,           // CKS torus field: wrong sign of B^1 = -dA2/dz
+  PLANT_HEAT_HOWES = 11,         // Howes: electron/proton mass ratio inverted under the root
+  PLANT_HEAT_KAWAZURA = 12,      // Kawazura: (beta/15)^(+1.4) instead of ^(-1.4)
+  PLANT_HEAT_WERNER = 13,        // Werner: sigma/2 instead of sigma/5
+  PLANT_HEAT_ROWAN = 14,         // Rowan: beta_max = 1/(2 sigma) instead of 1/(4 sigma)
+  PLANT_HEAT_SHARMA = 15,        // Sharma: T_p/T_e under the root instead of T_e/T_p
+  PLANT_BL_GTHTH = 16            // BL: g_thth = r^2 mu (Kerr's Sigma) instead of the reference's r^2/mu
 };
 
 // Kokkos::max / Kokkos::min semantics (ternary on operator<).

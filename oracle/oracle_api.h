@@ -112,6 +112,8 @@ void oracle_plant(int which);
 /* pointwise functions for unit tests */
 int oracle_metric_at(int kind, const double x[4], double a, double h, double gcov[16],
                      double gcon[16], double *gdet);
+/* physical-coordinate metric CalculatePhysicalMetric (before the code-basis Jacobian) */
+void oracle_physical_metric(int kind, const double x[4], double a, double h, double gcov[16]);
 void oracle_christoffel(int kind, const double x[4], double a, double h, double conn[64]);
 void oracle_state(const double prim[10], const double gcov[16], const double gcon[16],
                   double out[17]);
@@ -122,7 +124,10 @@ void oracle_fms(double gamma, const double prim[10], const double gcov[16],
 void oracle_recon(int limiter, const double q[5], double out[2]);
 int oracle_c2p(int scheme, const double U[9], double prim[9], double gamma,
                const double gcov[16], const double gcon[16], int *iters);
+/* cell = {rho, energy, b^2, advected entropy, recovered entropy, advected
+ * electron entropy}: heating_model.h apply() and computeHeatingFraction() */
 double oracle_heating(const oracle_cfg *cfg, const double cell[6]);
+double oracle_heating_fraction(const oracle_cfg *cfg, const double cell[6]);
 
 #ifdef __cplusplus
 }

@@ -90,6 +90,9 @@ def lib():
         L.oracle_plant.argtypes = [C.c_int]
         L.oracle_heating.restype = C.c_double
         L.oracle_heating.argtypes = [C.POINTER(OracleCfg), dp]
+        L.oracle_heating_fraction.restype = C.c_double
+        L.oracle_heating_fraction.argtypes = [C.POINTER(OracleCfg), dp]
+        L.oracle_physical_metric.argtypes = [C.c_int, dp, C.c_double, C.c_double, dp]
         _lib = L
     return _lib
 
@@ -295,7 +298,8 @@ class OracleSim:
 
 PLANTS = dict(none=0, conn_index=1, shift_sign=2, ppm_term=3, kastaun_qbar=4, ct_sign=5,
               emt_pressure=6, no_flux_correction=7, prolong_slope=8,
-              cks_jacobian=9, cks_curl=10)
+              cks_jacobian=9, cks_curl=10, heat_howes=11, heat_kawazura=12,
+              heat_werner=13, heat_rowan=14, heat_sharma=15, bl_gthth=16)
 
 
 def plant(name):
@@ -311,6 +315,13 @@ def metric_at(kind, x, a=0.0, h=0.0):
     gdet = C.c_double(0.0)
     ok = lib().oracle_metric_at(kind, _dp(x), a, h, _dp(gcov), _dp(gcon), C.byref(gdet))
     return gcov.reshape(4, 4), gcon.reshape(4, 4), gdet.value, ok
+
+
+def physical_metric(kind, x, a=0.0, h=0.0):
+    x = np.asarray(x, dtype=np.float64)
+    g = np.zeros(16)
+    lib().oracle_physical_metric(kind, _dp(x), a, h, _dp(g))
+    return g.reshape(4, 4)
 
 
 def christoffel(kind, x, a=0.0, h=0.0):
@@ -369,3 +380,9 @@ def heating(cfg_dict, cell):
     c = make_cfg(cfg_dict)
     cell = np.ascontiguousarray(cell, dtype=np.float64)
     return lib().oracle_heating(C.byref(c), _dp(cell))
+
+
+def heating_fraction(cfg_dict, cell):
+    c = make_cfg(cfg_dict)
+    cell = np.ascontiguousarray(cell, dtype=np.float64)
+    return lib().oracle_heating_fraction(C.byref(c), _dp(cell))

@@ -191,32 +191,35 @@ real heating_fraction(const Params &p, const HeatCell &c) {
       const real c2 = (trat <= 1.0) ? 1.6 / trat : 1.2 / trat;
       const real c3 = (trat <= 1.0) ? 18.0 + 5.0 * ltr : real(18.0);
       const real bp = std::pow(beta, mbeta);
+      const real mrat = (g_plant == PLANT_HEAT_HOWES) ? me / mp : mp / me;
       const real qrat = 0.92 * (c2 * c2 + bp) / (c3 * c3 + bp) * std::exp(-1.0 / beta) *
-                        std::sqrt(mp / me * trat);
+                        std::sqrt(mrat * trat);
       return 1.0 / (1.0 + qrat);
     }
     case HM_KAWAZURA: {
       const real trat = tpr / tel;
       const real pres = c.rho * tpr;
       const real beta = kmin(2.0 * pres / c.bsq, 1.0e20);
-      const real qiqe = 35.0 / (1.0 + std::pow(beta / 15.0, -1.4) * std::exp(-0.1 / trat));
+      const real ex = (g_plant == PLANT_HEAT_KAWAZURA) ? 1.4 : -1.4;
+      const real qiqe = 35.0 / (1.0 + std::pow(beta / 15.0, ex) * std::exp(-0.1 / trat));
       return 1.0 / (1.0 + qiqe);
     }
     case HM_WERNER: {
       const real sigma = c.bsq / c.rho;
-      return 0.25 * (1.0 + std::sqrt((sigma / 5.0) / (2.0 + sigma / 5.0)));
+      const real sd = (g_plant == PLANT_HEAT_WERNER) ? 2.0 : 5.0;
+      return 0.25 * (1.0 + std::sqrt((sigma / sd) / (2.0 + sigma / sd)));
     }
     case HM_ROWAN: {
       const real pres = (p.gamma_p - 1.0) * c.energy;
       const real pg = (p.gamma - 1.0) * c.energy;
       const real beta = 2.0 * pres / c.bsq;
       const real sigma = c.bsq / (c.rho + c.energy + pg);
-      const real betamax = 0.25 / sigma;
+      const real betamax = ((g_plant == PLANT_HEAT_ROWAN) ? 0.5 : 0.25) / sigma;
       return 0.5 * std::exp(-std::pow(1.0 - beta / betamax, 3.3) /
                             (1.0 + 1.2 * std::pow(sigma, 0.7)));
     }
     case HM_SHARMA: {
-      const real trinv = tel / tpr;
+      const real trinv = (g_plant == PLANT_HEAT_SHARMA) ? tpr / tel : tel / tpr;
       const real qeqi = 0.33 * std::sqrt(trinv);
       return 1.0 / (1.0 + 1.0 / qeqi);
     }

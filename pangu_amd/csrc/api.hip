@@ -978,8 +978,11 @@ int pangu_diagnostics(pangu_ctx *c, int md, const int *shell, double *out, void 
   for (int m = 0; m < c->nmesh; ++m) {
     const HostBlock &H = c->hb[c->mfirst[m]];
     const int nx1 = c->mcfg[m].nb[0] * (H.e[0] - H.s[0] + 1);
-    if (shell[m] < 0 || shell[m] >= nx1)
+    if (shell[m] < -1 || shell[m] >= nx1)
       return fail(c, PANGU_EINVAL, "diagnostics: shell index outside the mesh's x1 cells");
+    if (shell[m] >= 0 && c->L.ml.on)
+      return fail(c, PANGU_EINVAL,
+                  "diagnostics: shell integrals need a single-level mesh (pass shell -1 on a refined one)");
   }
   HIPCHK(c, hipSetDevice(c->device));
   if (!c->ddiag_part) {

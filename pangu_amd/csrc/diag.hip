@@ -6,7 +6,10 @@
 //
 // Reductions run in a fixed order (a tree per 256-thread tile, then one
 // workgroup per mesh over the tiles), so results repeat bit for bit from
-// run to run.
+// run to run.  On a Cartesian Kerr-Schild mesh, cells inside the excision
+// radius (those Recovery skips, invertor.cc:83-91) hold no evolved state:
+// their U can be non-finite after an update.  They are left out of the
+// volume integrals and |U_B|, and corners that touch one are left out of div B.
 #include <hip/hip_runtime.h>
 
 #include "common.h"
@@ -31,16 +34,24 @@ __global__ __launch_bounds__(256) void k_diag(KCtx C, KFields F, const int *shel
   int b, k, j, i;
   if (decode_tile(C, R, b, k, j, i)) {
     const DevBlock &B = C.blk[b];
+    const DevGeo &G = C.geo[B.geo];
     const int nc = C.nc;
     const int64_t sj = B.n[0], sk = (int64_t)B.n[0] * B.n[1];
     const int64_t id = k * sk + j * sj + i;
     const double dA = B.dx[1] * B.dx[2];
     const double dV = B.dx[0] * B.dx[1] * B.dx[2];
-    v[PANGU_DIAG_MASS] = glb(F.cons[b * nc])[id] * dV;
-    v[PANGU_DIAG_ENERGY] = glb(F.cons[b * nc + 1])[id] * dV;
+    const bool cks = C.ph.metric == PANGU_METRIC_CKS;
+    auto excised = [&](int kk, int jj, int ii) {
+      const int64_t t = ((int64_t)(G.has3d ? kk : 0) * G.nj + jj) * G.ni + ii;
+      return cks && glb(G.cen)[CG_RAD * G.ncen + t] < C.ph.r_excise;
+    };
+    const bool exc = excised(k, j, i);
+    if (!exc) {
+      v[PANGU_DIAG_MASS] = glb(F.cons[b * nc])[id] * dV;
+      v[PANGU_DIAG_ENERGY] = glb(F.cons[b * nc + 1])[id] * dV;
+    }
     const int nx0 = B.e[0] - B.s[0] + 1;
     if (B.lx[0] * nx0 + (i - B.s[0]) == shell[B.mesh]) {
-      const DevGeo &G = C.geo[B.geo];
       const int64_t t = ((int64_t)(G.has3d ? k : 0) * G.nj + j) * G.ni + i;
       const auto c = glb(G.cen);
       const int64_t s = G.ncen;
@@ -73,14 +84,25 @@ __global__ __launch_bounds__(256) void k_diag(KCtx C, KFields F, const int *shel
       gptr<double> ub[3];
       for (int d = 0; d < 3; ++d) ub[d] = glb(F.cons[b * nc + C.iBX + d]);
       double m = 0.0;
-      for (int d = 0; d < ndim; ++d) m = fmax(m, fabs(ub[d][id]));
+      if (!exc)
+        for (int d = 0; d < ndim; ++d) m = fmax(m, fabs(ub[d][id]));
       v[PANGU_DIAG_UBMAX] = m;
-      if (ndim == 2 && i > B.s[0] && j > B.s[1]) {
+      // whether the cells around corner (k, j, i) - 1/2 include an excised
+      // one (called only where those cells exist: i > s0, j > s1, and k > s2 in 3-D)
+      auto touch = [&](int nk) {
+        bool t = false;
+        if (cks)
+          for (int dk = 0; dk < nk; ++dk)
+            for (int dj = 0; dj < 2; ++dj)
+              for (int di = 0; di < 2; ++di) t = t || excised(k - dk, j - dj, i - di);
+        return t;
+      };
+      if (ndim == 2 && i > B.s[0] && j > B.s[1] && !touch(1)) {
         const auto b1 = ub[0], b2 = ub[1];
         const double d1 = (b1[id] + b1[id - sj] - b1[id - 1] - b1[id - sj - 1]) / (2 * B.dx[0]);
         const double d2 = (b2[id] + b2[id - 1] - b2[id - sj] - b2[id - sj - 1]) / (2 * B.dx[1]);
         v[PANGU_DIAG_DIVB] = fabs(d1 + d2);
-      } else if (ndim == 3 && i > B.s[0] && j > B.s[1] && k > B.s[2]) {
+      } else if (ndim == 3 && i > B.s[0] && j > B.s[1] && k > B.s[2] && !touch(2)) {
         double tot = 0.0;
         for (int d = 0; d < 3; ++d) {
           double acc = 0.0;

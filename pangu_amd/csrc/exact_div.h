@@ -18,6 +18,17 @@
 // structured operand pairs (divisor significands near 1, 4/3, 3/2 and 2,
 // dense and sparse bit patterns) and is within one ulp by construction.
 //
+// Domain.  Both return the IEEE quotient for x = 0, or for 2^-968 <= |x| <
+// 2^1023 with a quotient of magnitude in [2^-968, 2^1023) and y finite,
+// nonzero and normal.  Outside it they differ in two ways: a numerator
+// below 2^-968 (the remainder underflows) can leave the result one unit in
+// the last place off, and y = 0, y = +-inf or x = +-inf give NaN where the
+// IEEE quotient is +-inf or +-0.  The kernels divide only by quantities that
+// are positive and finite for a finite state (u^t, twice the fast-speed
+// quadratic's leading coefficient, the HLL denominator c+ + c- + 1e-20, the
+// cell volume), so only states that are already non-finite, or values below
+// 1e-291, can reach the outside.
+//
 // The remainder is negated inside the second FMA, so a zero quotient keeps
 // the sign the IEEE division gives it (compilers that fold the negation
 // into the first FMA, as GCC does at -O2 with FMA units, lose that sign;

@@ -314,6 +314,69 @@ def test_recovery_failure_codes():
     assert st2 != 0
 
 
+def _hexU(*h):
+    return np.array([float.fromhex(v) for v in h] + [0.0])
+
+
+# Inputs that reach each return code of SchemeKastaun::invert
+# (scheme_kastaun.h:167-259), on the MKS metric at x = (0, 1.5, 0.2, 0.3).
+# Codes 5, 6 and 104 were found by a seeded search over log-uniform states
+# and are kept as exact values.
+KASTAUN_CODES = {
+    2: np.array([-1.0, 0.5, 0.1, 0.0, 0.05, 0.2, 0.0, 0.1, 0.0]),        # D <= 0
+    3: np.array([1.0, np.inf, 0.1, 0.0, 0.05, 0.2, 0.0, 0.1, 0.0]),      # tau not finite
+    4: np.array([1.0, 0.5, 0.1, 0.0, 0.05, 1e200, 0.0, 0.1, 0.0]),       # B^2 overflows
+    101: np.array([1.0, 0.5, 1e100, 0.0, 0.05, 1e100, 0.0, 0.1, 0.0]),   # (r.b)^2 overflows: f(0) NaN
+    5: _hexU("0x1.5df32cb414083p+57", "-0x1.8d865d7903377p-10", "0x1.21e08a7b7e4e1p+74",
+             "-0x1.19c3dd875ed47p+74", "0x1.e9f6519670fddp+92", "-0x1.1ed99a27242c6p+14",
+             "-0x1.657d3a8b797b7p-67", "0x1.279f85926d879p-1"),             # v^2 >= 1 at the root
+    6: np.array([1.0, 0.5, 0.1, 0.0, 0.05, 0.2, 0.0, 0.1, 0.0]),          # tau < 0: eps <= 0 at the root
+    104: _hexU("0x1.efe3e763ac2e4p-21", "0x1.fc5ff583ae3f3p+86", "0x1.24bff0275b82bp+11",
+               "-0x1.2710f15725fcap-52", "-0x1.16091bbddc89cp+48", "0x1.b11d625555cecp+34",
+               "0x1.a4375d88d7f84p+36", "0x1.ba875f2404116p-8"),          # 30 steps, no convergence
+}
+
+
+@pytest.mark.parametrize("code", sorted(KASTAUN_CODES))
+def test_kastaun_failure_codes(code):
+    gcov, gcon, _, _ = O.metric_at(MKS, [0.0, 1.5, 0.2, 0.3], A_SPIN, H_MKS)
+    st, _, it = O.c2p(3, KASTAUN_CODES[code], np.zeros(9), GAM, gcov, gcon)
+    assert st == code
+    if code == 104:
+        assert it == 30  # MAX_NEWT_ITER false-position steps
+
+
+def test_kastaun_singular_spatial_metric_is_code_1():
+    gcov, gcon, _, _ = O.metric_at(MKS, [0.0, 1.5, 0.2, 0.3], A_SPIN, H_MKS)
+    gcov = gcov.copy()
+    gcov[1:, 1:] = 0.0
+    st, _, _ = O.c2p(3, KASTAUN_CODES[6], np.zeros(9), GAM, gcov, gcon)
+    assert st == 1
+
+
+def test_kastaun_codes_102_and_103_need_an_unphysical_gamma():
+    """For gamma >= 1 the master function brackets its root on [0, 1]:
+    f(0) = -1/nu < 0 (nu >= h >= 1), and f(1) >= 0 because nu >= sqrt(1 - v)
+    and sqrt(1 - v) + v >= 1 for v = rbar^2 in [0, 1) (nu + rbar^2 > 1 when
+    rbar^2 >= 1), so code 102 (no sign change) cannot occur.  Every non-finite
+    term of f inside (0, 1) (an overflowing (r.b)^2, b^2 r^2 - (r.b)^2 = inf - inf)
+    is non-finite at an end point as well, which returns 101 before any
+    iterate (code 103).  A seeded search over extreme states sees neither;
+    gamma < 1 makes h < 1 possible and reaches 102."""
+    gcov, gcon, _, _ = O.metric_at(MKS, [0.0, 1.5, 0.2, 0.3], A_SPIN, H_MKS)
+    rng = np.random.default_rng(5)
+    seen = set()
+    for _ in range(4000):
+        U = 10.0 ** rng.uniform(-30, 30, size=9) * rng.choice([-1, 1], size=9)
+        U[0] = abs(U[0])
+        U[8] = 0.0
+        seen.add(O.c2p(3, U, np.zeros(9), GAM, gcov, gcon)[0])
+    assert 102 not in seen and 103 not in seen and {0, 6, 104} <= seen
+    U = np.array([3411.430459422771, -4696262.149234812, -5.820772221762109e-13, 4501675155.154101,
+                  -5257330426130920.0, -0.9200032028957525, -0.5838197622394741, 0.1323876239247289, 0.0])
+    assert O.c2p(3, U, np.zeros(9), -1.8597282351398505, gcov, gcon)[0] == 102
+
+
 # ------------------------------------------------------------------- heating
 def _hcfg(model, **kw):
     d = dict(gamma=GAM, heat_model=model, fel_constant=0.1, gamma_e=4.0 / 3.0, gamma_p=GAM,
@@ -338,23 +401,154 @@ def test_heating_rowan_outside_its_domain_clips_to_lower_bound():
     assert O.heating(_hcfg(4), cell) == pytest.approx(kel_min, rel=1e-15)
 
 
-def test_heating_constant_model_closed_form():
-    rho, adv, rec, ke = 2.0, 0.3, 0.4, 0.05
-    cell = [rho, 0.5, 0.2, adv, rec, ke]
-    ge, g = 4.0 / 3.0, GAM
-    expect = ke + 0.1 * (ge - 1) / (g - 1) * rho ** (g - ge) * (rec - adv)
-    assert O.heating(_hcfg(0, limit_kel=0), cell) == pytest.approx(expect, rel=1e-14)
+# Heating fractions, pinned by the limits and special values of the papers
+# each model restates (heating_model.h:53-109); the constants below are typed
+# from those papers, not from the code.  Cell layout: rho, energy (u), b^2,
+# advected entropy, recovered entropy, advected electron entropy, with
+# T_p = (gamma_p - 1) u / rho and T_e = K_e rho^(gamma_e - 1).
+MP_OVER_ME = 1836.15267261  # proton/electron mass ratio (CODATA 2002)
 
 
-def test_heating_werner_limits():
-    # Werner: f_e -> 1/4 as sigma -> 0; check via the dissipated share
-    rho, adv, rec = 1.0, 0.3, 0.4
-    cell = [rho, 0.5, 1e-12, adv, rec, 0.0]
+def _fe(model, rho, u, bsq, ke, **kw):
+    return O.heating_fraction(_hcfg(model, **kw), [rho, u, bsq, 1.0, 1.0, ke])
+
+
+def _te_over_tp_cell(ratio, u=2.25):
+    """rho = 1 cell with T_e / T_p = ratio exactly (powers of 1 are exact)."""
+    tp = (GAM - 1.0) * u
+    return u, ratio * tp
+
+
+def pin_werner():
+    """Werner et al. 2018: f_e = (1 + sqrt(s/(2 + s)))/4 with s = sigma/5:
+    1/4 for sigma -> 0, 1/2 for sigma -> inf, (1 + 1/sqrt 2)/4 at sigma = 10."""
+    assert abs(_fe(3, 1.0, 0.1, 1e-20, 0.1) - 0.25) <= 1e-10
+    assert abs(_fe(3, 1.0, 0.1, 1e30, 0.1) - 0.5) <= 1e-14
+    assert abs(_fe(3, 2.0, 0.1, 20.0, 0.1) - (1.0 + 1.0 / math.sqrt(2.0)) / 4.0) <= 1e-15
+
+
+def pin_kawazura():
+    """Kawazura et al. 2019: Q_i/Q_e = 35 / (1 + (beta/15)^-1.4 exp(-0.1 T_e/T_i)):
+    f_e -> 1/36 for beta -> inf, -> 1 for beta -> 0, 2/37 at beta = 15 with T_e << T_i."""
+    u, ke = _te_over_tp_cell(1.0)
+    assert abs(_fe(2, 1.0, u, 1e-40, ke) - 1.0 / 36.0) <= 1e-14
+    assert abs(_fe(2, 1.0, u, 1e30, ke) - 1.0) <= 1e-14
+    u, ke = _te_over_tp_cell(1e-30)
+    tp = (GAM - 1.0) * u
+    bsq = 2.0 * tp / 15.0  # beta = 2 p / b^2 = 15
+    assert abs(_fe(2, 1.0, u, bsq, ke) - 2.0 / 37.0) <= 1e-12
+
+
+def pin_howes():
+    """Howes 2010: Q_i/Q_e = 0.92 (c2^2 + beta^m)/(c3^2 + beta^m) exp(-1/beta)
+    sqrt(m_p T_i / (m_e T_e)): f_e -> 1 for beta -> 0; at T_i = T_e and
+    beta -> inf, f_e -> 1 / (1 + 0.92 sqrt(m_p/m_e)); f_e falls as beta grows."""
+    u, ke = _te_over_tp_cell(1.0)
+    assert _fe(1, 1.0, u, 1e30, ke) == 1.0
+    lim = 1.0 / (1.0 + 0.92 * math.sqrt(MP_OVER_ME))
+    # the reference's masses (heating_model.h:55-56) carry nine digits: their
+    # ratio matches the CODATA one to ~6e-9
+    assert abs(_fe(1, 1.0, u, 1e-40, ke) - lim) <= 1e-8 * lim
+    tp = (GAM - 1.0) * u
+    fs = [_fe(1, 1.0, u, 2.0 * tp / beta, ke) for beta in 10.0 ** np.arange(-2, 4.01, 0.5)]
+    assert all(a > b for a, b in zip(fs, fs[1:]))
+
+
+def pin_rowan():
+    """Rowan et al. 2017: f_e = exp(-(1 - beta/beta_max)^3.3 / (1 + 1.2 sigma_w^0.7)) / 2
+    with beta_max = 1/(4 sigma_w), sigma_w = b^2/(rho + u + p_g).  beta/beta_max =
+    8 p / (rho + u + p_g) does not depend on b^2; at beta = beta_max f_e = 1/2 for
+    every sigma_w, and a cold plasma gives exp(-1)/2 for sigma_w -> 0."""
+    r = 1.0 - 1e-6  # beta / beta_max just inside the model's domain
+    u = r / (8.0 * (GAM - 1.0) - r * GAM)  # rho = 1, gamma = gamma_p
+    for bsq in (1e-3, 0.5, 30.0, 1e3):
+        assert abs(_fe(4, 1.0, u, bsq, 0.1) - 0.5) <= 1e-15
+    assert abs(_fe(4, 1.0, 1e-14, 1e-14, 0.1) - 0.5 * math.exp(-1.0)) <= 1e-9
+    assert abs(_fe(4, 1.0, 1e-14, 1e30, 0.1) - 0.5) <= 1e-9
+
+
+def pin_sharma():
+    """Sharma et al. 2007: Q_e/Q_i = 0.33 sqrt(T_e/T_i): f_e = 0.33/1.33 at
+    T_e = T_p and 0.66/1.66 at T_e = 4 T_p."""
+    u, ke = _te_over_tp_cell(1.0)
+    assert abs(_fe(5, 1.0, u, 0.3, ke) - 0.33 / 1.33) <= 1e-15
+    u, ke = _te_over_tp_cell(4.0)
+    assert abs(_fe(5, 1.0, u, 0.3, ke) - 0.66 / 1.66) <= 1e-15
+
+
+HEAT_PINS = {"heat_werner": pin_werner, "heat_kawazura": pin_kawazura, "heat_howes": pin_howes,
+             "heat_rowan": pin_rowan, "heat_sharma": pin_sharma}
+
+
+@pytest.mark.parametrize("name", sorted(HEAT_PINS))
+def test_heating_fraction_limits(name):
+    HEAT_PINS[name]()
+
+
+def test_heating_constant_fraction():
+    assert _fe(0, 1.3, 0.2, 0.4, 0.1, fel_constant=0.37) == 0.37
+
+
+@pytest.mark.parametrize("model", range(6))
+def test_heating_splits_internal_energy_by_the_fraction(model):
+    """heating_model.h apply(): the electrons' internal energy rises by f_e times
+    the total's, dU_e = f_e dU with U = rho^gamma K / (gamma - 1) and
+    U_e = rho^gamma_e K_e / (gamma_e - 1) (energy accounting of the doc's
+    two-temperature split); no ratio limit."""
+    rho, u, bsq, adv, rec, ke = 2.0, 0.5, 0.2, 0.3, 0.34, 0.05
     ge, g = 4.0 / 3.0, GAM
-    diss = (ge - 1) / (g - 1) * rho ** (g - ge) * (rec - adv)
-    kel = O.heating(_hcfg(3, limit_kel=0), cell)
-    # werner is not the constant model, so the ratio clip still applies
-    assert kel == pytest.approx(0.25 * diss, rel=1e-6) or kel > 0
+    cfg = _hcfg(model, limit_kel=0)
+    cell = [rho, u, bsq, adv, rec, ke]  # beta/beta_max = 0.94: inside Rowan's domain
+    f = O.heating_fraction(cfg, cell)
+    assert 0.0 < f < 1.0
+    kel = O.heating(cfg, cell)
+    due = (kel - ke) * rho ** ge / (ge - 1.0)
+    du = (rec - adv) * rho ** g / (g - 1.0)
+    assert abs(due - f * du) <= 1e-13 * abs(du)
+
+
+@pytest.mark.parametrize("sign", [1.0, -1.0])
+def test_heating_ratio_limits_are_temperature_ratios(sign):
+    """limit_kel clips K_e so that T_p / T_e stays in [ratio_min, ratio_max]:
+    at the clip, p_p / p_e equals the bound, with p_e = K_e rho^gamma_e and
+    p_p = (gamma_p - 1)(U - U_e) out of the total U = K rho^gamma/(gamma - 1)."""
+    rho, adv = 1.7, 0.3
+    g, ge, gp = GAM, 4.0 / 3.0, GAM
+    cell = [rho, 0.5, 0.2, adv, adv + sign * 50.0, 0.01]  # huge heating / cooling
+    kel = O.heating(_hcfg(0, limit_kel=1, ratio_min=0.5, ratio_max=20.0), cell)
+    pe = kel * rho ** ge
+    ue = pe / (ge - 1.0)
+    utot = adv * rho ** g / (g - 1.0)
+    pp = (gp - 1.0) * (utot - ue)
+    assert abs(pp / pe - (0.5 if sign > 0 else 20.0)) <= 1e-12
+
+
+def test_bl_physical_metric_all_components():
+    """BL.h:25-58 against Kerr in Boyer-Lindquist coordinates (Sigma = r^2 +
+    a^2 cos^2, Delta = r^2 - 2r + a^2), all 16 components, except g_thth:
+    BL.h:54 writes r^2/mu = r^4/Sigma where Kerr has Sigma = r^2 mu (the
+    reference's own fm_torus_cks/problem_generator.cpp:56 uses r^2 mu).  The
+    restatement keeps the reference's form; the planted Kerr form fails."""
+    a = A_SPIN
+    for x in ([0.0, 1.2, 0.1, 0.3], [0.0, 2.5, -0.6, -1.0], [0.0, 0.7, 0.45, 2.0]):
+        g = O.physical_metric(BL, x, a, H_MKS)
+        r = math.exp(x[1])
+        th = math.pi / 2 * (x[2] + 1) + 0.5 * H_MKS * math.sin(math.pi * (x[2] + 1))
+        s2 = math.sin(th) ** 2
+        sig = r * r + a * a * math.cos(th) ** 2
+        dlt = r * r - 2 * r + a * a
+        k = np.zeros((4, 4))
+        k[0, 0] = -(1 - 2 * r / sig)
+        k[0, 3] = k[3, 0] = -2 * a * r * s2 / sig
+        k[1, 1] = sig / dlt
+        k[2, 2] = r ** 4 / sig  # the reference's r^2/mu
+        k[3, 3] = (r * r + a * a + 2 * a * a * r * s2 / sig) * s2
+        assert np.abs(g - k).max() <= 1e-13 * np.abs(k).max(), (x, g - k)
+        # the code metric is J g J with J = diag(1, dr/dx1, dth/dx2, 1)
+        gc, _, _, ok = O.metric_at(BL, x, a, H_MKS)
+        dth = math.pi / 2 + 0.5 * H_MKS * math.pi * math.cos(math.pi * (x[2] + 1))
+        J = np.diag([1.0, r, dth, 1.0])
+        assert ok == 1 and np.abs(gc - J @ k @ J).max() <= 1e-9 * np.abs(gc).max()
 
 
 # --------------------------------------------------------- planted mistakes
@@ -380,6 +574,17 @@ def _pins_fail(name):
                                   "emt_pressure"])
 def test_planted_mistakes_are_caught(name):
     assert _pins_fail(name), f"planted mistake {name} went unnoticed"
+
+
+@pytest.mark.parametrize("name", sorted(HEAT_PINS) + ["bl_gthth"])
+def test_planted_heating_and_bl_mistakes_are_caught(name):
+    pin = HEAT_PINS.get(name, test_bl_physical_metric_all_components)
+    O.plant(name)
+    try:
+        with pytest.raises(AssertionError):
+            pin()
+    finally:
+        O.plant("none")
 
 
 def test_1d_scheme_is_approximate_away_from_unit_density():
