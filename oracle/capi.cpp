@@ -222,6 +222,51 @@ int oracle_c2p(int scheme, const double U[9], double prim[9], double gamma, cons
   return -1;
 }
 
+// |f(mu)| of Kastaun's master function (scheme_kastaun.h:84-108) at a
+// given mu, with the invariants computed from the conserved state as
+// SchemeKastaun::invert does (scheme_kastaun.h:167-213).  Ucont is the
+// conserved vector as a container holds it (sqrt(-g) times rho u^t,
+// T^t_t + rho u^t, T^t_i, B^i), normalised here by alpha / sqrt(-g) with
+// the energy slot holding U_E - U_rho (invertor.cc:149-174).  An
+// independent check of a root a recovery accepted.
+int oracle_kastaun_residual(const double Ucont[8], double mu, double gamma, const double gcov[16],
+                            const double gcon[16], double *res) {
+  const real(*g4)[4] = (const real(*)[4])gcov;
+  const real(*gn)[4] = (const real(*)[4])gcon;
+  const real sg = std::sqrt(std::fabs(determinant4(g4)));
+  const real alpha = 1.0 / std::sqrt(-gn[0][0]);
+  const real aos = alpha / sg;
+  real U[9];
+  U[RHO] = aos * Ucont[0];
+  U[ENY] = aos * (Ucont[1] - Ucont[0]);
+  for (int a = 0; a < 3; ++a) U[UX1 + a] = aos * Ucont[2 + a];
+  for (int a = 0; a < 3; ++a) U[BX1 + a] = aos * Ucont[5 + a];
+  const real g[6] = {g4[1][1], g4[1][2], g4[1][3], g4[2][2], g4[2][3], g4[3][3]};
+  const real det = g[0] * g[3] * g[5] + 2.0 * g[1] * g[2] * g[4] - g[0] * g[4] * g[4] -
+                   g[1] * g[1] * g[5] - g[2] * g[2] * g[3];
+  if (!(det > 0.0)) return 1;
+  const real gi[6] = {(g[3] * g[5] - g[4] * g[4]) / det, (g[2] * g[4] - g[1] * g[5]) / det,
+                      (g[1] * g[4] - g[2] * g[3]) / det, (g[0] * g[5] - g[2] * g[2]) / det,
+                      (g[1] * g[2] - g[0] * g[4]) / det, (g[0] * g[3] - g[1] * g[1]) / det};
+  const real D = U[RHO];
+  if (!(D > 0.0)) return 2;
+  const real lapse = std::sqrt(-1.0 / gn[0][0]);
+  const real Qcon0 = gn[0][0] * U[ENY] + gn[0][1] * U[UX1] + gn[0][2] * U[UX2] + gn[0][3] * U[UX3];
+  const real tau = lapse * Qcon0 - D;
+  const real S[3] = {U[UX1], U[UX2], U[UX3]};
+  const real Su[3] = {gi[0] * S[0] + gi[1] * S[1] + gi[2] * S[2], gi[1] * S[0] + gi[3] * S[1] + gi[4] * S[2],
+                      gi[2] * S[0] + gi[4] * S[1] + gi[5] * S[2]};
+  const real B[3] = {U[BX1], U[BX2], U[BX3]};
+  const real Bd[3] = {g[0] * B[0] + g[1] * B[1] + g[2] * B[2], g[1] * B[0] + g[3] * B[1] + g[4] * B[2],
+                      g[2] * B[0] + g[4] * B[1] + g[5] * B[2]};
+  const real Bsq = B[0] * Bd[0] + B[1] * Bd[1] + B[2] * Bd[2];
+  const real Ssq = Su[0] * S[0] + Su[1] * S[1] + Su[2] * S[2];
+  const real SdotB = B[0] * S[0] + B[1] * S[1] + B[2] * S[2];
+  const real bsq = Bsq / D, rsq = Ssq / (D * D), rbsq = SdotB * SdotB / (D * D * D), q = tau / D;
+  *res = std::fabs(kastaun_root(mu, D, q, bsq, rsq, rbsq, gamma));
+  return 0;
+}
+
 static Params heat_params(const oracle_cfg *c) {
   Params p{};
   p.gamma = c->gamma;

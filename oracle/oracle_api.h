@@ -124,6 +124,11 @@ void oracle_fms(double gamma, const double prim[10], const double gcov[16],
 void oracle_recon(int limiter, const double q[5], double out[2]);
 int oracle_c2p(int scheme, const double U[9], double prim[9], double gamma,
                const double gcov[16], const double gcon[16], int *iters);
+/* |f(mu)| of Kastaun's master function at mu for the container-form
+ * conserved vector Ucont (rho, energy, momentum[3], field[3]); 0 on
+ * success, 1 or 2 when the state has no invariants. */
+int oracle_kastaun_residual(const double Ucont[8], double mu, double gamma, const double gcov[16],
+                            const double gcon[16], double *res);
 /* cell = {rho, energy, b^2, advected entropy, recovered entropy, advected
  * electron entropy}: heating_model.h apply() and computeHeatingFraction() */
 double oracle_heating(const oracle_cfg *cfg, const double cell[6]);

@@ -87,6 +87,7 @@ def lib():
         L.oracle_fms.argtypes = [C.c_double, dp, dp, dp, C.c_int, dp]
         L.oracle_recon.argtypes = [C.c_int, dp, dp]
         L.oracle_c2p.argtypes = [C.c_int, dp, dp, C.c_double, dp, dp, ip]
+        L.oracle_kastaun_residual.argtypes = [dp, C.c_double, C.c_double, dp, dp, dp]
         L.oracle_plant.argtypes = [C.c_int]
         L.oracle_heating.restype = C.c_double
         L.oracle_heating.argtypes = [C.POINTER(OracleCfg), dp]
@@ -374,6 +375,17 @@ def c2p(scheme, U, prim_guess, gamma, gcov, gcon):
     it = C.c_int(0)
     st = lib().oracle_c2p(scheme, _dp(Uc), _dp(p), gamma, _dp(gc), _dp(gn), C.byref(it))
     return st, p, it.value
+
+
+def kastaun_residual(Ucont, mu, gamma, gcov, gcon):
+    """|f(mu)| of Kastaun's master function for the container-form conserved
+    vector Ucont (8 values); None where the state has no invariants."""
+    U = np.ascontiguousarray(Ucont[:8], dtype=np.float64)
+    gc = np.ascontiguousarray(gcov, dtype=np.float64)
+    gn = np.ascontiguousarray(gcon, dtype=np.float64)
+    out = C.c_double(0.0)
+    r = lib().oracle_kastaun_residual(_dp(U), float(mu), gamma, _dp(gc), _dp(gn), C.byref(out))
+    return out.value if r == 0 else None
 
 
 def heating(cfg_dict, cell):

@@ -587,6 +587,27 @@ def test_planted_heating_and_bl_mistakes_are_caught(name):
         O.plant("none")
 
 
+@pytest.mark.parametrize("seed", range(4))
+def test_kastaun_residual_vanishes_at_the_true_root(seed):
+    """oracle_kastaun_residual (the check of accepted roots) is ~0 at the
+    root mu = 1/(h W) of the primitives a container-form conserved vector
+    (sqrt(-g) U) was made from (Kastaun et al. 2021, eq. for mu), to the
+    rounding floor of its evaluation (which grows like W^2: 1e-13 W^2 here,
+    W up to about 40), and grows linearly when mu is off: mu (1 + 1e-6)
+    gives |f| of order 1e-6 mu."""
+    rng = np.random.default_rng(700 + seed)
+    gcov, gcon, gdet = _metric(rng)
+    for _ in range(20):
+        p = rand_prim(rng)
+        Uc = O.flux(GAM, p, gcov, gcon, gdet, 0)
+        W = math.sqrt(1.0 + p[2:5] @ gcov[1:, 1:] @ p[2:5])
+        mu = 1.0 / ((1.0 + GAM * p[1] / p[0]) * W)
+        r = O.kastaun_residual(Uc, mu, GAM, gcov, gcon)
+        assert r is not None and r <= 1e-13 * W * W, (r, W)
+        rq = O.kastaun_residual(Uc, mu * (1.0 + 1e-6), GAM, gcov, gcon)
+        assert 1e-8 * mu < rq < 1e-4 * mu, (rq, mu)
+
+
 def test_1d_scheme_is_approximate_away_from_unit_density():
     """Documents reference behaviour reproduced by the oracle: the 1D-W
     fallback reports success with an inexact state when rho != 1."""
