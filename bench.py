@@ -9,10 +9,12 @@ update) over the whole mesh; synthetic initial data (FM torus with a seeded
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode graph|fused|tasks]
 
 Prints one JSON line.  Besides the metric it reports the HBM roofline of the
-dominant kernel (algorithmic bytes per launch / launch duration from HIP
-events on the simulation stream) and the CPU baseline: the oracle
-restatement of the reference algorithm timed on this host's cores on two
-cycles of the same mesh.
+dominant kernel (the state-model bytes of one RK stage / its launch
+duration from HIP events on the simulation stream), the same kernel and the
+whole cycle against measured bytes and executed FP64 flops (per-launch
+figures from the committed rocprofv3 counter summary), and the CPU
+baseline: the oracle restatement of the reference algorithm timed on this
+host's cores on three cycles of the same mesh.
 """
 import argparse
 import json
@@ -29,41 +31,45 @@ METRIC = "FP64 zone-cycles/s, 3D MKS torus, 1 MI355X; % HBM roofline; batched/le
 HBM_PEAK_GBS = 8000.0
 FP64_VALU_PEAK_TFLOPS = 78.6
 
-# Algorithmic HBM bytes of one launch of each kernel, per interior zone of
-# the mesh (FP64 words x 8; n_p = n_c = 10 for this workload).  Each array
-# element a kernel must read or write counts once; stencil re-reads, the
-# one-cell halo of faces that CT needs (<= 6 %) and the geometry tables
-# (x3-independent, amortised over the 32 planes of a block) are excluded:
-#   flux_x1/2/3        read P (10) + write flux (10) + alfven (1)            = 21
-#   div_update_source  read the 7 non-B fluxes of 3 directions (21) + the 6
-#                      transverse Riemann B fluxes (the kernel forms the edge
-#                      EMFs and CT face B fluxes from them) + U_init (10) +
-#                      the P the source term uses (8: rho, u, u~, B) + write
-#                      U (10)                                                = 55
-#   recovery_heating   read U (10) + the stale entropy P_ENT the heating
-#                      reads (1) + write P (9: rho, u, u~, B, K_e)           = 20
-#   fix_cons           read P (10) + write P (2: the entropies; rho, u, u~
-#                      only where floored) + write U (10)                    = 22
-#   ct_emf             read 6 B-fluxes + write 3 EMFs                        = 9
-WORDS_PER_ZONE = {"flux_x1": 21, "flux_x2": 21, "flux_x3": 21, "div_update_source": 55,
-                  "recovery_heating": 20, "fix_cons": 22, "ct_emf": 9}
+# Algorithmic HBM bytes (the state model of the path): per interior zone
+# and RK stage, read P of the input container (10 words) and U of the
+# stage-initial container (10), write U (10) and P (9: the recovery leaves
+# the entropy untouched) of the output container, and read the entropy the
+# heating uses (1): 40 words = 320 B per zone-stage, 640 B per zone-cycle.
+# Kastaun reads no guess and the robuster path keeps no flag.  Geometry
+# (x3-independent tables, amortised over the 32 planes of a block) is left
+# out.  The dominant kernel is charged one stage's state bytes per launch:
+# what the whole stage must move at least, whichever kernel moves it.
+STATE_WORDS_PER_ZONE_STAGE = 40
 
 # Launch parameters of this workload as pangu_sim_autotune chose them on an
 # MI355X (tile order per kernel family: 0 plane-major, 1 column-major).  The
 # bench applies them instead of re-tuning on every run, so the HBM traffic
-# quoted beside the timing comes from the same launches: the tile order
-# changes how often the caches re-fetch a line.  --autotune re-tunes.
+# and FP64 work quoted beside the timing come from the same launches: the
+# tile order changes how often the caches re-fetch a line.  --autotune
+# re-tunes.
 TUNED = {"order": {"fix_cons": 1, "flux_x1": 1, "flux_x2": 0, "flux_x3": 1, "ct": 0,
                    "div_update": 1, "recovery": 1},
          "split_source": 0, "share_reconstruction": 0}
 
-# HBM bytes per launch under TUNED, measured with rocprofv3 --pmc FETCH_SIZE
-# and WRITE_SIZE on this workload, each scaled by the same counter's reading
-# for a plain copy kernel of known size (profiles/counters_torus3d_256x128x128.md).
-MEASURED_TRAFFIC = {
-    "div_update_source": 2218881421, "flux_x1": 881801660, "flux_x2": 931134937,
-    "flux_x3": 857023605, "fix_cons": 1120759286, "recovery_heating": 1121017255,
-    "ct_emf": 353569598,
+# Per launch under TUNED, taken from the rocprofv3 counter runs recorded in
+# PROFILE (they are not measured by this script):
+#  - bytes: FETCH_SIZE + WRITE_SIZE, each scaled by the same counter's
+#    reading for a plain copy kernel of known size;
+#  - fp64_flops: 64 x SQ_INSTS_VALU_FLOPS_FP64 (adds, muls and
+#    transcendentals once, FMAs twice), i.e. the FP64 work the GPU executes.
+PROFILE = "profiles/counters_torus3d_256x128x128.md"
+
+# Learned Kastaun bracket: the model serves cells whose previous recovery
+# took at least this many false-position steps.
+LEARNED_GATE = 4
+MEASURED = {
+    "div_update_source": {"bytes": 2218881421, "fp64_flops": 2.649e9},
+    "flux_x1": {"bytes": 881801660, "fp64_flops": 5.640e9},
+    "flux_x2": {"bytes": 931134937, "fp64_flops": 5.584e9},
+    "flux_x3": {"bytes": 857023605, "fp64_flops": 5.098e9},
+    "fix_cons": {"bytes": 1120759286, "fp64_flops": 4.790e9},
+    "recovery_heating": {"bytes": 1121017255, "fp64_flops": 6.558e9},
 }
 
 
@@ -83,27 +89,53 @@ def parse():
     ap.add_argument("--ensemble-scale", type=float, default=1.0,
                     help="fraction of the 96-torus + 3x121-block ensemble for the batched "
                          "vs plain comparison (0 skips it)")
-    ap.add_argument("--learned-samples", type=int, default=1 << 20,
-                    help="training states for the learned Kastaun bracket (0 skips it)")
+    ap.add_argument("--learned-samples", type=int, default=3 << 20,
+                    help="synthetic training states for the learned Kastaun bracket, besides "
+                         "about 1e6 harvested from the 2-D torus run (0 skips it)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="also run a plain copy kernel of this many doubles (counter calibration)")
     return ap.parse_args()
 
 
+def host_cpus():
+    """Host cores this process may use and how they were counted: the
+    affinity mask, capped by the cgroup CPU quota when one is set (a box
+    that shows more CPUs than its share would otherwise be oversubscribed);
+    with nproc, OMP_NUM_THREADS and the CPU model string."""
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except Exception:
+        affinity = os.cpu_count() or 1
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = max(1, int(int(q) // int(period)))
+    except Exception:
+        pass
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except Exception:
+        pass
+    use = min(affinity, quota) if quota else affinity
+    return use, {"nproc": os.cpu_count(), "affinity": affinity, "cgroup_quota_cpus": quota,
+                 "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"), "cpu_model": model}
+
+
 def cpu_baseline(cfg_small, cycles):
     """Oracle (C++ restatement of the reference algorithm, metric recomputed
-    per evaluation as the reference does) on a bounded sample."""
+    per evaluation as the reference does) on a bounded sample, OpenMP over
+    every host core this process may use."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py as O
     O.build()
     d = dict(cfg_small)
     d["cache_geometry"] = 0
-    nthreads = os.cpu_count() or 1
-    try:
-        nthreads = len(os.sched_getaffinity(0))
-    except Exception:
-        pass
-    nthreads = min(nthreads, 16)
+    nthreads, info = host_cpus()
     s = O.OracleSim(d, nthreads=nthreads)
     s.init()
     t0 = time.perf_counter()
@@ -115,6 +147,8 @@ def cpu_baseline(cfg_small, cycles):
         "unit": "zone-cycles/s",
         "cores": nthreads,
         "kind": "port",
+        "label": "PANGU-restatement CPU (the oracle; pangu.host needs Parthenon and Kokkos, absent)",
+        "host": info,
         "sample": (f"{cycles} RK2 cycle(s) of the same 3D torus physics at "
                    f"{d['nx'][0]}x{d['nx'][1]}x{d['nx'][2]} zones in "
                    f"{d['nb'][0] * d['nb'][1] * d['nb'][2]} blocks, "
@@ -219,34 +253,66 @@ def refined_run(cycles=20, warmup=2):
     return out
 
 
-def learned_vs_plain(cfg, samples, cycles=20, delta=0.01):
-    """Learned Kastaun bracket: trained on synthetic states, evaluated on a
-    held-out set and in the 3D torus run (iterations and speed)."""
-    from pangu_amd import mlp
+def learned_vs_plain(cfg3, samples, cycles=20, delta=0.01, gate=LEARNED_GATE):
+    """Learned Kastaun bracket on the 2-D MKS torus workload (512x256,
+    Kastaun recovery) and on the 3-D benchmark torus.  Training pairs: the
+    seeded synthetic draws (`samples` states) plus states harvested from the
+    plain 2-D run itself (every interior cell, every second cycle, eight
+    times).  Reported: held-out iterations (synthetic and harvested states,
+    plain vs learned at the same 1e-10 step tolerance), and for each run
+    zone-cycles/s and false-position steps per cell, over all cells and over
+    the cells the gate sends to the model."""
+    from pangu_amd import configs, mlp
     from pangu_amd.sim import HipSim
+    cfg2 = dict(configs.torus2d(512, 256), recovery=1)
     m = mlp.Mlp(seed=1)
     t0 = time.perf_counter()
-    m.make_dataset(samples, seed=11)
-    loss = m.train(epochs=20, batch=1024, lr=2e-3)
+    synth = m.make_dataset(samples, seed=11)
+    src = HipSim(cfg2)
+    src.init()
+    harvested = 0
+    for _ in range(8):
+        src.step(2, "graph")
+        harvested += m.harvest(src, 0)
+    t_data = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    loss = m.train(epochs=6, batch=1024, lr=2e-3)
     t_train = time.perf_counter() - t0
-    m.make_dataset(max(samples // 4, 1 << 16), seed=12)   # held-out states
-    ev = m.evaluate(delta=delta)
-    out = {"train_states": samples, "train_s": t_train, "loss_first_last": [float(loss[0]), float(loss[-1])],
-           "heldout": ev, "delta": delta,
-           "iteration_reduction": 1.0 - ev["learned_iters"] / ev["plain_iters"]}
-    zones = cfg["nx"][0] * cfg["nx"][1] * cfg["nx"][2]
-    for learned in (False, True):
-        h = HipSim(cfg)
-        h.init()
-        if learned:
-            m.attach(h, delta=delta)
-        h.step(2, "graph")
-        ms = h.timed_cycles(cycles, "graph")
-        it = float(h.interior(h.get("c2p_iters")).mean())
-        key = "learned" if learned else "plain"
-        out[f"torus3d_{key}_zone_cycles_per_s"] = zones * cycles / (ms * 1e-3)
-        out[f"torus3d_{key}_mean_iters"] = it
-        h.close()
+    out = {"train_pairs": synth + harvested, "train_synthetic": synth, "train_harvested": harvested,
+           "data_s": t_data, "train_s": t_train, "loss_first_last": [float(loss[0]), float(loss[-1])],
+           "delta": delta, "gate_min_prev_iters": gate}
+    m.make_dataset(1 << 18, seed=12)  # held-out synthetic states
+    out["heldout_synthetic"] = m.evaluate(delta=delta)
+    m.make_dataset(64, seed=13)       # held-out states of the same run, 16 cycles later
+    src.step(16, "graph")
+    m.harvest(src, 0)
+    src.close()
+    out["heldout_harvested"] = m.evaluate(delta=delta)
+    for name, cfg in (("torus2d_512x256", cfg2), ("torus3d", cfg3)):
+        zones = configs.zones(cfg)
+        res = {}
+        for learned in (False, True):
+            h = HipSim(cfg)
+            h.init()
+            if learned:
+                m.attach(h, delta=delta, gate=gate)
+            h.step(10, "graph")
+            ms = h.timed_cycles(cycles, "graph")
+            prev = h.interior(h.get("c2p_iters")).copy()
+            h.step(1, "graph")
+            it = h.interior(h.get("c2p_iters"))
+            res["learned" if learned else "plain"] = (zones * cycles / (ms * 1e-3), prev, it)
+            h.close()
+        hard = res["plain"][1] >= gate
+        r = {"zone_cycles_per_s_plain": res["plain"][0], "zone_cycles_per_s_learned": res["learned"][0],
+             "speed_ratio": res["learned"][0] / res["plain"][0],
+             "steps_per_cell_plain": float(res["plain"][2].mean()),
+             "steps_per_cell_learned": float(res["learned"][2].mean()),
+             "gated_fraction": float(hard.mean())}
+        if hard.any():
+            r["steps_gated_cells_plain"] = float(res["plain"][2][hard].mean())
+            r["steps_gated_cells_learned"] = float(res["learned"][2][hard].mean())
+        out[name] = r
     m.close()
     return out
 
@@ -289,12 +355,55 @@ def main():
     # Dominant kernel: HIP events recorded on the simulation stream after
     # every launch of a few fused cycles give each kernel's launch duration.
     kern = sim.time_kernels(ncycles=3)
-    dom = max((k for k in kern if k in WORDS_PER_ZONE), key=lambda k: kern[k][1])
+    dom = max((k for k in kern if k in MEASURED), key=lambda k: kern[k][1])
     calls, ms_cycle = kern[dom]
     launch_ms = ms_cycle / calls
-    bytes_launch = WORDS_PER_ZONE[dom] * 8 * zones
-    achieved = bytes_launch / (launch_ms * 1e-3) / 1e9
-    traffic = MEASURED_TRAFFIC.get(dom) if tuning == TUNED and not args.small else None
+    state_stage = STATE_WORDS_PER_ZONE_STAGE * 8 * zones
+    achieved = state_stage / (launch_ms * 1e-3) / 1e9
+    profiled = tuning == TUNED and not args.small and args.mode != "tasks"
+    ms_step = ms / args.steps
+    roofline = {
+        "bound": "hbm",
+        "kernel": dom,
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": MEASURED[dom]["bytes"] if profiled else None,
+        "traffic_source": PROFILE + " (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated; not measured in this run)",
+        "launch_ms": launch_ms,
+        "algorithmic_bytes": state_stage,
+        "algorithmic_bytes_model": "state model: 40 words per interior zone per RK stage (P in, U_init in, "
+                                   "U out, P out less the entropy, the heating's entropy)",
+    }
+    if profiled:
+        # the same kernel against measured bytes and against the binding roof
+        # (FP64 vector ALU), and the whole cycle against all three
+        m = MEASURED[dom]
+        roofline["hbm_measured"] = {
+            "achieved": m["bytes"] / (launch_ms * 1e-3) / 1e9, "unit": "GB/s",
+            "frac": m["bytes"] / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        roofline["fp64_valu"] = {
+            "achieved": m["fp64_flops"] / (launch_ms * 1e-3) / 1e12, "peak": FP64_VALU_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": m["fp64_flops"] / (launch_ms * 1e-3) / 1e12 / FP64_VALU_PEAK_TFLOPS,
+            "flops_source": PROFILE + " (64 x SQ_INSTS_VALU_FLOPS_FP64)"}
+        cyc_bytes = sum(v["bytes"] * kern[k][0] for k, v in MEASURED.items() if k in kern)
+        cyc_flops = sum(v["fp64_flops"] * kern[k][0] for k, v in MEASURED.items() if k in kern)
+        cycle = {
+            "ms": ms_step,
+            "state_bytes": 2 * state_stage,
+            "hbm_frac_state_model": 2 * state_stage / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "measured_bytes_main_kernels": cyc_bytes,
+            "hbm_frac_measured": cyc_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "fp64_flops_executed": cyc_flops,
+            "fp64_tflops": cyc_flops / (ms_step * 1e-3) / 1e12,
+            "fp64_frac": cyc_flops / (ms_step * 1e-3) / 1e12 / FP64_VALU_PEAK_TFLOPS,
+            "source": PROFILE + " for bytes and flops per launch (the six main kernels); "
+                                "ms from this run",
+        }
+    else:
+        cycle = {"ms": ms_step, "state_bytes": 2 * state_stage,
+                 "hbm_frac_state_model": 2 * state_stage / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS}
     out = {
         "metric": METRIC,
         "value": value,
@@ -318,17 +427,8 @@ def main():
             "ct": True, "electron_heating": True, "integrator": "rk2",
             "mode": args.mode,
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": dom,
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "launch_ms": launch_ms,
-            "algorithmic_bytes": bytes_launch,
-        },
+        "roofline": roofline,
+        "cycle_roofline": cycle,
         "autotune": tuning,
         "kernel_ms_per_cycle": {k: v[1] for k, v in kern.items()},
         "kernel_launches_per_cycle": {k: v[0] for k, v in kern.items()},

@@ -102,6 +102,8 @@ typedef struct pangu_shared {
   int32_t *const *c2p_status; /* [nblocks] recovery status code (interior) */
   int32_t *const *c2p_iters;  /* [nblocks] recovery iterations (interior) */
   double *const *flag_recovery; /* [nblocks] flag as left by recovery, before fix_recovery */
+  double *const *c2p_root;    /* [nblocks] Kastaun root mu of the last recovery where it
+                                 succeeded, NaN where it failed (interior; robuster only) */
 } pangu_shared;
 
 typedef struct pangu_ctx pangu_ctx;
@@ -148,7 +150,7 @@ int pangu_register_container(pangu_ctx *ctx, const pangu_container *c, int *hand
 /* Registers the tables shared by all containers (wave speeds, EMFs, flags,
  * masks, recovery codes): the "alfven" pack of riemann_solver/hll.cc:51-53
  * and the other package-level fields the reference tasks pack by name. */
-int pangu_register_shared(pangu_ctx *ctx, const pangu_shared *s);
+int pangu_register_shared(pangu_ctx *ctx, const pangu_shared *s);  /* again: replaces the tables */
 
 /* ---- task functions (one per reference task) ---- */
 /* FixPrimitive (fixer/primitive_fixer.cc:15) */
@@ -234,6 +236,12 @@ int pangu_set_split_source(pangu_ctx *ctx, int split);
  * interface value to the next face through a wavefront shuffle, instead of
  * reconstructing both neighbours per face.  Results are identical. */
 int pangu_set_share_reconstruction(pangu_ctx *ctx, int share);
+/* Bit d-1 of dir_mask set: the flux sweep along d runs two lanes per face,
+ * one per side (half the registers per lane, so more wavefronts in flight;
+ * the two lanes share the stencil loads and exchange their wave speeds and
+ * flux components).  Results are identical.  Ignored by an X1 sweep that
+ * shares reconstructions (pangu_set_share_reconstruction). */
+int pangu_set_pair_flux(pangu_ctx *ctx, int dir_mask);
 /* fuse = 1 (default): the fused stage runs no constrained-transport kernel;
  * the flux-divergence + update kernel forms the edge EMFs from the Riemann B
  * fluxes and the face B fluxes from them (the same arithmetic as
@@ -268,6 +276,7 @@ enum {
   PANGU_DIAG_ENERGY, PANGU_DIAG_DIVB, PANGU_DIAG_UBMAX, PANGU_NDIAG
 };
 int pangu_diagnostics(pangu_ctx *ctx, int md, const int *shell, double *out, void *stream);
+
 
 #ifdef __cplusplus
 }

@@ -85,11 +85,34 @@ int pangu_mlp_evaluate(pangu_mlp *m, double delta, double *out);
  * product every layer uses).  M, N multiples of 32, K of 16. */
 int pangu_mlp_gemm(int device, int M, int N, int K, const float *A, const float *B, float *C);
 
+/* Appends training pairs from a running simulation to the data set: every
+ * stride-th interior cell of container md of ctx (its conserved state in
+ * the recovery's normalisation and its metric), solved by the plain Kastaun
+ * recovery; solved states are kept with their root as target (*kept
+ * receives how many).  Waits for the device first.  The data set's states
+ * must share the context's adiabatic index. */
+int pangu_mlp_harvest(pangu_mlp *m, pangu_ctx *ctx, int md, int stride, int64_t *kept);
+
+/* The in-kernel inference path on host features [n][5]: the network as the
+ * recovery kernel evaluates it, one wavefront over 64 states with the
+ * activations kept in the matrix-core fragments (pangu_mlp_predict runs the
+ * training forward pass instead).  out[n] receives g in (0, 1). */
+int pangu_mlp_predict_wave(pangu_mlp *m, const float *feats, int n, float *out);
+
 /* Attaches the model to a context: its Kastaun recovery then starts from
  * the learned one-sided bracket of initial relative width delta.  NULL detaches.  The
- * model must outlive the attachment.  Runs with a model attached are off the
- * bit-for-bit parity path (results agree to the solver tolerance). */
+ * model must outlive the attachment.  The recovery kernel evaluates the
+ * network itself, in the wavefronts that hold a cell passing the gate
+ * (pangu_set_c2p_gate).  Runs with a model attached are off the bit-for-bit
+ * parity path (results agree to the solver tolerance). */
 int pangu_set_c2p_guess(pangu_ctx *ctx, pangu_mlp *m, double delta);
+
+/* Gate of the learned bracket: a cell uses it when its previous recovery
+ * (the iteration count in c2p_iters, which the context must have) took at
+ * least min_prev_iters false-position steps; other cells run the plain
+ * search, and a wavefront with no such cell skips the network.  0 (the
+ * default) uses it for every cell whose count is at hand. */
+int pangu_set_c2p_gate(pangu_ctx *ctx, int min_prev_iters);
 
 #ifdef __cplusplus
 }

@@ -79,6 +79,10 @@ int pangu_sim_nblocks(pangu_sim *s);
 /* pangu_diagnostics on container 0 (base) or 1 (stage "1"), one shell index
  * per mesh; out (host) receives nmesh * PANGU_NDIAG doubles. */
 int pangu_sim_diagnostics(pangu_sim *s, int container, const int *shell, double *out);
+/* The simulation's C-ABI context, the handles of its containers 0 (base)
+ * and 1 (stage "1") and its stream, for callers that drive pangu_hip
+ * entry points on the simulation's fields directly. */
+int pangu_sim_context(pangu_sim *s, pangu_ctx **ctx, int *handles, void **stream);
 int pangu_sim_destroy(pangu_sim *s);
 /* Writes the initial data of cfg.problem into the base container:
  * ProblemGenerator of pangu/problem/orszag_tang_vortex/problem_generator.cpp:16-69
@@ -123,6 +127,11 @@ int pangu_sim_autotune(pangu_sim *s, int cycles, int use_cache, int *out_orders,
                        int *out_share);
 /* pangu_set_c2p_guess on the simulation's context (drops the graph). */
 int pangu_sim_set_c2p_guess(pangu_sim *s, pangu_mlp *m, double delta);
+/* pangu_set_c2p_gate on the simulation's context (re-captures the cycle). */
+int pangu_sim_set_c2p_gate(pangu_sim *s, int min_prev_iters);
+/* on != 0: the Kastaun recovery records each cell's root (field 13,
+ * c2p_root: mu where it succeeded, NaN where it failed); 0 stops it. */
+int pangu_sim_record_roots(pangu_sim *s, int on);
 /* Sets a kernel family's tile order (pangu_set_tile_order) and drops the
  * captured graph so the next graph-mode step captures the new launches. */
 int pangu_sim_set_tile_order(pangu_sim *s, int kernel_family, int order);
@@ -132,6 +141,7 @@ int pangu_sim_set_split_source(pangu_sim *s, int split);
 int pangu_sim_set_fuse_ct(pangu_sim *s, int fuse);
 /* pangu_set_share_reconstruction on the simulation's context (drops the graph). */
 int pangu_sim_set_share_reconstruction(pangu_sim *s, int share);
+int pangu_sim_set_pair_flux(pangu_sim *s, int dir_mask);
 /* One task on container c0 (c1 output of update/source), host beta*dt. */
 int pangu_sim_task(pangu_sim *s, int task, int c0, int c1, double beta_dt, double *out);
 int64_t pangu_sim_field_size(pangu_sim *s, int field);

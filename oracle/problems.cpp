@@ -268,6 +268,10 @@ static int init_torus(Sim &S) {
     rhomax = kmax(rhomax, bm);
     umax = kmax(umax, um);
   }
+  // No cell inside the torus (unbound or unresolved): the reference divides
+  // by the zero maximum (NaN everywhere); the zero state is kept instead
+  // (floored to the atmosphere by the first FixPrimitive), with no field.
+  if (!(rhomax > 0.0)) return 0;
   for (int b = 0; b < S.nblk; ++b) {
     const Block &B = S.blk[b];
     for (int64_t id = 0; id < B.cells; ++id) {

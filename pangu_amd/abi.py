@@ -47,7 +47,7 @@ class PanguShared(C.Structure):
                 ("mask", C.POINTER(C.POINTER(C.c_uint8))),
                 ("c2p_status", C.POINTER(C.POINTER(C.c_int32))),
                 ("c2p_iters", C.POINTER(C.POINTER(C.c_int32))),
-                ("flag_recovery", _PPD)]
+                ("flag_recovery", _PPD), ("c2p_root", _PPD)]
 
 
 # Error codes and enumerations of the header.
@@ -92,6 +92,7 @@ PROTOTYPES = {
     "pangu_get_tile_order": (C.c_int, [_vp, C.c_int]),
     "pangu_set_split_source": (C.c_int, [_vp, C.c_int]),
     "pangu_set_share_reconstruction": (C.c_int, [_vp, C.c_int]),
+    "pangu_set_pair_flux": (C.c_int, [_vp, C.c_int]),
     "pangu_set_fuse_ct": (C.c_int, [_vp, C.c_int]),
     "pangu_diagnostics": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), _dp, _vp]),
     "pangu_stage": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_double, _dp, C.c_double,

@@ -51,12 +51,12 @@ struct pangu_ctx {
   double **dalfven = nullptr, **demf = nullptr, **dflag = nullptr, **dflag_rec = nullptr;
   uint8_t **dmask = nullptr;
   int32_t **dstatus = nullptr, **diters = nullptr;
+  double **droot = nullptr;
   bool shared_ok = false;
-  // learned Kastaun bracket (pangu_set_c2p_guess)
+  // learned Kastaun bracket (pangu_set_c2p_guess, pangu_set_c2p_gate)
   pangu_mlp *mlp = nullptr;
   double guess_delta = 0.0;
-  float *dguess = nullptr;             // one float per cell of every block
-  float **dguess_tab = nullptr;        // [nblocks] pointers into dguess
+  int guess_gate = 0;
   // scratch
   double *dred = nullptr;
   int *dcluster = nullptr;
@@ -64,6 +64,7 @@ struct pangu_ctx {
   // diagnostics: per-tile partial sums, per-mesh tile ranges, shell indices
   double *ddiag_part = nullptr;
   int *ddiag_mtile = nullptr, *ddiag_shell = nullptr;
+  std::vector<int> diag_mtile;         // first interior tile of each mesh (nmesh + 1)
   LaunchCtx L;
   std::string err;
 };
@@ -603,6 +604,18 @@ int pangu_create_batch(const pangu_config *cfgs, int nmesh, const pangu_block *b
   if (!r) r = upload(c, &c->dcluster, std::vector<int>((size_t)c->nplanes, 0).data(),
                      (size_t)c->nplanes);
   if (!r) r = upload(c, &c->dest, std::vector<double>(nmesh, 0.0).data(), nmesh);
+  if (!r) {
+    // interior tiles are grouped by block and therefore by mesh
+    c->diag_mtile.assign(nmesh + 1, 0);
+    const int zero[3] = {0, 0, 0};
+    int64_t tot = 0;
+    for (int m = 0; m < nmesh; ++m) {
+      c->diag_mtile[m] = (int)tot;
+      for (int b = c->mfirst[m]; b < c->mfirst[m + 1]; ++b) tot += range_tiles(c->hb[b], zero, zero, 0);
+    }
+    c->diag_mtile[nmesh] = (int)tot;
+    if (tot != c->L.interior.ntiles) r = fail(c, PANGU_EINVAL, "diagnostics: tile count mismatch");
+  }
   if (r) {
     fprintf(stderr, "pangu_create: %s\n", c->err.c_str());
     pangu_destroy(c);
@@ -667,8 +680,8 @@ int pangu_destroy(pangu_ctx *c) {
     for (int d = 0; d < 3; ++d) (void)hipFree(ct.dflux[d]);
   }
   void *ptrs[] = {c->dblk,    c->dgeo,  c->dnbr,     c->dalfven, c->demf,  c->dflag,
-                  c->dflag_rec, c->dmask, c->dstatus, c->diters,  c->dred,  c->dcluster,
-                  c->dest,    c->dguess, c->dguess_tab, c->ddiag_part, c->ddiag_mtile,
+                  c->dflag_rec, c->dmask, c->dstatus, c->diters,  c->droot, c->dred,  c->dcluster,
+                  c->dest,    c->ddiag_part, c->ddiag_mtile,
                   c->ddiag_shell, c->dxfill, c->dxprol, c->dxflux};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -701,21 +714,25 @@ int pangu_set_c2p_guess(pangu_ctx *c, pangu_mlp *m, double delta) {
   if (m && !(delta > 0.0 && delta < 1.0)) return fail(c, PANGU_EINVAL, "delta must be in (0, 1)");
   if (m && c->cfg.recovery != PANGU_RECOVERY_ROBUSTER)
     return fail(c, PANGU_EINVAL, "the learned bracket applies to the robuster (Kastaun) recovery");
-  (void)hipSetDevice(c->device);
-  if (m && !c->dguess) {
-    int64_t total = 0;
-    std::vector<int64_t> off(c->nblocks);
-    for (int b = 0; b < c->nblocks; ++b) {
-      off[b] = total;
-      total += c->hb[b].cells;
-    }
-    HIPCHK(c, hipMalloc((void **)&c->dguess, (size_t)total * sizeof(float)));
-    std::vector<float *> tab(c->nblocks);
-    for (int b = 0; b < c->nblocks; ++b) tab[b] = c->dguess + off[b];
-    if (upload(c, &c->dguess_tab, tab.data(), tab.size())) return PANGU_EDEVICE;
-  }
+  MlpFrag W;
+  if (m && !mlp_view(m, W)) return fail(c, PANGU_ESTATE, "learned recovery model has no weights");
   c->mlp = m;
   c->guess_delta = m ? delta : 0.0;
+  return PANGU_OK;
+}
+
+int pangu_set_c2p_gate(pangu_ctx *c, int min_prev_iters) {
+  if (!c) return PANGU_EINVAL;
+  if (min_prev_iters < 0) return fail(c, PANGU_EINVAL, "gate must be >= 0");
+  c->guess_gate = min_prev_iters;
+  return PANGU_OK;
+}
+
+
+int pangu_set_pair_flux(pangu_ctx *c, int dir_mask) {
+  if (!c) return PANGU_EINVAL;
+  if (dir_mask < 0 || dir_mask > 7) return fail(c, PANGU_EINVAL, "pair flux: mask of directions 0..7");
+  c->L.pair_flux = dir_mask;
   return PANGU_OK;
 }
 
@@ -766,6 +783,14 @@ int pangu_register_shared(pangu_ctx *c, const pangu_shared *s) {
   if (!c || !s || !s->alfven) return fail(c, PANGU_EINVAL, "shared fields need alfven");
   HIPCHK(c, hipSetDevice(c->device));
   const size_t nb = (size_t)c->nblocks;
+  // a second registration replaces the tables of the first
+  HIPCHK(c, hipDeviceSynchronize());
+  for (void **p : {(void **)&c->dalfven, (void **)&c->demf, (void **)&c->dflag, (void **)&c->dflag_rec,
+                   (void **)&c->dmask, (void **)&c->dstatus, (void **)&c->diters, (void **)&c->droot}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+  c->shared_ok = false;
   if (upload(c, &c->dalfven, s->alfven, nb * 3)) return PANGU_EDEVICE;
   if (s->emf && upload(c, &c->demf, s->emf, nb * 3)) return PANGU_EDEVICE;
   if (s->flag && upload(c, &c->dflag, s->flag, nb)) return PANGU_EDEVICE;
@@ -773,6 +798,7 @@ int pangu_register_shared(pangu_ctx *c, const pangu_shared *s) {
   if (s->mask && upload(c, &c->dmask, s->mask, nb)) return PANGU_EDEVICE;
   if (s->c2p_status && upload(c, &c->dstatus, s->c2p_status, nb)) return PANGU_EDEVICE;
   if (s->c2p_iters && upload(c, &c->diters, s->c2p_iters, nb)) return PANGU_EDEVICE;
+  if (s->c2p_root && upload(c, &c->droot, s->c2p_root, nb)) return PANGU_EDEVICE;
   if (c->cfg.enable_B && !c->demf) return fail(c, PANGU_EINVAL, "enable_B needs emf tables");
   if (c->cfg.recovery != PANGU_RECOVERY_ROBUSTER && !c->dflag)
     return fail(c, PANGU_EINVAL, "chainer/onedvsq recovery needs the flag field");
@@ -785,6 +811,18 @@ static int check_md(pangu_ctx *c, int md) {
   if (md < 0 || md >= (int)c->conts.size()) return fail(c, PANGU_ESTATE, "unknown container handle");
   if (!c->shared_ok) return fail(c, PANGU_ESTATE, "pangu_register_shared not called");
   return 0;
+}
+
+int pangu_mlp_harvest(pangu_mlp *m, pangu_ctx *c, int md, int stride, int64_t *kept) {
+  if (int r = check_md(c, md)) return r;
+  if (!m || stride < 1) return fail(c, PANGU_EINVAL, "harvest: null model or stride < 1");
+  if (c->device != mlp_device(m)) return fail(c, PANGU_EINVAL, "harvest: model and context on different devices");
+  HIPCHK(c, hipSetDevice(c->device));
+  // the container's fields are written on the caller's streams
+  HIPCHK(c, hipDeviceSynchronize());
+  if (int r = mlp_harvest(m, c->L, c->conts[md].kf(), stride, kept))
+    return fail(c, r, std::string("harvest: ") + pangu_mlp_last_error(m));
+  return PANGU_OK;
 }
 
 #define STREAM(s) ((hipStream_t)(s))
@@ -864,33 +902,31 @@ static SharedTabs shared_tabs(pangu_ctx *c) {
   S.flag_rec = c->dflag_rec;
   S.status = c->dstatus;
   S.iters = c->diters;
-  S.guess = c->mlp ? c->dguess_tab : nullptr;
+  S.root = c->droot;
   S.delta = c->guess_delta;
+  S.gate = c->guess_gate;
   return S;
 }
 
-// Learned Kastaun bracket: guesses for container md's cells before recovery.
-static int launch_guess(pangu_ctx *c, int md, hipStream_t s) {
-  if (!c->mlp || c->cfg.recovery != PANGU_RECOVERY_ROBUSTER) return 0;
-  MlpDev W;
-  if (!mlp_view(c->mlp, W)) return fail(c, PANGU_ESTATE, "learned recovery model has no weights");
-  if (launch_c2p_guess(c->L, c->conts[md].kf(), W, c->dguess_tab, s) != 0)
-    return fail(c, PANGU_EDEVICE, "c2p guess launch failed");
-  return 0;
+// The learned Kastaun bracket's weights when a model is attached (and the
+// recovery is Kastaun), else null.
+static const MlpFrag *learned(pangu_ctx *c, MlpFrag &W) {
+  if (!c->mlp || c->cfg.recovery != PANGU_RECOVERY_ROBUSTER) return nullptr;
+  return mlp_view(c->mlp, W) ? &W : nullptr;
 }
 
 int pangu_recovery(pangu_ctx *c, int md, void *stream) {
   if (int r = check_md(c, md)) return r;
-  if (int r = launch_guess(c, md, STREAM(stream))) return r;
+  MlpFrag W;
   LCHK(c, launch_recover(c->L, c->conts[md].kf(), shared_tabs(c), c->cfg.recovery, false,
-                         STREAM(stream)));
+                         learned(c, W), STREAM(stream)));
   return PANGU_OK;
 }
 
 int pangu_apply_electron_heating(pangu_ctx *c, int md, void *stream) {
   if (int r = check_md(c, md)) return r;
   if (!c->cfg.enable_heating) return PANGU_OK;
-  LCHK(c, launch_recover(c->L, c->conts[md].kf(), shared_tabs(c), -1, true, STREAM(stream)));
+  LCHK(c, launch_recover(c->L, c->conts[md].kf(), shared_tabs(c), -1, true, nullptr, STREAM(stream)));
   return PANGU_OK;
 }
 
@@ -960,8 +996,9 @@ int pangu_stage(pangu_ctx *c, int md_init, int md0, int md1, double beta, double
     LCHK(c, launch_div_update(c->L, 4, f0, fi, f1, nullptr, fuse_ct, dt_state, beta, 0.0, s));
   }
   if (exchange_after == 0) LCHK(c, exchange(c, f1, s));
-  if (int r = launch_guess(c, md1, s)) return r;
-  LCHK(c, launch_recover(c->L, f1, shared_tabs(c), c->cfg.recovery, c->cfg.enable_heating != 0, s));
+  MlpFrag W;
+  LCHK(c, launch_recover(c->L, f1, shared_tabs(c), c->cfg.recovery, c->cfg.enable_heating != 0,
+                         learned(c, W), s));
   if (c->cfg.recovery != PANGU_RECOVERY_ROBUSTER)
     LCHK(c, launch_fix_recovery(c->L, f1, c->dflag, c->dcluster, s));
   if (exchange_after != 0) LCHK(c, exchange(c, f1, s));
@@ -972,7 +1009,7 @@ int pangu_stage(pangu_ctx *c, int md_init, int md0, int md1, double beta, double
 
 }  // extern "C"
 
-int pangu_diagnostics(pangu_ctx *c, int md, const int *shell, double *out, void *stream) {
+static int check_diag_args(pangu_ctx *c, int md, const int *shell, const double *out) {
   if (int r = check_md(c, md)) return r;
   if (!shell || !out) return fail(c, PANGU_EINVAL, "diagnostics: null shell or output array");
   for (int m = 0; m < c->nmesh; ++m) {
@@ -984,20 +1021,16 @@ int pangu_diagnostics(pangu_ctx *c, int md, const int *shell, double *out, void 
       return fail(c, PANGU_EINVAL,
                   "diagnostics: shell integrals need a single-level mesh (pass shell -1 on a refined one)");
   }
+  return 0;
+}
+
+int pangu_diagnostics(pangu_ctx *c, int md, const int *shell, double *out, void *stream) {
+  if (int r = check_diag_args(c, md, shell, out)) return r;
   HIPCHK(c, hipSetDevice(c->device));
   if (!c->ddiag_part) {
-    // tiles of the interior range, grouped by block and therefore by mesh
-    std::vector<int> mtile(c->nmesh + 1, 0);
-    const int zero[3] = {0, 0, 0};
-    int64_t tot = 0;
-    for (int m = 0; m < c->nmesh; ++m) {
-      mtile[m] = (int)tot;
-      for (int b = c->mfirst[m]; b < c->mfirst[m + 1]; ++b) tot += range_tiles(c->hb[b], zero, zero, 0);
-    }
-    mtile[c->nmesh] = (int)tot;
-    if (tot != c->L.interior.ntiles) return fail(c, PANGU_EINVAL, "diagnostics: tile count mismatch");
+    const int64_t tot = c->diag_mtile[c->nmesh];
     HIPCHK(c, hipMalloc((void **)&c->ddiag_part, (size_t)(tot > 0 ? tot : 1) * PANGU_NDIAG * sizeof(double)));
-    if (int r = upload(c, &c->ddiag_mtile, mtile.data(), mtile.size())) return r;
+    if (int r = upload(c, &c->ddiag_mtile, c->diag_mtile.data(), c->diag_mtile.size())) return r;
     HIPCHK(c, hipMalloc((void **)&c->ddiag_shell, (size_t)c->nmesh * sizeof(int)));
   }
   hipStream_t s = STREAM(stream);

@@ -6,6 +6,7 @@
 
 #include "../../include/pangu_mlp.h"
 #include "common.h"
+#include "mlp_dev.h"
 #include "physics_dev.h"
 
 namespace pangu {
@@ -37,17 +38,13 @@ struct SharedTabs {
   double *const *flag_rec;
   int32_t *const *status;
   int32_t *const *iters;
-  const float *const *guess;  // learned Kastaun root guesses per cell, or null
-  double delta;               // relative half-width of the learned bracket
-};
-
-// Device view of a learned recovery-bracket model (pangu_mlp): BF16 weights
-// W1 [32][64] (rows >= 5 zero), W2 and W3 [64][64], W4 [64][32] (columns
-// >= 1 zero), FP32 biases and the feature standardisation.
-struct MlpDev {
-  const uint16_t *w1, *w2, *w3, *w4;
-  const float *b1, *b2, *b3, *b4;
-  const float *mean, *dev;
+  double *const *root;  // Kastaun root of each recovered cell (NaN on failure), or null
+  // learned Kastaun bracket (when the launch carries a model): relative
+  // half-width of the first bracket, and the gate: a cell uses the model
+  // when its previous recovery took at least `gate` false-position steps
+  // (read from iters before it is overwritten; 0 = every cell)
+  double delta;
+  int gate;
 };
 
 // Launch range of a kernel over every block of a context, as 256-thread
@@ -112,12 +109,20 @@ struct LaunchCtx {
   int split_source = 0;            // fused stage: source term in a kernel of its own
   int fuse_ct = 1;                 // fused stage: CT face fluxes formed in the update kernel
   int share_x1 = 0;                // X1 sweep reconstructs each cell once (k_flux_x1s)
+  int pair_flux = 0;               // bit d-1: the sweep along d runs two lanes per face (k_flux_pair)
   MultiLevel ml;                   // refinement tables (ml.on = 0 on a single-level mesh)
   KernelClock *clock = nullptr;
 };
 
-// Device view of a model's current weights; false before any weights exist.
-bool mlp_view(const pangu_mlp *m, MlpDev &out);
+// Device view of a model's current weights in fragment order; false before
+// any weights exist.
+bool mlp_view(const pangu_mlp *m, MlpFrag &out);
+// Appends the training pairs of container F (every stride-th interior cell,
+// solved by the plain Kastaun recovery) to the model's data set; *kept
+// receives the number of solved states added (pangu_mlp_harvest).
+int mlp_harvest(pangu_mlp *m, const LaunchCtx &L, const KFields &F, int stride, int64_t *kept);
+// Device the model lives on.
+int mlp_device(const pangu_mlp *m);
 
 // Attaches (or, with nullptr, detaches) a clock to a context's launches.
 void set_kernel_clock(pangu_ctx *c, KernelClock *clock);
@@ -136,13 +141,21 @@ int launch_ct(const LaunchCtx &L, const KFields &F, double *const *emf, hipStrea
 int launch_div_update(const LaunchCtx &L, int mode, const KFields &F0, const KFields &Finit,
                       const KFields &F1, double *const *dudt, bool ct_fused,
                       const double *dt_dev, double beta, double bdt_host, hipStream_t s);
-// rec: 0 chainer, 1 kastaun, 2 1dvsq, -1 none (heating only)
+// rec: 0 chainer, 1 kastaun, 2 1dvsq, -1 none (heating only).  With a
+// model W and rec 1, the recovery kernel evaluates the network in each
+// wavefront whose cells pass the gate and starts those cells' Kastaun
+// search from the learned bracket.
 int launch_recover(const LaunchCtx &L, const KFields &F, const SharedTabs &S, int rec, bool heat,
-                   hipStream_t s);
-// Learned Kastaun root guess of every interior cell of container F (the
-// inputs of k_recover), into guess[b][cell]; -1 where no guess applies.
-int launch_c2p_guess(const LaunchCtx &L, const KFields &F, const MlpDev &W, float *const *guess,
-                     hipStream_t s);
+                   const MlpFrag *W, hipStream_t s);
+// Training pairs from a container: U in the recovery's normalisation and
+// the cell's metric sample (35 doubles) of every stride-th interior cell
+// (stride >= 1), solved by the plain Kastaun recovery.  Writes slot
+// t = (tile * 256 + thread) / stride for t < cap: U [t][9], mtab [t][35],
+// feats [t][5], target [t] (logit of the root), iters [t], status [t] (0
+// solved, < 0 no cell / excised / unsolved).  Returns the slots written.
+int64_t launch_harvest(const LaunchCtx &L, const KFields &F, int stride, double gamma, int64_t cap,
+                       double *U, double *mtab, float *feats, float *target, int *iters, int *status,
+                       hipStream_t s);
 int launch_fix_recovery(const LaunchCtx &L, const KFields &F, double *const *flag, int *cluster,
                         hipStream_t s);
 int launch_exchange(const LaunchCtx &L, const KFields &F, double *const *flag, const int *nbr,
@@ -158,5 +171,6 @@ int launch_diagnostics(const LaunchCtx &L, const KFields &F, const int *shell, d
                        const int *mtile, int nmesh, double *out, hipStream_t s);
 int launch_dt(const LaunchCtx &L, double *const *alfven, double *red, double *state,
               double dt_factor, double *est_out, hipStream_t s);
+
 
 }  // namespace pangu

@@ -287,6 +287,149 @@ __global__ __launch_bounds__(256) void k_flux(KCtx C, KFields F, double *const *
   face_flux<DIR, RS, HB, HK>(C, F, alfven, b, k, j, i, pl, pr);
 }
 
+// Exchange with the other lane of an aligned lane pair (l ^ 1) through the
+// data-parallel-primitive path (quad_perm [1, 0, 3, 2]), 32 bits at a time.
+__device__ __forceinline__ double pair_swap(double x) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, x);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)u, 0xB1, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), 0xB1, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// Flux sweep with two lanes per face.  The even lane of a pair builds the
+// left state (cell i - 1 along DIR), the odd lane the right state (cell i):
+// each reconstructs one side (the stencil shared through the pair: each lane
+// loads every other stencil cell and takes the rest from its partner), forms
+// its fluid state, wave speeds and flux vectors, then the pair exchanges the
+// speeds and the flux components the other lane combines; the even lane
+// writes the first half of the components and the wave speed, the odd lane
+// the rest.  Every value is computed by the same expressions in the same
+// order as k_flux, so the results are identical; what changes is that one
+// lane holds one fluid state instead of two (fewer registers, more
+// wavefronts in flight).
+template <int DIR, int LIM, int RS, bool HB, bool HK>
+__global__ __launch_bounds__(256) void k_flux_pair(KCtx C, KFields F, double *const *alfven, Range3 R) {
+  int b = 0, k = 0, j = 0, i = 0;
+  if (!decode_tile_pair(C, R, b, k, j, i)) return;  // both lanes of a pair share the outcome
+  const int side = (int)threadIdx.x & 1;            // 0: left state, 1: right state
+  const DevBlock &B = C.blk[b];
+  const int64_t sj = B.n[0], sk = (int64_t)B.n[0] * B.n[1];
+  const int64_t st = (DIR == 1) ? 1 : (DIR == 2 ? sj : sk);
+  const int64_t id = k * sk + j * sj + i;
+  constexpr int NC = ncomp<HB, HK>();
+  constexpr int NS = (LIM == PANGU_LIMITER_PPM4) ? 6 : 4;  // stencil offsets -NS/2 .. NS/2 - 1
+  constexpr int NL = NS / 2;                               // cells each lane loads
+  double p[NPRIM];
+#pragma unroll
+  for (int q = 0; q < NPRIM; ++q) p[q] = 0.0;
+  {
+    // lane `side` loads offsets -NS/2 + side + 2m; its stencil for one side
+    // is offsets side - NS/2 + n, n = 0 .. NS - 2
+    double a[NC][NL];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+      const auto arr = glb(F.prim[b * NC + q]);
+#pragma unroll
+      for (int m = 0; m < NL; ++m) a[q][m] = arr[id + (2 * m + side - NS / 2) * st];
+    }
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+      double x[NL], w[NS - 1];
+#pragma unroll
+      for (int m = 0; m < NL; ++m) x[m] = pair_swap(a[q][m]);
+#pragma unroll
+      for (int n = 0; n < NS - 1; ++n) {
+        if ((n & 1) == 0)
+          w[n] = a[q][n >> 1];
+        else
+          w[n] = side ? x[((n - 1) >> 1) + 1] : x[(n - 1) >> 1];
+      }
+      const int s = slot_of<HB, HK>(q);
+      double t;
+      if (LIM == PANGU_LIMITER_PPM4) {
+        if (side == 0)
+          recon_ppm4(w[0], w[1], w[2], w[3], w[4], p[s], t);
+        else
+          recon_ppm4(w[0], w[1], w[2], w[3], w[4], t, p[s]);
+      } else {
+        if (side == 0)
+          recon_mc(w[0], w[1], w[2], p[s], t);
+        else
+          recon_mc(w[0], w[1], w[2], t, p[s]);
+      }
+    }
+  }
+  const DevGeo &G = C.geo[B.geo];
+  double g[16], gc0[4], gdd, sg, alpha, valid;
+  if (DIR == 3 && !G.face[2]) {
+    const int64_t t = cen_index(G, k, j, i);
+    const auto c = glb(G.cen);
+    const int64_t ns = G.ncen;
+#pragma unroll
+    for (int n = 0; n < 16; ++n) g[n] = c[(CG_GCOV + n) * ns + t];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) gc0[n] = c[(CG_GCON + n) * ns + t];
+    gdd = c[(CG_GCON + 15) * ns + t];
+    sg = c[CG_SG * ns + t];
+    alpha = c[CG_ALPHA * ns + t];
+    valid = 1.0;
+  } else {
+    const int d = DIR - 1;
+    const int64_t t = ((int64_t)(G.has3d ? k : 0) * G.fj[d] + j) * G.fi[d] + i;
+    const auto c = glb(G.face[d]);
+    const int64_t ns = G.nface[d];
+#pragma unroll
+    for (int n = 0; n < 16; ++n) g[n] = c[(FG_GCOV + n) * ns + t];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) gc0[n] = c[(FG_GCON0 + n) * ns + t];
+    gdd = c[FG_GCONDD * ns + t];
+    sg = c[FG_SG * ns + t];
+    alpha = c[FG_ALPHA * ns + t];
+    valid = c[FG_VALID * ns + t];
+  }
+  constexpr int NH = (NC + 1) / 2;  // components written by the even lane
+  const auto al = glb(alfven[b * 3 + (DIR - 1)]);
+  if (valid == 0.0) {  // the same for both lanes of the pair
+    // singular face metric (polar axis): zero flux and wave speed
+    if (side == 0) al[id] = 0.0;
+#pragma unroll
+    for (int q = 0; q < NC; ++q)
+      if ((q < NH) == (side == 0)) glb(F.flux[DIR - 1][b * NC + q])[id] = 0.0;
+    return;
+  }
+  const double gam = C.ph.gamma;
+  St sm;
+  calc_state(p, g, gc0, alpha, sm);
+  double vmax, vmin;
+  fast_ms<DIR>(gam, p, sm, gc0[0], gc0[DIR], gdd, vmax, vmin);
+  const double vmaxO = pair_swap(vmax), vminO = pair_swap(vmin);
+  const double vmaxL = side ? vmaxO : vmax, vminL = side ? vminO : vmin;
+  const double vmaxR = side ? vmax : vmaxO, vminR = side ? vmin : vminO;
+  const double cmax = fabs(kmax(kmax(0., vmaxL), vmaxR));
+  const double cmin = fabs(kmax(kmax(0., -vminL), -vminR));
+  const double cc = kmax(cmax, cmin);
+  if (side == 0) al[id] = cc;
+  double Um[NPRIM], Fm[NPRIM];
+  contravariant_flux<0>(gam, p, sm, sg, Um);
+  contravariant_flux<DIR>(gam, p, sm, sg, Fm);
+  const double den = cmax + cmin + 1e-20, rden = 1.0 / den;
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
+    const int s = slot_of<HB, HK>(q);
+    const double Uo = pair_swap(Um[s]), Fo = pair_swap(Fm[s]);
+    if ((q < NH) == (side == 0)) {
+      const double UL = side ? Uo : Um[s], UR = side ? Um[s] : Uo;
+      const double FL = side ? Fo : Fm[s], FR = side ? Fm[s] : Fo;
+      double f;
+      if (RS == PANGU_RIEMANN_LAXF)
+        f = 0.5 * (FL + FR - cc * (UR - UL));
+      else
+        f = div_by(cmax * FL + cmin * FR - cmax * cmin * (UR - UL), den, rden);
+      glb(F.flux[DIR - 1][b * NC + q])[id] = f;
+    }
+  }
+}
+
 // X1 sweep with one reconstruction per cell.  Lane l of a wavefront
 // reconstructs cell c (both interface values, the same PPM4 / MC call the
 // face kernel makes) and takes the right interface value of cell c - 1 from
@@ -592,10 +735,35 @@ __global__ __launch_bounds__(256, (HB && MODE != 1) ? 5 : 4) void k_div_update(K
 }
 
 // -------------------------------------------------- recovery (+ heating)
+// ApplyElectronHeating (electron_heating.cc:66-109) on one recovered cell.
+template <bool HB, bool HK>
+__device__ __forceinline__ void heat_cell(const KCtx &C, const KFields &F, const DevGeo &G, int64_t t,
+                                          int b, int64_t id, const gptr<double> *pp) {
+  constexpr int NC = ncomp<HB, HK>();
+  double p[NPRIM];
+#pragma unroll
+  for (int s = 0; s < NPRIM; ++s) p[s] = 0.0;
+#pragma unroll
+  for (int q = 0; q < NC; ++q) p[slot_of<HB, HK>(q)] = pp[q][id];
+  CenLoad m;
+  load_cen_state(G, t, m);
+  St s;
+  calc_state(p, m.g, m.gc0, m.alpha, s);
+  const double crho = glb(F.cons[b * NC])[id];
+  HeatIn c;
+  c.rho = p[RHO];
+  c.energy = p[ENY];
+  c.bsq = s.bsq;
+  c.adv_total = glb(F.cons[b * NC + ient<HB>()])[id] / crho;
+  c.rec_total = p[ENT];
+  c.adv_e = glb(F.cons[b * NC + ikel<HB>()])[id] / crho;
+  pp[ikel<HB>()][id] = heat_apply(C.ph, c);
+}
+
 // Recovery (invertor.cc:184-319) and, when HEAT, ApplyElectronHeating
 // (electron_heating.cc:66-109) on the recovered cell.  REC: 0 chainer,
 // 1 Kastaun, 2 1Dvsq alone, -1 none (heating only).
-template <int REC, bool HB, bool HK, bool HEAT, bool GUESS>
+template <int REC, bool HB, bool HK, bool HEAT>
 __global__ __launch_bounds__(256) void k_recover(KCtx C, KFields F, SharedTabs S, Range3 R) {
   int b, k, j, i;
   if (!decode_tile(C, R, b, k, j, i)) return;
@@ -665,12 +833,9 @@ __global__ __launch_bounds__(256) void k_recover(KCtx C, KFields F, SharedTabs S
       int st = 0, it = 0, itn = 0;
       bool write = true;
       if (REC == 1) {
-        if (GUESS) {
-          int ev = 0;
-          st = c2p_kastaun_guess(U, Ph, P.gamma, m, (double)glb(S.guess[b])[id], S.delta, it, ev);
-        } else {
-          st = c2p_kastaun(U, Ph, P.gamma, m, it);
-        }
+        double mu = 0.0;
+        st = c2p_kastaun(U, Ph, P.gamma, m, it, &mu);
+        if (S.root) glb(S.root[b])[id] = (st == 0) ? mu : __builtin_nan("");
         itn = it;
         write = (st == 0);
       } else if (REC == 2) {
@@ -704,26 +869,124 @@ __global__ __launch_bounds__(256) void k_recover(KCtx C, KFields F, SharedTabs S
       }
     }
   }
-  if (HEAT && HK) {
-    double p[NPRIM];
+  if (HEAT && HK) heat_cell<HB, HK>(C, F, G, t, b, id, pp);
+}
+
+// Kastaun recovery with the learned starting bracket (and, when HEAT, the
+// electron heating): k_recover<1, ...> whose cells that pass the gate (their
+// previous recovery took at least S.gate false-position steps) start the
+// search from the network's prediction (kastaun_guess_k).  The network runs
+// on the matrix cores inside the wavefront (wave_mlp), only in wavefronts
+// that hold such a cell.  Status, iteration and field outputs are those of
+// k_recover; the recovered state agrees with it to the solver tolerance
+// (another false-position path to a root of the same function), not bit for
+// bit.
+template <bool HB, bool HK>
+__device__ __forceinline__ int learned_setup(const KCtx &C, const KFields &F, const DevGeo &G, int64_t t,
+                                             int b, int64_t id, KastaunIn &kin) {
+  constexpr int NC = ncomp<HB, HK>();
+  const int64_t ns = G.ncen;
+  CenMetric m;
 #pragma unroll
-    for (int s = 0; s < NPRIM; ++s) p[s] = 0.0;
-#pragma unroll
-    for (int q = 0; q < NC; ++q) p[slot_of<HB, HK>(q)] = pp[q][id];
-    CenLoad m;
-    load_cen_state(G, t, m);
-    St s;
-    calc_state(p, m.g, m.gc0, m.alpha, s);
-    const double crho = glb(F.cons[b * NC])[id];
-    HeatIn c;
-    c.rho = p[RHO];
-    c.energy = p[ENY];
-    c.bsq = s.bsq;
-    c.adv_total = glb(F.cons[b * NC + ient<HB>()])[id] / crho;
-    c.rec_total = p[ENT];
-    c.adv_e = glb(F.cons[b * NC + ikel<HB>()])[id] / crho;
-    pp[ikel<HB>()][id] = heat_apply(P, c);
+  for (int n = 0; n < 16; ++n) {
+    m.gcov[n] = glb(G.cen)[(CG_GCOV + n) * ns + t];
+    m.gcon[n] = glb(G.cen)[(CG_GCON + n) * ns + t];
   }
+  m.lapse = glb(G.cen)[CG_LAPSE * ns + t];
+  const double aos = glb(G.cen)[CG_ALPHA * ns + t] * (1.0 / glb(G.cen)[CG_SG * ns + t]);
+  const double *const *cons = F.cons + (int64_t)b * NC;
+  const double c_rho = glb(cons[0])[id], c_eny = glb(cons[1])[id];
+  double U[9];
+  U[RHO] = aos * c_rho;
+  U[ENY] = aos * (c_eny - c_rho);
+  U[UX1] = aos * glb(cons[2])[id];
+  U[UX2] = aos * glb(cons[3])[id];
+  U[UX3] = aos * glb(cons[4])[id];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) U[BX1 + a] = HB ? aos * glb(cons[5 + a])[id] : 0.0;
+  U[ENT] = aos * glb(cons[ient<HB>()])[id];
+  return kastaun_setup(U, m, kin);
+}
+
+template <bool HB, bool HK, bool HEAT>
+__global__ __launch_bounds__(256) void k_recover_learned(KCtx C, KFields F, SharedTabs S, MlpFrag W,
+                                                         Range3 R) {
+  int b = 0, k = 0, j = 0, i = 0;
+  const bool cell = decode_tile(C, R, b, k, j, i);  // false on the whole workgroup past the range
+  const int lane = threadIdx.x & 63;
+  constexpr int NC = ncomp<HB, HK>();
+  const Phys &P = C.ph;
+  const DevBlock &B = C.blk[cell ? b : 0];
+  const DevGeo &G = C.geo[B.geo];
+  const int64_t id = cell ? ((int64_t)k * B.n[1] + j) * B.n[0] + i : 0;
+  const int64_t t = cell ? cen_index(G, k, j, i) : 0;
+  const bool active =
+      cell && !(P.metric == PANGU_METRIC_CKS && glb(G.cen)[CG_RAD * G.ncen + t] < P.r_excise);
+  KastaunIn kin;
+  int setup = -1, prev = 0;
+  if (active) {
+    setup = learned_setup<HB, HK>(C, F, G, t, b, id, kin);
+    if (S.iters) prev = glb(S.iters[b])[id];
+  }
+  const bool want = setup == 0 && S.iters && prev >= S.gate;
+  double guess = -1.0;
+  __shared__ MlpLds lds;
+  // every thread of the workgroup reaches the vote (no early return above)
+  if (__syncthreads_or(want)) {
+    mlp_stage(W, lds, threadIdx.x, blockDim.x);
+    __syncthreads();
+    if (__ballot(want)) {  // uniform over the wavefront: all 64 lanes run the network
+      float x[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+      if (want) {
+        float f[5];
+        c2p_features(kin, f);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) x[q] = bf2f(f2bf((f[q] - W.mean[q]) / W.dev[q]));
+      }
+      const float z = wave_mlp(lds, W.b4[0], x, lane);
+      if (want) guess = (double)(1.0f / (1.0f + expf(-z)));
+      // the setup is recomputed from memory (same inputs, same bits)
+      // instead of being held in registers across the network
+      __asm__ volatile("" ::: "memory");
+      if (active) setup = learned_setup<HB, HK>(C, F, G, t, b, id, kin);
+    }
+  }
+  if (!cell) return;
+  gptr<double> pp[NC];
+#pragma unroll
+  for (int q = 0; q < NC; ++q) pp[q] = glb(F.prim[b * NC + q]);
+  if (!active) {
+    if (S.status) glb(S.status[b])[id] = -1;
+    if (S.iters) glb(S.iters[b])[id] = 0;
+  } else {
+    if (HB) {
+      const double isg = 1.0 / glb(G.cen)[CG_SG * G.ncen + t];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) pp[5 + a][id] = glb(F.cons[b * NC + 5 + a])[id] * isg;
+    }
+    double Ph[9];
+    double mu = 0.0;
+    int st, it = 0;
+    if (setup != 0) {
+      st = setup;
+    } else if (guess > 0.0) {
+      int ev = 0;
+      st = kastaun_guess_k(kin, P.gamma, Ph, guess, S.delta, it, ev, &mu);
+    } else {
+      st = kastaun_plain_k(kin, P.gamma, Ph, it, &mu);
+    }
+    if (S.root) glb(S.root[b])[id] = (st == 0) ? mu : __builtin_nan("");
+    if (S.status) glb(S.status[b])[id] = st;
+    if (S.iters) glb(S.iters[b])[id] = it;
+    if (st == 0) {
+      pp[0][id] = Ph[RHO];
+      pp[1][id] = Ph[ENY];
+      pp[2][id] = Ph[UX1];
+      pp[3][id] = Ph[UX2];
+      pp[4][id] = Ph[UX3];
+    }
+  }
+  if (HEAT && HK) heat_cell<HB, HK>(C, F, G, t, b, id, pp);
 }
 
 // ---------------------------------------------------------- FixRecovery
@@ -1098,11 +1361,15 @@ static void flux_dir(const LaunchCtx &L, const KFields &F, double *const *alfven
   R.order = L.order[PANGU_KF_FLUX_X1 + DIR - 1];
   if (R.ntiles == 0) return;
   const dim3 g = tile_grid(R);
+  const bool pair = !shared && ((L.pair_flux >> (DIR - 1)) & 1);
+  const dim3 g2((unsigned)(((2 * R.ntiles + 7) / 8) * 8));
   const bool hb = L.k.iBX >= 0, hk = L.k.iKEL >= 0;
   const int lim = L.limiter, rs = L.riemann;
 #define FL(LIM, RS, HB_, HK_)                                                                      \
   if (shared)                                                                                     \
     hipLaunchKernelGGL((k_flux_x1s<LIM, RS, HB_, HK_>), g, kTile, 0, s, L.k, F, alfven, R);       \
+  else if (pair)                                                                                  \
+    hipLaunchKernelGGL((k_flux_pair<DIR, LIM, RS, HB_, HK_>), g2, kTile, 0, s, L.k, F, alfven, R); \
   else                                                                                            \
     hipLaunchKernelGGL((k_flux<DIR, LIM, RS, HB_, HK_>), g, kTile, 0, s, L.k, F, alfven, R)
 #define FL2(LIM, RS)                    \
@@ -1194,17 +1461,17 @@ int launch_div_update(const LaunchCtx &L, int mode, const KFields &F0, const KFi
 }
 
 int launch_recover(const LaunchCtx &L, const KFields &F, const SharedTabs &S, int rec, bool heat,
-                   hipStream_t s) {
+                   const MlpFrag *W, hipStream_t s) {
   Range3 R = L.interior;
   R.order = L.order[PANGU_KF_RECOVERY];
   if (R.ntiles == 0) return 0;
   const dim3 g = tile_grid(R);
   const bool hb = L.k.iBX >= 0, hk = L.k.iKEL >= 0;
-#define RC(R_, HB_, HK_, HE_)                                                                 \
-  if (R_ == 1 && S.guess)                                                                      \
-    hipLaunchKernelGGL((k_recover<R_, HB_, HK_, HE_, (R_ == 1)>), g, kTile, 0, s, L.k, F, S, R); \
-  else                                                                                         \
-    hipLaunchKernelGGL((k_recover<R_, HB_, HK_, HE_, false>), g, kTile, 0, s, L.k, F, S, R)
+#define RC(R_, HB_, HK_, HE_)                                                                        \
+  if (R_ == 1 && W)                                                                                   \
+    hipLaunchKernelGGL((k_recover_learned<HB_, HK_, HE_>), g, kTile, 0, s, L.k, F, S, *W, R);         \
+  else                                                                                                \
+    hipLaunchKernelGGL((k_recover<R_, HB_, HK_, HE_>), g, kTile, 0, s, L.k, F, S, R)
 #define RC2(R_, HE_)                     \
   if (hb && hk) RC(R_, true, true, HE_);  \
   else if (hb) RC(R_, true, false, HE_);  \

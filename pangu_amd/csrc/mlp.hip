@@ -22,37 +22,21 @@
 #include "common.h"
 #include "host.h"
 #include "kernels.h"
+#include "mlp_dev.h"
 #include "physics_dev.h"
 #include "recovery_dev.h"
 #include "tiles_dev.h"
 
 namespace pangu {
 
-constexpr int NH = 64;    // hidden width
-constexpr int KIN = 32;   // padded input width (5 features)
-constexpr int NOUT = 32;  // padded output width (1 output)
+constexpr int NH = MLP_NH;      // hidden width
+constexpr int KIN = MLP_KIN;    // padded input width (5 features)
+constexpr int NOUT = MLP_NOUT;  // padded output width (1 output)
 constexpr int NMET = 35;  // metric sample: gcov, gcon, sqrt|g|, alpha, lapse
+// Offsets of the biases in the FP32 parameter arena (W1 [KIN][NH], b1, W2,
+// b2, W3, b3, W4 [NH][NOUT], b4 [NOUT]).
+constexpr int64_t OFF_B1_ = KIN * NH, OFF_B2_ = OFF_B1_ + NH + NH * NH, OFF_B3_ = OFF_B2_ + NH + NH * NH;
 
-typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
-typedef __attribute__((__vector_size__(8 * sizeof(unsigned short)))) unsigned short u16x8;
-typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
-
-__host__ __device__ inline uint16_t f2bf(float f) {
-  uint32_t u = __builtin_bit_cast(uint32_t, f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-__host__ __device__ inline float bf2f(uint16_t h) {
-  return __builtin_bit_cast(float, (uint32_t)h << 16);
-}
-
-__device__ inline f32x16 mfma32(u16x8 a, u16x8 b, f32x16 c) {
-  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
-                                                 __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
-}
-
-__device__ inline float silu(float z) { return z / (1.0f + expf(-z)); }
 __device__ inline float silu_grad(float z) {
   const float s = 1.0f / (1.0f + expf(-z));
   return s * (1.0f + z * (1.0f - s));
@@ -96,17 +80,6 @@ static int gemm(int M, int N, int K, const uint16_t *A, int64_t sam, int64_t sak
 }
 
 // -------------------------------------------------------- element kernels
-__device__ inline void c2p_features(const KastaunIn &k, float f[5]) {
-  const double q = k.q < -0.5 ? -0.5 : k.q;
-  f[0] = (float)log1p(q);
-  f[1] = (float)log1p(k.rsq);
-  f[2] = (float)log1p(k.bsq);
-  f[3] = (float)log1p(k.rbsq);
-  double c = k.rbsq / (k.rsq * k.bsq + 1e-300);
-  c = c < 0.0 ? 0.0 : (c > 1.0 ? 1.0 : c);
-  f[4] = (float)c;
-}
-
 // Minibatch input: standardised features of rows idx[i] as BF16 [n][KIN].
 __global__ void k_gather(int n, const int *idx, const float *feats, const float *target,
                          const float *mean, const float *dev, uint16_t *X, float *T) {
@@ -325,111 +298,129 @@ __global__ void k_c2p_eval(int n, const int *idx, const double *Uin, const int *
   rdiff[i] = d;
 }
 
-// --------------------------------------------------- fused inference kernel
-// One wavefront evaluates the network for its 64 cells: layer 1 (5 inputs)
-// and layer 4 (1 output) on the vector ALU, layers 2 and 3 as MFMA tiles
-// through LDS (rows padded to 72 BF16 to spread banks).
-constexpr int LROW = 72;
-
-__device__ inline void mfma_layer(const uint16_t *in, uint16_t *out, const uint16_t *W,
-                                  const float *bias, int lane) {
-  const int r = lane & 31, half = lane >> 5;
-#pragma unroll 1
-  for (int g = 0; g < 2; ++g)
-#pragma unroll 1
-    for (int nt = 0; nt < 2; ++nt) {
-      f32x16 acc;
-#pragma unroll
-      for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
-#pragma unroll
-      for (int k0 = 0; k0 < NH; k0 += 16) {
-        u16x8 a, b;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          const int kk = k0 + half * 8 + q;
-          a[q] = in[(g * 32 + r) * LROW + kk];
-          b[q] = W[kk * NH + nt * 32 + r];
-        }
-        acc = mfma32(a, b, acc);
-      }
-#pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        const int i = (v >> 2) * 8 + half * 4 + (v & 3);
-        const int j = nt * 32 + r;
-        out[(g * 32 + i) * LROW + j] = f2bf(silu(acc[v] + bias[j]));
-      }
-    }
+// ------------------------------------------------- in-wave inference
+// Fragment order of the weights for wave_mlp (mlp_dev.h): one thread per
+// BF16 entry of a1 [g][ot][lane][8], a2 and a3 [ot][s][lane][8].
+__global__ void k_pack_frag(const uint16_t *W1, const uint16_t *W2, const uint16_t *W3, const uint16_t *W4,
+                            const float *P, uint16_t *a1, uint16_t *a2, uint16_t *a3, float *fb) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < 4 * 64) {
+    // fb: bias[layer][h][16 ot + v] for layers 0..2, then w4[h][16 ot + v]
+    const int slot = e & 31, h = (e >> 5) & 1, layer = e >> 6;
+    const int f = frag_feature(slot >> 4, slot & 15, h);
+    const int64_t off_b[3] = {OFF_B1_, OFF_B2_, OFF_B3_};
+    fb[e] = layer < 3 ? P[off_b[layer] + f] : bf2f(W4[f * NOUT]);
+  }
+  if (e < 2 * 2 * 64 * 8) {
+    const int q = e & 7, lane = (e >> 3) & 63, ot = (e >> 9) & 1, g = e >> 10;
+    const int h = lane >> 5, o = 32 * ot + (lane & 31);
+    a1[e] = (h == g && q < PANGU_MLP_IN) ? W1[q * NH + o] : (uint16_t)0;
+  }
+  if (e < 2 * 4 * 64 * 8) {
+    const int q = e & 7, lane = (e >> 3) & 63, st = (e >> 9) & 3, ot = e >> 11;
+    const int h = lane >> 5, o = 32 * ot + (lane & 31);
+    const int phi = frag_feature(st >> 1, 8 * (st & 1) + q, h);
+    a2[e] = W2[phi * NH + o];
+    a3[e] = W3[phi * NH + o];
+  }
 }
 
-__global__ __launch_bounds__(256, 2) void k_c2p_guess(KCtx C, KFields F, MlpDev Wt, float *const *guess,
-                                                   Range3 R) {
-  __shared__ uint16_t lds[4][2][64 * LROW];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  uint16_t *buf0 = lds[wave][0], *buf1 = lds[wave][1];
-  int b = 0, k = 0, j = 0, i = 0;
-  const bool cell = decode_tile(C, R, b, k, j, i);
-  float x[5] = {0, 0, 0, 0, 0};
-  bool valid = false;
-  int64_t id = 0;
-  if (cell) {
+// wave_mlp on host-given raw features (the in-kernel path, for checking it
+// against the training forward pass): out[i] = logistic(z).
+__global__ __launch_bounds__(64) void k_wave_predict(int n, const float *feats, MlpFrag W, float *out) {
+  __shared__ MlpLds lds;
+  const int lane = threadIdx.x;
+  const int i = blockIdx.x * 64 + lane;
+  mlp_stage(W, lds, lane, 64);
+  __syncthreads();
+  float x[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  if (i < n)
+    for (int q = 0; q < 5; ++q) x[q] = bf2f(f2bf((feats[(int64_t)i * 5 + q] - W.mean[q]) / W.dev[q]));
+  const float z = wave_mlp(lds, W.b4[0], x, lane);
+  if (i < n) out[i] = 1.0f / (1.0f + expf(-z));
+}
+
+// Training pairs from a container (launch_harvest, kernels.h).
+__global__ __launch_bounds__(256) void k_harvest(KCtx C, KFields F, Range3 R, int stride, double gamma,
+                                                 int64_t cap, double *Uo, double *mto, float *feats,
+                                                 float *target, int *iters, int *status) {
+  const int N = R.ntiles;
+  const int n8 = (N + 7) >> 3;
+  const int L = (int)blockIdx.x;
+  const int T = (L & 7) * n8 + (L >> 3);
+  if (T >= N) return;
+  const int64_t lin = (int64_t)T * 256 + threadIdx.x;
+  if (lin % stride) return;
+  const int64_t slot = lin / stride;
+  if (slot >= cap) return;
+  int b, k, j, i;
+  int st = -9;  // no cell (tail of a partial tile)
+  double U[9];
+  double mt[NMET];
+  for (int q = 0; q < 9; ++q) U[q] = 0.0;
+  for (int q = 0; q < NMET; ++q) mt[q] = 0.0;
+  int it = 0;
+  double mu = 0.0;
+  KastaunIn kin;
+  if (decode_tile(C, R, b, k, j, i)) {
     const DevBlock &B = C.blk[b];
-    id = ((int64_t)k * B.n[1] + j) * B.n[0] + i;
     const DevGeo &G = C.geo[B.geo];
-    const int64_t t = ((int64_t)(G.has3d ? k : 0) * G.nj + j) * G.ni + i;
-    const int64_t ns = G.ncen;
-    CenMetric m;
+    const int64_t id = ((int64_t)k * B.n[1] + j) * B.n[0] + i;
+    const int64_t t = ((int64_t)(G.has3d ? k : 0) * G.nj + j) * G.ni + i, ns = G.ncen;
+    const auto cen = glb(G.cen);
     for (int n = 0; n < 16; ++n) {
-      m.gcov[n] = glb(G.cen)[(CG_GCOV + n) * ns + t];
-      m.gcon[n] = glb(G.cen)[(CG_GCON + n) * ns + t];
+      mt[n] = cen[(CG_GCOV + n) * ns + t];
+      mt[16 + n] = cen[(CG_GCON + n) * ns + t];
     }
-    m.lapse = glb(G.cen)[CG_LAPSE * ns + t];
-    const double sg = glb(G.cen)[CG_SG * ns + t];
-    const double alpha = glb(G.cen)[CG_ALPHA * ns + t];
-    const int NC = C.nc;
-    const double *const *cons = F.cons + (int64_t)b * NC;
-    const double aos = alpha * (1.0 / sg);
-    double U[9];
-    U[RHO] = aos * glb(cons[0])[id];
-    U[ENY] = aos * (glb(cons[1])[id] - glb(cons[0])[id]);
-    U[UX1] = aos * glb(cons[2])[id];
-    U[UX2] = aos * glb(cons[3])[id];
-    U[UX3] = aos * glb(cons[4])[id];
-    for (int a = 0; a < 3; ++a) U[BX1 + a] = C.iBX >= 0 ? aos * glb(cons[C.iBX + a])[id] : 0.0;
-    U[ENT] = aos * glb(cons[C.iENT])[id];
-    KastaunIn kin;
-    if (kastaun_setup(U, m, kin) == 0) {
-      float f[5];
-      c2p_features(kin, f);
-      for (int q = 0; q < 5; ++q) x[q] = bf2f(f2bf((f[q] - Wt.mean[q]) / Wt.dev[q]));
-      valid = true;
+    mt[32] = cen[CG_SG * ns + t];
+    mt[33] = cen[CG_ALPHA * ns + t];
+    mt[34] = cen[CG_LAPSE * ns + t];
+    if (C.ph.metric == PANGU_METRIC_CKS && cen[CG_RAD * ns + t] < C.ph.r_excise) {
+      st = -8;  // excised: no recovery there
+    } else {
+      const int NC = C.nc;
+      const double *const *cons = F.cons + (int64_t)b * NC;
+      const double aos = mt[33] * (1.0 / mt[32]);
+      U[RHO] = aos * glb(cons[0])[id];
+      U[ENY] = aos * (glb(cons[1])[id] - glb(cons[0])[id]);
+      U[UX1] = aos * glb(cons[2])[id];
+      U[UX2] = aos * glb(cons[3])[id];
+      U[UX3] = aos * glb(cons[4])[id];
+      for (int a = 0; a < 3; ++a) U[BX1 + a] = C.iBX >= 0 ? aos * glb(cons[C.iBX + a])[id] : 0.0;
+      U[ENT] = aos * glb(cons[C.iENT])[id];
+      CenMetric m;
+      for (int n = 0; n < 16; ++n) {
+        m.gcov[n] = mt[n];
+        m.gcon[n] = mt[16 + n];
+      }
+      m.lapse = mt[34];
+      double Ph[9];
+      for (int q = 0; q < 9; ++q) Ph[q] = 0.0;
+      st = c2p_kastaun(U, Ph, gamma, m, it, &mu);
+      if (st == 0 && kastaun_setup(U, m, kin) != 0) st = -2;
+      if (st == 0 && !(mu > 0.0 && mu < 1.0)) st = -3;
     }
   }
-  // layer 1 (vector ALU): BF16 inputs and weights, FP32 accumulation
-  for (int jn = 0; jn < NH; ++jn) {
-    float z = 0.0f;
-#pragma unroll
-    for (int q = 0; q < 5; ++q) z += x[q] * bf2f(Wt.w1[q * NH + jn]);
-    buf0[lane * LROW + jn] = f2bf(silu(z + Wt.b1[jn]));
-  }
-  __syncthreads();
-  mfma_layer(buf0, buf1, Wt.w2, Wt.b2, lane);
-  __syncthreads();
-  mfma_layer(buf1, buf0, Wt.w3, Wt.b3, lane);
-  __syncthreads();
-  float z = 0.0f;
-  for (int jn = 0; jn < NH; ++jn) z += bf2f(buf0[lane * LROW + jn]) * bf2f(Wt.w4[jn * NOUT]);
-  z += Wt.b4[0];
-  if (cell) glb(guess[b])[id] = valid ? 1.0f / (1.0f + expf(-z)) : -1.0f;
+  for (int q = 0; q < 9; ++q) Uo[slot * 9 + q] = U[q];
+  for (int q = 0; q < NMET; ++q) mto[slot * NMET + q] = mt[q];
+  float f[5] = {0, 0, 0, 0, 0};
+  if (st == 0) c2p_features(kin, f);
+  for (int q = 0; q < 5; ++q) feats[slot * 5 + q] = f[q];
+  target[slot] = st == 0 ? (float)log(mu / (1.0 - mu)) : 0.0f;
+  iters[slot] = it;
+  status[slot] = st;
 }
 
-int launch_c2p_guess(const LaunchCtx &L, const KFields &F, const MlpDev &W, float *const *guess,
-                     hipStream_t s) {
+int64_t launch_harvest(const LaunchCtx &L, const KFields &F, int stride, double gamma, int64_t cap,
+                       double *U, double *mtab, float *feats, float *target, int *iters, int *status,
+                       hipStream_t s) {
   const Range3 &R = L.interior;
-  if (R.ntiles == 0) return 0;
-  hipLaunchKernelGGL(k_c2p_guess, dim3((unsigned)(((R.ntiles + 7) / 8) * 8)), dim3(256), 0, s, L.k, F, W,
-                     guess, R);
-  if (L.clock) (void)L.clock->mark("c2p_guess", s);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  if (R.ntiles == 0 || stride < 1) return 0;
+  const int64_t slots = ((int64_t)R.ntiles * 256 + stride - 1) / stride;
+  hipLaunchKernelGGL(k_harvest, dim3((unsigned)(((R.ntiles + 7) / 8) * 8)), dim3(256), 0, s, L.k, F, R,
+                     stride, gamma, cap, U, mtab, feats, target, iters, status);
+  if (hipGetLastError() != hipSuccess) return -1;
+  return slots < cap ? slots : cap;
 }
 
 }  // namespace pangu
@@ -446,11 +437,13 @@ struct pangu_mlp {
   // laid out W1 [KIN][NH], b1, W2, b2, W3, b3, W4 [NH][NOUT], b4 [NOUT]
   float *P = nullptr, *G = nullptr, *M = nullptr, *V = nullptr;
   uint16_t *W1h = nullptr, *W2h = nullptr, *W3h = nullptr, *W4h = nullptr;
+  uint16_t *Fa1 = nullptr, *Fa2 = nullptr, *Fa3 = nullptr;  // fragment order (wave_mlp)
+  float *Fb = nullptr;                                       // biases and output weights, same
   float *mean = nullptr, *dev = nullptr;  // [5] each
   bool stats_set = false, have_weights = false;
   int adam_t = 0;
   // data set
-  int64_t n = 0;
+  int64_t n = 0, npts = 0;  // states, metric samples
   float *feats = nullptr, *target = nullptr;
   double *U = nullptr, *mtab = nullptr;
   int *midx = nullptr, *iters = nullptr, *status = nullptr;
@@ -469,6 +462,7 @@ namespace {
 constexpr int64_t OFF_W1 = 0, OFF_B1 = OFF_W1 + KIN * NH, OFF_W2 = OFF_B1 + NH,
                   OFF_B2 = OFF_W2 + NH * NH, OFF_W3 = OFF_B2 + NH, OFF_B3 = OFF_W3 + NH * NH,
                   OFF_W4 = OFF_B3 + NH, OFF_B4 = OFF_W4 + NH * NOUT, NPAR = OFF_B4 + NOUT;
+static_assert(OFF_B1 == OFF_B1_ && OFF_B2 == OFF_B2_ && OFF_B3 == OFF_B3_, "bias offsets");
 
 int mfail(pangu_mlp *m, int code, const std::string &msg) {
   if (m) m->err = msg;
@@ -498,7 +492,19 @@ int refresh_bf16(pangu_mlp *m) {
   hipLaunchKernelGGL(k_to_bf16, dim3(blocks_for(NH * NH)), dim3(256), 0, s, (int64_t)NH * NH, m->P + OFF_W2, m->W2h);
   hipLaunchKernelGGL(k_to_bf16, dim3(blocks_for(NH * NH)), dim3(256), 0, s, (int64_t)NH * NH, m->P + OFF_W3, m->W3h);
   hipLaunchKernelGGL(k_to_bf16, dim3(blocks_for(NH * NOUT)), dim3(256), 0, s, (int64_t)NH * NOUT, m->P + OFF_W4, m->W4h);
+  hipLaunchKernelGGL(k_pack_frag, dim3(blocks_for(2 * 4 * 64 * 8)), dim3(256), 0, s, m->W1h, m->W2h, m->W3h,
+                     m->W4h, m->P, m->Fa1, m->Fa2, m->Fa3, m->Fb);
   MCHK(m, hipGetLastError());
+  return 0;
+}
+
+template <class T>
+int grow(pangu_mlp *m, T **p, int64_t old_n, int64_t new_n) {
+  T *q = nullptr;
+  if (dmalloc(m, &q, new_n)) return PANGU_ENOMEM;
+  if (*p && old_n > 0) MCHK(m, hipMemcpy(q, *p, (size_t)old_n * sizeof(T), hipMemcpyDeviceToDevice));
+  if (*p) (void)hipFree(*p);
+  *p = q;
   return 0;
 }
 
@@ -577,19 +583,65 @@ int adam_step(pangu_mlp *m, float lr) {
 }  // namespace
 
 namespace pangu {
-bool mlp_view(const pangu_mlp *m, MlpDev &out) {
+bool mlp_view(const pangu_mlp *m, MlpFrag &out) {
   if (!m || !m->have_weights) return false;
-  out.w1 = m->W1h;
-  out.w2 = m->W2h;
-  out.w3 = m->W3h;
-  out.w4 = m->W4h;
-  out.b1 = m->P + OFF_B1;
-  out.b2 = m->P + OFF_B2;
-  out.b3 = m->P + OFF_B3;
+  out.a1 = (const u16x8 *)m->Fa1;
+  out.a2 = (const u16x8 *)m->Fa2;
+  out.a3 = (const u16x8 *)m->Fa3;
+  out.bias = m->Fb;
+  out.w4 = m->Fb + 3 * 64;
   out.b4 = m->P + OFF_B4;
   out.mean = m->mean;
   out.dev = m->dev;
   return true;
+}
+
+int mlp_device(const pangu_mlp *m) { return m ? m->device : -1; }
+
+// Appends the harvested states of one container to the data set
+// (pangu_mlp_harvest).
+int mlp_harvest(pangu_mlp *m, const LaunchCtx &L, const KFields &F, int stride, int64_t *kept) {
+  if (!m || stride < 1) return PANGU_EINVAL;
+  (void)hipSetDevice(m->device);
+  const double gamma = L.k.ph.gamma;
+  if (m->n > 0 && m->gamma != gamma)
+    return mfail(m, PANGU_EINVAL, "harvest: the data set holds states of another adiabatic index");
+  m->gamma = gamma;
+  const int64_t cap = ((int64_t)L.interior.ntiles * 256 + stride - 1) / stride;
+  if (cap <= 0) {
+    if (kept) *kept = 0;
+    return PANGU_OK;
+  }
+  const int64_t n0 = m->n, p0 = m->npts, n1 = n0 + cap, p1 = p0 + cap;
+  if (n1 > ((int64_t)1 << 30)) return mfail(m, PANGU_EINVAL, "harvest: data set would exceed 2^30 states");
+  if (grow(m, &m->feats, n0 * 5, n1 * 5) || grow(m, &m->target, n0, n1) || grow(m, &m->U, n0 * 9, n1 * 9) ||
+      grow(m, &m->midx, n0, n1) || grow(m, &m->iters, n0, n1) || grow(m, &m->status, n0, n1) ||
+      grow(m, &m->mtab, p0 * NMET, p1 * NMET))
+    return PANGU_ENOMEM;
+  const int64_t w = launch_harvest(L, F, stride, gamma, cap, m->U + n0 * 9, m->mtab + p0 * NMET,
+                                   m->feats + n0 * 5, m->target + n0, m->iters + n0, m->status + n0,
+                                   m->stream);
+  if (w != cap) return mfail(m, PANGU_EDEVICE, "harvest launch failed");
+  std::vector<int> mi(cap);
+  for (int64_t q = 0; q < cap; ++q) mi[q] = (int)(p0 + q);
+  MCHK(m, hipMemcpyAsync(m->midx + n0, mi.data(), cap * sizeof(int), hipMemcpyHostToDevice, m->stream));
+  std::vector<int> st(cap);
+  MCHK(m, hipMemcpyAsync(st.data(), m->status + n0, cap * sizeof(int), hipMemcpyDeviceToHost, m->stream));
+  MCHK(m, hipStreamSynchronize(m->stream));
+  int64_t add = 0;
+  for (int64_t q = 0; q < cap; ++q)
+    if (st[q] == 0) {
+      m->kept.push_back((int)(n0 + q));
+      ++add;
+    }
+  m->n = n1;
+  m->npts = p1;
+  if (m->dkept) (void)hipFree(m->dkept);
+  m->dkept = nullptr;
+  if (dmalloc(m, &m->dkept, (int64_t)m->kept.size())) return PANGU_ENOMEM;
+  MCHK(m, hipMemcpy(m->dkept, m->kept.data(), m->kept.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (kept) *kept = add;
+  return PANGU_OK;
 }
 }  // namespace pangu
 
@@ -605,6 +657,8 @@ int pangu_mlp_create(int device, uint64_t seed, pangu_mlp **out) {
   if (hipStreamCreate(&m->stream) != hipSuccess || dmalloc(m, &m->P, NPAR) || dmalloc(m, &m->G, NPAR) ||
       dmalloc(m, &m->M, NPAR) || dmalloc(m, &m->V, NPAR) || dmalloc(m, &m->W1h, KIN * NH) ||
       dmalloc(m, &m->W2h, NH * NH) || dmalloc(m, &m->W3h, NH * NH) || dmalloc(m, &m->W4h, NH * NOUT) ||
+      dmalloc(m, &m->Fa1, 2 * 2 * 64 * 8) || dmalloc(m, &m->Fa2, 2 * 4 * 64 * 8) ||
+      dmalloc(m, &m->Fa3, 2 * 4 * 64 * 8) || dmalloc(m, &m->Fb, 4 * 64) ||
       dmalloc(m, &m->mean, 5) || dmalloc(m, &m->dev, 5)) {
     pangu_mlp_destroy(m);
     return PANGU_ENOMEM;
@@ -638,7 +692,8 @@ int pangu_mlp_destroy(pangu_mlp *m) {
   if (!m) return PANGU_OK;
   (void)hipSetDevice(m->device);
   if (m->stream) (void)hipStreamSynchronize(m->stream);
-  void *ptrs[] = {m->P,  m->G,    m->M,    m->V,  m->W1h,  m->W2h,   m->W3h, m->W4h, m->mean,
+  void *ptrs[] = {m->Fa1, m->Fa2, m->Fa3, m->Fb,
+                  m->P,  m->G,    m->M,    m->V,  m->W1h,  m->W2h,   m->W3h, m->W4h, m->mean,
                   m->dev, m->feats, m->target, m->U, m->mtab, m->midx, m->iters, m->status, m->dkept,
                   m->X,  m->A1,   m->A2,   m->A3, m->dZ4,  m->dZ,    m->T,   m->Z1,  m->Z2,
                   m->Z3, m->Z4,   m->dA,   m->part, m->bidx};
@@ -754,6 +809,7 @@ int pangu_mlp_make_dataset(pangu_mlp *m, int64_t n, uint64_t seed, double a, dou
   for (int64_t i = 0; i < n; ++i)
     if (st[i] == 0) m->kept.push_back((int)i);
   m->n = n;
+  m->npts = npts;
   if (dmalloc(m, &m->dkept, (int64_t)m->kept.size())) return PANGU_ENOMEM;
   MCHK(m, hipMemcpy(m->dkept, m->kept.data(), m->kept.size() * sizeof(int), hipMemcpyHostToDevice));
   if (kept) *kept = (int64_t)m->kept.size();
@@ -872,6 +928,27 @@ int pangu_mlp_predict(pangu_mlp *m, const float *feats, int n, float *out) {
   MCHK(m, hipMemcpy(z.data(), m->Z4, z.size() * sizeof(float), hipMemcpyDeviceToHost));
   for (int i = 0; i < n; ++i) out[i] = 1.0f / (1.0f + std::exp(-z[(size_t)i * NOUT]));
   return PANGU_OK;
+}
+
+int pangu_mlp_predict_wave(pangu_mlp *m, const float *feats, int n, float *out) {
+  if (!m || !feats || !out || n <= 0) return PANGU_EINVAL;
+  (void)hipSetDevice(m->device);
+  MlpFrag W;
+  if (!mlp_view(m, W)) return mfail(m, PANGU_ESTATE, "no weights");
+  float *df = nullptr, *dout = nullptr;
+  if (dmalloc(m, &df, (int64_t)n * 5) || dmalloc(m, &dout, n)) return PANGU_ENOMEM;
+  int rc = 0;
+  if (hipMemcpy(df, feats, (size_t)n * 5 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+    rc = mfail(m, PANGU_EDEVICE, "copy failed");
+  if (!rc) {
+    hipLaunchKernelGGL(k_wave_predict, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, m->stream, n, df, W, dout);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(m->stream) != hipSuccess ||
+        hipMemcpy(out, dout, (size_t)n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+      rc = mfail(m, PANGU_EDEVICE, "wave prediction failed");
+  }
+  (void)hipFree(df);
+  (void)hipFree(dout);
+  return rc;
 }
 
 int pangu_mlp_evaluate(pangu_mlp *m, double delta, double *out) {

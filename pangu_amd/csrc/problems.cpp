@@ -480,6 +480,12 @@ int generate_problem(const pangu_config &cfg, const ProblemParams &pp,
     rhomax = kmax(rhomax, bm);
     umax = kmax(umax, um);
   }
+  // No cell inside the torus (an unbound torus, r_in inside the marginally
+  // bound orbit for the spin, or one the mesh does not resolve): the
+  // reference's division by the zero maximum fills the mesh with NaN; the
+  // mesh is left as the zero state instead, which the first FixPrimitive
+  // floors to the atmosphere, and no field is made.
+  if (!(rhomax > 0.0)) return 0;
   for (size_t b = 0; b < blocks.size(); ++b)
     for (int64_t id = 0; id < blocks[b].cells; ++id) {
       P(b, 0)[id] /= rhomax;

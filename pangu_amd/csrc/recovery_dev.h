@@ -489,6 +489,30 @@ __device__ __forceinline__ int kastaun_finish(const KastaunIn &k, double mu, dou
   return 0;
 }
 
+// The search and reconstruction of c2p_kastaun from a completed setup.
+__device__ __forceinline__ int kastaun_plain_k(const KastaunIn &k, double gamma, double p[9],
+                                               int &iters, double *mu_out = nullptr) {
+  double mu;
+  const int rs = kastaun_solve(k, gamma, 0.0, 1.0, mu, iters);
+  if (rs != 0) return 100 + rs;
+  if (mu_out) *mu_out = mu;
+  return kastaun_finish(k, mu, p);
+}
+
+// The five dimensionless invariants the learned bracket's network reads:
+// log(1 + max(q, -0.5)), log(1 + r^2), log(1 + b^2), log(1 + (r.b)^2) and
+// (r.b)^2 / (r^2 b^2) clamped to [0, 1].
+__device__ inline void c2p_features(const KastaunIn &k, float f[5]) {
+  const double q = k.q < -0.5 ? -0.5 : k.q;
+  f[0] = (float)log1p(q);
+  f[1] = (float)log1p(k.rsq);
+  f[2] = (float)log1p(k.bsq);
+  f[3] = (float)log1p(k.rbsq);
+  double c = k.rbsq / (k.rsq * k.bsq + 1e-300);
+  c = c < 0.0 ? 0.0 : (c > 1.0 ? 1.0 : c);
+  f[4] = (float)c;
+}
+
 // scheme_kastaun.h:110-259: Illinois false position on [0,1].  Status 0,
 // 1..4 from the setup, 101..104 from the root search, 5..6 from the
 // reconstruction.  mu_out (optional) receives the root.
@@ -512,13 +536,11 @@ __device__ int c2p_kastaun(const double U[9], double p[9], double gamma, const C
 // false-position path differs from c2p_kastaun (agreement to the 1e-10 step
 // tolerance, not bit for bit).  iters counts false-position steps, evals
 // every evaluation of the master function.
-__device__ int c2p_kastaun_guess(const double U[9], double p[9], double gamma, const CenMetric &m,
-                                 double guess, double delta, int &iters, int &evals,
-                                 double *mu_out = nullptr) {
+__device__ __forceinline__ int kastaun_guess_k(const KastaunIn &k, double gamma, double p[9],
+                                               double guess, double delta, int &iters, int &evals,
+                                               double *mu_out = nullptr) {
   iters = 0;
   evals = 0;
-  KastaunIn k;
-  if (int e = kastaun_setup(U, m, k)) return e;
   double mu = 0.0;
   int rs = 2;
   if (guess > 0.0 && guess < 1.0) {
@@ -562,6 +584,16 @@ __device__ int c2p_kastaun_guess(const double U[9], double p[9], double gamma, c
   if (rs != 0) return 100 + rs;
   if (mu_out) *mu_out = mu;
   return kastaun_finish(k, mu, p);
+}
+
+__device__ int c2p_kastaun_guess(const double U[9], double p[9], double gamma, const CenMetric &m,
+                                 double guess, double delta, int &iters, int &evals,
+                                 double *mu_out = nullptr) {
+  iters = 0;
+  evals = 0;
+  KastaunIn k;
+  if (int e = kastaun_setup(U, m, k)) return e;
+  return kastaun_guess_k(k, gamma, p, guess, delta, iters, evals, mu_out);
 }
 
 }  // namespace pangu

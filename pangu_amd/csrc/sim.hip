@@ -18,7 +18,7 @@ using namespace pangu;
 // field ids (shared with the test harness)
 enum {
   F_PRIM = 0, F_CONS = 1, F_FLUX1 = 2, F_FLUX2 = 3, F_FLUX3 = 4, F_ALFVEN = 5, F_EMF = 6,
-  F_FLAG = 7, F_MASK = 8, F_STATUS = 9, F_ITERS = 10, F_FLAG_REC = 11, F_DUDT = 12
+  F_FLAG = 7, F_MASK = 8, F_STATUS = 9, F_ITERS = 10, F_FLAG_REC = 11, F_DUDT = 12, F_ROOT = 13
 };
 // task ids
 enum {
@@ -39,6 +39,7 @@ struct pangu_sim {
   double *alfven = nullptr, *emf = nullptr, *flag = nullptr, *flag_rec = nullptr;
   uint8_t *mask = nullptr;
   int32_t *status = nullptr, *iters = nullptr;
+  double *root = nullptr;    // Kastaun roots, allocated while recording (pangu_sim_record_roots)
   int h_base = -1, h_s1 = -1, h_dudt = -1;
   double *dstate = nullptr;  // {dt, time} per mesh
   double *ddiag = nullptr;   // diagnostics output, PANGU_NDIAG per mesh
@@ -312,6 +313,38 @@ static pangu_config physics_of(const pangu_sim_cfg *c) {
 
 extern "C" {
 
+// Registers the shared fields (wave speeds, EMFs, flags, masks, recovery
+// codes and, while recording, the Kastaun roots) with the context.
+static int register_shared(pangu_sim *s) {
+  auto table = [&](double *base, int ncomp) {
+    std::vector<double *> t((size_t)s->nblk * ncomp);
+    for (int b = 0; b < s->nblk; ++b)
+      for (int q = 0; q < ncomp; ++q)
+        t[(size_t)b * ncomp + q] = base + s->boff[b] * ncomp + (int64_t)q * s->hb[b].cells;
+    return t;
+  };
+  std::vector<double *> ta = table(s->alfven, 3), te = table(s->emf, 3), tfl = table(s->flag, 1),
+                        tfr = table(s->flag_rec, 1);
+  std::vector<double *> tr = s->root ? table(s->root, 1) : std::vector<double *>();
+  std::vector<uint8_t *> tm(s->nblk);
+  std::vector<int32_t *> tst(s->nblk), tit(s->nblk);
+  for (int b = 0; b < s->nblk; ++b) {
+    tm[b] = s->mask + s->boff[b];
+    tst[b] = s->status + s->boff[b];
+    tit[b] = s->iters + s->boff[b];
+  }
+  pangu_shared sh;
+  sh.alfven = ta.data();
+  sh.emf = te.data();
+  sh.flag = tfl.data();
+  sh.mask = tm.data();
+  sh.c2p_status = tst.data();
+  sh.c2p_iters = tit.data();
+  sh.flag_recovery = tfr.data();
+  sh.c2p_root = s->root ? tr.data() : nullptr;
+  return pangu_register_shared(s->ctx, &sh);
+}
+
 int pangu_sim_create_batch(const pangu_sim_cfg *cfgs, int nmesh, int device, pangu_sim **out) {
   if (!cfgs || nmesh <= 0 || !out) return PANGU_EINVAL;
   *out = nullptr;
@@ -410,24 +443,7 @@ int pangu_sim_create_batch(const pangu_sim_cfg *cfgs, int nmesh, int device, pan
   r = register_cont(s->P[0], s->U[0], true, &s->h_base);
   if (!r) r = register_cont(s->P[1], s->U[1], true, &s->h_s1);
   if (!r) r = register_cont(s->P[1], s->dudt, false, &s->h_dudt);
-  std::vector<double *> ta = table(s->alfven, 3), te = table(s->emf, 3), tfl = table(s->flag, 1),
-                        tfr = table(s->flag_rec, 1);
-  std::vector<uint8_t *> tm(s->nblk);
-  std::vector<int32_t *> tst(s->nblk), tit(s->nblk);
-  for (int b = 0; b < s->nblk; ++b) {
-    tm[b] = s->mask + s->boff[b];
-    tst[b] = s->status + s->boff[b];
-    tit[b] = s->iters + s->boff[b];
-  }
-  pangu_shared sh;
-  sh.alfven = ta.data();
-  sh.emf = te.data();
-  sh.flag = tfl.data();
-  sh.mask = tm.data();
-  sh.c2p_status = tst.data();
-  sh.c2p_iters = tit.data();
-  sh.flag_recovery = tfr.data();
-  if (!r) r = pangu_register_shared(s->ctx, &sh);
+  if (!r) r = register_shared(s);
   if (r) {
     fprintf(stderr, "pangu_sim_create: %s\n", pangu_last_error(s->ctx));
     pangu_sim_destroy(s);
@@ -882,8 +898,20 @@ int pangu_sim_set_c2p_guess(pangu_sim *s, pangu_mlp *m, double delta) {
   return 0;
 }
 
+int pangu_sim_set_c2p_gate(pangu_sim *s, int min_prev_iters) {
+  PCHK(s, pangu_set_c2p_gate(s->ctx, min_prev_iters));
+  drop_graph(s);
+  return 0;
+}
+
 int pangu_sim_set_share_reconstruction(pangu_sim *s, int share) {
   PCHK(s, pangu_set_share_reconstruction(s->ctx, share));
+  drop_graph(s);
+  return 0;
+}
+
+int pangu_sim_set_pair_flux(pangu_sim *s, int dir_mask) {
+  PCHK(s, pangu_set_pair_flux(s->ctx, dir_mask));
   drop_graph(s);
   return 0;
 }
@@ -1001,6 +1029,7 @@ static void *field_ptr(pangu_sim *s, int f, int c, int64_t *bytes) {
     case F_ITERS: *bytes = T * 4; return s->iters;
     case F_FLAG_REC: *bytes = T * 8; return s->flag_rec;
     case F_DUDT: *bytes = T * s->nc * 8; return s->dudt;
+    case F_ROOT: *bytes = s->root ? T * 8 : 0; return s->root;
   }
   *bytes = 0;
   return nullptr;
@@ -1065,6 +1094,29 @@ int pangu_sim_diagnostics(pangu_sim *s, int container, const int *shell, double 
   if (!s->ddiag) SCHK(s, hipMalloc((void **)&s->ddiag, (size_t)s->nmesh * PANGU_NDIAG * sizeof(double)));
   PCHK(s, pangu_diagnostics(s->ctx, container == 0 ? s->h_base : s->h_s1, shell, s->ddiag, s->stream));
   SCHK(s, hipMemcpy(out, s->ddiag, (size_t)s->nmesh * PANGU_NDIAG * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pangu_sim_record_roots(pangu_sim *s, int on) {
+  if (!s) return PANGU_EINVAL;
+  SCHK(s, hipStreamSynchronize(s->stream));
+  if (on && !s->root) {
+    if (dalloc(s, &s->root, s->total)) return PANGU_ENOMEM;
+  } else if (!on && s->root) {
+    (void)hipFree(s->root);
+    s->root = nullptr;
+  }
+  if (int r = register_shared(s)) return sfail(s, r, pangu_last_error(s->ctx));
+  drop_graph(s);  // a captured cycle holds the old tables
+  return 0;
+}
+
+int pangu_sim_context(pangu_sim *s, pangu_ctx **ctx, int *handles, void **stream) {
+  if (!s || !ctx || !handles || !stream) return PANGU_EINVAL;
+  *ctx = s->ctx;
+  handles[0] = s->h_base;
+  handles[1] = s->h_s1;
+  *stream = (void *)s->stream;
   return 0;
 }
 

@@ -75,6 +75,53 @@ __device__ __forceinline__ bool decode_tile(const KCtx &C, const Range3 &R, int 
   return true;
 }
 
+// Lane-pair launches: every 256-cell tile of R is split over two
+// workgroups of 128 lane pairs; thread t handles cell (t >> 1) of its half
+// tile, and the two lanes of a pair share it.  Grid: 2 * ntiles workgroups
+// (padded to a multiple of 8), remapped to tiles like decode_tile.
+__device__ __forceinline__ bool decode_tile_pair(const KCtx &C, const Range3 &R, int &b, int &k, int &j,
+                                                 int &i) {
+  const int N2 = 2 * R.ntiles;
+  const int n8 = (N2 + 7) >> 3;
+  const int L = (int)blockIdx.x;
+  const int T2 = (L & 7) * n8 + (L >> 3);
+  if (T2 >= N2) return false;
+  const int T = T2 >> 1, sub = T2 & 1;
+  int lo[3], cnt[3], local, tpp;
+  if (R.uni) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = R.ulo[d];
+      cnt[d] = R.ucnt[d];
+    }
+    tpp = R.utpp;
+    const int per_block = cnt[2] * tpp;
+    b = T / per_block;
+    local = T - b * per_block;
+  } else {
+    b = R.tblock[T];
+    local = T - R.tfirst[b];
+    block_range(C.blk[b], R, lo, cnt);
+    tpp = (cnt[0] * cnt[1] + TILE - 1) / TILE;
+  }
+  const int plane = cnt[0] * cnt[1];
+  int tile, kk;
+  if (R.order) {
+    tile = local / cnt[2];
+    kk = local - tile * cnt[2];
+  } else {
+    kk = local / tpp;
+    tile = local - kk * tpp;
+  }
+  const int t = tile * TILE + sub * (TILE / 2) + ((int)threadIdx.x >> 1);
+  if (t >= plane) return false;
+  k = lo[2] + kk;
+  const int jj = t / cnt[0];
+  j = lo[1] + jj;
+  i = lo[0] + (t - jj * cnt[0]);
+  return true;
+}
+
 // Block and item index of this thread in an item launch.
 __device__ __forceinline__ void decode_item(const Items &I, int &b, int &item) {
   b = I.tblock[blockIdx.x];

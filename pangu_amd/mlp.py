@@ -33,6 +33,11 @@ def lib():
         L.pangu_mlp_evaluate.argtypes = [vp, C.c_double, dp]
         L.pangu_mlp_gemm.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, fp, fp, fp]
         L.pangu_sim_set_c2p_guess.argtypes = [vp, vp, C.c_double]
+        L.pangu_sim_set_c2p_gate.argtypes = [vp, C.c_int]
+        L.pangu_last_error.restype = C.c_char_p
+        L.pangu_last_error.argtypes = [vp]
+        L.pangu_mlp_harvest.argtypes = [vp, vp, C.c_int, C.c_int, C.POINTER(C.c_int64)]
+        L.pangu_mlp_predict_wave.argtypes = [vp, fp, C.c_int, fp]
         _bound = True
     return L
 
@@ -118,6 +123,25 @@ class Mlp:
         self._check(lib().pangu_mlp_predict(self._p, _fp(feats), n, _fp(out)), "predict")
         return out
 
+    def predict_wave(self, feats):
+        """The recovery kernel's in-wave inference path on raw features."""
+        feats = _f32(feats)
+        n = feats.shape[0]
+        out = np.zeros(n, dtype=np.float32)
+        self._check(lib().pangu_mlp_predict_wave(self._p, _fp(feats), n, _fp(out)), "predict_wave")
+        return out
+
+    def harvest(self, hipsim, container=0, stride=1):
+        """Appends the states of a HipSim container (every stride-th interior
+        cell, solved by plain Kastaun) to the data set; returns how many
+        solved states were added."""
+        ctx, handles, _ = hipsim.context()
+        kept = C.c_int64()
+        r = lib().pangu_mlp_harvest(self._p, ctx, handles[container], stride, C.byref(kept))
+        if r != 0:
+            raise RuntimeError(f"harvest failed ({r}): {lib().pangu_last_error(ctx).decode()}")
+        return kept.value
+
     def evaluate(self, delta=0.02):
         out = np.zeros(7)
         self._check(lib().pangu_mlp_evaluate(self._p, delta, out.ctypes.data_as(C.POINTER(C.c_double))),
@@ -126,11 +150,16 @@ class Mlp:
                 "max_rel_diff", "failures")
         return dict(zip(keys, out.tolist()))
 
-    def attach(self, hipsim, delta=0.02):
-        """Kastaun recovery of a HipSim starts from this model's bracket."""
+    def attach(self, hipsim, delta=0.02, gate=0):
+        """Kastaun recovery of a HipSim starts from this model's bracket in
+        the cells whose previous recovery took at least `gate` false-position
+        steps (0: every cell)."""
         r = lib().pangu_sim_set_c2p_guess(hipsim._p, self._p, delta)
         if r != 0:
             raise RuntimeError(f"attach failed ({r})")
+        r = lib().pangu_sim_set_c2p_gate(hipsim._p, int(gate))
+        if r != 0:
+            raise RuntimeError(f"gate failed ({r})")
 
 
 def detach(hipsim):

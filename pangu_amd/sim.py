@@ -60,7 +60,7 @@ class SimCfg(C.Structure):
 
 
 FIELDS = dict(prim=0, cons=1, flux1=2, flux2=3, flux3=4, alfven=5, emf=6, flag=7, mask=8,
-              c2p_status=9, c2p_iters=10, flag_recovery=11, dudt=12)
+              c2p_status=9, c2p_iters=10, flag_recovery=11, dudt=12, c2p_root=13)
 TASKS = dict(fix_primitive=0, conservative=1, fluxes=2, ct=3, flux_div=4, update=5, source=6,
              recovery=7, heating=8, fix_recovery=9, exchange=10, bc=11, timestep=12,
              flux_correction=13)
@@ -85,6 +85,10 @@ def lib():
         L.pangu_sim_create_batch.argtypes = [C.POINTER(SimCfg), C.c_int, C.c_int, C.POINTER(vp)]
         L.pangu_sim_mesh_state.argtypes = [vp, C.c_int, dp, dp, ip, ip]
         L.pangu_sim_diagnostics.argtypes = [vp, C.c_int, ip, dp]
+        L.pangu_sim_context.argtypes = [vp, C.POINTER(vp), ip, C.POINTER(vp)]
+        L.pangu_sim_record_roots.argtypes = [vp, C.c_int]
+        L.pangu_sim_set_pair_flux.argtypes = [vp, C.c_int]
+        L.pangu_sim_set_c2p_gate.argtypes = [vp, C.c_int]
         L.pangu_sim_nblocks.argtypes = [vp]
         L.pangu_sim_plain_loop.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_int, dp]
         L.pangu_sim_autotune.argtypes = [vp, C.c_int, C.c_int, ip, ip, ip]
@@ -194,6 +198,18 @@ class HipSim:
         res = [dict(zip(DIAG_NAMES, out[m * len(DIAG_NAMES):(m + 1) * len(DIAG_NAMES)])) for m in range(nm)]
         return res[0] if nm == 1 else res
 
+    def record_roots(self, on=True):
+        """Kastaun recovery records each cell's root (get("c2p_root"))."""
+        self._check(lib().pangu_sim_record_roots(self._p, int(on)), "record_roots")
+
+    def context(self):
+        """(pangu_ctx pointer, container handles (base, stage 1), stream) of
+        this simulation, for calling pangu_hip entry points directly."""
+        ctx, st = C.c_void_p(), C.c_void_p()
+        h = (C.c_int * 2)()
+        self._check(lib().pangu_sim_context(self._p, C.byref(ctx), h, C.byref(st)), "context")
+        return ctx, (h[0], h[1]), st
+
     def history(self, cycles, every, shell, mode="graph"):
         """Steps `cycles` cycles and records the diagnostics every `every`
         cycles: a list of (cycles done, time of mesh 0, diagnostics)."""
@@ -284,6 +300,11 @@ class HipSim:
         """Fused stage: geometric source in a kernel of its own (1) or fused
         with the flux divergence and update (0).  Results are unchanged."""
         self._check(lib().pangu_sim_set_split_source(self._p, int(split)), "set_split_source")
+
+    def set_pair_flux(self, dir_mask):
+        """Bit d-1: the sweep along d runs two lanes per face (k_flux_pair).
+        Results are unchanged."""
+        self._check(lib().pangu_sim_set_pair_flux(self._p, int(dir_mask)), "set_pair_flux")
 
     def set_fuse_ct(self, fuse):
         """Fused stage: constrained-transport face B fluxes formed inside the

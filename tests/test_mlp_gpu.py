@@ -89,7 +89,8 @@ def test_predict_matches_fp64_reference(trained):
 
 @pytest.mark.gpu
 def test_learned_bracket_in_torus_run(trained):
-    """The fused per-cell inference kernel drives the recovery of a 3D torus:
+    """The learned bracket (model evaluated inside the recovery kernel, gate
+    0: every cell) drives the recovery of a 3D torus:
     fields stay within the solver tolerance of the plain run and the
     recovery takes no more false-position steps.  (Most torus cells are cold
     atmosphere with a root near 1, where the plain search already needs only
@@ -111,3 +112,105 @@ def test_learned_bracket_in_torus_run(trained):
     assert max(per_comp_l1(runs[1][0], runs[0][0])) <= 1e-9
     assert max(per_comp_l1(runs[1][1], runs[0][1])) <= 1e-9
     assert runs[1][2] <= runs[0][2]
+
+
+@pytest.mark.gpu
+def test_in_wave_inference_matches_training_forward(trained):
+    """The recovery kernel's inference (one wavefront over 64 states, the
+    activations kept in the matrix-core fragments, weights staged in LDS)
+    gives the training forward pass's prediction up to its hardware exp and
+    reciprocal in the activation (a BF16 rounding flip at most here and
+    there)."""
+    m, _ = trained
+    rng = np.random.default_rng(6)
+    n = 1000  # not a multiple of 64: the last wavefront is partial
+    feats = np.stack([rng.uniform(-0.5, 3, n), rng.uniform(0, 6, n), rng.uniform(0, 4, n),
+                      rng.uniform(0, 6, n), rng.uniform(0, 1, n)], 1).astype(np.float32)
+    gw = m.predict_wave(feats)
+    gf = m.predict(feats)
+    d = np.abs(gw - gf)
+    print("wave vs forward: max %.2e mean %.2e" % (d.max(), d.mean()), flush=True)
+    assert d.max() <= 2e-3 and d.mean() <= 2e-4
+
+
+def _cell_metrics(h, cfg):
+    """Code-coordinate metric at the interior cell centres of a one-block
+    2-D mesh (the oracle's evaluation of the metric the tables hold)."""
+    import oracle_py as O
+    xg, n = h.block_geom(0)
+    s, e = h.s, h.e
+    out = {}
+    for j in range(s[1], e[1] + 1):
+        for i in range(s[0], e[0] + 1):
+            x = [0.0, xg[0] + (i + 0.5) * xg[3], xg[1] + (j + 0.5) * xg[4], xg[2] + (s[2] + 0.5) * xg[5]]
+            gcov, gcon, _, _ = O.metric_at(3, x, cfg["a"], cfg["h"])
+            out[(j, i)] = (gcov, gcon)
+    return out
+
+
+@pytest.mark.gpu
+def test_learned_bracket_on_the_2d_torus_workload():
+    """The learned bracket on the 2-D MKS torus workload (the 512x256 run at
+    half resolution per dimension, Kastaun recovery), trained on seeded
+    synthetic draws plus states harvested from the plain run itself:
+    - every accepted cell's root mu satisfies |f(mu)| <= 1e-10 (Kastaun's
+      master function evaluated by the oracle from the cell's conserved
+      state), as every root of the plain run does;
+    - P and U stay within the solver tolerance of the plain run after ten
+      cycles;
+    - on the cells the gate sends to the model (previous recovery at least
+      GATE steps) the learned search takes fewer false-position steps."""
+    import oracle_py as O
+    from pangu_amd import configs, mlp
+    from pangu_amd.sim import HipSim
+    GATE = 4
+    cfg = dict(configs.torus2d(256, 128), recovery=1)
+    m = mlp.Mlp(seed=3)
+    m.make_dataset(1 << 18, seed=21)
+    src = HipSim(cfg)
+    src.init()
+    harvested = 0
+    for _ in range(4):
+        src.step(2, "graph")
+        harvested += m.harvest(src, 0)
+    src.close()
+    print("harvested", harvested, flush=True)
+    assert harvested > 0.95 * 4 * 256 * 128
+    m.train(epochs=8, batch=1024, lr=2e-3)
+    runs = {}
+    for name in ("plain", "learned"):
+        h = HipSim(cfg)
+        h.init()
+        h.record_roots(True)
+        if name == "learned":
+            m.attach(h, delta=0.01, gate=GATE)
+        h.step(9, "graph")
+        prev = h.interior(h.get("c2p_iters")).copy()
+        h.step(1, "graph")
+        runs[name] = dict(P=h.interior(h.get("prim", 0)), U=h.interior(h.get("cons", 0)),
+                          it=h.interior(h.get("c2p_iters")), st=h.interior(h.get("c2p_status")),
+                          mu=h.interior(h.get("c2p_root")), prev=prev, h=h)
+    geo = _cell_metrics(runs["plain"]["h"], cfg)
+    for name, r in runs.items():
+        worst, nacc = 0.0, 0
+        U, mu, st = r["U"][0], r["mu"][0, 0, 0], r["st"][0, 0, 0]
+        for (j, i), (gcov, gcon) in geo.items():
+            jj, ii = j - r["h"].s[1], i - r["h"].s[0]
+            if st[jj, ii] != 0:
+                continue
+            nacc += 1
+            res = O.kastaun_residual(U[:, 0, jj, ii], mu[jj, ii], cfg["gamma"], gcov, gcon)
+            worst = max(worst, res)
+        print(name, "accepted", nacc, "of", st.size, "max |f(mu)|", worst, flush=True)
+        assert nacc > 0.99 * st.size and worst <= 1e-10
+    lp = per_comp_l1(runs["learned"]["P"], runs["plain"]["P"])
+    lu = per_comp_l1(runs["learned"]["U"], runs["plain"]["U"])
+    print("learned vs plain after 10 cycles: P", max(lp), "U", max(lu), flush=True)
+    assert max(lp) <= 1e-8 and max(lu) <= 1e-8
+    hard = runs["plain"]["prev"] >= GATE
+    ip, il = runs["plain"]["it"][hard].mean(), runs["learned"]["it"][hard].mean()
+    print("gated cells %d of %d: steps plain %.2f learned %.2f; all cells plain %.3f learned %.3f" % (
+        hard.sum(), hard.size, ip, il, runs["plain"]["it"].mean(), runs["learned"]["it"].mean()), flush=True)
+    assert hard.sum() > 0 and il < ip
+    for r in runs.values():
+        r["h"].close()

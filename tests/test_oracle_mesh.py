@@ -221,3 +221,25 @@ def test_ct_task_without_field_is_a_no_op():
     after = [s.get(f) for f in ("flux1", "flux2", "emf")]
     for a, b in zip(before, after):
         assert np.array_equal(a, b)
+
+
+def test_unbound_torus_leaves_the_atmosphere():
+    """A Fishbone-Moncrief torus whose inner edge lies inside the marginally
+    bound orbit of the spin (a = 0, r_in = 5 < 6) has no cell with positive
+    enthalpy: the mesh keeps the zero state (no NaN from a division by the
+    zero maximum density), and one cycle floors it to a finite atmosphere.
+    The ensemble workload draws such tori (r_in ~ U(5, 7) at any spin)."""
+    d = configs.torus2d(64, 32)
+    d.update(a=0.0, rin=5.0, rmax=11.0)
+    o = O.OracleSim(d)
+    o.init()
+    P = o.get("prim", 0)
+    assert np.isfinite(P).all() and not P[:, :2].any()
+    o.step(1)
+    Pi = o.interior(o.get("prim", 0))
+    assert np.isfinite(Pi).all() and (Pi[:, 0] > 0).all()
+    # a bound torus of the same spin (r_in = 6.5) is normalised to max rho 1
+    d.update(rin=6.5, rmax=12.5)
+    o = O.OracleSim(d)
+    o.init()
+    assert o.get("prim", 0)[:, 0].max() == 1.0

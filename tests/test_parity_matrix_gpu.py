@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import oracle_py as O
-from helpers import per_comp_l1
+from helpers import assert_nonfinite_confined, per_comp_l1
 from pangu_amd import configs
 
 
@@ -73,6 +73,10 @@ MATRIX = {
 }
 
 
+PAIR_CASES = ["ot_minkowski_1d", "torus3d_odd_blocks", "ot_cks3d_laxf_mc_chainer_heat",
+              "torus2d_mc_hll", "ot_bl2d_hll_mc_chainer", "ot_cks3d_excision_kastaun"]
+
+
 def _compare(o, h, label, tol=1e-12):
     for c in (0, 1):
         lp = per_comp_l1(h.interior(h.get("prim", c)), o.interior(o.get("prim", c)))
@@ -104,6 +108,8 @@ def test_matrix_one_cycle(name, mode):
     o.step(1)
     h.step(1, mode)
     _compare(o, h, name)
+    for s in (o, h):
+        assert_nonfinite_confined(s, cfg, name)
 
 
 # Failed-recovery patterns (block, k, j, i in interior indices) on 16x8x8 blocks.
@@ -116,6 +122,31 @@ FAILURES = [
     (4, 1, 4, 4), (4, 2, 4, 4),                    # neighbours in k only
     (5, 5, 6, 1), (5, 5, 7, 2), (5, 5, 6, 3),      # diagonal chain
 ]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", PAIR_CASES)
+def test_matrix_pair_flux(name):
+    """The two-lanes-per-face flux sweeps (both limiters and solvers, 1-D to
+    3-D, polar and excised faces) against the oracle and bit for bit against
+    the one-lane sweeps."""
+    from pangu_amd.sim import HipSim
+    cfg = MATRIX[name]
+    o = O.OracleSim(cfg)
+    o.init()
+    o.step(1)
+    runs = []
+    for pair in (0, 7):
+        h = HipSim(cfg)
+        h.init()
+        h.set_pair_flux(pair)
+        h.step(1, "graph")
+        if pair:
+            _compare(o, h, name + " pair")
+        runs.append((h.get("prim", 0), h.get("cons", 0), h.get("prim", 1), h.get("alfven")))
+        h.close()
+    for a, b in zip(runs[0], runs[1]):
+        assert np.array_equal(a, b)
 
 
 @pytest.mark.gpu
@@ -151,6 +182,96 @@ def test_fix_recovery_patterns():
     assert np.array_equal(o.interior(o.get("flag")), h.interior(h.get("flag")))
 
 
+def _third_stage_states(n, seed=11):
+    """States whose recovery reaches the chainer's third scheme and succeeds
+    there (invertor.cc:184-198): magnetised flat-space states whose momentum
+    is inflated past the energy's reach, with the exact primitives as the
+    guess.  2D (scheme_2d.h) and 1Dvsq (scheme_1d_vsq.h) then stop without
+    converging (status 101) and 1D (scheme_1d.h:411-568) converges.  Found by
+    a seeded search with the oracle's per-scheme solvers; each is checked
+    scheme by scheme.  Returns (U as the conserved container holds it,
+    guess) pairs and the per-scheme (status, iterations)."""
+    gam = 5.0 / 3.0
+    eta = np.diag([-1.0, 1.0, 1.0, 1.0])
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(200000):
+        p = np.zeros(10)
+        p[0] = 10 ** rng.uniform(-3, 1)
+        p[1] = p[0] * 10 ** rng.uniform(-6, 1)
+        p[2:5] = rng.normal(size=3) * 10 ** rng.uniform(-1, 1.5)
+        p[5:8] = rng.normal(size=3) * np.sqrt(p[0]) * 10 ** rng.uniform(-1, 2)
+        p[8] = (gam - 1.0) * p[1] * p[0] ** (-gam)
+        U = O.flux(gam, p, eta, eta, -1.0, 0)[:9]
+        U[2:5] *= 1.0 + 10 ** rng.uniform(-3, 1)
+        Uh = U.copy()
+        Uh[1] = U[1] - U[0]  # invertor.cc:149-174 with alpha = sqrt|g| = 1
+        g = p[:9].copy()
+        s1, _, i1 = O.c2p(1, Uh, g, gam, eta, eta)
+        if s1 == 0:
+            continue
+        s0, _, i0 = O.c2p(0, Uh, g, gam, eta, eta)
+        s2, _, i2 = O.c2p(2, Uh, g, gam, eta, eta)
+        if s0 != 0 and s2 == 0:
+            out.append((U, g, ((s0, i0), (s1, i1), (s2, i2))))
+            if len(out) == n:
+                break
+    assert len(out) == n
+    return out
+
+
+@pytest.mark.gpu
+def test_chainer_third_stage_succeeds():
+    """Cells whose 2D and 1Dvsq recoveries fail and whose 1D recovery
+    succeeds, planted in a chainer run on a flat mesh next to D < 0 cells
+    that fail all three: status, iteration counts (the sum over the schemes
+    tried), flags and recovered fields equal the oracle's."""
+    from pangu_amd.sim import HipSim
+    cfg = dict(configs.orszag_tang(32, (2, 2, 1)), recovery=0)
+    states = _third_stage_states(12)
+    sims = [O.OracleSim(cfg), HipSim(cfg)]
+    for s in sims:
+        s.init()
+        s.task("fix_primitive", 0)
+        s.task("conservative", 0)
+    o, h = sims
+    U = o.get("cons", 0)
+    P = o.get("prim", 0)
+    cells = [(b, 0, j, i) for b in range(4) for j, i in ((2, 3), (9, 12), (14, 0))]
+    for (b, k, j, i), (u, g, _) in zip(cells, states):
+        at = (b, slice(None), o.s[2] + k, o.s[1] + j, o.s[0] + i)
+        U[at] = u
+        P[at] = g
+    dneg = [(0, 0, 7, 7), (3, 0, 5, 10)]
+    for b, k, j, i in dneg:
+        U[b, 0, o.s[2] + k, o.s[1] + j, o.s[0] + i] = -1.0
+    for s in sims:
+        s.set("cons", U, 1)
+        s.set("prim", P, 1)
+        s.set("flag", np.ones_like(o.get("flag")))
+        s.task("recovery", 0, 1)
+    shp = (o.nblocks, 1, o.n[2], o.n[1], o.n[0])
+    st = {}
+    for f in ("flag_recovery", "c2p_status", "c2p_iters"):
+        a = o.get(f).reshape(shp)
+        b = h.get(f).reshape(shp)
+        assert np.array_equal(o.interior(a), h.interior(b)), f
+        st[f] = a
+    for (b, k, j, i), (_, _, per) in zip(cells, states):
+        at = (b, 0, o.s[2] + k, o.s[1] + j, o.s[0] + i)
+        assert st["c2p_status"][at] == 0 and st["flag_recovery"][at] == 1.0
+        assert st["c2p_iters"][at] == per[0][1] + per[1][1] + per[2][1]
+    for b, k, j, i in dneg:
+        assert st["c2p_status"][b, 0, o.s[2] + k, o.s[1] + j, o.s[0] + i] != 0
+    lp = per_comp_l1(h.interior(h.get("prim", 1)), o.interior(o.get("prim", 1)))
+    print("third stage P", max(lp), flush=True)
+    assert max(lp) <= 1e-12
+    Po, Ph = o.get("prim", 1), h.get("prim", 1)
+    for b, k, j, i in cells:
+        at = (b, slice(0, 5), o.s[2] + k, o.s[1] + j, o.s[0] + i)
+        assert np.allclose(Ph[at], Po[at], rtol=1e-12, atol=0.0)
+
+
 @pytest.mark.gpu
 def test_launch_tuning_leaves_results_unchanged():
     """Tile order, the split of the source kernel and forming the CT face
@@ -159,15 +280,18 @@ def test_launch_tuning_leaves_results_unchanged():
     from pangu_amd.sim import HipSim, KERNEL_FAMILIES
     cfg = _torus3d()
     runs = []
-    for variant in ("default", "split", "column", "plane", "share", "ct_faces", "split_ct_faces"):
+    for variant in ("default", "split", "column", "plane", "share", "ct_faces", "split_ct_faces",
+                    "pair", "pair_share"):
         h = HipSim(cfg)
         h.init()
         if variant in ("split", "split_ct_faces"):
             h.set_split_source(1)
         if variant in ("ct_faces", "split_ct_faces"):
             h.set_fuse_ct(0)
-        if variant == "share":
+        if variant in ("share", "pair_share"):
             h.set_share_reconstruction(1)
+        if variant in ("pair", "pair_share"):
+            h.set_pair_flux(7)
         for fam in KERNEL_FAMILIES:
             if variant == "column":
                 h.set_tile_order(fam, 1)

@@ -7,6 +7,7 @@ import pytest
 
 import oracle_py as O
 from pangu_amd import configs
+from helpers import assert_nonfinite_confined
 from test_parity_matrix_gpu import _compare
 from test_refinement import cks_smr_mesh
 
@@ -103,6 +104,8 @@ def test_refined_one_cycle(name, mode):
     o.step(1)
     h.step(1, mode)
     _compare(o, h, name)
+    for s in (o, h):
+        assert_nonfinite_confined(s, cfg, name)
 
 
 @pytest.mark.gpu
@@ -118,6 +121,8 @@ def test_refined_ten_cycles(name):
     o.step(10)
     h.step(10, "graph")
     _compare(o, h, name, tol=1e-9)
+    for s in (o, h):
+        assert_nonfinite_confined(s, cfg, name)
 
 
 @pytest.mark.gpu
