@@ -277,6 +277,18 @@ enum {
 };
 int pangu_diagnostics(pangu_ctx *ctx, int md, const int *shell, double *out, void *stream);
 
+/* pangu_diagnostics without any device allocation (no counterpart in the
+ * reference).  For callers that cannot allocate device memory: code under
+ * stream capture, tight serving loops, hosts that forbid hipMalloc.  Same
+ * arguments, checks and results (bit for bit) as pangu_diagnostics, which
+ * allocates its per-tile workspace on first use; this variant uses a static
+ * workspace of 8 MiB per device instead (held until the process ends) and
+ * returns PANGU_EINVAL when the context needs more: about 3.3e7 interior
+ * zones (131 000 tiles of 256 cells, less one double per two meshes).  Every
+ * call on a device shares that workspace, so calls that overlap on one
+ * device (other host threads, other streams) race; calls on one stream from
+ * one thread do not.  Synchronous, like pangu_diagnostics. */
+int pangu_diagnostics_noalloc(pangu_ctx *ctx, int md, const int *shell, double *out, void *stream);
 
 #ifdef __cplusplus
 }

@@ -65,6 +65,9 @@ struct pangu_ctx {
   double *ddiag_part = nullptr;
   int *ddiag_mtile = nullptr, *ddiag_shell = nullptr;
   std::vector<int> diag_mtile;         // first interior tile of each mesh (nmesh + 1)
+  // the device's static diagnostics workspace (pangu_diagnostics_noalloc)
+  double *dscratch = nullptr;
+  std::string scratch_err;
   LaunchCtx L;
   std::string err;
 };
@@ -621,6 +624,16 @@ int pangu_create_batch(const pangu_config *cfgs, int nmesh, const pangu_block *b
     pangu_destroy(c);
     return r;
   }
+  {
+    // address of this device's static diagnostics workspace, looked up once
+    // per device with the device current; a failure here only disables
+    // pangu_diagnostics_noalloc
+    const hipError_t e = diag_scratch_address(device, &c->dscratch);
+    if (e != hipSuccess) {
+      c->dscratch = nullptr;
+      c->scratch_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    }
+  }
   KCtx &k = c->L.k;
   k.blk = c->dblk;
   k.geo = c->dgeo;
@@ -1038,6 +1051,37 @@ int pangu_diagnostics(pangu_ctx *c, int md, const int *shell, double *out, void 
   LCHK(c, launch_diagnostics(c->L, c->conts[md].kf(), c->ddiag_shell, c->ddiag_part, c->ddiag_mtile,
                              c->nmesh, out, s));
   // the shell indices are read from pageable host memory: wait before returning
+  HIPCHK(c, hipStreamSynchronize(s));
+  return PANGU_OK;
+}
+
+int pangu_diagnostics_noalloc(pangu_ctx *c, int md, const int *shell, double *out, void *stream) {
+  if (int r = check_diag_args(c, md, shell, out)) return r;
+  if (!c->dscratch)
+    return fail(c, PANGU_EDEVICE, "diagnostics_noalloc: no static workspace on this device (" +
+                                      c->scratch_err + ")");
+  // workspace: tiles x PANGU_NDIAG partials, then nmesh + 1 tile offsets and
+  // nmesh shell indices (ints, rounded up to whole doubles)
+  const int64_t tiles = c->diag_mtile[c->nmesh];
+  const int64_t nint = 2 * (int64_t)c->nmesh + 1;
+  const int64_t need = tiles * PANGU_NDIAG + (nint + 1) / 2;
+  if (need > kDiagScratchDoubles)
+    return fail(c, PANGU_EINVAL,
+                "diagnostics_noalloc: this context needs " + std::to_string(need) +
+                    " doubles of workspace (" + std::to_string(tiles) +
+                    " interior tiles); the static workspace holds " +
+                    std::to_string(kDiagScratchDoubles) + " (8 MiB): use pangu_diagnostics");
+  HIPCHK(c, hipSetDevice(c->device));
+  hipStream_t s = STREAM(stream);
+  double *part = c->dscratch;
+  int *mtile = (int *)(c->dscratch + tiles * PANGU_NDIAG);
+  int *dshell = mtile + c->nmesh + 1;
+  // every tile writes all of its partials, and both int arrays are written
+  // here, so nothing left by an earlier call is read
+  HIPCHK(c, hipMemcpyAsync(mtile, c->diag_mtile.data(), (size_t)(c->nmesh + 1) * sizeof(int),
+                           hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(dshell, shell, (size_t)c->nmesh * sizeof(int), hipMemcpyHostToDevice, s));
+  LCHK(c, launch_diagnostics(c->L, c->conts[md].kf(), dshell, part, mtile, c->nmesh, out, s));
   HIPCHK(c, hipStreamSynchronize(s));
   return PANGU_OK;
 }

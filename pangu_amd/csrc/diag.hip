@@ -12,12 +12,45 @@
 // volume integrals and |U_B|, and corners that touch one are left out of div B.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "common.h"
 #include "kernels.h"
 #include "physics_dev.h"
 #include "tiles_dev.h"
 
 namespace pangu {
+
+// Workspace of pangu_diagnostics_noalloc: the per-tile partial sums, then
+// the per-mesh tile ranges and shell indices as ints.  One copy per device,
+// held until the process ends; every call on a device shares it.  Host code
+// never passes `scratch` itself to a kernel (on the host the name is only a
+// stand-in for the device symbol): it uses the address that
+// diag_scratch_address() looks up once per device.
+static __device__ __align__(256) double scratch[kDiagScratchDoubles];
+
+namespace {
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kDiagMaxDevices] = {};
+}  // namespace
+
+hipError_t diag_scratch_address(int device, double **out) {
+  *out = nullptr;
+  if (device < 0 || device >= kDiagMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device) return hipErrorInvalidDevice;  // the caller makes `device` current
+    double *ptr = nullptr;
+    e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = ptr;
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
 
 template <bool HB>
 __global__ __launch_bounds__(256) void k_diag(KCtx C, KFields F, const int *shell, double *part,

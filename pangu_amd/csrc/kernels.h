@@ -172,5 +172,13 @@ int launch_diagnostics(const LaunchCtx &L, const KFields &F, const int *shell, d
 int launch_dt(const LaunchCtx &L, double *const *alfven, double *red, double *state,
               double dt_factor, double *est_out, hipStream_t s);
 
+// Static workspace of the allocation-free diagnostics entry point (diag.hip):
+// 2^20 doubles (8 MiB) per device, enough for the per-tile partials of
+// 131 000 interior tiles (about 3.3e7 interior zones) plus the per-mesh ints.
+constexpr int64_t kDiagScratchDoubles = int64_t(1) << 20;
+constexpr int kDiagMaxDevices = 64;
+// Device address of that workspace on `device`, which must be the current
+// device; looked up once per device (hipGetSymbolAddress) and kept.
+hipError_t diag_scratch_address(int device, double **out);
 
 }  // namespace pangu
