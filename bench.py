@@ -50,7 +50,7 @@ STATE_WORDS_PER_ZONE_STAGE = 40
 # re-tunes.
 TUNED = {"order": {"fix_cons": 1, "flux_x1": 1, "flux_x2": 0, "flux_x3": 1, "ct": 0,
                    "div_update": 1, "recovery": 1},
-         "split_source": 0, "share_reconstruction": 0}
+         "split_source": 0, "share_reconstruction": 0, "pair_flux": 0, "flux_march": [0, 0]}
 
 # Per launch under TUNED, taken from the rocprofv3 counter runs recorded in
 # PROFILE (they are not measured by this script):
@@ -60,9 +60,12 @@ TUNED = {"order": {"fix_cons": 1, "flux_x1": 1, "flux_x2": 0, "flux_x3": 1, "ct"
 #    transcendentals once, FMAs twice), i.e. the FP64 work the GPU executes.
 PROFILE = "profiles/counters_torus3d_256x128x128.md"
 
-# Learned Kastaun bracket: the model serves cells whose previous recovery
-# took at least this many false-position steps.
-LEARNED_GATE = 4
+# Learned Kastaun bracket: runs with the model serving every wavefront
+# (gate 0), only wavefronts holding a cell whose previous recovery took at
+# least 5 false-position steps (about 4 % of the torus cells), and none
+# (gate 7: no torus cell takes more than 6), the last measuring what the
+# gated kernel costs when the network never runs.
+LEARNED_GATES = (0, 5, 7)
 MEASURED = {
     "div_update_source": {"bytes": 2218881421, "fp64_flops": 2.649e9},
     "flux_x1": {"bytes": 881801660, "fp64_flops": 5.640e9},
@@ -253,7 +256,7 @@ def refined_run(cycles=20, warmup=2):
     return out
 
 
-def learned_vs_plain(cfg3, samples, cycles=20, delta=0.01, gate=LEARNED_GATE):
+def learned_vs_plain(cfg3, samples, cycles=20, delta=0.01, gates=LEARNED_GATES):
     """Learned Kastaun bracket on the 2-D MKS torus workload (512x256,
     Kastaun recovery) and on the 3-D benchmark torus.  Training pairs: the
     seeded synthetic draws (`samples` states) plus states harvested from the
@@ -261,7 +264,7 @@ def learned_vs_plain(cfg3, samples, cycles=20, delta=0.01, gate=LEARNED_GATE):
     times).  Reported: held-out iterations (synthetic and harvested states,
     plain vs learned at the same 1e-10 step tolerance), and for each run
     zone-cycles/s and false-position steps per cell, over all cells and over
-    the cells the gate sends to the model."""
+    the cells passing each gate."""
     from pangu_amd import configs, mlp
     from pangu_amd.sim import HipSim
     cfg2 = dict(configs.torus2d(512, 256), recovery=1)
@@ -276,11 +279,11 @@ def learned_vs_plain(cfg3, samples, cycles=20, delta=0.01, gate=LEARNED_GATE):
         harvested += m.harvest(src, 0)
     t_data = time.perf_counter() - t0
     t0 = time.perf_counter()
-    loss = m.train(epochs=6, batch=1024, lr=2e-3)
+    loss = m.train(epochs=8, batch=1024, lr=2e-3)
     t_train = time.perf_counter() - t0
     out = {"train_pairs": synth + harvested, "train_synthetic": synth, "train_harvested": harvested,
            "data_s": t_data, "train_s": t_train, "loss_first_last": [float(loss[0]), float(loss[-1])],
-           "delta": delta, "gate_min_prev_iters": gate}
+           "delta": delta}
     m.make_dataset(1 << 18, seed=12)  # held-out synthetic states
     out["heldout_synthetic"] = m.evaluate(delta=delta)
     m.make_dataset(64, seed=13)       # held-out states of the same run, 16 cycles later
@@ -290,28 +293,29 @@ def learned_vs_plain(cfg3, samples, cycles=20, delta=0.01, gate=LEARNED_GATE):
     out["heldout_harvested"] = m.evaluate(delta=delta)
     for name, cfg in (("torus2d_512x256", cfg2), ("torus3d", cfg3)):
         zones = configs.zones(cfg)
-        res = {}
-        for learned in (False, True):
+        runs = {}
+        for gate in (None,) + tuple(gates):
             h = HipSim(cfg)
             h.init()
-            if learned:
+            if gate is not None:
                 m.attach(h, delta=delta, gate=gate)
             h.step(10, "graph")
             ms = h.timed_cycles(cycles, "graph")
             prev = h.interior(h.get("c2p_iters")).copy()
             h.step(1, "graph")
-            it = h.interior(h.get("c2p_iters"))
-            res["learned" if learned else "plain"] = (zones * cycles / (ms * 1e-3), prev, it)
+            runs[gate] = (zones * cycles / (ms * 1e-3), prev, h.interior(h.get("c2p_iters")))
             h.close()
-        hard = res["plain"][1] >= gate
-        r = {"zone_cycles_per_s_plain": res["plain"][0], "zone_cycles_per_s_learned": res["learned"][0],
-             "speed_ratio": res["learned"][0] / res["plain"][0],
-             "steps_per_cell_plain": float(res["plain"][2].mean()),
-             "steps_per_cell_learned": float(res["learned"][2].mean()),
-             "gated_fraction": float(hard.mean())}
-        if hard.any():
-            r["steps_gated_cells_plain"] = float(res["plain"][2][hard].mean())
-            r["steps_gated_cells_learned"] = float(res["learned"][2][hard].mean())
+        plain = runs[None]
+        r = {"zone_cycles_per_s_plain": plain[0], "steps_per_cell_plain": float(plain[2].mean())}
+        for gate in gates:
+            zc, _, it = runs[gate]
+            hard = plain[1] >= gate
+            g = {"zone_cycles_per_s": zc, "speed_ratio": zc / plain[0],
+                 "steps_per_cell": float(it.mean()), "gated_fraction": float(hard.mean())}
+            if hard.any():
+                g["steps_gated_cells_plain"] = float(plain[2][hard].mean())
+                g["steps_gated_cells_learned"] = float(it[hard].mean())
+            r[f"gate_{gate}"] = g
         out[name] = r
     m.close()
     return out
@@ -345,6 +349,9 @@ def main():
             sim.set_tile_order(fam, order)
         sim.set_split_source(TUNED["split_source"])
         sim.set_share_reconstruction(TUNED["share_reconstruction"])
+        sim.set_pair_flux(TUNED["pair_flux"])
+        for d, depth in zip((2, 3), TUNED["flux_march"]):
+            sim.set_flux_march(d, depth)
     sim.step(args.warmup, args.mode)
     ms = sim.timed_cycles(args.steps, args.mode)
     wall0 = time.perf_counter()

@@ -242,6 +242,13 @@ int pangu_set_share_reconstruction(pangu_ctx *ctx, int share);
  * flux components).  Results are identical.  Ignored by an X1 sweep that
  * shares reconstructions (pangu_set_share_reconstruction). */
 int pangu_set_pair_flux(pangu_ctx *ctx, int dir_mask);
+/* depth > 0: the flux sweep along dir (2 or 3) runs `depth` consecutive
+ * faces per thread along dir, reconstructing each cell of the column once
+ * (depth + 1 reconstructions instead of 2 depth).  0 (default): one face
+ * per thread.  Takes precedence over pangu_set_pair_flux for that
+ * direction.  Results are identical.  Tile order (pangu_set_tile_order of
+ * the sweep's family) applies to its chunks of faces. */
+int pangu_set_flux_march(pangu_ctx *ctx, int dir, int depth);
 /* fuse = 1 (default): the fused stage runs no constrained-transport kernel;
  * the flux-divergence + update kernel forms the edge EMFs from the Riemann B
  * fluxes and the face B fluxes from them (the same arithmetic as

@@ -13,9 +13,10 @@
  * with FP32 accumulation; FP32 master weights are trained with Adam.
  *
  * Features of a state (from D, q = tau/D, r^2 = S^2/D^2, b^2 = B^2/D and
- * (r.b)^2 of the Kastaun setup): log(1 + max(q, -0.5)), log(1 + r^2),
- * log(1 + b^2), log(1 + (r.b)^2) and (r.b)^2 / (r^2 b^2 + 1e-300) clamped
- * to [0, 1], standardised with the training set's mean and deviation.
+ * (r.b)^2 of the Kastaun setup): log q, log r^2, log b^2, log (r.b)^2 (each
+ * floored at 1e-12, 1e-24, 1e-24, 1e-48) and (r.b)^2 / (r^2 b^2 + 1e-300)
+ * clamped to [0, 1], standardised with the training set's mean and
+ * deviation.
  * Target: logit(mu); the prediction is g = 1 / (1 + exp(-z)).
  */
 #ifndef PANGU_MLP_H
@@ -102,16 +103,18 @@ int pangu_mlp_predict_wave(pangu_mlp *m, const float *feats, int n, float *out);
 /* Attaches the model to a context: its Kastaun recovery then starts from
  * the learned one-sided bracket of initial relative width delta.  NULL detaches.  The
  * model must outlive the attachment.  The recovery kernel evaluates the
- * network itself, in the wavefronts that hold a cell passing the gate
+ * network itself, in the wavefronts the gate selects
  * (pangu_set_c2p_gate).  Runs with a model attached are off the bit-for-bit
  * parity path (results agree to the solver tolerance). */
 int pangu_set_c2p_guess(pangu_ctx *ctx, pangu_mlp *m, double delta);
 
-/* Gate of the learned bracket: a cell uses it when its previous recovery
- * (the iteration count in c2p_iters, which the context must have) took at
- * least min_prev_iters false-position steps; other cells run the plain
- * search, and a wavefront with no such cell skips the network.  0 (the
- * default) uses it for every cell whose count is at hand. */
+/* Gate of the learned bracket: a wavefront (64 cells of the recovery
+ * launch) evaluates the network when one of its cells took at least
+ * min_prev_iters false-position steps in its previous recovery (the
+ * iteration count in c2p_iters, which the context must have); every cell
+ * of that wavefront then starts from the learned bracket.  Wavefronts with
+ * no such cell skip the network and run the plain search.  0 (the default)
+ * uses it for every cell whose count is at hand. */
 int pangu_set_c2p_gate(pangu_ctx *ctx, int min_prev_iters);
 
 #ifdef __cplusplus

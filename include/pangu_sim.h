@@ -116,15 +116,20 @@ const char *pangu_sim_kernel_name(pangu_sim *s, int n);
  * HIP events around the loop. */
 int pangu_sim_plain_loop(pangu_sim **sims, int n, int ncycles, int mode, double *ms);
 /* Autotuner: whether the X1 sweep shares reconstructions, then for each
- * kernel family the faster tile order, then whether to split the source
- * kernel, each candidate timed with the kernel clock over
- * `cycles` fused cycles.  Every field, dt and time is restored afterwards,
- * so tuning does not advance the run.  With use_cache != 0 a result stored
- * for the same block shapes and options in this process is reused without
- * measuring.  out_orders (PANGU_KF_COUNT ints), out_split and out_share may
- * be NULL. */
+ * kernel family the faster tile order, then for each flux sweep the
+ * workgroup shape (one lane per face, or two: pangu_set_pair_flux), then
+ * the faces per thread of the X2 and X3 sweeps (0 or 2, 3, 4, 6, 8, 12:
+ * pangu_set_flux_march), then whether to split the source kernel, each
+ * candidate timed with the kernel clock over `cycles` fused cycles.  Every
+ * field, dt and time is restored afterwards, so tuning does not advance the
+ * run.  With use_cache != 0 a result stored for the same block shapes and
+ * options in this process is reused without measuring.  out_orders
+ * (PANGU_KF_COUNT ints), out_split, out_share, out_pair (direction mask)
+ * and out_march (2 ints: X2, X3) may be NULL. */
 int pangu_sim_autotune(pangu_sim *s, int cycles, int use_cache, int *out_orders, int *out_split,
-                       int *out_share);
+                       int *out_share, int *out_pair, int *out_march);
+/* pangu_set_flux_march on the simulation's context (drops the graph). */
+int pangu_sim_set_flux_march(pangu_sim *s, int dir, int depth);
 /* pangu_set_c2p_guess on the simulation's context (drops the graph). */
 int pangu_sim_set_c2p_guess(pangu_sim *s, pangu_mlp *m, double delta);
 /* pangu_set_c2p_gate on the simulation's context (re-captures the cycle). */

@@ -472,6 +472,43 @@ static int make_range_x1s(pangu_ctx *c, Range3 &R) {
   return upload_tiles(c, tiles, &R.tblock, &R.tfirst, &R.ntiles);
 }
 
+// Marching range of the sweep along dir (2 or 3) over the face range F
+// (decode_march): per block, ceil(count along dir / step) chunks of
+// ceil(plane / 256) tiles, the plane spanned by x1 and the remaining
+// direction.
+static int make_range_march(pangu_ctx *c, Range3 &R, const Range3 &F, int dir, int step) {
+  R = F;
+  R.step = step;
+  R.order = 0;
+  const int D = dir - 1, E = (dir == 2) ? 2 : 1;
+  std::vector<int64_t> tiles(c->nblocks, 0);
+  for (int b = 0; b < c->nblocks; ++b) {
+    const HostBlock &H = c->hb[b];
+    if (H.n[D] == 1) continue;
+    int cnt[3];
+    bool empty = false;
+    for (int d = 0; d < 3; ++d) {
+      int l, h;
+      if (F.entire) {
+        l = 0;
+        h = H.n[d] - 1;
+      } else if (H.n[d] > 1) {
+        l = H.s[d] + F.rlo[d];
+        h = H.e[d] + F.rhi[d];
+      } else {
+        l = H.s[d];
+        h = H.e[d];
+      }
+      cnt[d] = h - l + 1;
+      if (cnt[d] <= 0) empty = true;
+    }
+    if (empty) continue;
+    const int64_t plane = (int64_t)cnt[0] * cnt[E];
+    tiles[b] = (int64_t)((cnt[D] + step - 1) / step) * ((plane + 255) / 256);
+  }
+  return upload_tiles(c, tiles, &R.tblock, &R.tfirst, &R.ntiles);
+}
+
 static int make_items(pangu_ctx *c, Items &I, const std::vector<int64_t> &items) {
   std::vector<int64_t> tiles(items.size());
   for (size_t b = 0; b < items.size(); ++b) tiles[b] = (items[b] + 255) / 256;
@@ -746,6 +783,22 @@ int pangu_set_pair_flux(pangu_ctx *c, int dir_mask) {
   if (!c) return PANGU_EINVAL;
   if (dir_mask < 0 || dir_mask > 7) return fail(c, PANGU_EINVAL, "pair flux: mask of directions 0..7");
   c->L.pair_flux = dir_mask;
+  return PANGU_OK;
+}
+
+int pangu_set_flux_march(pangu_ctx *c, int dir, int depth) {
+  if (!c) return PANGU_EINVAL;
+  if (dir != 2 && dir != 3) return fail(c, PANGU_EINVAL, "flux march: direction 2 or 3");
+  if (depth < 0 || depth > 64) return fail(c, PANGU_EINVAL, "flux march: depth 0..64");
+  LaunchCtx &L = c->L;
+  if (depth > 0 && (L.march_depth[dir - 2] != depth || L.march[dir - 2].ntiles == 0)) {
+    // the face ranges are built on creation; the tables of a previous depth
+    // stay with the context until it is destroyed
+    Range3 R{};
+    if (int r = make_range_march(c, R, L.face[dir - 1], dir, depth)) return fail(c, r, "flux march: tables");
+    L.march[dir - 2] = R;
+  }
+  L.march_depth[dir - 2] = depth;
   return PANGU_OK;
 }
 

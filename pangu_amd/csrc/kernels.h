@@ -66,6 +66,9 @@ struct Range3 {
   // arithmetically from these (no table loads): lo and cnt of the range on
   // each block, tiles per block plane.
   int uni, ulo[3], ucnt[3], utpp;
+  // Marching ranges of a sweep (decode_march): faces per thread along the
+  // sweep direction; 0 elsewhere.
+  int step;
 };
 
 // Work items counted per block (ghost cells, boundary slab cells), in tiles
@@ -110,6 +113,8 @@ struct LaunchCtx {
   int fuse_ct = 1;                 // fused stage: CT face fluxes formed in the update kernel
   int share_x1 = 0;                // X1 sweep reconstructs each cell once (k_flux_x1s)
   int pair_flux = 0;               // bit d-1: the sweep along d runs two lanes per face (k_flux_pair)
+  Range3 march[2];                 // marching ranges of the X2 and X3 sweeps (k_flux_march)
+  int march_depth[2] = {};         // faces per thread of the X2, X3 sweeps; 0: one face per thread
   MultiLevel ml;                   // refinement tables (ml.on = 0 on a single-level mesh)
   KernelClock *clock = nullptr;
 };

@@ -500,14 +500,15 @@ __device__ __forceinline__ int kastaun_plain_k(const KastaunIn &k, double gamma,
 }
 
 // The five dimensionless invariants the learned bracket's network reads:
-// log(1 + max(q, -0.5)), log(1 + r^2), log(1 + b^2), log(1 + (r.b)^2) and
-// (r.b)^2 / (r^2 b^2) clamped to [0, 1].
+// log q, log r^2, log b^2, log (r.b)^2 (each floored: 1e-12, 1e-24, 1e-24,
+// 1e-48) and (r.b)^2 / (r^2 b^2) clamped to [0, 1].  Logarithms resolve the
+// cold, slow and weakly magnetised states, where 1 - mu scales with q and
+// the plain values sit within rounding of 0.
 __device__ inline void c2p_features(const KastaunIn &k, float f[5]) {
-  const double q = k.q < -0.5 ? -0.5 : k.q;
-  f[0] = (float)log1p(q);
-  f[1] = (float)log1p(k.rsq);
-  f[2] = (float)log1p(k.bsq);
-  f[3] = (float)log1p(k.rbsq);
+  f[0] = (float)log(k.q > 1e-12 ? k.q : 1e-12);
+  f[1] = (float)log(k.rsq > 1e-24 ? k.rsq : 1e-24);
+  f[2] = (float)log(k.bsq > 1e-24 ? k.bsq : 1e-24);
+  f[3] = (float)log(k.rbsq > 1e-48 ? k.rbsq : 1e-48);
   double c = k.rbsq / (k.rsq * k.bsq + 1e-300);
   c = c < 0.0 ? 0.0 : (c > 1.0 ? 1.0 : c);
   f[4] = (float)c;

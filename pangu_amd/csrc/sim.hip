@@ -741,6 +741,8 @@ struct Tuning {
   int order[PANGU_KF_COUNT];
   int split;
   int share;
+  int pair;  // bit d-1: the sweep along d runs two lanes per face
+  int march[2];  // faces per thread of the X2 and X3 sweeps (0: one)
 };
 static std::map<std::string, Tuning> g_tune_cache;
 
@@ -784,12 +786,14 @@ static int apply_tuning(pangu_sim *s, const Tuning &T) {
   for (int f = 0; f < PANGU_KF_COUNT; ++f) PCHK(s, pangu_set_tile_order(s->ctx, f, T.order[f]));
   PCHK(s, pangu_set_split_source(s->ctx, T.split));
   PCHK(s, pangu_set_share_reconstruction(s->ctx, T.share));
+  PCHK(s, pangu_set_pair_flux(s->ctx, T.pair));
+  for (int d = 0; d < 2; ++d) PCHK(s, pangu_set_flux_march(s->ctx, d + 2, T.march[d]));
   drop_graph(s);
   return 0;
 }
 
 int pangu_sim_autotune(pangu_sim *s, int cycles, int use_cache, int *out_orders, int *out_split,
-                       int *out_share) {
+                       int *out_share, int *out_pair, int *out_march) {
   if (!s) return PANGU_EINVAL;
   (void)hipSetDevice(s->device);
   if (cycles < 1) cycles = 1;
@@ -833,6 +837,8 @@ int pangu_sim_autotune(pangu_sim *s, int cycles, int use_cache, int *out_orders,
     for (int f = 0; f < PANGU_KF_COUNT; ++f) best.order[f] = pangu_get_tile_order(s->ctx, f);
     best.split = 0;
     best.share = 0;
+    best.pair = 0;
+    best.march[0] = best.march[1] = 0;
     int rc = apply_tuning(s, best);
     // coordinate descent: one family at a time, candidates timed with the
     // kernel clock over `cycles` fused cycles
@@ -861,6 +867,40 @@ int pangu_sim_autotune(pangu_sim *s, int cycles, int use_cache, int *out_orders,
       if (!rc) rc = measure(alt, f, &alt_ms);
       if (!rc && alt_ms > 0 && alt_ms < base_ms) best = alt;
     }
+    // workgroup shape of each flux sweep: 256 faces of one lane each, or 128
+    // faces of two lanes each (the X1 sweep only when it does not share
+    // reconstructions)
+    for (int d = 0; d < 3 && !rc; ++d) {
+      const int f = PANGU_KF_FLUX_X1 + d;
+      if (d == 0 && best.share) continue;
+      double base_ms = 0, alt_ms = 0;
+      Tuning alt = best;
+      alt.pair = best.pair ^ (1 << d);
+      rc = measure(best, f, &base_ms);
+      if (!rc) rc = measure(alt, f, &alt_ms);
+      if (!rc && base_ms > 0 && alt_ms > 0 && alt_ms < base_ms) best = alt;
+    }
+    // faces per thread of the X2 and X3 sweeps (marching columns; chunks
+    // of the sweep's tile order)
+    for (int d = 0; d < 2 && !rc; ++d) {
+      const int f = PANGU_KF_FLUX_X2 + d;
+      static const int depths[] = {2, 3, 4, 6, 8, 12};
+      double best_ms = 0;
+      rc = measure(best, f, &best_ms);
+      Tuning win = best;
+      for (int depth : depths) {
+        if (rc) break;
+        Tuning alt = best;
+        alt.march[d] = depth;
+        double ms = 0;
+        rc = measure(alt, f, &ms);
+        if (!rc && ms > 0 && best_ms > 0 && ms < best_ms) {
+          best_ms = ms;
+          win = alt;
+        }
+      }
+      best = win;
+    }
     if (!rc) {
       double base_ms = 0, alt_ms = 0;
       Tuning alt = best;
@@ -883,6 +923,9 @@ int pangu_sim_autotune(pangu_sim *s, int cycles, int use_cache, int *out_orders,
     for (int f = 0; f < PANGU_KF_COUNT; ++f) out_orders[f] = best.order[f];
   if (out_split) *out_split = best.split;
   if (out_share) *out_share = best.share;
+  if (out_pair) *out_pair = best.pair;
+  if (out_march)
+    for (int d = 0; d < 2; ++d) out_march[d] = best.march[d];
   return 0;
 }
 
@@ -912,6 +955,12 @@ int pangu_sim_set_share_reconstruction(pangu_sim *s, int share) {
 
 int pangu_sim_set_pair_flux(pangu_sim *s, int dir_mask) {
   PCHK(s, pangu_set_pair_flux(s->ctx, dir_mask));
+  drop_graph(s);
+  return 0;
+}
+
+int pangu_sim_set_flux_march(pangu_sim *s, int dir, int depth) {
+  PCHK(s, pangu_set_flux_march(s->ctx, dir, depth));
   drop_graph(s);
   return 0;
 }

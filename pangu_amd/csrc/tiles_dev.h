@@ -122,6 +122,67 @@ __device__ __forceinline__ bool decode_tile_pair(const KCtx &C, const Range3 &R,
   return true;
 }
 
+// Marching launches of the sweep along DIR (2 or 3): a thread owns one
+// position of the plane spanned by the other two directions (x1 fastest,
+// 256 positions per tile) and R.step consecutive faces along DIR from
+// `m` on (m < mend).  Tiles per block: ceil(count along DIR / step) chunks
+// of ceil(plane / 256) tiles, ordered chunk-major (R.order 0) or
+// tile-major (1), and remapped to XCDs like decode_tile.
+template <int DIR>
+__device__ __forceinline__ bool decode_march(const KCtx &C, const Range3 &R, int &b, int &k, int &j,
+                                             int &i, int &mend) {
+  const int N = R.ntiles;
+  const int n8 = (N + 7) >> 3;
+  const int L = (int)blockIdx.x;
+  const int T = (L & 7) * n8 + (L >> 3);
+  if (T >= N) return false;
+  constexpr int D = DIR - 1, E = (DIR == 2) ? 2 : 1;  // marched and transverse dims
+  int lo[3], cnt[3], local;
+  if (R.uni) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = R.ulo[d];
+      cnt[d] = R.ucnt[d];
+    }
+  } else {
+    b = R.tblock[T];
+    block_range(C.blk[b], R, lo, cnt);
+  }
+  const int plane = cnt[0] * cnt[E];
+  const int tpp = (plane + TILE - 1) / TILE;
+  const int nch = (cnt[D] + R.step - 1) / R.step;
+  if (R.uni) {
+    const int per_block = nch * tpp;
+    b = T / per_block;
+    local = T - b * per_block;
+  } else {
+    local = T - R.tfirst[b];
+  }
+  int tile, ch;
+  if (R.order) {
+    tile = local / nch;
+    ch = local - tile * nch;
+  } else {
+    ch = local / tpp;
+    tile = local - ch * tpp;
+  }
+  const int t = tile * TILE + (int)threadIdx.x;
+  if (t >= plane) return false;
+  const int u = t / cnt[0];
+  i = lo[0] + (t - u * cnt[0]);
+  const int m0 = lo[D] + ch * R.step;
+  const int mhi = lo[D] + cnt[D];
+  mend = (m0 + R.step < mhi) ? m0 + R.step : mhi;
+  if (DIR == 2) {
+    k = lo[2] + u;
+    j = m0;
+  } else {
+    j = lo[1] + u;
+    k = m0;
+  }
+  return true;
+}
+
 // Block and item index of this thread in an item launch.
 __device__ __forceinline__ void decode_item(const Items &I, int &b, int &item) {
   b = I.tblock[blockIdx.x];

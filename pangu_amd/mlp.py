@@ -152,8 +152,8 @@ class Mlp:
 
     def attach(self, hipsim, delta=0.02, gate=0):
         """Kastaun recovery of a HipSim starts from this model's bracket in
-        the cells whose previous recovery took at least `gate` false-position
-        steps (0: every cell)."""
+        the wavefronts holding a cell whose previous recovery took at least
+        `gate` false-position steps (0: every cell)."""
         r = lib().pangu_sim_set_c2p_guess(hipsim._p, self._p, delta)
         if r != 0:
             raise RuntimeError(f"attach failed ({r})")

@@ -88,10 +88,11 @@ def lib():
         L.pangu_sim_context.argtypes = [vp, C.POINTER(vp), ip, C.POINTER(vp)]
         L.pangu_sim_record_roots.argtypes = [vp, C.c_int]
         L.pangu_sim_set_pair_flux.argtypes = [vp, C.c_int]
+        L.pangu_sim_set_flux_march.argtypes = [vp, C.c_int, C.c_int]
         L.pangu_sim_set_c2p_gate.argtypes = [vp, C.c_int]
         L.pangu_sim_nblocks.argtypes = [vp]
         L.pangu_sim_plain_loop.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_int, dp]
-        L.pangu_sim_autotune.argtypes = [vp, C.c_int, C.c_int, ip, ip, ip]
+        L.pangu_sim_autotune.argtypes = [vp, C.c_int, C.c_int, ip, ip, ip, ip, ip]
         L.pangu_sim_destroy.argtypes = [vp]
         L.pangu_sim_init.argtypes = [vp]
         L.pangu_sim_step.argtypes = [vp, C.c_int, C.c_int]
@@ -225,12 +226,14 @@ class HipSim:
         """Runs the autotuner (state restored afterwards); returns the chosen
         tile order per kernel family and the source split."""
         orders = (C.c_int * len(KERNEL_FAMILIES))()
-        split, share = C.c_int(), C.c_int()
+        split, share, pair = C.c_int(), C.c_int(), C.c_int()
+        march = (C.c_int * 2)()
         self._check(lib().pangu_sim_autotune(self._p, cycles, int(use_cache), orders, C.byref(split),
-                                             C.byref(share)), "autotune")
+                                             C.byref(share), C.byref(pair), march), "autotune")
         names = sorted(KERNEL_FAMILIES, key=KERNEL_FAMILIES.get)
         return {"order": {n: int(orders[KERNEL_FAMILIES[n]]) for n in names},
-                "split_source": split.value, "share_reconstruction": share.value}
+                "split_source": split.value, "share_reconstruction": share.value,
+                "pair_flux": pair.value, "flux_march": [march[0], march[1]]}
 
     def close(self):
         if getattr(self, "_p", None):
@@ -305,6 +308,12 @@ class HipSim:
         """Bit d-1: the sweep along d runs two lanes per face (k_flux_pair).
         Results are unchanged."""
         self._check(lib().pangu_sim_set_pair_flux(self._p, int(dir_mask)), "set_pair_flux")
+
+    def set_flux_march(self, direction, depth):
+        """The sweep along `direction` (2 or 3) runs `depth` faces per thread,
+        one reconstruction per cell (0: one face per thread).  Results are
+        unchanged."""
+        self._check(lib().pangu_sim_set_flux_march(self._p, int(direction), int(depth)), "set_flux_march")
 
     def set_fuse_ct(self, fuse):
         """Fused stage: constrained-transport face B fluxes formed inside the

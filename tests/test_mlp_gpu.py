@@ -158,8 +158,9 @@ def test_learned_bracket_on_the_2d_torus_workload():
       state), as every root of the plain run does;
     - P and U stay within the solver tolerance of the plain run after ten
       cycles;
-    - on the cells the gate sends to the model (previous recovery at least
-      GATE steps) the learned search takes fewer false-position steps."""
+    - on the cells passing the gate (previous recovery at least GATE
+      steps; their wavefronts run the model) the learned search takes fewer
+      false-position steps."""
     import oracle_py as O
     from pangu_amd import configs, mlp
     from pangu_amd.sim import HipSim

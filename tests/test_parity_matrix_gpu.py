@@ -150,6 +150,36 @@ def test_matrix_pair_flux(name):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("name", PAIR_CASES)
+def test_matrix_march_flux(name):
+    """The marching X2/X3 sweeps (several faces per thread, one
+    reconstruction per cell; depths that do and do not divide the face
+    counts, both tile orders) against the oracle and bit for bit against
+    the one-face sweeps: fields, face fluxes and wave speeds."""
+    from pangu_amd.sim import HipSim
+    cfg = MATRIX[name]
+    o = O.OracleSim(cfg)
+    o.init()
+    o.step(1)
+    runs = []
+    for depth, order in ((0, 0), (3, 0), (4, 1), (7, 0), (64, 1)):
+        h = HipSim(cfg)
+        h.init()
+        for d in (2, 3):
+            h.set_flux_march(d, depth)
+            h.set_tile_order("flux_x%d" % d, order)
+        h.step(1, "graph")
+        if depth:
+            _compare(o, h, name + " march %d" % depth)
+        runs.append((depth, [h.get("prim", 0), h.get("cons", 0), h.get("prim", 1), h.get("alfven"),
+                             h.get("flux2"), h.get("flux3")]))
+        h.close()
+    for depth, fields in runs[1:]:
+        for a, b in zip(runs[0][1], fields):
+            assert np.array_equal(a, b, equal_nan=True), depth
+
+
+@pytest.mark.gpu
 def test_fix_recovery_patterns():
     """Recovery failures planted by making D < 0 in chosen cells; the
     averaging fixer must reproduce the serial lexicographic result of
@@ -281,7 +311,7 @@ def test_launch_tuning_leaves_results_unchanged():
     cfg = _torus3d()
     runs = []
     for variant in ("default", "split", "column", "plane", "share", "ct_faces", "split_ct_faces",
-                    "pair", "pair_share"):
+                    "pair", "pair_share", "march", "march_column"):
         h = HipSim(cfg)
         h.init()
         if variant in ("split", "split_ct_faces"):
@@ -292,8 +322,11 @@ def test_launch_tuning_leaves_results_unchanged():
             h.set_share_reconstruction(1)
         if variant in ("pair", "pair_share"):
             h.set_pair_flux(7)
+        if variant in ("march", "march_column"):
+            h.set_flux_march(2, 5)
+            h.set_flux_march(3, 3)
         for fam in KERNEL_FAMILIES:
-            if variant == "column":
+            if variant in ("column", "march_column"):
                 h.set_tile_order(fam, 1)
             if variant == "plane":
                 h.set_tile_order(fam, 0)
