@@ -50,7 +50,10 @@ STATE_WORDS_PER_ZONE_STAGE = 40
 # re-tunes.
 TUNED = {"order": {"fix_cons": 1, "flux_x1": 1, "flux_x2": 0, "flux_x3": 1, "ct": 0,
                    "div_update": 1, "recovery": 1},
-         "split_source": 0, "share_reconstruction": 0, "pair_flux": 0, "flux_march": [0, 0]}
+         "split_source": 0, "share_reconstruction": 0, "pair_flux": 0, "flux_march": [0, 0],
+         "tile": {"fix_cons": 256, "flux_x1": 256, "flux_x2": 256, "flux_x3": 256, "ct": 256,
+                  "div_update": 256, "recovery": 256},
+         "persistent_recovery": 0}
 
 # Per launch under TUNED, taken from the rocprofv3 counter runs recorded in
 # PROFILE (they are not measured by this script):
@@ -352,6 +355,9 @@ def main():
         sim.set_pair_flux(TUNED["pair_flux"])
         for d, depth in zip((2, 3), TUNED["flux_march"]):
             sim.set_flux_march(d, depth)
+        for fam, cells in TUNED["tile"].items():
+            sim.set_tile_size(fam, cells)
+        sim.set_persistent_recovery(TUNED["persistent_recovery"])
     sim.step(args.warmup, args.mode)
     ms = sim.timed_cycles(args.steps, args.mode)
     wall0 = time.perf_counter()

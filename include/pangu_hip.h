@@ -228,6 +228,13 @@ enum {
 enum { PANGU_ORDER_PLANE = 0, PANGU_ORDER_COLUMN = 1 };
 int pangu_set_tile_order(pangu_ctx *ctx, int kernel_family, int order);
 int pangu_get_tile_order(const pangu_ctx *ctx, int kernel_family);
+/* Cells per tile (threads per workgroup) of a kernel family's launches:
+ * 256 (default), 128 or 64.  Smaller tiles waste fewer lanes on planes of
+ * small blocks (a 10x10 face plane fills 100 of 256 lanes).  The flux
+ * sweeps use it in their one-lane-per-face form only.  Results are
+ * identical. */
+int pangu_set_tile_size(pangu_ctx *ctx, int kernel_family, int cells);
+int pangu_get_tile_size(const pangu_ctx *ctx, int kernel_family);
 /* Fused stage only: split = 1 runs the geometric source in a kernel of its
  * own after the flux divergence + update kernel (fewer registers per
  * kernel, one more pass over U); 0 fuses all three.  Results are identical. */
@@ -249,6 +256,12 @@ int pangu_set_pair_flux(pangu_ctx *ctx, int dir_mask);
  * direction.  Results are identical.  Tile order (pangu_set_tile_order of
  * the sweep's family) applies to its chunks of faces. */
 int pangu_set_flux_march(pangu_ctx *ctx, int dir, int depth);
+/* on = 1: recovery (and heating) runs as a persistent kernel whose
+ * workgroups, no more than the GPU holds at once, pull tiles from a queue
+ * across every block of every mesh until none is left (the learned bracket
+ * keeps its own kernel).  0 (default): one workgroup per tile.  Results
+ * are identical. */
+int pangu_set_persistent_recovery(pangu_ctx *ctx, int on);
 /* fuse = 1 (default): the fused stage runs no constrained-transport kernel;
  * the flux-divergence + update kernel forms the edge EMFs from the Riemann B
  * fluxes and the face B fluxes from them (the same arithmetic as

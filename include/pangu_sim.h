@@ -119,15 +119,25 @@ int pangu_sim_plain_loop(pangu_sim **sims, int n, int ncycles, int mode, double 
  * kernel family the faster tile order, then for each flux sweep the
  * workgroup shape (one lane per face, or two: pangu_set_pair_flux), then
  * the faces per thread of the X2 and X3 sweeps (0 or 2, 3, 4, 6, 8, 12:
- * pangu_set_flux_march), then whether to split the source kernel, each
- * candidate timed with the kernel clock over `cycles` fused cycles.  Every
- * field, dt and time is restored afterwards, so tuning does not advance the
- * run.  With use_cache != 0 a result stored for the same block shapes and
- * options in this process is reused without measuring.  out_orders
- * (PANGU_KF_COUNT ints), out_split, out_share, out_pair (direction mask)
- * and out_march (2 ints: X2, X3) may be NULL. */
+ * pangu_set_flux_march), then the cells per tile of each family (256, 128
+ * or 64: pangu_set_tile_size), then whether recovery runs as a persistent
+ * work-queue kernel (pangu_set_persistent_recovery), then whether to split
+ * the source kernel,
+ * each candidate timed with the kernel clock over `cycles` fused cycles.
+ * Every field, dt and time is restored afterwards, so tuning does not
+ * advance the run.  Results are cached per block-shape class: with
+ * use_cache != 0 a result stored for the same block shapes and options in
+ * this process is reused without measuring.  out_orders (PANGU_KF_COUNT
+ * ints), out_split, out_share, out_pair (direction mask), out_march (2
+ * ints: X2, X3), out_tiles (PANGU_KF_COUNT ints) and out_persist may be
+ * NULL. */
 int pangu_sim_autotune(pangu_sim *s, int cycles, int use_cache, int *out_orders, int *out_split,
-                       int *out_share, int *out_pair, int *out_march);
+                       int *out_share, int *out_pair, int *out_march, int *out_tiles,
+                       int *out_persist);
+/* pangu_set_persistent_recovery on the simulation's context (drops the graph). */
+int pangu_sim_set_persistent_recovery(pangu_sim *s, int on);
+/* pangu_set_tile_size on the simulation's context (drops the graph). */
+int pangu_sim_set_tile_size(pangu_sim *s, int kernel_family, int cells);
 /* pangu_set_flux_march on the simulation's context (drops the graph). */
 int pangu_sim_set_flux_march(pangu_sim *s, int dir, int depth);
 /* pangu_set_c2p_guess on the simulation's context (drops the graph). */

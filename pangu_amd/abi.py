@@ -94,6 +94,9 @@ PROTOTYPES = {
     "pangu_set_share_reconstruction": (C.c_int, [_vp, C.c_int]),
     "pangu_set_pair_flux": (C.c_int, [_vp, C.c_int]),
     "pangu_set_flux_march": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "pangu_set_tile_size": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "pangu_set_persistent_recovery": (C.c_int, [_vp, C.c_int]),
+    "pangu_get_tile_size": (C.c_int, [_vp, C.c_int]),
     "pangu_set_fuse_ct": (C.c_int, [_vp, C.c_int]),
     "pangu_diagnostics": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), _dp, _vp]),
     "pangu_diagnostics_noalloc": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), _vp, _vp]),

@@ -60,6 +60,7 @@ struct pangu_ctx {
   // scratch
   double *dred = nullptr;
   int *dcluster = nullptr;
+  unsigned *dqueue = nullptr;  // work queue of the persistent recovery kernel
   double *dest = nullptr;
   // diagnostics: per-tile partial sums, per-mesh tile ranges, shell indices
   double *ddiag_part = nullptr;
@@ -385,7 +386,8 @@ static int upload_tiles(pangu_ctx *c, const std::vector<int64_t> &tiles_per_bloc
 }
 
 // Host replica of the device block_range: tiles of one block for a range.
-static int64_t range_tiles(const HostBlock &H, const int rlo[3], const int rhi[3], int entire) {
+static int64_t range_tiles(const HostBlock &H, const int rlo[3], const int rhi[3], int entire,
+                           int tile) {
   int cnt[3];
   for (int d = 0; d < 3; ++d) {
     int l, h;
@@ -403,11 +405,11 @@ static int64_t range_tiles(const HostBlock &H, const int rlo[3], const int rhi[3
     if (cnt[d] <= 0) return 0;
   }
   const int64_t plane = (int64_t)cnt[0] * cnt[1];
-  return (int64_t)cnt[2] * ((plane + 255) / 256);
+  return (int64_t)cnt[2] * ((plane + tile - 1) / tile);
 }
 
 static int make_range(pangu_ctx *c, Range3 &R, int l0, int l1, int l2, int h0, int h1, int h2,
-                      int entire, int need_dim) {
+                      int entire, int need_dim, int tile = 256) {
   R.rlo[0] = l0;
   R.rlo[1] = l1;
   R.rlo[2] = l2;
@@ -416,9 +418,13 @@ static int make_range(pangu_ctx *c, Range3 &R, int l0, int l1, int l2, int h0, i
   R.rhi[2] = h2;
   R.entire = entire;
   R.order = 0;
+  R.step = 0;
+  R.tile = tile;
   std::vector<int64_t> tiles(c->nblocks);
   for (int b = 0; b < c->nblocks; ++b)
-    tiles[b] = (need_dim >= 0 && c->hb[b].n[need_dim] == 1) ? 0 : range_tiles(c->hb[b], R.rlo, R.rhi, entire);
+    tiles[b] = (need_dim >= 0 && c->hb[b].n[need_dim] == 1)
+                   ? 0
+                   : range_tiles(c->hb[b], R.rlo, R.rhi, entire, tile);
   // one block shape: arithmetic decoding
   R.uni = 0;
   bool same = true;
@@ -443,7 +449,7 @@ static int make_range(pangu_ctx *c, Range3 &R, int l0, int l1, int l2, int h0, i
       R.ulo[d] = l;
       R.ucnt[d] = h - l + 1;
     }
-    R.utpp = (int)(((int64_t)R.ucnt[0] * R.ucnt[1] + 255) / 256);
+    R.utpp = (int)(((int64_t)R.ucnt[0] * R.ucnt[1] + tile - 1) / tile);
     R.uni = 1;
   }
   return upload_tiles(c, tiles, &R.tblock, &R.tfirst, &R.ntiles);
@@ -458,6 +464,8 @@ static int make_range_x1s(pangu_ctx *c, Range3 &R) {
   R.entire = 0;
   R.order = 0;
   R.uni = 0;
+  R.step = 0;
+  R.tile = 256;
   std::vector<int64_t> tiles(c->nblocks, 0);
   for (int b = 0; b < c->nblocks; ++b) {
     const HostBlock &H = c->hb[b];
@@ -522,17 +530,21 @@ static int make_items(pangu_ctx *c, Items &I, const std::vector<int64_t> &items)
 // (hll.cc:295-296).  CT covers s..e+1 on active dims.
 static int build_tables(pangu_ctx *c) {
   LaunchCtx &L = c->L;
-  int r = make_range(c, L.interior, 0, 0, 0, 0, 0, 0, 0, -1);
-  if (!r) r = make_range(c, L.entire, 0, 0, 0, 0, 0, 0, 1, -1);
-  if (!r) r = make_range(c, L.face[0], 0, -1, -1, 1, 1, 1, 0, 0);
-  if (!r) r = make_range(c, L.face[1], -1, 0, -1, 1, 1, 1, 0, 1);
-  if (!r) {
-    if (c->cfg.riemann == PANGU_RIEMANN_HLL)
-      r = make_range(c, L.face[2], 0, 0, 0, 0, 0, 1, 0, 2);
-    else
-      r = make_range(c, L.face[2], -1, -1, 0, 1, 1, 1, 0, 2);
+  // interior, entire, the three face ranges and CT; each in tiles of 256
+  // cells and again of 128 and 64
+  const bool hll = c->cfg.riemann == PANGU_RIEMANN_HLL;
+  const int spec[6][8] = {{0, 0, 0, 0, 0, 0, 0, -1},  {0, 0, 0, 0, 0, 0, 1, -1},
+                          {0, -1, -1, 1, 1, 1, 0, 0}, {-1, 0, -1, 1, 1, 1, 0, 1},
+                          {hll ? 0 : -1, hll ? 0 : -1, 0, hll ? 0 : 1, hll ? 0 : 1, 1, 0, 2},
+                          {0, 0, 0, 1, 1, 1, 0, -1}};
+  Range3 *const base[6] = {&L.interior, &L.entire, &L.face[0], &L.face[1], &L.face[2], &L.ct};
+  int r = 0;
+  for (int id = 0; id < 6 && !r; ++id) {
+    const int *q = spec[id];
+    r = make_range(c, *base[id], q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], 256);
+    for (int z = 0; z < 2 && !r; ++z)
+      r = make_range(c, L.small[id][z], q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], z ? 64 : 128);
   }
-  if (!r) r = make_range(c, L.ct, 0, 0, 0, 1, 1, 1, 0, -1);
   if (!r) r = make_range_x1s(c, L.x1cells);
   std::vector<int64_t> ghosts(c->nblocks), planes(c->nblocks);
   for (int b = 0; b < c->nblocks; ++b) {
@@ -643,6 +655,8 @@ int pangu_create_batch(const pangu_config *cfgs, int nmesh, const pangu_block *b
                      (size_t)c->nplanes * 3);
   if (!r) r = upload(c, &c->dcluster, std::vector<int>((size_t)c->nplanes, 0).data(),
                      (size_t)c->nplanes);
+  if (!r) r = upload(c, &c->dqueue, std::vector<unsigned>(2, 0u).data(), 2);
+  if (!r) c->L.queue = c->dqueue;
   if (!r) r = upload(c, &c->dest, std::vector<double>(nmesh, 0.0).data(), nmesh);
   if (!r) {
     // interior tiles are grouped by block and therefore by mesh
@@ -651,7 +665,7 @@ int pangu_create_batch(const pangu_config *cfgs, int nmesh, const pangu_block *b
     int64_t tot = 0;
     for (int m = 0; m < nmesh; ++m) {
       c->diag_mtile[m] = (int)tot;
-      for (int b = c->mfirst[m]; b < c->mfirst[m + 1]; ++b) tot += range_tiles(c->hb[b], zero, zero, 0);
+      for (int b = c->mfirst[m]; b < c->mfirst[m + 1]; ++b) tot += range_tiles(c->hb[b], zero, zero, 0, 256);
     }
     c->diag_mtile[nmesh] = (int)tot;
     if (tot != c->L.interior.ntiles) r = fail(c, PANGU_EINVAL, "diagnostics: tile count mismatch");
@@ -731,7 +745,7 @@ int pangu_destroy(pangu_ctx *c) {
   }
   void *ptrs[] = {c->dblk,    c->dgeo,  c->dnbr,     c->dalfven, c->demf,  c->dflag,
                   c->dflag_rec, c->dmask, c->dstatus, c->diters,  c->droot, c->dred,  c->dcluster,
-                  c->dest,    c->ddiag_part, c->ddiag_mtile,
+                  c->dqueue,  c->dest,    c->ddiag_part, c->ddiag_mtile,
                   c->ddiag_shell, c->dxfill, c->dxprol, c->dxflux};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -783,6 +797,25 @@ int pangu_set_pair_flux(pangu_ctx *c, int dir_mask) {
   if (!c) return PANGU_EINVAL;
   if (dir_mask < 0 || dir_mask > 7) return fail(c, PANGU_EINVAL, "pair flux: mask of directions 0..7");
   c->L.pair_flux = dir_mask;
+  return PANGU_OK;
+}
+
+int pangu_set_tile_size(pangu_ctx *c, int fam, int cells) {
+  if (!c || fam < 0 || fam >= PANGU_KF_COUNT) return PANGU_EINVAL;
+  if (cells != 64 && cells != 128 && cells != 256)
+    return fail(c, PANGU_EINVAL, "tile size: 64, 128 or 256 cells");
+  c->L.tile[fam] = cells == 256 ? 0 : cells;
+  return PANGU_OK;
+}
+
+int pangu_get_tile_size(const pangu_ctx *c, int fam) {
+  if (!c || fam < 0 || fam >= PANGU_KF_COUNT) return PANGU_EINVAL;
+  return c->L.tile[fam] ? c->L.tile[fam] : 256;
+}
+
+int pangu_set_persistent_recovery(pangu_ctx *c, int on) {
+  if (!c) return PANGU_EINVAL;
+  c->L.persist_recovery = on != 0;
   return PANGU_OK;
 }
 

@@ -69,6 +69,8 @@ struct Range3 {
   // Marching ranges of a sweep (decode_march): faces per thread along the
   // sweep direction; 0 elsewhere.
   int step;
+  // Cells per tile = threads per workgroup of the launch (256, 128 or 64).
+  int tile;
 };
 
 // Work items counted per block (ghost cells, boundary slab cells), in tiles
@@ -114,6 +116,11 @@ struct LaunchCtx {
   int share_x1 = 0;                // X1 sweep reconstructs each cell once (k_flux_x1s)
   int pair_flux = 0;               // bit d-1: the sweep along d runs two lanes per face (k_flux_pair)
   Range3 march[2];                 // marching ranges of the X2 and X3 sweeps (k_flux_march)
+  // interior, entire, face[0..2] and ct again in tiles of 128 and 64 cells
+  Range3 small[6][2];
+  int tile[PANGU_KF_COUNT] = {};   // cells per tile of each kernel family (0: 256)
+  int persist_recovery = 0;        // recovery as a persistent work-queue kernel (k_recover_queue)
+  unsigned *queue = nullptr;       // its queue: next tile, workgroups done (zero between launches)
   int march_depth[2] = {};         // faces per thread of the X2, X3 sweeps; 0: one face per thread
   MultiLevel ml;                   // refinement tables (ml.on = 0 on a single-level mesh)
   KernelClock *clock = nullptr;

@@ -851,9 +851,8 @@ __device__ __forceinline__ void heat_cell(const KCtx &C, const KFields &F, const
 // (electron_heating.cc:66-109) on the recovered cell.  REC: 0 chainer,
 // 1 Kastaun, 2 1Dvsq alone, -1 none (heating only).
 template <int REC, bool HB, bool HK, bool HEAT>
-__global__ __launch_bounds__(256) void k_recover(KCtx C, KFields F, SharedTabs S, Range3 R) {
-  int b, k, j, i;
-  if (!decode_tile(C, R, b, k, j, i)) return;
+__device__ __forceinline__ void recover_cell(const KCtx &C, const KFields &F, const SharedTabs &S, int b,
+                                             int k, int j, int i) {
   const DevBlock &B = C.blk[b];
   const int64_t id = ((int64_t)k * B.n[1] + j) * B.n[0] + i;
   constexpr int NC = ncomp<HB, HK>();
@@ -957,6 +956,38 @@ __global__ __launch_bounds__(256) void k_recover(KCtx C, KFields F, SharedTabs S
     }
   }
   if (HEAT && HK) heat_cell<HB, HK>(C, F, G, t, b, id, pp);
+}
+
+template <int REC, bool HB, bool HK, bool HEAT>
+__global__ __launch_bounds__(256) void k_recover(KCtx C, KFields F, SharedTabs S, Range3 R) {
+  int b, k, j, i;
+  if (!decode_tile(C, R, b, k, j, i)) return;
+  recover_cell<REC, HB, HK, HEAT>(C, F, S, b, k, j, i);
+}
+
+// Persistent form of k_recover: a grid no larger than what the GPU holds
+// at once pulls tiles from a queue (queue[0]: next tile, queue[1]:
+// workgroups done) until every tile is taken; the last workgroup out resets
+// the queue for the next launch on the stream, so a captured graph replays
+// it without a memset.  Tiles carry no dependencies, so no workgroup ever
+// waits for another.  Same cells, same arithmetic: identical results.
+template <int REC, bool HB, bool HK, bool HEAT>
+__global__ __launch_bounds__(256) void k_recover_queue(KCtx C, KFields F, SharedTabs S, Range3 R,
+                                                       unsigned *queue) {
+  __shared__ int tile_sh;
+  for (;;) {
+    if (threadIdx.x == 0) tile_sh = (int)atomicAdd(&queue[0], 1u);
+    __syncthreads();
+    const int T = tile_sh;
+    __syncthreads();  // every thread has read the tile before the next pull
+    if (T >= R.ntiles) break;
+    int b, k, j, i;
+    if (decode_tile_at(C, R, T, b, k, j, i)) recover_cell<REC, HB, HK, HEAT>(C, F, S, b, k, j, i);
+  }
+  if (threadIdx.x == 0 && atomicAdd(&queue[1], 1u) == gridDim.x - 1) {
+    atomicExch(&queue[0], 0u);
+    atomicExch(&queue[1], 0u);
+  }
 }
 
 // Kastaun recovery with the learned starting bracket (and, when HEAT, the
@@ -1401,6 +1432,28 @@ __global__ __launch_bounds__(256) void k_dt_final(KCtx C, const double *red, dou
 static dim3 tile_grid(const Range3 &R) { return dim3((unsigned)(((R.ntiles + 7) / 8) * 8)); }
 static const dim3 kTile(TILE, 1, 1);
 
+// Grid of a persistent kernel: the workgroups the GPU holds at once (at
+// most ntiles), so every workgroup runs from the start.
+template <typename K>
+static int resident_groups(K kernel, int threads, int ntiles) {
+  int dev = 0, per_cu = 0, ncu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 1;
+  const int64_t n = (int64_t)(per_cu > 0 ? per_cu : 1) * (ncu > 0 ? ncu : 1);
+  return (int)(n < ntiles ? n : ntiles);
+}
+
+// Range of a launch of kernel family fam over base range id (0 interior,
+// 1 entire, 2..4 the faces, 5 CT) in that family's tile size and order.
+static Range3 family_range(const LaunchCtx &L, int id, const Range3 &base, int fam) {
+  const int t = L.tile[fam];
+  Range3 R = (t == 128) ? L.small[id][0] : (t == 64) ? L.small[id][1] : base;
+  R.order = L.order[fam];
+  return R;
+}
+
 #define PG_CHECK_LAUNCH() (hipGetLastError() == hipSuccess ? 0 : -2)
 #define TICK(name) \
   if (L.clock) (void)L.clock->mark(name, s)
@@ -1425,12 +1478,12 @@ KernelClock::~KernelClock() {
 
 int launch_fix_cons(const LaunchCtx &L, const KFields &F, uint8_t *const *mask, bool fix, bool cons,
                     hipStream_t s) {
-  Range3 R = fix ? L.entire : L.interior;
-  R.order = L.order[PANGU_KF_FIX_CONS];
+  const Range3 R = fix ? family_range(L, 1, L.entire, PANGU_KF_FIX_CONS)
+                       : family_range(L, 0, L.interior, PANGU_KF_FIX_CONS);
   if (R.ntiles == 0) return 0;
-  const dim3 g = tile_grid(R);
+  const dim3 g = tile_grid(R), wg(R.tile);
   const bool hb = L.k.iBX >= 0, hk = L.k.iKEL >= 0;
-#define FC(FX, CS, HB_, HK_) hipLaunchKernelGGL((k_fix_cons<FX, CS, HB_, HK_>), g, kTile, 0, s, L.k, F, mask, R)
+#define FC(FX, CS, HB_, HK_) hipLaunchKernelGGL((k_fix_cons<FX, CS, HB_, HK_>), g, wg, 0, s, L.k, F, mask, R)
 #define FC2(FX, CS)                      \
   if (hb && hk) FC(FX, CS, true, true);  \
   else if (hb) FC(FX, CS, true, false);  \
@@ -1461,8 +1514,11 @@ static void flux_dir(const LaunchCtx &L, const KFields &F, double *const *alfven
     R.order = L.order[PANGU_KF_FLUX_X1 + DIR - 1];
   }
   const bool pair = !shared && !march && ((L.pair_flux >> (DIR - 1)) & 1);
+  // the one-lane sweep in the family's tile size (the other forms decode
+  // 256-cell tiles)
+  if (!shared && !march && !pair) R = family_range(L, 1 + DIR, L.face[DIR - 1], PANGU_KF_FLUX_X1 + DIR - 1);
   if (R.ntiles == 0) return;
-  const dim3 g = tile_grid(R);
+  const dim3 g = tile_grid(R), wg(shared || march || pair ? TILE : R.tile);
   const dim3 g2((unsigned)(((2 * R.ntiles + 7) / 8) * 8));
   const bool hb = L.k.iBX >= 0, hk = L.k.iKEL >= 0;
   const int lim = L.limiter, rs = L.riemann;
@@ -1475,7 +1531,7 @@ static void flux_dir(const LaunchCtx &L, const KFields &F, double *const *alfven
   else if (pair)                                                                                  \
     hipLaunchKernelGGL((k_flux_pair<DIR, LIM, RS, HB_, HK_>), g2, kTile, 0, s, L.k, F, alfven, R); \
   else                                                                                            \
-    hipLaunchKernelGGL((k_flux<DIR, LIM, RS, HB_, HK_>), g, kTile, 0, s, L.k, F, alfven, R)
+    hipLaunchKernelGGL((k_flux<DIR, LIM, RS, HB_, HK_>), g, wg, 0, s, L.k, F, alfven, R)
 #define FL2(LIM, RS)                    \
   if (hb && hk) FL(LIM, RS, true, true); \
   else if (hb) FL(LIM, RS, true, false); \
@@ -1503,13 +1559,12 @@ int launch_fluxes(const LaunchCtx &L, const KFields &F, double *const *alfven, h
 }
 
 int launch_ct(const LaunchCtx &L, const KFields &F, double *const *emf, hipStream_t s) {
-  Range3 R = L.ct;
-  R.order = L.order[PANGU_KF_CT];
+  const Range3 R = family_range(L, 5, L.ct, PANGU_KF_CT);
   if (R.ntiles == 0) return 0;
-  const dim3 g = tile_grid(R);
-  hipLaunchKernelGGL(k_emf, g, kTile, 0, s, L.k, F, emf, R);
+  const dim3 g = tile_grid(R), wg(R.tile);
+  hipLaunchKernelGGL(k_emf, g, wg, 0, s, L.k, F, emf, R);
   TICK("ct_emf");
-  hipLaunchKernelGGL(k_ct_faces, g, kTile, 0, s, L.k, F, emf, R);
+  hipLaunchKernelGGL(k_ct_faces, g, wg, 0, s, L.k, F, emf, R);
   TICK("ct_faces");
   return PG_CHECK_LAUNCH();
 }
@@ -1517,16 +1572,15 @@ int launch_ct(const LaunchCtx &L, const KFields &F, double *const *emf, hipStrea
 int launch_div_update(const LaunchCtx &L, int mode, const KFields &F0, const KFields &Finit,
                       const KFields &F1, double *const *dudt, bool ct_fused,
                       const double *dt_dev, double beta, double bdt_host, hipStream_t s) {
-  Range3 R = L.interior;
-  R.order = L.order[PANGU_KF_DIV_UPDATE];
+  const Range3 R = family_range(L, 0, L.interior, PANGU_KF_DIV_UPDATE);
   if (R.ntiles == 0) return 0;
-  const dim3 g = tile_grid(R);
+  const dim3 g = tile_grid(R), wg(R.tile);
   const bool hb = L.k.iBX >= 0, hk = L.k.iKEL >= 0;
   const bool src = L.metric != PANGU_METRIC_MINKOWSKI;
   // CT fused into the divergence (modes 4 and 5 with a field)
   const bool ctf = ct_fused && hb && (mode == 4 || mode == 5);
 #define DU(M, SR, HB_, HK_, CT_)                                                               \
-  hipLaunchKernelGGL((k_div_update<M, SR, HB_, HK_, CT_>), g, kTile, 0, s, L.k, F0, Finit, F1, \
+  hipLaunchKernelGGL((k_div_update<M, SR, HB_, HK_, CT_>), g, wg, 0, s, L.k, F0, Finit, F1, \
                      dudt, dt_dev, beta, bdt_host, R)
 #define DU2(M, SR)                                      \
   if (hb && hk) {                                       \
@@ -1566,16 +1620,20 @@ int launch_div_update(const LaunchCtx &L, int mode, const KFields &F0, const KFi
 
 int launch_recover(const LaunchCtx &L, const KFields &F, const SharedTabs &S, int rec, bool heat,
                    const MlpFrag *W, hipStream_t s) {
-  Range3 R = L.interior;
-  R.order = L.order[PANGU_KF_RECOVERY];
+  const Range3 R = family_range(L, 0, L.interior, PANGU_KF_RECOVERY);
   if (R.ntiles == 0) return 0;
-  const dim3 g = tile_grid(R);
+  const dim3 g = tile_grid(R), wg(R.tile);
   const bool hb = L.k.iBX >= 0, hk = L.k.iKEL >= 0;
+  const bool persist = L.persist_recovery && L.queue;
 #define RC(R_, HB_, HK_, HE_)                                                                        \
   if (R_ == 1 && W)                                                                                   \
-    hipLaunchKernelGGL((k_recover_learned<HB_, HK_, HE_>), g, kTile, 0, s, L.k, F, S, *W, R);         \
+    hipLaunchKernelGGL((k_recover_learned<HB_, HK_, HE_>), g, wg, 0, s, L.k, F, S, *W, R);            \
+  else if (persist)                                                                                   \
+    hipLaunchKernelGGL((k_recover_queue<R_, HB_, HK_, HE_>),                                          \
+                       dim3(resident_groups(k_recover_queue<R_, HB_, HK_, HE_>, R.tile, R.ntiles)), wg, \
+                       0, s, L.k, F, S, R, L.queue);                                                  \
   else                                                                                                \
-    hipLaunchKernelGGL((k_recover<R_, HB_, HK_, HE_>), g, kTile, 0, s, L.k, F, S, R)
+    hipLaunchKernelGGL((k_recover<R_, HB_, HK_, HE_>), g, wg, 0, s, L.k, F, S, R)
 #define RC2(R_, HE_)                     \
   if (hb && hk) RC(R_, true, true, HE_);  \
   else if (hb) RC(R_, true, false, HE_);  \

@@ -743,6 +743,8 @@ struct Tuning {
   int share;
   int pair;  // bit d-1: the sweep along d runs two lanes per face
   int march[2];  // faces per thread of the X2 and X3 sweeps (0: one)
+  int tile[PANGU_KF_COUNT];  // cells per tile of each family
+  int persist;               // recovery as a persistent work-queue kernel
 };
 static std::map<std::string, Tuning> g_tune_cache;
 
@@ -788,12 +790,15 @@ static int apply_tuning(pangu_sim *s, const Tuning &T) {
   PCHK(s, pangu_set_share_reconstruction(s->ctx, T.share));
   PCHK(s, pangu_set_pair_flux(s->ctx, T.pair));
   for (int d = 0; d < 2; ++d) PCHK(s, pangu_set_flux_march(s->ctx, d + 2, T.march[d]));
+  for (int f = 0; f < PANGU_KF_COUNT; ++f) PCHK(s, pangu_set_tile_size(s->ctx, f, T.tile[f]));
+  PCHK(s, pangu_set_persistent_recovery(s->ctx, T.persist));
   drop_graph(s);
   return 0;
 }
 
 int pangu_sim_autotune(pangu_sim *s, int cycles, int use_cache, int *out_orders, int *out_split,
-                       int *out_share, int *out_pair, int *out_march) {
+                       int *out_share, int *out_pair, int *out_march, int *out_tiles,
+                       int *out_persist) {
   if (!s) return PANGU_EINVAL;
   (void)hipSetDevice(s->device);
   if (cycles < 1) cycles = 1;
@@ -839,6 +844,8 @@ int pangu_sim_autotune(pangu_sim *s, int cycles, int use_cache, int *out_orders,
     best.share = 0;
     best.pair = 0;
     best.march[0] = best.march[1] = 0;
+    for (int f = 0; f < PANGU_KF_COUNT; ++f) best.tile[f] = 256;
+    best.persist = 0;
     int rc = apply_tuning(s, best);
     // coordinate descent: one family at a time, candidates timed with the
     // kernel clock over `cycles` fused cycles
@@ -901,6 +908,38 @@ int pangu_sim_autotune(pangu_sim *s, int cycles, int use_cache, int *out_orders,
       }
       best = win;
     }
+    // cells per tile of each family (the flux sweeps in their one-lane form)
+    for (int f = 0; f < PANGU_KF_COUNT && !rc; ++f) {
+      if (f >= PANGU_KF_FLUX_X1 && f <= PANGU_KF_FLUX_X3) {
+        const int d = f - PANGU_KF_FLUX_X1;
+        if ((d == 0 && best.share) || ((best.pair >> d) & 1) || (d > 0 && best.march[d - 1])) continue;
+      }
+      double best_ms = 0;
+      rc = measure(best, f, &best_ms);
+      Tuning win = best;
+      for (int cells : {128, 64}) {
+        if (rc) break;
+        Tuning alt = best;
+        alt.tile[f] = cells;
+        double ms = 0;
+        rc = measure(alt, f, &ms);
+        if (!rc && ms > 0 && best_ms > 0 && ms < best_ms) {
+          best_ms = ms;
+          win = alt;
+        }
+      }
+      best = win;
+    }
+    // recovery: one workgroup per tile, or persistent workgroups pulling
+    // tiles from a queue
+    if (!rc) {
+      double base_ms = 0, alt_ms = 0;
+      Tuning alt = best;
+      alt.persist = 1;
+      rc = measure(best, PANGU_KF_RECOVERY, &base_ms);
+      if (!rc) rc = measure(alt, PANGU_KF_RECOVERY, &alt_ms);
+      if (!rc && base_ms > 0 && alt_ms > 0 && alt_ms < base_ms) best = alt;
+    }
     if (!rc) {
       double base_ms = 0, alt_ms = 0;
       Tuning alt = best;
@@ -926,6 +965,9 @@ int pangu_sim_autotune(pangu_sim *s, int cycles, int use_cache, int *out_orders,
   if (out_pair) *out_pair = best.pair;
   if (out_march)
     for (int d = 0; d < 2; ++d) out_march[d] = best.march[d];
+  if (out_tiles)
+    for (int f = 0; f < PANGU_KF_COUNT; ++f) out_tiles[f] = best.tile[f];
+  if (out_persist) *out_persist = best.persist;
   return 0;
 }
 
@@ -955,6 +997,18 @@ int pangu_sim_set_share_reconstruction(pangu_sim *s, int share) {
 
 int pangu_sim_set_pair_flux(pangu_sim *s, int dir_mask) {
   PCHK(s, pangu_set_pair_flux(s->ctx, dir_mask));
+  drop_graph(s);
+  return 0;
+}
+
+int pangu_sim_set_persistent_recovery(pangu_sim *s, int on) {
+  PCHK(s, pangu_set_persistent_recovery(s->ctx, on));
+  drop_graph(s);
+  return 0;
+}
+
+int pangu_sim_set_tile_size(pangu_sim *s, int kernel_family, int cells) {
+  PCHK(s, pangu_set_tile_size(s->ctx, kernel_family, cells));
   drop_graph(s);
   return 0;
 }

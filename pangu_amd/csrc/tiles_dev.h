@@ -32,7 +32,11 @@ __device__ __forceinline__ void block_range(const DevBlock &B, const Range3 &R, 
 
 // Decodes (block, k, j, i) of this thread; false for padding threads.  The
 // workgroup index L is remapped to tile T = (L mod 8) * ceil(N/8) + L / 8,
-// so each XCD runs a contiguous run of tiles.
+// so each XCD runs a contiguous run of tiles.  Tiles hold R.tile cells of a
+// plane (the workgroup size of the launch).
+__device__ __forceinline__ bool decode_tile_at(const KCtx &C, const Range3 &R, int T, int &b, int &k,
+                                               int &j, int &i);
+
 __device__ __forceinline__ bool decode_tile(const KCtx &C, const Range3 &R, int &b, int &k, int &j,
                                             int &i) {
   const int N = R.ntiles;
@@ -40,6 +44,13 @@ __device__ __forceinline__ bool decode_tile(const KCtx &C, const Range3 &R, int 
   const int L = (int)blockIdx.x;
   const int T = (L & 7) * n8 + (L >> 3);
   if (T >= N) return false;
+  return decode_tile_at(C, R, T, b, k, j, i);
+}
+
+// (block, k, j, i) of this thread in tile T (0 <= T < R.ntiles).
+__device__ __forceinline__ bool decode_tile_at(const KCtx &C, const Range3 &R, int T, int &b, int &k,
+                                               int &j, int &i) {
+  const int TL = R.tile;  // cells per tile, the workgroup size
   int lo[3], cnt[3], local, tpp;
   if (R.uni) {
 #pragma unroll
@@ -55,7 +66,7 @@ __device__ __forceinline__ bool decode_tile(const KCtx &C, const Range3 &R, int 
     b = R.tblock[T];
     local = T - R.tfirst[b];
     block_range(C.blk[b], R, lo, cnt);
-    tpp = (cnt[0] * cnt[1] + TILE - 1) / TILE;
+    tpp = (cnt[0] * cnt[1] + TL - 1) / TL;
   }
   const int plane = cnt[0] * cnt[1];
   int tile, kk;
@@ -66,7 +77,7 @@ __device__ __forceinline__ bool decode_tile(const KCtx &C, const Range3 &R, int 
     kk = local / tpp;
     tile = local - kk * tpp;
   }
-  const int t = tile * TILE + (int)threadIdx.x;
+  const int t = tile * TL + (int)threadIdx.x;
   if (t >= plane) return false;
   k = lo[2] + kk;
   const int jj = t / cnt[0];

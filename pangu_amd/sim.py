@@ -92,7 +92,9 @@ def lib():
         L.pangu_sim_set_c2p_gate.argtypes = [vp, C.c_int]
         L.pangu_sim_nblocks.argtypes = [vp]
         L.pangu_sim_plain_loop.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_int, dp]
-        L.pangu_sim_autotune.argtypes = [vp, C.c_int, C.c_int, ip, ip, ip, ip, ip]
+        L.pangu_sim_autotune.argtypes = [vp, C.c_int, C.c_int, ip, ip, ip, ip, ip, ip, ip]
+        L.pangu_sim_set_persistent_recovery.argtypes = [vp, C.c_int]
+        L.pangu_sim_set_tile_size.argtypes = [vp, C.c_int, C.c_int]
         L.pangu_sim_destroy.argtypes = [vp]
         L.pangu_sim_init.argtypes = [vp]
         L.pangu_sim_step.argtypes = [vp, C.c_int, C.c_int]
@@ -228,12 +230,17 @@ class HipSim:
         orders = (C.c_int * len(KERNEL_FAMILIES))()
         split, share, pair = C.c_int(), C.c_int(), C.c_int()
         march = (C.c_int * 2)()
+        tiles = (C.c_int * len(KERNEL_FAMILIES))()
+        persist = C.c_int()
         self._check(lib().pangu_sim_autotune(self._p, cycles, int(use_cache), orders, C.byref(split),
-                                             C.byref(share), C.byref(pair), march), "autotune")
+                                             C.byref(share), C.byref(pair), march, tiles,
+                                             C.byref(persist)), "autotune")
         names = sorted(KERNEL_FAMILIES, key=KERNEL_FAMILIES.get)
         return {"order": {n: int(orders[KERNEL_FAMILIES[n]]) for n in names},
                 "split_source": split.value, "share_reconstruction": share.value,
-                "pair_flux": pair.value, "flux_march": [march[0], march[1]]}
+                "pair_flux": pair.value, "flux_march": [march[0], march[1]],
+                "tile": {n: int(tiles[KERNEL_FAMILIES[n]]) for n in names},
+                "persistent_recovery": persist.value}
 
     def close(self):
         if getattr(self, "_p", None):
@@ -308,6 +315,17 @@ class HipSim:
         """Bit d-1: the sweep along d runs two lanes per face (k_flux_pair).
         Results are unchanged."""
         self._check(lib().pangu_sim_set_pair_flux(self._p, int(dir_mask)), "set_pair_flux")
+
+    def set_persistent_recovery(self, on):
+        """Recovery as a persistent kernel pulling tiles from a queue (1) or
+        one workgroup per tile (0).  Results are unchanged."""
+        self._check(lib().pangu_sim_set_persistent_recovery(self._p, int(on)), "set_persistent_recovery")
+
+    def set_tile_size(self, family, cells):
+        """Cells per tile (workgroup size) of a kernel family's launches:
+        256, 128 or 64.  Results are unchanged."""
+        self._check(lib().pangu_sim_set_tile_size(self._p, KERNEL_FAMILIES[family], int(cells)),
+                    "set_tile_size")
 
     def set_flux_march(self, direction, depth):
         """The sweep along `direction` (2 or 3) runs `depth` faces per thread,

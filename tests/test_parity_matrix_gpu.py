@@ -180,6 +180,36 @@ def test_matrix_march_flux(name):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("name", PAIR_CASES + ["torus2d_kastaun"])
+def test_matrix_small_tiles(name):
+    """Every kernel family in tiles of 128 and of 64 cells (workgroups of
+    that size; 1-D to 3-D, both tile orders; the 128-cell run with the
+    persistent work-queue recovery) against the oracle and bit for bit
+    against 256-cell tiles."""
+    from pangu_amd.sim import HipSim, KERNEL_FAMILIES
+    cfg = MATRIX[name]
+    o = O.OracleSim(cfg)
+    o.init()
+    o.step(1)
+    runs = []
+    for cells, order in ((256, 0), (128, 1), (64, 0)):
+        h = HipSim(cfg)
+        h.init()
+        h.set_persistent_recovery(1 if cells == 128 else 0)
+        for fam in KERNEL_FAMILIES:
+            h.set_tile_size(fam, cells)
+            h.set_tile_order(fam, order)
+        h.step(1, "graph")
+        if cells != 256:
+            _compare(o, h, name + " tile %d" % cells)
+        runs.append((cells, [h.get("prim", 0), h.get("cons", 0), h.get("prim", 1), h.get("alfven")]))
+        h.close()
+    for cells, fields in runs[1:]:
+        for a, b in zip(runs[0][1], fields):
+            assert np.array_equal(a, b, equal_nan=True), cells
+
+
+@pytest.mark.gpu
 def test_fix_recovery_patterns():
     """Recovery failures planted by making D < 0 in chosen cells; the
     averaging fixer must reproduce the serial lexicographic result of
@@ -311,7 +341,8 @@ def test_launch_tuning_leaves_results_unchanged():
     cfg = _torus3d()
     runs = []
     for variant in ("default", "split", "column", "plane", "share", "ct_faces", "split_ct_faces",
-                    "pair", "pair_share", "march", "march_column"):
+                    "pair", "pair_share", "march", "march_column", "tile128", "tile64",
+                    "tile64_column", "persistent", "persistent_tile64"):
         h = HipSim(cfg)
         h.init()
         if variant in ("split", "split_ct_faces"):
@@ -325,8 +356,14 @@ def test_launch_tuning_leaves_results_unchanged():
         if variant in ("march", "march_column"):
             h.set_flux_march(2, 5)
             h.set_flux_march(3, 3)
+        if variant.startswith("persistent"):
+            h.set_persistent_recovery(1)
         for fam in KERNEL_FAMILIES:
-            if variant in ("column", "march_column"):
+            if variant.startswith("tile"):
+                h.set_tile_size(fam, int(variant[4:].split("_")[0]))
+            if variant == "persistent_tile64":
+                h.set_tile_size(fam, 64)
+            if variant in ("column", "march_column", "tile64_column"):
                 h.set_tile_order(fam, 1)
             if variant == "plane":
                 h.set_tile_order(fam, 0)
