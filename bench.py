@@ -324,6 +324,57 @@ def learned_vs_plain(cfg3, samples, cycles=20, delta=0.01, gates=LEARNED_GATES):
     return out
 
 
+def learned_1dvsq(samples, cycles=20, gates=(0, 8)):
+    """Learned start of the 1Dvsq Newton iteration (a model of kind 1) on
+    the Orszag-Tang run at 256^2 (1Dvsq recovery, Minkowski).  Training
+    pairs: `samples` synthetic draws plus the run's own states (every
+    interior cell, every second cycle, eight times).  Reported: held-out
+    synthetic states (Newton iterations from v^2 = 0, the start with no
+    previous state, against the learned start), and the OT run plain (the
+    reference's start, the previous primitives' v^2) against the learned
+    start at each gate: zone-cycles/s and Newton iterations per cell."""
+    from pangu_amd import configs, mlp
+    from pangu_amd.sim import HipSim
+    cfg = configs.orszag_tang(256)
+    m = mlp.Mlp(seed=2, kind=1)
+    synth = m.make_dataset(samples, seed=21)
+    src = HipSim(cfg)
+    src.init()
+    harvested = 0
+    for _ in range(8):
+        src.step(2, "graph")
+        harvested += m.harvest(src, 0)
+    src.close()
+    t0 = time.perf_counter()
+    loss = m.train(epochs=8, batch=1024, lr=2e-3)
+    out = {"train_pairs": synth + harvested, "train_synthetic": synth, "train_harvested": harvested,
+           "train_s": time.perf_counter() - t0, "loss_first_last": [float(loss[0]), float(loss[-1])]}
+    m.make_dataset(1 << 18, seed=22)
+    ev = m.evaluate(delta=0.01)
+    out["heldout_synthetic"] = {"newton_iters_from_zero": ev["plain_iters"],
+                                "newton_iters_learned": ev["learned_iters"],
+                                "max_residual_rel": ev["max_residual"], "failures": ev["failures"]}
+    zones = configs.zones(cfg)
+    runs = {}
+    for gate in (None,) + tuple(gates):
+        h = HipSim(cfg)
+        h.init()
+        if gate is not None:
+            m.attach(h, delta=0.01, gate=gate)
+        h.step(10, "graph")
+        ms = h.timed_cycles(cycles, "graph")
+        h.step(1, "graph")
+        runs[gate] = (zones * cycles / (ms * 1e-3), float(h.interior(h.get("c2p_iters")).mean()))
+        h.close()
+    r = {"zone_cycles_per_s_plain": runs[None][0], "newton_iters_plain": runs[None][1]}
+    for gate in gates:
+        r[f"gate_{gate}"] = {"zone_cycles_per_s": runs[gate][0], "speed_ratio": runs[gate][0] / runs[None][0],
+                             "newton_iters": runs[gate][1]}
+    out["ot256"] = r
+    m.close()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -473,6 +524,7 @@ def main():
         out["batched"] = batched_vs_plain(args.ensemble_scale)
     if args.learned_samples > 0:
         out["learned"] = learned_vs_plain(cfg, args.learned_samples)
+        out["learned_1dvsq"] = learned_1dvsq(args.learned_samples)
     if not args.no_refined and not args.small:
         out["refined"] = refined_run()
     if not args.no_cpu_baseline:

@@ -1,6 +1,7 @@
 /*
- * pangu_mlp: learned starting bracket for the Kastaun primitive recovery
- * (no reference counterpart).
+ * pangu_mlp: learned starting bracket for the Kastaun primitive recovery,
+ * and learned Newton start for the 1Dvsq recovery (no reference
+ * counterpart).
  *
  * scheme_kastaun.h:110-165 searches the root mu of its master function by
  * false position on the fixed bracket [0, 1], with no initial guess.  This
@@ -17,7 +18,9 @@
  * floored at 1e-12, 1e-24, 1e-24, 1e-48) and (r.b)^2 / (r^2 b^2 + 1e-300)
  * clamped to [0, 1], standardised with the training set's mean and
  * deviation.
- * Target: logit(mu); the prediction is g = 1 / (1 + exp(-z)).
+ * Target: logit(mu); the prediction is g = 1 / (1 + exp(-z)).  A model of
+ * kind 1 (pangu_mlp_set_kind) predicts z = log u^2 instead, and the 1Dvsq
+ * iteration starts from v^2 = 1 / (1 + exp(-z)).
  */
 #ifndef PANGU_MLP_H
 #define PANGU_MLP_H
@@ -44,14 +47,23 @@ const char *pangu_mlp_last_error(const pangu_mlp *m);
 int pangu_mlp_get_params(pangu_mlp *m, float *out);
 int pangu_mlp_set_params(pangu_mlp *m, const float *in);
 
+/* What the model predicts (before any data set is made): kind 0 (default)
+ * the Kastaun root as logit(mu), for the learned bracket of the robuster
+ * recovery; kind 1 log u^2 = log(W^2 - 1), the start of the 1Dvsq Newton
+ * iteration (v^2 = u^2 / (1 + u^2)) of the onedvsq recovery, which
+ * otherwise starts from the previous primitives' v^2
+ * (scheme_1d_vsq.h:274-301).  Same features, network and trainer. */
+int pangu_mlp_set_kind(pangu_mlp *m, int kind);
+
 /* Synthetic recovery states on the GPU (the training / evaluation set):
  * rho ~ logU(1e-6, 1), u/rho ~ logU(1e-4, 10), W ~ U(1, 20),
  * sigma = b^2/rho ~ logU(1e-3, 1e2), random directions of u~ and B, Kerr
  * metric in MKS coordinates (a, h) at r ~ U(1.3, 50), x2 ~ U(-0.98, 0.98),
  * gamma as given.  Each state is converted to conserved variables and
  * solved by the plain Kastaun recovery; states it solves (status 0) are
- * kept with their root as target.  Returns the number kept in *kept.  A
- * new call replaces the previous set. */
+ * kept with their root as target (kind 1: log(W^2 - 1) of the drawn state,
+ * which both solvers recover).  Returns the number kept in *kept.  A new
+ * call replaces the previous set. */
 int pangu_mlp_make_dataset(pangu_mlp *m, int64_t n, uint64_t seed, double a, double h,
                            double gamma, int64_t *kept);
 
@@ -78,7 +90,11 @@ int pangu_mlp_predict(pangu_mlp *m, const float *feats, int n, float *out);
  * learned roots, the largest relative difference of the recovered rho, u
  * and u~ between the two, and the number of states the learned path failed
  * on.  out: {plain_iters, learned_iters, plain_evals, learned_evals,
- * max_residual, max_rel_diff, failures}. */
+ * max_residual, max_rel_diff, failures}.  Kind 1: 1Dvsq Newton iterations
+ * from v^2 = 0 (no previous state) against the learned start, with entropy
+ * U_ent / D; max_residual is the largest |f| / (Q~^2 + (B^2 + W)^2 v^2) of
+ * the 1Dvsq residual (func_1d_gnr) at the learned results, and the
+ * evaluation counts equal the iteration counts. */
 int pangu_mlp_evaluate(pangu_mlp *m, double delta, double *out);
 
 /* C = A B for host FP32 matrices A [M][K] and B [K][N] (row-major), rounded
@@ -89,9 +105,10 @@ int pangu_mlp_gemm(int device, int M, int N, int K, const float *A, const float 
 /* Appends training pairs from a running simulation to the data set: every
  * stride-th interior cell of container md of ctx (its conserved state in
  * the recovery's normalisation and its metric), solved by the plain Kastaun
- * recovery; solved states are kept with their root as target (*kept
- * receives how many).  Waits for the device first.  The data set's states
- * must share the context's adiabatic index. */
+ * recovery (kind 1: then by 1Dvsq from Kastaun's v^2, with entropy
+ * U_ent / D); solved states are kept with their root (kind 1: log u^2) as
+ * target (*kept receives how many).  Waits for the device first.  The data
+ * set's states must share the context's adiabatic index. */
 int pangu_mlp_harvest(pangu_mlp *m, pangu_ctx *ctx, int md, int stride, int64_t *kept);
 
 /* The in-kernel inference path on host features [n][5]: the network as the
@@ -100,10 +117,12 @@ int pangu_mlp_harvest(pangu_mlp *m, pangu_ctx *ctx, int md, int stride, int64_t 
  * training forward pass instead).  out[n] receives g in (0, 1). */
 int pangu_mlp_predict_wave(pangu_mlp *m, const float *feats, int n, float *out);
 
-/* Attaches the model to a context: its Kastaun recovery then starts from
- * the learned one-sided bracket of initial relative width delta.  NULL detaches.  The
- * model must outlive the attachment.  The recovery kernel evaluates the
- * network itself, in the wavefronts the gate selects
+/* Attaches the model to a context.  Kind 0 (robuster recovery): Kastaun
+ * then starts from the learned one-sided bracket of initial relative width
+ * delta.  Kind 1 (onedvsq recovery): the 1Dvsq Newton iteration starts from
+ * the learned v^2 (delta unused but must lie in (0, 1)).  NULL detaches.
+ * The model must outlive the attachment.  The recovery kernel evaluates
+ * the network itself, in the wavefronts the gate selects
  * (pangu_set_c2p_gate).  Runs with a model attached are off the bit-for-bit
  * parity path (results agree to the solver tolerance). */
 int pangu_set_c2p_guess(pangu_ctx *ctx, pangu_mlp *m, double delta);

@@ -267,6 +267,26 @@ int oracle_kastaun_residual(const double Ucont[8], double mu, double gamma, cons
   return 0;
 }
 
+// The 1Dvsq residual (func_1d_vsq, scheme_1d_vsq.h:91-117) at v^2 = vsq
+// for a container-form conserved vector, normalised as above, with the
+// entropy K the scheme holds fixed; res and scale as vsq_residual.
+int oracle_vsq_residual(const double Ucont[8], double K, double vsq, double gamma, const double gcov[16],
+                        const double gcon[16], double *res, double *scale) {
+  const real(*g4)[4] = (const real(*)[4])gcov;
+  const real(*gn)[4] = (const real(*)[4])gcon;
+  const real sg = std::sqrt(std::fabs(determinant4(g4)));
+  const real alpha = 1.0 / std::sqrt(-gn[0][0]);
+  const real aos = alpha / sg;
+  real U[NPRIM_RECV] = {0};
+  U[RHO] = aos * Ucont[0];
+  U[ENY] = aos * (Ucont[1] - Ucont[0]);
+  for (int a = 0; a < 3; ++a) U[UX1 + a] = aos * Ucont[2 + a];
+  for (int a = 0; a < 3; ++a) U[BX1 + a] = aos * Ucont[5 + a];
+  if (!(U[RHO] > 0.0) || !(vsq >= 0.0 && vsq < 1.0)) return 1;
+  vsq_residual(U, K, vsq, gamma, g4, gn, res, scale);
+  return 0;
+}
+
 static Params heat_params(const oracle_cfg *c) {
   Params p{};
   p.gamma = c->gamma;

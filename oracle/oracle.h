@@ -165,5 +165,7 @@ int c2p_1d(real U[NPRIM_RECV], real prim[NPRIM_RECV], real gamma,
 int c2p_kastaun(real U[NPRIM_RECV], real prim[NPRIM_RECV], real gamma,
                 const real gcov[4][4], const real gcon[4][4], int *iters);
 real kastaun_root(real mu, real D, real q, real bsq, real rsq, real rbsq, real gamma);
+void vsq_residual(const real U[NPRIM_RECV], real K, real vsq, real gamma, const real gcov[4][4],
+                  const real gcon[4][4], real *res, real *scale);
 
 }  // namespace oracle

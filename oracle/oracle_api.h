@@ -129,6 +129,11 @@ int oracle_c2p(int scheme, const double U[9], double prim[9], double gamma,
  * success, 1 or 2 when the state has no invariants. */
 int oracle_kastaun_residual(const double Ucont[8], double mu, double gamma, const double gcov[16],
                             const double gcon[16], double *res);
+/* 1Dvsq residual (scheme_1d_vsq.h:91-117) at v^2 = vsq, entropy K, for a
+ * container-form conserved vector; *scale = Q~^2 + vsq (B^2 + W)^2.
+ * Returns 1 when D <= 0 or vsq is outside [0, 1). */
+int oracle_vsq_residual(const double Ucont[8], double K, double vsq, double gamma, const double gcov[16],
+                        const double gcon[16], double *res, double *scale);
 /* cell = {rho, energy, b^2, advected entropy, recovered entropy, advected
  * electron entropy}: heating_model.h apply() and computeHeatingFraction() */
 double oracle_heating(const oracle_cfg *cfg, const double cell[6]);

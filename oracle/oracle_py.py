@@ -88,6 +88,7 @@ def lib():
         L.oracle_recon.argtypes = [C.c_int, dp, dp]
         L.oracle_c2p.argtypes = [C.c_int, dp, dp, C.c_double, dp, dp, ip]
         L.oracle_kastaun_residual.argtypes = [dp, C.c_double, C.c_double, dp, dp, dp]
+        L.oracle_vsq_residual.argtypes = [dp, C.c_double, C.c_double, C.c_double, dp, dp, dp, dp]
         L.oracle_plant.argtypes = [C.c_int]
         L.oracle_heating.restype = C.c_double
         L.oracle_heating.argtypes = [C.POINTER(OracleCfg), dp]
@@ -386,6 +387,19 @@ def kastaun_residual(Ucont, mu, gamma, gcov, gcon):
     out = C.c_double(0.0)
     r = lib().oracle_kastaun_residual(_dp(U), float(mu), gamma, _dp(gc), _dp(gn), C.byref(out))
     return out.value if r == 0 else None
+
+
+def vsq_residual(Ucont, K, vsq, gamma, gcov, gcon):
+    """(f, scale) of the 1Dvsq residual at v^2 = vsq with entropy K for the
+    container-form conserved vector Ucont (8 values); None when D <= 0 or
+    vsq is outside [0, 1)."""
+    U = np.ascontiguousarray(Ucont[:8], dtype=np.float64)
+    gc = np.ascontiguousarray(gcov, dtype=np.float64)
+    gn = np.ascontiguousarray(gcon, dtype=np.float64)
+    res, sc = C.c_double(0.0), C.c_double(0.0)
+    r = lib().oracle_vsq_residual(_dp(U), float(K), float(vsq), gamma, _dp(gc), _dp(gn), C.byref(res),
+                                  C.byref(sc))
+    return (res.value, sc.value) if r == 0 else None
 
 
 def heating(cfg_dict, cell):

@@ -228,6 +228,21 @@ static void func_1d_vsq(real Bsq, real QdotBsq, real Qtsq, real D, real K, const
   *df = -2. * (*f);
 }
 
+// The residual func_1d_vsq computes (scheme_1d_vsq.h:91-117) at v^2 = vsq,
+// for the recovery-normalised state U and entropy K, and its scale
+// Q~^2 + vsq (B^2 + W)^2 (the two terms it balances).  An independent check
+// of a 1Dvsq result.
+void vsq_residual(const real U[NPRIM_RECV], real K, real vsq, real gamma, const real gcov[4][4],
+                  const real gcon[4][4], real *res, real *scale) {
+  NobleInv v;
+  noble_setup(U, gcov, gcon, v);
+  real p, rho, uu;
+  const real W = w_of_vsq(v.D, K, vsq, &p, &rho, &uu, gamma);
+  const real fact = v.Bsq + W;
+  *res = v.Qtsq - vsq * fact * fact + v.QdotBsq * (v.Bsq + 2. * W) / (W * W);
+  *scale = v.Qtsq + vsq * fact * fact;
+}
+
 // scheme_1d_vsq.h:119-220.
 static int gnr_1dvsq(real Bsq, real QdotBsq, real Qtsq, real D, real K, real x[1], real gamma,
                      int *iters) {

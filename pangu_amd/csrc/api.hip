@@ -776,8 +776,10 @@ int pangu_set_split_source(pangu_ctx *c, int split) {
 int pangu_set_c2p_guess(pangu_ctx *c, pangu_mlp *m, double delta) {
   if (!c) return PANGU_EINVAL;
   if (m && !(delta > 0.0 && delta < 1.0)) return fail(c, PANGU_EINVAL, "delta must be in (0, 1)");
-  if (m && c->cfg.recovery != PANGU_RECOVERY_ROBUSTER)
-    return fail(c, PANGU_EINVAL, "the learned bracket applies to the robuster (Kastaun) recovery");
+  if (m && mlp_kind(m) == 0 && c->cfg.recovery != PANGU_RECOVERY_ROBUSTER)
+    return fail(c, PANGU_EINVAL, "a Kastaun-bracket model applies to the robuster (Kastaun) recovery");
+  if (m && mlp_kind(m) == 1 && c->cfg.recovery != PANGU_RECOVERY_ONEDVSQ)
+    return fail(c, PANGU_EINVAL, "a 1Dvsq-start model applies to the onedvsq recovery");
   MlpFrag W;
   if (m && !mlp_view(m, W)) return fail(c, PANGU_ESTATE, "learned recovery model has no weights");
   c->mlp = m;
@@ -1010,7 +1012,11 @@ static SharedTabs shared_tabs(pangu_ctx *c) {
 // The learned Kastaun bracket's weights when a model is attached (and the
 // recovery is Kastaun), else null.
 static const MlpFrag *learned(pangu_ctx *c, MlpFrag &W) {
-  if (!c->mlp || c->cfg.recovery != PANGU_RECOVERY_ROBUSTER) return nullptr;
+  if (!c->mlp) return nullptr;
+  const int kind = mlp_kind(c->mlp);
+  if (!((kind == 0 && c->cfg.recovery == PANGU_RECOVERY_ROBUSTER) ||
+        (kind == 1 && c->cfg.recovery == PANGU_RECOVERY_ONEDVSQ)))
+    return nullptr;
   return mlp_view(c->mlp, W) ? &W : nullptr;
 }
 

@@ -135,6 +135,8 @@ bool mlp_view(const pangu_mlp *m, MlpFrag &out);
 int mlp_harvest(pangu_mlp *m, const LaunchCtx &L, const KFields &F, int stride, int64_t *kept);
 // Device the model lives on.
 int mlp_device(const pangu_mlp *m);
+// Target of the model: 0 Kastaun bracket, 1 1Dvsq start.
+int mlp_kind(const pangu_mlp *m);
 
 // Attaches (or, with nullptr, detaches) a clock to a context's launches.
 void set_kernel_clock(pangu_ctx *c, KernelClock *clock);
@@ -161,13 +163,15 @@ int launch_recover(const LaunchCtx &L, const KFields &F, const SharedTabs &S, in
                    const MlpFrag *W, hipStream_t s);
 // Training pairs from a container: U in the recovery's normalisation and
 // the cell's metric sample (35 doubles) of every stride-th interior cell
-// (stride >= 1), solved by the plain Kastaun recovery.  Writes slot
+// (stride >= 1), solved by the plain Kastaun recovery (kind 1: then by
+// 1Dvsq from Kastaun's v^2, with entropy U_ent / D).  Writes slot
 // t = (tile * 256 + thread) / stride for t < cap: U [t][9], mtab [t][35],
-// feats [t][5], target [t] (logit of the root), iters [t], status [t] (0
-// solved, < 0 no cell / excised / unsolved).  Returns the slots written.
-int64_t launch_harvest(const LaunchCtx &L, const KFields &F, int stride, double gamma, int64_t cap,
-                       double *U, double *mtab, float *feats, float *target, int *iters, int *status,
-                       hipStream_t s);
+// feats [t][5], target [t] (logit of the root; kind 1: log u^2), iters [t],
+// status [t] (0 solved, < 0 no cell / excised / unsolved, 10 + code a
+// failed 1Dvsq).  Returns the slots written.
+int64_t launch_harvest(const LaunchCtx &L, const KFields &F, int stride, double gamma, int kind,
+                       int64_t cap, double *U, double *mtab, float *feats, float *target, int *iters,
+                       int *status, hipStream_t s);
 int launch_fix_recovery(const LaunchCtx &L, const KFields &F, double *const *flag, int *cluster,
                         hipStream_t s);
 int launch_exchange(const LaunchCtx &L, const KFields &F, double *const *flag, const int *nbr,

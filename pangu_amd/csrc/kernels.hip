@@ -852,7 +852,7 @@ __device__ __forceinline__ void heat_cell(const KCtx &C, const KFields &F, const
 // 1 Kastaun, 2 1Dvsq alone, -1 none (heating only).
 template <int REC, bool HB, bool HK, bool HEAT>
 __device__ __forceinline__ void recover_cell(const KCtx &C, const KFields &F, const SharedTabs &S, int b,
-                                             int k, int j, int i) {
+                                             int k, int j, int i, double x0 = -1.0) {
   const DevBlock &B = C.blk[b];
   const int64_t id = ((int64_t)k * B.n[1] + j) * B.n[0] + i;
   constexpr int NC = ncomp<HB, HK>();
@@ -925,7 +925,8 @@ __device__ __forceinline__ void recover_cell(const KCtx &C, const KFields &F, co
         itn = it;
         write = (st == 0);
       } else if (REC == 2) {
-        st = c2p_1dvsq(U, Ph, P.gamma, m, it);
+        // x0 >= 0: the learned start (k_recover_learned_vsq)
+        st = c2p_1dvsq(U, Ph, P.gamma, m, it, x0);
         itn = it;
       } else {
         st = c2p_2d(U, Ph, P.gamma, m, it);
@@ -1113,6 +1114,55 @@ __global__ __launch_bounds__(256) void k_recover_learned(KCtx C, KFields F, Shar
     }
   }
   if (HEAT && HK) heat_cell<HB, HK>(C, F, G, t, b, id, pp);
+}
+
+// The 1Dvsq recovery with the learned start (and, when HEAT, the heating):
+// the vote, the staging and the in-wave network of k_recover_learned, on
+// the same features (the Kastaun invariants of the cell's U and metric),
+// predicting z = log u^2; every cell of a wavefront that ran the network
+// then starts its Newton iteration from v^2 = u^2 / (1 + u^2) =
+// logistic(z) instead of the previous primitives' v^2.  Everything else is
+// recover_cell<2>: the result agrees with k_recover<2> to the Newton
+// tolerance, not bit for bit.
+template <bool HB, bool HK, bool HEAT>
+__global__ __launch_bounds__(256) void k_recover_learned_vsq(KCtx C, KFields F, SharedTabs S, MlpFrag W,
+                                                             Range3 R) {
+  int b = 0, k = 0, j = 0, i = 0;
+  const bool cell = decode_tile(C, R, b, k, j, i);  // false on the whole workgroup past the range
+  const int lane = threadIdx.x & 63;
+  const Phys &P = C.ph;
+  const DevBlock &B = C.blk[cell ? b : 0];
+  const DevGeo &G = C.geo[B.geo];
+  const int64_t id = cell ? ((int64_t)k * B.n[1] + j) * B.n[0] + i : 0;
+  const int64_t t = cell ? cen_index(G, k, j, i) : 0;
+  const bool active =
+      cell && !(P.metric == PANGU_METRIC_CKS && glb(G.cen)[CG_RAD * G.ncen + t] < P.r_excise);
+  int prev = 0;
+  if (active && S.iters) prev = glb(S.iters[b])[id];
+  const bool gated = active && S.iters && prev >= S.gate;
+  double x0 = -1.0;
+  __shared__ MlpLds lds;
+  // every thread of the workgroup reaches the vote (no early return above)
+  if (__syncthreads_or(gated)) {
+    mlp_stage(W, lds, threadIdx.x, blockDim.x);
+    KastaunIn kin;
+    const int setup = active ? learned_setup<HB, HK>(C, F, G, t, b, id, kin) : -1;
+    __syncthreads();
+    const bool want = setup == 0;
+    if (__ballot(gated && want)) {  // uniform over the wavefront: all 64 lanes run the network
+      float x[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+      if (want) {
+        float f[5];
+        c2p_features(kin, f);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) x[q] = bf2f(f2bf((f[q] - W.mean[q]) / W.dev[q]));
+      }
+      const float z = wave_mlp(lds, W.b4[0], x, lane);
+      if (want) x0 = (double)(1.0f / (1.0f + expf(-z)));
+    }
+  }
+  if (!cell) return;
+  recover_cell<2, HB, HK, HEAT>(C, F, S, b, k, j, i, x0);
 }
 
 // ---------------------------------------------------------- FixRecovery
@@ -1626,8 +1676,10 @@ int launch_recover(const LaunchCtx &L, const KFields &F, const SharedTabs &S, in
   const bool hb = L.k.iBX >= 0, hk = L.k.iKEL >= 0;
   const bool persist = L.persist_recovery && L.queue;
 #define RC(R_, HB_, HK_, HE_)                                                                        \
-  if (R_ == 1 && W)                                                                                   \
+  if (R_ == 1 && W && W->kind == 0)                                                                   \
     hipLaunchKernelGGL((k_recover_learned<HB_, HK_, HE_>), g, wg, 0, s, L.k, F, S, *W, R);            \
+  else if (R_ == 2 && W && W->kind == 1)                                                              \
+    hipLaunchKernelGGL((k_recover_learned_vsq<HB_, HK_, HE_>), g, wg, 0, s, L.k, F, S, *W, R);        \
   else if (persist)                                                                                   \
     hipLaunchKernelGGL((k_recover_queue<R_, HB_, HK_, HE_>),                                          \
                        dim3(resident_groups(k_recover_queue<R_, HB_, HK_, HE_>, R.tile, R.ntiles)), wg, \

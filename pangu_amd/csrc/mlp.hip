@@ -202,9 +202,11 @@ __device__ inline void metric_of(const double *mt, CenMetric &m) {
   m.lapse = mt[34];
 }
 
+// kind 0: target logit(mu), Kastaun's root; kind 1: log u^2 = log(W^2 - 1)
+// of the drawn state, the 1Dvsq start (both from the same features).
 __global__ void k_c2p_dataset(int64_t n, uint64_t seed, const double *mtab, int npts, double gamma,
-                              float *feats, float *target, double *Uout, int *midx, int *iters,
-                              int *status) {
+                              int kind, float *feats, float *target, double *Uout, int *midx,
+                              int *iters, int *status) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double u[11];
@@ -256,12 +258,18 @@ __global__ void k_c2p_dataset(int64_t n, uint64_t seed, const double *mtab, int 
   float f[5] = {0, 0, 0, 0, 0};
   if (st == 0) c2p_features(kin, f);
   for (int q = 0; q < 5; ++q) feats[i * 5 + q] = f[q];
-  target[i] = st == 0 ? (float)log(mu / (1.0 - mu)) : 0.0f;
+  const double usq = W * W - 1.0;
+  target[i] = st != 0 ? 0.0f : kind ? (float)log(usq > 1e-20 ? usq : 1e-20) : (float)log(mu / (1.0 - mu));
 }
 
 // Plain and learned recovery of sample idx[i]; guess[i] from the model.
+// kind 0: Kastaun from [0, 1] against the learned bracket.  kind 1: 1Dvsq
+// (entropy K = U_ent / D, the conserved entropy per unit D) started from
+// v^2 = 0, the start with no previous state, against the learned start
+// v^2 = guess; resid is |f| / scale of the 1Dvsq residual at the learned
+// root.
 __global__ void k_c2p_eval(int n, const int *idx, const double *Uin, const int *midx,
-                           const double *mtab, const float *guess, double gamma, double delta,
+                           const double *mtab, const float *guess, double gamma, double delta, int kind,
                            int *it_plain, int *it_learn, int *ev_learn, double *resid, double *rdiff,
                            int *fail) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -275,6 +283,38 @@ __global__ void k_c2p_eval(int n, const int *idx, const double *Uin, const int *
   }
   CenMetric m;
   metric_of(mtab + (int64_t)midx[s] * NMET, m);
+  if (kind == 1) {
+    const double K = U[ENT] / U[RHO];
+    Pa[ENT] = K;
+    Pb[ENT] = K;
+    int ia = 0, ib = 0;
+    const int sa = c2p_1dvsq(U, Pa, gamma, m, ia, 0.0);
+    const int sb = c2p_1dvsq(U, Pb, gamma, m, ib, (double)guess[i]);
+    it_plain[i] = ia;
+    it_learn[i] = ib;
+    ev_learn[i] = ib;
+    fail[i] = (sa == 0 && sb != 0) ? 1 : 0;
+    double r = 0.0, d = 0.0;
+    if (sb == 0) {
+      // v^2 of the learned result from its primitives
+      double usq = 0.0;
+      for (int a = 0; a < 3; ++a)
+        for (int c = 0; c < 3; ++c) usq += m.gcov[4 * (a + 1) + (c + 1)] * Pb[UX1 + a] * Pb[UX1 + c];
+      double sc = 1.0;
+      const double f = vsq_residual(U, m, K, gamma, usq / (1.0 + usq), &sc);
+      r = fabs(f) / (sc > 0.0 ? sc : 1.0);
+    }
+    if (sa == 0 && sb == 0) {
+      for (int q = 0; q < 5; ++q) {
+        const double den = fabs(Pa[q]) > 1e-300 ? fabs(Pa[q]) : 1e-300;
+        const double e = fabs(Pb[q] - Pa[q]) / den;
+        d = e > d ? e : d;
+      }
+    }
+    resid[i] = r;
+    rdiff[i] = d;
+    return;
+  }
   int ia = 0, ib = 0, eb = 0;
   double mub = 0.0;
   const int sa = c2p_kastaun(U, Pa, gamma, m, ia);
@@ -342,8 +382,8 @@ __global__ __launch_bounds__(64) void k_wave_predict(int n, const float *feats, 
 
 // Training pairs from a container (launch_harvest, kernels.h).
 __global__ __launch_bounds__(256) void k_harvest(KCtx C, KFields F, Range3 R, int stride, double gamma,
-                                                 int64_t cap, double *Uo, double *mto, float *feats,
-                                                 float *target, int *iters, int *status) {
+                                                 int kind, int64_t cap, double *Uo, double *mto,
+                                                 float *feats, float *target, int *iters, int *status) {
   const int N = R.ntiles;
   const int n8 = (N + 7) >> 3;
   const int L = (int)blockIdx.x;
@@ -399,6 +439,25 @@ __global__ __launch_bounds__(256) void k_harvest(KCtx C, KFields F, Range3 R, in
       st = c2p_kastaun(U, Ph, gamma, m, it, &mu);
       if (st == 0 && kastaun_setup(U, m, kin) != 0) st = -2;
       if (st == 0 && !(mu > 0.0 && mu < 1.0)) st = -3;
+      if (st == 0 && kind == 1) {
+        // the 1Dvsq root (entropy K = U_ent / D) from Kastaun's v^2
+        double usq = 0.0;
+        for (int a = 0; a < 3; ++a)
+          for (int c = 0; c < 3; ++c) usq += m.gcov[4 * (a + 1) + (c + 1)] * Ph[UX1 + a] * Ph[UX1 + c];
+        double Pv[9];
+        for (int q = 0; q < 9; ++q) Pv[q] = 0.0;
+        Pv[ENT] = U[ENT] / U[RHO];
+        int iv = 0;
+        st = c2p_1dvsq(U, Pv, gamma, m, iv, usq / (1.0 + usq));
+        if (st == 0) {
+          usq = 0.0;
+          for (int a = 0; a < 3; ++a)
+            for (int c = 0; c < 3; ++c) usq += m.gcov[4 * (a + 1) + (c + 1)] * Pv[UX1 + a] * Pv[UX1 + c];
+          mu = usq;  // carried to the target below
+        } else {
+          st = 10 + st;
+        }
+      }
     }
   }
   for (int q = 0; q < 9; ++q) Uo[slot * 9 + q] = U[q];
@@ -406,19 +465,19 @@ __global__ __launch_bounds__(256) void k_harvest(KCtx C, KFields F, Range3 R, in
   float f[5] = {0, 0, 0, 0, 0};
   if (st == 0) c2p_features(kin, f);
   for (int q = 0; q < 5; ++q) feats[slot * 5 + q] = f[q];
-  target[slot] = st == 0 ? (float)log(mu / (1.0 - mu)) : 0.0f;
+  target[slot] = st != 0 ? 0.0f : kind ? (float)log(mu > 1e-20 ? mu : 1e-20) : (float)log(mu / (1.0 - mu));
   iters[slot] = it;
   status[slot] = st;
 }
 
-int64_t launch_harvest(const LaunchCtx &L, const KFields &F, int stride, double gamma, int64_t cap,
-                       double *U, double *mtab, float *feats, float *target, int *iters, int *status,
-                       hipStream_t s) {
+int64_t launch_harvest(const LaunchCtx &L, const KFields &F, int stride, double gamma, int kind,
+                       int64_t cap, double *U, double *mtab, float *feats, float *target, int *iters,
+                       int *status, hipStream_t s) {
   const Range3 &R = L.interior;
   if (R.ntiles == 0 || stride < 1) return 0;
   const int64_t slots = ((int64_t)R.ntiles * 256 + stride - 1) / stride;
   hipLaunchKernelGGL(k_harvest, dim3((unsigned)(((R.ntiles + 7) / 8) * 8)), dim3(256), 0, s, L.k, F, R,
-                     stride, gamma, cap, U, mtab, feats, target, iters, status);
+                     stride, gamma, kind, cap, U, mtab, feats, target, iters, status);
   if (hipGetLastError() != hipSuccess) return -1;
   return slots < cap ? slots : cap;
 }
@@ -450,6 +509,7 @@ struct pangu_mlp {
   std::vector<int> kept;
   int *dkept = nullptr;
   double gamma = 5.0 / 3.0;
+  int kind = 0;  // 0: Kastaun bracket (logit mu); 1: 1Dvsq start (log u^2)
   // minibatch work buffers
   int cap = 0;
   uint16_t *X = nullptr, *A1 = nullptr, *A2 = nullptr, *A3 = nullptr, *dZ4 = nullptr, *dZ = nullptr;
@@ -593,8 +653,11 @@ bool mlp_view(const pangu_mlp *m, MlpFrag &out) {
   out.b4 = m->P + OFF_B4;
   out.mean = m->mean;
   out.dev = m->dev;
+  out.kind = m->kind;
   return true;
 }
+
+int mlp_kind(const pangu_mlp *m) { return m ? m->kind : -1; }
 
 int mlp_device(const pangu_mlp *m) { return m ? m->device : -1; }
 
@@ -618,7 +681,7 @@ int mlp_harvest(pangu_mlp *m, const LaunchCtx &L, const KFields &F, int stride, 
       grow(m, &m->midx, n0, n1) || grow(m, &m->iters, n0, n1) || grow(m, &m->status, n0, n1) ||
       grow(m, &m->mtab, p0 * NMET, p1 * NMET))
     return PANGU_ENOMEM;
-  const int64_t w = launch_harvest(L, F, stride, gamma, cap, m->U + n0 * 9, m->mtab + p0 * NMET,
+  const int64_t w = launch_harvest(L, F, stride, gamma, m->kind, cap, m->U + n0 * 9, m->mtab + p0 * NMET,
                                    m->feats + n0 * 5, m->target + n0, m->iters + n0, m->status + n0,
                                    m->stream);
   if (w != cap) return mfail(m, PANGU_EDEVICE, "harvest launch failed");
@@ -766,6 +829,15 @@ int pangu_mlp_set_params(pangu_mlp *m, const float *in) {
   return PANGU_OK;
 }
 
+int pangu_mlp_set_kind(pangu_mlp *m, int kind) {
+  if (!m) return PANGU_EINVAL;
+  if (kind != 0 && kind != 1) return mfail(m, PANGU_EINVAL, "kind: 0 (Kastaun bracket) or 1 (1Dvsq start)");
+  if (m->n > 0 && kind != m->kind)
+    return mfail(m, PANGU_ESTATE, "kind: the data set holds targets of the other kind");
+  m->kind = kind;
+  return PANGU_OK;
+}
+
 int pangu_mlp_make_dataset(pangu_mlp *m, int64_t n, uint64_t seed, double a, double h, double gamma,
                            int64_t *kept) {
   if (!m || n <= 0 || n > (int64_t)1 << 30) return PANGU_EINVAL;
@@ -800,7 +872,7 @@ int pangu_mlp_make_dataset(pangu_mlp *m, int64_t n, uint64_t seed, double a, dou
     return PANGU_ENOMEM;
   MCHK(m, hipMemcpy(m->mtab, mt.data(), mt.size() * sizeof(double), hipMemcpyHostToDevice));
   hipLaunchKernelGGL(k_c2p_dataset, dim3(blocks_for(n)), dim3(256), 0, m->stream, n, seed, m->mtab, npts,
-                     gamma, m->feats, m->target, m->U, m->midx, m->iters, m->status);
+                     gamma, m->kind, m->feats, m->target, m->U, m->midx, m->iters, m->status);
   MCHK(m, hipGetLastError());
   std::vector<int> st(n);
   MCHK(m, hipMemcpyAsync(st.data(), m->status, n * sizeof(int), hipMemcpyDeviceToHost, m->stream));
@@ -979,7 +1051,7 @@ int pangu_mlp_evaluate(pangu_mlp *m, double delta, double *out) {
     rc = mfail(m, PANGU_EDEVICE, "copy failed");
   if (!rc) {
     hipLaunchKernelGGL(k_c2p_eval, dim3(blocks_for(nk)), dim3(256), 0, m->stream, nk, m->dkept, m->U, m->midx,
-                       m->mtab, dg, m->gamma, delta, ia, ib, eb, rs, rd, fl);
+                       m->mtab, dg, m->gamma, delta, m->kind, ia, ib, eb, rs, rd, fl);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(m->stream) != hipSuccess ||
         hipMemcpy(va.data(), ia, nk * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
         hipMemcpy(vb.data(), ib, nk * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
@@ -1003,7 +1075,7 @@ int pangu_mlp_evaluate(pangu_mlp *m, double delta, double *out) {
   }
   out[0] = sa / nk;
   out[1] = sb / nk;
-  out[2] = 2.0 + sa / nk;
+  out[2] = (m->kind ? 0.0 : 2.0) + sa / nk;  // Kastaun evaluates both ends of [0, 1] first
   out[3] = se / nk;
   out[4] = mr;
   out[5] = md;

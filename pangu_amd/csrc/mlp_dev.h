@@ -79,6 +79,7 @@ struct MlpFrag {
   const u16x8 *a1, *a2, *a3;
   const float *bias, *w4, *b4;
   const float *mean, *dev;
+  int kind;  // 0: Kastaun bracket (logit mu); 1: 1Dvsq start (log u^2)
 };
 
 // The fragments staged in LDS by the workgroup before its wavefronts run

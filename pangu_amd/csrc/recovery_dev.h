@@ -184,6 +184,12 @@ __device__ int c2p_2d(const double U[9], double p[9], double gamma, const CenMet
 }
 
 // -------------------------------------------------- 1D v^2 isentropic scheme
+// Residual of the 1Dvsq equation (func_1d_gnr, scheme_1d_vsq.h:175-197) at
+// v^2 = x for conserved state U, metric m and entropy K, and its scale
+// Q~^2 + (B^2 + W)^2 v^2 for relative checks.
+__device__ inline double vsq_residual(const double U[9], const CenMetric &m, double K, double gamma,
+                                      double x, double *scale);
+
 __device__ __forceinline__ double w_of_vsq(double D, double K, double vsq, double gamma,
                                            double &pr, double &rho, double &uu) {
   const double gtmp = (1. - vsq);
@@ -193,17 +199,35 @@ __device__ __forceinline__ double w_of_vsq(double D, double K, double vsq, doubl
   return ((rho + uu + pr) / gtmp);
 }
 
-// scheme_1d_vsq.h:52-351 (K = previous primitive entropy).
+__device__ inline double vsq_residual(const double U[9], const CenMetric &m, double K, double gamma,
+                                      double x, double *scale) {
+  Noble v;
+  noble_setup(U, m, v);
+  double pr, rho, uu;
+  const double W = w_of_vsq(v.D, K, x, gamma, pr, rho, uu);
+  const double fact = v.Bsq + W;
+  if (scale) *scale = v.Qtsq + x * fact * fact;
+  return v.Qtsq - x * fact * fact + v.QdotBsq * (v.Bsq + 2. * W) / (W * W);
+}
+
+// scheme_1d_vsq.h:52-351 (K = previous primitive entropy).  The Newton
+// iteration starts from the v^2 of the previous primitives, as the
+// reference does, or from x0 when x0 >= 0 (the learned start).
 __device__ int c2p_1dvsq(const double U[9], double p[9], double gamma, const CenMetric &m,
-                         int &iters) {
+                         int &iters, double x0 = -1.0) {
   iters = 0;
   const double K = p[ENT];
   Noble v;
   noble_setup(U, m, v);
-  double utsq;
-  if (!noble_utsq(p, m, utsq)) return 2;
-  const double gammasq = 1. + utsq;
-  double x = 1. - 1. / gammasq;
+  double x;
+  if (x0 >= 0.0) {
+    x = x0;
+  } else {
+    double utsq;
+    if (!noble_utsq(p, m, utsq)) return 2;
+    const double gammasq = 1. + utsq;
+    x = 1. - 1. / gammasq;
+  }
   const double Bsq = v.Bsq, QdotBsq = v.QdotBsq, Qtsq = v.Qtsq, D = v.D;
   double f = 1., df = 1., errx = 1., W = 0., pr, rho, uu;
   int i_extra = 0, doing_extra = 0, n_iter = 0;

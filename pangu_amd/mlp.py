@@ -38,6 +38,7 @@ def lib():
         L.pangu_last_error.argtypes = [vp]
         L.pangu_mlp_harvest.argtypes = [vp, vp, C.c_int, C.c_int, C.POINTER(C.c_int64)]
         L.pangu_mlp_predict_wave.argtypes = [vp, fp, C.c_int, fp]
+        L.pangu_mlp_set_kind.argtypes = [vp, C.c_int]
         _bound = True
     return L
 
@@ -64,14 +65,17 @@ def gemm(A, B, device=0):
 
 
 class Mlp:
-    """The 5-64-64-64-1 SiLU network that predicts Kastaun's root."""
+    """The 5-64-64-64-1 SiLU network that predicts Kastaun's root (kind 0)
+    or the start of the 1Dvsq Newton iteration, log u^2 (kind 1)."""
 
-    def __init__(self, seed=0, device=0):
+    def __init__(self, seed=0, device=0, kind=0):
         p = C.c_void_p()
         r = lib().pangu_mlp_create(device, seed, C.byref(p))
         if r != 0 or not p:
             raise RuntimeError(f"pangu_mlp_create failed ({r})")
         self._p = p
+        self.kind = kind
+        self._check(lib().pangu_mlp_set_kind(self._p, int(kind)), "set_kind")
 
     def close(self):
         if getattr(self, "_p", None):

@@ -1,5 +1,6 @@
-"""Learned Kastaun bracket on the GPU: BF16 MFMA products, gradients against
-the FP64 reference, training, and recovery with the learned bracket."""
+"""Learned recovery on the GPU: BF16 MFMA products, gradients against the
+FP64 reference, training, recovery with the learned Kastaun bracket and
+with the learned 1Dvsq start."""
 import numpy as np
 import pytest
 
@@ -215,3 +216,76 @@ def test_learned_bracket_on_the_2d_torus_workload():
     assert hard.sum() > 0 and il < ip
     for r in runs.values():
         r["h"].close()
+
+
+@pytest.mark.gpu
+def test_learned_1dvsq_start():
+    """A model of kind 1 predicts the start of the 1Dvsq Newton iteration
+    (log u^2, the iteration starting from v^2 = u^2 / (1 + u^2)), trained on
+    synthetic draws plus states harvested from the Orszag-Tang run, whose
+    recovery is 1Dvsq (Minkowski):
+    - held-out synthetic states: fewer Newton iterations than the start with
+      no previous state (v^2 = 0), no state the learned start loses, and
+      every learned result within 1e-10 of the 1Dvsq residual's scale;
+    - the OT run with the model attached (gate 0, every wavefront): every
+      cell recovered in the last stage satisfies the 1Dvsq equation to
+      1e-10 of its scale (the oracle's residual from the cell's U, its v^2
+      and the entropy the recovery held fixed), as the plain run's cells do,
+      and P and U stay within the Newton tolerance of the plain run after
+      ten cycles.  The mean iterations against the reference's start (the
+      previous primitives' v^2) are reported, not asserted: on a running
+      simulation that start is already close."""
+    import oracle_py as O
+    from pangu_amd import configs, mlp
+    from pangu_amd.sim import HipSim
+    cfg = configs.orszag_tang(128)
+    m = mlp.Mlp(seed=7, kind=1)
+    m.make_dataset(1 << 18, seed=41)
+    src = HipSim(cfg)
+    src.init()
+    harvested = 0
+    for _ in range(4):
+        src.step(2, "graph")
+        harvested += m.harvest(src, 0)
+    src.close()
+    print("harvested", harvested, flush=True)
+    assert harvested > 0.95 * 4 * 128 * 128
+    loss = m.train(epochs=8, batch=1024, lr=2e-3)
+    print("loss first/last", loss[0], loss[-1], flush=True)
+    m.make_dataset(1 << 16, seed=42)
+    ev = m.evaluate(delta=0.01)
+    print("held-out 1Dvsq", ev, flush=True)
+    assert ev["failures"] == 0
+    assert ev["learned_iters"] < ev["plain_iters"]
+    assert ev["max_residual"] <= 1e-10
+    runs = {}
+    for name in ("plain", "learned"):
+        h = HipSim(cfg)
+        h.init()
+        if name == "learned":
+            m.attach(h, delta=0.01, gate=0)
+        h.step(10, "graph")
+        runs[name] = dict(P=h.interior(h.get("prim", 0)), U=h.interior(h.get("cons", 0)),
+                          it=h.interior(h.get("c2p_iters")), st=h.interior(h.get("c2p_status")))
+        h.close()
+    gam = cfg["gamma"]
+    eta = np.diag([-1.0, 1.0, 1.0, 1.0])
+    for name, r in runs.items():
+        P, U, st = r["P"][0], r["U"][0], r["st"][0, 0, 0]
+        worst, nacc = 0.0, 0
+        for jj in range(st.shape[0]):
+            for ii in range(st.shape[1]):
+                if st[jj, ii] != 0:
+                    continue
+                nacc += 1
+                ut = P[2:5, 0, jj, ii]
+                usq = float(ut @ ut)
+                f, sc = O.vsq_residual(U[:8, 0, jj, ii], P[8, 0, jj, ii], usq / (1.0 + usq), gam, eta, eta)
+                worst = max(worst, abs(f) / sc)
+        print(name, "accepted", nacc, "of", st.size, "max |f|/scale", worst,
+              "mean iterations", float(r["it"].mean()), flush=True)
+        assert nacc > 0.99 * st.size and worst <= 1e-10
+    lp = per_comp_l1(runs["learned"]["P"], runs["plain"]["P"])
+    lu = per_comp_l1(runs["learned"]["U"], runs["plain"]["U"])
+    print("learned vs plain after 10 cycles: P", max(lp), "U", max(lu), flush=True)
+    assert max(lp) <= 1e-8 and max(lu) <= 1e-8

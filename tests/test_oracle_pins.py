@@ -608,6 +608,29 @@ def test_kastaun_residual_vanishes_at_the_true_root(seed):
         assert 1e-8 * mu < rq < 1e-4 * mu, (rq, mu)
 
 
+@pytest.mark.parametrize("seed", range(4))
+def test_vsq_residual_vanishes_at_the_true_state(seed):
+    """oracle_vsq_residual (the check of accepted 1Dvsq results) is ~0 at the
+    v^2 = u^2 / (1 + u^2) of the primitives a container-form conserved vector
+    was made from, with the entropy K = (gamma - 1) u rho^-gamma of those
+    primitives: then rho = D sqrt(1 - v^2), p = K rho^gamma and
+    W = (rho + u + p) / (1 - v^2) are the state's own, and the two sides of
+    the equation balance to rounding (1e-12 of their scale).  Off the root
+    (v^2 (1 + 1e-6)) the residual is of order 1e-6 of the scale."""
+    rng = np.random.default_rng(720 + seed)
+    gcov, gcon, gdet = _metric(rng)
+    for _ in range(20):
+        p = rand_prim(rng)
+        Uc = O.flux(GAM, p, gcov, gcon, gdet, 0)
+        usq = p[2:5] @ gcov[1:, 1:] @ p[2:5]
+        vsq = usq / (1.0 + usq)
+        K = (GAM - 1.0) * p[1] * p[0] ** (-GAM)
+        f, sc = O.vsq_residual(Uc, K, vsq, GAM, gcov, gcon)
+        assert abs(f) <= 1e-12 * sc, (f, sc)
+        f2, sc2 = O.vsq_residual(Uc, K, vsq * (1.0 + 1e-6), GAM, gcov, gcon)
+        assert 1e-9 * sc2 < abs(f2) < 1e-4 * sc2, (f2, sc2, vsq)
+
+
 def test_1d_scheme_is_approximate_away_from_unit_density():
     """Documents reference behaviour reproduced by the oracle: the 1D-W
     fallback reports success with an inexact state when rho != 1."""
