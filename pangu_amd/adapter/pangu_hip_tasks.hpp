@@ -36,36 +36,43 @@ inline std::map<int, Partition> &Registry() {
   return r;
 }
 
+// Index of s in the list, or a failure naming the parameter, as the
+// reference rejects unknown choices (package_registration.cc:38-40,
+// riemann_solver.cc:25, heating_model.h:118).
+inline int Choice(const std::string &what, const std::string &s, const std::vector<std::string> &opts) {
+  for (size_t i = 0; i < opts.size(); ++i)
+    if (opts[i] == s) return (int)i;
+  PARTHENON_FAIL(("unknown " + what + ": " + s).c_str());
+  return -1;
+}
+
 // The <core>, <electron> and <metric> blocks with the reference's names and
-// defaults (package_registration.cc:21-57,158-167).
+// defaults (package_registration.cc:21-57,158-167).  recovery = "onedvsq"
+// (1Dvsq alone) is this library's extension; the reference accepts
+// "chainer" and "robuster".
 inline pangu_config MakeConfig(ParameterInput *pin) {
   pangu_config c{};
   const std::string mt = pin->GetOrAddString("metric", "type", "minkowski");
-  c.metric = mt == "bl" ? PANGU_METRIC_BL : mt == "cks" ? PANGU_METRIC_CKS
-           : mt == "mks" ? PANGU_METRIC_MKS : PANGU_METRIC_MINKOWSKI;
+  c.metric = Choice("metric type", mt, {"minkowski", "bl", "cks", "mks"});  // PANGU_METRIC_* order
   c.a = pin->GetOrAddReal("metric", "a", 0.0);
   c.h = pin->GetOrAddReal("metric", "h", 0.0);
   c.cfl = pin->GetOrAddReal("core", "cfl_number", 0.8);
   c.gamma = pin->GetOrAddReal("core", "adiabatic_index", 5. / 3.);
-  c.riemann = pin->GetOrAddString("core", "riemann_solver", "laxf") == "hll" ? PANGU_RIEMANN_HLL
-                                                                            : PANGU_RIEMANN_LAXF;
-  c.limiter = pin->GetOrAddString("core", "limiter", "ppm4") == "mc" ? PANGU_LIMITER_MC
-                                                                     : PANGU_LIMITER_PPM4;
+  c.riemann = Choice("riemann_solver", pin->GetOrAddString("core", "riemann_solver", "laxf"),
+                     {"laxf", "hll"});
+  c.limiter = Choice("limiter", pin->GetOrAddString("core", "limiter", "ppm4"), {"mc", "ppm4"});
   c.density_floor = pin->GetOrAddReal("core", "density_floor", 1.0e-6);
   c.energy_floor = pin->GetOrAddReal("core", "energy_floor", 1.0e-8);
   c.density_floor_pow = pin->GetOrAddReal("core", "density_floor_pow", -1.5);
   c.energy_floor_pow = pin->GetOrAddReal("core", "energy_floor_pow", -2.5);
   c.enable_B = pin->GetOrAddBoolean("core", "enable_B", false);
-  const std::string rec = pin->GetOrAddString("core", "recovery", "chainer");
-  c.recovery = rec == "robuster" ? PANGU_RECOVERY_ROBUSTER
-             : rec == "onedvsq" ? PANGU_RECOVERY_ONEDVSQ : PANGU_RECOVERY_CHAINER;
+  c.recovery = Choice("recovery", pin->GetOrAddString("core", "recovery", "chainer"),
+                      {"chainer", "robuster", "onedvsq"});
   c.enable_heating = pin->GetOrAddBoolean("core", "enable_heating", false);
   c.sigma_max = pin->GetOrAddReal("core", "sigma_max", 50);
   c.lorentz_max = pin->GetOrAddReal("core", "lorentz_max", 50);
-  const std::string model = pin->GetOrAddString("electron", "model", "constant");
-  c.heat_model = model == "howes" ? PANGU_HEAT_HOWES : model == "kawazura" ? PANGU_HEAT_KAWAZURA
-               : model == "werner" ? PANGU_HEAT_WERNER : model == "rowan" ? PANGU_HEAT_ROWAN
-               : model == "sharma" ? PANGU_HEAT_SHARMA : PANGU_HEAT_CONSTANT;
+  c.heat_model = Choice("electron model", pin->GetOrAddString("electron", "model", "constant"),
+                        {"constant", "howes", "kawazura", "werner", "rowan", "sharma"});
   c.fel_constant = pin->GetOrAddReal("electron", "fel_constant", 0.1);
   c.gamma_e = pin->GetOrAddReal("electron", "gamma_e", 4. / 3.);
   c.gamma_p = pin->GetOrAddReal("electron", "gamma_p", 5. / 3.);
@@ -79,9 +86,8 @@ inline pangu_config MakeConfig(ParameterInput *pin) {
               pin->GetOrAddInteger("parthenon/meshblock", "nx" + std::to_string(d + 1), 1);
     for (int s = 0; s < 2; ++s) {
       const std::string key = std::string(s ? "o" : "i") + "x" + std::to_string(d + 1) + "_bc";
-      const std::string bc = pin->GetOrAddString("parthenon/mesh", key, "periodic");
-      c.bc[d][s] = bc == "outflow" ? PANGU_BC_OUTFLOW : bc == "reflecting" ? PANGU_BC_REFLECT
-                                                                        : PANGU_BC_PERIODIC;
+      c.bc[d][s] = Choice(key, pin->GetOrAddString("parthenon/mesh", key, "periodic"),
+                          {"periodic", "outflow", "reflecting"});
     }
   }
   return c;

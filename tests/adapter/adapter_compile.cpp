@@ -34,7 +34,17 @@ int main() {
                   c.recovery == PANGU_RECOVERY_ROBUSTER && c.heat_model == PANGU_HEAT_KAWAZURA &&
                   c.bc[0][0] == PANGU_BC_OUTFLOW && c.bc[0][1] == PANGU_BC_PERIODIC && c.nb[0] == 4 &&
                   c.gamma == 5. / 3. && c.cfl == 0.8 && c.limiter == PANGU_LIMITER_PPM4;
-  std::printf("tasks %zu source %d dt %d setup %d config %s\n", sizeof(tasks) / sizeof(tasks[0]),
-              source != nullptr, dt != nullptr, setup != nullptr, ok ? "ok" : "wrong");
-  return ok ? 0 : 1;
+  // an unknown choice fails as the reference's std::invalid_argument does
+  bool rejected = false;
+  try {
+    parthenon::ParameterInput bad;
+    bad.GetOrAddString("core", "recovery", "newton");
+    (void)T::MakeConfig(&bad);
+  } catch (const std::exception &) {
+    rejected = true;
+  }
+  std::printf("tasks %zu source %d dt %d setup %d config %s unknown-choice %s\n",
+              sizeof(tasks) / sizeof(tasks[0]), source != nullptr, dt != nullptr, setup != nullptr,
+              ok ? "ok" : "wrong", rejected ? "rejected" : "accepted");
+  return ok && rejected ? 0 : 1;
 }

@@ -28,4 +28,4 @@ def test_adapter_compiles_links_and_maps_the_input(tmp_path):
     out = subprocess.run([exe], capture_output=True, text=True)
     print(out.stdout)
     assert out.returncode == 0, out.stdout + out.stderr
-    assert "config ok" in out.stdout
+    assert "config ok" in out.stdout and "unknown-choice rejected" in out.stdout
