@@ -224,6 +224,26 @@ def batched_vs_plain(scale, cycles=10, warmup=2):
     }
 
 
+def other_configs(cycles=50, warmup=5):
+    """Plain throughput of the other two single-mesh workloads as configured:
+    the Orszag-Tang vortex at 256^2 (Minkowski, MC, LAXF, CT, 1Dvsq) and the
+    2-D MKS torus at 512x256 (PPM4, LAXF, chainer, heating)."""
+    from pangu_amd import configs
+    from pangu_amd.sim import HipSim
+    out = {}
+    for name, cfg in (("orszag_tang_256x256", configs.orszag_tang(256)),
+                      ("torus2d_512x256_chainer", configs.torus2d(512, 256))):
+        h = HipSim(cfg)
+        h.init()
+        h.step(warmup, "graph")
+        ms = h.timed_cycles(cycles, "graph")
+        zones = configs.zones(cfg)
+        out[name] = {"zones": zones, "cycles": cycles, "ms_per_cycle": ms / cycles,
+                     "zone_cycles_per_s": zones * cycles / (ms * 1e-3)}
+        h.close()
+    return out
+
+
 def refined_run(cycles=20, warmup=2):
     """The reference's static-refinement run (fm_torus_cks/inputfile): a
     hydrodynamic FM torus in Cartesian Kerr-Schild coordinates with
@@ -520,6 +540,8 @@ def main():
         cms = sim.copy_calibration(n, reps=5)
         out["copy_calibration"] = {"doubles": n, "ms": cms, "GBps": 16.0 * n / (cms * 1e-3) / 1e9}
     sim.close()
+    if not args.small:
+        out["other_configs"] = other_configs()
     if args.ensemble_scale > 0:
         out["batched"] = batched_vs_plain(args.ensemble_scale)
     if args.learned_samples > 0:
