@@ -34,7 +34,7 @@ struct pangu_ctx {
   std::vector<int> bmesh;           // mesh of each block
   std::vector<int> mfirst;          // first block of each mesh (nmesh + 1)
   int device = 0;
-  int nmesh = 1, nplanes = 0;
+  int nmesh = 1, nplanes = 0, ndt = 0;
   int nblocks = 0, ndim = 1, np = 0, nc = 0, iBX = -1, iENT = 0, iKEL = -1;
   std::vector<HostBlock> hb;
   std::vector<int *> tab_bufs;      // launch tables
@@ -577,6 +577,21 @@ static int build_tables(pangu_ctx *c) {
   L.k.pblock = pb;
   L.k.pfirst = pf;
   L.k.nplanes = np;
+  // dt reduction items: every interior plane in chunks of kDtChunk cells,
+  // so that a large single-block plane is scanned by many workgroups
+  std::vector<int64_t> dtitems(c->nblocks);
+  for (int b = 0; b < c->nblocks; ++b) {
+    const HostBlock &H = c->hb[b];
+    const int64_t plane = (int64_t)(H.e[0] - H.s[0] + 1) * (H.e[1] - H.s[1] + 1);
+    dtitems[b] = planes[b] * ((plane + kDtChunk - 1) / kDtChunk);
+  }
+  const int *db = nullptr, *dfi = nullptr;
+  int nd = 0;
+  if (int e = upload_tiles(c, dtitems, &db, &dfi, &nd)) return e;
+  c->ndt = nd;
+  L.k.dtblock = db;
+  L.k.dtfirst = dfi;
+  L.k.ndt = nd;
   int *dm = nullptr;
   if (upload(c, &dm, c->mfirst.data(), c->mfirst.size())) return PANGU_EDEVICE;
   c->tab_bufs.push_back(dm);
@@ -651,8 +666,8 @@ int pangu_create_batch(const pangu_config *cfgs, int nmesh, const pangu_block *b
   if (!r) r = build_neighbours(c);
   if (!r) r = build_multilevel(c, rt);
   if (!r) r = build_tables(c);
-  if (!r) r = upload(c, &c->dred, std::vector<double>((size_t)c->nplanes * 3, 0.0).data(),
-                     (size_t)c->nplanes * 3);
+  if (!r) r = upload(c, &c->dred, std::vector<double>((size_t)c->ndt * 3, 0.0).data(),
+                     (size_t)c->ndt * 3);
   if (!r) r = upload(c, &c->dcluster, std::vector<int>((size_t)c->nplanes, 0).data(),
                      (size_t)c->nplanes);
   if (!r) r = upload(c, &c->dqueue, std::vector<unsigned>(2, 0u).data(), 2);

@@ -21,6 +21,10 @@ struct KCtx {
   int nplanes;         // interior (block, k) planes over all blocks
   const int *pblock;   // plane -> block
   const int *pfirst;   // first plane of each block (nblocks + 1 entries)
+  // dt reduction items: each interior plane in chunks of kDtChunk cells
+  int ndt;
+  const int *dtblock;  // item -> block
+  const int *dtfirst;  // first item of each block (nblocks + 1 entries)
   const int *mfirst;   // first block of each mesh (nmesh + 1 entries)
   Phys ph;
 };
@@ -102,6 +106,10 @@ struct MultiLevel {
   const XFlux *flux = nullptr;
   int nfill = 0, nprol = 0, nflux = 0;
 };
+
+// Cells of one interior plane that one workgroup of the dt reduction
+// scans (a 64x64 plane is one chunk; a 512x256 plane 32).
+constexpr int kDtChunk = 4096;
 
 // Host-side launch context: kernel constants plus the launch ranges.
 struct LaunchCtx {

@@ -1397,13 +1397,17 @@ __global__ __launch_bounds__(256) void k_bc(KCtx C, KFields F, double *const *fl
 // result is deterministic.
 __global__ __launch_bounds__(256) void k_dt_reduce(KCtx C, double *const *alfven, double *red) {
   __shared__ double sm[4][3];
-  const int b = C.pblock[blockIdx.x];
-  const int kk = (int)blockIdx.x - C.pfirst[b];
+  // item = (block, interior plane, chunk of kDtChunk cells of the plane)
+  const int b = C.dtblock[blockIdx.x];
+  const int local = (int)blockIdx.x - C.dtfirst[b];
   const DevBlock &B = C.blk[b];
-  const int k = B.s[2] + kk;
   const int ni = B.e[0] - B.s[0] + 1, nj = B.e[1] - B.s[1] + 1;
+  const int nch = (ni * nj + kDtChunk - 1) / kDtChunk;
+  const int kk = local / nch, ch = local - kk * nch;
+  const int k = B.s[2] + kk;
+  const int t0 = ch * kDtChunk, t1 = (t0 + kDtChunk < ni * nj) ? t0 + kDtChunk : ni * nj;
   double c[3] = {0.0, 0.0, 0.0};
-  for (int t = threadIdx.x; t < ni * nj; t += blockDim.x) {
+  for (int t = t0 + (int)threadIdx.x; t < t1; t += blockDim.x) {
     const int j = B.s[1] + t / ni, i = B.s[0] + t % ni;
     const int64_t id = ((int64_t)k * B.n[1] + j) * B.n[0] + i;
     for (int d = 0; d < 3; ++d) {
@@ -1445,7 +1449,7 @@ __global__ __launch_bounds__(256) void k_dt_final(KCtx C, const double *red, dou
     for (int d = 0; d < 3; ++d) {
       if (B.n[d] == 1) continue;
       double cmax = 0.0;
-      for (int p = C.pfirst[b]; p < C.pfirst[b + 1]; ++p) {
+      for (int p = C.dtfirst[b]; p < C.dtfirst[b + 1]; ++p) {
         const double v = red[(int64_t)p * 3 + d];
         cmax = (v > cmax) ? v : cmax;
       }
@@ -1737,7 +1741,7 @@ int launch_bc(const LaunchCtx &L, const KFields &F, double *const *flag, hipStre
 
 int launch_dt(const LaunchCtx &L, double *const *alfven, double *red, double *state,
               double dt_factor, double *est_out, hipStream_t s) {
-  hipLaunchKernelGGL(k_dt_reduce, dim3(L.k.nplanes), dim3(256), 0, s, L.k, alfven, red);
+  hipLaunchKernelGGL(k_dt_reduce, dim3(L.k.ndt), dim3(256), 0, s, L.k, alfven, red);
   TICK("dt_reduce");
   hipLaunchKernelGGL(k_dt_final, dim3(L.k.nmesh), dim3(256), 0, s, L.k, red, state, dt_factor,
                      est_out);
