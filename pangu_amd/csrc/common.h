@@ -24,6 +24,25 @@ template <class T>
 __device__ __forceinline__ gptr<T> glb(T *p) {
   return (gptr<T>)p;
 }
+// Element i of a device array through a 32-bit byte offset: with a base
+// the compiler knows to be uniform this is one global_* instruction with
+// the base in scalar registers and one 32-bit vector offset shared by
+// neighbouring accesses, instead of a 64-bit address computed per access.
+// The per-thread index times the element size must stay below 4 GiB
+// (pangu_create rejects blocks of more than 2^29 cells).
+// ld32(base, u, i) reads base[u + i] where u is the same for the whole
+// workgroup (a stencil offset, a table component) and i the thread's own
+// index: base + u goes to scalar registers, i to the vector offset.
+template <class T>
+__device__ __forceinline__ T ld32(const T *base, int64_t u, uint32_t i) {
+  return *(const __attribute__((address_space(1))) T *)((const __attribute__((address_space(1))) char *)(base + u) +
+                                                        i * (uint32_t)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ void st32(T *base, uint32_t i, T v) {
+  *(__attribute__((address_space(1))) T *)((__attribute__((address_space(1))) char *)base +
+                                           i * (uint32_t)sizeof(T)) = v;
+}
 #endif
 
 // Static mesh refinement: gather tables built on the host from the block

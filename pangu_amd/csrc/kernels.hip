@@ -48,14 +48,15 @@ struct CenLoad {
   double g[16], gc0[4], alpha, sg;
 };
 __device__ __forceinline__ void load_cen_state(const DevGeo &G, int64_t t, CenLoad &m) {
-  const auto c = glb(G.cen);
+  const double *c = G.cen;
   const int64_t s = G.ncen;
+  const uint32_t t32 = (uint32_t)t;
 #pragma unroll
-  for (int n = 0; n < 16; ++n) m.g[n] = c[(CG_GCOV + n) * s + t];
+  for (int n = 0; n < 16; ++n) m.g[n] = ld32(c, (CG_GCOV + n) * s, t32);
 #pragma unroll
-  for (int n = 0; n < 4; ++n) m.gc0[n] = c[(CG_GCON + n) * s + t];
-  m.alpha = c[CG_ALPHA * s + t];
-  m.sg = c[CG_SG * s + t];
+  for (int n = 0; n < 4; ++n) m.gc0[n] = ld32(c, (CG_GCON + n) * s, t32);
+  m.alpha = ld32(c, CG_ALPHA * s, t32);
+  m.sg = ld32(c, CG_SG * s, t32);
 }
 
 // ------------------------------------------------ fix_primitive + conservative
@@ -91,8 +92,8 @@ __global__ __launch_bounds__(256) void k_fix_cons(KCtx C, KFields F, uint8_t *co
     const double small = 1.0e-20;
     const double sms = kmax(P.sigma_max, small);
     const double rho_before = kmax(p[RHO], small);
-    double rho_floor = glb(G.cen)[CG_RHOFL * G.ncen + t];
-    const double eng_floor = glb(G.cen)[CG_ENGFL * G.ncen + t];
+    double rho_floor = ld32(G.cen, (int64_t)CG_RHOFL * G.ncen, (uint32_t)t);
+    const double eng_floor = ld32(G.cen, (int64_t)CG_ENGFL * G.ncen, (uint32_t)t);
     St st;
     calc_state(p, m.g, m.gc0, m.alpha, st);
     const double rho = p[RHO], ug = p[ENY];
@@ -127,7 +128,7 @@ __global__ __launch_bounds__(256) void k_fix_cons(KCtx C, KFields F, uint8_t *co
     if (HK && rho_after != rho_before) p[KEL] *= pow(rho_after / rho_before, -P.gamma);
     p[ENT] = (P.gamma - 1.0) * p[ENY] * pow(p[RHO], -P.gamma);
     if (HK) p[KEL] = clamp_by_ratio(P.ratio_min, P.ratio_max, p[ENT], p[KEL]);
-    if (P.metric == PANGU_METRIC_CKS && glb(G.cen)[CG_RAD * G.ncen + t] < P.r_excise) {
+    if (P.metric == PANGU_METRIC_CKS && ld32(G.cen, (int64_t)CG_RAD * G.ncen, (uint32_t)t) < P.r_excise) {
       const double e_exc = P.pexcise / (P.gamma - 1.0);
       p[RHO] = P.dexcise;
       p[ENY] = e_exc;
@@ -181,37 +182,38 @@ __device__ __forceinline__ void face_flux(const KCtx &C, const KFields &F, doubl
   const DevGeo &G = C.geo[B.geo];
   double g[16], gc0[4], gdd, sg, alpha, valid;
   if (DIR == 3 && !G.face[2]) {
-    const int64_t t = cen_index(G, k, j, i);
-    const auto c = glb(G.cen);
-    const int64_t s = G.ncen;
+    const uint32_t t = (uint32_t)cen_index(G, k, j, i);
+    const double *c = G.cen;
+    const uint32_t s = (uint32_t)G.ncen;
 #pragma unroll
-    for (int n = 0; n < 16; ++n) g[n] = c[(CG_GCOV + n) * s + t];
+    for (int n = 0; n < 16; ++n) g[n] = ld32(c, (int64_t)(CG_GCOV + n) * s, t);
 #pragma unroll
-    for (int n = 0; n < 4; ++n) gc0[n] = c[(CG_GCON + n) * s + t];
-    gdd = c[(CG_GCON + 15) * s + t];
-    sg = c[CG_SG * s + t];
-    alpha = c[CG_ALPHA * s + t];
+    for (int n = 0; n < 4; ++n) gc0[n] = ld32(c, (int64_t)(CG_GCON + n) * s, t);
+    gdd = ld32(c, (int64_t)(CG_GCON + 15) * s, t);
+    sg = ld32(c, (int64_t)CG_SG * s, t);
+    alpha = ld32(c, (int64_t)CG_ALPHA * s, t);
     valid = 1.0;
   } else {
     const int d = DIR - 1;
-    const int64_t t = ((int64_t)(G.has3d ? k : 0) * G.fj[d] + j) * G.fi[d] + i;
-    const auto c = glb(G.face[d]);
-    const int64_t s = G.nface[d];
+    const uint32_t t = (uint32_t)(((G.has3d ? k : 0) * G.fj[d] + j) * G.fi[d] + i);
+    const double *c = G.face[d];
+    const uint32_t s = (uint32_t)G.nface[d];
 #pragma unroll
-    for (int n = 0; n < 16; ++n) g[n] = c[(FG_GCOV + n) * s + t];
+    for (int n = 0; n < 16; ++n) g[n] = ld32(c, (int64_t)(FG_GCOV + n) * s, t);
 #pragma unroll
-    for (int n = 0; n < 4; ++n) gc0[n] = c[(FG_GCON0 + n) * s + t];
-    gdd = c[FG_GCONDD * s + t];
-    sg = c[FG_SG * s + t];
-    alpha = c[FG_ALPHA * s + t];
-    valid = c[FG_VALID * s + t];
+    for (int n = 0; n < 4; ++n) gc0[n] = ld32(c, (int64_t)(FG_GCON0 + n) * s, t);
+    gdd = ld32(c, (int64_t)FG_GCONDD * s, t);
+    sg = ld32(c, (int64_t)FG_SG * s, t);
+    alpha = ld32(c, (int64_t)FG_ALPHA * s, t);
+    valid = ld32(c, (int64_t)FG_VALID * s, t);
   }
-  const auto al = glb(alfven[b * 3 + (DIR - 1)]);
+  double *const al = alfven[b * 3 + (DIR - 1)];
+  const uint32_t id32 = (uint32_t)id;
   if (valid == 0.0) {
     // singular face metric (polar axis): zero flux and wave speed
-    al[id] = 0.0;
+    st32(al, id32, 0.0);
 #pragma unroll
-    for (int q = 0; q < NC; ++q) glb(F.flux[DIR - 1][b * NC + q])[id] = 0.0;
+    for (int q = 0; q < NC; ++q) st32(F.flux[DIR - 1][b * NC + q], id32, 0.0);
     return;
   }
   const double gam = C.ph.gamma;
@@ -224,7 +226,7 @@ __device__ __forceinline__ void face_flux(const KCtx &C, const KFields &F, doubl
   const double cmax = fabs(kmax(kmax(0., vmaxL), vmaxR));
   const double cmin = fabs(kmax(kmax(0., -vminL), -vminR));
   const double cc = kmax(cmax, cmin);
-  al[id] = cc;
+  st32(al, id32, cc);
   double UL[NPRIM], UR[NPRIM], FL[NPRIM], FR[NPRIM];
   contravariant_flux<0>(gam, pl, sl, sg, UL);
   contravariant_flux<0>(gam, pr, sr, sg, UR);
@@ -239,7 +241,7 @@ __device__ __forceinline__ void face_flux(const KCtx &C, const KFields &F, doubl
       f = 0.5 * (FL[s] + FR[s] - cc * (UR[s] - UL[s]));
     else
       f = div_by(cmax * FL[s] + cmin * FR[s] - cmax * cmin * (UR[s] - UL[s]), den, rden);
-    glb(F.flux[DIR - 1][b * NC + q])[id] = f;
+    st32(F.flux[DIR - 1][b * NC + q], id32, f);
   }
 }
 
@@ -268,9 +270,12 @@ __global__ __launch_bounds__(256) void k_flux(KCtx C, KFields F, double *const *
   double v[NC][NS];
 #pragma unroll
   for (int q = 0; q < NC; ++q) {
-    const auto a = glb(F.prim[b * NC + q]);
+    const double *a = F.prim[b * NC + q];
 #pragma unroll
-    for (int n = 0; n < NS; ++n) v[q][n] = a[id + (n - NS / 2) * st];
+    for (int n = 0; n < NS; ++n)
+      // x1 neighbours fold into the load's immediate offset; x2/x3 offsets
+      // (a row, a plane) into the scalar base
+      v[q][n] = DIR == 1 ? glb(a)[id + (n - NS / 2)] : ld32(a, (n - NS / 2) * st, (uint32_t)id);
   }
 #pragma unroll
   for (int q = 0; q < NC; ++q) {
@@ -801,8 +806,8 @@ __global__ __launch_bounds__(256, (HB && MODE != 1) ? 5 : 4) void k_div_update(K
       T[2][c] = tmunu<2>(s, etot, ptot, c);
       T[3][c] = tmunu<3>(s, etot, ptot, c);
     }
-    const auto conn = glb(G.cen) + (int64_t)CG_CONN * G.ncen + t;
     const bool sym = G.conn_sym != 0;
+    const uint32_t t32 = (uint32_t)t;
 #pragma unroll
     for (int dd = 0; dd < 4; ++dd) {
       double con = 0.0;
@@ -811,7 +816,7 @@ __global__ __launch_bounds__(256, (HB && MODE != 1) ? 5 : 4) void k_div_update(K
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
           const int slot = sym ? 10 * c + conn_pair(dd, r) : 16 * c + 4 * dd + r;
-          con += T[r][c] * conn[(int64_t)slot * G.ncen];
+          con += T[r][c] * ld32(G.cen, (int64_t)(CG_CONN + slot) * G.ncen, t32);
         }
       const int q = (dd == 0) ? 1 : 1 + dd;
       unew[q] += con * m.sg * bdt;
@@ -866,7 +871,7 @@ __device__ __forceinline__ void recover_cell(const KCtx &C, const KFields &F, co
   if (REC >= 0) {
     const bool chainer = (REC != 1);
     bool excised = false;
-    if (P.metric == PANGU_METRIC_CKS && glb(G.cen)[CG_RAD * ns + t] < P.r_excise) excised = true;
+    if (P.metric == PANGU_METRIC_CKS && ld32(G.cen, (int64_t)CG_RAD * ns, (uint32_t)t) < P.r_excise) excised = true;
     if (excised) {
       if (chainer && S.flag) glb(S.flag[b])[id] = 0.0;
       if (S.status) glb(S.status[b])[id] = -1;
@@ -875,12 +880,12 @@ __device__ __forceinline__ void recover_cell(const KCtx &C, const KFields &F, co
       CenMetric m;
 #pragma unroll
       for (int n = 0; n < 16; ++n) {
-        m.gcov[n] = glb(G.cen)[(CG_GCOV + n) * ns + t];
-        m.gcon[n] = glb(G.cen)[(CG_GCON + n) * ns + t];
+        m.gcov[n] = ld32(G.cen, (int64_t)(CG_GCOV + n) * ns, (uint32_t)t);
+        m.gcon[n] = ld32(G.cen, (int64_t)(CG_GCON + n) * ns, (uint32_t)t);
       }
-      m.lapse = glb(G.cen)[CG_LAPSE * ns + t];
-      const double sg = glb(G.cen)[CG_SG * ns + t];
-      const double alpha = glb(G.cen)[CG_ALPHA * ns + t];
+      m.lapse = ld32(G.cen, (int64_t)CG_LAPSE * ns, (uint32_t)t);
+      const double sg = ld32(G.cen, (int64_t)CG_SG * ns, (uint32_t)t);
+      const double alpha = ld32(G.cen, (int64_t)CG_ALPHA * ns, (uint32_t)t);
       double U[9], Ph[9];
       gptr<double> cons[NC];
 #pragma unroll
@@ -1009,11 +1014,11 @@ __device__ __forceinline__ int learned_setup(const KCtx &C, const KFields &F, co
   CenMetric m;
 #pragma unroll
   for (int n = 0; n < 16; ++n) {
-    m.gcov[n] = glb(G.cen)[(CG_GCOV + n) * ns + t];
-    m.gcon[n] = glb(G.cen)[(CG_GCON + n) * ns + t];
+    m.gcov[n] = ld32(G.cen, (int64_t)(CG_GCOV + n) * ns, (uint32_t)t);
+    m.gcon[n] = ld32(G.cen, (int64_t)(CG_GCON + n) * ns, (uint32_t)t);
   }
-  m.lapse = glb(G.cen)[CG_LAPSE * ns + t];
-  const double aos = glb(G.cen)[CG_ALPHA * ns + t] * (1.0 / glb(G.cen)[CG_SG * ns + t]);
+  m.lapse = ld32(G.cen, (int64_t)CG_LAPSE * ns, (uint32_t)t);
+  const double aos = ld32(G.cen, (int64_t)CG_ALPHA * ns, (uint32_t)t) * (1.0 / ld32(G.cen, (int64_t)CG_SG * ns, (uint32_t)t));
   const double *const *cons = F.cons + (int64_t)b * NC;
   const double c_rho = glb(cons[0])[id], c_eny = glb(cons[1])[id];
   double U[9];
@@ -1041,7 +1046,7 @@ __global__ __launch_bounds__(256) void k_recover_learned(KCtx C, KFields F, Shar
   const int64_t id = cell ? ((int64_t)k * B.n[1] + j) * B.n[0] + i : 0;
   const int64_t t = cell ? cen_index(G, k, j, i) : 0;
   const bool active =
-      cell && !(P.metric == PANGU_METRIC_CKS && glb(G.cen)[CG_RAD * G.ncen + t] < P.r_excise);
+      cell && !(P.metric == PANGU_METRIC_CKS && ld32(G.cen, (int64_t)CG_RAD * G.ncen, (uint32_t)t) < P.r_excise);
   // The workgroup votes on the previous iteration counts alone, before any
   // other load: a workgroup with no cell passing the gate then runs as
   // k_recover<1> does, with no further barrier.  The network's cost is per
@@ -1087,7 +1092,7 @@ __global__ __launch_bounds__(256) void k_recover_learned(KCtx C, KFields F, Shar
     if (S.iters) glb(S.iters[b])[id] = 0;
   } else {
     if (HB) {
-      const double isg = 1.0 / glb(G.cen)[CG_SG * G.ncen + t];
+      const double isg = 1.0 / ld32(G.cen, (int64_t)CG_SG * G.ncen, (uint32_t)t);
 #pragma unroll
       for (int a = 0; a < 3; ++a) pp[5 + a][id] = glb(F.cons[b * NC + 5 + a])[id] * isg;
     }
@@ -1136,7 +1141,7 @@ __global__ __launch_bounds__(256) void k_recover_learned_vsq(KCtx C, KFields F, 
   const int64_t id = cell ? ((int64_t)k * B.n[1] + j) * B.n[0] + i : 0;
   const int64_t t = cell ? cen_index(G, k, j, i) : 0;
   const bool active =
-      cell && !(P.metric == PANGU_METRIC_CKS && glb(G.cen)[CG_RAD * G.ncen + t] < P.r_excise);
+      cell && !(P.metric == PANGU_METRIC_CKS && ld32(G.cen, (int64_t)CG_RAD * G.ncen, (uint32_t)t) < P.r_excise);
   int prev = 0;
   if (active && S.iters) prev = glb(S.iters[b])[id];
   const bool gated = active && S.iters && prev >= S.gate;
