@@ -230,11 +230,11 @@ static int build_geometry(pangu_ctx *c, const pangu_block *blocks) {
     G.nk = has3d ? H.n[2] : 1;
     G.has3d = has3d ? 1 : 0;
     G.ncen = (int64_t)G.ni * G.nj * G.nk;
-    // the kernels address a cell of a field array or a table entry through
-    // a 32-bit byte offset
-    if (G.ncen * (int64_t)sizeof(double) >= ((int64_t)1 << 32) ||
+    // the kernels address a cell of a field array, a table entry and the
+    // 64-slot connection block of a table through 32-bit byte offsets
+    if (G.ncen * 64 * (int64_t)sizeof(double) >= ((int64_t)1 << 32) ||
         (int64_t)H.cells * (int64_t)sizeof(double) >= ((int64_t)1 << 32))
-      return fail(c, PANGU_EINVAL, "block too large: more than 2^29 cells");
+      return fail(c, PANGU_EINVAL, "block too large: its geometry table would exceed 2^23 entries");
     std::vector<double> cen((size_t)(G.ncen * CG_NCOMP));
     geom::fill_centre_table(cfg, H, G.ni, G.nj, G.nk, has3d, cen.data());
     // Compact Christoffel storage when the table is symmetric in its lower

@@ -29,7 +29,8 @@ __device__ __forceinline__ gptr<T> glb(T *p) {
 // the base in scalar registers and one 32-bit vector offset shared by
 // neighbouring accesses, instead of a 64-bit address computed per access.
 // The per-thread index times the element size must stay below 4 GiB
-// (pangu_create rejects blocks of more than 2^29 cells).
+// (pangu_create rejects blocks whose geometry tables would exceed 2^23
+// entries, which keeps every index the kernels form within that).
 // ld32(base, u, i) reads base[u + i] where u is the same for the whole
 // workgroup (a stencil offset, a table component) and i the thread's own
 // index: base + u goes to scalar registers, i to the vector offset.
@@ -43,6 +44,23 @@ __device__ __forceinline__ void st32(T *base, uint32_t i, T v) {
   *(__attribute__((address_space(1))) T *)((__attribute__((address_space(1))) char *)base +
                                            i * (uint32_t)sizeof(T)) = v;
 }
+// An array read and written element-wise through ld32 / st32: a[i] reads
+// or assigns element i.
+template <class T>
+struct Ref32 {
+  T *p;
+  uint32_t i;
+  __device__ __forceinline__ operator T() const { return ld32(p, 0, i); }
+  __device__ __forceinline__ const Ref32 &operator=(T v) const {
+    st32(p, i, v);
+    return *this;
+  }
+};
+template <class T>
+struct Arr32 {
+  T *p;
+  __device__ __forceinline__ Ref32<T> operator[](int64_t i) const { return Ref32<T>{p, (uint32_t)i}; }
+};
 #endif
 
 // Static mesh refinement: gather tables built on the host from the block

@@ -662,75 +662,79 @@ __device__ __forceinline__ void flux_div(const DevBlock &B, const KFields &F0, c
   const double vol = B.dx[0] * B.dx[1] * B.dx[2];
   const double rvol = 1.0 / vol;
   double f1p[NC], f1m[NC], f2p[NC], f2m[NC], f3p[NC], f3m[NC], u0[NC];
-  // edge EMFs at offset o from the cell, with the k_emf expressions
-  gptr<double> f1b2 = nullptr, f1b3 = nullptr, f2b1 = nullptr, f2b3 = nullptr, f3b1 = nullptr,
-               f3b2 = nullptr;
+  // Every access is base[u + id]: u is a workgroup-uniform offset (a row
+  // sj, a plane sk, or 1) folded into the scalar base, id the cell.
+  const uint32_t id32 = (uint32_t)id;
+  auto at = [&](const double *p, int64_t u) { return ld32(p, u, id32); };
+  // edge EMFs at uniform offset u from the cell, with the k_emf expressions
+  const double *f1b2 = nullptr, *f1b3 = nullptr, *f2b1 = nullptr, *f2b3 = nullptr, *f3b1 = nullptr,
+               *f3b2 = nullptr;
   if (CTF) {
-    f1b2 = glb(F0.flux[0][b * NC + 6]);
-    f1b3 = glb(F0.flux[0][b * NC + 7]);
+    f1b2 = F0.flux[0][b * NC + 6];
+    f1b3 = F0.flux[0][b * NC + 7];
     if (D2) {
-      f2b1 = glb(F0.flux[1][b * NC + 5]);
-      f2b3 = glb(F0.flux[1][b * NC + 7]);
+      f2b1 = F0.flux[1][b * NC + 5];
+      f2b3 = F0.flux[1][b * NC + 7];
     }
     if (D3) {
-      f3b1 = glb(F0.flux[2][b * NC + 5]);
-      f3b2 = glb(F0.flux[2][b * NC + 6]);
+      f3b1 = F0.flux[2][b * NC + 5];
+      f3b2 = F0.flux[2][b * NC + 6];
     }
   }
-  auto E1 = [&](int64_t o) { return 0.25 * ((f2b3[o] + f2b3[o - sk]) - (f3b2[o] + f3b2[o - sj])); };
-  auto E2 = [&](int64_t o) { return 0.25 * ((f3b1[o] + f3b1[o - 1]) - (f1b3[o] + f1b3[o - sk])); };
-  auto E3 = [&](int64_t o) { return 0.25 * ((f1b2[o] + f1b2[o - sj]) - (f2b1[o] + f2b1[o - 1])); };
+  auto E1 = [&](int64_t u) { return 0.25 * ((at(f2b3, u) + at(f2b3, u - sk)) - (at(f3b2, u) + at(f3b2, u - sj))); };
+  auto E2 = [&](int64_t u) { return 0.25 * ((at(f3b1, u) + at(f3b1, u - 1)) - (at(f1b3, u) + at(f1b3, u - sk))); };
+  auto E3 = [&](int64_t u) { return 0.25 * ((at(f1b2, u) + at(f1b2, u - sj)) - (at(f2b1, u) + at(f2b1, u - 1))); };
 #pragma unroll
   for (int q = 0; q < NC; ++q) {
     const int c = q - 5;  // B component when CTF and 0 <= c < 3
     const bool ctb = CTF && c >= 0 && c < 3;
-    const auto f1 = glb(F0.flux[0][b * NC + q]);
+    const double *f1 = F0.flux[0][b * NC + q];
     if (ctb && c == 0) {  // normal component: zero
       f1p[q] = 0;
       f1m[q] = 0;
     } else if (ctb && c == 1 && D2) {
-      f1m[q] = 0.5 * (E3(id) + E3(id + sj));
-      f1p[q] = 0.5 * (E3(id + 1) + E3(id + 1 + sj));
+      f1m[q] = 0.5 * (E3(0) + E3(sj));
+      f1p[q] = 0.5 * (E3(1) + E3(1 + sj));
     } else if (ctb && c == 2 && D3) {
-      f1m[q] = -0.5 * (E2(id) + E2(id + sk));
-      f1p[q] = -0.5 * (E2(id + 1) + E2(id + 1 + sk));
+      f1m[q] = -0.5 * (E2(0) + E2(sk));
+      f1p[q] = -0.5 * (E2(1) + E2(1 + sk));
     } else {
-      f1p[q] = f1[id + 1];
-      f1m[q] = f1[id];
+      f1p[q] = glb(f1)[id + 1];
+      f1m[q] = glb(f1)[id];
     }
     if (D2) {
-      const auto f2 = glb(F0.flux[1][b * NC + q]);
+      const double *f2 = F0.flux[1][b * NC + q];
       if (ctb && c == 0) {
-        f2m[q] = -0.5 * (E3(id) + E3(id + 1));
-        f2p[q] = -0.5 * (E3(id + sj) + E3(id + sj + 1));
+        f2m[q] = -0.5 * (E3(0) + E3(1));
+        f2p[q] = -0.5 * (E3(sj) + E3(sj + 1));
       } else if (ctb && c == 1) {
         f2p[q] = 0.;
         f2m[q] = 0.;
       } else if (ctb && c == 2 && D3) {
-        f2m[q] = 0.5 * (E1(id) + E1(id + sk));
-        f2p[q] = 0.5 * (E1(id + sj) + E1(id + sj + sk));
+        f2m[q] = 0.5 * (E1(0) + E1(sk));
+        f2p[q] = 0.5 * (E1(sj) + E1(sj + sk));
       } else {
-        f2p[q] = f2[id + sj];
-        f2m[q] = f2[id];
+        f2p[q] = at(f2, sj);
+        f2m[q] = at(f2, 0);
       }
     }
     if (D3) {
-      const auto f3 = glb(F0.flux[2][b * NC + q]);
+      const double *f3 = F0.flux[2][b * NC + q];
       if (ctb && c == 0) {
-        f3m[q] = 0.5 * (E2(id) + E2(id + 1));
-        f3p[q] = 0.5 * (E2(id + sk) + E2(id + sk + 1));
+        f3m[q] = 0.5 * (E2(0) + E2(1));
+        f3p[q] = 0.5 * (E2(sk) + E2(sk + 1));
       } else if (ctb && c == 1 && D2) {
-        f3m[q] = -0.5 * (E1(id) + E1(id + sj));
-        f3p[q] = -0.5 * (E1(id + sk) + E1(id + sk + sj));
+        f3m[q] = -0.5 * (E1(0) + E1(sj));
+        f3p[q] = -0.5 * (E1(sk) + E1(sk + sj));
       } else if (ctb && c == 2) {
         f3p[q] = 0;
         f3m[q] = 0;
       } else {
-        f3p[q] = f3[id + sk];
-        f3m[q] = f3[id];
+        f3p[q] = at(f3, sk);
+        f3m[q] = at(f3, 0);
       }
     }
-    if (MODE == 4) u0[q] = glb(Finit.cons[b * NC + q])[id];
+    if (MODE == 4) u0[q] = at(Finit.cons[b * NC + q], 0);
   }
 #pragma unroll
   for (int q = 0; q < NC; ++q) {
@@ -806,8 +810,12 @@ __global__ __launch_bounds__(256, (HB && MODE != 1) ? 5 : 4) void k_div_update(K
       T[2][c] = tmunu<2>(s, etot, ptot, c);
       T[3][c] = tmunu<3>(s, etot, ptot, c);
     }
+    // one scalar base for the connection block (40 or 64 slots) and a
+    // 32-bit index per slot: per-slot scalar bases would exhaust the scalar
+    // registers
+    const double *conn = G.cen + (int64_t)CG_CONN * G.ncen;
+    const uint32_t ncen32 = (uint32_t)G.ncen, t32 = (uint32_t)t;
     const bool sym = G.conn_sym != 0;
-    const uint32_t t32 = (uint32_t)t;
 #pragma unroll
     for (int dd = 0; dd < 4; ++dd) {
       double con = 0.0;
@@ -816,7 +824,7 @@ __global__ __launch_bounds__(256, (HB && MODE != 1) ? 5 : 4) void k_div_update(K
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
           const int slot = sym ? 10 * c + conn_pair(dd, r) : 16 * c + 4 * dd + r;
-          con += T[r][c] * ld32(G.cen, (int64_t)(CG_CONN + slot) * G.ncen, t32);
+          con += T[r][c] * ld32(conn, 0, (uint32_t)slot * ncen32 + t32);
         }
       const int q = (dd == 0) ? 1 : 1 + dd;
       unew[q] += con * m.sg * bdt;
@@ -830,7 +838,7 @@ __global__ __launch_bounds__(256, (HB && MODE != 1) ? 5 : 4) void k_div_update(K
 // ApplyElectronHeating (electron_heating.cc:66-109) on one recovered cell.
 template <bool HB, bool HK>
 __device__ __forceinline__ void heat_cell(const KCtx &C, const KFields &F, const DevGeo &G, int64_t t,
-                                          int b, int64_t id, const gptr<double> *pp) {
+                                          int b, int64_t id, const Arr32<double> *pp) {
   constexpr int NC = ncomp<HB, HK>();
   double p[NPRIM];
 #pragma unroll
@@ -841,14 +849,15 @@ __device__ __forceinline__ void heat_cell(const KCtx &C, const KFields &F, const
   load_cen_state(G, t, m);
   St s;
   calc_state(p, m.g, m.gc0, m.alpha, s);
-  const double crho = glb(F.cons[b * NC])[id];
+  const uint32_t id32 = (uint32_t)id;
+  const double crho = ld32(F.cons[b * NC], 0, id32);
   HeatIn c;
   c.rho = p[RHO];
   c.energy = p[ENY];
   c.bsq = s.bsq;
-  c.adv_total = glb(F.cons[b * NC + ient<HB>()])[id] / crho;
+  c.adv_total = ld32(F.cons[b * NC + ient<HB>()], 0, id32) / crho;
   c.rec_total = p[ENT];
-  c.adv_e = glb(F.cons[b * NC + ikel<HB>()])[id] / crho;
+  c.adv_e = ld32(F.cons[b * NC + ikel<HB>()], 0, id32) / crho;
   pp[ikel<HB>()][id] = heat_apply(C.ph, c);
 }
 
@@ -865,9 +874,9 @@ __device__ __forceinline__ void recover_cell(const KCtx &C, const KFields &F, co
   const int64_t t = cen_index(G, k, j, i);
   const int64_t ns = G.ncen;
   const Phys &P = C.ph;
-  gptr<double> pp[NC];
+  Arr32<double> pp[NC];
 #pragma unroll
-  for (int q = 0; q < NC; ++q) pp[q] = glb(F.prim[b * NC + q]);
+  for (int q = 0; q < NC; ++q) pp[q] = Arr32<double>{F.prim[b * NC + q]};
   if (REC >= 0) {
     const bool chainer = (REC != 1);
     bool excised = false;
@@ -887,9 +896,9 @@ __device__ __forceinline__ void recover_cell(const KCtx &C, const KFields &F, co
       const double sg = ld32(G.cen, (int64_t)CG_SG * ns, (uint32_t)t);
       const double alpha = ld32(G.cen, (int64_t)CG_ALPHA * ns, (uint32_t)t);
       double U[9], Ph[9];
-      gptr<double> cons[NC];
+      Arr32<double> cons[NC];
 #pragma unroll
-      for (int q = 0; q < NC; ++q) cons[q] = glb(F.cons[b * NC + q]);
+      for (int q = 0; q < NC; ++q) cons[q] = Arr32<double>{F.cons[b * NC + q]};
       const double c_rho = cons[0][id], c_eny = cons[1][id];
       const double isg = 1.0 / sg;
       const double aos = alpha * isg;
@@ -1084,9 +1093,9 @@ __global__ __launch_bounds__(256) void k_recover_learned(KCtx C, KFields F, Shar
     setup = learned_setup<HB, HK>(C, F, G, t, b, id, kin);
   }
   if (!cell) return;
-  gptr<double> pp[NC];
+  Arr32<double> pp[NC];
 #pragma unroll
-  for (int q = 0; q < NC; ++q) pp[q] = glb(F.prim[b * NC + q]);
+  for (int q = 0; q < NC; ++q) pp[q] = Arr32<double>{F.prim[b * NC + q]};
   if (!active) {
     if (S.status) glb(S.status[b])[id] = -1;
     if (S.iters) glb(S.iters[b])[id] = 0;
