@@ -1406,12 +1406,16 @@ __global__ __launch_bounds__(256) void k_bc(KCtx C, KFields F, double *const *fl
 // Per block and active direction the maximum face speed over interior
 // cells; min_cells(cfl dx / c) equals cfl dx / max_cells(c) because IEEE
 // division is monotone.  NaN speeds are skipped, as the reference's
-// `if (dt_min > dt)` reduction skips NaN.  One workgroup per (block, k)
-// plane writes its maxima to red[(b*nk + k)*3 + d]; no atomics, so the
-// result is deterministic.
+// `if (dt_min > dt)` reduction skips NaN.  One workgroup per item (block,
+// interior plane, chunk of kDtChunk cells of the plane) writes its maxima
+// to red[item*3 + d]; no atomics, so the result is deterministic.
 __global__ __launch_bounds__(256) void k_dt_reduce(KCtx C, double *const *alfven, double *red) {
   __shared__ double sm[4][3];
-  // item = (block, interior plane, chunk of kDtChunk cells of the plane)
+  // item = (block, interior plane, chunk of kDtChunk cells of the plane);
+  // thread x takes cells x, x + 256, ... of the chunk, all loads in flight
+  // together (the maximum does not depend on the order)
+  constexpr int kPer = kDtChunk / 256;
+  static_assert(kPer * 256 == kDtChunk, "dt chunk must be a multiple of 256 cells");
   const int b = C.dtblock[blockIdx.x];
   const int local = (int)blockIdx.x - C.dtfirst[b];
   const DevBlock &B = C.blk[b];
@@ -1420,16 +1424,31 @@ __global__ __launch_bounds__(256) void k_dt_reduce(KCtx C, double *const *alfven
   const int kk = local / nch, ch = local - kk * nch;
   const int k = B.s[2] + kk;
   const int t0 = ch * kDtChunk, t1 = (t0 + kDtChunk < ni * nj) ? t0 + kDtChunk : ni * nj;
-  double c[3] = {0.0, 0.0, 0.0};
-  for (int t = t0 + (int)threadIdx.x; t < t1; t += blockDim.x) {
-    const int j = B.s[1] + t / ni, i = B.s[0] + t % ni;
-    const int64_t id = ((int64_t)k * B.n[1] + j) * B.n[0] + i;
-    for (int d = 0; d < 3; ++d) {
-      if (B.n[d] == 1) continue;
-      const double v = glb(alfven[b * 3 + d])[id];
-      if (v > c[d]) c[d] = v;
+  // (j, i) of cell t, stepped by 256 cells without a division per step
+  const int dj = 256 / ni, di = 256 - dj * ni;
+  int t = t0 + (int)threadIdx.x;
+  int j = t / ni, i = t - j * ni;
+  double v[kPer][3];
+#pragma unroll
+  for (int r = 0; r < kPer; ++r) {
+    const int64_t id = ((int64_t)k * B.n[1] + (B.s[1] + j)) * B.n[0] + (B.s[0] + i);
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      v[r][d] = (t < t1 && B.n[d] != 1) ? glb(alfven[b * 3 + d])[id] : 0.0;
+    t += 256;
+    i += di;
+    j += dj;
+    if (i >= ni) {
+      i -= ni;
+      ++j;
     }
   }
+  double c[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int r = 0; r < kPer; ++r)
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      if (v[r][d] > c[d]) c[d] = v[r][d];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int d = 0; d < 3; ++d) {
     double v = c[d];
@@ -1462,10 +1481,15 @@ __global__ __launch_bounds__(256) void k_dt_final(KCtx C, const double *red, dou
     double q3[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
     for (int d = 0; d < 3; ++d) {
       if (B.n[d] == 1) continue;
+      // the block's items, 16 independent loads at a time
       double cmax = 0.0;
-      for (int p = C.dtfirst[b]; p < C.dtfirst[b + 1]; ++p) {
-        const double v = red[(int64_t)p * 3 + d];
-        cmax = (v > cmax) ? v : cmax;
+      const int p1 = C.dtfirst[b + 1];
+      for (int p0 = C.dtfirst[b]; p0 < p1; p0 += 16) {
+        double w[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) w[u] = (p0 + u < p1) ? red[(int64_t)(p0 + u) * 3 + d] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) cmax = (w[u] > cmax) ? w[u] : cmax;
       }
       const double q = C.ph.cfl * B.dx[d] / cmax;
       q3[d] = (q < DBL_MAX) ? q : DBL_MAX;
