@@ -582,13 +582,20 @@ static int build_tables(pangu_ctx *c) {
   L.k.pblock = pb;
   L.k.pfirst = pf;
   L.k.nplanes = np;
-  // dt reduction items: every interior plane in chunks of kDtChunk cells,
+  // dt reduction items: every interior plane in chunks of 256 * dtper cells,
   // so that a large single-block plane is scanned by many workgroups
+  int64_t maxplane = 1;
+  for (int b = 0; b < c->nblocks; ++b) {
+    const HostBlock &H = c->hb[b];
+    maxplane = std::max(maxplane, (int64_t)(H.e[0] - H.s[0] + 1) * (H.e[1] - H.s[1] + 1));
+  }
+  L.k.dtper = dt_per_thread(maxplane);
+  const int64_t chunk = 256 * (int64_t)L.k.dtper;
   std::vector<int64_t> dtitems(c->nblocks);
   for (int b = 0; b < c->nblocks; ++b) {
     const HostBlock &H = c->hb[b];
     const int64_t plane = (int64_t)(H.e[0] - H.s[0] + 1) * (H.e[1] - H.s[1] + 1);
-    dtitems[b] = planes[b] * ((plane + kDtChunk - 1) / kDtChunk);
+    dtitems[b] = planes[b] * ((plane + chunk - 1) / chunk);
   }
   const int *db = nullptr, *dfi = nullptr;
   int nd = 0;

@@ -21,8 +21,9 @@ struct KCtx {
   int nplanes;         // interior (block, k) planes over all blocks
   const int *pblock;   // plane -> block
   const int *pfirst;   // first plane of each block (nblocks + 1 entries)
-  // dt reduction items: each interior plane in chunks of kDtChunk cells
+  // dt reduction items: each interior plane in chunks of 256 * dtper cells
   int ndt;
+  int dtper;           // cells per thread of one item: 1, 4 or 16
   const int *dtblock;  // item -> block
   const int *dtfirst;  // first item of each block (nblocks + 1 entries)
   const int *mfirst;   // first block of each mesh (nmesh + 1 entries)
@@ -107,9 +108,13 @@ struct MultiLevel {
   int nfill = 0, nprol = 0, nflux = 0;
 };
 
-// Cells of one interior plane that one workgroup of the dt reduction
-// scans (a 64x64 plane is one chunk; a 512x256 plane 32).
-constexpr int kDtChunk = 4096;
+// Cells per thread of one dt reduction item for a mesh whose largest
+// interior plane holds `plane` cells: an item (one workgroup of 256 threads)
+// scans 256 * dt_per_thread(plane) cells of a plane, at most 4096 (a 64x64
+// plane is one item, a 512x256 plane 32), with that many loads per thread
+// in flight together; a mesh of small planes keeps the register count (and
+// the workgroups resident) of one load per thread.
+inline int dt_per_thread(int64_t plane) { return plane <= 256 ? 1 : (plane <= 1024 ? 4 : 16); }
 
 // Host-side launch context: kernel constants plus the launch ranges.
 struct LaunchCtx {

@@ -1407,15 +1407,14 @@ __global__ __launch_bounds__(256) void k_bc(KCtx C, KFields F, double *const *fl
 // cells; min_cells(cfl dx / c) equals cfl dx / max_cells(c) because IEEE
 // division is monotone.  NaN speeds are skipped, as the reference's
 // `if (dt_min > dt)` reduction skips NaN.  One workgroup per item (block,
-// interior plane, chunk of kDtChunk cells of the plane) writes its maxima
+// interior plane, chunk of 256 * PER cells of the plane) writes its maxima
 // to red[item*3 + d]; no atomics, so the result is deterministic.
+template <int PER>
 __global__ __launch_bounds__(256) void k_dt_reduce(KCtx C, double *const *alfven, double *red) {
   __shared__ double sm[4][3];
-  // item = (block, interior plane, chunk of kDtChunk cells of the plane);
   // thread x takes cells x, x + 256, ... of the chunk, all loads in flight
   // together (the maximum does not depend on the order)
-  constexpr int kPer = kDtChunk / 256;
-  static_assert(kPer * 256 == kDtChunk, "dt chunk must be a multiple of 256 cells");
+  constexpr int kPer = PER, kDtChunk = 256 * PER;
   const int b = C.dtblock[blockIdx.x];
   const int local = (int)blockIdx.x - C.dtfirst[b];
   const DevBlock &B = C.blk[b];
@@ -1428,13 +1427,16 @@ __global__ __launch_bounds__(256) void k_dt_reduce(KCtx C, double *const *alfven
   const int dj = 256 / ni, di = 256 - dj * ni;
   int t = t0 + (int)threadIdx.x;
   int j = t / ni, i = t - j * ni;
+  // steps holding a cell for some thread (the same for the whole workgroup:
+  // a small plane skips the rest)
+  const int nr = (t1 - t0 + 255) / 256;
   double v[kPer][3];
 #pragma unroll
   for (int r = 0; r < kPer; ++r) {
     const int64_t id = ((int64_t)k * B.n[1] + (B.s[1] + j)) * B.n[0] + (B.s[0] + i);
 #pragma unroll
     for (int d = 0; d < 3; ++d)
-      v[r][d] = (t < t1 && B.n[d] != 1) ? glb(alfven[b * 3 + d])[id] : 0.0;
+      v[r][d] = (r < nr && t < t1 && B.n[d] != 1) ? glb(alfven[b * 3 + d])[id] : 0.0;
     t += 256;
     i += di;
     j += dj;
@@ -1779,7 +1781,12 @@ int launch_bc(const LaunchCtx &L, const KFields &F, double *const *flag, hipStre
 
 int launch_dt(const LaunchCtx &L, double *const *alfven, double *red, double *state,
               double dt_factor, double *est_out, hipStream_t s) {
-  hipLaunchKernelGGL(k_dt_reduce, dim3(L.k.ndt), dim3(256), 0, s, L.k, alfven, red);
+  if (L.k.dtper == 1)
+    hipLaunchKernelGGL(k_dt_reduce<1>, dim3(L.k.ndt), dim3(256), 0, s, L.k, alfven, red);
+  else if (L.k.dtper == 4)
+    hipLaunchKernelGGL(k_dt_reduce<4>, dim3(L.k.ndt), dim3(256), 0, s, L.k, alfven, red);
+  else
+    hipLaunchKernelGGL(k_dt_reduce<16>, dim3(L.k.ndt), dim3(256), 0, s, L.k, alfven, red);
   TICK("dt_reduce");
   hipLaunchKernelGGL(k_dt_final, dim3(L.k.nmesh), dim3(256), 0, s, L.k, red, state, dt_factor,
                      est_out);
