@@ -921,15 +921,19 @@ __device__ __forceinline__ void recover_cell(const KCtx &C, const KFields &F, co
         U[BX3] = 0.0;
       }
       U[ENT] = aos * cons[ient<HB>()][id];
-      Ph[RHO] = pp[0][id];
-      Ph[ENY] = pp[1][id];
-      Ph[UX1] = pp[2][id];
-      Ph[UX2] = pp[3][id];
-      Ph[UX3] = pp[4][id];
-      Ph[BX1] = 0.0;
-      Ph[BX2] = 0.0;
-      Ph[BX3] = 0.0;
-      Ph[ENT] = pp[ient<HB>()][id];
+      // the Newton schemes start from the previous primitives; Kastaun's
+      // bracket needs none (it writes rho, u and u~ only on success), so
+      // their loads are skipped
+#pragma unroll
+      for (int s = 0; s < 9; ++s) Ph[s] = 0.0;
+      if (REC != 1) {
+        Ph[RHO] = pp[0][id];
+        Ph[ENY] = pp[1][id];
+        Ph[UX1] = pp[2][id];
+        Ph[UX2] = pp[3][id];
+        Ph[UX3] = pp[4][id];
+        Ph[ENT] = pp[ient<HB>()][id];
+      }
       int st = 0, it = 0, itn = 0;
       bool write = true;
       if (REC == 1) {
