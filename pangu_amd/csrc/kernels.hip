@@ -835,16 +835,14 @@ __global__ __launch_bounds__(256, (HB && MODE != 1) ? 5 : 4) void k_div_update(K
 }
 
 // -------------------------------------------------- recovery (+ heating)
-// ApplyElectronHeating (electron_heating.cc:66-109) on one recovered cell.
+// ApplyElectronHeating (electron_heating.cc:66-109) on one recovered cell
+// from its primitives p (rho, u, u~, B and the stale entropy; K_e is not
+// read); the centre metric is loaded here.
 template <bool HB, bool HK>
-__device__ __forceinline__ void heat_cell(const KCtx &C, const KFields &F, const DevGeo &G, int64_t t,
-                                          int b, int64_t id, const Arr32<double> *pp) {
+__device__ __forceinline__ void heat_vals(const KCtx &C, const KFields &F, const DevGeo &G, int64_t t,
+                                          int b, int64_t id, const Arr32<double> *pp,
+                                          const double p[NPRIM]) {
   constexpr int NC = ncomp<HB, HK>();
-  double p[NPRIM];
-#pragma unroll
-  for (int s = 0; s < NPRIM; ++s) p[s] = 0.0;
-#pragma unroll
-  for (int q = 0; q < NC; ++q) p[slot_of<HB, HK>(q)] = pp[q][id];
   CenLoad m;
   load_cen_state(G, t, m);
   St s;
@@ -859,6 +857,19 @@ __device__ __forceinline__ void heat_cell(const KCtx &C, const KFields &F, const
   c.rec_total = p[ENT];
   c.adv_e = ld32(F.cons[b * NC + ikel<HB>()], 0, id32) / crho;
   pp[ikel<HB>()][id] = heat_apply(C.ph, c);
+}
+
+// The same with the primitives read from memory.
+template <bool HB, bool HK>
+__device__ __forceinline__ void heat_cell(const KCtx &C, const KFields &F, const DevGeo &G, int64_t t,
+                                          int b, int64_t id, const Arr32<double> *pp) {
+  constexpr int NC = ncomp<HB, HK>();
+  double p[NPRIM];
+#pragma unroll
+  for (int s = 0; s < NPRIM; ++s) p[s] = 0.0;
+#pragma unroll
+  for (int q = 0; q < NC; ++q) p[slot_of<HB, HK>(q)] = pp[q][id];
+  heat_vals<HB, HK>(C, F, G, t, b, id, pp, p);
 }
 
 // Recovery (invertor.cc:184-319) and, when HEAT, ApplyElectronHeating
@@ -895,7 +906,7 @@ __device__ __forceinline__ void recover_cell(const KCtx &C, const KFields &F, co
       m.lapse = ld32(G.cen, (int64_t)CG_LAPSE * ns, (uint32_t)t);
       const double sg = ld32(G.cen, (int64_t)CG_SG * ns, (uint32_t)t);
       const double alpha = ld32(G.cen, (int64_t)CG_ALPHA * ns, (uint32_t)t);
-      double U[9], Ph[9];
+      double U[9], Ph[9], bnew[3] = {0.0, 0.0, 0.0};
       Arr32<double> cons[NC];
 #pragma unroll
       for (int q = 0; q < NC; ++q) cons[q] = Arr32<double>{F.cons[b * NC + q]};
@@ -912,9 +923,12 @@ __device__ __forceinline__ void recover_cell(const KCtx &C, const KFields &F, co
         U[BX1] = aos * cb1;
         U[BX2] = aos * cb2;
         U[BX3] = aos * cb3;
-        pp[5][id] = cb1 * isg;
-        pp[6][id] = cb2 * isg;
-        pp[7][id] = cb3 * isg;
+        bnew[0] = cb1 * isg;
+        bnew[1] = cb2 * isg;
+        bnew[2] = cb3 * isg;
+        pp[5][id] = bnew[0];
+        pp[6][id] = bnew[1];
+        pp[7][id] = bnew[2];
       } else {
         U[BX1] = 0.0;
         U[BX2] = 0.0;
@@ -971,6 +985,35 @@ __device__ __forceinline__ void recover_cell(const KCtx &C, const KFields &F, co
         pp[2][id] = Ph[UX1];
         pp[3][id] = Ph[UX2];
         pp[4][id] = Ph[UX3];
+      }
+      if (HEAT && HK) {
+        // the heating reads the primitives just written from registers (the
+        // compiler cannot forward them: the stores may alias other arrays);
+        // a cell whose recovery failed keeps its previous primitives
+        double hp[NPRIM];
+#pragma unroll
+        for (int s = 0; s < NPRIM; ++s) hp[s] = 0.0;
+        if (write) {
+          hp[RHO] = Ph[RHO];
+          hp[ENY] = Ph[ENY];
+          hp[UX1] = Ph[UX1];
+          hp[UX2] = Ph[UX2];
+          hp[UX3] = Ph[UX3];
+        } else {
+          hp[RHO] = pp[0][id];
+          hp[ENY] = pp[1][id];
+          hp[UX1] = pp[2][id];
+          hp[UX2] = pp[3][id];
+          hp[UX3] = pp[4][id];
+        }
+        if (HB) {
+          hp[BX1] = bnew[0];
+          hp[BX2] = bnew[1];
+          hp[BX3] = bnew[2];
+        }
+        hp[ENT] = pp[ient<HB>()][id];
+        heat_vals<HB, HK>(C, F, G, t, b, id, pp, hp);
+        return;
       }
     }
   }
