@@ -70,6 +70,11 @@ MATRIX = {
     "ot_minkowski_1d": dict(configs.orszag_tang(64, (2, 1, 1)), nx=[64, 1, 1]),
     "torus3d_odd_blocks": configs.torus3d((40, 24, 12), (2, 2, 1)),
     "torus3d_exchange_after_recovery": _torus3d(exchange_after=1),
+    # dt reduction with a partial last step: 20x30 planes (600 cells, four
+    # cells per thread, the third step part-filled) and one 50x30 plane
+    # (1500 cells, sixteen per thread, six steps used)
+    "torus3d_dt_600_cell_planes": configs.torus3d((40, 30, 8), (2, 1, 1)),
+    "torus2d_dt_1500_cell_plane": configs.torus2d(50, 30, (1, 1, 1)),
 }
 
 
