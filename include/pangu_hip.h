@@ -271,6 +271,18 @@ int pangu_set_persistent_recovery(pangu_ctx *ctx, int on);
  * fluxes), as the task-level entry points always do.  On a refined mesh it
  * always runs first, because flux correction overwrites some face fluxes. */
 int pangu_set_fuse_ct(pangu_ctx *ctx, int fuse);
+/* on = 1: with the exchange after the update (exchange_after 0) and
+ * distinct input and output containers, pangu_stage runs the ghost
+ * exchange of the output container on a second stream of the context from
+ * the start of the stage, beside FixPrimitive, the flux sweeps and the
+ * update, and the stage's stream waits for it before the recovery.  That
+ * exchange reads the output container's interior P and flag and writes its
+ * ghosts, which nothing before the recovery writes or reads, so results are
+ * identical.  0 runs it on the stage's stream after the update.  -1
+ * (default): 1 on a refined mesh, whose exchange is a large share of the
+ * stage, 0 otherwise (on small meshes the cross-stream join costs more than
+ * the overlap hides).  The task-level entry points are not affected. */
+int pangu_set_overlap_exchange(pangu_ctx *ctx, int on);
 
 /* ---- run diagnostics (no reference counterpart) ----
  * doc/grmhd_intro.md:556-574 defines the fluxes through a sphere that a

@@ -69,6 +69,10 @@ struct pangu_ctx {
   // the device's static diagnostics workspace (pangu_diagnostics_noalloc)
   double *dscratch = nullptr;
   std::string scratch_err;
+  // second stream of the fused stage (default priority) and its fork / join
+  // events: the ghost exchange of the output container runs there
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   LaunchCtx L;
   std::string err;
 };
@@ -712,6 +716,14 @@ int pangu_create_batch(const pangu_config *cfgs, int nmesh, const pangu_block *b
       c->scratch_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
     }
   }
+  if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+    c->err = "pangu_create: side stream or events";
+    fprintf(stderr, "%s\n", c->err.c_str());
+    pangu_destroy(c);
+    return PANGU_EDEVICE;
+  }
   KCtx &k = c->L.k;
   k.blk = c->dblk;
   k.geo = c->dgeo;
@@ -776,6 +788,12 @@ int pangu_destroy(pangu_ctx *c) {
                   c->ddiag_shell, c->dxfill, c->dxprol, c->dxflux};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  if (c->side) {
+    (void)hipStreamSynchronize(c->side);
+    (void)hipStreamDestroy(c->side);
+  }
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   delete c;
   return PANGU_OK;
 }
@@ -791,6 +809,13 @@ int pangu_set_tile_order(pangu_ctx *c, int fam, int order) {
 int pangu_set_fuse_ct(pangu_ctx *c, int fuse) {
   if (!c) return PANGU_EINVAL;
   c->L.fuse_ct = fuse != 0;
+  return PANGU_OK;
+}
+
+int pangu_set_overlap_exchange(pangu_ctx *c, int on) {
+  if (!c) return PANGU_EINVAL;
+  if (on < -1 || on > 1) return fail(c, PANGU_EINVAL, "overlap_exchange: -1, 0 or 1");
+  c->L.overlap_exchange = on;
   return PANGU_OK;
 }
 
@@ -1111,6 +1136,23 @@ int pangu_stage(pangu_ctx *c, int md_init, int md0, int md1, double beta, double
   if (!dt_state) return fail(c, PANGU_EINVAL, "dt_state is null");
   hipStream_t s = STREAM(stream);
   const KFields f0 = c->conts[md0].kf(), fi = c->conts[md_init].kf(), f1 = c->conts[md1].kf();
+  // The exchange after the update copies interior P and flag of the output
+  // container into its ghosts.  Nothing before the recovery writes that
+  // interior P (the update writes U, FixPrimitive the input container's P)
+  // or reads those ghosts or the flag, so with distinct input and output
+  // containers it runs on the side stream from the start of the stage,
+  // beside FixPrimitive and the flux sweeps, and joins before the recovery.
+  // (Sequential when a kernel clock times the launches on one stream.)  By
+  // default only on a refined mesh: elsewhere the cross-stream join costs
+  // more than the overlap hides.
+  const bool want = c->L.overlap_exchange > 0 || (c->L.overlap_exchange < 0 && c->L.ml.on);
+  const bool overlap = exchange_after == 0 && md1 != md0 && want && c->side && !c->L.clock;
+  if (overlap) {
+    HIPCHK(c, hipEventRecord(c->ev_fork, s));
+    HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    LCHK(c, exchange(c, f1, c->side));
+    HIPCHK(c, hipEventRecord(c->ev_join, c->side));
+  }
   LCHK(c, launch_fix_cons(c->L, f0, c->dmask, true, true, s));
   LCHK(c, launch_fluxes(c->L, f0, c->dalfven, s));
   // With fuse_ct the update kernel forms the edge EMFs and the CT face B
@@ -1127,7 +1169,10 @@ int pangu_stage(pangu_ctx *c, int md_init, int md0, int md1, double beta, double
   } else {
     LCHK(c, launch_div_update(c->L, 4, f0, fi, f1, nullptr, fuse_ct, dt_state, beta, 0.0, s));
   }
-  if (exchange_after == 0) LCHK(c, exchange(c, f1, s));
+  if (overlap)
+    HIPCHK(c, hipStreamWaitEvent(s, c->ev_join, 0));
+  else if (exchange_after == 0)
+    LCHK(c, exchange(c, f1, s));
   MlpFrag W;
   LCHK(c, launch_recover(c->L, f1, shared_tabs(c), c->cfg.recovery, c->cfg.enable_heating != 0,
                          learned(c, W), s));

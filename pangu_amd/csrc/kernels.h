@@ -126,6 +126,7 @@ struct LaunchCtx {
   int order[PANGU_KF_COUNT] = {};  // tile order per kernel family (PANGU_ORDER_*)
   int split_source = 0;            // fused stage: source term in a kernel of its own
   int fuse_ct = 1;                 // fused stage: CT face fluxes formed in the update kernel
+  int overlap_exchange = -1;       // fused stage: ghost exchange on the side stream (-1: refined meshes)
   int share_x1 = 0;                // X1 sweep reconstructs each cell once (k_flux_x1s)
   int pair_flux = 0;               // bit d-1: the sweep along d runs two lanes per face (k_flux_pair)
   Range3 march[2];                 // marching ranges of the X2 and X3 sweeps (k_flux_march)

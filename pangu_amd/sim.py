@@ -117,6 +117,7 @@ def lib():
         L.pangu_sim_set_tile_order.argtypes = [vp, C.c_int, C.c_int]
         L.pangu_sim_set_split_source.argtypes = [vp, C.c_int]
         L.pangu_sim_set_fuse_ct.argtypes = [vp, C.c_int]
+        L.pangu_sim_set_overlap_exchange.argtypes = [vp, C.c_int]
         L.pangu_sim_set_share_reconstruction.argtypes = [vp, C.c_int]
         L.pangu_sim_time_kernels.argtypes = [vp, C.c_int, dp, ip, C.c_int]
         L.pangu_sim_kernel_name.restype = C.c_char_p
@@ -338,6 +339,13 @@ class HipSim:
         update kernel (1, default) or written by their own kernel first (0).
         Results are unchanged."""
         self._check(lib().pangu_sim_set_fuse_ct(self._p, int(fuse)), "set_fuse_ct")
+
+    def set_overlap_exchange(self, on):
+        """Fused stage: the ghost exchange after the update runs on a second
+        stream from the start of the stage (1) or on the stage's stream after
+        the update (0); -1 (default): 1 on a refined mesh only.  Results are
+        unchanged."""
+        self._check(lib().pangu_sim_set_overlap_exchange(self._p, int(on)), "set_overlap_exchange")
 
     def set_share_reconstruction(self, share):
         """X1 sweep: one reconstruction per cell shared between its two faces
