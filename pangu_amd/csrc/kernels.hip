@@ -292,6 +292,15 @@ __global__ __launch_bounds__(256) void k_flux(KCtx C, KFields F, double *const *
   face_flux<DIR, RS, HB, HK>(C, F, alfven, b, k, j, i, pl, pr);
 }
 
+// The value of lane l - 1 (lane 0 gets 0) through the DPP wavefront shift
+// (wave_shr:1), 32 bits at a time.
+__device__ __forceinline__ double wave_shr1(double x) {
+  const uint64_t u = (uint64_t)__double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, 0x138, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), 0x138, 0xf, 0xf, false);
+  return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
 // Exchange with the other lane of an aligned lane pair (l ^ 1) through the
 // data-parallel-primitive path (quad_perm [1, 0, 3, 2]), 32 bits at a time.
 __device__ __forceinline__ double pair_swap(double x) {
@@ -474,17 +483,25 @@ __global__ __launch_bounds__(256) void k_flux_x1s(KCtx C, KFields F, double *con
     pl[s] = 0.0;
     pr[s] = 0.0;
   }
+  // the cell stencils of every variable are loaded before the first
+  // reconstruction (one wait for memory), then each cell's right interface
+  // value moves one lane up through a DPP wavefront shift
+  constexpr int NS = (LIM == PANGU_LIMITER_PPM4) ? 5 : 3;  // offsets -NS/2 .. NS/2
+  double v[NC][NS];
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
+    const auto a = glb(F.prim[b * NC + q]);
+#pragma unroll
+    for (int n = 0; n < NS; ++n) v[q][n] = have ? a[id + (n - NS / 2)] : 0.0;
+  }
 #pragma unroll
   for (int q = 0; q < NC; ++q) {
     const int s = slot_of<HB, HK>(q);
-    if (have) {
-      const auto a = glb(F.prim[b * NC + q]);
-      if (LIM == PANGU_LIMITER_PPM4)
-        recon_ppm4(a[id - 2], a[id - 1], a[id], a[id + 1], a[id + 2], rv[s], pr[s]);
-      else
-        recon_mc(a[id - 1], a[id], a[id + 1], rv[s], pr[s]);
-    }
-    pl[s] = __shfl_up(rv[s], 1, 64);
+    if (LIM == PANGU_LIMITER_PPM4)
+      recon_ppm4(v[q][0], v[q][1], v[q][2], v[q][3], v[q][4], rv[s], pr[s]);
+    else
+      recon_mc(v[q][0], v[q][1], v[q][2], rv[s], pr[s]);
+    pl[s] = wave_shr1(rv[s]);
   }
   if (!have || lane == 0 || ic == 0) return;
   face_flux<1, RS, HB, HK>(C, F, alfven, b, k, j, i, pl, pr);
