@@ -283,6 +283,16 @@ int pangu_set_fuse_ct(pangu_ctx *ctx, int fuse);
  * stage, 0 otherwise (on small meshes the cross-stream join costs more than
  * the overlap hides).  The task-level entry points are not affected. */
 int pangu_set_overlap_exchange(pangu_ctx *ctx, int on);
+/* on = 1 (default): pangu_stage runs the flux-divergence update (with the
+ * geometric source) and the recovery (+ heating) of each cell in one
+ * kernel, the recovery taking the new U from registers; the stage's
+ * exchange (after the update) then runs before that kernel, which changes
+ * nothing.  Applies with the chainer, Kastaun or 1Dvsq recovery, no split
+ * source, no learned start and no persistent recovery, and on a magnetised
+ * mesh with the CT face fluxes formed in the update (pangu_set_fuse_ct);
+ * the separate kernels run otherwise.  Same operations per cell: results
+ * are identical.  0: separate kernels. */
+int pangu_set_fuse_update_recovery(pangu_ctx *ctx, int on);
 
 /* ---- run diagnostics (no reference counterpart) ----
  * doc/grmhd_intro.md:556-574 defines the fluxes through a sphere that a

@@ -156,6 +156,8 @@ int pangu_sim_set_split_source(pangu_sim *s, int split);
 int pangu_sim_set_fuse_ct(pangu_sim *s, int fuse);
 /* pangu_set_overlap_exchange on the simulation's context (drops the graph). */
 int pangu_sim_set_overlap_exchange(pangu_sim *s, int on);
+/* pangu_set_fuse_update_recovery on the simulation's context (drops the graph). */
+int pangu_sim_set_fuse_update_recovery(pangu_sim *s, int on);
 /* pangu_set_share_reconstruction on the simulation's context (drops the graph). */
 int pangu_sim_set_share_reconstruction(pangu_sim *s, int share);
 int pangu_sim_set_pair_flux(pangu_sim *s, int dir_mask);

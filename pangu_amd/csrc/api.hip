@@ -812,6 +812,12 @@ int pangu_set_fuse_ct(pangu_ctx *c, int fuse) {
   return PANGU_OK;
 }
 
+int pangu_set_fuse_update_recovery(pangu_ctx *c, int on) {
+  if (!c) return PANGU_EINVAL;
+  c->L.fuse_update_recovery = on != 0;
+  return PANGU_OK;
+}
+
 int pangu_set_overlap_exchange(pangu_ctx *c, int on) {
   if (!c) return PANGU_EINVAL;
   if (on < -1 || on > 1) return fail(c, PANGU_EINVAL, "overlap_exchange: -1, 0 or 1");
@@ -1163,19 +1169,35 @@ int pangu_stage(pangu_ctx *c, int md_init, int md0, int md1, double beta, double
   const bool fuse_ct = c->cfg.enable_B && c->L.fuse_ct && !c->L.ml.on;
   if (c->cfg.enable_B && !fuse_ct) LCHK(c, launch_ct(c->L, f0, c->demf, s));
   if (c->L.ml.on) LCHK(c, launch_flux_correction(c->L, f0, s));
-  if (c->L.split_source && c->cfg.metric != PANGU_METRIC_MINKOWSKI) {
+  MlpFrag W;
+  const MlpFrag *Wl = learned(c, W);
+  // update and recovery in one kernel: the stage's exchange (after the
+  // update) then runs before it, which changes nothing (it touches neither
+  // U nor the update's inputs)
+  const bool hb = c->L.k.iBX >= 0, hk = c->L.k.iKEL >= 0;
+  const bool fuse_ur = c->L.fuse_update_recovery && (hb ? fuse_ct : !hk) && !c->L.split_source && !Wl &&
+                       !c->L.persist_recovery && c->cfg.recovery >= 0 && c->cfg.recovery <= 2;
+  if (fuse_ur) {
+    if (overlap)
+      HIPCHK(c, hipStreamWaitEvent(s, c->ev_join, 0));
+    else if (exchange_after == 0)
+      LCHK(c, exchange(c, f1, s));
+    LCHK(c, launch_update_recover(c->L, f0, fi, f1, shared_tabs(c), c->cfg.recovery,
+                                  c->cfg.enable_heating != 0, fuse_ct, dt_state, beta, 0.0, s));
+  } else if (c->L.split_source && c->cfg.metric != PANGU_METRIC_MINKOWSKI) {
     LCHK(c, launch_div_update(c->L, 5, f0, fi, f1, nullptr, fuse_ct, dt_state, beta, 0.0, s));
     LCHK(c, launch_div_update(c->L, 3, f0, fi, f1, nullptr, false, dt_state, beta, 0.0, s));
   } else {
     LCHK(c, launch_div_update(c->L, 4, f0, fi, f1, nullptr, fuse_ct, dt_state, beta, 0.0, s));
   }
-  if (overlap)
-    HIPCHK(c, hipStreamWaitEvent(s, c->ev_join, 0));
-  else if (exchange_after == 0)
-    LCHK(c, exchange(c, f1, s));
-  MlpFrag W;
-  LCHK(c, launch_recover(c->L, f1, shared_tabs(c), c->cfg.recovery, c->cfg.enable_heating != 0,
-                         learned(c, W), s));
+  if (!fuse_ur) {
+    if (overlap)
+      HIPCHK(c, hipStreamWaitEvent(s, c->ev_join, 0));
+    else if (exchange_after == 0)
+      LCHK(c, exchange(c, f1, s));
+    LCHK(c, launch_recover(c->L, f1, shared_tabs(c), c->cfg.recovery, c->cfg.enable_heating != 0, Wl,
+                           s));
+  }
   if (c->cfg.recovery != PANGU_RECOVERY_ROBUSTER)
     LCHK(c, launch_fix_recovery(c->L, f1, c->dflag, c->dcluster, s));
   if (exchange_after != 0) LCHK(c, exchange(c, f1, s));

@@ -127,6 +127,7 @@ struct LaunchCtx {
   int split_source = 0;            // fused stage: source term in a kernel of its own
   int fuse_ct = 1;                 // fused stage: CT face fluxes formed in the update kernel
   int overlap_exchange = -1;       // fused stage: ghost exchange on the side stream (-1: refined meshes)
+  int fuse_update_recovery = 1;    // fused stage: update and recovery in one kernel
   int share_x1 = 0;                // X1 sweep reconstructs each cell once (k_flux_x1s)
   int pair_flux = 0;               // bit d-1: the sweep along d runs two lanes per face (k_flux_pair)
   Range3 march[2];                 // marching ranges of the X2 and X3 sweeps (k_flux_march)
@@ -175,6 +176,13 @@ int launch_div_update(const LaunchCtx &L, int mode, const KFields &F0, const KFi
 // search from the learned bracket.
 int launch_recover(const LaunchCtx &L, const KFields &F, const SharedTabs &S, int rec, bool heat,
                    const MlpFrag *W, hipStream_t s);
+// The mode-4 update and the recovery (+ heating) of each cell in one kernel
+// (k_update_recover): the chainer, Kastaun or 1Dvsq recovery, with the CT
+// face fluxes formed in the update on a magnetised mesh, and K_e only with
+// a field (-1 otherwise).
+int launch_update_recover(const LaunchCtx &L, const KFields &F0, const KFields &Finit,
+                          const KFields &F1, const SharedTabs &S, int rec, bool heat, bool ct_fused,
+                          const double *dt_dev, double beta, double bdt_host, hipStream_t s);
 // Training pairs from a container: U in the recovery's normalisation and
 // the cell's metric sample (35 doubles) of every stride-th interior cell
 // (stride >= 1), solved by the plain Kastaun recovery (kind 1: then by

@@ -769,13 +769,12 @@ __device__ __forceinline__ void flux_div(const DevBlock &B, const KFields &F0, c
 // Five waves per SIMD is the most the magnetised source-term variants reach
 // without spilling (the field-free ones spill at five and run at four); the
 // kernel is memory bound, so occupancy pays.
+// One cell of k_div_update; the new U also goes to uout (when given).
 template <int MODE, bool SRC, bool HB, bool HK, bool CTF>
-__global__ __launch_bounds__(256, (HB && MODE != 1) ? 5 : 4) void k_div_update(KCtx C, KFields F0, KFields Finit, KFields F1,
-                                                    double *const *dudt,
-                                                    const double *dt_dev, double beta,
-                                                    double bdt_host, Range3 R) {
-  int b, k, j, i;
-  if (!decode_tile(C, R, b, k, j, i)) return;
+__device__ __forceinline__ void div_update_cell(const KCtx &C, const KFields &F0, const KFields &Finit,
+                                                const KFields &F1, double *const *dudt,
+                                                const double *dt_dev, double beta, double bdt_host,
+                                                int b, int k, int j, int i, double *uout) {
   const DevBlock &B = C.blk[b];
   const int64_t sj = B.n[0], sk = (int64_t)B.n[0] * B.n[1];
   const int64_t id = k * sk + j * sj + i;
@@ -849,6 +848,21 @@ __global__ __launch_bounds__(256, (HB && MODE != 1) ? 5 : 4) void k_div_update(K
   }
 #pragma unroll
   for (int q = 0; q < NC; ++q) glb(F1.cons[b * NC + q])[id] = unew[q];
+  if (uout) {
+#pragma unroll
+    for (int q = 0; q < NC; ++q) uout[q] = unew[q];
+  }
+}
+
+template <int MODE, bool SRC, bool HB, bool HK, bool CTF>
+__global__ __launch_bounds__(256, (HB && MODE != 1) ? 5 : 4) void k_div_update(KCtx C, KFields F0, KFields Finit, KFields F1,
+                                                    double *const *dudt,
+                                                    const double *dt_dev, double beta,
+                                                    double bdt_host, Range3 R) {
+  int b, k, j, i;
+  if (!decode_tile(C, R, b, k, j, i)) return;
+  div_update_cell<MODE, SRC, HB, HK, CTF>(C, F0, Finit, F1, dudt, dt_dev, beta, bdt_host, b, k, j, i,
+                                          nullptr);
 }
 
 // -------------------------------------------------- recovery (+ heating)
@@ -858,21 +872,24 @@ __global__ __launch_bounds__(256, (HB && MODE != 1) ? 5 : 4) void k_div_update(K
 template <bool HB, bool HK>
 __device__ __forceinline__ void heat_vals(const KCtx &C, const KFields &F, const DevGeo &G, int64_t t,
                                           int b, int64_t id, const Arr32<double> *pp,
-                                          const double p[NPRIM]) {
+                                          const double p[NPRIM], const double *ureg = nullptr) {
   constexpr int NC = ncomp<HB, HK>();
   CenLoad m;
   load_cen_state(G, t, m);
   St s;
   calc_state(p, m.g, m.gc0, m.alpha, s);
   const uint32_t id32 = (uint32_t)id;
-  const double crho = ld32(F.cons[b * NC], 0, id32);
+  // conserved components: from memory, or from registers (ureg, in
+  // component order) when the caller has just written them
+  auto cu = [&](int q) { return ureg ? ureg[q] : ld32(F.cons[b * NC + q], 0, id32); };
+  const double crho = cu(0);
   HeatIn c;
   c.rho = p[RHO];
   c.energy = p[ENY];
   c.bsq = s.bsq;
-  c.adv_total = ld32(F.cons[b * NC + ient<HB>()], 0, id32) / crho;
+  c.adv_total = cu(ient<HB>()) / crho;
   c.rec_total = p[ENT];
-  c.adv_e = ld32(F.cons[b * NC + ikel<HB>()], 0, id32) / crho;
+  c.adv_e = cu(ikel<HB>()) / crho;
   pp[ikel<HB>()][id] = heat_apply(C.ph, c);
 }
 
@@ -894,7 +911,8 @@ __device__ __forceinline__ void heat_cell(const KCtx &C, const KFields &F, const
 // 1 Kastaun, 2 1Dvsq alone, -1 none (heating only).
 template <int REC, bool HB, bool HK, bool HEAT>
 __device__ __forceinline__ void recover_cell(const KCtx &C, const KFields &F, const SharedTabs &S, int b,
-                                             int k, int j, int i, double x0 = -1.0) {
+                                             int k, int j, int i, double x0 = -1.0,
+                                             const double *ureg = nullptr) {
   const DevBlock &B = C.blk[b];
   const int64_t id = ((int64_t)k * B.n[1] + j) * B.n[0] + i;
   constexpr int NC = ncomp<HB, HK>();
@@ -927,16 +945,18 @@ __device__ __forceinline__ void recover_cell(const KCtx &C, const KFields &F, co
       Arr32<double> cons[NC];
 #pragma unroll
       for (int q = 0; q < NC; ++q) cons[q] = Arr32<double>{F.cons[b * NC + q]};
-      const double c_rho = cons[0][id], c_eny = cons[1][id];
+      // U from memory, or from registers (ureg) when the update just wrote it
+      auto cv = [&](int q) { return ureg ? ureg[q] : (double)cons[q][id]; };
+      const double c_rho = cv(0), c_eny = cv(1);
       const double isg = 1.0 / sg;
       const double aos = alpha * isg;
       U[RHO] = aos * c_rho;
       U[ENY] = aos * (c_eny - c_rho);
-      U[UX1] = aos * cons[2][id];
-      U[UX2] = aos * cons[3][id];
-      U[UX3] = aos * cons[4][id];
+      U[UX1] = aos * cv(2);
+      U[UX2] = aos * cv(3);
+      U[UX3] = aos * cv(4);
       if (HB) {
-        const double cb1 = cons[5][id], cb2 = cons[6][id], cb3 = cons[7][id];
+        const double cb1 = cv(5), cb2 = cv(6), cb3 = cv(7);
         U[BX1] = aos * cb1;
         U[BX2] = aos * cb2;
         U[BX3] = aos * cb3;
@@ -951,7 +971,7 @@ __device__ __forceinline__ void recover_cell(const KCtx &C, const KFields &F, co
         U[BX2] = 0.0;
         U[BX3] = 0.0;
       }
-      U[ENT] = aos * cons[ient<HB>()][id];
+      U[ENT] = aos * cv(ient<HB>());
       // the Newton schemes start from the previous primitives; Kastaun's
       // bracket needs none (it writes rho, u and u~ only on success), so
       // their loads are skipped
@@ -1029,7 +1049,7 @@ __device__ __forceinline__ void recover_cell(const KCtx &C, const KFields &F, co
           hp[BX3] = bnew[2];
         }
         hp[ENT] = pp[ient<HB>()][id];
-        heat_vals<HB, HK>(C, F, G, t, b, id, pp, hp);
+        heat_vals<HB, HK>(C, F, G, t, b, id, pp, hp, ureg);
         return;
       }
     }
@@ -1042,6 +1062,24 @@ __global__ __launch_bounds__(256) void k_recover(KCtx C, KFields F, SharedTabs S
   int b, k, j, i;
   if (!decode_tile(C, R, b, k, j, i)) return;
   recover_cell<REC, HB, HK, HEAT>(C, F, S, b, k, j, i);
+}
+
+// The update of k_div_update (mode 4: flux divergence, RK update, source)
+// and the recovery (+ heating) of k_recover for the same cell in one pass:
+// the recovery takes the new U from registers instead of memory.  The update
+// is memory-bound and the recovery VALU-bound, so their phases overlap
+// across the wavefronts of one launch.  Same operations per cell in the same
+// order: identical results.  Called only when nothing has to run between
+// the two (the stage's exchange runs before).
+template <bool SRC, bool HB, bool HK, bool CTF, int REC, bool HEAT>
+__global__ __launch_bounds__(256) void k_update_recover(KCtx C, KFields F0, KFields Finit, KFields F1,
+                                                        SharedTabs S, const double *dt_dev, double beta,
+                                                        double bdt_host, Range3 R) {
+  int b, k, j, i;
+  if (!decode_tile(C, R, b, k, j, i)) return;
+  double u[NPRIM];
+  div_update_cell<4, SRC, HB, HK, CTF>(C, F0, Finit, F1, nullptr, dt_dev, beta, bdt_host, b, k, j, i, u);
+  recover_cell<REC, HB, HK, HEAT>(C, F1, S, b, k, j, i, -1.0, u);
 }
 
 // Persistent form of k_recover: a grid no larger than what the GPU holds
@@ -1812,6 +1850,62 @@ int launch_recover(const LaunchCtx &L, const KFields &F, const SharedTabs &S, in
 #undef RC2
 #undef RC
   if (rec >= 0 || h) TICK(rec < 0 ? "heating" : h ? "recovery_heating" : "recovery");
+  return PG_CHECK_LAUNCH();
+}
+
+// Arguments of one k_update_recover launch, and the dispatch over its
+// compile-time options: field (with CT formed in the update), K_e, heating
+// and the recovery scheme.
+struct URArgs {
+  const KCtx &k;
+  const KFields &F0, &Finit, &F1;
+  const SharedTabs &S;
+  const double *dt_dev;
+  double beta, bdt_host;
+  const Range3 &R;
+};
+template <bool SRC, bool HB, bool HK, int REC, bool HEAT>
+static void launch_ur(const URArgs &a, dim3 g, dim3 wg, hipStream_t s) {
+  hipLaunchKernelGGL((k_update_recover<SRC, HB, HK, HB, REC, HEAT>), g, wg, 0, s, a.k, a.F0, a.Finit,
+                     a.F1, a.S, a.dt_dev, a.beta, a.bdt_host, a.R);
+}
+template <bool SRC, int REC>
+static void launch_ur_fields(const URArgs &a, dim3 g, dim3 wg, hipStream_t s, bool hb, bool hk, bool h) {
+  if (!hb)
+    launch_ur<SRC, false, false, REC, false>(a, g, wg, s);
+  else if (h)
+    launch_ur<SRC, true, true, REC, true>(a, g, wg, s);
+  else if (hk)
+    launch_ur<SRC, true, true, REC, false>(a, g, wg, s);
+  else
+    launch_ur<SRC, true, false, REC, false>(a, g, wg, s);
+}
+template <bool SRC>
+static void launch_ur_rec(const URArgs &a, dim3 g, dim3 wg, hipStream_t s, int rec, bool hb, bool hk,
+                          bool h) {
+  if (rec == 0)
+    launch_ur_fields<SRC, 0>(a, g, wg, s, hb, hk, h);
+  else if (rec == 1)
+    launch_ur_fields<SRC, 1>(a, g, wg, s, hb, hk, h);
+  else
+    launch_ur_fields<SRC, 2>(a, g, wg, s, hb, hk, h);
+}
+
+int launch_update_recover(const LaunchCtx &L, const KFields &F0, const KFields &Finit,
+                          const KFields &F1, const SharedTabs &S, int rec, bool heat, bool ct_fused,
+                          const double *dt_dev, double beta, double bdt_host, hipStream_t s) {
+  const Range3 R = family_range(L, 0, L.interior, PANGU_KF_DIV_UPDATE);
+  if (R.ntiles == 0) return 0;
+  const dim3 g = tile_grid(R), wg(R.tile);
+  const bool hb = L.k.iBX >= 0, hk = L.k.iKEL >= 0, h = heat && hk;
+  const bool src = L.metric != PANGU_METRIC_MINKOWSKI;
+  if ((hb && !ct_fused) || (!hb && hk) || rec < 0 || rec > 2) return -1;
+  const URArgs a{L.k, F0, Finit, F1, S, dt_dev, beta, bdt_host, R};
+  if (src)
+    launch_ur_rec<true>(a, g, wg, s, rec, hb, hk, h);
+  else
+    launch_ur_rec<false>(a, g, wg, s, rec, hb, hk, h);
+  TICK("update_recovery");
   return PG_CHECK_LAUNCH();
 }
 

@@ -118,6 +118,7 @@ def lib():
         L.pangu_sim_set_split_source.argtypes = [vp, C.c_int]
         L.pangu_sim_set_fuse_ct.argtypes = [vp, C.c_int]
         L.pangu_sim_set_overlap_exchange.argtypes = [vp, C.c_int]
+        L.pangu_sim_set_fuse_update_recovery.argtypes = [vp, C.c_int]
         L.pangu_sim_set_share_reconstruction.argtypes = [vp, C.c_int]
         L.pangu_sim_time_kernels.argtypes = [vp, C.c_int, dp, ip, C.c_int]
         L.pangu_sim_kernel_name.restype = C.c_char_p
@@ -339,6 +340,11 @@ class HipSim:
         update kernel (1, default) or written by their own kernel first (0).
         Results are unchanged."""
         self._check(lib().pangu_sim_set_fuse_ct(self._p, int(fuse)), "set_fuse_ct")
+
+    def set_fuse_update_recovery(self, on):
+        """Fused stage: flux-divergence update and recovery in one kernel (1,
+        default) or two (0).  Results are unchanged."""
+        self._check(lib().pangu_sim_set_fuse_update_recovery(self._p, int(on)), "set_fuse_update_recovery")
 
     def set_overlap_exchange(self, on):
         """Fused stage: the ghost exchange after the update runs on a second

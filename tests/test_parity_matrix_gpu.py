@@ -347,7 +347,7 @@ def test_launch_tuning_leaves_results_unchanged():
     runs = []
     for variant in ("default", "split", "column", "plane", "share", "ct_faces", "split_ct_faces",
                     "pair", "pair_share", "march", "march_column", "tile128", "tile64",
-                    "tile64_column", "persistent", "persistent_tile64", "overlap"):
+                    "tile64_column", "persistent", "persistent_tile64", "overlap", "separate_update_recovery"):
         h = HipSim(cfg)
         h.init()
         if variant in ("split", "split_ct_faces"):
@@ -365,6 +365,8 @@ def test_launch_tuning_leaves_results_unchanged():
             h.set_persistent_recovery(1)
         if variant == "overlap":
             h.set_overlap_exchange(1)
+        if variant == "separate_update_recovery":
+            h.set_fuse_update_recovery(0)
         for fam in KERNEL_FAMILIES:
             if variant.startswith("tile"):
                 h.set_tile_size(fam, int(variant[4:].split("_")[0]))
