@@ -39,6 +39,14 @@ __device__ __forceinline__ T ld32(const T *base, int64_t u, uint32_t i) {
   return *(const __attribute__((address_space(1))) T *)((const __attribute__((address_space(1))) char *)(base + u) +
                                                         i * (uint32_t)sizeof(T));
 }
+// ld32 through the constant address space, for the geometry tables, which
+// no kernel writes: the compiler may then move the load across the kernel's
+// stores (which it must otherwise assume could alias the table).
+template <class T>
+__device__ __forceinline__ T ldc32(const T *base, int64_t u, uint32_t i) {
+  return *(const __attribute__((address_space(4))) T *)((const __attribute__((address_space(4))) char *)(base + u) +
+                                                        i * (uint32_t)sizeof(T));
+}
 template <class T>
 __device__ __forceinline__ void st32(T *base, uint32_t i, T v) {
   *(__attribute__((address_space(1))) T *)((__attribute__((address_space(1))) char *)base +

@@ -52,11 +52,11 @@ __device__ __forceinline__ void load_cen_state(const DevGeo &G, int64_t t, CenLo
   const int64_t s = G.ncen;
   const uint32_t t32 = (uint32_t)t;
 #pragma unroll
-  for (int n = 0; n < 16; ++n) m.g[n] = ld32(c, (CG_GCOV + n) * s, t32);
+  for (int n = 0; n < 16; ++n) m.g[n] = ldc32(c, (CG_GCOV + n) * s, t32);
 #pragma unroll
-  for (int n = 0; n < 4; ++n) m.gc0[n] = ld32(c, (CG_GCON + n) * s, t32);
-  m.alpha = ld32(c, CG_ALPHA * s, t32);
-  m.sg = ld32(c, CG_SG * s, t32);
+  for (int n = 0; n < 4; ++n) m.gc0[n] = ldc32(c, (CG_GCON + n) * s, t32);
+  m.alpha = ldc32(c, CG_ALPHA * s, t32);
+  m.sg = ldc32(c, CG_SG * s, t32);
 }
 
 // ------------------------------------------------ fix_primitive + conservative
@@ -92,8 +92,8 @@ __global__ __launch_bounds__(256) void k_fix_cons(KCtx C, KFields F, uint8_t *co
     const double small = 1.0e-20;
     const double sms = kmax(P.sigma_max, small);
     const double rho_before = kmax(p[RHO], small);
-    double rho_floor = ld32(G.cen, (int64_t)CG_RHOFL * G.ncen, (uint32_t)t);
-    const double eng_floor = ld32(G.cen, (int64_t)CG_ENGFL * G.ncen, (uint32_t)t);
+    double rho_floor = ldc32(G.cen, (int64_t)CG_RHOFL * G.ncen, (uint32_t)t);
+    const double eng_floor = ldc32(G.cen, (int64_t)CG_ENGFL * G.ncen, (uint32_t)t);
     St st;
     calc_state(p, m.g, m.gc0, m.alpha, st);
     const double rho = p[RHO], ug = p[ENY];
@@ -128,7 +128,7 @@ __global__ __launch_bounds__(256) void k_fix_cons(KCtx C, KFields F, uint8_t *co
     if (HK && rho_after != rho_before) p[KEL] *= pow(rho_after / rho_before, -P.gamma);
     p[ENT] = (P.gamma - 1.0) * p[ENY] * pow(p[RHO], -P.gamma);
     if (HK) p[KEL] = clamp_by_ratio(P.ratio_min, P.ratio_max, p[ENT], p[KEL]);
-    if (P.metric == PANGU_METRIC_CKS && ld32(G.cen, (int64_t)CG_RAD * G.ncen, (uint32_t)t) < P.r_excise) {
+    if (P.metric == PANGU_METRIC_CKS && ldc32(G.cen, (int64_t)CG_RAD * G.ncen, (uint32_t)t) < P.r_excise) {
       const double e_exc = P.pexcise / (P.gamma - 1.0);
       p[RHO] = P.dexcise;
       p[ENY] = e_exc;
@@ -186,12 +186,12 @@ __device__ __forceinline__ void face_flux(const KCtx &C, const KFields &F, doubl
     const double *c = G.cen;
     const uint32_t s = (uint32_t)G.ncen;
 #pragma unroll
-    for (int n = 0; n < 16; ++n) g[n] = ld32(c, (int64_t)(CG_GCOV + n) * s, t);
+    for (int n = 0; n < 16; ++n) g[n] = ldc32(c, (int64_t)(CG_GCOV + n) * s, t);
 #pragma unroll
-    for (int n = 0; n < 4; ++n) gc0[n] = ld32(c, (int64_t)(CG_GCON + n) * s, t);
-    gdd = ld32(c, (int64_t)(CG_GCON + 15) * s, t);
-    sg = ld32(c, (int64_t)CG_SG * s, t);
-    alpha = ld32(c, (int64_t)CG_ALPHA * s, t);
+    for (int n = 0; n < 4; ++n) gc0[n] = ldc32(c, (int64_t)(CG_GCON + n) * s, t);
+    gdd = ldc32(c, (int64_t)(CG_GCON + 15) * s, t);
+    sg = ldc32(c, (int64_t)CG_SG * s, t);
+    alpha = ldc32(c, (int64_t)CG_ALPHA * s, t);
     valid = 1.0;
   } else {
     const int d = DIR - 1;
@@ -199,13 +199,13 @@ __device__ __forceinline__ void face_flux(const KCtx &C, const KFields &F, doubl
     const double *c = G.face[d];
     const uint32_t s = (uint32_t)G.nface[d];
 #pragma unroll
-    for (int n = 0; n < 16; ++n) g[n] = ld32(c, (int64_t)(FG_GCOV + n) * s, t);
+    for (int n = 0; n < 16; ++n) g[n] = ldc32(c, (int64_t)(FG_GCOV + n) * s, t);
 #pragma unroll
-    for (int n = 0; n < 4; ++n) gc0[n] = ld32(c, (int64_t)(FG_GCON0 + n) * s, t);
-    gdd = ld32(c, (int64_t)FG_GCONDD * s, t);
-    sg = ld32(c, (int64_t)FG_SG * s, t);
-    alpha = ld32(c, (int64_t)FG_ALPHA * s, t);
-    valid = ld32(c, (int64_t)FG_VALID * s, t);
+    for (int n = 0; n < 4; ++n) gc0[n] = ldc32(c, (int64_t)(FG_GCON0 + n) * s, t);
+    gdd = ldc32(c, (int64_t)FG_GCONDD * s, t);
+    sg = ldc32(c, (int64_t)FG_SG * s, t);
+    alpha = ldc32(c, (int64_t)FG_ALPHA * s, t);
+    valid = ldc32(c, (int64_t)FG_VALID * s, t);
   }
   double *const al = alfven[b * 3 + (DIR - 1)];
   const uint32_t id32 = (uint32_t)id;
@@ -840,7 +840,7 @@ __device__ __forceinline__ void div_update_cell(const KCtx &C, const KFields &F0
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
           const int slot = sym ? 10 * c + conn_pair(dd, r) : 16 * c + 4 * dd + r;
-          con += T[r][c] * ld32(conn, 0, (uint32_t)slot * ncen32 + t32);
+          con += T[r][c] * ldc32(conn, 0, (uint32_t)slot * ncen32 + t32);
         }
       const int q = (dd == 0) ? 1 : 1 + dd;
       unew[q] += con * m.sg * bdt;
@@ -926,7 +926,7 @@ __device__ __forceinline__ void recover_cell(const KCtx &C, const KFields &F, co
   if (REC >= 0) {
     const bool chainer = (REC != 1);
     bool excised = false;
-    if (P.metric == PANGU_METRIC_CKS && ld32(G.cen, (int64_t)CG_RAD * ns, (uint32_t)t) < P.r_excise) excised = true;
+    if (P.metric == PANGU_METRIC_CKS && ldc32(G.cen, (int64_t)CG_RAD * ns, (uint32_t)t) < P.r_excise) excised = true;
     if (excised) {
       if (chainer && S.flag) glb(S.flag[b])[id] = 0.0;
       if (S.status) glb(S.status[b])[id] = -1;
@@ -935,12 +935,12 @@ __device__ __forceinline__ void recover_cell(const KCtx &C, const KFields &F, co
       CenMetric m;
 #pragma unroll
       for (int n = 0; n < 16; ++n) {
-        m.gcov[n] = ld32(G.cen, (int64_t)(CG_GCOV + n) * ns, (uint32_t)t);
-        m.gcon[n] = ld32(G.cen, (int64_t)(CG_GCON + n) * ns, (uint32_t)t);
+        m.gcov[n] = ldc32(G.cen, (int64_t)(CG_GCOV + n) * ns, (uint32_t)t);
+        m.gcon[n] = ldc32(G.cen, (int64_t)(CG_GCON + n) * ns, (uint32_t)t);
       }
-      m.lapse = ld32(G.cen, (int64_t)CG_LAPSE * ns, (uint32_t)t);
-      const double sg = ld32(G.cen, (int64_t)CG_SG * ns, (uint32_t)t);
-      const double alpha = ld32(G.cen, (int64_t)CG_ALPHA * ns, (uint32_t)t);
+      m.lapse = ldc32(G.cen, (int64_t)CG_LAPSE * ns, (uint32_t)t);
+      const double sg = ldc32(G.cen, (int64_t)CG_SG * ns, (uint32_t)t);
+      const double alpha = ldc32(G.cen, (int64_t)CG_ALPHA * ns, (uint32_t)t);
       double U[9], Ph[9], bnew[3] = {0.0, 0.0, 0.0};
       Arr32<double> cons[NC];
 #pragma unroll
@@ -1125,11 +1125,11 @@ __device__ __forceinline__ int learned_setup(const KCtx &C, const KFields &F, co
   CenMetric m;
 #pragma unroll
   for (int n = 0; n < 16; ++n) {
-    m.gcov[n] = ld32(G.cen, (int64_t)(CG_GCOV + n) * ns, (uint32_t)t);
-    m.gcon[n] = ld32(G.cen, (int64_t)(CG_GCON + n) * ns, (uint32_t)t);
+    m.gcov[n] = ldc32(G.cen, (int64_t)(CG_GCOV + n) * ns, (uint32_t)t);
+    m.gcon[n] = ldc32(G.cen, (int64_t)(CG_GCON + n) * ns, (uint32_t)t);
   }
-  m.lapse = ld32(G.cen, (int64_t)CG_LAPSE * ns, (uint32_t)t);
-  const double aos = ld32(G.cen, (int64_t)CG_ALPHA * ns, (uint32_t)t) * (1.0 / ld32(G.cen, (int64_t)CG_SG * ns, (uint32_t)t));
+  m.lapse = ldc32(G.cen, (int64_t)CG_LAPSE * ns, (uint32_t)t);
+  const double aos = ldc32(G.cen, (int64_t)CG_ALPHA * ns, (uint32_t)t) * (1.0 / ldc32(G.cen, (int64_t)CG_SG * ns, (uint32_t)t));
   const double *const *cons = F.cons + (int64_t)b * NC;
   const double c_rho = glb(cons[0])[id], c_eny = glb(cons[1])[id];
   double U[9];
@@ -1157,7 +1157,7 @@ __global__ __launch_bounds__(256) void k_recover_learned(KCtx C, KFields F, Shar
   const int64_t id = cell ? ((int64_t)k * B.n[1] + j) * B.n[0] + i : 0;
   const int64_t t = cell ? cen_index(G, k, j, i) : 0;
   const bool active =
-      cell && !(P.metric == PANGU_METRIC_CKS && ld32(G.cen, (int64_t)CG_RAD * G.ncen, (uint32_t)t) < P.r_excise);
+      cell && !(P.metric == PANGU_METRIC_CKS && ldc32(G.cen, (int64_t)CG_RAD * G.ncen, (uint32_t)t) < P.r_excise);
   // The workgroup votes on the previous iteration counts alone, before any
   // other load: a workgroup with no cell passing the gate then runs as
   // k_recover<1> does, with no further barrier.  The network's cost is per
@@ -1203,7 +1203,7 @@ __global__ __launch_bounds__(256) void k_recover_learned(KCtx C, KFields F, Shar
     if (S.iters) glb(S.iters[b])[id] = 0;
   } else {
     if (HB) {
-      const double isg = 1.0 / ld32(G.cen, (int64_t)CG_SG * G.ncen, (uint32_t)t);
+      const double isg = 1.0 / ldc32(G.cen, (int64_t)CG_SG * G.ncen, (uint32_t)t);
 #pragma unroll
       for (int a = 0; a < 3; ++a) pp[5 + a][id] = glb(F.cons[b * NC + 5 + a])[id] * isg;
     }
@@ -1252,7 +1252,7 @@ __global__ __launch_bounds__(256) void k_recover_learned_vsq(KCtx C, KFields F, 
   const int64_t id = cell ? ((int64_t)k * B.n[1] + j) * B.n[0] + i : 0;
   const int64_t t = cell ? cen_index(G, k, j, i) : 0;
   const bool active =
-      cell && !(P.metric == PANGU_METRIC_CKS && ld32(G.cen, (int64_t)CG_RAD * G.ncen, (uint32_t)t) < P.r_excise);
+      cell && !(P.metric == PANGU_METRIC_CKS && ldc32(G.cen, (int64_t)CG_RAD * G.ncen, (uint32_t)t) < P.r_excise);
   int prev = 0;
   if (active && S.iters) prev = glb(S.iters[b])[id];
   const bool gated = active && S.iters && prev >= S.gate;
