@@ -293,6 +293,14 @@ int pangu_set_overlap_exchange(pangu_ctx *ctx, int on);
  * the separate kernels run otherwise.  Same operations per cell: results
  * are identical.  0: separate kernels. */
 int pangu_set_fuse_update_recovery(pangu_ctx *ctx, int on);
+/* on = 1: pangu_stage runs FixPrimitive + CalculateConservative and the X1
+ * flux sweep in one kernel: a workgroup fixes a run of 248 cells of a
+ * ghosted plane plus four halo cells on each side (fixed again, to the same
+ * bits, by the neighbouring workgroup), keeps the fixed primitives in LDS
+ * and forms the X1 faces of its cells from them.  Applies when the X1 sweep
+ * uses its plain one-lane-per-face form.  Same operations: results are
+ * identical.  0 (default): separate kernels. */
+int pangu_set_fuse_fix_x1(pangu_ctx *ctx, int on);
 
 /* ---- run diagnostics (no reference counterpart) ----
  * doc/grmhd_intro.md:556-574 defines the fluxes through a sphere that a

@@ -158,6 +158,8 @@ int pangu_sim_set_fuse_ct(pangu_sim *s, int fuse);
 int pangu_sim_set_overlap_exchange(pangu_sim *s, int on);
 /* pangu_set_fuse_update_recovery on the simulation's context (drops the graph). */
 int pangu_sim_set_fuse_update_recovery(pangu_sim *s, int on);
+/* pangu_set_fuse_fix_x1 on the simulation's context (drops the graph). */
+int pangu_sim_set_fuse_fix_x1(pangu_sim *s, int on);
 /* pangu_set_share_reconstruction on the simulation's context (drops the graph). */
 int pangu_sim_set_share_reconstruction(pangu_sim *s, int share);
 int pangu_sim_set_pair_flux(pangu_sim *s, int dir_mask);

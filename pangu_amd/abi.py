@@ -100,6 +100,7 @@ PROTOTYPES = {
     "pangu_set_fuse_ct": (C.c_int, [_vp, C.c_int]),
     "pangu_set_overlap_exchange": (C.c_int, [_vp, C.c_int]),
     "pangu_set_fuse_update_recovery": (C.c_int, [_vp, C.c_int]),
+    "pangu_set_fuse_fix_x1": (C.c_int, [_vp, C.c_int]),
     "pangu_diagnostics": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), _dp, _vp]),
     "pangu_diagnostics_noalloc": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), _vp, _vp]),
     "pangu_stage": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_double, _dp, C.c_double,

@@ -555,6 +555,7 @@ static int build_tables(pangu_ctx *c) {
       r = make_range(c, L.small[id][z], q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], z ? 64 : 128);
   }
   if (!r) r = make_range_x1s(c, L.x1cells);
+  if (!r) r = make_range(c, L.fixx1, 0, 0, 0, 0, 0, 0, 1, -1, 248);
   std::vector<int64_t> ghosts(c->nblocks), planes(c->nblocks);
   for (int b = 0; b < c->nblocks; ++b) {
     const HostBlock &H = c->hb[b];
@@ -809,6 +810,12 @@ int pangu_set_tile_order(pangu_ctx *c, int fam, int order) {
 int pangu_set_fuse_ct(pangu_ctx *c, int fuse) {
   if (!c) return PANGU_EINVAL;
   c->L.fuse_ct = fuse != 0;
+  return PANGU_OK;
+}
+
+int pangu_set_fuse_fix_x1(pangu_ctx *c, int on) {
+  if (!c) return PANGU_EINVAL;
+  c->L.fuse_fix_x1 = on != 0;
   return PANGU_OK;
 }
 
@@ -1159,8 +1166,16 @@ int pangu_stage(pangu_ctx *c, int md_init, int md0, int md1, double beta, double
     LCHK(c, exchange(c, f1, c->side));
     HIPCHK(c, hipEventRecord(c->ev_join, c->side));
   }
-  LCHK(c, launch_fix_cons(c->L, f0, c->dmask, true, true, s));
-  LCHK(c, launch_fluxes(c->L, f0, c->dalfven, s));
+  // FixPrimitive + conservative and the X1 sweep in one kernel (the plain
+  // per-face X1 form only)
+  const bool fuse_fx = c->L.fuse_fix_x1 && !c->L.share_x1 && !(c->L.pair_flux & 1);
+  if (fuse_fx) {
+    LCHK(c, launch_fix_flux_x1(c->L, f0, c->dmask, c->dalfven, s));
+    LCHK(c, launch_fluxes_23(c->L, f0, c->dalfven, s));
+  } else {
+    LCHK(c, launch_fix_cons(c->L, f0, c->dmask, true, true, s));
+    LCHK(c, launch_fluxes(c->L, f0, c->dalfven, s));
+  }
   // With fuse_ct the update kernel forms the edge EMFs and the CT face B
   // fluxes itself, so no CT kernel runs: the flux arrays keep the Riemann B
   // fluxes and the EMF arrays their last values after a fused stage

@@ -121,6 +121,7 @@ struct LaunchCtx {
   KCtx k;
   int limiter, riemann, metric, recovery;
   Range3 interior, entire, face[3], ct;
+  Range3 fixx1;  // the entire range in runs of 248 cells (k_fix_flux_x1)
   Range3 x1cells;                  // cell range of the shared-reconstruction X1 sweep
   Items ghosts, bcs[3];
   int order[PANGU_KF_COUNT] = {};  // tile order per kernel family (PANGU_ORDER_*)
@@ -128,6 +129,7 @@ struct LaunchCtx {
   int fuse_ct = 1;                 // fused stage: CT face fluxes formed in the update kernel
   int overlap_exchange = -1;       // fused stage: ghost exchange on the side stream (-1: refined meshes)
   int fuse_update_recovery = 1;    // fused stage: update and recovery in one kernel
+  int fuse_fix_x1 = 0;             // fused stage: FixPrimitive + conservative and the X1 sweep in one kernel
   int share_x1 = 0;                // X1 sweep reconstructs each cell once (k_flux_x1s)
   int pair_flux = 0;               // bit d-1: the sweep along d runs two lanes per face (k_flux_pair)
   Range3 march[2];                 // marching ranges of the X2 and X3 sweeps (k_flux_march)
@@ -180,6 +182,11 @@ int launch_recover(const LaunchCtx &L, const KFields &F, const SharedTabs &S, in
 // (k_update_recover): the chainer, Kastaun or 1Dvsq recovery, with the CT
 // face fluxes formed in the update on a magnetised mesh, and K_e only with
 // a field (-1 otherwise).
+// FixPrimitive + CalculateConservative and the X1 sweep in one kernel
+// (k_fix_flux_x1), and the X2 and X3 sweeps alone.
+int launch_fix_flux_x1(const LaunchCtx &L, const KFields &F, uint8_t *const *mask, double *const *alfven,
+                       hipStream_t s);
+int launch_fluxes_23(const LaunchCtx &L, const KFields &F, double *const *alfven, hipStream_t s);
 int launch_update_recover(const LaunchCtx &L, const KFields &F0, const KFields &Finit,
                           const KFields &F1, const SharedTabs &S, int rec, bool heat, bool ct_fused,
                           const double *dt_dev, double beta, double bdt_host, hipStream_t s);

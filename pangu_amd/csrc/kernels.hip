@@ -65,10 +65,12 @@ __device__ __forceinline__ void load_cen_state(const DevGeo &G, int64_t t, CenLo
 // primitives and conservatives are written with non-temporal stores: on
 // MI355X that leaves L2 to the stencil reads of the flux sweep that follows
 // (measured 1 % faster per cycle on the 3-D torus than ordinary stores).
+// One cell of k_fix_cons.  write = false computes without storing (a halo
+// cell of k_fix_flux_x1, which another workgroup owns); pout (optional)
+// receives the fixed primitives, every slot.
 template <bool FIX, bool CONS, bool HB, bool HK>
-__global__ __launch_bounds__(256) void k_fix_cons(KCtx C, KFields F, uint8_t *const *mask, Range3 R) {
-  int b, k, j, i;
-  if (!decode_tile(C, R, b, k, j, i)) return;
+__device__ __forceinline__ void fix_cons_cell(const KCtx &C, const KFields &F, uint8_t *const *mask, int b,
+                                              int k, int j, int i, bool write, double *pout) {
   const DevBlock &B = C.blk[b];
   const bool interior = i >= B.s[0] && i <= B.e[0] && j >= B.s[1] && j <= B.e[1] &&
                         k >= B.s[2] && k <= B.e[2];
@@ -142,18 +144,24 @@ __global__ __launch_bounds__(256) void k_fix_cons(KCtx C, KFields F, uint8_t *co
     // rho, u and u~ change only where a floor, the Lorentz cap or excision
     // set a mask bit; elsewhere memory already holds them (the entropy is
     // recomputed in every cell)
-    if (mk) {
+    if (write && mk) {
       __builtin_nontemporal_store(p[RHO], &pp[0][id]);
       __builtin_nontemporal_store(p[ENY], &pp[1][id]);
       __builtin_nontemporal_store(p[UX1], &pp[2][id]);
       __builtin_nontemporal_store(p[UX2], &pp[3][id]);
       __builtin_nontemporal_store(p[UX3], &pp[4][id]);
     }
-    __builtin_nontemporal_store(p[ENT], &pp[ient<HB>()][id]);
-    if (HK) pp[ikel<HB>()][id] = p[KEL];
-    if (mask) glb(mask[b])[id] = mk;
+    if (write) {
+      __builtin_nontemporal_store(p[ENT], &pp[ient<HB>()][id]);
+      if (HK) pp[ikel<HB>()][id] = p[KEL];
+      if (mask) glb(mask[b])[id] = mk;
+    }
   }
-  if (CONS && interior) {
+  if (pout) {
+#pragma unroll
+    for (int q = 0; q < NPRIM; ++q) pout[q] = p[q];
+  }
+  if (CONS && interior && write) {
     St st;
     calc_state(p, m.g, m.gc0, m.alpha, st);
     double f[NPRIM];
@@ -161,6 +169,13 @@ __global__ __launch_bounds__(256) void k_fix_cons(KCtx C, KFields F, uint8_t *co
 #pragma unroll
     for (int q = 0; q < NC; ++q) __builtin_nontemporal_store(f[slot_of<HB, HK>(q)], &glb(F.cons[b * NC + q])[id]);
   }
+}
+
+template <bool FIX, bool CONS, bool HB, bool HK>
+__global__ __launch_bounds__(256) void k_fix_cons(KCtx C, KFields F, uint8_t *const *mask, Range3 R) {
+  int b, k, j, i;
+  if (!decode_tile(C, R, b, k, j, i)) return;
+  fix_cons_cell<FIX, CONS, HB, HK>(C, F, mask, b, k, j, i, true, nullptr);
 }
 
 // ------------------------------------------------------------- flux sweep
@@ -290,6 +305,103 @@ __global__ __launch_bounds__(256) void k_flux(KCtx C, KFields F, double *const *
     }
   }
   face_flux<DIR, RS, HB, HK>(C, F, alfven, b, k, j, i, pl, pr);
+}
+
+// FixPrimitive + CalculateConservative (k_fix_cons) and the X1 sweep
+// (k_flux<1>) in one pass.  A workgroup of 256 threads covers cells
+// t0 - 4 .. t0 + R.tile + 3 of one ghosted plane (R: the entire range in runs
+// of R.tile = 248 cells, decoded as decode_tile_at does), fixes every one of
+// them and stores the outputs of the cells it owns (t0 .. t0 + 247); the
+// four halo cells on each side are fixed again by the neighbouring
+// workgroup (FixPrimitive is pointwise, so both compute the same bits).  The
+// fixed primitives wait in LDS, and each owned cell that is an X1 face
+// (s..e+1, one transverse ghost layer) reconstructs its two states from
+// them: the stencil i-3..i+2 lies in the cell's row and in the window.  Same
+// operations and operands as the two kernels: identical results.
+template <int LIM, int RS, bool HB, bool HK>
+__global__ __launch_bounds__(256) void k_fix_flux_x1(KCtx C, KFields F, uint8_t *const *mask,
+                                                     double *const *alfven, Range3 R) {
+  __shared__ double sp[NPRIM][256];
+  const int N = R.ntiles;
+  const int n8 = (N + 7) >> 3;
+  const int L = (int)blockIdx.x;
+  const int T = (L & 7) * n8 + (L >> 3);
+  if (T >= N) return;  // uniform over the workgroup
+  const int TL = R.tile;
+  int lo[3], cnt[3], local, tpp, b;
+  if (R.uni) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = R.ulo[d];
+      cnt[d] = R.ucnt[d];
+    }
+    tpp = R.utpp;
+    const int per_block = cnt[2] * tpp;
+    b = T / per_block;
+    local = T - b * per_block;
+  } else {
+    b = R.tblock[T];
+    local = T - R.tfirst[b];
+    block_range(C.blk[b], R, lo, cnt);
+    tpp = (cnt[0] * cnt[1] + TL - 1) / TL;
+  }
+  const int plane = cnt[0] * cnt[1];
+  int tile, kk;
+  if (R.order) {
+    tile = local / cnt[2];
+    kk = local - tile * cnt[2];
+  } else {
+    kk = local / tpp;
+    tile = local - kk * tpp;
+  }
+  const int x = (int)threadIdx.x;
+  const int t = tile * TL + x - 4;
+  const bool valid = t >= 0 && t < plane && x < TL + 8;
+  const bool own = valid && x >= 4 && x < TL + 4;
+  const int k = lo[2] + kk;
+  int j = 0, i = 0;
+  if (valid) {
+    const int jj = t / cnt[0];
+    j = lo[1] + jj;
+    i = lo[0] + (t - jj * cnt[0]);
+  }
+  double p[NPRIM];
+#pragma unroll
+  for (int q = 0; q < NPRIM; ++q) p[q] = 0.0;
+  if (valid) fix_cons_cell<true, true, HB, HK>(C, F, mask, b, k, j, i, own, p);
+#pragma unroll
+  for (int q = 0; q < NPRIM; ++q) sp[q][x] = p[q];
+  __syncthreads();
+  if (!own) return;
+  const DevBlock &B = C.blk[b];
+  const bool face = i >= B.s[0] && i <= B.e[0] + 1 &&
+                    (B.n[1] > 1 ? (j >= B.s[1] - 1 && j <= B.e[1] + 1) : j == B.s[1]) &&
+                    (B.n[2] > 1 ? (k >= B.s[2] - 1 && k <= B.e[2] + 1) : k == B.s[2]);
+  if (!face) return;
+  constexpr int NC = ncomp<HB, HK>();
+  constexpr int NS = (LIM == PANGU_LIMITER_PPM4) ? 6 : 4;  // offsets -NS/2 .. NS/2 - 1
+  double pl[NPRIM], pr[NPRIM];
+#pragma unroll
+  for (int q = 0; q < NPRIM; ++q) {
+    pl[q] = 0.0;
+    pr[q] = 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
+    const int sl = slot_of<HB, HK>(q);
+    double v[NS];
+#pragma unroll
+    for (int n = 0; n < NS; ++n) v[n] = sp[sl][x + n - NS / 2];
+    double t0, t1;
+    if (LIM == PANGU_LIMITER_PPM4) {
+      recon_ppm4(v[0], v[1], v[2], v[3], v[4], pl[sl], t0);
+      recon_ppm4(v[1], v[2], v[3], v[4], v[5], t1, pr[sl]);
+    } else {
+      recon_mc(v[0], v[1], v[2], pl[sl], t0);
+      recon_mc(v[1], v[2], v[3], t1, pr[sl]);
+    }
+  }
+  face_flux<1, RS, HB, HK>(C, F, alfven, b, k, j, i, pl, pr);
 }
 
 // The value of lane l - 1 (lane 0 gets 0) through the DPP wavefront shift
@@ -1745,6 +1857,51 @@ static void flux_dir(const LaunchCtx &L, const KFields &F, double *const *alfven
 #undef FL2
 #undef FL
   TICK(DIR == 1 ? "flux_x1" : DIR == 2 ? "flux_x2" : "flux_x3");
+}
+
+int launch_fluxes_23(const LaunchCtx &L, const KFields &F, double *const *alfven, hipStream_t s) {
+  flux_dir<2>(L, F, alfven, s);
+  flux_dir<3>(L, F, alfven, s);
+  return PG_CHECK_LAUNCH();
+}
+
+template <int LIM, int RS, bool HB, bool HK>
+static void launch_fx(const LaunchCtx &L, const Range3 &R, const KFields &F, uint8_t *const *mask,
+                      double *const *alfven, hipStream_t s) {
+  hipLaunchKernelGGL((k_fix_flux_x1<LIM, RS, HB, HK>), tile_grid(R), kTile, 0, s, L.k, F, mask, alfven, R);
+}
+template <int LIM, int RS>
+static void launch_fx_fields(const LaunchCtx &L, const Range3 &R, const KFields &F, uint8_t *const *mask,
+                             double *const *alfven, hipStream_t s) {
+  const bool hb = L.k.iBX >= 0, hk = L.k.iKEL >= 0;
+  if (hb && hk)
+    launch_fx<LIM, RS, true, true>(L, R, F, mask, alfven, s);
+  else if (hb)
+    launch_fx<LIM, RS, true, false>(L, R, F, mask, alfven, s);
+  else if (hk)
+    launch_fx<LIM, RS, false, true>(L, R, F, mask, alfven, s);
+  else
+    launch_fx<LIM, RS, false, false>(L, R, F, mask, alfven, s);
+}
+
+int launch_fix_flux_x1(const LaunchCtx &L, const KFields &F, uint8_t *const *mask, double *const *alfven,
+                       hipStream_t s) {
+  Range3 R = L.fixx1;
+  R.order = L.order[PANGU_KF_FIX_CONS];
+  if (R.ntiles == 0 || R.tile + 8 != TILE) return -1;
+  if (L.limiter == PANGU_LIMITER_PPM4) {
+    if (L.riemann == PANGU_RIEMANN_HLL)
+      launch_fx_fields<PANGU_LIMITER_PPM4, PANGU_RIEMANN_HLL>(L, R, F, mask, alfven, s);
+    else
+      launch_fx_fields<PANGU_LIMITER_PPM4, PANGU_RIEMANN_LAXF>(L, R, F, mask, alfven, s);
+  } else {
+    if (L.riemann == PANGU_RIEMANN_HLL)
+      launch_fx_fields<PANGU_LIMITER_MC, PANGU_RIEMANN_HLL>(L, R, F, mask, alfven, s);
+    else
+      launch_fx_fields<PANGU_LIMITER_MC, PANGU_RIEMANN_LAXF>(L, R, F, mask, alfven, s);
+  }
+  TICK("fix_cons_flux_x1");
+  return PG_CHECK_LAUNCH();
 }
 
 int launch_fluxes(const LaunchCtx &L, const KFields &F, double *const *alfven, hipStream_t s) {

@@ -1031,6 +1031,12 @@ int pangu_sim_set_fuse_ct(pangu_sim *s, int fuse) {
   return 0;
 }
 
+int pangu_sim_set_fuse_fix_x1(pangu_sim *s, int on) {
+  PCHK(s, pangu_set_fuse_fix_x1(s->ctx, on));
+  drop_graph(s);
+  return 0;
+}
+
 int pangu_sim_set_fuse_update_recovery(pangu_sim *s, int on) {
   PCHK(s, pangu_set_fuse_update_recovery(s->ctx, on));
   drop_graph(s);

@@ -119,6 +119,7 @@ def lib():
         L.pangu_sim_set_fuse_ct.argtypes = [vp, C.c_int]
         L.pangu_sim_set_overlap_exchange.argtypes = [vp, C.c_int]
         L.pangu_sim_set_fuse_update_recovery.argtypes = [vp, C.c_int]
+        L.pangu_sim_set_fuse_fix_x1.argtypes = [vp, C.c_int]
         L.pangu_sim_set_share_reconstruction.argtypes = [vp, C.c_int]
         L.pangu_sim_time_kernels.argtypes = [vp, C.c_int, dp, ip, C.c_int]
         L.pangu_sim_kernel_name.restype = C.c_char_p
@@ -340,6 +341,11 @@ class HipSim:
         update kernel (1, default) or written by their own kernel first (0).
         Results are unchanged."""
         self._check(lib().pangu_sim_set_fuse_ct(self._p, int(fuse)), "set_fuse_ct")
+
+    def set_fuse_fix_x1(self, on):
+        """Fused stage: FixPrimitive + conservative and the X1 sweep in one
+        kernel (1) or two (0, default).  Results are unchanged."""
+        self._check(lib().pangu_sim_set_fuse_fix_x1(self._p, int(on)), "set_fuse_fix_x1")
 
     def set_fuse_update_recovery(self, on):
         """Fused stage: flux-divergence update and recovery in one kernel (1,

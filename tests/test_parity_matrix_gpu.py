@@ -407,6 +407,29 @@ def test_fused_ct_is_bit_identical(name):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("name", ["ot_minkowski_noB", "torus2d_mc_hll", "torus2d_kastaun",
+                                  "ot_cks3d_laxf_mc_chainer_heat", "torus3d_odd_blocks", "ot_minkowski_1d"])
+def test_fused_fix_x1_is_bit_identical(name):
+    """FixPrimitive + conservative and the X1 sweep in one kernel (halo cells
+    fixed twice, the fixed primitives kept in LDS) give the same bits as the
+    two kernels: fields, masks, X1 fluxes and wave speeds, for both limiters
+    and solvers, with and without a field and K_e."""
+    from pangu_amd.sim import HipSim
+    cfg = MATRIX[name]
+    out = []
+    for fuse in (0, 1):
+        h = HipSim(cfg)
+        h.init()
+        h.set_fuse_fix_x1(fuse)
+        h.step(3, "graph")
+        out.append([h.get(f, c) for f, c in (("prim", 0), ("cons", 0), ("prim", 1), ("mask", 0),
+                                              ("flux1", 0), ("alfven", 0))])
+        h.close()
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("name", ["ot_minkowski_noB", "torus2d_mc_hll", "ot_cks3d_laxf_mc_chainer_heat"])
 def test_shared_reconstruction_is_bit_identical(name):
     """The X1 sweep with one reconstruction per cell (wavefront shuffle of
