@@ -287,11 +287,12 @@ int pangu_set_overlap_exchange(pangu_ctx *ctx, int on);
  * geometric source) and the recovery (+ heating) of each cell in one
  * kernel, the recovery taking the new U from registers; the stage's
  * exchange (after the update) then runs before that kernel, which changes
- * nothing.  Applies with the chainer, Kastaun or 1Dvsq recovery, no split
- * source, no learned start and no persistent recovery, and on a magnetised
- * mesh with the CT face fluxes formed in the update (pangu_set_fuse_ct);
- * the separate kernels run otherwise.  Same operations per cell: results
- * are identical.  0: separate kernels. */
+ * nothing.  Applies with the chainer, Kastaun or 1Dvsq recovery (with a
+ * learned start attached, its kernel runs the update first), no split
+ * source and no persistent recovery, and on a magnetised mesh with the CT
+ * face fluxes formed in the update (pangu_set_fuse_ct); the separate
+ * kernels run otherwise.  Same operations per cell: results are identical.
+ * 0: separate kernels. */
 int pangu_set_fuse_update_recovery(pangu_ctx *ctx, int on);
 /* on = 1: pangu_stage runs FixPrimitive + CalculateConservative and the X1
  * flux sweep in one kernel: a workgroup fixes a run of 248 cells of a

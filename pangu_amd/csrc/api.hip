@@ -1190,15 +1190,26 @@ int pangu_stage(pangu_ctx *c, int md_init, int md0, int md1, double beta, double
   // update) then runs before it, which changes nothing (it touches neither
   // U nor the update's inputs)
   const bool hb = c->L.k.iBX >= 0, hk = c->L.k.iKEL >= 0;
-  const bool fuse_ur = c->L.fuse_update_recovery && (hb ? fuse_ct : !hk) && !c->L.split_source && !Wl &&
+  const bool fuse_ur = c->L.fuse_update_recovery && (hb ? fuse_ct : !hk) && !c->L.split_source &&
                        !c->L.persist_recovery && c->cfg.recovery >= 0 && c->cfg.recovery <= 2;
   if (fuse_ur) {
     if (overlap)
       HIPCHK(c, hipStreamWaitEvent(s, c->ev_join, 0));
     else if (exchange_after == 0)
       LCHK(c, exchange(c, f1, s));
-    LCHK(c, launch_update_recover(c->L, f0, fi, f1, shared_tabs(c), c->cfg.recovery,
-                                  c->cfg.enable_heating != 0, fuse_ct, dt_state, beta, 0.0, s));
+    if (Wl) {
+      // the learned kernel of the scheme, with the update run first
+      UpdArgs u;
+      u.F0 = f0;
+      u.Finit = fi;
+      u.dt_dev = dt_state;
+      u.beta = beta;
+      LCHK(c, launch_recover(c->L, f1, shared_tabs(c), c->cfg.recovery, c->cfg.enable_heating != 0, Wl, s,
+                             &u));
+    } else {
+      LCHK(c, launch_update_recover(c->L, f0, fi, f1, shared_tabs(c), c->cfg.recovery,
+                                    c->cfg.enable_heating != 0, fuse_ct, dt_state, beta, 0.0, s));
+    }
   } else if (c->L.split_source && c->cfg.metric != PANGU_METRIC_MINKOWSKI) {
     LCHK(c, launch_div_update(c->L, 5, f0, fi, f1, nullptr, fuse_ct, dt_state, beta, 0.0, s));
     LCHK(c, launch_div_update(c->L, 3, f0, fi, f1, nullptr, false, dt_state, beta, 0.0, s));

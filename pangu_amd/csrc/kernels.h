@@ -176,8 +176,17 @@ int launch_div_update(const LaunchCtx &L, int mode, const KFields &F0, const KFi
 // model W and rec 1, the recovery kernel evaluates the network in each
 // wavefront whose cells pass the gate and starts those cells' Kastaun
 // search from the learned bracket.
+// Inputs of the mode-4 update a learned recovery kernel runs first when the
+// stage fuses the two (see k_update_recover).
+struct UpdArgs {
+  KFields F0, Finit;
+  const double *dt_dev = nullptr;
+  double beta = 0.0, bdt_host = 0.0;
+};
+// upd (optional): the learned kernels run the cell's update first (the
+// plain schemes then take launch_update_recover).
 int launch_recover(const LaunchCtx &L, const KFields &F, const SharedTabs &S, int rec, bool heat,
-                   const MlpFrag *W, hipStream_t s);
+                   const MlpFrag *W, hipStream_t s, const UpdArgs *upd = nullptr);
 // The mode-4 update and the recovery (+ heating) of each cell in one kernel
 // (k_update_recover): the chainer, Kastaun or 1Dvsq recovery, with the CT
 // face fluxes formed in the update on a magnetised mesh, and K_e only with

@@ -1256,11 +1256,17 @@ __device__ __forceinline__ int learned_setup(const KCtx &C, const KFields &F, co
   return kastaun_setup(U, m, kin);
 }
 
-template <bool HB, bool HK, bool HEAT>
+// UPD 1 or 2: the cell's update (k_div_update mode 4; 2 with the source)
+// runs first, as in k_update_recover; the recovery then reads the U this
+// thread has just stored.
+template <bool HB, bool HK, bool HEAT, int UPD>
 __global__ __launch_bounds__(256) void k_recover_learned(KCtx C, KFields F, SharedTabs S, MlpFrag W,
-                                                         Range3 R) {
+                                                         Range3 R, UpdArgs A) {
   int b = 0, k = 0, j = 0, i = 0;
   const bool cell = decode_tile(C, R, b, k, j, i);  // false on the whole workgroup past the range
+  if (UPD && cell)
+    div_update_cell<4, UPD == 2, HB, HK, HB>(C, A.F0, A.Finit, F, nullptr, A.dt_dev, A.beta, A.bdt_host, b,
+                                             k, j, i, nullptr);
   const int lane = threadIdx.x & 63;
   constexpr int NC = ncomp<HB, HK>();
   const Phys &P = C.ph;
@@ -1352,11 +1358,14 @@ __global__ __launch_bounds__(256) void k_recover_learned(KCtx C, KFields F, Shar
 // logistic(z) instead of the previous primitives' v^2.  Everything else is
 // recover_cell<2>: the result agrees with k_recover<2> to the Newton
 // tolerance, not bit for bit.
-template <bool HB, bool HK, bool HEAT>
+template <bool HB, bool HK, bool HEAT, int UPD>
 __global__ __launch_bounds__(256) void k_recover_learned_vsq(KCtx C, KFields F, SharedTabs S, MlpFrag W,
-                                                             Range3 R) {
+                                                             Range3 R, UpdArgs A) {
   int b = 0, k = 0, j = 0, i = 0;
   const bool cell = decode_tile(C, R, b, k, j, i);  // false on the whole workgroup past the range
+  if (UPD && cell)
+    div_update_cell<4, UPD == 2, HB, HK, HB>(C, A.F0, A.Finit, F, nullptr, A.dt_dev, A.beta, A.bdt_host, b,
+                                             k, j, i, nullptr);
   const int lane = threadIdx.x & 63;
   const Phys &P = C.ph;
   const DevBlock &B = C.blk[cell ? b : 0];
@@ -1971,18 +1980,42 @@ int launch_div_update(const LaunchCtx &L, int mode, const KFields &F0, const KFi
   return PG_CHECK_LAUNCH();
 }
 
+template <bool HB, bool HK, bool HEAT>
+static void launch_learned(const LaunchCtx &L, const KFields &F, const SharedTabs &S, const MlpFrag &W,
+                           const Range3 &R, dim3 g, dim3 wg, hipStream_t s, const UpdArgs *upd) {
+  const UpdArgs A = upd ? *upd : UpdArgs{};
+  const int u = !upd ? 0 : (L.metric != PANGU_METRIC_MINKOWSKI ? 2 : 1);
+  if (W.kind == 0) {
+    if (u == 2)
+      hipLaunchKernelGGL((k_recover_learned<HB, HK, HEAT, 2>), g, wg, 0, s, L.k, F, S, W, R, A);
+    else if (u == 1)
+      hipLaunchKernelGGL((k_recover_learned<HB, HK, HEAT, 1>), g, wg, 0, s, L.k, F, S, W, R, A);
+    else
+      hipLaunchKernelGGL((k_recover_learned<HB, HK, HEAT, 0>), g, wg, 0, s, L.k, F, S, W, R, A);
+  } else {
+    if (u == 2)
+      hipLaunchKernelGGL((k_recover_learned_vsq<HB, HK, HEAT, 2>), g, wg, 0, s, L.k, F, S, W, R, A);
+    else if (u == 1)
+      hipLaunchKernelGGL((k_recover_learned_vsq<HB, HK, HEAT, 1>), g, wg, 0, s, L.k, F, S, W, R, A);
+    else
+      hipLaunchKernelGGL((k_recover_learned_vsq<HB, HK, HEAT, 0>), g, wg, 0, s, L.k, F, S, W, R, A);
+  }
+}
+
 int launch_recover(const LaunchCtx &L, const KFields &F, const SharedTabs &S, int rec, bool heat,
-                   const MlpFrag *W, hipStream_t s) {
-  const Range3 R = family_range(L, 0, L.interior, PANGU_KF_RECOVERY);
+                   const MlpFrag *W, hipStream_t s, const UpdArgs *upd) {
+  // a fused update needs the learned kernel of the scheme (k_update_recover
+  // serves the plain ones) and the range of the update
+  if (upd && !(W && ((rec == 1 && W->kind == 0) || (rec == 2 && W->kind == 1)))) return -1;
+  const Range3 R = upd ? family_range(L, 0, L.interior, PANGU_KF_DIV_UPDATE)
+                       : family_range(L, 0, L.interior, PANGU_KF_RECOVERY);
   if (R.ntiles == 0) return 0;
   const dim3 g = tile_grid(R), wg(R.tile);
   const bool hb = L.k.iBX >= 0, hk = L.k.iKEL >= 0;
   const bool persist = L.persist_recovery && L.queue;
 #define RC(R_, HB_, HK_, HE_)                                                                        \
-  if (R_ == 1 && W && W->kind == 0)                                                                   \
-    hipLaunchKernelGGL((k_recover_learned<HB_, HK_, HE_>), g, wg, 0, s, L.k, F, S, *W, R);            \
-  else if (R_ == 2 && W && W->kind == 1)                                                              \
-    hipLaunchKernelGGL((k_recover_learned_vsq<HB_, HK_, HE_>), g, wg, 0, s, L.k, F, S, *W, R);        \
+  if ((R_ == 1 && W && W->kind == 0) || (R_ == 2 && W && W->kind == 1))                              \
+    launch_learned<HB_, HK_, HE_>(L, F, S, *W, R, g, wg, s, upd);                                     \
   else if (persist)                                                                                   \
     hipLaunchKernelGGL((k_recover_queue<R_, HB_, HK_, HE_>),                                          \
                        dim3(resident_groups(k_recover_queue<R_, HB_, HK_, HE_>, R.tile, R.ntiles)), wg, \
