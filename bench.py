@@ -69,6 +69,8 @@ PROFILE = "profiles/counters_torus3d_256x128x128.md"
 # (gate 7: no torus cell takes more than 6), the last measuring what the
 # gated kernel costs when the network never runs.
 LEARNED_GATES = (0, 5, 7)
+# (div_update_source and recovery_heating: the two kernels update_recovery
+# fuses, as they measured when they ran separately.)
 MEASURED = {
     "update_recovery": {"bytes": 2618017375, "fp64_flops": 9.184e9},
     "div_update_source": {"bytes": 2222250845, "fp64_flops": 2.634e9},
