@@ -1320,10 +1320,14 @@ __global__ __launch_bounds__(256) void k_recover_learned(KCtx C, KFields F, Shar
     if (S.status) glb(S.status[b])[id] = -1;
     if (S.iters) glb(S.iters[b])[id] = 0;
   } else {
+    double bnew[3] = {0.0, 0.0, 0.0};
     if (HB) {
       const double isg = 1.0 / ldc32(G.cen, (int64_t)CG_SG * G.ncen, (uint32_t)t);
 #pragma unroll
-      for (int a = 0; a < 3; ++a) pp[5 + a][id] = glb(F.cons[b * NC + 5 + a])[id] * isg;
+      for (int a = 0; a < 3; ++a) {
+        bnew[a] = glb(F.cons[b * NC + 5 + a])[id] * isg;
+        pp[5 + a][id] = bnew[a];
+      }
     }
     double Ph[9];
     double mu = 0.0;
@@ -1345,6 +1349,33 @@ __global__ __launch_bounds__(256) void k_recover_learned(KCtx C, KFields F, Shar
       pp[2][id] = Ph[UX1];
       pp[3][id] = Ph[UX2];
       pp[4][id] = Ph[UX3];
+    }
+    if (HEAT && HK) {
+      // as recover_cell: the heating takes the primitives just written from
+      // registers (a failed cell keeps its previous ones, from memory)
+      double hp[NPRIM];
+#pragma unroll
+      for (int q = 0; q < NPRIM; ++q) hp[q] = 0.0;
+      if (st == 0) {
+        hp[RHO] = Ph[RHO];
+        hp[ENY] = Ph[ENY];
+        hp[UX1] = Ph[UX1];
+        hp[UX2] = Ph[UX2];
+        hp[UX3] = Ph[UX3];
+      } else {
+        hp[RHO] = pp[0][id];
+        hp[ENY] = pp[1][id];
+        hp[UX1] = pp[2][id];
+        hp[UX2] = pp[3][id];
+        hp[UX3] = pp[4][id];
+      }
+      if (HB) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) hp[BX1 + a] = bnew[a];
+      }
+      hp[ENT] = pp[ient<HB>()][id];
+      heat_vals<HB, HK>(C, F, G, t, b, id, pp, hp);
+      return;
     }
   }
   if (HEAT && HK) heat_cell<HB, HK>(C, F, G, t, b, id, pp);
